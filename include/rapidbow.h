@@ -1,0 +1,179 @@
+/*
+ * rapidbow.h -- C ABI of the MI355X (gfx950) RapidBoW hot path.
+ *
+ * Drop-in boundary for the reference's public surface (RapidBoW.h:29-109):
+ * vocabulary build, descriptor -> leaf lookup, serialisation, and the
+ * inverted-file database.  Plain pointers and sizes only; every function
+ * returns an rbow_status and never throws; rbow_last_error() returns the
+ * calling thread's last error message.  Device pointers are HIP device memory
+ * (hipMalloc); "host" pointers are ordinary host memory.
+ *
+ * The C++ facade include/RapidBoW.h rebuilds the reference's C++ names,
+ * ownership and exception messages on top of these functions.
+ */
+#ifndef RAPIDBOW_AMD_RAPIDBOW_H
+#define RAPIDBOW_AMD_RAPIDBOW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* rbow_stream; /* == hipStream_t; NULL = default stream */
+
+typedef enum rbow_status {
+    RBOW_OK = 0,
+    RBOW_ERR_INVALID_ARG = 1,
+    RBOW_ERR_UNSUPPORTED = 2,          /* RapidBoW.cpp:37-40 (reference returns nullptr) */
+    RBOW_ERR_INSUFFICIENT_DESC = 3,    /* Vocabulary.cpp:39-41 "Insufficient descriptors" */
+    RBOW_ERR_NB_DOC = 4,               /* Vocabulary.cpp:42-44 "nbDoc must be positive" */
+    RBOW_ERR_INVALID_BLOB = 5,         /* Vocabulary.cpp:129-131,143-145 "invalid blob" */
+    RBOW_ERR_DUPLICATE_DOC = 6,        /* DataBase.cpp:41-43 "Duplicate document index" */
+    RBOW_ERR_HIP = 7,                  /* replaces the reference's cudaCheck aborts */
+    RBOW_ERR_OUT_OF_MEMORY = 8,
+    RBOW_ERR_CAPACITY = 9              /* a fixed limit of this implementation was exceeded */
+} rbow_status;
+
+/* RapidBoW.h:31-48 */
+typedef enum { RBOW_DATA_BINARY = 0, RBOW_DATA_UINT8 = 1 } rbow_data_type;
+typedef enum { RBOW_DIST_HAMMING = 0, RBOW_DIST_L1 = 1, RBOW_DIST_L2 = 2 } rbow_distance_type;
+typedef struct {
+    int32_t data_type;     /* rbow_data_type */
+    int32_t distance_type; /* rbow_distance_type */
+    uint64_t nb_dims;
+} rbow_descriptor_attributes;
+
+typedef struct rbow_vocabulary rbow_vocabulary;
+typedef struct rbow_database rbow_database;
+
+/* Thread-local message of the last failing call on this thread ("" if none). */
+const char* rbow_last_error(void);
+
+/* ---------------------------------------------------------------- build -- */
+
+/* Options of the hierarchical k-means build.  rbow_build_options_default()
+ * gives the reference's behaviour (kmeans.h:112-118, kmeans.cpp:141-196). */
+typedef struct {
+    uint32_t max_iters;     /* Lloyd pass cap per node (reference: 1000) */
+    uint32_t seed_mode;     /* 0: reference random picks; 1: k-means++ over <=256 sampled members */
+    uint64_t seed;          /* engine seed; 5489 = std::mt19937_64's default (the reference) */
+    uint32_t centroid_type; /* 0: uint8 centroids (reference, b = 16); 1: float32 (b = 16 or 32) */
+    uint32_t kernel;        /* 0: MFMA kernels; 1: plain scalar kernels (same results) */
+} rbow_build_options;
+
+void rbow_build_options_default(rbow_build_options* opts);
+
+/* RapidBoW.h:73-88 impl::buildVocabularyImpl / rbow::buildVocabulary.
+ * dev_desc: device uint8[nb_desc][128].  Blocks until the tree is built (the
+ * reference synchronises internally).  Unsupported attributes or branch
+ * factor -> RBOW_ERR_UNSUPPORTED (the reference returns nullptr). */
+rbow_status rbow_build_vocabulary(rbow_descriptor_attributes attr, uint32_t branch_factor, const void* dev_desc,
+                                  uint32_t nb_desc, uint32_t nb_doc, uint32_t nb_levels, rbow_stream stream,
+                                  rbow_vocabulary** out);
+/* Same with explicit options (NULL = defaults).  Extends the reference with
+ * branch factor 32 and float32 centroids (centroid_type 1). */
+rbow_status rbow_build_vocabulary_ex(rbow_descriptor_attributes attr, uint32_t branch_factor, const void* dev_desc,
+                                     uint32_t nb_desc, uint32_t nb_doc, uint32_t nb_levels,
+                                     const rbow_build_options* opts, rbow_stream stream, rbow_vocabulary** out);
+void rbow_voc_destroy(rbow_vocabulary* voc);
+
+/* ------------------------------------------------------ IVocabulary ------ */
+/* RapidBoW.h:60-71 */
+rbow_status rbow_voc_get_attributes(const rbow_vocabulary* voc, rbow_descriptor_attributes* attr);
+uint32_t rbow_voc_branch_factor(const rbow_vocabulary* voc);
+uint32_t rbow_voc_nb_levels(const rbow_vocabulary* voc);
+/* 0: uint8 centroids, 1: float32 centroids */
+uint32_t rbow_voc_centroid_type(const rbow_vocabulary* voc);
+
+/* RapidBoW.h:66-68 IVocabulary::lookUp: asynchronous on `stream`; dev_desc is
+ * device uint8[nb_desc][128], dev_leaf device uint32[nb_desc].  nb_desc == 0
+ * is a no-op (the reference launches an empty grid and errors). */
+rbow_status rbow_voc_lookup(const rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc, uint32_t* dev_leaf,
+                            rbow_stream stream);
+
+typedef enum {
+    RBOW_LOOKUP_AUTO = 0,  /* MFMA batched form */
+    RBOW_LOOKUP_PLAIN = 1, /* plain scalar per-descriptor descent */
+    RBOW_LOOKUP_MFMA = 2   /* level-synchronous bucketed MFMA form */
+} rbow_lookup_kernel;
+/* Lookup with an explicit kernel.  near_ties (optional, device uint32[1],
+ * accumulated, not cleared) counts (descriptor, level) pairs whose MFMA
+ * candidate set held more than one child and was re-scored in FP32. */
+rbow_status rbow_voc_lookup_ex(const rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc,
+                               uint32_t* dev_leaf, int32_t kernel, uint32_t* dev_near_ties, rbow_stream stream);
+
+/* RapidBoW.h:69 IVocabulary::serialize.  Two-call form: blob may be NULL to
+ * query *size.  uint8/b16 trees use the reference blob byte layout
+ * (Vocabulary.cpp:175-197); float32 or b32 trees a versioned "RBW2" blob. */
+rbow_status rbow_voc_serialize(const rbow_vocabulary* voc, uint8_t* blob, size_t capacity, size_t* size);
+/* RapidBoW.h:90 deserializeVocabulary (Vocabulary.cpp:126-157). */
+rbow_status rbow_voc_deserialize(const uint8_t* blob, size_t size, rbow_vocabulary** out);
+/* RapidBoWInternal.h:29-30 getLevelWeights: host float[levelSize(level)][b]. */
+const float* rbow_voc_level_weights(const rbow_vocabulary* voc, uint32_t level);
+/* Host copy of one level's centroids as float32 [levelSize(level)][b][128]
+ * (uint8 trees are widened); for tests and tools. */
+rbow_status rbow_voc_copy_centroids(const rbow_vocabulary* voc, uint32_t level, float* host_out, size_t capacity_floats);
+
+/* ------------------------------------------------ kernel-level entries -- */
+/* These expose the batched k-means kernels the build drives, one pass at a
+ * time, for callers that run their own Lloyd loop (and for parity tests).
+ * Points are device uint8 rows [sum(task_cnt)][128], tasks contiguous in
+ * order; task_cnt is a HOST array of nb_tasks sizes.  b = 16 (uint8 centers
+ * [task][16][128]) or b = 16/32 with float centers [task][b][128]. */
+
+/* kmeans.cuh:102 launchAssignCenter (kmeans.cu:226-331): one assign pass.
+ * dev_assign [rows] holds the previous assignment and receives the new one;
+ * dev_counts [task][b], dev_sums [task][b][128] (int64) and dev_changed
+ * [task] are ACCUMULATED into (zero them first, as the reference does).
+ * kernel: 0 = I8 MFMA (uint8 centers only), 1 = plain scalar. */
+rbow_status rbow_kmeans_assign(const uint8_t* dev_points, const uint32_t* task_cnt, uint32_t nb_tasks, uint32_t b,
+                               uint32_t centroid_type, const void* dev_centers, uint8_t* dev_assign,
+                               uint32_t* dev_counts, int64_t* dev_sums, uint32_t* dev_changed, int32_t kernel,
+                               rbow_stream stream);
+/* kmeans.cuh:103 launchKMeansUpdateCenters (kmeans.cu:334-384): for tasks
+ * with dev_changed != 0, every center with count > 0 becomes
+ * round(sum/count) (uint8) or sum/count (float); then the task's counts, sums
+ * and changed are cleared for the next pass. */
+rbow_status rbow_kmeans_update(const uint32_t* task_cnt, uint32_t nb_tasks, uint32_t b, uint32_t centroid_type,
+                               void* dev_centers, uint32_t* dev_counts, int64_t* dev_sums, uint32_t* dev_changed,
+                               rbow_stream stream);
+/* reformatDesc.cu:78-96 launchReformatDesc / launchKMeansInitCenters:
+ * dev_tiles[t][word][slot] = word of desc[idx[t][slot]] (32 words per
+ * descriptor, tile_size 32 or 16); idx 0xFFFFFFFF -> 0xFFFFFFFF. */
+rbow_status rbow_reformat_tiles(const void* dev_desc, const uint32_t* dev_idx, uint32_t nb_tiles, uint32_t tile_size,
+                                uint32_t* dev_tiles, rbow_stream stream);
+/* Row gather used by the build: dev_out[p] = desc[idx[p]] (invalid -> 0xFF). */
+rbow_status rbow_gather_rows(const void* dev_desc, const uint32_t* dev_idx, uint64_t n, uint8_t* dev_out,
+                             rbow_stream stream);
+/* Lloyd passes run per node by the last build of this vocabulary (host
+ * uint32[nodes]); NULL when the vocabulary was deserialised. */
+const uint32_t* rbow_voc_build_passes(const rbow_vocabulary* voc);
+
+/* Timing of the last build (zeros for deserialised vocabularies).
+ * assign_ops = point_passes * 2 * b * 128 (the distance contraction). */
+typedef struct {
+    double total_ms;       /* host wall time of the build */
+    double assign_ms;      /* summed assign-kernel time (HIP events) */
+    double update_ms;      /* summed update-kernel time (HIP events) */
+    uint64_t point_passes; /* sum over nodes of points x Lloyd passes */
+    uint64_t assign_ops;
+} rbow_build_stats;
+rbow_status rbow_voc_build_stats(const rbow_vocabulary* voc, rbow_build_stats* out);
+
+/* ------------------------------------------------ synthetic data -------- */
+/* No counterpart in the reference: seeded SIFT-like data for benchmarks.
+ * nb_centres Gaussian centres c ~ N(0, I_128) from centre_seed; descriptor i
+ * (global index first+i) takes centre centre_ids[i] (or a hashed uniform pick
+ * when centre_ids is NULL), x = |c + sigma n| with sigma = sigma_scale*||c||/sqrt(128),
+ * L2-normalise, clip at 0.2, renormalise, x512, round, saturate to 255.
+ * Host memory in and out; uses all OpenMP threads. */
+void rbow_synth_centres(uint64_t centre_seed, uint32_t nb_centres, float* out);
+void rbow_synth_descriptors(uint64_t centre_seed, uint32_t nb_centres, uint64_t seed, uint64_t first, uint64_t n,
+                            const uint32_t* centre_ids, float sigma_scale, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

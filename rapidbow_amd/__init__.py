@@ -1,0 +1,249 @@
+"""Python binding of the RapidBoW MI355X library (librapidbow_amd.so).
+
+Mirrors the reference's public surface (RapidBoW.h:29-109): build a
+vocabulary from device descriptors, look descriptors up into leaf ids,
+serialise / deserialise, and the inverted-file database.  All compute runs in
+the HIP library; importing this package fails loudly when it is not built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import hip
+from .hip import DeviceArray, Stream, Event
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librapidbow_amd.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+lib = ctypes.CDLL(LIB_PATH)
+
+_vp, _u32, _u64, _i32, _sz = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_size_t
+
+# status codes (include/rapidbow.h)
+OK, ERR_INVALID_ARG, ERR_UNSUPPORTED, ERR_INSUFFICIENT_DESC, ERR_NB_DOC, ERR_INVALID_BLOB, ERR_DUPLICATE_DOC, \
+    ERR_HIP, ERR_OUT_OF_MEMORY, ERR_CAPACITY = range(10)
+LOOKUP_AUTO, LOOKUP_PLAIN, LOOKUP_MFMA = 0, 1, 2
+
+
+class DescriptorAttributes(ctypes.Structure):
+    _fields_ = [("data_type", ctypes.c_int32), ("distance_type", ctypes.c_int32), ("nb_dims", ctypes.c_uint64)]
+
+
+SIFT_ATTR = DescriptorAttributes(1, 2, 128)  # RapidBoW.h:50 SiftAttr
+
+
+class BuildStats(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("assign_ms", ctypes.c_double), ("update_ms", ctypes.c_double),
+                ("point_passes", ctypes.c_uint64), ("assign_ops", ctypes.c_uint64)]
+
+
+class BuildOptions(ctypes.Structure):
+    _fields_ = [("max_iters", ctypes.c_uint32), ("seed_mode", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+                ("centroid_type", ctypes.c_uint32), ("kernel", ctypes.c_uint32)]
+
+
+def _sig(name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_sig("rbow_last_error", ctypes.c_char_p, [])
+_sig("rbow_build_options_default", None, [ctypes.POINTER(BuildOptions)])
+_sig("rbow_build_vocabulary", _i32, [DescriptorAttributes, _u32, _vp, _u32, _u32, _u32, _vp, ctypes.POINTER(_vp)])
+_sig("rbow_build_vocabulary_ex", _i32,
+     [DescriptorAttributes, _u32, _vp, _u32, _u32, _u32, ctypes.POINTER(BuildOptions), _vp, ctypes.POINTER(_vp)])
+_sig("rbow_voc_destroy", None, [_vp])
+_sig("rbow_voc_get_attributes", _i32, [_vp, ctypes.POINTER(DescriptorAttributes)])
+_sig("rbow_voc_branch_factor", _u32, [_vp])
+_sig("rbow_voc_nb_levels", _u32, [_vp])
+_sig("rbow_voc_centroid_type", _u32, [_vp])
+_sig("rbow_voc_lookup", _i32, [_vp, _vp, _u32, _vp, _vp])
+_sig("rbow_voc_lookup_ex", _i32, [_vp, _vp, _u32, _vp, _i32, _vp, _vp])
+_sig("rbow_voc_serialize", _i32, [_vp, _vp, _sz, ctypes.POINTER(_sz)])
+_sig("rbow_voc_deserialize", _i32, [_vp, _sz, ctypes.POINTER(_vp)])
+_sig("rbow_voc_level_weights", ctypes.POINTER(ctypes.c_float), [_vp, _u32])
+_sig("rbow_voc_build_stats", _i32, [_vp, ctypes.POINTER(BuildStats)])
+_sig("rbow_voc_build_passes", ctypes.POINTER(ctypes.c_uint32), [_vp])
+_sig("rbow_voc_copy_centroids", _i32, [_vp, _u32, _vp, _sz])
+_sig("rbow_kmeans_assign", _i32, [_vp, _vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _i32, _vp])
+_sig("rbow_kmeans_update", _i32, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp])
+_sig("rbow_reformat_tiles", _i32, [_vp, _vp, _u32, _u32, _vp, _vp])
+_sig("rbow_gather_rows", _i32, [_vp, _vp, _u64, _vp, _vp])
+_sig("rbow_synth_centres", None, [_u64, _u32, _vp])
+_sig("rbow_synth_descriptors", None, [_u64, _u32, _u64, _u64, _u64, _vp, ctypes.c_float, _vp])
+
+
+class RbowError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(msg)
+        self.status = status
+
+
+def check(status):
+    if status != OK:
+        raise RbowError(status, lib.rbow_last_error().decode())
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, DeviceArray):
+        return x.ptr
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data
+    return int(x)
+
+
+def _stream(s):
+    return s.handle if isinstance(s, Stream) else s
+
+
+class Vocabulary:
+    """IVocabulary (RapidBoW.h:60-71)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.rbow_voc_destroy(self.handle)
+            self.handle = None
+
+    @property
+    def branch_factor(self):
+        return lib.rbow_voc_branch_factor(self.handle)
+
+    @property
+    def nb_levels(self):
+        return lib.rbow_voc_nb_levels(self.handle)
+
+    @property
+    def centroid_type(self):
+        return lib.rbow_voc_centroid_type(self.handle)
+
+    @property
+    def nb_nodes(self):
+        b, L = self.branch_factor, self.nb_levels
+        return (b ** L - 1) // (b - 1)
+
+    def attributes(self):
+        a = DescriptorAttributes()
+        check(lib.rbow_voc_get_attributes(self.handle, ctypes.byref(a)))
+        return a
+
+    def lookup(self, dev_desc, nb_desc, dev_leaf, kernel=LOOKUP_AUTO, dev_near_ties=None, stream=None):
+        check(lib.rbow_voc_lookup_ex(self.handle, _ptr(dev_desc), nb_desc, _ptr(dev_leaf), kernel,
+                                     _ptr(dev_near_ties), _stream(stream)))
+
+    def lookup_host(self, desc, kernel=LOOKUP_AUTO, return_near_ties=False):
+        """Convenience: host uint8[n][128] -> host uint32[n] leaf ids (synchronous)."""
+        desc = np.ascontiguousarray(desc, dtype=np.uint8)
+        n = desc.shape[0]
+        d = DeviceArray.from_numpy(desc) if n else None
+        out = DeviceArray((max(n, 1),), np.uint32)
+        ties = DeviceArray.zeros((1,), np.uint32)
+        self.lookup(d, n, out, kernel, ties)
+        hip.synchronize()
+        leaf = out.numpy(n)
+        return (leaf, int(ties.numpy()[0])) if return_near_ties else leaf
+
+    def serialize(self):
+        size = _sz(0)
+        check(lib.rbow_voc_serialize(self.handle, None, 0, ctypes.byref(size)))
+        buf = (ctypes.c_uint8 * size.value)()
+        check(lib.rbow_voc_serialize(self.handle, buf, size.value, ctypes.byref(size)))
+        return bytes(buf)
+
+    def level_weights(self, level):
+        p = lib.rbow_voc_level_weights(self.handle, level)
+        if not p:
+            raise IndexError(level)
+        n = self.branch_factor ** (level + 1)
+        return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+    def weights(self):
+        return np.concatenate([self.level_weights(l) for l in range(self.nb_levels)])
+
+    def build_stats(self):
+        st = BuildStats()
+        check(lib.rbow_voc_build_stats(self.handle, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in BuildStats._fields_}
+
+    def build_passes(self):
+        p = lib.rbow_voc_build_passes(self.handle)
+        return None if not p else np.ctypeslib.as_array(p, shape=(self.nb_nodes,)).copy()
+
+    def centroids(self, level):
+        n = self.branch_factor ** (level + 1) * 128
+        out = np.empty(n, np.float32)
+        check(lib.rbow_voc_copy_centroids(self.handle, level, out.ctypes.data, n))
+        return out.reshape(-1, self.branch_factor, 128)
+
+
+def build_options(max_iters=1000, seed_mode=0, seed=5489, centroid_type=0, kernel=0):
+    o = BuildOptions()
+    lib.rbow_build_options_default(ctypes.byref(o))
+    o.max_iters, o.seed_mode, o.seed, o.centroid_type, o.kernel = max_iters, seed_mode, seed, centroid_type, kernel
+    return o
+
+
+def build_vocabulary(dev_desc, nb_desc, nb_doc, nb_levels, branch_factor=16, stream=None, attr=SIFT_ATTR, **opts):
+    """buildVocabulary (RapidBoW.h:82-88); keyword options -> rbow_build_options."""
+    h = _vp()
+    o = build_options(**opts)
+    check(lib.rbow_build_vocabulary_ex(attr, branch_factor, _ptr(dev_desc), nb_desc, nb_doc, nb_levels,
+                                       ctypes.byref(o), _stream(stream), ctypes.byref(h)))
+    return Vocabulary(h.value)
+
+
+def deserialize_vocabulary(blob):
+    """deserializeVocabulary (RapidBoW.h:90)."""
+    h = _vp()
+    b = np.frombuffer(bytes(blob), np.uint8)
+    check(lib.rbow_voc_deserialize(b.ctypes.data if b.size else None, b.size, ctypes.byref(h)))
+    return Vocabulary(h.value)
+
+
+def kmeans_assign(dev_points, task_cnt, b, centroid_type, dev_centers, dev_assign, dev_counts, dev_sums,
+                  dev_changed, kernel=0, stream=None):
+    tc = np.ascontiguousarray(task_cnt, dtype=np.uint32)
+    check(lib.rbow_kmeans_assign(_ptr(dev_points), tc.ctypes.data, tc.size, b, centroid_type, _ptr(dev_centers),
+                                 _ptr(dev_assign), _ptr(dev_counts), _ptr(dev_sums), _ptr(dev_changed), kernel,
+                                 _stream(stream)))
+
+
+def kmeans_update(task_cnt, b, centroid_type, dev_centers, dev_counts, dev_sums, dev_changed, stream=None):
+    tc = np.ascontiguousarray(task_cnt, dtype=np.uint32)
+    check(lib.rbow_kmeans_update(tc.ctypes.data, tc.size, b, centroid_type, _ptr(dev_centers), _ptr(dev_counts),
+                                 _ptr(dev_sums), _ptr(dev_changed), _stream(stream)))
+
+
+def reformat_tiles(dev_desc, dev_idx, nb_tiles, tile_size, dev_tiles, stream=None):
+    check(lib.rbow_reformat_tiles(_ptr(dev_desc), _ptr(dev_idx), nb_tiles, tile_size, _ptr(dev_tiles),
+                                  _stream(stream)))
+
+
+def gather_rows(dev_desc, dev_idx, n, dev_out, stream=None):
+    check(lib.rbow_gather_rows(_ptr(dev_desc), _ptr(dev_idx), n, _ptr(dev_out), _stream(stream)))
+
+
+def synth_centres(centre_seed=0, nb_centres=2000):
+    out = np.empty((nb_centres, 128), np.float32)
+    lib.rbow_synth_centres(centre_seed, nb_centres, out.ctypes.data)
+    return out
+
+
+def synth_descriptors(n, seed, first=0, centre_seed=0, nb_centres=2000, centre_ids=None, sigma_scale=0.3):
+    """Seeded SIFT-like uint8[n][128] (host)."""
+    out = np.empty((n, 128), np.uint8)
+    ids = None
+    if centre_ids is not None:
+        ids = np.ascontiguousarray(centre_ids, dtype=np.uint32)
+        assert ids.size == n
+    lib.rbow_synth_descriptors(centre_seed, nb_centres, seed, first, n, None if ids is None else ids.ctypes.data,
+                               sigma_scale, out.ctypes.data)
+    return out

@@ -1,0 +1,296 @@
+// capi.cpp -- extern "C" boundary (include/rapidbow.h).  Every entry point
+// catches internal errors and turns them into an rbow_status plus a
+// thread-local message; nothing throws across the ABI.
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "vocabulary.hpp"
+
+using namespace rbow_amd;
+
+struct rbow_vocabulary {
+    std::unique_ptr<Vocabulary> v;
+    BuildStats stats;
+};
+
+namespace {
+thread_local std::string g_lastError;
+
+template <typename F>
+rbow_status guarded(F&& f) {
+    try {
+        f();
+        return RBOW_OK;
+    } catch (const Error& e) {
+        g_lastError = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        g_lastError = "host out of memory";
+        return RBOW_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_lastError = e.what();
+        return RBOW_ERR_INVALID_ARG;
+    }
+}
+
+hipStream_t hs(rbow_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+// RapidBoW.cpp:29-40: only {kUInt8, kL2, 128} is supported.
+void checkAttributes(const rbow_descriptor_attributes& a) {
+    RBOW_REQUIRE(a.data_type == RBOW_DATA_UINT8 && a.distance_type == RBOW_DIST_L2 && a.nb_dims == 128,
+                 RBOW_ERR_UNSUPPORTED,
+                 "Only {DataType::kUInt8, DistanceType::kL2, 128}, i.e. quantized SIFT, is supported");
+}
+
+void checkAligned(const void* p, const char* what) {
+    RBOW_REQUIRE((reinterpret_cast<uintptr_t>(p) & 15u) == 0, RBOW_ERR_INVALID_ARG,
+                 std::string(what) + " must be 16-byte aligned");
+}
+
+// Task table + tiles for the kernel-level k-means entries.
+struct KmScratch {
+    DevBuf<uint32_t> beg, cnt, active, passes, nbActive;
+    DevBuf<KmTile> tiles;
+    KmLevel k;
+    uint64_t rows = 0;
+    KmScratch(const uint32_t* taskCnt, uint32_t nbTasks, uint32_t b, uint32_t ct, hipStream_t s) {
+        RBOW_REQUIRE(taskCnt != nullptr || nbTasks == 0, RBOW_ERR_INVALID_ARG, "null task_cnt");
+        RBOW_REQUIRE(ct <= 1 && (b == 16 || (b == 32 && ct == 1)), RBOW_ERR_UNSUPPORTED,
+                     "b must be 16 (uint8 or float centers) or 32 (float centers)");
+        std::vector<uint32_t> hb(nbTasks);
+        std::vector<KmTile> ht;
+        for (uint32_t g = 0; g < nbTasks; g++) {
+            hb[g] = uint32_t(rows);
+            for (uint32_t r = 0; r < taskCnt[g]; r += 64)
+                ht.push_back(KmTile{g, uint32_t(rows) + r, std::min(64u, taskCnt[g] - r)});
+            rows += taskCnt[g];
+        }
+        RBOW_REQUIRE(rows < (1ull << 32), RBOW_ERR_CAPACITY, "too many points");
+        beg.resize(std::max(1u, nbTasks));
+        cnt.resize(std::max(1u, nbTasks));
+        active.resize(std::max(1u, nbTasks));
+        passes.resize(std::max(1u, nbTasks));
+        nbActive.resize(1);
+        tiles.resize(std::max<size_t>(1, ht.size()));
+        if (nbTasks) {
+            RBOW_HIP(hipMemcpyAsync(beg.get(), hb.data(), 4ull * nbTasks, hipMemcpyHostToDevice, s));
+            RBOW_HIP(hipMemcpyAsync(cnt.get(), taskCnt, 4ull * nbTasks, hipMemcpyHostToDevice, s));
+        }
+        if (!ht.empty())
+            RBOW_HIP(hipMemcpyAsync(tiles.get(), ht.data(), sizeof(KmTile) * ht.size(), hipMemcpyHostToDevice, s));
+        launchFillU32(active.get(), 1u, nbTasks, s);
+        launchFillU32(passes.get(), 0u, nbTasks, s);
+        k.b = b;
+        k.nbTasks = nbTasks;
+        k.nbTiles = uint32_t(ht.size());
+        k.centroidType = ct;
+        k.tiles = tiles.get();
+        k.taskBeg = beg.get();
+        k.taskCnt = cnt.get();
+        k.active = active.get();
+        k.passes = passes.get();
+        k.nbActive = nbActive.get();
+    }
+};
+}  // namespace
+
+extern "C" {
+
+const char* rbow_last_error(void) { return g_lastError.c_str(); }
+
+void rbow_build_options_default(rbow_build_options* o) {
+    if (!o) return;
+    o->max_iters = 1000;  // kmeans.h:118
+    o->seed_mode = 0;
+    o->seed = 5489u;      // std::mt19937_64 default seed (kmeans.cpp:145)
+    o->centroid_type = 0;
+    o->kernel = 0;
+}
+
+rbow_status rbow_build_vocabulary_ex(rbow_descriptor_attributes attr, uint32_t branch_factor, const void* dev_desc,
+                                     uint32_t nb_desc, uint32_t nb_doc, uint32_t nb_levels,
+                                     const rbow_build_options* opts, rbow_stream stream, rbow_vocabulary** out) {
+    return guarded([&] {
+        RBOW_REQUIRE(out != nullptr, RBOW_ERR_INVALID_ARG, "null output handle");
+        *out = nullptr;
+        rbow_build_options o;
+        rbow_build_options_default(&o);
+        if (opts) o = *opts;
+        checkAttributes(attr);
+        RBOW_REQUIRE(o.centroid_type <= 1, RBOW_ERR_INVALID_ARG, "centroid_type must be 0 or 1");
+        RBOW_REQUIRE(branch_factor == 16 || (branch_factor == 32 && o.centroid_type == 1), RBOW_ERR_UNSUPPORTED,
+                     "branchFactor must be 16 (or 32 with float32 centroids)");
+        RBOW_REQUIRE(dev_desc != nullptr || nb_desc == 0, RBOW_ERR_INVALID_ARG, "null descriptors");
+        checkAligned(dev_desc, "descriptors");
+        std::unique_ptr<rbow_vocabulary> h(new rbow_vocabulary);
+        h->v.reset(buildVocabulary(static_cast<const uint8_t*>(dev_desc), nb_desc, nb_doc, branch_factor, nb_levels,
+                                   o, hs(stream), &h->stats));
+        *out = h.release();
+    });
+}
+
+rbow_status rbow_build_vocabulary(rbow_descriptor_attributes attr, uint32_t branch_factor, const void* dev_desc,
+                                  uint32_t nb_desc, uint32_t nb_doc, uint32_t nb_levels, rbow_stream stream,
+                                  rbow_vocabulary** out) {
+    return rbow_build_vocabulary_ex(attr, branch_factor, dev_desc, nb_desc, nb_doc, nb_levels, nullptr, stream, out);
+}
+
+void rbow_voc_destroy(rbow_vocabulary* voc) { delete voc; }
+
+rbow_status rbow_voc_get_attributes(const rbow_vocabulary* voc, rbow_descriptor_attributes* attr) {
+    return guarded([&] {
+        RBOW_REQUIRE(voc && attr, RBOW_ERR_INVALID_ARG, "null argument");
+        *attr = rbow_descriptor_attributes{RBOW_DATA_UINT8, RBOW_DIST_L2, 128u};
+    });
+}
+uint32_t rbow_voc_branch_factor(const rbow_vocabulary* voc) { return voc ? voc->v->b : 0u; }
+uint32_t rbow_voc_nb_levels(const rbow_vocabulary* voc) { return voc ? voc->v->L : 0u; }
+uint32_t rbow_voc_centroid_type(const rbow_vocabulary* voc) { return voc ? voc->v->centroidType : 0u; }
+
+rbow_status rbow_voc_lookup_ex(const rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc, uint32_t* dev_leaf,
+                               int32_t kernel, uint32_t* dev_near_ties, rbow_stream stream) {
+    return guarded([&] {
+        RBOW_REQUIRE(voc != nullptr, RBOW_ERR_INVALID_ARG, "null vocabulary");
+        if (nb_desc == 0) return;
+        RBOW_REQUIRE(dev_desc && dev_leaf, RBOW_ERR_INVALID_ARG, "null descriptors or output");
+        checkAligned(dev_desc, "descriptors");
+        RBOW_REQUIRE(kernel >= RBOW_LOOKUP_AUTO && kernel <= RBOW_LOOKUP_MFMA, RBOW_ERR_INVALID_ARG, "bad kernel id");
+        const TreeView t = voc->v->view();
+        const auto* d = static_cast<const uint8_t*>(dev_desc);
+        if (kernel == RBOW_LOOKUP_PLAIN || t.L == 0) launchLookupPlain(t, d, nb_desc, dev_leaf, hs(stream));
+        else launchLookupMfma(t, d, nb_desc, dev_leaf, dev_near_ties, hs(stream));
+    });
+}
+
+rbow_status rbow_voc_lookup(const rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc, uint32_t* dev_leaf,
+                            rbow_stream stream) {
+    return rbow_voc_lookup_ex(voc, dev_desc, nb_desc, dev_leaf, RBOW_LOOKUP_AUTO, nullptr, stream);
+}
+
+rbow_status rbow_voc_serialize(const rbow_vocabulary* voc, uint8_t* blob, size_t capacity, size_t* size) {
+    return guarded([&] {
+        RBOW_REQUIRE(voc && size, RBOW_ERR_INVALID_ARG, "null argument");
+        const std::vector<uint8_t> b = serialize(*voc->v);
+        *size = b.size();
+        if (blob) {
+            RBOW_REQUIRE(capacity >= b.size(), RBOW_ERR_CAPACITY, "blob buffer too small");
+            std::memcpy(blob, b.data(), b.size());
+        }
+    });
+}
+
+rbow_status rbow_voc_deserialize(const uint8_t* blob, size_t size, rbow_vocabulary** out) {
+    return guarded([&] {
+        RBOW_REQUIRE(out != nullptr, RBOW_ERR_INVALID_ARG, "null output handle");
+        *out = nullptr;
+        std::unique_ptr<rbow_vocabulary> h(new rbow_vocabulary);
+        h->v.reset(deserialize(blob, size));
+        *out = h.release();
+    });
+}
+
+const float* rbow_voc_level_weights(const rbow_vocabulary* voc, uint32_t level) {
+    if (!voc || level >= voc->v->L) return nullptr;
+    return voc->v->weights.data() + levelBeg(voc->v->lg2b, level) * voc->v->b;
+}
+
+rbow_status rbow_voc_build_stats(const rbow_vocabulary* voc, rbow_build_stats* out) {
+    return guarded([&] {
+        RBOW_REQUIRE(voc && out, RBOW_ERR_INVALID_ARG, "null argument");
+        const BuildStats& st = voc->stats;
+        out->total_ms = st.totalMs;
+        out->assign_ms = st.assignMs;
+        out->update_ms = st.updateMs;
+        out->point_passes = st.pointPasses;
+        out->assign_ops = st.pointPasses * 2ull * voc->v->b * kDims;
+    });
+}
+
+const uint32_t* rbow_voc_build_passes(const rbow_vocabulary* voc) {
+    if (!voc || voc->stats.passes.empty()) return nullptr;
+    return voc->stats.passes.data();
+}
+
+rbow_status rbow_voc_copy_centroids(const rbow_vocabulary* voc, uint32_t level, float* host_out,
+                                    size_t capacity_floats) {
+    return guarded([&] {
+        RBOW_REQUIRE(voc && host_out, RBOW_ERR_INVALID_ARG, "null argument");
+        const Vocabulary& v = *voc->v;
+        RBOW_REQUIRE(level < v.L, RBOW_ERR_INVALID_ARG, "level out of range");
+        const uint64_t cnt = levelSize(v.lg2b, level) * v.b * kDims;
+        RBOW_REQUIRE(capacity_floats >= cnt, RBOW_ERR_CAPACITY, "output too small");
+        const uint64_t off = levelBeg(v.lg2b, level) * v.b * kDims;
+        if (v.centroidType == 1) {
+            RBOW_HIP(hipMemcpy(host_out, v.f32rows.get() + off, cnt * 4, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<uint8_t> tmp(cnt);
+            RBOW_HIP(hipMemcpy(tmp.data(), v.u8rows.get() + off, cnt, hipMemcpyDeviceToHost));
+            for (uint64_t i = 0; i < cnt; i++) host_out[i] = float(tmp[i]);
+        }
+    });
+}
+
+rbow_status rbow_kmeans_assign(const uint8_t* dev_points, const uint32_t* task_cnt, uint32_t nb_tasks, uint32_t b,
+                               uint32_t centroid_type, const void* dev_centers, uint8_t* dev_assign,
+                               uint32_t* dev_counts, int64_t* dev_sums, uint32_t* dev_changed, int32_t kernel,
+                               rbow_stream stream) {
+    return guarded([&] {
+        RBOW_REQUIRE(kernel == 0 || kernel == 1, RBOW_ERR_INVALID_ARG, "kernel must be 0 (MFMA) or 1 (plain)");
+        checkAligned(dev_points, "points");
+        KmScratch sc(task_cnt, nb_tasks, b, centroid_type, hs(stream));
+        if (sc.rows) RBOW_REQUIRE(dev_points && dev_assign, RBOW_ERR_INVALID_ARG, "null points or assignment");
+        RBOW_REQUIRE(dev_centers && dev_counts && dev_sums && dev_changed, RBOW_ERR_INVALID_ARG, "null argument");
+        sc.k.X = dev_points;
+        if (centroid_type == 0) sc.k.centersU8 = static_cast<uint8_t*>(const_cast<void*>(dev_centers));
+        else sc.k.centersF32 = static_cast<float*>(const_cast<void*>(dev_centers));
+        sc.k.assign = dev_assign;
+        sc.k.counts = dev_counts;
+        sc.k.sums = reinterpret_cast<long long*>(dev_sums);
+        sc.k.changed = dev_changed;
+        launchKmAssign(sc.k, kernel, hs(stream));
+        RBOW_HIP(hipStreamSynchronize(hs(stream)));  // scratch lifetime
+    });
+}
+
+rbow_status rbow_kmeans_update(const uint32_t* task_cnt, uint32_t nb_tasks, uint32_t b, uint32_t centroid_type,
+                               void* dev_centers, uint32_t* dev_counts, int64_t* dev_sums, uint32_t* dev_changed,
+                               rbow_stream stream) {
+    return guarded([&] {
+        KmScratch sc(task_cnt, nb_tasks, b, centroid_type, hs(stream));
+        RBOW_REQUIRE(dev_centers && dev_counts && dev_sums && dev_changed, RBOW_ERR_INVALID_ARG, "null argument");
+        if (centroid_type == 0) sc.k.centersU8 = static_cast<uint8_t*>(dev_centers);
+        else sc.k.centersF32 = static_cast<float*>(dev_centers);
+        sc.k.counts = dev_counts;
+        sc.k.sums = reinterpret_cast<long long*>(dev_sums);
+        sc.k.changed = dev_changed;
+        launchKmUpdate(sc.k, hs(stream));
+        RBOW_HIP(hipStreamSynchronize(hs(stream)));
+    });
+}
+
+rbow_status rbow_reformat_tiles(const void* dev_desc, const uint32_t* dev_idx, uint32_t nb_tiles, uint32_t tile_size,
+                                uint32_t* dev_tiles, rbow_stream stream) {
+    return guarded([&] {
+        RBOW_REQUIRE(tile_size == 16 || tile_size == 32, RBOW_ERR_INVALID_ARG, "tile_size must be 16 or 32");
+        if (!nb_tiles) return;
+        RBOW_REQUIRE(dev_desc && dev_idx && dev_tiles, RBOW_ERR_INVALID_ARG, "null argument");
+        checkAligned(dev_desc, "descriptors");
+        launchReformatTiles(static_cast<const uint8_t*>(dev_desc), dev_idx, nb_tiles, tile_size, dev_tiles,
+                            hs(stream));
+    });
+}
+
+rbow_status rbow_gather_rows(const void* dev_desc, const uint32_t* dev_idx, uint64_t n, uint8_t* dev_out,
+                             rbow_stream stream) {
+    return guarded([&] {
+        if (!n) return;
+        RBOW_REQUIRE(dev_desc && dev_idx && dev_out, RBOW_ERR_INVALID_ARG, "null argument");
+        checkAligned(dev_desc, "descriptors");
+        checkAligned(dev_out, "output");
+        launchGatherRows(static_cast<const uint8_t*>(dev_desc), dev_idx, n, dev_out, hs(stream));
+    });
+}
+
+}  // extern "C"

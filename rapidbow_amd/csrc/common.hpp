@@ -1,0 +1,99 @@
+// common.hpp -- shared internals of the RapidBoW MI355X library: errors, tree
+// index math, device buffers.  Host + device.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+#include "../../include/rapidbow.h"
+
+namespace rbow_amd {
+
+constexpr uint32_t kDims = 128;                 // SIFT dims (RapidBoW.h:50)
+constexpr uint32_t kIdxInvalid = 0xFFFFFFFFu;   // kmeans.cuh:31
+
+// Error carried from the library internals to the C ABI, where it becomes a
+// status code plus a thread-local message; nothing throws across the ABI.
+struct Error : std::runtime_error {
+    rbow_status status;
+    Error(rbow_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define RBOW_HIP(expr)                                                                            \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw ::rbow_amd::Error(RBOW_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define RBOW_REQUIRE(cond, status, msg)                      \
+    do {                                                     \
+        if (!(cond)) throw ::rbow_amd::Error((status), (msg)); \
+    } while (0)
+
+// FullTree.h:33-61: breadth-first full tree.  levelBeg(l) = (b^l - 1)/(b - 1).
+__host__ __device__ inline uint64_t levelBeg(uint32_t lg2b, uint32_t level) {
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < level; i++) acc = (acc << lg2b) + 1;
+    return acc;
+}
+__host__ __device__ inline uint64_t levelSize(uint32_t lg2b, uint32_t level) {
+    return uint64_t(1) << (lg2b * level);
+}
+
+inline uint32_t divUp(uint64_t a, uint64_t b) { return uint32_t((a + b - 1) / b); }
+
+// Owning device allocation.
+template <typename T>
+class DevBuf {
+public:
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { resize(n); }
+    ~DevBuf() { reset(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(std::exchange(o.p_, nullptr)), n_(std::exchange(o.n_, 0)) {}
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            reset();
+            p_ = std::exchange(o.p_, nullptr);
+            n_ = std::exchange(o.n_, 0);
+        }
+        return *this;
+    }
+    void resize(size_t n) {
+        if (n == n_) return;
+        reset();
+        if (n) {
+            hipError_t e = hipMalloc(&p_, n * sizeof(T));
+            if (e != hipSuccess) {
+                p_ = nullptr;
+                throw Error(e == hipErrorOutOfMemory ? RBOW_ERR_OUT_OF_MEMORY : RBOW_ERR_HIP,
+                            std::string("hipMalloc(") + std::to_string(n * sizeof(T)) + " B): " + hipGetErrorString(e));
+            }
+        }
+        n_ = n;
+    }
+    // grows only; keeps capacity
+    void reserve(size_t n) {
+        if (n > n_) resize(n);
+    }
+    void reset() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+    size_t bytes() const { return n_ * sizeof(T); }
+
+private:
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+}  // namespace rbow_amd

@@ -1,0 +1,86 @@
+// kernels.hpp -- host-side launchers of the gfx950 kernels.
+#pragma once
+#include "common.hpp"
+
+namespace rbow_amd {
+
+// ---------------------------------------------------------------- trees --
+// Device-resident vocabulary tree in the library's own layout:
+//   rows      [node][b][128]  centroid rows, child-major (uint8 or float32)
+//   u8 trees: normU[node*b+j]  = ||c||^2            (plain dot4 form)
+//             normS[node*b+j]  = ||c - 128||^2      (I8 MFMA form, offset operands)
+//   f32 trees: hi/lo          = bf16 split of the rows (c ~= hi + lo)
+//             cn2[node*b+j]   = ||c||^2 (fp32 of the binary64 value)
+//             e1[node*b+j]    = certified bound coefficient of ||x|| (see lookup.hip)
+//             e2[node*b+j]    = certified bound constant term
+struct TreeView {
+    uint32_t b = 16, lg2b = 4, L = 0, centroidType = 0;
+    const uint8_t* u8rows = nullptr;
+    const int32_t* normU = nullptr;
+    const int32_t* normS = nullptr;
+    const float* f32rows = nullptr;
+    const uint16_t* hi = nullptr;
+    const uint16_t* lo = nullptr;
+    const float* cn2 = nullptr;
+    const float* e1 = nullptr;
+    const float* e2 = nullptr;
+};
+
+// Derived per-child data (norms, bf16 split, bounds) for nodes [0, nbNodes).
+void launchPrepU8Tree(const uint8_t* rows, uint64_t nbChildren, int32_t* normU, int32_t* normS, hipStream_t s);
+void launchPrepF32Tree(const float* rows, uint64_t nbChildren, uint16_t* hi, uint16_t* lo, float* cn2, float* e1,
+                       float* e2, hipStream_t s);
+
+// Plain scalar per-descriptor descent (vocTreeLookUp.cu:48-160 semantics).
+void launchLookupPlain(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s);
+// Level-synchronous bucketed MFMA descent (I8 for uint8 trees, split-BF16 with
+// certified FP32 re-score for float trees).  Allocates its workspace stream-ordered.
+void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
+                      hipStream_t s);
+
+// ----------------------------------------------------------------- build --
+// Gather rows: out[p] = desc[idx[p]] (idx == kIdxInvalid -> 0xFF bytes).
+void launchGatherRows(const uint8_t* desc, const uint32_t* idx, uint64_t n, uint8_t* out, hipStream_t s);
+// reformatDesc.cu:38-74 tile layout: tiles[t][word][slot] (tileSize 32 or 16).
+void launchReformatTiles(const uint8_t* desc, const uint32_t* idx, uint32_t nbTiles, uint32_t tileSize,
+                         uint32_t* tiles, hipStream_t s);
+
+struct KmTile {        // one work item of the assign kernels: <= 64 rows of one task
+    uint32_t task;
+    uint32_t row;      // first row in the level's gathered array
+    uint32_t cnt;      // rows in this tile (1..64)
+};
+
+struct KmLevel {       // device state of one level's batched k-means
+    uint32_t b = 16, nbTasks = 0, nbTiles = 0, centroidType = 0;
+    const uint8_t* X = nullptr;      // gathered rows [N][128]
+    const KmTile* tiles = nullptr;
+    const uint32_t* taskBeg = nullptr;
+    const uint32_t* taskCnt = nullptr;
+    uint8_t* centersU8 = nullptr;    // [task][16][128]
+    float* centersF32 = nullptr;     // [task][b][128]
+    uint8_t* assign = nullptr;       // [N]
+    uint32_t* counts = nullptr;      // [task][b]
+    long long* sums = nullptr;       // [task][b][128]
+    uint32_t* changed = nullptr;     // [task]
+    uint32_t* active = nullptr;      // [task]
+    uint32_t* passes = nullptr;      // [task]
+    uint32_t* nbActive = nullptr;    // [1]
+};
+
+// centers[task][j] = desc[seedIdx[task*b+j]] (invalid -> 0xFF / 255.0f)
+void launchInitCenters(const KmLevel& k, const uint8_t* desc, const uint32_t* seedIdx, hipStream_t s);
+// Builder-defined k-means++ picks: seedIdx[task*b+j], masks[task].
+void launchKmeansPP(const uint8_t* desc, const uint32_t* order, const uint32_t* taskBeg, const uint32_t* taskCnt,
+                    uint32_t nbTasks, uint32_t b, uint64_t seed, uint32_t level, uint32_t* seedIdx, uint32_t* masks,
+                    hipStream_t s);
+void launchKmLevelReset(const KmLevel& k, uint64_t nbRows, hipStream_t s);
+// One assign pass over all active tasks; kernel 0 = MFMA, 1 = plain.
+void launchKmAssign(const KmLevel& k, int kernel, hipStream_t s);
+// Update pass: converged tasks become inactive, others move their centers
+// and clear their accumulators.  Counts remaining active tasks into nbActive.
+void launchKmUpdate(const KmLevel& k, hipStream_t s);
+
+void launchFillU32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+
+}  // namespace rbow_amd

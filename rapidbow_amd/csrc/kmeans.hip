@@ -1,0 +1,532 @@
+// kmeans.hip -- batched hierarchical k-means kernels (one launch covers every
+// task = tree node of a level, whatever their sizes).
+//
+// Reference semantics (kmeans.cu:120-198, 334-375; kmeans.cpp:118-133):
+//   assign: nearest of the b centers, ties to the lowest index; per task the
+//           counts, the per-center sums of every dimension and the number of
+//           points whose assignment changed.
+//   update: c = round(sum / count) (uint8 centroids, IEEE division, half away
+//           from zero) or c = float(sum) / float(count) (float centroids) for
+//           every center with count > 0; others keep their position.
+//   loop:   a task stops after the first pass with changed == 0 (no update on
+//           that pass) or after max_iters passes.
+// Sums are exact: int32 per wave in LDS, flushed into int64 per task.
+#include "kernels.hpp"
+
+namespace rbow_amd {
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr uint32_t kWaves = 4;  // waves per 256-thread block
+
+__device__ inline v4i ld16(const uint8_t* p) { return *reinterpret_cast<const v4i*>(p); }
+__device__ inline v4i toSigned(v4i v) {
+    return v ^ v4i{int(0x80808080), int(0x80808080), int(0x80808080), int(0x80808080)};
+}
+
+// Orders this wave's LDS accesses across lanes (the wave's own program order).
+__device__ inline void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ inline uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ inline uint64_t hash4(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
+    return mix64(mix64(mix64(seed ^ mix64(a)) ^ b) ^ c);
+}
+
+// ------------------------------------------------------------- gathers --
+__global__ void k_gatherRows(const uint8_t* __restrict__ desc, const uint32_t* __restrict__ idx, uint64_t n,
+                             uint8_t* __restrict__ out) {
+    for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n * 8;
+         t += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t p = t >> 3, part = t & 7;
+        const uint32_t i = idx[p];
+        uint4 v = make_uint4(~0u, ~0u, ~0u, ~0u);
+        if (i != kIdxInvalid) v = reinterpret_cast<const uint4*>(desc + uint64_t(i) * kDims)[part];
+        reinterpret_cast<uint4*>(out + p * kDims)[part] = v;
+    }
+}
+
+// reformatDesc.cu:38-74: tiles[t][word][slot] = word of desc[idx[t][slot]].
+__global__ void k_reformatTiles(const uint8_t* __restrict__ desc, const uint32_t* __restrict__ idx, uint32_t nbTiles,
+                                uint32_t tileSize, uint32_t* __restrict__ tiles) {
+    const uint64_t total = uint64_t(nbTiles) * 32 * tileSize;
+    for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t t = e / (32 * tileSize);
+        const uint32_t w = uint32_t((e / tileSize) % 32), s = uint32_t(e % tileSize);
+        const uint32_t i = idx[t * tileSize + s];
+        tiles[e] = i == kIdxInvalid ? ~0u : reinterpret_cast<const uint32_t*>(desc + uint64_t(i) * kDims)[w];
+    }
+}
+
+// ----------------------------------------------------------- seeding ----
+__global__ void k_initCenters(uint32_t b, uint32_t nbTasks, const uint8_t* __restrict__ desc,
+                              const uint32_t* __restrict__ seedIdx, uint8_t* __restrict__ cU8,
+                              float* __restrict__ cF32) {
+    const uint64_t total = uint64_t(nbTasks) * b * kDims;
+    for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t i = seedIdx[e / kDims];
+        const uint8_t v = i == kIdxInvalid ? uint8_t(0xFF) : desc[uint64_t(i) * kDims + e % kDims];
+        if (cU8) cU8[e] = v;
+        else cF32[e] = float(v);
+    }
+}
+
+__device__ inline uint64_t blockSumU64(uint64_t v, uint64_t* red) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += (uint64_t)__shfl_xor((unsigned long long)v, d, 64);
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    uint64_t s = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; w++) s += red[w];
+    return s;
+}
+
+// Builder-defined k-means++ (same definition as the oracle's):
+// candidates = one hashed member per stratum of the group (<= 256), then
+// D^2-weighted picks with integer weights and hashed draws.
+__global__ __launch_bounds__(256) void k_kmeansPP(const uint8_t* __restrict__ desc, const uint32_t* __restrict__ order,
+                                                 const uint32_t* __restrict__ taskBeg,
+                                                 const uint32_t* __restrict__ taskCnt, uint32_t b, uint64_t seed,
+                                                 uint32_t level, uint32_t* __restrict__ seedIdx,
+                                                 uint32_t* __restrict__ masks) {
+    __shared__ uint32_t sx[256][33];
+    __shared__ uint32_t sD[256], sNorm[256], sCand[256];
+    __shared__ uint64_t red[8], scanW[4];
+    __shared__ uint32_t sChosen[32], sPick;
+    const uint32_t g = blockIdx.x, tid = threadIdx.x;
+    const uint32_t n = taskCnt[g];
+    if (n == 0) {
+        if (tid < b) seedIdx[g * b + tid] = kIdxInvalid;
+        if (tid == 0) masks[g] = 0;
+        return;
+    }
+    const uint32_t m = n < 256 ? n : 256;
+    if (tid < m) {
+        uint32_t pos = tid;
+        if (n > 256) {
+            const uint64_t lo = uint64_t(tid) * n / m, hi = uint64_t(tid + 1) * n / m;
+            pos = uint32_t(lo + hash4(seed, level, g, 0x100000000ull + tid) % (hi - lo));
+        }
+        const uint32_t di = order[taskBeg[g] + pos];
+        sCand[tid] = di;
+        uint32_t nrm = 0;
+        for (uint32_t w = 0; w < 32; w++) {
+            const uint32_t v = reinterpret_cast<const uint32_t*>(desc + uint64_t(di) * kDims)[w];
+            sx[tid][w] = v;
+            nrm = __builtin_amdgcn_udot4(v, v, nrm, false);
+        }
+        sNorm[tid] = nrm;
+        sD[tid] = 0xFFFFFFFFu;
+    }
+    if (tid == 0) sChosen[0] = uint32_t(hash4(seed, level, g, 0x200000000ull) % m);
+    __syncthreads();
+    auto relax = [&](uint32_t c) {
+        if (tid < m) {
+            uint32_t dot = 0;
+            for (uint32_t w = 0; w < 32; w++) dot = __builtin_amdgcn_udot4(sx[tid][w], sx[c][w], dot, false);
+            const uint32_t d = sNorm[tid] + sNorm[c] - 2 * dot;
+            sD[tid] = min(sD[tid], d);
+        }
+        __syncthreads();
+    };
+    relax(sChosen[0]);
+    uint32_t nValid = 1;
+    for (uint32_t r = 1; r < b; r++) {
+        const uint64_t dv = tid < m ? sD[tid] : 0;
+        const uint64_t total = blockSumU64(dv, red);
+        if (total == 0) {
+            if (tid == 0) sChosen[r] = sChosen[r - 1];
+            __syncthreads();
+            continue;
+        }
+        const uint64_t t = hash4(seed, level, g, 0x300000000ull + r) % total;
+        // inclusive prefix of D over candidates
+        const uint32_t lane = tid & 63, wave = tid >> 6;
+        uint64_t inc = dv;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t o = (uint64_t)__shfl_up((unsigned long long)inc, d, 64);
+            if (lane >= uint32_t(d)) inc += o;
+        }
+        if (lane == 63) scanW[wave] = inc;
+        __syncthreads();
+        uint64_t off = 0;
+        for (uint32_t w = 0; w < wave; w++) off += scanW[w];
+        inc += off;
+        if (tid < m && inc > t && inc - dv <= t) sPick = tid;
+        __syncthreads();
+        if (tid == 0) sChosen[r] = sPick;
+        nValid++;
+        __syncthreads();
+        relax(sChosen[r]);
+    }
+    if (tid < b) seedIdx[g * b + tid] = sCand[sChosen[tid]];
+    if (tid == 0) masks[g] = nValid >= 32 ? 0xFFFFFFFFu : ((1u << nValid) - 1u);
+}
+
+// ------------------------------------------------------- accumulators ----
+// Per-wave LDS accumulators: sums [B][128] u32 and counts [B].  A wave owns a
+// contiguous range of tiles and flushes into the task's int64 sums whenever
+// the task changes.
+template <int B>
+struct WaveAcc {
+    uint32_t* sums;   // [B][128]
+    uint32_t* cnt;    // [B]
+    __device__ void clear(uint32_t lane) {
+        for (uint32_t e = lane; e < B * kDims; e += 64) sums[e] = 0;
+        if (lane < B) cnt[lane] = 0;
+        waveSync();
+    }
+    __device__ void flush(uint32_t lane, uint32_t task, uint32_t changedLocal, const KmLevel& k) {
+        unsigned long long* gs = reinterpret_cast<unsigned long long*>(k.sums) + uint64_t(task) * B * kDims;
+        waveSync();
+        for (uint32_t e = lane; e < B * kDims; e += 64) {
+            const uint32_t v = sums[e];
+            if (v) atomicAdd(&gs[e], (unsigned long long)v);
+            sums[e] = 0;
+        }
+        if (lane < B) {
+            if (cnt[lane]) atomicAdd(&k.counts[task * B + lane], cnt[lane]);
+            cnt[lane] = 0;
+        }
+        if (lane == 0 && changedLocal) atomicAdd(&k.changed[task], changedLocal);
+        waveSync();
+    }
+};
+
+// Epilogue shared by all assign kernels: lane r of the wave holds the new
+// assignment a of tile row r (valid when r < cnt).
+template <int B>
+__device__ inline uint32_t tileEpilogue(const KmLevel& k, const KmTile& t, uint32_t lane, uint32_t a, WaveAcc<B>& acc) {
+    const bool valid = lane < t.cnt;
+    uint32_t changed = 0;
+    if (valid) {
+        const uint32_t prev = k.assign[t.row + lane];
+        k.assign[t.row + lane] = uint8_t(a);
+        changed = prev != a;
+        atomicAdd(&acc.cnt[a], 1u);
+    }
+    const uint32_t nbChanged = __builtin_popcountll(__ballot(changed));
+    // sums: lane owns dims 2*lane and 2*lane+1
+    for (uint32_t r = 0; r < t.cnt; r++) {
+        const uint32_t ar = __builtin_amdgcn_readlane(a, r);
+        const uint32_t v = reinterpret_cast<const uint16_t*>(k.X + uint64_t(t.row + r) * kDims)[lane];
+        uint2* p = reinterpret_cast<uint2*>(acc.sums + ar * kDims + 2 * lane);
+        uint2 s = *p;
+        s.x += v & 0xFFu;
+        s.y += v >> 8;
+        *p = s;
+    }
+    return nbChanged;
+}
+
+// Contiguous tile range of wave w among W waves.
+__device__ inline void waveRange(uint32_t nbTiles, uint32_t& beg, uint32_t& end) {
+    const uint32_t W = gridDim.x * kWaves, w = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    beg = uint32_t(uint64_t(nbTiles) * w / W);
+    end = uint32_t(uint64_t(nbTiles) * (w + 1) / W);
+}
+
+// ----------------------------------------------- assign: uint8, I8 MFMA --
+__global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
+    __shared__ uint32_t sSums[kWaves][16 * kDims];
+    __shared__ uint32_t sCnt[kWaves][16];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t col = lane & 15, grp = lane >> 4;
+    WaveAcc<16> acc{sSums[wave], sCnt[wave]};
+    acc.clear(lane);
+    uint32_t beg, end;
+    waveRange(k.nbTiles, beg, end);
+    uint32_t curTask = kIdxInvalid, changedLocal = 0;
+    v4i b0 = {0, 0, 0, 0}, b1 = {0, 0, 0, 0};
+    int ns = 0;
+    for (uint32_t ti = beg; ti < end; ti++) {
+        const KmTile t = k.tiles[ti];
+        if (!k.active[t.task]) continue;
+        if (t.task != curTask) {
+            if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
+            changedLocal = 0;
+            curTask = t.task;
+            const uint8_t* crow = k.centersU8 + (uint64_t(curTask) * 16 + col) * kDims;
+            b0 = toSigned(ld16(crow + grp * 16));
+            b1 = toSigned(ld16(crow + 64 + grp * 16));
+            int part = 0;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                part = __builtin_amdgcn_sdot4(b0[e], b0[e], part, false);
+                part = __builtin_amdgcn_sdot4(b1[e], b1[e], part, false);
+            }
+            part += __shfl_xor(part, 16, 64);
+            part += __shfl_xor(part, 32, 64);
+            ns = part;  // ||c - 128||^2 of center `col`
+        }
+        uint32_t myA = 0;
+        for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
+            const uint32_t r = rb * 16 + col;
+            v4i a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
+            if (r < t.cnt) {
+                const uint8_t* x = k.X + uint64_t(t.row + r) * kDims;
+                a0 = toSigned(ld16(x + grp * 16));
+                a1 = toSigned(ld16(x + 64 + grp * 16));
+            }
+            v4i d = {0, 0, 0, 0};
+            d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b0, d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b1, d, 0, 0, 0);
+            uint32_t key[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) key[i] = (uint32_t(ns - 2 * d[i] + (1 << 25)) << 4) | col;
+#pragma unroll
+            for (int m = 1; m < 16; m <<= 1)
+#pragma unroll
+                for (int i = 0; i < 4; i++) key[i] = min(key[i], uint32_t(__shfl_xor(int(key[i]), m, 16)));
+            // lane (16*grp + i) holds row 4*grp + i of this block; move to lane = row
+            const uint32_t mine = key[col & 3];
+            const uint32_t w = lane & 15;
+            const uint32_t src = 16 * (w >> 2) + (w & 3);
+            const uint32_t got = uint32_t(__shfl(int(mine), int(src), 64));
+            if ((lane >> 4) == rb) myA = got & 15u;
+        }
+        changedLocal += tileEpilogue<16>(k, t, lane, myA, acc);
+    }
+    if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
+}
+
+// ----------------------------------------------- assign: uint8, plain ----
+// 16 lanes per point (lane j: center j), 4 points per step.
+__global__ __launch_bounds__(256) void k_assignU8Plain(KmLevel k) {
+    __shared__ uint32_t sSums[kWaves][16 * kDims];
+    __shared__ uint32_t sCnt[kWaves][16];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t j = lane & 15, sub = lane >> 4;
+    WaveAcc<16> acc{sSums[wave], sCnt[wave]};
+    acc.clear(lane);
+    uint32_t beg, end;
+    waveRange(k.nbTiles, beg, end);
+    uint32_t curTask = kIdxInvalid, changedLocal = 0;
+    uint32_t cw[32];
+    int nu = 0;
+    for (uint32_t ti = beg; ti < end; ti++) {
+        const KmTile t = k.tiles[ti];
+        if (!k.active[t.task]) continue;
+        if (t.task != curTask) {
+            if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
+            changedLocal = 0;
+            curTask = t.task;
+            const uint32_t* c = reinterpret_cast<const uint32_t*>(k.centersU8 + (uint64_t(curTask) * 16 + j) * kDims);
+            uint32_t s = 0;
+#pragma unroll
+            for (int w = 0; w < 32; w++) {
+                cw[w] = c[w];
+                s = __builtin_amdgcn_udot4(cw[w], cw[w], s, false);
+            }
+            nu = int(s);
+        }
+        uint32_t myA = 0;
+        for (uint32_t r0 = 0; r0 < t.cnt; r0 += 4) {
+            const uint32_t r = r0 + sub;
+            uint32_t dot = 0;
+            if (r < t.cnt) {
+                const uint32_t* x = reinterpret_cast<const uint32_t*>(k.X + uint64_t(t.row + r) * kDims);
+#pragma unroll
+                for (int w = 0; w < 32; w++) dot = __builtin_amdgcn_udot4(x[w], cw[w], dot, false);
+            }
+            uint32_t key = (uint32_t(nu - 2 * int(dot) + (1 << 25)) << 4) | j;
+#pragma unroll
+            for (int m = 1; m < 16; m <<= 1) key = min(key, uint32_t(__shfl_xor(int(key), m, 16)));
+            const uint32_t got = uint32_t(__shfl(int(key), int(16 * (lane & 3)), 64));
+            if ((lane >> 2) == (r0 >> 2)) myA = got & 15u;
+        }
+        changedLocal += tileEpilogue<16>(k, t, lane, myA, acc);
+    }
+    if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
+}
+
+// ----------------------------------------------- assign: float, plain ----
+// B lanes per point (lane j: center j, its own fma chain over the 128 dims in
+// order), 64/B points per step.  Centers are read from global (L2 resident).
+template <int B>
+__global__ __launch_bounds__(256) void k_assignF32Plain(KmLevel k) {
+    __shared__ uint32_t sSums[kWaves][B * kDims];
+    __shared__ uint32_t sCnt[kWaves][B];
+    constexpr uint32_t PPS = 64 / B;  // points per step
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t j = lane % B, sub = lane / B;
+    WaveAcc<B> acc{sSums[wave], sCnt[wave]};
+    acc.clear(lane);
+    uint32_t beg, end;
+    waveRange(k.nbTiles, beg, end);
+    uint32_t curTask = kIdxInvalid, changedLocal = 0;
+    for (uint32_t ti = beg; ti < end; ti++) {
+        const KmTile t = k.tiles[ti];
+        if (!k.active[t.task]) continue;
+        if (t.task != curTask) {
+            if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
+            changedLocal = 0;
+            curTask = t.task;
+        }
+        const float4* cr = reinterpret_cast<const float4*>(k.centersF32 + (uint64_t(curTask) * B + j) * kDims);
+        uint32_t myA = 0;
+        for (uint32_t r0 = 0; r0 < t.cnt; r0 += PPS) {
+            const uint32_t r = r0 + sub;
+            float a = 0.0f;
+            if (r < t.cnt) {
+                const uint4* x = reinterpret_cast<const uint4*>(k.X + uint64_t(t.row + r) * kDims);
+#pragma unroll 2
+                for (int w4 = 0; w4 < 8; w4++) {
+                    const uint4 xv = x[w4];
+                    const uint32_t xw[4] = {xv.x, xv.y, xv.z, xv.w};
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const float4 c = cr[w4 * 4 + q];
+                        float d = float(xw[q] & 0xFFu) - c.x;
+                        a = fmaf(d, d, a);
+                        d = float((xw[q] >> 8) & 0xFFu) - c.y;
+                        a = fmaf(d, d, a);
+                        d = float((xw[q] >> 16) & 0xFFu) - c.z;
+                        a = fmaf(d, d, a);
+                        d = float(xw[q] >> 24) - c.w;
+                        a = fmaf(d, d, a);
+                    }
+                }
+            }
+            float bd = a;
+            uint32_t bj = j;
+#pragma unroll
+            for (int m = 1; m < B; m <<= 1) {
+                const float od = __shfl_xor(bd, m, B);
+                const uint32_t oj = uint32_t(__shfl_xor(int(bj), m, B));
+                if (od < bd || (od == bd && oj < bj)) {
+                    bd = od;
+                    bj = oj;
+                }
+            }
+            // point r0 + s lives in lanes [s*B, s*B+B)
+            const uint32_t got = uint32_t(__shfl(int(bj), int(B * (lane % PPS)), 64));
+            if (lane / PPS == r0 / PPS) myA = got;
+        }
+        changedLocal += tileEpilogue<B>(k, t, lane, myA, acc);
+    }
+    if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
+}
+
+// ------------------------------------------------------------ update -----
+template <int B>
+__global__ __launch_bounds__(256) void k_update(KmLevel k) {
+    const uint32_t g = blockIdx.x;
+    if (!k.active[g]) return;
+    __syncthreads();
+    if (k.changed[g] == 0) {  // converged on this pass: no update, task stops
+        if (threadIdx.x == 0) {
+            k.active[g] = 0;
+            k.passes[g]++;
+        }
+        return;
+    }
+    unsigned long long* sums = reinterpret_cast<unsigned long long*>(k.sums) + uint64_t(g) * B * kDims;
+    for (uint32_t e = threadIdx.x; e < B * kDims; e += blockDim.x) {
+        const uint32_t c = k.counts[g * B + e / kDims];
+        if (c) {
+            const float q = float((long long)sums[e]) / float(c);
+            if (k.centersU8) k.centersU8[uint64_t(g) * B * kDims + e] = uint8_t(roundf(q));
+            else k.centersF32[uint64_t(g) * B * kDims + e] = q;
+        }
+        sums[e] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x < B) k.counts[g * B + threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        k.changed[g] = 0;
+        k.passes[g]++;
+        atomicAdd(k.nbActive, 1u);
+    }
+}
+
+__global__ void k_levelReset(KmLevel k, uint64_t nbRows) {
+    const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x, stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = tid; i < nbRows; i += stride) k.assign[i] = 0xFF;  // "no previous assignment"
+    for (uint64_t i = tid; i < uint64_t(k.nbTasks) * k.b * kDims; i += stride) k.sums[i] = 0;
+    for (uint64_t i = tid; i < uint64_t(k.nbTasks) * k.b; i += stride) k.counts[i] = 0;
+    for (uint64_t i = tid; i < k.nbTasks; i += stride) {
+        k.changed[i] = 0;
+        k.active[i] = 1;
+        k.passes[i] = 0;
+    }
+}
+
+uint32_t gridFor(uint64_t work, uint32_t perBlock, uint32_t cap) {
+    const uint64_t g = (work + perBlock - 1) / perBlock;
+    return uint32_t(g == 0 ? 1 : (g > cap ? cap : g));
+}
+
+}  // namespace
+
+void launchGatherRows(const uint8_t* desc, const uint32_t* idx, uint64_t n, uint8_t* out, hipStream_t s) {
+    if (!n) return;
+    k_gatherRows<<<gridFor(n * 8, 256, 8192), 256, 0, s>>>(desc, idx, n, out);
+    RBOW_HIP(hipGetLastError());
+}
+
+void launchReformatTiles(const uint8_t* desc, const uint32_t* idx, uint32_t nbTiles, uint32_t tileSize,
+                         uint32_t* tiles, hipStream_t s) {
+    if (!nbTiles) return;
+    k_reformatTiles<<<gridFor(uint64_t(nbTiles) * 32 * tileSize, 256, 8192), 256, 0, s>>>(desc, idx, nbTiles, tileSize,
+                                                                                       tiles);
+    RBOW_HIP(hipGetLastError());
+}
+
+void launchInitCenters(const KmLevel& k, const uint8_t* desc, const uint32_t* seedIdx, hipStream_t s) {
+    if (!k.nbTasks) return;
+    k_initCenters<<<gridFor(uint64_t(k.nbTasks) * k.b * kDims, 256, 8192), 256, 0, s>>>(
+        k.b, k.nbTasks, desc, seedIdx, k.centroidType == 0 ? k.centersU8 : nullptr,
+        k.centroidType == 0 ? nullptr : k.centersF32);
+    RBOW_HIP(hipGetLastError());
+}
+
+void launchKmeansPP(const uint8_t* desc, const uint32_t* order, const uint32_t* taskBeg, const uint32_t* taskCnt,
+                    uint32_t nbTasks, uint32_t b, uint64_t seed, uint32_t level, uint32_t* seedIdx, uint32_t* masks,
+                    hipStream_t s) {
+    if (!nbTasks) return;
+    k_kmeansPP<<<nbTasks, 256, 0, s>>>(desc, order, taskBeg, taskCnt, b, seed, level, seedIdx, masks);
+    RBOW_HIP(hipGetLastError());
+}
+
+void launchKmLevelReset(const KmLevel& k, uint64_t nbRows, hipStream_t s) {
+    k_levelReset<<<gridFor(std::max<uint64_t>(nbRows, uint64_t(k.nbTasks) * k.b * kDims), 256, 4096), 256, 0, s>>>(
+        k, nbRows);
+    RBOW_HIP(hipGetLastError());
+}
+
+void launchKmAssign(const KmLevel& k, int kernel, hipStream_t s) {
+    if (!k.nbTiles) return;
+    const uint32_t grid = gridFor(k.nbTiles, kWaves, 2048);
+    if (k.centroidType == 0) {
+        if (kernel == 1) k_assignU8Plain<<<grid, 256, 0, s>>>(k);
+        else k_assignU8Mfma<<<grid, 256, 0, s>>>(k);
+    } else if (k.b == 32) {
+        k_assignF32Plain<32><<<grid, 256, 0, s>>>(k);
+    } else {
+        k_assignF32Plain<16><<<grid, 256, 0, s>>>(k);
+    }
+    RBOW_HIP(hipGetLastError());
+}
+
+void launchKmUpdate(const KmLevel& k, hipStream_t s) {
+    if (!k.nbTasks) return;
+    if (k.b == 32) k_update<32><<<k.nbTasks, 256, 0, s>>>(k);
+    else k_update<16><<<k.nbTasks, 256, 0, s>>>(k);
+    RBOW_HIP(hipGetLastError());
+}
+
+}  // namespace rbow_amd

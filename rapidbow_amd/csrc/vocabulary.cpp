@@ -1,0 +1,402 @@
+// vocabulary.cpp -- hierarchical k-means build driver, serialisation.
+//
+// Level loop of Vocabulary.cpp:67-99: every node of a level is one k-means
+// task; all tasks of the level run in the same kernel launches (no 128-task
+// batches, no per-batch device sync).  Convergence is tracked per task on the
+// device; the host only polls a counter when the pass cap is large.
+#include "vocabulary.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <unordered_set>
+
+namespace rbow_amd {
+
+void Vocabulary::allocate(uint32_t b_, uint32_t L_, uint32_t ct) {
+    b = b_;
+    lg2b = b_ == 16 ? 4 : 5;
+    L = L_;
+    centroidType = ct;
+    nbNodes = levelBeg(lg2b, L);
+    const uint64_t nc = nbNodes * b;
+    if (ct == 0) {
+        u8rows.resize(nc * kDims);
+        normU.resize(nc);
+        normS.resize(nc);
+    } else {
+        f32rows.resize(nc * kDims);
+        hi.resize(nc * kDims);
+        lo.resize(nc * kDims);
+        cn2.resize(nc);
+        e1.resize(nc);
+        e2.resize(nc);
+    }
+    masks.assign(nbNodes, 0u);
+    weights.assign(nc, 0.0f);
+}
+
+void Vocabulary::prepare(hipStream_t s) {
+    if (centroidType == 0) launchPrepU8Tree(u8rows.get(), nbChildren(), normU.get(), normS.get(), s);
+    else launchPrepF32Tree(f32rows.get(), nbChildren(), hi.get(), lo.get(), cn2.get(), e1.get(), e2.get(), s);
+}
+
+TreeView Vocabulary::view() const {
+    TreeView t;
+    t.b = b;
+    t.lg2b = lg2b;
+    t.L = L;
+    t.centroidType = centroidType;
+    t.u8rows = u8rows.get();
+    t.normU = normU.get();
+    t.normS = normS.get();
+    t.f32rows = f32rows.get();
+    t.hi = hi.get();
+    t.lo = lo.get();
+    t.cn2 = cn2.get();
+    t.e1 = e1.get();
+    t.e2 = e2.get();
+    return t;
+}
+
+namespace {
+
+// kmeans.cpp:141-168 randPick: pick min(randSpace, nbToPick) distinct members
+// with the persistent engine, sorted; pad with the last pick (0 if none).
+uint32_t randPick(std::mt19937_64& rng, uint32_t* dst, uint32_t randSpace, uint32_t nbToPick) {
+    const uint32_t nbPick = std::min(randSpace, nbToPick);
+    std::unordered_set<uint32_t> results;
+    if (nbPick * 16 < randSpace) {
+        while (results.size() != nbPick) results.insert(uint32_t(rng() % randSpace));
+        std::copy_n(results.begin(), nbPick, dst);
+        std::sort(dst, dst + nbPick);
+    } else {
+        std::vector<uint32_t> space(randSpace);
+        std::iota(space.begin(), space.end(), 0u);
+        std::shuffle(space.begin(), space.end(), rng);
+        std::copy_n(space.begin(), nbPick, dst);
+        std::sort(dst, dst + nbPick);
+    }
+    if (nbPick < nbToPick) std::fill(dst + nbPick, dst + nbToPick, nbPick != 0 ? dst[nbPick - 1u] : 0u);
+    return nbPick;
+}
+
+template <typename T>
+void h2d(T* dst, const T* src, size_t n, hipStream_t s) {
+    if (n) RBOW_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <typename T>
+void d2h(T* dst, const T* src, size_t n, hipStream_t s) {
+    if (n) RBOW_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+}  // namespace
+
+Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nbDoc, uint32_t b, uint32_t L,
+                            const rbow_build_options& opts, hipStream_t s, BuildStats* stats) {
+    RBOW_REQUIRE(nbDesc >= b, RBOW_ERR_INSUFFICIENT_DESC, "Insufficient descriptors");  // Vocabulary.cpp:39-41
+    RBOW_REQUIRE(nbDoc > 0, RBOW_ERR_NB_DOC, "nbDoc must be positive");                 // Vocabulary.cpp:42-44
+    RBOW_REQUIRE(opts.max_iters > 0, RBOW_ERR_INVALID_ARG, "max_iters must be positive");
+    RBOW_REQUIRE(opts.seed_mode <= 1, RBOW_ERR_INVALID_ARG, "seed_mode must be 0 or 1");
+    RBOW_REQUIRE(L >= 1 && uint64_t(L) * (b == 16 ? 4 : 5) <= 28, RBOW_ERR_CAPACITY,
+                 "tree too deep (at most 2^28 leaves)");
+    const uint32_t ct = opts.centroid_type;
+    const int kernel = int(opts.kernel);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<hipEvent_t> ev;  // start/stop pairs around assign and update launches
+    struct EventsFree {
+        std::vector<hipEvent_t>& v;
+        ~EventsFree() {
+            for (auto e : v) (void)hipEventDestroy(e);
+        }
+    } evGuard{ev};
+    auto mark = [&](size_t i) {
+        while (ev.size() <= i) {
+            hipEvent_t e;
+            RBOW_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        RBOW_HIP(hipEventRecord(ev[i], s));
+    };
+    double assignMs = 0, updateMs = 0;
+    uint64_t pointPasses = 0;
+
+    std::unique_ptr<Vocabulary> voc(new Vocabulary);
+    voc->allocate(b, L, ct);
+    const uint32_t lg2b = voc->lg2b;
+    std::vector<uint32_t> occ(voc->nbChildren(), 0u);
+    std::vector<uint32_t> passesAll(voc->nbNodes, 0u);
+
+    // host grouping state: descriptor order of the level, tasks contiguous
+    std::vector<uint32_t> order(nbDesc), next(nbDesc);
+    std::iota(order.begin(), order.end(), 0u);
+    std::vector<uint32_t> taskBeg{0u}, taskCnt{nbDesc};
+
+    DevBuf<uint32_t> dOrder(nbDesc);
+    DevBuf<uint8_t> dX(uint64_t(nbDesc) * kDims), dAssign(nbDesc);
+    DevBuf<uint32_t> dTaskBeg, dTaskCnt, dCounts, dChanged, dActive, dPasses, dNbActive(1), dSeed, dMasks;
+    DevBuf<long long> dSums;
+    DevBuf<KmTile> dTiles;
+    std::vector<uint8_t> hAssign(nbDesc);
+    std::vector<uint32_t> hPasses, hMasks;
+    std::vector<KmTile> tiles;
+    std::mt19937_64 rng(opts.seed);
+    uint32_t* hNbActive = nullptr;
+    RBOW_HIP(hipHostMalloc(&hNbActive, sizeof(uint32_t)));
+    struct PinnedFree {
+        uint32_t* p;
+        ~PinnedFree() { (void)hipHostFree(p); }
+    } pinnedGuard{hNbActive};
+
+    for (uint32_t level = 0; level < L; level++) {
+        const uint32_t nG = uint32_t(taskBeg.size());
+        const uint64_t nodeBase = levelBeg(lg2b, level);
+        // ---- tiles of <= 64 rows, tasks contiguous (DescGrouping, kmeans.cpp:27-42) ----
+        tiles.clear();
+        for (uint32_t g = 0; g < nG; g++)
+            for (uint32_t r = 0; r < taskCnt[g]; r += 64) tiles.push_back(KmTile{g, taskBeg[g] + r, std::min(64u, taskCnt[g] - r)});
+        dTaskBeg.reserve(nG);
+        dTaskCnt.reserve(nG);
+        dChanged.reserve(nG);
+        dActive.reserve(nG);
+        dPasses.reserve(nG);
+        dMasks.reserve(nG);
+        dSeed.reserve(uint64_t(nG) * b);
+        dCounts.reserve(uint64_t(nG) * b);
+        dSums.reserve(uint64_t(nG) * b * kDims);
+        dTiles.reserve(std::max<size_t>(tiles.size(), 1));
+        h2d(dOrder.get(), order.data(), nbDesc, s);
+        h2d(dTaskBeg.get(), taskBeg.data(), nG, s);
+        h2d(dTaskCnt.get(), taskCnt.data(), nG, s);
+        h2d(dTiles.get(), tiles.data(), tiles.size(), s);
+        launchGatherRows(devDesc, dOrder.get(), nbDesc, dX.get(), s);
+
+        KmLevel k;
+        k.b = b;
+        k.nbTasks = nG;
+        k.nbTiles = uint32_t(tiles.size());
+        k.centroidType = ct;
+        k.X = dX.get();
+        k.tiles = dTiles.get();
+        k.taskBeg = dTaskBeg.get();
+        k.taskCnt = dTaskCnt.get();
+        if (ct == 0) k.centersU8 = voc->u8rows.get() + nodeBase * b * kDims;
+        else k.centersF32 = voc->f32rows.get() + nodeBase * b * kDims;
+        k.assign = dAssign.get();
+        k.counts = dCounts.get();
+        k.sums = dSums.get();
+        k.changed = dChanged.get();
+        k.active = dActive.get();
+        k.passes = dPasses.get();
+        k.nbActive = dNbActive.get();
+
+        // ---- seeds (kmeans.cpp:170-196 reference picks, or k-means++) ----
+        if (opts.seed_mode == 0) {
+            std::vector<uint32_t> seedIdx(uint64_t(nG) * b);
+            uint32_t pos[32];
+            for (uint32_t g = 0; g < nG; g++) {
+                const uint32_t n = taskCnt[g];
+                const uint32_t nValid = randPick(rng, pos, n, b);
+                for (uint32_t j = 0; j < b; j++) {
+                    uint32_t idx = kIdxInvalid;
+                    if (n > 0) {
+                        idx = order[taskBeg[g] + pos[j]];
+                    } else {
+                        // empty group: the reference reads the next group's first
+                        // index (kmeans.cpp:180-186); none left -> invalid (0xFF row)
+                        for (uint32_t h = g + 1; h < nG; h++)
+                            if (taskCnt[h]) {
+                                idx = order[taskBeg[h]];
+                                break;
+                            }
+                    }
+                    seedIdx[uint64_t(g) * b + j] = idx;
+                }
+                voc->masks[nodeBase + g] = nValid >= 32 ? 0xFFFFFFFFu : ((1u << nValid) - 1u);
+            }
+            h2d(dSeed.get(), seedIdx.data(), seedIdx.size(), s);
+        } else {
+            launchKmeansPP(devDesc, dOrder.get(), dTaskBeg.get(), dTaskCnt.get(), nG, b, opts.seed, level,
+                           dSeed.get(), dMasks.get(), s);
+            hMasks.resize(nG);
+            d2h(hMasks.data(), dMasks.get(), nG, s);
+        }
+        launchInitCenters(k, devDesc, dSeed.get(), s);
+        launchKmLevelReset(k, nbDesc, s);
+
+        // ---- Lloyd passes (kmeans.cpp:118-133) ----
+        const bool poll = opts.max_iters > 16;
+        uint32_t nbLaunched = 0;
+        for (uint32_t it = 0; it < opts.max_iters; it++) {
+            if (stats) mark(4 * it);
+            launchKmAssign(k, kernel, s);
+            if (stats) mark(4 * it + 1);
+            if (poll) launchFillU32(dNbActive.get(), 0u, 1, s);
+            if (stats) mark(4 * it + 2);
+            launchKmUpdate(k, s);
+            if (stats) mark(4 * it + 3);
+            nbLaunched = it + 1;
+            if (poll && (it % 4 == 3)) {  // kmeans.cpp:125-131 polls every 4 passes
+                d2h(hNbActive, dNbActive.get(), 1, s);
+                RBOW_HIP(hipStreamSynchronize(s));
+                if (*hNbActive == 0) break;
+            }
+        }
+        hPasses.resize(nG);
+        d2h(hAssign.data(), dAssign.get(), nbDesc, s);
+        d2h(hPasses.data(), dPasses.get(), nG, s);
+        RBOW_HIP(hipStreamSynchronize(s));
+        if (opts.seed_mode == 1)
+            for (uint32_t g = 0; g < nG; g++) voc->masks[nodeBase + g] = hMasks[g];
+        for (uint32_t g = 0; g < nG; g++) {
+            passesAll[nodeBase + g] = hPasses[g];
+            pointPasses += uint64_t(taskCnt[g]) * hPasses[g];
+        }
+        if (stats)
+            for (uint32_t it = 0; it < nbLaunched; it++) {
+                float a = 0, u = 0;
+                RBOW_HIP(hipEventElapsedTime(&a, ev[4 * it], ev[4 * it + 1]));
+                RBOW_HIP(hipEventElapsedTime(&u, ev[4 * it + 2], ev[4 * it + 3]));
+                assignMs += a;
+                updateMs += u;
+            }
+
+        // ---- stable regroup + occurrence (Vocabulary.cpp:83-98) ----
+        std::vector<uint32_t> childBeg(uint64_t(nG) * b), childCnt(uint64_t(nG) * b, 0u);
+        for (uint32_t g = 0; g < nG; g++)
+            for (uint32_t i = taskBeg[g]; i < taskBeg[g] + taskCnt[g]; i++) childCnt[uint64_t(g) * b + hAssign[i]]++;
+        uint32_t run = 0;
+        for (uint64_t c = 0; c < childCnt.size(); c++) {
+            childBeg[c] = run;
+            run += childCnt[c];
+        }
+        {
+            std::vector<uint32_t> cursor = childBeg;
+            for (uint32_t g = 0; g < nG; g++)
+                for (uint32_t i = taskBeg[g]; i < taskBeg[g] + taskCnt[g]; i++)
+                    next[cursor[uint64_t(g) * b + hAssign[i]]++] = order[i];
+        }
+        for (uint64_t c = 0; c < childCnt.size(); c++) occ[nodeBase * b + c] = childCnt[c];
+        order.swap(next);
+        taskBeg.swap(childBeg);
+        taskCnt.swap(childCnt);
+    }
+
+    // ---- IDF weights (Vocabulary.cpp:103-108): log of the binary32 quotient
+    // evaluated in binary64, rounded to binary32 ----
+    for (uint64_t i = 0; i < occ.size(); i++) {
+        const uint32_t o = occ[i];
+        voc->weights[i] = o == 0 ? FLT_MIN : std::max(FLT_MIN, float(std::log(double(float(nbDoc) / float(o)))));
+    }
+    voc->prepare(s);
+    RBOW_HIP(hipStreamSynchronize(s));
+    if (stats) {
+        stats->passes = std::move(passesAll);
+        stats->assignMs = assignMs;
+        stats->updateMs = updateMs;
+        stats->pointPasses = pointPasses;
+        stats->totalMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return voc.release();
+}
+
+// ------------------------------------------------------------ blobs -----
+namespace {
+constexpr uint32_t kMagic = 0x32574252u;  // "RBW2"
+struct Rbw2Header {
+    uint32_t magic, version, b, L, centroidType, reserved;
+};
+}  // namespace
+
+std::vector<uint8_t> serialize(const Vocabulary& v) {
+    const uint64_t n = v.nbNodes, nc = v.nbChildren();
+    if (v.b == 16 && v.centroidType == 0) {
+        // Vocabulary.cpp:175-197: [u32 L][Node x n][BitSet<16> x n][float[16] x n],
+        // Node byte(dim k, child j) = (k/4)*64 + 4j + k%4 (Vocabulary.h:42).
+        std::vector<uint8_t> rows(nc * kDims);
+        if (!rows.empty()) RBOW_HIP(hipMemcpy(rows.data(), v.u8rows.get(), rows.size(), hipMemcpyDeviceToHost));
+        std::vector<uint8_t> out(4 + n * 2048 + n * 2 + n * 64);
+        std::memcpy(out.data(), &v.L, 4);
+        uint8_t* nodes = out.data() + 4;
+        for (uint64_t i = 0; i < n; i++)
+            for (uint32_t j = 0; j < 16; j++)
+                for (uint32_t k = 0; k < kDims; k++)
+                    nodes[i * 2048 + (k >> 2) * 64 + j * 4 + (k & 3)] = rows[(i * 16 + j) * kDims + k];
+        uint8_t* m = nodes + n * 2048;
+        for (uint64_t i = 0; i < n; i++) {
+            const uint16_t mk = uint16_t(v.masks[i]);
+            std::memcpy(m + 2 * i, &mk, 2);
+        }
+        std::memcpy(m + 2 * n, v.weights.data(), n * 64);
+        return out;
+    }
+    const uint64_t rowBytes = nc * kDims * (v.centroidType == 0 ? 1 : 4);
+    std::vector<uint8_t> out(sizeof(Rbw2Header) + rowBytes + n * 4 + nc * 4);
+    const Rbw2Header h{kMagic, 1u, v.b, v.L, v.centroidType, 0u};
+    std::memcpy(out.data(), &h, sizeof h);
+    uint8_t* p = out.data() + sizeof h;
+    if (rowBytes)
+        RBOW_HIP(hipMemcpy(p, v.centroidType == 0 ? static_cast<const void*>(v.u8rows.get()) : v.f32rows.get(),
+                           rowBytes, hipMemcpyDeviceToHost));
+    p += rowBytes;
+    std::memcpy(p, v.masks.data(), n * 4);
+    std::memcpy(p + n * 4, v.weights.data(), nc * 4);
+    return out;
+}
+
+Vocabulary* deserialize(const uint8_t* blob, size_t size) {
+    RBOW_REQUIRE(blob != nullptr || size == 0, RBOW_ERR_INVALID_ARG, "null blob");
+    RBOW_REQUIRE(size >= 4, RBOW_ERR_INVALID_BLOB, "invalid blob");  // Vocabulary.cpp:129-131
+    std::unique_ptr<Vocabulary> v(new Vocabulary);
+    uint32_t first;
+    std::memcpy(&first, blob, 4);
+    if (first == kMagic && size >= sizeof(Rbw2Header)) {
+        Rbw2Header h;
+        std::memcpy(&h, blob, sizeof h);
+        RBOW_REQUIRE(h.version == 1 && (h.b == 16 || h.b == 32) && h.centroidType <= 1 &&
+                         (h.centroidType == 1 || h.b == 16) && uint64_t(h.L) * (h.b == 16 ? 4 : 5) <= 28,
+                     RBOW_ERR_INVALID_BLOB, "invalid blob");
+        const uint32_t lg2b = h.b == 16 ? 4 : 5;
+        const uint64_t n = levelBeg(lg2b, h.L), nc = n * h.b;
+        const uint64_t rowBytes = nc * kDims * (h.centroidType == 0 ? 1 : 4);
+        RBOW_REQUIRE(size == sizeof h + rowBytes + n * 4 + nc * 4, RBOW_ERR_INVALID_BLOB, "invalid blob");
+        v->allocate(h.b, h.L, h.centroidType);
+        const uint8_t* p = blob + sizeof h;
+        if (rowBytes)
+            RBOW_HIP(hipMemcpy(h.centroidType == 0 ? static_cast<void*>(v->u8rows.get()) : v->f32rows.get(), p,
+                               rowBytes, hipMemcpyHostToDevice));
+        p += rowBytes;
+        std::memcpy(v->masks.data(), p, n * 4);
+        std::memcpy(v->weights.data(), p + n * 4, nc * 4);
+    } else {
+        // Vocabulary.cpp:126-157
+        const uint32_t L = first;
+        RBOW_REQUIRE(L <= 7, RBOW_ERR_INVALID_BLOB, "invalid blob");
+        const uint64_t n = levelBeg(4, L);
+        RBOW_REQUIRE(size == 4 + n * 2048 + n * 2 + n * 64, RBOW_ERR_INVALID_BLOB, "invalid blob");
+        v->allocate(16, L, 0);
+        const uint8_t* nodes = blob + 4;
+        std::vector<uint8_t> rows(n * 16 * kDims);
+        for (uint64_t i = 0; i < n; i++)
+            for (uint32_t j = 0; j < 16; j++)
+                for (uint32_t k = 0; k < kDims; k++)
+                    rows[(i * 16 + j) * kDims + k] = nodes[i * 2048 + (k >> 2) * 64 + j * 4 + (k & 3)];
+        if (!rows.empty()) RBOW_HIP(hipMemcpy(v->u8rows.get(), rows.data(), rows.size(), hipMemcpyHostToDevice));
+        const uint8_t* m = nodes + n * 2048;
+        for (uint64_t i = 0; i < n; i++) {
+            uint16_t mk;
+            std::memcpy(&mk, m + 2 * i, 2);
+            v->masks[i] = mk;
+        }
+        std::memcpy(v->weights.data(), m + 2 * n, n * 64);
+    }
+    v->prepare(nullptr);
+    RBOW_HIP(hipDeviceSynchronize());
+    return v.release();
+}
+
+}  // namespace rbow_amd

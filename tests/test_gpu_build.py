@@ -1,0 +1,86 @@
+"""Vocabulary build on the GPU vs the oracle: nodes, masks, weights and Lloyd passes bit-exact."""
+import numpy as np
+import pytest
+
+import oracle
+import rapidbow_amd as rb
+from tests.helpers import level_beg
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_build(desc, nb_doc, L, **kw):
+    d = rb.DeviceArray.from_numpy(desc)
+    return rb.build_vocabulary(d, desc.shape[0], nb_doc, L, **kw)
+
+
+def _check_u8(voc, res, L):
+    blob = voc.serialize()
+    n = level_beg(16, L)
+    nodes = np.frombuffer(blob, np.uint8, n * 2048, 4).reshape(n, 2048)
+    masks = np.frombuffer(blob, np.uint16, n, 4 + n * 2048)
+    w = np.frombuffer(blob, np.float32, n * 16, 4 + n * 2050).reshape(n, 16)
+    bad = np.nonzero((nodes != res["nodes"]).any(1))[0]
+    assert bad.size == 0, f"nodes differ: {bad[:10]}"
+    np.testing.assert_array_equal(masks, res["masks"].astype(np.uint16))
+    np.testing.assert_array_equal(w, res["weights"])
+    np.testing.assert_array_equal(voc.build_passes(), res["passes"])
+
+
+@pytest.mark.parametrize("seed_mode", [0, 1])
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_build_u8_parity(seed_mode, kernel):
+    desc = rb.synth_descriptors(30000, seed=1)
+    L, iters, nb_doc = 3, 8, 100
+    res = oracle.build(desc, nb_doc, L, max_iters=iters, seed_mode=seed_mode)
+    voc = _gpu_build(desc, nb_doc, L, max_iters=iters, seed_mode=seed_mode, kernel=kernel)
+    _check_u8(voc, res, L)
+
+
+def test_build_u8_reference_defaults_converge():
+    # default options = reference: random picks, up to 1000 passes, polled every 4
+    desc = rb.synth_descriptors(6000, seed=3, nb_centres=300)
+    res = oracle.build(desc, 20, 2)
+    voc = _gpu_build(desc, 20, 2)
+    _check_u8(voc, res, 2)
+    assert res["passes"].max() > 8
+
+
+def test_build_voc_tests_invariants():
+    # VocTests.cpp:86-132 (RMS == 0) and :38-84 (RMS <= 10) on the GPU build
+    for which, L, nb_doc, lim in ((1, 3, 4, 0), (0, 2, 16, 10)):
+        desc = oracle.ref_voc_test_inputs(which)
+        voc = _gpu_build(desc, nb_doc, L)
+        leaf = voc.lookup_host(desc)
+        cent = voc.centroids(L - 1).reshape(-1, 128)[leaf]
+        inertia = int(np.sum((cent.astype(np.int64) - desc) ** 2))
+        assert np.sqrt(inertia // desc.shape[0]) <= lim
+        res = oracle.build(desc, nb_doc, L)
+        _check_u8(voc, res, L)
+
+
+@pytest.mark.parametrize("b,L", [(32, 2), (16, 2)])
+def test_build_f32_parity(b, L):
+    desc = rb.synth_descriptors(20000, seed=4)
+    res = oracle.build(desc, 64, L, b=b, max_iters=10, seed_mode=1, centroid_type=1)
+    voc = _gpu_build(desc, 64, L, branch_factor=b, max_iters=10, seed_mode=1, centroid_type=1, kernel=1)
+    for l in range(L):
+        got = voc.centroids(l)
+        exp = res["nodes"][level_beg(b, l):level_beg(b, l + 1)]
+        bad = np.nonzero((got != exp).any(axis=(1, 2)))[0]
+        assert bad.size == 0, f"level {l}: nodes differ {bad[:10]}"
+    np.testing.assert_array_equal(voc.weights().reshape(-1, b), res["weights"])
+    np.testing.assert_array_equal(voc.build_passes(), res["passes"])
+    q = rb.synth_descriptors(5000, seed=5)
+    exp = oracle.lookup_f32(res["nodes"], b, L, q)
+    np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_MFMA), exp)
+    np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_PLAIN), exp)
+    blob = voc.serialize()
+    assert rb.deserialize_vocabulary(blob).serialize() == blob
+
+
+def test_build_errors():
+    d = rb.DeviceArray.from_numpy(np.zeros((15, 128), np.uint8))
+    with pytest.raises(rb.RbowError) as e:
+        rb.build_vocabulary(d, 15, 1, 2)
+    assert str(e.value) == "Insufficient descriptors"

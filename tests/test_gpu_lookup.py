@@ -1,0 +1,95 @@
+"""Lookup parity on the GPU: plain and MFMA kernels vs the oracle, bit-exact leaf ids."""
+import numpy as np
+import pytest
+
+import oracle
+import rapidbow_amd as rb
+from tests.helpers import level_beg, rbw2_blob, ref_blob
+
+pytestmark = pytest.mark.gpu
+
+
+def _u8_voc_from_oracle(desc, L, nb_doc=50, iters=8, seed_mode=1):
+    res = oracle.build(desc, nb_doc, L, max_iters=iters, seed_mode=seed_mode)
+    blob = ref_blob(L, res["nodes"], res["masks"], res["weights"])
+    return rb.deserialize_vocabulary(blob), res, blob
+
+
+@pytest.mark.parametrize("kernel", [rb.LOOKUP_PLAIN, rb.LOOKUP_MFMA])
+def test_lookup_u8_cfg1_shape_parity(kernel):
+    # b=16, L=3 (4,096 words) built on SIFT-like data; 50k queries, ragged tail
+    desc = rb.synth_descriptors(60000, seed=1)
+    voc, res, _ = _u8_voc_from_oracle(desc, 3)
+    q = rb.synth_descriptors(50001, seed=2)
+    expect = oracle.lookup_u8(res["nodes"], 3, q)
+    got = voc.lookup_host(q, kernel)
+    np.testing.assert_array_equal(got, expect)
+
+
+@pytest.mark.parametrize("kernel", [rb.LOOKUP_PLAIN, rb.LOOKUP_MFMA])
+def test_lookup_u8_ties_and_edges(kernel):
+    rng = np.random.default_rng(3)
+    for L in (1, 2, 4):
+        n = level_beg(16, L)
+        rows = rng.integers(0, 256, (n, 16, 128), dtype=np.uint8)
+        rows[:, 11] = rows[:, 4]  # duplicate children: ties must go to the lower index
+        rows[:, 14] = rows[:, 1]
+        blob = ref_blob(L, oracle.rows_to_ref_nodes(rows), np.full(n, 0xFFFF), np.ones((n, 16), np.float32))
+        voc = rb.deserialize_vocabulary(blob)
+        for nq in (1, 15, 16, 17, 129, 1000):
+            q = rng.integers(0, 256, (nq, 128), dtype=np.uint8)
+            q[: min(nq, 16)] = rows[0, : min(nq, 16)]
+            got = voc.lookup_host(q, kernel)
+            np.testing.assert_array_equal(got, oracle.lookup_u8(oracle.rows_to_ref_nodes(rows), L, q))
+
+
+def test_lookup_empty_is_noop():
+    rng = np.random.default_rng(4)
+    rows = rng.integers(0, 256, (17, 16, 128), dtype=np.uint8)
+    voc = rb.deserialize_vocabulary(ref_blob(2, oracle.rows_to_ref_nodes(rows), np.zeros(17), np.zeros((17, 16))))
+    out = rb.DeviceArray((4,), np.uint32)
+    out.upload(np.array([7, 7, 7, 7], np.uint32))
+    voc.lookup(None, 0, out)
+    rb.hip.synchronize()
+    np.testing.assert_array_equal(out.numpy(), [7, 7, 7, 7])
+
+
+def _f32_tree(rng, b, L, desc, near_ties=True):
+    n = level_beg(b, L)
+    # centroids near data points so that distances are realistic
+    pick = desc[rng.integers(0, desc.shape[0], n * b)].astype(np.float32).reshape(n, b, 128)
+    rows = pick + rng.normal(0, 3, pick.shape).astype(np.float32)
+    if near_ties:
+        rows[:, 5] = rows[:, 2]                       # exact duplicate
+        rows[:, 9] = rows[:, 8] + np.float32(1e-3)    # near duplicate: forces the FP32 re-score
+    return rows
+
+
+@pytest.mark.parametrize("b,L", [(32, 3), (16, 3), (32, 1)])
+def test_lookup_f32_parity_and_near_ties(b, L):
+    rng = np.random.default_rng(5)
+    desc = rb.synth_descriptors(20000, seed=7)
+    rows = _f32_tree(rng, b, L, desc)
+    n = rows.shape[0]
+    voc = rb.deserialize_vocabulary(rbw2_blob(b, L, 1, rows, np.full(n, 0xFFFFFFFF), np.ones((n, b))))
+    q = rb.synth_descriptors(30000, seed=8)
+    q[:b] = np.clip(np.rint(rows[0]), 0, 255).astype(np.uint8)
+    expect = oracle.lookup_f32(rows, b, L, q)
+    plain = voc.lookup_host(q, rb.LOOKUP_PLAIN)
+    np.testing.assert_array_equal(plain, expect)
+    mfma, ties = voc.lookup_host(q, rb.LOOKUP_MFMA, return_near_ties=True)
+    np.testing.assert_array_equal(mfma, expect)
+    print(f"b={b} L={L} near ties re-scored: {ties}")
+    assert ties > 0
+
+
+def test_blob_roundtrip_u8():
+    desc = rb.synth_descriptors(8000, seed=9)
+    voc, res, blob = _u8_voc_from_oracle(desc, 2)
+    out = voc.serialize()
+    assert out == blob
+    n = level_beg(16, 2)
+    assert len(out) == 4 + 2114 * n  # Vocabulary.cpp:177-182
+    np.testing.assert_array_equal(voc.weights().reshape(-1, 16), res["weights"])
+    voc2 = rb.deserialize_vocabulary(out)
+    assert voc2.serialize() == blob
