@@ -116,6 +116,42 @@ const float* rbow_voc_level_weights(const rbow_vocabulary* voc, uint32_t level);
  * (uint8 trees are widened); for tests and tools. */
 rbow_status rbow_voc_copy_centroids(const rbow_vocabulary* voc, uint32_t level, float* host_out, size_t capacity_floats);
 
+/* ---------------------------------------------------------- IDataBase --- */
+/* RapidBoW.h:107 createDataBase: borrows the vocabulary, which must outlive
+ * the database.  Postings, document norms and IDF weights live in HBM. */
+rbow_status rbow_db_create(const rbow_vocabulary* voc, rbow_database** out);
+void rbow_db_destroy(rbow_database* db);
+/* RapidBoW.h:97 IDataBase::addDoc (DataBase.cpp:39-69); host leaf ids.
+ * RBOW_ERR_DUPLICATE_DOC "Duplicate document index" for a known doc id. */
+rbow_status rbow_db_add_doc(rbow_database* db, uint32_t doc, const uint32_t* host_leaf, uint32_t n);
+/* RapidBoW.h:99-100 IDataBase::query (DataBase.cpp:71-190); host leaf ids.
+ * Writes up to max_results (<= 1024) doc ids ranked by (score, doc id)
+ * ascending and optionally their scores; *out_count = number written.
+ * Queries hold a shared lock; at most 16384 leaf ids per query. */
+rbow_status rbow_db_query(const rbow_database* db, const uint32_t* host_leaf, uint32_t n, uint32_t max_results,
+                          uint32_t* out_docs, float* out_scores, uint32_t* out_count);
+/* RapidBoW.h:102-103 IDataBase::queryAndAddDoc (DataBase.cpp:192-197). */
+rbow_status rbow_db_query_and_add_doc(rbow_database* db, uint32_t doc, const uint32_t* host_leaf, uint32_t n,
+                                      uint32_t max_results, uint32_t* out_docs, uint32_t* out_count);
+/* Batched forms (no counterpart in the reference, which has one host query
+ * at a time): doc host_doc_ids[i] owns dev_leaf[host_offsets[i] ..
+ * host_offsets[i+1]) (offsets are indices into dev_leaf). */
+rbow_status rbow_db_add_docs_device(rbow_database* db, const uint32_t* host_doc_ids, uint32_t nb_docs,
+                                    const uint32_t* dev_leaf, const uint64_t* host_offsets, rbow_stream stream);
+/* Scores nb_queries queries in one pass: results dev_docs[q][max_results],
+ * dev_scores[q][max_results] (optional), dev_counts[q].  Returns when done. */
+rbow_status rbow_db_query_batch(const rbow_database* db, const uint32_t* dev_leaf, const uint64_t* host_offsets,
+                                uint32_t nb_queries, uint32_t max_results, uint32_t* dev_docs, float* dev_scores,
+                                uint32_t* dev_counts, rbow_stream stream);
+uint32_t rbow_db_nb_docs(const rbow_database* db);
+uint64_t rbow_db_nb_postings(const rbow_database* db);
+/* Posting list of one leaf as sorted doc ids (duplicates kept), the
+ * reference's InvertedFile (DataBase.h:58-59); *length = full length. */
+rbow_status rbow_db_postings(const rbow_database* db, uint32_t leaf, uint32_t* host_out, uint32_t capacity,
+                             uint32_t* length);
+/* float(1 / L1 norm) of a document (DataBase.cpp:68). */
+rbow_status rbow_db_doc_rcp(const rbow_database* db, uint32_t doc, float* rcp);
+
 /* ------------------------------------------------ kernel-level entries -- */
 /* These expose the batched k-means kernels the build drives, one pass at a
  * time, for callers that run their own Lloyd loop (and for parity tests).
@@ -172,6 +208,25 @@ rbow_status rbow_voc_build_stats(const rbow_vocabulary* voc, rbow_build_stats* o
 void rbow_synth_centres(uint64_t centre_seed, uint32_t nb_centres, float* out);
 void rbow_synth_descriptors(uint64_t centre_seed, uint32_t nb_centres, uint64_t seed, uint64_t first, uint64_t n,
                             const uint32_t* centre_ids, float sigma_scale, uint8_t* out);
+/* Device forms (same distribution, generated in HBM; not bit-equal to the
+ * host form).  dev_sigma[c] = sigma_scale * ||c|| / sqrt(128). */
+rbow_status rbow_synth_descriptors_device(const float* dev_centres, const float* dev_sigma, uint32_t nb_centres,
+                                          uint64_t seed, uint64_t first, uint64_t n, const uint32_t* dev_centre_ids,
+                                          uint8_t* dev_out, rbow_stream stream);
+/* Image i of [first_image, first_image + nb_images) owns descriptors
+ * dev_offsets[i] .. dev_offsets[i+1]; each takes a centre from the image's
+ * subset of subset_size centres (dev_ids[nb_desc]). */
+rbow_status rbow_synth_image_centres_device(const uint64_t* dev_offsets, uint32_t nb_images, uint64_t first_image,
+                                            uint64_t image_seed, uint64_t desc_seed, uint32_t nb_centres,
+                                            uint32_t subset_size, uint32_t* dev_ids, uint64_t nb_desc,
+                                            rbow_stream stream);
+/* Posting lists without descriptors: posting j of image i is a leaf drawn
+ * from dev_cloud[c][cloud_size], the looked-up leaves of cloud_size samples
+ * around centre c, with c taken from the image's centre subset. */
+rbow_status rbow_synth_image_postings_device(const uint64_t* dev_offsets, uint32_t nb_images, uint64_t first_image,
+                                             uint64_t image_seed, uint64_t post_seed, uint32_t nb_centres,
+                                             uint32_t subset_size, const uint32_t* dev_cloud, uint32_t cloud_size,
+                                             uint32_t* dev_leaf, uint64_t nb_postings, rbow_stream stream);
 
 #ifdef __cplusplus
 }

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librapidbow_amd.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["lookup.hip", "kmeans.hip", "database.hip", "vocabulary.cpp", "database.cpp", "capi.cpp", "synth.cpp"]
+SOURCES = ["lookup.hip", "kmeans.hip", "database.hip", "vocabulary.cpp", "database.cpp", "capi.cpp", "synth.cpp", "synth.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-Wall", "-Wno-unused-result", f"-I{os.path.join(ROOT, 'include')}"]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -36,7 +36,7 @@ def build(verbose=False):
     procs = []
     for s in srcs:
         src = os.path.join(CSRC, s)
-        obj = os.path.join(BUILD_DIR, s + ".o")
+        obj = os.path.join(BUILD_DIR, s.replace(".", "_") + ".o")
         objs.append(obj)
         cmd = [HIPCC] + COMMON + ["-c", src, "-o", obj]
         if s.endswith(".hip"):

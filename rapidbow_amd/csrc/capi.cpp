@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "database.hpp"
 #include "vocabulary.hpp"
 
 using namespace rbow_amd;
@@ -13,6 +14,10 @@ using namespace rbow_amd;
 struct rbow_vocabulary {
     std::unique_ptr<Vocabulary> v;
     BuildStats stats;
+};
+
+struct rbow_database {
+    std::unique_ptr<Database> db;
 };
 
 namespace {
@@ -290,6 +295,83 @@ rbow_status rbow_gather_rows(const void* dev_desc, const uint32_t* dev_idx, uint
         checkAligned(dev_desc, "descriptors");
         checkAligned(dev_out, "output");
         launchGatherRows(static_cast<const uint8_t*>(dev_desc), dev_idx, n, dev_out, hs(stream));
+    });
+}
+
+// ------------------------------------------------------------ database --
+
+rbow_status rbow_db_create(const rbow_vocabulary* voc, rbow_database** out) {
+    return guarded([&] {
+        RBOW_REQUIRE(voc && out, RBOW_ERR_INVALID_ARG, "null argument");
+        *out = nullptr;
+        std::unique_ptr<rbow_database> h(new rbow_database);
+        h->db.reset(new Database(*voc->v));
+        *out = h.release();
+    });
+}
+
+void rbow_db_destroy(rbow_database* db) { delete db; }
+
+rbow_status rbow_db_add_doc(rbow_database* db, uint32_t doc, const uint32_t* host_leaf, uint32_t n) {
+    return guarded([&] {
+        RBOW_REQUIRE(db != nullptr, RBOW_ERR_INVALID_ARG, "null database");
+        db->db->addDoc(doc, host_leaf, n);
+    });
+}
+
+rbow_status rbow_db_query(const rbow_database* db, const uint32_t* host_leaf, uint32_t n, uint32_t max_results,
+                          uint32_t* out_docs, float* out_scores, uint32_t* out_count) {
+    return guarded([&] {
+        RBOW_REQUIRE(db && out_count && (out_docs || max_results == 0), RBOW_ERR_INVALID_ARG, "null argument");
+        *out_count = db->db->query(host_leaf, n, max_results, out_docs, out_scores);
+    });
+}
+
+rbow_status rbow_db_query_and_add_doc(rbow_database* db, uint32_t doc, const uint32_t* host_leaf, uint32_t n,
+                                      uint32_t max_results, uint32_t* out_docs, uint32_t* out_count) {
+    // DataBase.cpp:192-197: query, then addDoc (two separate lock acquisitions)
+    const rbow_status s = rbow_db_query(db, host_leaf, n, max_results, out_docs, nullptr, out_count);
+    if (s != RBOW_OK) return s;
+    return rbow_db_add_doc(db, doc, host_leaf, n);
+}
+
+rbow_status rbow_db_add_docs_device(rbow_database* db, const uint32_t* host_doc_ids, uint32_t nb_docs,
+                                    const uint32_t* dev_leaf, const uint64_t* host_offsets, rbow_stream stream) {
+    return guarded([&] {
+        RBOW_REQUIRE(db != nullptr, RBOW_ERR_INVALID_ARG, "null database");
+        if (!nb_docs) return;
+        RBOW_REQUIRE(dev_leaf || host_offsets[nb_docs] == host_offsets[0], RBOW_ERR_INVALID_ARG, "null leaf ids");
+        db->db->addDocsDevice(host_doc_ids, nb_docs, dev_leaf, host_offsets, hs(stream));
+    });
+}
+
+rbow_status rbow_db_query_batch(const rbow_database* db, const uint32_t* dev_leaf, const uint64_t* host_offsets,
+                                uint32_t nb_queries, uint32_t max_results, uint32_t* dev_docs, float* dev_scores,
+                                uint32_t* dev_counts, rbow_stream stream) {
+    return guarded([&] {
+        RBOW_REQUIRE(db != nullptr, RBOW_ERR_INVALID_ARG, "null database");
+        if (!nb_queries) return;
+        RBOW_REQUIRE(dev_docs && dev_counts && max_results > 0, RBOW_ERR_INVALID_ARG, "null output");
+        db->db->queryBatch(dev_leaf, host_offsets, nb_queries, max_results, dev_docs, dev_scores, dev_counts,
+                           hs(stream));
+    });
+}
+
+uint32_t rbow_db_nb_docs(const rbow_database* db) { return db ? db->db->nbDocs() : 0u; }
+uint64_t rbow_db_nb_postings(const rbow_database* db) { return db ? db->db->nbPostings() : 0u; }
+
+rbow_status rbow_db_postings(const rbow_database* db, uint32_t leaf, uint32_t* host_out, uint32_t capacity,
+                             uint32_t* length) {
+    return guarded([&] {
+        RBOW_REQUIRE(db && length, RBOW_ERR_INVALID_ARG, "null argument");
+        *length = db->db->postings(leaf, host_out, capacity);
+    });
+}
+
+rbow_status rbow_db_doc_rcp(const rbow_database* db, uint32_t doc, float* rcp) {
+    return guarded([&] {
+        RBOW_REQUIRE(db && rcp, RBOW_ERR_INVALID_ARG, "null argument");
+        RBOW_REQUIRE(db->db->docRcp(doc, rcp), RBOW_ERR_INVALID_ARG, "unknown document");
     });
 }
 

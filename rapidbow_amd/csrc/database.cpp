@@ -1,0 +1,243 @@
+// database.cpp -- host side of the HBM-resident inverted file (DataBase.cpp).
+#include "database.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+namespace rbow_amd {
+
+namespace {
+inline float ftz(float x) { return std::fabs(x) < FLT_MIN ? std::copysign(0.0f, x) : x; }
+}  // namespace
+
+Database::Database(const Vocabulary& voc) : voc_(voc), b_(voc.b), lg2b_(voc.lg2b), L_(voc.L) {
+    RBOW_REQUIRE(L_ >= 1, RBOW_ERR_INVALID_ARG, "vocabulary has no levels");
+    nbLeaves_ = levelSize(lg2b_, L_);
+    W_.resize(voc.weights.size());
+    RBOW_HIP(hipMemcpy(W_.get(), voc.weights.data(), voc.weights.size() * 4, hipMemcpyHostToDevice));
+    leafOff_.resize(nbLeaves_ + 1);
+    RBOW_HIP(hipMemset(leafOff_.get(), 0, leafOff_.bytes()));
+    post_.resize(1);
+    growDocs(1024, nullptr);
+    RBOW_HIP(hipDeviceSynchronize());
+}
+
+Database::~Database() { (void)hipDeviceSynchronize(); }
+
+DbView Database::view() const {
+    DbView v;
+    v.b = b_;
+    v.lg2b = lg2b_;
+    v.L = L_;
+    v.nbLeaves = nbLeaves_;
+    v.nbDocs = docsOnDevice_;
+    v.W = W_.get();
+    v.leafOff = leafOff_.get();
+    v.post = post_.get();
+    v.rcp = rcp_.get();
+    v.docId = docId_.get();
+    return v;
+}
+
+void Database::growDocs(uint32_t nb, hipStream_t s) const {
+    if (nb <= rcp_.size()) return;
+    const size_t cap = std::max<size_t>(nb, rcp_.size() * 2);
+    DevBuf<float> r(cap);
+    DevBuf<uint32_t> d(cap);
+    if (docsOnDevice_) {
+        RBOW_HIP(hipMemcpyAsync(r.get(), rcp_.get(), 4ull * docsOnDevice_, hipMemcpyDeviceToDevice, s));
+        RBOW_HIP(hipMemcpyAsync(d.get(), docId_.get(), 4ull * docsOnDevice_, hipMemcpyDeviceToDevice, s));
+    }
+    RBOW_HIP(hipStreamSynchronize(s));
+    rcp_ = std::move(r);
+    docId_ = std::move(d);
+}
+
+void Database::addDoc(uint32_t doc, const uint32_t* leaf, uint32_t n) {
+    std::unique_lock<std::shared_mutex> lk(mtx_);  // DataBase.cpp:40: exclusive
+    RBOW_REQUIRE(!slotOf_.count(doc), RBOW_ERR_DUPLICATE_DOC, "Duplicate document index");  // DataBase.cpp:41-43
+    RBOW_REQUIRE(leaf != nullptr || n == 0, RBOW_ERR_INVALID_ARG, "null leaf ids");
+    for (uint32_t i = 0; i < n; i++)
+        RBOW_REQUIRE(leaf[i] < nbLeaves_, RBOW_ERR_INVALID_ARG, "leaf id out of range");
+    // DataBase.cpp:54-68: binary64 L1 norm, level-major over the sorted leaves
+    std::vector<uint32_t> nxt(leaf, leaf + n);
+    std::sort(nxt.begin(), nxt.end());
+    double l1 = 0;
+    for (uint32_t t = 0; t < L_; t++) {
+        const float* W = voc_.weights.data() + levelBeg(lg2b_, L_ - 1 - t) * b_;
+        for (uint32_t i = 0; i < n; i++) {
+            l1 += W[nxt[i]];
+            nxt[i] >>= lg2b_;
+        }
+    }
+    const uint32_t slot = uint32_t(docIds_.size());
+    slotOf_[doc] = slot;
+    docIds_.push_back(doc);
+    rcpHost_.push_back(ftz(float(1 / l1)));
+    for (uint32_t i = 0; i < n; i++) {
+        pendLeaf_.push_back(leaf[i]);
+        pendSlot_.push_back(slot);
+    }
+}
+
+void Database::ensureMerged(hipStream_t s) const {
+    std::lock_guard<std::mutex> g(mergeMtx_);
+    const uint32_t nDocs = uint32_t(docIds_.size());
+    if (nDocs > docsOnDevice_) {
+        growDocs(nDocs, s);
+        RBOW_HIP(hipMemcpyAsync(rcp_.get() + docsOnDevice_, rcpHost_.data() + docsOnDevice_,
+                                4ull * (nDocs - docsOnDevice_), hipMemcpyHostToDevice, s));
+        RBOW_HIP(hipMemcpyAsync(docId_.get() + docsOnDevice_, docIds_.data() + docsOnDevice_,
+                                4ull * (nDocs - docsOnDevice_), hipMemcpyHostToDevice, s));
+        docsOnDevice_ = nDocs;
+    }
+    if (pendLeaf_.empty()) {
+        RBOW_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    const uint64_t n = pendLeaf_.size();
+    DevBuf<uint32_t> l(n), sl(n);
+    RBOW_HIP(hipMemcpyAsync(l.get(), pendLeaf_.data(), 4 * n, hipMemcpyHostToDevice, s));
+    RBOW_HIP(hipMemcpyAsync(sl.get(), pendSlot_.data(), 4 * n, hipMemcpyHostToDevice, s));
+    mergeDevice(l.get(), sl.get(), n, s);
+    pendLeaf_.clear();
+    pendSlot_.clear();
+}
+
+void Database::mergeDevice(const uint32_t* dLeaf, const uint32_t* dSlot, uint64_t n, hipStream_t s) const {
+    DevBuf<uint64_t> off(nbLeaves_ + 1);
+    DevBuf<uint32_t> post(nbPost_ + n);
+    launchMergePostings(view(), nbPost_, dLeaf, dSlot, n, off.get(), post.get(), s);
+    RBOW_HIP(hipDeviceSynchronize());  // no query still reads the old CSR
+    leafOff_ = std::move(off);
+    post_ = std::move(post);
+    nbPost_ += n;
+}
+
+void Database::addDocsDevice(const uint32_t* docs, uint32_t nbDocs, const uint32_t* dLeaf, const uint64_t* offsets,
+                             hipStream_t s) {
+    std::unique_lock<std::shared_mutex> lk(mtx_);
+    RBOW_REQUIRE(docs && offsets, RBOW_ERR_INVALID_ARG, "null argument");
+    {
+        std::unordered_map<uint32_t, uint32_t> seen;
+        for (uint32_t d = 0; d < nbDocs; d++) {
+            RBOW_REQUIRE(offsets[d + 1] >= offsets[d], RBOW_ERR_INVALID_ARG, "offsets must be non-decreasing");
+            RBOW_REQUIRE(!slotOf_.count(docs[d]) && seen.emplace(docs[d], d).second, RBOW_ERR_DUPLICATE_DOC,
+                         "Duplicate document index");
+        }
+    }
+    ensureMerged(s);  // host-added postings keep their (smaller) slots first in every leaf
+    const uint32_t first = uint32_t(docIds_.size());
+    for (uint32_t d = 0; d < nbDocs; d++) {
+        slotOf_[docs[d]] = first + d;
+        docIds_.push_back(docs[d]);
+    }
+    growDocs(first + nbDocs, s);
+    // norms on the device, chunked merges of <= 2^30 postings
+    DevBuf<uint64_t> dOff(nbDocs + 1);
+    RBOW_HIP(hipMemcpyAsync(dOff.get(), offsets, 8ull * (nbDocs + 1), hipMemcpyHostToDevice, s));
+    DbView v = view();
+    launchDocNorms(v, dLeaf, dOff.get(), nbDocs, first, rcp_.get(), s);
+    RBOW_HIP(hipMemcpyAsync(docId_.get() + first, docs, 4ull * nbDocs, hipMemcpyHostToDevice, s));
+    rcpHost_.resize(first + nbDocs);
+    RBOW_HIP(hipMemcpyAsync(rcpHost_.data() + first, rcp_.get() + first, 4ull * nbDocs, hipMemcpyDeviceToHost, s));
+    docsOnDevice_ = first + nbDocs;
+    uint32_t d0 = 0;
+    while (d0 < nbDocs) {
+        uint32_t d1 = d0 + 1;
+        while (d1 < nbDocs && offsets[d1 + 1] - offsets[d0] < (1ull << 30)) d1++;
+        const uint64_t n = offsets[d1] - offsets[d0];
+        RBOW_REQUIRE(n < (1ull << 31), RBOW_ERR_CAPACITY, "one document has too many descriptors");
+        if (n) {
+            DevBuf<uint32_t> slots(n);
+            launchExpandSlots(dOff.get() + d0, d1 - d0, first + d0, slots.get(), s);
+            mergeDevice(dLeaf + offsets[d0], slots.get(), n, s);
+        }
+        d0 = d1;
+    }
+    RBOW_HIP(hipStreamSynchronize(s));
+}
+
+void Database::queryBatch(const uint32_t* dLeaf, const uint64_t* offsets, uint32_t nbQueries, uint32_t K,
+                          uint32_t* dDocs, float* dScores, uint32_t* dCounts, hipStream_t s) const {
+    RBOW_REQUIRE(K <= kMaxResults, RBOW_ERR_CAPACITY, "maxNbResults above 1024");
+    RBOW_REQUIRE(offsets != nullptr || nbQueries == 0, RBOW_ERR_INVALID_ARG, "null offsets");
+    for (uint32_t q = 0; q < nbQueries; q++) {
+        RBOW_REQUIRE(offsets[q + 1] >= offsets[q], RBOW_ERR_INVALID_ARG, "offsets must be non-decreasing");
+        RBOW_REQUIRE(offsets[q + 1] - offsets[q] <= kMaxQueryDesc, RBOW_ERR_CAPACITY,
+                     "query above 16384 descriptors");
+    }
+    if (nbQueries == 0) return;
+    {
+        std::unique_lock<std::shared_mutex> lk(mtx_);
+        ensureMerged(s);
+    }
+    std::shared_lock<std::shared_mutex> lk(mtx_);  // DataBase.cpp:73: queries share the lock
+    // rebase offsets so that the kernels index dLeaf from 0
+    std::vector<uint64_t> h(nbQueries + 1);
+    for (uint32_t q = 0; q <= nbQueries; q++) h[q] = offsets[q] - offsets[0];
+    void* dOff = nullptr;
+    RBOW_HIP(hipMallocAsync(&dOff, 8ull * (nbQueries + 1), s));
+    RBOW_HIP(hipMemcpyAsync(dOff, h.data(), 8ull * (nbQueries + 1), hipMemcpyHostToDevice, s));
+    launchQueryBatch(view(), dLeaf + offsets[0], static_cast<uint64_t*>(dOff), h.data(), nbQueries, K, dDocs,
+                     dScores, dCounts, s);
+    RBOW_HIP(hipFreeAsync(dOff, s));
+    RBOW_HIP(hipStreamSynchronize(s));  // h (host) must outlive the copy; results ready on return
+}
+
+uint32_t Database::query(const uint32_t* leaf, uint32_t n, uint32_t K, uint32_t* docs, float* scores) const {
+    RBOW_REQUIRE(leaf != nullptr || n == 0, RBOW_ERR_INVALID_ARG, "null leaf ids");
+    for (uint32_t i = 0; i < n; i++) RBOW_REQUIRE(leaf[i] < nbLeaves_, RBOW_ERR_INVALID_ARG, "leaf id out of range");
+    if (K == 0) return 0;
+    DevBuf<uint32_t> dl(std::max(1u, n)), dd(K), dc(1);
+    DevBuf<float> ds(K);
+    if (n) RBOW_HIP(hipMemcpy(dl.get(), leaf, 4ull * n, hipMemcpyHostToDevice));
+    const uint64_t off[2] = {0, n};
+    queryBatch(dl.get(), off, 1, K, dd.get(), ds.get(), dc.get(), nullptr);
+    uint32_t cnt = 0;
+    RBOW_HIP(hipMemcpy(&cnt, dc.get(), 4, hipMemcpyDeviceToHost));
+    if (cnt) {
+        RBOW_HIP(hipMemcpy(docs, dd.get(), 4ull * cnt, hipMemcpyDeviceToHost));
+        if (scores) RBOW_HIP(hipMemcpy(scores, ds.get(), 4ull * cnt, hipMemcpyDeviceToHost));
+    }
+    return cnt;
+}
+
+uint32_t Database::nbDocs() const {
+    std::shared_lock<std::shared_mutex> lk(mtx_);
+    return uint32_t(docIds_.size());
+}
+
+uint64_t Database::nbPostings() const {
+    std::shared_lock<std::shared_mutex> lk(mtx_);
+    return nbPost_ + pendLeaf_.size();
+}
+
+bool Database::docRcp(uint32_t doc, float* rcp) const {
+    std::shared_lock<std::shared_mutex> lk(mtx_);
+    auto it = slotOf_.find(doc);
+    if (it == slotOf_.end()) return false;
+    *rcp = rcpHost_[it->second];
+    return true;
+}
+
+uint32_t Database::postings(uint32_t leaf, uint32_t* out, uint32_t cap) const {
+    RBOW_REQUIRE(leaf < nbLeaves_, RBOW_ERR_INVALID_ARG, "leaf id out of range");
+    {
+        std::unique_lock<std::shared_mutex> lk(mtx_);
+        ensureMerged(nullptr);
+    }
+    std::shared_lock<std::shared_mutex> lk(mtx_);
+    uint64_t o[2];
+    RBOW_HIP(hipMemcpy(o, leafOff_.get() + leaf, 16, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> slots(o[1] - o[0]);
+    if (!slots.empty())
+        RBOW_HIP(hipMemcpy(slots.data(), post_.get() + o[0], 4 * slots.size(), hipMemcpyDeviceToHost));
+    for (auto& x : slots) x = docIds_[x];
+    std::sort(slots.begin(), slots.end());  // the reference keeps each list sorted by doc id
+    if (out) std::copy_n(slots.begin(), std::min<size_t>(cap, slots.size()), out);
+    return uint32_t(slots.size());
+}
+
+}  // namespace rbow_amd

@@ -1,0 +1,217 @@
+"""Inverted-file database on the GPU vs the oracle's restatement of DataBase.cpp."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+import rapidbow_amd as rb
+from tests.helpers import level_beg, ref_blob, rbw2_blob
+
+pytestmark = pytest.mark.gpu
+
+SCORE_TOL = 2e-6  # |GPU float score - reference-semantics float score|; both accumulate in binary64
+
+
+class GpuDb:
+    def __init__(self, voc):
+        h = ctypes.c_void_p()
+        rb.check(rb.lib.rbow_db_create(ctypes.c_void_p(voc.handle), ctypes.byref(h)))
+        self.h = h
+        self.voc = voc
+
+    def __del__(self):
+        rb.lib.rbow_db_destroy(self.h)
+
+    def add(self, doc, leaf):
+        leaf = np.ascontiguousarray(leaf, np.uint32)
+        rb.check(rb.lib.rbow_db_add_doc(self.h, ctypes.c_uint32(doc), leaf.ctypes.data_as(ctypes.c_void_p),
+                                        ctypes.c_uint32(leaf.size)))
+
+    def query(self, leaf, K):
+        leaf = np.ascontiguousarray(leaf, np.uint32)
+        docs = np.zeros(max(K, 1), np.uint32)
+        sc = np.zeros(max(K, 1), np.float32)
+        n = ctypes.c_uint32()
+        rb.check(rb.lib.rbow_db_query(self.h, leaf.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint32(leaf.size),
+                                      ctypes.c_uint32(K), docs.ctypes.data_as(ctypes.c_void_p),
+                                      sc.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n)))
+        return docs[:n.value], sc[:n.value]
+
+    def postings(self, leaf):
+        n = ctypes.c_uint32()
+        rb.check(rb.lib.rbow_db_postings(self.h, ctypes.c_uint32(leaf), None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, np.uint32)
+        rb.check(rb.lib.rbow_db_postings(self.h, ctypes.c_uint32(leaf), out.ctypes.data_as(ctypes.c_void_p),
+                                         ctypes.c_uint32(n.value), ctypes.byref(n)))
+        return out
+
+    def rcp(self, doc):
+        r = ctypes.c_float()
+        rb.check(rb.lib.rbow_db_doc_rcp(self.h, ctypes.c_uint32(doc), ctypes.byref(r)))
+        return r.value
+
+    def add_batch(self, docs, leaf_concat, offsets):
+        d = rb.DeviceArray.from_numpy(np.ascontiguousarray(leaf_concat, np.uint32))
+        ids = np.ascontiguousarray(docs, np.uint32)
+        off = np.ascontiguousarray(offsets, np.uint64)
+        rb.check(rb.lib.rbow_db_add_docs_device(self.h, ids.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint32(ids.size),
+                                                ctypes.c_void_p(d.ptr), off.ctypes.data_as(ctypes.c_void_p), None))
+
+    def query_batch(self, leaf_concat, offsets, K):
+        d = rb.DeviceArray.from_numpy(np.ascontiguousarray(leaf_concat, np.uint32))
+        off = np.ascontiguousarray(offsets, np.uint64)
+        B = off.size - 1
+        docs = rb.DeviceArray((B, K), np.uint32)
+        sc = rb.DeviceArray((B, K), np.float32)
+        cnt = rb.DeviceArray((B,), np.uint32)
+        rb.check(rb.lib.rbow_db_query_batch(self.h, ctypes.c_void_p(d.ptr), off.ctypes.data_as(ctypes.c_void_p),
+                                            ctypes.c_uint32(B), ctypes.c_uint32(K), ctypes.c_void_p(docs.ptr),
+                                            ctypes.c_void_p(sc.ptr), ctypes.c_void_p(cnt.ptr), None))
+        c = cnt.numpy()
+        D, S = docs.numpy(), sc.numpy()
+        return [(D[q, :c[q]], S[q, :c[q]]) for q in range(B)]
+
+
+def _voc_and_weights(L=2, b=16, seed=0):
+    rng = np.random.default_rng(seed)
+    n = level_beg(b, L)
+    w = (rng.random((n, b)) * 4).astype(np.float32)
+    w[rng.random(w.shape) < 0.15] = np.finfo(np.float32).tiny  # FLT_MIN weights exercise the flush-to-zero path
+    if b == 16:
+        rows = rng.integers(0, 256, (n, 16, 128), dtype=np.uint8)
+        blob = ref_blob(L, oracle.rows_to_ref_nodes(rows), np.full(n, 0xFFFF), w)
+    else:
+        rows = rng.random((n, b, 128)).astype(np.float32) * 255
+        blob = rbw2_blob(b, L, 1, rows, np.full(n, 0xFFFFFFFF), w)
+    return rb.deserialize_vocabulary(blob), w
+
+
+def _random_docs(rng, nb, nleaves, lo=1, hi=300, hot=None):
+    docs = {}
+    for d in range(nb):
+        n = int(rng.integers(lo, hi))
+        if hot is not None:  # skewed: a few hot leaves with long posting lists
+            x = np.where(rng.random(n) < 0.3, rng.choice(hot, n), rng.integers(0, nleaves, n))
+        else:
+            x = rng.integers(0, nleaves, n)
+        docs[d] = x.astype(np.uint32)
+    return docs
+
+
+def _assert_ranked_equal(gd, gs, od, os_, tol=SCORE_TOL):
+    """Ranked ids must be identical; the only allowed difference is an order swap between
+    neighbours whose scores differ by less than tol (reported)."""
+    assert len(gd) == len(od)
+    np.testing.assert_allclose(gs, os_, atol=tol, rtol=0)
+    if not np.array_equal(gd, od):
+        for i in np.nonzero(gd != od)[0]:
+            j = int(np.nonzero(od == gd[i])[0][0]) if gd[i] in od else -1
+            assert j >= 0 and abs(float(os_[j]) - float(gs[i])) < tol, (i, gd[i], od[i])
+
+
+@pytest.mark.parametrize("b,L", [(16, 2), (16, 3), (32, 2)])
+def test_db_reference_api_parity(b, L):
+    voc, w = _voc_and_weights(L, b, seed=L + b)
+    rng = np.random.default_rng(1)
+    nleaves = b ** L
+    docs = _random_docs(rng, 300, nleaves, hot=rng.integers(0, nleaves, 8))
+    g = GpuDb(voc)
+    o = oracle.DataBase(b, L, w)
+    for d, lf in docs.items():
+        g.add(d * 7 + 3, lf)  # sparse doc ids
+        o.add(d * 7 + 3, lf)
+    for leaf in rng.integers(0, nleaves, 30):
+        np.testing.assert_array_equal(g.postings(int(leaf)), o.postings(int(leaf)))
+    for d in list(docs)[:20]:
+        assert g.rcp(d * 7 + 3) == o.rcp(d * 7 + 3)
+    for qi, K in ((5, 10), (17, 50), (40, 1), (3, 300), (9, 1024)):
+        q = docs[qi]
+        gd, gs = g.query(q, K)
+        od, os_ = o.query(q, K)
+        _assert_ranked_equal(gd, gs, od, os_)
+        assert gd[0] == qi * 7 + 3  # self-query first (ApiTests.cpp:136)
+    # a query sharing nothing with the DB, and an empty query
+    gd, _ = g.query(np.array([], np.uint32), 10)
+    assert gd.size == 0
+
+
+def test_db_errors_and_query_and_add():
+    voc, w = _voc_and_weights(2, 16, seed=3)
+    g = GpuDb(voc)
+    o = oracle.DataBase(16, 2, w)
+    rng = np.random.default_rng(2)
+    docs = _random_docs(rng, 50, 256)
+    for d, lf in docs.items():
+        g.add(d, lf)
+        o.add(d, lf)
+    with pytest.raises(rb.RbowError) as e:
+        g.add(3, docs[3])
+    assert e.value.status == rb.ERR_DUPLICATE_DOC and str(e.value) == "Duplicate document index"
+    # queryAndAddDoc = query (without the new doc) then add
+    q = rng.integers(0, 256, 40).astype(np.uint32)
+    out = np.zeros(20, np.uint32)
+    n = ctypes.c_uint32()
+    rb.check(rb.lib.rbow_db_query_and_add_doc(g.h, ctypes.c_uint32(999), q.ctypes.data_as(ctypes.c_void_p),
+                                              ctypes.c_uint32(q.size), ctypes.c_uint32(20),
+                                              out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n)))
+    od, _ = o.query(q, 20)
+    o.add(999, q)
+    np.testing.assert_array_equal(out[:n.value], od)
+    gd, _ = g.query(q, 5)
+    assert gd[0] == 999
+
+
+def test_db_batch_add_and_batch_query():
+    voc, w = _voc_and_weights(3, 16, seed=4)
+    rng = np.random.default_rng(5)
+    nleaves = 16 ** 3
+    docs = _random_docs(rng, 400, nleaves, lo=50, hi=600, hot=rng.integers(0, nleaves, 16))
+    ids = np.array([d * 3 for d in docs], np.uint32)
+    cat = np.concatenate([docs[d] for d in docs])
+    off = np.concatenate([[0], np.cumsum([docs[d].size for d in docs])]).astype(np.uint64)
+    g = GpuDb(voc)
+    # first 100 docs one at a time (host path), the rest in two device batches
+    for d in range(100):
+        g.add(d * 3, docs[d])
+    g.add_batch(ids[100:250], cat, off[100:251])
+    g.add_batch(ids[250:], cat, off[250:])
+    o = oracle.DataBase(16, 3, w)
+    for d, lf in docs.items():
+        o.add(d * 3, lf)
+    for leaf in rng.integers(0, nleaves, 40):
+        np.testing.assert_array_equal(g.postings(int(leaf)), o.postings(int(leaf)))
+    qs = [docs[i] for i in rng.integers(0, 400, 64)]
+    qs[3] = np.array([], np.uint32)
+    qcat = np.concatenate(qs)
+    qoff = np.concatenate([[0], np.cumsum([x.size for x in qs])]).astype(np.uint64)
+    res = g.query_batch(qcat, qoff, 50)
+    for q, (gd, gs) in zip(qs, res):
+        od, os_ = o.query(q, 50)
+        _assert_ranked_equal(gd, gs, od, os_)
+    # each result is the same alone as in the batch
+    for i in (0, 7, 63):
+        gd, gs = g.query(qs[i], 50)
+        np.testing.assert_array_equal(gd, res[i][0])
+        np.testing.assert_array_equal(gs, res[i][1])
+
+
+def test_db_score_error_vs_fp64():
+    voc, w = _voc_and_weights(2, 32, seed=6)
+    rng = np.random.default_rng(7)
+    docs = _random_docs(rng, 200, 1024, lo=100, hi=400)
+    g = GpuDb(voc)
+    o = oracle.DataBase(32, 2, w)
+    for d, lf in docs.items():
+        g.add(d, lf)
+        o.add(d, lf)
+    worst = 0.0
+    for qi in range(0, 200, 10):
+        gd, gs = g.query(docs[qi], 50)
+        fd, fs = o.query_fp64(docs[qi], 50)
+        ref = {int(d): s for d, s in zip(fd, fs)}
+        for d, s in zip(gd, gs):
+            if int(d) in ref:
+                worst = max(worst, abs(float(s) - ref[int(d)]))
+    print(f"max |GPU score - FP64 ideal| = {worst:.3e}")
+    assert worst < 1e-5
