@@ -5,11 +5,19 @@ k-means, branch 16, depth 4 (65,536 words), 2,000,000 SIFT-like uint8[128]
 descriptors, 10 Lloyd passes per node, k-means++ seeds, every node of a level
 in the same launches.  `value` is the k-means assign throughput in tera-ops/s
 (2*16*128 int8 multiply-adds per point per pass) of the I8-MFMA kernel, from
-HIP events around the assign launches inside the timed builds.  Beside it:
-the same build with the plain scalar kernel, descriptor lookup throughput,
-and the oracle on the host cores as the CPU baseline.
+HIP events around the assign launches inside the timed builds.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Beside it on the same JSON line:
+  * the same build with the plain scalar kernel (assign TOPS, "MFMA vs plain");
+  * leaf lookups/s on a 1,048,576-word tree (branch 32, depth 4, float32
+    centroids built here by k-means on the 2M descriptors) for 10,000,000
+    queries: split-BF16 MFMA candidate filter + FP32 re-score vs the plain FP32
+    kernel, with the identity check of the leaf ids and the near-tie count;
+  * leaf lookups/s on a 4,096-word uint8 tree (branch 16, depth 3) for 50,000
+    queries (I8 MFMA vs plain);
+  * the CPU baseline: the oracle, on the host cores, on the same build.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
 import argparse
 import json
@@ -24,32 +32,38 @@ sys.path.insert(0, ROOT)
 
 METRIC = "leaf lookups/s & k-means assign TOPS (MFMA vs plain); query p50/p99 latency"
 I8_PEAK_TOPS = 5000.0     # MI355X dense int8 MFMA, 5.0 POPS
-HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, 8 TB/s
+LOOKUP_BYTES = 132        # algorithmic bytes per descriptor looked up: 128 in + 4 out
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_once(rb, d, n, kernel):
-    voc = rb.build_vocabulary(d, n, 1000, 4, max_iters=10, seed_mode=1, kernel=kernel)
-    return voc, voc.build_stats()
+def timed_lookup(rb, voc, q, n, out, kernel, reps, ties=None):
+    voc.lookup(q, n, out, kernel, ties)  # warm
+    rb.hip.synchronize()
+    e0, e1 = rb.Event(), rb.Event()
+    e0.record()
+    for _ in range(reps):
+        voc.lookup(q, n, out, kernel)
+    e1.record()
+    e1.synchronize()
+    return n * reps / (e0.elapsed_ms(e1) * 1e-3)
 
 
-def cpu_baseline(rb, oracle, threads):
-    """Oracle k-means build on a bounded sample (same shape, 1/10 of the points)."""
-    n = 200_000
-    desc = rb.synth_descriptors(n, seed=1)
+def cpu_baseline(rb, oracle, desc, threads):
+    """Oracle build of the same workload (b16 L4, 10 passes, k-means++) on the host cores."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    n = desc.shape[0]
     t = time.perf_counter()
     res = oracle.build(desc, 1000, 4, max_iters=10, seed_mode=1)
     dt = time.perf_counter() - t
-    # point-passes: node size x passes; node sizes from the parent occupancy
     sizes = np.concatenate([[n], res["occ"].reshape(-1)[: res["passes"].size - 1]])
     ops = float((sizes.astype(np.float64) * res["passes"]).sum()) * 2 * 16 * 128
-    return {"value": ops / dt / 1e12, "unit": "TOPS", "cores": threads, "kind": "port",
-            "sample": f"oracle hierarchical k-means b16 L4, {n} descriptors, 10 passes, k-means++ "
-                      f"({dt:.1f} s wall)"}
+    return res, {"value": round(ops / dt / 1e12, 5), "unit": "TOPS", "cores": threads, "kind": "port",
+                 "sample": f"oracle hierarchical k-means b16 L4 on the same {n} descriptors, 10 passes, "
+                           f"k-means++ ({dt:.1f} s wall; includes the host regroup)"}
 
 
 def main():
@@ -67,19 +81,22 @@ def main():
     import rapidbow_amd as rb
     from rapidbow_amd import hip
 
+    syn = rb.DeviceSynth()
     N = 2_000_000
-    log(f"bench: generating {N} SIFT-like descriptors")
-    desc = rb.synth_descriptors(N, seed=1)
-    d = rb.DeviceArray.from_numpy(desc)
+    log(f"bench: generating {N} SIFT-like descriptors in HBM")
+    d = syn.descriptors(N, seed=1)
+    hip.synchronize()
 
+    # ---------------- batched vocabulary build (the timed steps) ----------------
     log("bench: warmup builds")
     for _ in range(args.warmup):
-        build_once(rb, d, N, 0)
+        rb.build_vocabulary(d, N, 1000, 4, max_iters=10, seed_mode=1, kernel=0)
     hip.synchronize()
     stats = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        voc, st = build_once(rb, d, N, 0)
+        voc = rb.build_vocabulary(d, N, 1000, 4, max_iters=10, seed_mode=1, kernel=0)
+        st = voc.build_stats()
         stats.append(st)
         log(f"bench: step {i}: build {st['total_ms']:.1f} ms, assign {st['assign_ms']:.2f} ms")
     hip.synchronize()
@@ -87,49 +104,79 @@ def main():
     assign_ms = float(np.median([s["assign_ms"] for s in stats]))
     ops = stats[-1]["assign_ops"]
     tops = ops / (assign_ms * 1e-3) / 1e12
-    build_ms = float(np.median([s["total_ms"] for s in stats]))
-
     log("bench: plain-kernel build")
-    _, stp = build_once(rb, d, N, 1)
+    vplain = rb.build_vocabulary(d, N, 1000, 4, max_iters=10, seed_mode=1, kernel=1)
+    stp = vplain.build_stats()
     plain_tops = stp["assign_ops"] / (stp["assign_ms"] * 1e-3) / 1e12
+    same_tree = vplain.serialize() == voc.serialize()
 
-    # lookup on the built tree (b16 L4 uint8), 10M queries resident in HBM
+    # ---------------- 1M-word float32 tree, 10M lookups ----------------
+    log("bench: building the b32 L4 float32 tree (1,048,576 words) on the 2M descriptors")
+    v32 = rb.build_vocabulary(d, N, 1000, 4, branch_factor=32, max_iters=10, seed_mode=1, centroid_type=1, kernel=1)
+    st32 = v32.build_stats()
     NQ = 10_000_000
-    log(f"bench: lookup {NQ} queries")
-    q = rb.DeviceArray.from_numpy(rb.synth_descriptors(NQ, seed=2))
-    out = rb.DeviceArray((NQ,), np.uint32)
-    look = {}
-    for name, k in (("mfma", rb.LOOKUP_MFMA), ("plain", rb.LOOKUP_PLAIN)):
-        voc.lookup(q, NQ, out, k)
-        hip.synchronize()
-        e0, e1 = rb.Event(), rb.Event()
-        e0.record()
-        reps = 3
-        for _ in range(reps):
-            voc.lookup(q, NQ, out, k)
-        e1.record()
-        e1.synchronize()
-        look[name] = NQ * reps / (e0.elapsed_ms(e1) * 1e-3)
-        log(f"bench: lookup {name}: {look[name] / 1e9:.2f} G desc/s")
+    q = syn.descriptors(NQ, seed=2)
+    out_m = rb.DeviceArray((NQ,), np.uint32)
+    out_p = rb.DeviceArray((NQ,), np.uint32)
+    ties = rb.DeviceArray.zeros((1,), np.uint32)
+    log("bench: lookups b32 L4 float32")
+    r_m = timed_lookup(rb, v32, q, NQ, out_m, rb.LOOKUP_MFMA, 5, ties)
+    r_p = timed_lookup(rb, v32, q, NQ, out_p, rb.LOOKUP_PLAIN, 1)
+    lm, lp = out_m.numpy(), out_p.numpy()
+    identical = bool(np.array_equal(lm, lp))
+    near_ties = int(ties.numpy()[0])
+    log(f"bench: b32 lookups mfma {r_m / 1e9:.3f} G/s plain {r_p / 1e9:.3f} G/s identical={identical} "
+        f"near_ties={near_ties}")
+
+    # ---------------- 4,096-word uint8 tree, 50k lookups ----------------
+    d1 = syn.descriptors(200_000, seed=3)
+    v1 = rb.build_vocabulary(d1, 200_000, 1000, 3, max_iters=8, seed_mode=1)
+    q1 = syn.descriptors(50_000, seed=4)
+    o1 = rb.DeviceArray((50_000,), np.uint32)
+    o1p = rb.DeviceArray((50_000,), np.uint32)
+    r1_m = timed_lookup(rb, v1, q1, 50_000, o1, rb.LOOKUP_MFMA, 20)
+    r1_p = timed_lookup(rb, v1, q1, 50_000, o1p, rb.LOOKUP_PLAIN, 20)
+    id1 = bool(np.array_equal(o1.numpy(), o1p.numpy()))
 
     line = {
         "metric": METRIC, "value": round(tops, 3), "unit": "TOPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int8", "data": "synthetic",
         "config": {"workload": "batched vocab build: b16 L4 (65,536 words), 2M SIFT-like uint8[128], "
-                               "10 Lloyd passes/node, k-means++, one launch per level pass",
+                               "10 Lloyd passes/node, k-means++, every node of a level per launch",
                    "nb_desc": N, "branch": 16, "levels": 4, "passes": 10},
         "roofline": {"bound": "mfma", "achieved": round(tops, 3), "peak": I8_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": round(tops / I8_PEAK_TOPS, 5), "traffic": None},
         "assign_tops_plain": round(plain_tops, 3),
-        "build_ms_mfma": round(build_ms, 2), "build_ms_plain": round(stp["total_ms"], 2),
-        "lookup_b16L4_u8_desc_per_s": {k: round(v, 1) for k, v in look.items()},
+        "build_ms": {"mfma": round(float(np.median([s["total_ms"] for s in stats])), 2),
+                     "plain": round(stp["total_ms"], 2), "identical_trees": same_tree,
+                     "b32L4_f32": round(st32["total_ms"], 1)},
+        "lookup_b32L4_f32_10M": {"mfma_desc_per_s": round(r_m, 1), "plain_desc_per_s": round(r_p, 1),
+                                 "identical_leaf_ids": identical, "near_ties": near_ties,
+                                 "hbm_gbs_mfma": round(r_m * LOOKUP_BYTES / 1e9, 1),
+                                 "hbm_frac_mfma": round(r_m * LOOKUP_BYTES / 1e9 / HBM_PEAK_GBS, 4)},
+        "lookup_b16L3_u8_50k": {"mfma_desc_per_s": round(r1_m, 1), "plain_desc_per_s": round(r1_p, 1),
+                                "identical_leaf_ids": id1},
     }
     if not args.no_cpu:
         import oracle
         threads = min(os.cpu_count() or 1, 16)
         log(f"bench: CPU baseline on {threads} threads")
-        line["cpu_baseline"] = cpu_baseline(rb, oracle, threads)
+        host = d.numpy()
+        _, cb = cpu_baseline(rb, oracle, host, threads)
+        line["cpu_baseline"] = cb
+        nq = 200_000
+        qh = q.numpy()[:nq]
+        tr = np.zeros((v32.nb_nodes, 32, 128), np.float32)
+        for l in range(4):
+            b0 = (32 ** l - 1) // 31
+            tr[b0:b0 + 32 ** l] = v32.centroids(l)
+        t = time.perf_counter()
+        lc = oracle.lookup_f32(tr, 32, 4, qh, threads)
+        dt = time.perf_counter() - t
+        line["cpu_lookup_b32L4_f32"] = {"desc_per_s": round(nq / dt, 1), "cores": threads, "kind": "port",
+                                        "sample": f"oracle descent of the first {nq} queries",
+                                        "matches_gpu": bool(np.array_equal(lc, lm[:nq]))}
     print(json.dumps(line), flush=True)
 
 

@@ -74,6 +74,9 @@ _sig("rbow_kmeans_update", _i32, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp
 _sig("rbow_reformat_tiles", _i32, [_vp, _vp, _u32, _u32, _vp, _vp])
 _sig("rbow_gather_rows", _i32, [_vp, _vp, _u64, _vp, _vp])
 _sig("rbow_synth_centres", None, [_u64, _u32, _vp])
+_sig("rbow_synth_descriptors_device", _i32, [_vp, _vp, _u32, _u64, _u64, _u64, _vp, _vp, _vp])
+_sig("rbow_synth_image_centres_device", _i32, [_vp, _u32, _u64, _u64, _u64, _u32, _u32, _vp, _u64, _vp])
+_sig("rbow_synth_image_postings_device", _i32, [_vp, _u32, _u64, _u64, _u64, _u32, _u32, _vp, _u32, _vp, _u64, _vp])
 _sig("rbow_synth_descriptors", None, [_u64, _u32, _u64, _u64, _u64, _vp, ctypes.c_float, _vp])
 
 
@@ -247,3 +250,35 @@ def synth_descriptors(n, seed, first=0, centre_seed=0, nb_centres=2000, centre_i
     lib.rbow_synth_descriptors(centre_seed, nb_centres, seed, first, n, None if ids is None else ids.ctypes.data,
                                sigma_scale, out.ctypes.data)
     return out
+
+
+class DeviceSynth:
+    """SIFT-like data generated directly in HBM (same distribution as synth_descriptors)."""
+
+    def __init__(self, centre_seed=0, nb_centres=2000, sigma_scale=0.3):
+        c = synth_centres(centre_seed, nb_centres)
+        sigma = (sigma_scale * np.sqrt((c.astype(np.float64) ** 2).sum(1)) / np.sqrt(128.0)).astype(np.float32)
+        self.nb_centres = nb_centres
+        self.centres = DeviceArray.from_numpy(c)
+        self.sigma = DeviceArray.from_numpy(sigma)
+
+    def descriptors(self, n, seed, first=0, dev_centre_ids=None, out=None, stream=None):
+        out = out if out is not None else DeviceArray((max(n, 1), 128), np.uint8)
+        check(lib.rbow_synth_descriptors_device(self.centres.ptr, self.sigma.ptr, self.nb_centres, seed, first, n,
+                                                _ptr(dev_centre_ids), out.ptr, _stream(stream)))
+        return out
+
+    def image_centre_ids(self, dev_offsets, nb_images, first_image, image_seed, desc_seed, nb_desc, subset=64,
+                         stream=None):
+        ids = DeviceArray((max(nb_desc, 1),), np.uint32)
+        check(lib.rbow_synth_image_centres_device(_ptr(dev_offsets), nb_images, first_image, image_seed, desc_seed,
+                                                  self.nb_centres, subset, ids.ptr, nb_desc, _stream(stream)))
+        return ids
+
+    def image_postings(self, dev_offsets, nb_images, first_image, image_seed, post_seed, dev_cloud, cloud_size,
+                       nb_postings, subset=64, out=None, stream=None):
+        out = out if out is not None else DeviceArray((max(nb_postings, 1),), np.uint32)
+        check(lib.rbow_synth_image_postings_device(_ptr(dev_offsets), nb_images, first_image, image_seed, post_seed,
+                                                   self.nb_centres, subset, _ptr(dev_cloud), cloud_size, out.ptr,
+                                                   nb_postings, _stream(stream)))
+        return out

@@ -56,8 +56,13 @@ void checkAligned(const void* p, const char* what) {
 
 // Task table + tiles for the kernel-level k-means entries.
 struct KmScratch {
-    DevBuf<uint32_t> beg, cnt, active, passes, nbActive;
+    DevBuf<uint32_t> beg, cnt, active, passes, nbActive, order, tcBeg, tcCnt, partCnt, partChanged;
     DevBuf<KmTile> tiles;
+    DevBuf<KmChunk> chunks, red;
+    DevBuf<int8_t> XA, XT;
+    DevBuf<int32_t> partSums;
+    uint32_t nbRed = 0;
+    std::vector<KmTile> ht;
     KmLevel k;
     uint64_t rows = 0;
     KmScratch(const uint32_t* taskCnt, uint32_t nbTasks, uint32_t b, uint32_t ct, hipStream_t s) {
@@ -65,7 +70,6 @@ struct KmScratch {
         RBOW_REQUIRE(ct <= 1 && (b == 16 || (b == 32 && ct == 1)), RBOW_ERR_UNSUPPORTED,
                      "b must be 16 (uint8 or float centers) or 32 (float centers)");
         std::vector<uint32_t> hb(nbTasks);
-        std::vector<KmTile> ht;
         for (uint32_t g = 0; g < nbTasks; g++) {
             hb[g] = uint32_t(rows);
             for (uint32_t r = 0; r < taskCnt[g]; r += 64)
@@ -97,6 +101,50 @@ struct KmScratch {
         k.active = active.get();
         k.passes = passes.get();
         k.nbActive = nbActive.get();
+    }
+    // operands and partials of the I8 MFMA assign for row-major points
+    void prepareMfma(const uint8_t* points, const uint32_t* taskCnt, hipStream_t s) {
+        std::vector<uint32_t> hOrder(rows);
+        for (uint64_t i = 0; i < rows; i++) hOrder[i] = uint32_t(i);
+        std::vector<KmChunk> hc, hr;
+        std::vector<uint32_t> cb(k.nbTasks), cc(k.nbTasks);
+        uint32_t ti = 0;
+        for (uint32_t g = 0; g < k.nbTasks; g++) {
+            const uint32_t nt = divUp(taskCnt[g], 64);
+            cb[g] = uint32_t(hc.size());
+            for (uint32_t c = 0; c < nt; c += kChunkTiles) hc.push_back(KmChunk{g, ti + c, ti + std::min(nt, c + kChunkTiles)});
+            cc[g] = uint32_t(hc.size()) - cb[g];
+            for (uint32_t c = 0; c < cc[g]; c += 32) hr.push_back(KmChunk{g, cb[g] + c, cb[g] + std::min(cc[g], c + 32)});
+            ti += nt;
+        }
+        order.resize(std::max<uint64_t>(rows, 1));
+        chunks.resize(std::max<size_t>(hc.size(), 1));
+        red.resize(std::max<size_t>(hr.size(), 1));
+        tcBeg.resize(std::max(1u, k.nbTasks));
+        tcCnt.resize(std::max(1u, k.nbTasks));
+        XA.resize(std::max<size_t>(ht.size(), 1) * 8192);
+        XT.resize(std::max<size_t>(ht.size(), 1) * 8192);
+        partSums.resize(std::max<size_t>(hc.size(), 1) * 16 * kDims);
+        partCnt.resize(std::max<size_t>(hc.size(), 1) * 16);
+        partChanged.resize(std::max<size_t>(hc.size(), 1));
+        if (rows) RBOW_HIP(hipMemcpyAsync(order.get(), hOrder.data(), 4 * rows, hipMemcpyHostToDevice, s));
+        if (!hc.empty()) RBOW_HIP(hipMemcpyAsync(chunks.get(), hc.data(), sizeof(KmChunk) * hc.size(), hipMemcpyHostToDevice, s));
+        if (!hr.empty()) RBOW_HIP(hipMemcpyAsync(red.get(), hr.data(), sizeof(KmChunk) * hr.size(), hipMemcpyHostToDevice, s));
+        if (k.nbTasks) {
+            RBOW_HIP(hipMemcpyAsync(tcBeg.get(), cb.data(), 4ull * k.nbTasks, hipMemcpyHostToDevice, s));
+            RBOW_HIP(hipMemcpyAsync(tcCnt.get(), cc.data(), 4ull * k.nbTasks, hipMemcpyHostToDevice, s));
+        }
+        launchGatherTiles(points, order.get(), tiles.get(), uint32_t(ht.size()), XA.get(), XT.get(), s);
+        k.XA = XA.get();
+        k.XT = XT.get();
+        k.chunks = chunks.get();
+        k.nbChunks = uint32_t(hc.size());
+        k.taskChunkBeg = tcBeg.get();
+        k.taskChunkCnt = tcCnt.get();
+        k.partSums = partSums.get();
+        k.partCnt = partCnt.get();
+        k.partChanged = partChanged.get();
+        nbRed = uint32_t(hr.size());
     }
 };
 }  // namespace
@@ -254,7 +302,10 @@ rbow_status rbow_kmeans_assign(const uint8_t* dev_points, const uint32_t* task_c
         sc.k.counts = dev_counts;
         sc.k.sums = reinterpret_cast<long long*>(dev_sums);
         sc.k.changed = dev_changed;
-        launchKmAssign(sc.k, kernel, hs(stream));
+        const bool mfma = kernel == 0 && centroid_type == 0;
+        if (mfma) sc.prepareMfma(dev_points, task_cnt, hs(stream));
+        launchKmAssign(sc.k, mfma ? 0 : 1, hs(stream));
+        if (mfma) launchKmReduce(sc.k, sc.red.get(), sc.nbRed, hs(stream));
         RBOW_HIP(hipStreamSynchronize(hs(stream)));  // scratch lifetime
     });
 }

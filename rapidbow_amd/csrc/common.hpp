@@ -47,6 +47,35 @@ __host__ __device__ inline uint64_t levelSize(uint32_t lg2b, uint32_t level) {
 
 inline uint32_t divUp(uint64_t a, uint64_t b) { return uint32_t((a + b - 1) / b); }
 
+// Minimum over each row of 16 lanes (DPP: quad swaps, half-row and row mirrors).
+template <int CTRL>
+__device__ inline uint32_t dppU(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ inline uint32_t min16u(uint32_t v) {
+    v = min(v, dppU<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = min(v, dppU<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = min(v, dppU<0x141>(v));  // row_half_mirror
+    v = min(v, dppU<0x140>(v));  // row_mirror
+    return v;
+}
+__device__ inline float min16f(float v) {
+    v = fminf(v, __uint_as_float(dppU<0xB1>(__float_as_uint(v))));
+    v = fminf(v, __uint_as_float(dppU<0x4E>(__float_as_uint(v))));
+    v = fminf(v, __uint_as_float(dppU<0x141>(__float_as_uint(v))));
+    v = fminf(v, __uint_as_float(dppU<0x140>(__float_as_uint(v))));
+    return v;
+}
+__device__ inline float min4f(float v) {  // within each quad
+    v = fminf(v, __uint_as_float(dppU<0xB1>(__float_as_uint(v))));
+    return fminf(v, __uint_as_float(dppU<0x4E>(__float_as_uint(v))));
+}
+__device__ inline uint32_t min4u(uint32_t v) {
+    v = min(v, dppU<0xB1>(v));
+    return min(v, dppU<0x4E>(v));
+}
+
+
 // Owning device allocation.
 template <typename T>
 class DevBuf {

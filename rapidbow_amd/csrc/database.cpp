@@ -186,6 +186,19 @@ void Database::queryBatch(const uint32_t* dLeaf, const uint64_t* offsets, uint32
     RBOW_HIP(hipStreamSynchronize(s));  // h (host) must outlive the copy; results ready on return
 }
 
+void Database::prepareForQueries(hipStream_t s) const {
+    std::unique_lock<std::shared_mutex> lk(mtx_);
+    ensureMerged(s);
+}
+
+void Database::scoreAsync(const uint32_t* dLeaf, const uint64_t* dOffsets, const uint64_t* hOffsets,
+                          uint32_t nbQueries, uint32_t K, uint32_t* dDocs, float* dScores, uint32_t* dCounts,
+                          hipStream_t s) const {
+    std::shared_lock<std::shared_mutex> lk(mtx_);
+    RBOW_REQUIRE(pendLeaf_.empty(), RBOW_ERR_INVALID_ARG, "database has unmerged inserts");
+    launchQueryBatch(view(), dLeaf, dOffsets, hOffsets, nbQueries, K, dDocs, dScores, dCounts, s);
+}
+
 uint32_t Database::query(const uint32_t* leaf, uint32_t n, uint32_t K, uint32_t* docs, float* scores) const {
     RBOW_REQUIRE(leaf != nullptr || n == 0, RBOW_ERR_INVALID_ARG, "null leaf ids");
     for (uint32_t i = 0; i < n; i++) RBOW_REQUIRE(leaf[i] < nbLeaves_, RBOW_ERR_INVALID_ARG, "leaf id out of range");

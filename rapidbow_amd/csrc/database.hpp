@@ -45,6 +45,14 @@ public:
     // single query from host leaf ids (synchronous)
     uint32_t query(const uint32_t* leaf, uint32_t n, uint32_t K, uint32_t* docs, float* scores) const;
 
+    // Serving: fold pending inserts into the CSR, then score batches
+    // asynchronously (no insert may run concurrently with scoreAsync).
+    void prepareForQueries(hipStream_t s) const;
+    void scoreAsync(const uint32_t* dLeaf, const uint64_t* dOffsets, const uint64_t* hOffsets, uint32_t nbQueries,
+                    uint32_t K, uint32_t* dDocs, float* dScores, uint32_t* dCounts, hipStream_t s) const;
+    // HBM scratch of one query of n descriptors during scoring
+    uint64_t scratchBytes(uint32_t n) const { return uint64_t(nbDocs()) * 9 + uint64_t(n) * (L_ * 40 + 8) + 64; }
+
     uint32_t nbDocs() const;
     uint64_t nbPostings() const;
     bool docRcp(uint32_t doc, float* rcp) const;

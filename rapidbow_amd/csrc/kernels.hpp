@@ -9,8 +9,9 @@ namespace rbow_amd {
 //   rows      [node][b][128]  centroid rows, child-major (uint8 or float32)
 //   u8 trees: normU[node*b+j]  = ||c||^2            (plain dot4 form)
 //             normS[node*b+j]  = ||c - 128||^2      (I8 MFMA form, offset operands)
-//   f32 trees: hi/lo          = bf16 split of the rows (c ~= hi + lo)
-//             cn2[node*b+j]   = ||c||^2 (fp32 of the binary64 value)
+//   f32 trees: mu[node][128]  = integer centre of the node's children (uint8)
+//             hi/lo           = bf16 split of c - mu (c - mu ~= hi + lo)
+//             cn2[node*b+j]   = ||c - mu||^2 (fp32 of the binary64 value)
 //             e1[node*b+j]    = certified bound coefficient of ||x|| (see lookup.hip)
 //             e2[node*b+j]    = certified bound constant term
 struct TreeView {
@@ -19,6 +20,7 @@ struct TreeView {
     const int32_t* normU = nullptr;
     const int32_t* normS = nullptr;
     const float* f32rows = nullptr;
+    const uint8_t* mu = nullptr;
     const uint16_t* hi = nullptr;
     const uint16_t* lo = nullptr;
     const float* cn2 = nullptr;
@@ -28,8 +30,8 @@ struct TreeView {
 
 // Derived per-child data (norms, bf16 split, bounds) for nodes [0, nbNodes).
 void launchPrepU8Tree(const uint8_t* rows, uint64_t nbChildren, int32_t* normU, int32_t* normS, hipStream_t s);
-void launchPrepF32Tree(const float* rows, uint64_t nbChildren, uint16_t* hi, uint16_t* lo, float* cn2, float* e1,
-                       float* e2, hipStream_t s);
+void launchPrepF32Tree(const float* rows, uint64_t nbNodes, uint32_t b, uint8_t* mu, uint16_t* hi, uint16_t* lo,
+                       float* cn2, float* e1, float* e2, hipStream_t s);
 
 // Plain scalar per-descriptor descent (vocTreeLookUp.cu:48-160 semantics).
 void launchLookupPlain(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s);
@@ -51,6 +53,13 @@ struct KmTile {        // one work item of the assign kernels: <= 64 rows of one
     uint32_t cnt;      // rows in this tile (1..64)
 };
 
+struct KmChunk {       // a run of <= kChunkTiles tiles of one task, processed by one wave
+    uint32_t task;
+    uint32_t tileBeg;
+    uint32_t tileEnd;
+};
+constexpr uint32_t kChunkTiles = 16;
+
 struct KmLevel {       // device state of one level's batched k-means
     uint32_t b = 16, nbTasks = 0, nbTiles = 0, centroidType = 0;
     const uint8_t* X = nullptr;      // gathered rows [N][128]
@@ -66,6 +75,16 @@ struct KmLevel {       // device state of one level's batched k-means
     uint32_t* active = nullptr;      // [task]
     uint32_t* passes = nullptr;      // [task]
     uint32_t* nbActive = nullptr;    // [1]
+    // MFMA path (uint8 centroids): per-tile swizzled operands and per-chunk partials
+    const int8_t* XA = nullptr;      // [tile][rb 4][kb 2][lane 64][16]  distance A fragments (x - 128)
+    const int8_t* XT = nullptr;      // [tile][db 8][lane 64][16]        sum B fragments (x - 128), transposed
+    const KmChunk* chunks = nullptr;
+    uint32_t nbChunks = 0;
+    const uint32_t* taskChunkBeg = nullptr;  // [task]
+    const uint32_t* taskChunkCnt = nullptr;  // [task]
+    int32_t* partSums = nullptr;     // [chunk][16][128]  sums of (x - 128)
+    uint32_t* partCnt = nullptr;     // [chunk][16]
+    uint32_t* partChanged = nullptr; // [chunk]
 };
 
 // centers[task][j] = desc[seedIdx[task*b+j]] (invalid -> 0xFF / 255.0f)
@@ -75,6 +94,12 @@ void launchKmeansPP(const uint8_t* desc, const uint32_t* order, const uint32_t* 
                     uint32_t nbTasks, uint32_t b, uint64_t seed, uint32_t level, uint32_t* seedIdx, uint32_t* masks,
                     hipStream_t s);
 void launchKmLevelReset(const KmLevel& k, uint64_t nbRows, hipStream_t s);
+// Per-tile swizzled operands XA / XT of the MFMA assign from desc[order[row]].
+void launchGatherTiles(const uint8_t* desc, const uint32_t* order, const KmTile* tiles, uint32_t nbTiles, int8_t* XA,
+                       int8_t* XT, hipStream_t s);
+// Adds the per-chunk partials of the MFMA assign into the task totals
+// counts / sums / changed.  red: (task, chunkBeg, chunkEnd) items.
+void launchKmReduce(const KmLevel& k, const KmChunk* red, uint32_t nbRed, hipStream_t s);
 // One assign pass over all active tasks; kernel 0 = MFMA, 1 = plain.
 void launchKmAssign(const KmLevel& k, int kernel, hipStream_t s);
 // Update pass: converged tasks become inactive, others move their centers

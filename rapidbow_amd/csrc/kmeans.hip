@@ -240,67 +240,164 @@ __device__ inline void waveRange(uint32_t nbTiles, uint32_t& beg, uint32_t& end)
 }
 
 // ----------------------------------------------- assign: uint8, I8 MFMA --
-__global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
-    __shared__ uint32_t sSums[kWaves][16 * kDims];
-    __shared__ uint32_t sCnt[kWaves][16];
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t col = lane & 15, grp = lane >> 4;
-    WaveAcc<16> acc{sSums[wave], sCnt[wave]};
-    acc.clear(lane);
-    uint32_t beg, end;
-    waveRange(k.nbTiles, beg, end);
-    uint32_t curTask = kIdxInvalid, changedLocal = 0;
-    v4i b0 = {0, 0, 0, 0}, b1 = {0, 0, 0, 0};
-    int ns = 0;
-    for (uint32_t ti = beg; ti < end; ti++) {
-        const KmTile t = k.tiles[ti];
-        if (!k.active[t.task]) continue;
-        if (t.task != curTask) {
-            if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
-            changedLocal = 0;
-            curTask = t.task;
-            const uint8_t* crow = k.centersU8 + (uint64_t(curTask) * 16 + col) * kDims;
-            b0 = toSigned(ld16(crow + grp * 16));
-            b1 = toSigned(ld16(crow + 64 + grp * 16));
-            int part = 0;
+// Per 64-row tile the level gather writes two swizzled copies of x' = x - 128:
+//   XA[tile][rb][kb][lane][16]: A fragments of the distance MFMA, row rb*16 + (lane&15),
+//                               dims kb*64 + (lane>>4)*16 .. +16;
+//   XT[tile][db][lane][16]:     B fragments of the sum MFMA, byte q = point
+//                               16*(q>>2) + 4*(lane>>4) + (q&3), dim 16*db + (lane&15).
+// Distances: mfma_i32_16x16x64_i8(XA, c') -> ||c'||^2 - 2 x'.c' (exact int32).
+// Sums: mfma_i32_16x16x64_i8(onehot^T, XT) -> per (center, dim) sums of x' over
+// the tile's points (exact int32), i.e. the update's centroid sums on the matrix
+// cores with no atomics; the 4x4 rows a lane owns after the distance MFMA are
+// exactly the 16 points of its K slice in XT, so the one-hot operand is local.
+__global__ __launch_bounds__(256) void k_gatherTiles(const uint8_t* __restrict__ desc, const uint32_t* __restrict__ order,
+                                                    const KmTile* __restrict__ tiles, int8_t* __restrict__ XA,
+                                                    int8_t* __restrict__ XT) {
+    __shared__ __align__(16) uint8_t sx[64][144];
+    const uint32_t t = blockIdx.x, tid = threadIdx.x;
+    const KmTile tl = tiles[t];
+    for (uint32_t c = tid; c < 512; c += 256) {
+        const uint32_t r = c >> 3, part = c & 7;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);  // padding rows: x' = 0
+        if (r < tl.cnt) {
+            v = reinterpret_cast<const uint4*>(desc + uint64_t(order[tl.row + r]) * kDims)[part];
+            v.x ^= 0x80808080u;
+            v.y ^= 0x80808080u;
+            v.z ^= 0x80808080u;
+            v.w ^= 0x80808080u;
+        }
+        *reinterpret_cast<uint4*>(&sx[r][part * 16]) = v;
+    }
+    __syncthreads();
+    int8_t* xa = XA + uint64_t(t) * 8192;
+    int8_t* xt = XT + uint64_t(t) * 8192;
+    for (uint32_t c = tid; c < 512; c += 256) {
+        const uint32_t rb = c >> 7, kb = (c >> 6) & 1, l = c & 63;
+        *reinterpret_cast<uint4*>(xa + c * 16) =
+            *reinterpret_cast<const uint4*>(&sx[rb * 16 + (l & 15)][kb * 64 + (l >> 4) * 16]);
+    }
+    for (uint32_t c = tid; c < 512; c += 256) {
+        const uint32_t db = c >> 6, l = c & 63;
+        uint32_t w[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            uint32_t v = 0;
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                part = __builtin_amdgcn_sdot4(b0[e], b0[e], part, false);
-                part = __builtin_amdgcn_sdot4(b1[e], b1[e], part, false);
+                const uint32_t q = 4 * d + e;
+                const uint32_t row = 16 * (q >> 2) + 4 * (l >> 4) + (q & 3);
+                v |= uint32_t(sx[row][16 * db + (l & 15)]) << (8 * e);
             }
-            part += __shfl_xor(part, 16, 64);
-            part += __shfl_xor(part, 32, 64);
-            ns = part;  // ||c - 128||^2 of center `col`
+            w[d] = v;
         }
-        uint32_t myA = 0;
-        for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
-            const uint32_t r = rb * 16 + col;
-            v4i a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
-            if (r < t.cnt) {
-                const uint8_t* x = k.X + uint64_t(t.row + r) * kDims;
-                a0 = toSigned(ld16(x + grp * 16));
-                a1 = toSigned(ld16(x + 64 + grp * 16));
-            }
-            v4i d = {0, 0, 0, 0};
-            d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b0, d, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b1, d, 0, 0, 0);
-            uint32_t key[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) key[i] = (uint32_t(ns - 2 * d[i] + (1 << 25)) << 4) | col;
-#pragma unroll
-            for (int m = 1; m < 16; m <<= 1)
-#pragma unroll
-                for (int i = 0; i < 4; i++) key[i] = min(key[i], uint32_t(__shfl_xor(int(key[i]), m, 16)));
-            // lane (16*grp + i) holds row 4*grp + i of this block; move to lane = row
-            const uint32_t mine = key[col & 3];
-            const uint32_t w = lane & 15;
-            const uint32_t src = 16 * (w >> 2) + (w & 3);
-            const uint32_t got = uint32_t(__shfl(int(mine), int(src), 64));
-            if ((lane >> 4) == rb) myA = got & 15u;
-        }
-        changedLocal += tileEpilogue<16>(k, t, lane, myA, acc);
+        *reinterpret_cast<uint4*>(xt + c * 16) = make_uint4(w[0], w[1], w[2], w[3]);
     }
-    if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
+}
+
+__global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t col = lane & 15, grp = lane >> 4;
+    for (uint32_t ch = blockIdx.x * kWaves + wave; ch < k.nbChunks; ch += gridDim.x * kWaves) {
+        const KmChunk c = k.chunks[ch];
+        if (!k.active[c.task]) continue;
+        const uint8_t* crow = k.centersU8 + (uint64_t(c.task) * 16 + col) * kDims;
+        const v4i b0 = toSigned(ld16(crow + grp * 16)), b1 = toSigned(ld16(crow + 64 + grp * 16));
+        int ns = 0;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            ns = __builtin_amdgcn_sdot4(b0[e], b0[e], ns, false);
+            ns = __builtin_amdgcn_sdot4(b1[e], b1[e], ns, false);
+        }
+        ns += __shfl_xor(ns, 16, 64);
+        ns += __shfl_xor(ns, 32, 64);  // ||c - 128||^2 of center `col`
+        v4i acc[8];
+#pragma unroll
+        for (int db = 0; db < 8; db++) acc[db] = v4i{0, 0, 0, 0};
+        uint32_t myCnt = 0, myChanged = 0;
+        for (uint32_t ti = c.tileBeg; ti < c.tileEnd; ti++) {
+            const KmTile t = k.tiles[ti];
+            const int8_t* xa = k.XA + uint64_t(ti) * 8192;
+            const int8_t* xt = k.XT + uint64_t(ti) * 8192;
+            uint32_t key[4][4];
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) {
+                const v4i a0 = *reinterpret_cast<const v4i*>(xa + ((rb * 2 + 0) * 64 + lane) * 16);
+                const v4i a1 = *reinterpret_cast<const v4i*>(xa + ((rb * 2 + 1) * 64 + lane) * 16);
+                v4i d = {0, 0, 0, 0};
+                d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b0, d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b1, d, 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 4; i++) key[rb][i] = (uint32_t(ns - 2 * d[i] + (1 << 25)) << 4) | col;
+            }
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) key[rb][i] = min16u(key[rb][i]);
+            v4i oh;
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) {
+                uint32_t w = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t row = rb * 16 + 4 * grp + i;
+                    const uint32_t a = key[rb][i] & 15u;
+                    const bool valid = row < t.cnt;
+                    if (valid && col == uint32_t(i)) {  // one writer per row
+                        const uint32_t p = t.row + row;
+                        myChanged += k.assign[p] != a;
+                        k.assign[p] = uint8_t(a);
+                    }
+                    w |= uint32_t(valid && a == col) << (8 * i);
+                }
+                oh[rb] = int(w);
+            }
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) myCnt = __builtin_amdgcn_udot4(uint32_t(oh[rb]), 0x01010101u, myCnt, false);
+#pragma unroll
+            for (int db = 0; db < 8; db++) {
+                const v4i bt = *reinterpret_cast<const v4i*>(xt + (db * 64 + lane) * 16);
+                acc[db] = __builtin_amdgcn_mfma_i32_16x16x64_i8(oh, bt, acc[db], 0, 0, 0);
+            }
+        }
+        int32_t* ps = k.partSums + uint64_t(ch) * 16 * kDims;
+#pragma unroll
+        for (int db = 0; db < 8; db++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) ps[(4 * grp + i) * kDims + 16 * db + col] = acc[db][i];
+        myCnt += uint32_t(__shfl_xor(int(myCnt), 16, 64));
+        myCnt += uint32_t(__shfl_xor(int(myCnt), 32, 64));
+        if (lane < 16) k.partCnt[ch * 16 + lane] = myCnt;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) myChanged += uint32_t(__shfl_xor(int(myChanged), m, 64));
+        if (lane == 0) k.partChanged[ch] = myChanged;
+    }
+}
+
+// Reduce items: (task, chunk range); adds the partials into the task totals.
+__global__ __launch_bounds__(256) void k_reduceChunks(KmLevel k, const KmChunk* __restrict__ red, uint32_t nbRed) {
+    __shared__ uint32_t sCnt[16];
+    const KmChunk r = red[blockIdx.x];
+    const uint32_t g = r.task, tid = threadIdx.x;
+    if (!k.active[g]) return;
+    if (tid < 16) {
+        uint32_t c = 0;
+        for (uint32_t ch = r.tileBeg; ch < r.tileEnd; ch++) c += k.partCnt[ch * 16 + tid];
+        sCnt[tid] = c;
+        if (c) atomicAdd(&k.counts[g * 16 + tid], c);
+    }
+    if (tid == 0) {
+        uint32_t c = 0;
+        for (uint32_t ch = r.tileBeg; ch < r.tileEnd; ch++) c += k.partChanged[ch];
+        if (c) atomicAdd(&k.changed[g], c);
+    }
+    __syncthreads();
+    unsigned long long* gs = reinterpret_cast<unsigned long long*>(k.sums) + uint64_t(g) * 16 * kDims;
+    for (uint32_t e = tid; e < 16 * kDims; e += 256) {
+        long long s = 0;
+        for (uint32_t ch = r.tileBeg; ch < r.tileEnd; ch++) s += k.partSums[uint64_t(ch) * 16 * kDims + e];
+        s += 128ll * sCnt[e / kDims];  // partials are sums of x - 128
+        if (s) atomicAdd(&gs[e], (unsigned long long)s);
+    }
 }
 
 // ----------------------------------------------- assign: uint8, plain ----
@@ -342,9 +439,7 @@ __global__ __launch_bounds__(256) void k_assignU8Plain(KmLevel k) {
 #pragma unroll
                 for (int w = 0; w < 32; w++) dot = __builtin_amdgcn_udot4(x[w], cw[w], dot, false);
             }
-            uint32_t key = (uint32_t(nu - 2 * int(dot) + (1 << 25)) << 4) | j;
-#pragma unroll
-            for (int m = 1; m < 16; m <<= 1) key = min(key, uint32_t(__shfl_xor(int(key), m, 16)));
+            const uint32_t key = min16u((uint32_t(nu - 2 * int(dot) + (1 << 25)) << 4) | j);
             const uint32_t got = uint32_t(__shfl(int(key), int(16 * (lane & 3)), 64));
             if ((lane >> 2) == (r0 >> 2)) myA = got & 15u;
         }
@@ -453,6 +548,65 @@ __global__ __launch_bounds__(256) void k_update(KmLevel k) {
     }
 }
 
+// Update of the MFMA path: tasks with <= 32 chunks reduce their partials here
+// (no atomics); larger tasks were reduced into the totals by k_reduceChunks.
+__global__ __launch_bounds__(256) void k_updateU8Part(KmLevel k) {
+    __shared__ uint32_t sCnt[16];
+    __shared__ uint32_t sChanged;
+    const uint32_t g = blockIdx.x, tid = threadIdx.x;
+    if (!k.active[g]) return;
+    const uint32_t cb = k.taskChunkBeg[g], nc = k.taskChunkCnt[g];
+    const bool direct = nc <= 32;
+    if (tid < 16) {
+        uint32_t c = 0;
+        if (direct)
+            for (uint32_t ch = cb; ch < cb + nc; ch++) c += k.partCnt[ch * 16 + tid];
+        else
+            c = k.counts[g * 16 + tid];
+        sCnt[tid] = c;
+    }
+    if (tid == 0) {
+        uint32_t c = 0;
+        if (direct)
+            for (uint32_t ch = cb; ch < cb + nc; ch++) c += k.partChanged[ch];
+        else
+            c = k.changed[g];
+        sChanged = c;
+    }
+    __syncthreads();
+    unsigned long long* sums = reinterpret_cast<unsigned long long*>(k.sums) + uint64_t(g) * 16 * kDims;
+    if (sChanged == 0) {  // converged on this pass: no update, task stops
+        if (tid == 0) {
+            k.active[g] = 0;
+            k.passes[g]++;
+        }
+        if (!direct) {
+            for (uint32_t e = tid; e < 16 * kDims; e += 256) sums[e] = 0;
+            if (tid < 16) k.counts[g * 16 + tid] = 0;
+            if (tid == 0) k.changed[g] = 0;
+        }
+        return;
+    }
+    for (uint32_t e = tid; e < 16 * kDims; e += 256) {
+        const uint32_t c = sCnt[e / kDims];
+        long long sum;
+        if (direct) {
+            sum = 128ll * c;  // partials are sums of x - 128
+            for (uint32_t ch = cb; ch < cb + nc; ch++) sum += k.partSums[uint64_t(ch) * 16 * kDims + e];
+        } else {
+            sum = (long long)sums[e];
+            sums[e] = 0;
+        }
+        if (c) k.centersU8[uint64_t(g) * 16 * kDims + e] = uint8_t(roundf(float(sum) / float(c)));
+    }
+    if (!direct && tid < 16) k.counts[g * 16 + tid] = 0;
+    if (tid == 0) {
+        if (!direct) k.changed[g] = 0;
+        k.passes[g]++;
+        atomicAdd(k.nbActive, 1u);
+    }
+}
+
 __global__ void k_levelReset(KmLevel k, uint64_t nbRows) {
     const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x, stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t i = tid; i < nbRows; i += stride) k.assign[i] = 0xFF;  // "no previous assignment"
@@ -508,12 +662,25 @@ void launchKmLevelReset(const KmLevel& k, uint64_t nbRows, hipStream_t s) {
     RBOW_HIP(hipGetLastError());
 }
 
+void launchGatherTiles(const uint8_t* desc, const uint32_t* order, const KmTile* tiles, uint32_t nbTiles, int8_t* XA,
+                       int8_t* XT, hipStream_t s) {
+    if (!nbTiles) return;
+    k_gatherTiles<<<nbTiles, 256, 0, s>>>(desc, order, tiles, XA, XT);
+    RBOW_HIP(hipGetLastError());
+}
+
+void launchKmReduce(const KmLevel& k, const KmChunk* red, uint32_t nbRed, hipStream_t s) {
+    if (!nbRed) return;
+    k_reduceChunks<<<nbRed, 256, 0, s>>>(k, red, nbRed);
+    RBOW_HIP(hipGetLastError());
+}
+
 void launchKmAssign(const KmLevel& k, int kernel, hipStream_t s) {
     if (!k.nbTiles) return;
     const uint32_t grid = gridFor(k.nbTiles, kWaves, 2048);
     if (k.centroidType == 0) {
         if (kernel == 1) k_assignU8Plain<<<grid, 256, 0, s>>>(k);
-        else k_assignU8Mfma<<<grid, 256, 0, s>>>(k);
+        else k_assignU8Mfma<<<gridFor(k.nbChunks, kWaves, 4096), 256, 0, s>>>(k);
     } else if (k.b == 32) {
         k_assignF32Plain<32><<<grid, 256, 0, s>>>(k);
     } else {
@@ -524,7 +691,8 @@ void launchKmAssign(const KmLevel& k, int kernel, hipStream_t s) {
 
 void launchKmUpdate(const KmLevel& k, hipStream_t s) {
     if (!k.nbTasks) return;
-    if (k.b == 32) k_update<32><<<k.nbTasks, 256, 0, s>>>(k);
+    if (k.partSums && k.centroidType == 0) k_updateU8Part<<<k.nbTasks, 256, 0, s>>>(k);
+    else if (k.b == 32) k_update<32><<<k.nbTasks, 256, 0, s>>>(k);
     else k_update<16><<<k.nbTasks, 256, 0, s>>>(k);
     RBOW_HIP(hipGetLastError());
 }

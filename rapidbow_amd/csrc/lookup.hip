@@ -13,11 +13,15 @@
 //    (histogram -> scan -> scatter), so each wave multiplies up to kTQ
 //    descriptors of ONE node against that node's b children on the matrix
 //    cores.  uint8 trees: I8 MFMA on offset operands (x-128, c-128), exact
-//    int32.  Float trees: split-BF16 MFMA (c ~= hi + lo) with FP32
-//    accumulation gives an approximate score with a certified error bound;
-//    every child whose interval can reach the minimum is a candidate, and when
-//    more than one survives the candidates are re-scored in the plain FP32
-//    order, so the argmin equals the plain kernel's.
+//    int32.  Float trees: both operands are centred on the node's integer
+//    mean mu (x' = x - mu is exact in BF16), c' = c - mu is split as
+//    hi + lo + r (BF16, BF16, exact binary64 residual); split-BF16 MFMA with
+//    FP32 accumulation gives an approximate score with a certified error
+//    bound; every child whose interval can reach the minimum is a candidate,
+//    and when more than one survives the candidates are re-scored in the plain
+//    FP32 order, so the argmin equals the plain kernel's.
+#include <hipcub/hipcub.hpp>
+
 #include "kernels.hpp"
 
 namespace rbow_amd {
@@ -29,12 +33,21 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr int kScoreBias = 1 << 25;   // makes integer scores positive before key packing
 constexpr uint32_t kTQ = 128;         // descriptors per MFMA work item (8 row blocks of 16)
-constexpr float kRelXsq = 1.0f / 16384.0f;  // 2^-14: plain-order + rounding bound coefficient
+constexpr float kGammaPlain = 1.0f / 65536.0f;  // 2^-16 >= 131 * 2^-24: relative error of the plain FP32 chain
 
+// A work item: up to kTQ descriptors that sit at the same node.  Items of a
+// node are consecutive; itemBase/nItems locate the node's items so that the
+// per-item child counts can be laid out node-major, child-major, item-minor:
+//   C[itemBase*b + j*nItems + (item - itemBase)]
+// One exclusive scan over C then gives, for every (item, child j), the first
+// position of its descriptors in the next level's bucketed order.
 struct LkItem {
-    uint32_t node;   // node index within the level
-    uint32_t start;  // first position in the bucketed permutation
-    uint32_t cnt;    // descriptors (1..kTQ)
+    uint32_t node;      // node index within the level
+    uint32_t start;     // first position in the bucketed permutation
+    uint32_t cnt;       // descriptors (1..kTQ)
+    uint32_t itemBase;  // first item of this node
+    uint32_t nItems;    // items of this node
+    uint32_t pad;
 };
 
 __device__ inline v4i ld16(const uint8_t* p) { return *reinterpret_cast<const v4i*>(p); }
@@ -73,33 +86,51 @@ __global__ void k_prepU8(const uint8_t* rows, uint64_t nbChildren, int32_t* norm
     normS[i] = ss;
 }
 
-// Bound of |score_mfma - score_plain| for one child (see the header comment):
-//   e1 * ||x|| + 2^-14 * ||x||^2 + e2, with
-//   e1 = 2||c-hi-lo|| + 2^-15 (||hi|| + ||lo||)   (operand split + MFMA accumulation)
-//   e2 = 2^-14 ||c||^2                               (plain fma-chain rounding, norms)
-__global__ void k_prepF32(const float* rows, uint64_t nbChildren, uint16_t* hi, uint16_t* lo, float* cn2, float* e1,
-                          float* e2) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nbChildren) return;
-    double cn = 0, rn = 0, hn = 0, ln = 0;
-    for (uint32_t k = 0; k < kDims; k++) {
-        const float c = rows[i * kDims + k];
-        const uint16_t h = bf16Rne(c);
-        const float hf = bf16ToF(h);
-        const uint16_t l = bf16Rne(c - hf);
-        const float lf = bf16ToF(l);
-        const double r = double(c) - double(hf) - double(lf);
-        cn += double(c) * c;
-        rn += r * r;
-        hn += double(hf) * hf;
-        ln += double(lf) * lf;
-        hi[i * kDims + k] = h;
-        lo[i * kDims + k] = l;
+// One 128-thread block per node (thread k = dimension k):
+//   mu_k = rint(mean_j c_jk)                    per-node centring (integer, 0..255)
+//   c'   = c - mu (binary64), hi = bf16(c'), lo = bf16(c' - hi), r = c' - hi - lo
+//   cn2  = ||c'||^2
+//   e1   = (2||r|| + 2^-13 (||hi|| + ||lo||)) (1 + 2^-10)
+//            -> |MFMA score - (||c'||^2 - 2 x'.c')| <= e1 ||x'|| + e2 : operand split, worst-case
+//               sequential FP32 accumulation of the 256 products, rounding of the score
+//   e2   = 2^-22 ||c'||^2 + 1e-4
+__global__ __launch_bounds__(128) void k_prepF32(const float* rows, uint32_t b, uint8_t* mu, uint16_t* hi,
+                                                uint16_t* lo, float* cn2, float* e1, float* e2) {
+    __shared__ double red[4][2][32];
+    const uint64_t node = blockIdx.x;
+    const uint32_t k = threadIdx.x, lane = k & 63, wave = k >> 6;
+    const float* c = rows + node * b * kDims;
+    double m = 0;
+    for (uint32_t j = 0; j < b; j++) m += c[j * kDims + k];
+    const double muk = fmin(255.0, fmax(0.0, rint(m / b)));
+    mu[node * kDims + k] = uint8_t(muk);
+    for (uint32_t j = 0; j < b; j++) {
+        const uint64_t child = node * b + j;
+        const double cd = double(c[j * kDims + k]) - muk;
+        const uint16_t h = bf16Rne(float(cd));
+        const double hf = bf16ToF(h);
+        const uint16_t l = bf16Rne(float(cd - hf));
+        const double lf = bf16ToF(l);
+        const double r = cd - hf - lf;
+        hi[child * kDims + k] = h;
+        lo[child * kDims + k] = l;
+        double v[4] = {cd * cd, r * r, hf * hf, lf * lf};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            for (int d = 32; d >= 1; d >>= 1) v[q] += __shfl_xor(v[q], d, 64);
+            if (lane == 0) red[q][wave][j] = v[q];
+        }
     }
-    const double slack = 1.0 + 1.0 / 1024;
-    cn2[i] = float(cn);
-    e1[i] = float((2.0 * sqrt(rn) + (sqrt(hn) + sqrt(ln)) / 32768.0) * slack);
-    e2[i] = float(cn / 16384.0 * slack + 1e-3);
+    __syncthreads();
+    if (k < b) {
+        const uint64_t child = node * b + k;
+        const double cn = red[0][0][k] + red[0][1][k], rn = red[1][0][k] + red[1][1][k];
+        const double hn = red[2][0][k] + red[2][1][k], ln = red[3][0][k] + red[3][1][k];
+        const double slack = 1.0 + 1.0 / 1024;
+        cn2[child] = float(cn);
+        e1[child] = float((2.0 * sqrt(rn) + (sqrt(hn) + sqrt(ln)) / 8192.0) * slack);
+        e2[child] = float(cn / 4194304.0 * slack + 1e-4);
+    }
 }
 
 // -------------------------------------------------------- plain kernels --
@@ -130,9 +161,7 @@ __global__ __launch_bounds__(256) void k_lookupPlainU8(const uint8_t* __restrict
             dot = __builtin_amdgcn_udot4(sx[slot][4 * w4 + 3], c.w, dot, false);
         }
         const int score = normU[child] - 2 * int(dot);
-        uint32_t key = (uint32_t(score + kScoreBias) << 4) | lane;
-#pragma unroll
-        for (int m = 1; m < 16; m <<= 1) key = min(key, uint32_t(__shfl_xor(int(key), m, 16)));
+        const uint32_t key = min16u((uint32_t(score + kScoreBias) << 4) | lane);
         node = node * 16 + (key & 15u);
     }
     const uint64_t q = uint64_t(blockIdx.x) * 16 + slot;
@@ -197,49 +226,13 @@ __device__ inline uint32_t waveInclusiveScan(uint32_t v, uint32_t lane) {
     return v;
 }
 
-// One 1024-thread block: cursor[v] = exclusive prefix of hist[v], and the
-// work-item list (<= kTQ descriptors of one node per item).
-__global__ __launch_bounds__(1024) void k_scanItems(const uint32_t* __restrict__ hist, uint32_t nb,
-                                                   uint32_t* __restrict__ cursor, LkItem* __restrict__ items,
-                                                   uint32_t* __restrict__ nbItems) {
-    __shared__ uint32_t wsC[16], wsI[16];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t per = (nb + 1023) / 1024;
-    const uint32_t beg = min(nb, tid * per), end = min(nb, beg + per);
-    uint32_t sC = 0, sI = 0;
-    for (uint32_t v = beg; v < end; v++) {
-        const uint32_t c = hist[v];
-        sC += c;
-        sI += (c + kTQ - 1) / kTQ;
-    }
-    const uint32_t iC = waveInclusiveScan(sC, lane), iI = waveInclusiveScan(sI, lane);
-    if (lane == 63) {
-        wsC[wave] = iC;
-        wsI[wave] = iI;
-    }
-    __syncthreads();
-    uint32_t offC = 0, offI = 0;
-    for (uint32_t w = 0; w < wave; w++) {
-        offC += wsC[w];
-        offI += wsI[w];
-    }
-    uint32_t runC = offC + iC - sC, runI = offI + iI - sI;
-    for (uint32_t v = beg; v < end; v++) {
-        const uint32_t c = hist[v];
-        cursor[v] = runC;
-        for (uint32_t t = 0; t * kTQ < c; t++) items[runI + t] = LkItem{v, runC + t * kTQ, min(kTQ, c - t * kTQ)};
-        runC += c;
-        runI += (c + kTQ - 1) / kTQ;
-    }
-    if (tid == 1023) *nbItems = runI;
+__device__ inline void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
-__global__ void k_scatter(const uint32_t* __restrict__ cur, uint32_t n, uint32_t* __restrict__ cursor,
-                          uint32_t* __restrict__ perm) {
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        const uint32_t pos = atomicAdd(&cursor[cur[q]], 1u);
-        perm[pos] = q;
-    }
+__device__ inline uint32_t mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
 }
 
 __global__ void k_fillU32(uint32_t* p, uint32_t v, uint64_t n) {
@@ -247,24 +240,106 @@ __global__ void k_fillU32(uint32_t* p, uint32_t v, uint64_t n) {
         p[i] = v;
 }
 
-// Resolve an item: node, first position, count.  kIdentity = level 0 (no bucketing).
+// Resolve an item.  kIdentity = level 0: the descriptors in input order,
+// every item at the root.
 template <bool kIdentity>
-__device__ inline void itemAt(uint32_t it, uint32_t n, const LkItem* items, uint32_t& node, uint32_t& start,
-                              uint32_t& cnt) {
+__device__ inline LkItem itemAt(uint32_t it, uint32_t n, const LkItem* items) {
     if (kIdentity) {
-        node = 0;
-        start = it * kTQ;
-        cnt = min(kTQ, n - start);
-    } else {
-        const LkItem t = items[it];
-        node = t.node;
-        start = t.start;
-        cnt = t.cnt;
+        const uint32_t start = it * kTQ;
+        return LkItem{0u, start, min(kTQ, n - start), 0u, (n + kTQ - 1) / kTQ, 0u};
+    }
+    return items[it];
+}
+
+// Epilogue of a (non-last) level step for one item: sj[r] holds the chosen
+// child of row r.  Writes code[start+r] = child | rank << 8 (rank = earlier
+// rows of the item with the same child) and the item's child counts into C.
+__device__ inline void itemCounts(uint32_t lane, uint32_t lg2b, uint32_t it, const LkItem& t, const uint8_t* sj,
+                                  uint32_t* sc, uint32_t* __restrict__ code, uint32_t* __restrict__ C) {
+    const uint32_t b = 1u << lg2b;
+    if (lane < b) sc[lane] = 0;
+    waveSync();
+    for (uint32_t g = 0; g * 64 < t.cnt; g++) {
+        const uint32_t r = g * 64 + lane;
+        const bool valid = r < t.cnt;
+        const uint32_t k = valid ? sj[r] : 0u;
+        uint64_t m = __ballot(valid);
+        for (uint32_t bit = 0; bit < lg2b; bit++) {
+            const uint64_t bb = __ballot(valid && ((k >> bit) & 1u));
+            m &= ((k >> bit) & 1u) ? bb : ~bb;
+        }
+        const uint32_t below = mbcnt(m);
+        const uint32_t base = valid ? sc[k] : 0u;
+        waveSync();
+        if (valid && below == 0) sc[k] = base + uint32_t(__builtin_popcountll(m));
+        if (valid) code[t.start + r] = k | ((base + below) << 8);
+        waveSync();
+    }
+    if (lane < b) C[uint64_t(t.itemBase) * b + lane * t.nItems + (it - t.itemBase)] = sc[lane];
+    waveSync();
+}
+
+// Moves every descriptor of an item to its place in the next level's order.
+template <bool kIdentity>
+__global__ __launch_bounds__(256) void k_scatter(const LkItem* __restrict__ items,
+                                                const uint32_t* __restrict__ nbItemsPtr, uint32_t nbItemsImplicit,
+                                                uint32_t n, uint32_t lg2b, const uint32_t* __restrict__ perm,
+                                                const uint32_t* __restrict__ code, const uint32_t* __restrict__ O,
+                                                uint32_t* __restrict__ permNext) {
+    const uint32_t lane = threadIdx.x & 63, b = 1u << lg2b;
+    const uint32_t nbItems = kIdentity ? nbItemsImplicit : *nbItemsPtr;
+    for (uint32_t it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nbItems; it += gridDim.x * 4) {
+        const LkItem t = itemAt<kIdentity>(it, n, items);
+        for (uint32_t r = lane; r < t.cnt; r += 64) {
+            const uint32_t p = t.start + r, c = code[p];
+            const uint32_t dst = O[uint64_t(t.itemBase) * b + (c & 0xFFu) * t.nItems + (it - t.itemBase)] + (c >> 8);
+            permNext[dst] = kIdentity ? p : perm[p];
+        }
+    }
+}
+
+// Items of the next level.  Child v = u*b + j of parent u occupies
+// O[base + j*nI] .. O[base + (j+1)*nI) of the scanned counts (base, nI: the
+// parent's items; level 0 has the single parent 0 with nItems0 items).
+__device__ inline void childRange(const uint32_t* O, const uint32_t* pBase, const uint32_t* pN, uint32_t nItems0,
+                                  uint32_t lg2b, uint32_t v, uint32_t& s0, uint32_t& c0) {
+    const uint32_t b = 1u << lg2b, u = v >> lg2b, j = v & (b - 1);
+    const uint32_t base = pBase ? pBase[u] : 0u, ni = pBase ? pN[u] : nItems0;
+    s0 = O[uint64_t(base) * b + j * ni];
+    c0 = O[uint64_t(base) * b + (j + 1) * ni] - s0;
+}
+
+__global__ void k_nodeItemCounts(const uint32_t* __restrict__ O, const uint32_t* __restrict__ pBase,
+                                 const uint32_t* __restrict__ pN, uint32_t nItems0, uint32_t nb, uint32_t lg2b,
+                                 uint32_t* __restrict__ cN) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= nb; v += gridDim.x * blockDim.x) {
+        if (v == nb) {
+            cN[v] = 0;  // scan tail: cBase[nb] = total items
+            continue;
+        }
+        uint32_t s0, c0;
+        childRange(O, pBase, pN, nItems0, lg2b, v, s0, c0);
+        cN[v] = (c0 + kTQ - 1) / kTQ;
+    }
+}
+
+__global__ void k_writeItems(const uint32_t* __restrict__ O, const uint32_t* __restrict__ pBase,
+                             const uint32_t* __restrict__ pN, uint32_t nItems0, uint32_t nb, uint32_t lg2b,
+                             const uint32_t* __restrict__ cBase, LkItem* __restrict__ items,
+                             uint32_t* __restrict__ nbItems) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= nb; v += gridDim.x * blockDim.x) {
+        if (v == nb) {
+            *nbItems = cBase[nb];
+            continue;
+        }
+        uint32_t s0, c0;
+        childRange(O, pBase, pN, nItems0, lg2b, v, s0, c0);
+        const uint32_t base = cBase[v], ni = (c0 + kTQ - 1) / kTQ;
+        for (uint32_t t = 0; t < ni; t++) items[base + t] = LkItem{v, s0 + t * kTQ, min(kTQ, c0 - t * kTQ), base, ni, 0u};
     }
 }
 
 // --------------------------------------------------- I8 MFMA level step --
-// Each wave takes one item: up to kTQ descriptors that sit at the same node.
 // mfma_i32_16x16x64_i8: lane l feeds A row (l&15) / B column (l&15) with the
 // 16 bytes of dims [16*(l>>4) + 64*kb, +16), kb = 0,1 (same placement for A
 // and B, so the sum runs over all 128 dims); the result lane l holds rows
@@ -274,22 +349,24 @@ __global__ __launch_bounds__(256) void k_levelI8(const uint8_t* __restrict__ row
                                                 uint64_t levelBase, const uint8_t* __restrict__ desc, uint32_t n,
                                                 const uint32_t* __restrict__ perm, const LkItem* __restrict__ items,
                                                 const uint32_t* __restrict__ nbItemsPtr, uint32_t nbItemsImplicit,
-                                                uint32_t* __restrict__ out, uint32_t* __restrict__ histNext) {
+                                                uint32_t* __restrict__ out, uint32_t* __restrict__ C) {
+    __shared__ uint8_t sjAll[4][kTQ];
+    __shared__ uint32_t scAll[4][32];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 15, grp = lane >> 4;
+    uint8_t* sj = sjAll[wave];
     const uint32_t nbItems = kIdentity ? nbItemsImplicit : *nbItemsPtr;
     for (uint32_t it = blockIdx.x * 4 + wave; it < nbItems; it += gridDim.x * 4) {
-        uint32_t node, start, cnt;
-        itemAt<kIdentity>(it, n, items, node, start, cnt);
-        const uint64_t child = (levelBase + node) * 16 + col;
+        const LkItem t = itemAt<kIdentity>(it, n, items);
+        const uint64_t child = (levelBase + t.node) * 16 + col;
         const uint8_t* crow = rows + child * kDims;
         const v4i b0 = toSigned(ld16(crow + grp * 16)), b1 = toSigned(ld16(crow + 64 + grp * 16));
         const int ns = normS[child];
-        for (uint32_t rb = 0; rb * 16 < cnt; rb++) {
+        for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
             const uint32_t r = rb * 16 + col;
             v4i a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
-            if (r < cnt) {
-                const uint32_t q = kIdentity ? start + r : perm[start + r];
+            if (r < t.cnt) {
+                const uint32_t q = kIdentity ? t.start + r : perm[t.start + r];
                 const uint8_t* x = desc + uint64_t(q) * kDims;
                 a0 = toSigned(ld16(x + grp * 16));
                 a1 = toSigned(ld16(x + 64 + grp * 16));
@@ -299,22 +376,21 @@ __global__ __launch_bounds__(256) void k_levelI8(const uint8_t* __restrict__ row
             acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b1, acc, 0, 0, 0);
             uint32_t key[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) key[i] = (uint32_t(ns - 2 * acc[i] + kScoreBias) << 4) | col;
-#pragma unroll
-            for (int m = 1; m < 16; m <<= 1)
-#pragma unroll
-                for (int i = 0; i < 4; i++) key[i] = min(key[i], uint32_t(__shfl_xor(int(key[i]), m, 16)));
+            for (int i = 0; i < 4; i++) key[i] = min16u((uint32_t(ns - 2 * acc[i] + kScoreBias) << 4) | col);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const uint32_t row = rb * 16 + 4 * grp + i;
-                if (col == uint32_t(i) && row < cnt) {
-                    const uint32_t q = kIdentity ? start + row : perm[start + row];
-                    const uint32_t nn = node * 16 + (key[i] & 15u);
-                    out[q] = nn;
-                    if (!kLast) atomicAdd(&histNext[nn], 1u);
+                if (col == uint32_t(i) && row < t.cnt) {
+                    if (kLast) {
+                        const uint32_t q = kIdentity ? t.start + row : perm[t.start + row];
+                        out[q] = t.node * 16 + (key[i] & 15u);
+                    } else {
+                        sj[row] = uint8_t(key[i] & 15u);
+                    }
                 }
             }
         }
+        if (!kLast) itemCounts(lane, 4, it, t, sj, scAll[wave], out, C);
     }
 }
 
@@ -322,22 +398,30 @@ __global__ __launch_bounds__(256) void k_levelI8(const uint8_t* __restrict__ row
 // mfma_f32_16x16x32_bf16: lane l feeds 8 dims [8*(l>>4) + 32*kb, +8), kb=0..3.
 template <bool kIdentity, bool kLast, int B>
 __global__ __launch_bounds__(256) void k_levelBf16(
-    const float* __restrict__ rows32, const uint16_t* __restrict__ hi, const uint16_t* __restrict__ lo,
-    const float* __restrict__ cn2, const float* __restrict__ e1, const float* __restrict__ e2, uint64_t levelBase,
+    const float* __restrict__ rows32, const uint8_t* __restrict__ mu, const uint16_t* __restrict__ hi,
+    const uint16_t* __restrict__ lo, const float* __restrict__ cn2, const float* __restrict__ e1,
+    const float* __restrict__ e2, uint64_t levelBase,
     const uint8_t* __restrict__ desc, uint32_t n, const uint32_t* __restrict__ perm, const LkItem* __restrict__ items,
     const uint32_t* __restrict__ nbItemsPtr, uint32_t nbItemsImplicit, uint32_t* __restrict__ out,
-    uint32_t* __restrict__ histNext, uint32_t* __restrict__ nearTies) {
+    uint32_t* __restrict__ C, uint32_t* __restrict__ nearTies) {
     constexpr int CB = B / 16;
+    constexpr uint32_t LG2B = B == 32 ? 5 : 4;
+    __shared__ uint8_t sjAll[4][kTQ];
+    __shared__ uint32_t scAll[4][32];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 15, grp = lane >> 4;
+    uint8_t* sj = sjAll[wave];
     const uint32_t nbItems = kIdentity ? nbItemsImplicit : *nbItemsPtr;
     uint32_t ties = 0;
     for (uint32_t it = blockIdx.x * 4 + wave; it < nbItems; it += gridDim.x * 4) {
-        uint32_t node, start, cnt;
-        itemAt<kIdentity>(it, n, items, node, start, cnt);
-        const uint64_t childBase = (levelBase + node) * B;
+        const LkItem t = itemAt<kIdentity>(it, n, items);
+        const uint64_t childBase = (levelBase + t.node) * B;
         v8bf bh[CB][4], bl[CB][4];
         float cn[CB], ea[CB], eb[CB];
+        uint2 muw[4];  // the node's centre on this lane's dims
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++)
+            muw[kb] = *reinterpret_cast<const uint2*>(mu + (levelBase + t.node) * kDims + kb * 32 + grp * 8);
 #pragma unroll
         for (int cb = 0; cb < CB; cb++) {
             const uint64_t child = childBase + cb * 16 + col;
@@ -350,19 +434,23 @@ __global__ __launch_bounds__(256) void k_levelBf16(
             ea[cb] = e1[child];
             eb[cb] = e2[child];
         }
-        for (uint32_t rb = 0; rb * 16 < cnt; rb++) {
+        for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
             const uint32_t r = rb * 16 + col;
             v8bf ax[4];
             uint32_t xsq = 0;
-            if (r < cnt) {
-                const uint32_t q = kIdentity ? start + r : perm[start + r];
+            if (r < t.cnt) {
+                const uint32_t q = kIdentity ? t.start + r : perm[t.start + r];
                 const uint8_t* x = desc + uint64_t(q) * kDims;
 #pragma unroll
                 for (int kb = 0; kb < 4; kb++) {
                     const uint2 w = *reinterpret_cast<const uint2*>(x + kb * 32 + grp * 8);
-                    ax[kb] = bytesToBf16(w);
-                    xsq = __builtin_amdgcn_udot4(w.x, w.x, xsq, false);
-                    xsq = __builtin_amdgcn_udot4(w.y, w.y, xsq, false);
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        const uint32_t xw = e < 4 ? w.x : w.y, mw = e < 4 ? muw[kb].x : muw[kb].y;
+                        const int dv = int((xw >> (8 * (e & 3))) & 0xFFu) - int((mw >> (8 * (e & 3))) & 0xFFu);
+                        ax[kb][e] = __builtin_bit_cast(__bf16, uint16_t(__float_as_uint(float(dv)) >> 16));  // exact
+                        xsq += uint32_t(dv * dv);
+                    }
                 }
             } else {
 #pragma unroll
@@ -380,56 +468,111 @@ __global__ __launch_bounds__(256) void k_levelBf16(
                     acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[kb], bl[cb][kb], acc[cb], 0, 0, 0);
                 }
             }
+            uint32_t cm[4];  // candidate children of rows 4*grp + i (same in all 16 lanes of the group)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const uint32_t rowLocal = 4 * grp + i;
-                const float xn2 = float(uint32_t(__shfl(int(xsq), int(rowLocal), 64)));
+                const float xn2 = float(uint32_t(__shfl(int(xsq), int(rowLocal), 64)));  // ||x - mu||^2
                 const float xn = sqrtf(xn2) * (1.0f + 1.0f / 1048576.0f);
-                float s[CB], E[CB];
+                float sc[CB], E[CB];
                 float U = INFINITY;
 #pragma unroll
                 for (int cb = 0; cb < CB; cb++) {
-                    s[cb] = cn[cb] - 2.0f * acc[cb][i];
-                    E[cb] = ea[cb] * xn + kRelXsq * xn2 + eb[cb];
-                    U = fminf(U, s[cb] + E[cb]);
+                    sc[cb] = cn[cb] - 2.0f * acc[cb][i];
+                    const float e1v = ea[cb] * xn + eb[cb];  // MFMA score vs exact distance
+                    // plus the plain chain's relative error on the distance itself
+                    E[cb] = (e1v + kGammaPlain * (fmaxf(xn2 + sc[cb], 0.0f) + e1v)) * (1.0f + 1.0f / 1024.0f);
+                    U = fminf(U, sc[cb] + E[cb]);
                 }
-#pragma unroll
-                for (int m = 1; m < 16; m <<= 1) U = fminf(U, __shfl_xor(U, m, 16));
-                uint32_t cmask = 0;
+                U = min16f(U);
+                uint32_t m = 0;
 #pragma unroll
                 for (int cb = 0; cb < CB; cb++) {
-                    const uint64_t bal = __ballot(s[cb] - E[cb] <= U);
-                    cmask |= uint32_t((bal >> (16 * grp)) & 0xFFFFu) << (16 * cb);
+                    const uint64_t bal = __ballot(sc[cb] - E[cb] <= U);
+                    m |= uint32_t((bal >> (16 * grp)) & 0xFFFFu) << (16 * cb);
                 }
-                const uint32_t row = rb * 16 + rowLocal;
-                if (col == uint32_t(i) && row < cnt) {
-                    const uint32_t q = kIdentity ? start + row : perm[start + row];
-                    uint32_t j = __builtin_ctz(cmask);
-                    if (__builtin_popcount(cmask) > 1) {
-                        // near tie: exact plain FP32 order over the candidates, ascending j
-                        ties++;
-                        const uint8_t* x = desc + uint64_t(q) * kDims;
-                        float best = INFINITY;
-                        for (uint32_t m = cmask; m; m &= m - 1) {
-                            const uint32_t c = __builtin_ctz(m);
-                            const float* cr = rows32 + (childBase + c) * kDims;
-                            float a = 0.0f;
-                            for (uint32_t k = 0; k < kDims; k++) {
-                                const float d = float(x[k]) - cr[k];
+                cm[i] = m;
+            }
+            // Near ties: the candidates of the 4 rows of a group are re-scored in
+            // parallel, lane 4*i + s taking candidate s (+4 per extra pass) of row i,
+            // each lane running the plain FP32 chain; a quad minimum picks the child.
+            uint32_t best[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) best[i] = __builtin_ctz(cm[i]);
+            const bool tieHere = __builtin_popcount(cm[0]) > 1 || __builtin_popcount(cm[1]) > 1 ||
+                                 __builtin_popcount(cm[2]) > 1 || __builtin_popcount(cm[3]) > 1;
+            if (__ballot(tieHere)) {
+                const uint32_t i = col >> 2, slot = col & 3;
+                const uint32_t mine = i == 0 ? cm[0] : i == 1 ? cm[1] : i == 2 ? cm[2] : cm[3];
+                const uint32_t row = rb * 16 + 4 * grp + i;
+                const bool rowTie = __builtin_popcount(mine) > 1 && row < t.cnt;
+                float bd = INFINITY;
+                uint32_t bj = 0xFFFFFFFFu;
+                const uint32_t npass = (32u + 3u) / 4u;
+                for (uint32_t pass = 0; pass < npass; pass++) {
+                    const uint32_t need = __ballot(rowTie && uint32_t(__builtin_popcount(mine)) > 4 * pass) != 0;
+                    if (!need) break;
+                    // candidate number slot + 4*pass of this row
+                    uint32_t mm = mine, k = slot + 4 * pass, c = 0xFFFFFFFFu;
+                    for (; mm; mm &= mm - 1, k--)
+                        if (k == 0) {
+                            c = __builtin_ctz(mm);
+                            break;
+                        }
+                    float a = INFINITY;
+                    if (rowTie && c != 0xFFFFFFFFu) {
+                        const uint32_t q = kIdentity ? t.start + row : perm[t.start + row];
+                        const uint4* x4 = reinterpret_cast<const uint4*>(desc + uint64_t(q) * kDims);
+                        const float4* cr = reinterpret_cast<const float4*>(rows32 + (childBase + c) * kDims);
+                        a = 0.0f;
+#pragma unroll 4
+                        for (uint32_t w4 = 0; w4 < 8; w4++) {  // 16 dims per step, k ascending
+                            const uint4 xv = x4[w4];
+                            const uint32_t xw[4] = {xv.x, xv.y, xv.z, xv.w};
+#pragma unroll
+                            for (int e = 0; e < 4; e++) {
+                                const float4 cv = cr[4 * w4 + e];
+                                float d = float(xw[e] & 0xFFu) - cv.x;
                                 a = fmaf(d, d, a);
-                            }
-                            if (a < best) {
-                                best = a;
-                                j = c;
+                                d = float((xw[e] >> 8) & 0xFFu) - cv.y;
+                                a = fmaf(d, d, a);
+                                d = float((xw[e] >> 16) & 0xFFu) - cv.z;
+                                a = fmaf(d, d, a);
+                                d = float(xw[e] >> 24) - cv.w;
+                                a = fmaf(d, d, a);
                             }
                         }
                     }
-                    const uint32_t nn = node * B + j;
-                    out[q] = nn;
-                    if (!kLast) atomicAdd(&histNext[nn], 1u);
+                    // keep (distance, child) minimum per quad; lowest child among equal distances
+                    const float qa = min4f(a);
+                    const uint32_t qj = min4u(a == qa && c != 0xFFFFFFFFu ? c : 0xFFFFFFFFu);
+                    if (qa < bd || (qa == bd && qj < bj)) {
+                        bd = qa;
+                        bj = qj;
+                    }
+                }
+                // lane 4*i of the group now holds row i's plain-order argmin
+#pragma unroll
+                for (int ii = 0; ii < 4; ii++) {
+                    const uint32_t v = uint32_t(__shfl(int(bj), int(16 * grp + 4 * ii), 64));
+                    if (__builtin_popcount(cm[ii]) > 1) best[ii] = v;
+                }
+                if (slot == 0 && rowTie) ties++;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t row = rb * 16 + 4 * grp + i;
+                if (col == uint32_t(i) && row < t.cnt) {
+                    if (kLast) {
+                        const uint32_t q = kIdentity ? t.start + row : perm[t.start + row];
+                        out[q] = t.node * B + best[i];
+                    } else {
+                        sj[row] = uint8_t(best[i]);
+                    }
                 }
             }
         }
+        if (!kLast) itemCounts(lane, LG2B, it, t, sj, scAll[wave], out, C);
     }
     if (nearTies && ties) atomicAdd(nearTies, ties);
 }
@@ -441,20 +584,18 @@ uint32_t gridFor(uint64_t work, uint32_t perBlock, uint32_t cap) {
 
 template <bool kIdentity, bool kLast>
 void launchLevel(const TreeView& t, uint64_t levelBase, const uint8_t* desc, uint32_t n, const uint32_t* perm,
-                 const LkItem* items, const uint32_t* nbItemsPtr, uint32_t itemsMax, uint32_t* out,
-                 uint32_t* histNext, uint32_t* nearTies, hipStream_t s) {
-    const uint32_t grid = gridFor(itemsMax, 4, 4096);
+                 const LkItem* items, const uint32_t* nbItemsPtr, uint32_t itemsMax, uint32_t* out, uint32_t* C,
+                 uint32_t* nearTies, hipStream_t s) {
+    const uint32_t grid = gridFor(itemsMax, 4, 8192);
     if (t.centroidType == 0) {
         k_levelI8<kIdentity, kLast><<<grid, 256, 0, s>>>(t.u8rows, t.normS, levelBase, desc, n, perm, items, nbItemsPtr,
-                                                       itemsMax, out, histNext);
+                                                       itemsMax, out, C);
     } else if (t.b == 32) {
-        k_levelBf16<kIdentity, kLast, 32><<<grid, 256, 0, s>>>(t.f32rows, t.hi, t.lo, t.cn2, t.e1, t.e2, levelBase, desc,
-                                                             n, perm, items, nbItemsPtr, itemsMax, out, histNext,
-                                                             nearTies);
+        k_levelBf16<kIdentity, kLast, 32><<<grid, 256, 0, s>>>(t.f32rows, t.mu, t.hi, t.lo, t.cn2, t.e1, t.e2, levelBase, desc,
+                                                             n, perm, items, nbItemsPtr, itemsMax, out, C, nearTies);
     } else {
-        k_levelBf16<kIdentity, kLast, 16><<<grid, 256, 0, s>>>(t.f32rows, t.hi, t.lo, t.cn2, t.e1, t.e2, levelBase, desc,
-                                                             n, perm, items, nbItemsPtr, itemsMax, out, histNext,
-                                                             nearTies);
+        k_levelBf16<kIdentity, kLast, 16><<<grid, 256, 0, s>>>(t.f32rows, t.mu, t.hi, t.lo, t.cn2, t.e1, t.e2, levelBase, desc,
+                                                             n, perm, items, nbItemsPtr, itemsMax, out, C, nearTies);
     }
     RBOW_HIP(hipGetLastError());
 }
@@ -473,10 +614,10 @@ void launchPrepU8Tree(const uint8_t* rows, uint64_t nbChildren, int32_t* normU, 
     RBOW_HIP(hipGetLastError());
 }
 
-void launchPrepF32Tree(const float* rows, uint64_t nbChildren, uint16_t* hi, uint16_t* lo, float* cn2, float* e1,
-                       float* e2, hipStream_t s) {
-    if (!nbChildren) return;
-    k_prepF32<<<gridFor(nbChildren, 256, 1u << 30), 256, 0, s>>>(rows, nbChildren, hi, lo, cn2, e1, e2);
+void launchPrepF32Tree(const float* rows, uint64_t nbNodes, uint32_t b, uint8_t* mu, uint16_t* hi, uint16_t* lo,
+                       float* cn2, float* e1, float* e2, hipStream_t s) {
+    if (!nbNodes) return;
+    k_prepF32<<<uint32_t(nbNodes), 128, 0, s>>>(rows, b, mu, hi, lo, cn2, e1, e2);
     RBOW_HIP(hipGetLastError());
 }
 
@@ -492,6 +633,11 @@ void launchLookupPlain(const TreeView& t, const uint8_t* desc, uint32_t n, uint3
     RBOW_HIP(hipGetLastError());
 }
 
+// Workspace of one bucketed lookup of at most kChunk descriptors.
+namespace {
+constexpr uint32_t kChunk = 1u << 26;  // descriptors per pass (item ids stay below 2^20)
+}
+
 void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
                       hipStream_t s) {
     if (!n) return;
@@ -499,42 +645,77 @@ void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32
         launchFillU32(leaf, 0u, n, s);
         return;
     }
-    const uint32_t b = t.b;
-    const uint64_t maxBins = levelSize(t.lg2b, t.L - 1);
-    const uint64_t itemsMax = (n + kTQ - 1) / kTQ + maxBins;
-    // stream-ordered workspace: cur[n], perm[n], hist[maxBins], cursor[maxBins], items, nbItems
-    void* ws = nullptr;
-    const size_t bytes = sizeof(uint32_t) * (2ull * n + 2 * maxBins + 1) + sizeof(LkItem) * itemsMax + 256;
-    if (t.L > 1) RBOW_HIP(hipMallocAsync(&ws, bytes, s));
-    uint32_t* cur = static_cast<uint32_t*>(ws);
-    uint32_t* perm = cur + n;
-    uint32_t* hist = perm + n;
-    uint32_t* cursor = hist + maxBins;
-    uint32_t* nbItems = cursor + maxBins;
-    LkItem* items = reinterpret_cast<LkItem*>((reinterpret_cast<uintptr_t>(nbItems + 1) + 255) & ~uintptr_t(255));
-
+    if (n > kChunk) {  // independent chunks
+        for (uint64_t o = 0; o < n; o += kChunk)
+            launchLookupMfma(t, desc + o * kDims, uint32_t(std::min<uint64_t>(kChunk, n - o)), leaf + o, nearTies, s);
+        return;
+    }
+    const uint32_t b = t.b, lg2b = t.lg2b;
     const uint32_t items0 = (n + kTQ - 1) / kTQ;
     if (t.L == 1) {
         launchLevel<true, true>(t, 0, desc, n, nullptr, nullptr, nullptr, items0, leaf, nullptr, nearTies, s);
         return;
     }
-    launchFillU32(hist, 0u, b, s);
-    launchLevel<true, false>(t, 0, desc, n, nullptr, nullptr, nullptr, items0, cur, hist, nearTies, s);
+    const uint64_t maxNodes = levelSize(lg2b, t.L - 1);
+    const uint64_t itemsMax = items0 + maxNodes;
+    const uint64_t cMax = itemsMax * b + 1;
+    size_t scanBytes = 0, scanBytes2 = 0;
+    RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, (uint32_t*)nullptr, (uint32_t*)nullptr, int(cMax), s));
+    RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes2, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                              int(maxNodes + 1), s));
+    scanBytes = std::max(scanBytes, scanBytes2);
+    // workspace: perm[2][n], code[n], C[cMax], O[cMax], node arrays [2][2][maxNodes+1], nbItems, items[2][itemsMax], scan temp
+    const size_t words = 3ull * n + 2 * cMax + 4 * (maxNodes + 1) + 1;
+    const size_t bytes = 4 * words + 256 + 2 * sizeof(LkItem) * itemsMax + 256 + scanBytes + 256;
+    void* ws = nullptr;
+    RBOW_HIP(hipMallocAsync(&ws, bytes, s));
+    uint32_t* perm[2] = {static_cast<uint32_t*>(ws), static_cast<uint32_t*>(ws) + n};
+    uint32_t* code = perm[1] + n;
+    uint32_t* C = code + n;
+    uint32_t* O = C + cMax;
+    uint32_t* nBase[2] = {O + cMax, O + cMax + (maxNodes + 1)};
+    uint32_t* nCnt[2] = {nBase[1] + (maxNodes + 1), nBase[1] + 2 * (maxNodes + 1)};
+    uint32_t* nbItems = nCnt[1] + (maxNodes + 1);
+    auto align = [](void* p) { return reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(p) + 255) & ~uintptr_t(255)); };
+    LkItem* items[2];
+    items[0] = static_cast<LkItem*>(align(nbItems + 1));
+    items[1] = items[0] + itemsMax;
+    void* temp = align(items[1] + itemsMax);
+
+    // level 0: all descriptors against the root
+    uint64_t cLen = uint64_t(items0) * b + 1;
+    launchFillU32(C, 0u, cLen, s);
+    launchLevel<true, false>(t, 0, desc, n, nullptr, nullptr, nullptr, items0, code, C, nearTies, s);
+    RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, C, O, int(cLen), s));
+    k_scatter<true><<<gridFor(items0, 4, 8192), 256, 0, s>>>(nullptr, nullptr, items0, n, lg2b, nullptr, code, O,
+                                                            perm[0]);
+    auto buildItems = [&](const uint32_t* pBase, const uint32_t* pN, uint32_t nItems0, uint32_t nbChildren, int dst) {
+        k_nodeItemCounts<<<gridFor(nbChildren + 1, 256, 4096), 256, 0, s>>>(O, pBase, pN, nItems0, nbChildren, lg2b,
+                                                                          nCnt[dst]);
+        RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, nCnt[dst], nBase[dst], int(nbChildren + 1), s));
+        k_writeItems<<<gridFor(nbChildren + 1, 256, 4096), 256, 0, s>>>(O, pBase, pN, nItems0, nbChildren, lg2b,
+                                                                      nBase[dst], items[dst], nbItems);
+        RBOW_HIP(hipGetLastError());
+    };
+    buildItems(nullptr, nullptr, items0, b, 0);
+    int cur = 0;
     for (uint32_t l = 1; l < t.L; l++) {
-        const uint32_t nb = uint32_t(levelSize(t.lg2b, l));
-        k_scanItems<<<1, 1024, 0, s>>>(hist, nb, cursor, items, nbItems);
-        RBOW_HIP(hipGetLastError());
-        k_scatter<<<gridFor(n, 256, 4096), 256, 0, s>>>(cur, n, cursor, perm);
-        RBOW_HIP(hipGetLastError());
-        const uint32_t itemsL = uint32_t((n + kTQ - 1) / kTQ + nb);
-        if (l + 1 < t.L) {
-            launchFillU32(hist, 0u, uint64_t(nb) * b, s);
-            launchLevel<false, false>(t, levelBeg(t.lg2b, l), desc, n, perm, items, nbItems, itemsL, cur, hist,
-                                      nearTies, s);
-        } else {
-            launchLevel<false, true>(t, levelBeg(t.lg2b, l), desc, n, perm, items, nbItems, itemsL, leaf, nullptr,
-                                     nearTies, s);
+        const uint64_t nodesL = levelSize(lg2b, l);
+        const uint32_t itemsL = uint32_t(items0 + nodesL);
+        if (l + 1 == t.L) {
+            launchLevel<false, true>(t, levelBeg(lg2b, l), desc, n, perm[cur], items[cur], nbItems, itemsL, leaf,
+                                     nullptr, nearTies, s);
+            break;
         }
+        cLen = uint64_t(itemsL) * b + 1;
+        launchFillU32(C, 0u, cLen, s);
+        launchLevel<false, false>(t, levelBeg(lg2b, l), desc, n, perm[cur], items[cur], nbItems, itemsL, code, C,
+                                  nearTies, s);
+        RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, C, O, int(cLen), s));
+        k_scatter<false><<<gridFor(itemsL, 4, 8192), 256, 0, s>>>(items[cur], nbItems, 0, n, lg2b, perm[cur], code, O,
+                                                                 perm[cur ^ 1]);
+        buildItems(nBase[cur], nCnt[cur], 0, uint32_t(nodesL * b), cur ^ 1);
+        cur ^= 1;
     }
     RBOW_HIP(hipFreeAsync(ws, s));
 }

@@ -1,0 +1,200 @@
+// server.cpp -- continuous-batching query server (see server.hpp).
+#include "server.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace rbow_amd {
+
+Server::Server(const Vocabulary& voc, const Database& db, const ServerOptions& o) : voc_(voc), db_(db), opt_(o) {
+    RBOW_REQUIRE(o.maxBatch >= 1 && o.maxBatch <= 4096, RBOW_ERR_INVALID_ARG, "max_batch must be in [1, 4096]");
+    RBOW_REQUIRE(o.nbStreams >= 1 && o.nbStreams <= 8, RBOW_ERR_INVALID_ARG, "nb_streams must be in [1, 8]");
+    RBOW_REQUIRE(o.K >= 1 && o.K <= Database::kMaxResults, RBOW_ERR_CAPACITY, "max_results must be in [1, 1024]");
+    RBOW_REQUIRE(o.lookupKernel == RBOW_LOOKUP_PLAIN || o.lookupKernel == RBOW_LOOKUP_MFMA ||
+                     o.lookupKernel == RBOW_LOOKUP_AUTO,
+                 RBOW_ERR_INVALID_ARG, "bad lookup kernel");
+    db_.prepareForQueries(nullptr);
+    RBOW_HIP(hipDeviceSynchronize());
+    slots_.resize(o.nbStreams);
+    for (auto& s : slots_) {
+        RBOW_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));  // default priority
+        RBOW_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    }
+    t0_ = Clock::now();
+    worker_ = std::thread([this] { loop(); });
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(m_);
+        stop_ = true;
+    }
+    cvIn_.notify_all();
+    if (worker_.joinable()) worker_.join();
+    for (auto& s : slots_) {
+        if (s.stream) (void)hipStreamSynchronize(s.stream);
+        if (s.done) (void)hipEventDestroy(s.done);
+        if (s.stream) (void)hipStreamDestroy(s.stream);
+        if (s.hDesc) (void)hipHostFree(s.hDesc);
+        if (s.hDocs) (void)hipHostFree(s.hDocs);
+        if (s.hScores) (void)hipHostFree(s.hScores);
+        if (s.hCounts) (void)hipHostFree(s.hCounts);
+    }
+}
+
+uint64_t Server::submit(const uint8_t* desc, uint32_t n) {
+    RBOW_REQUIRE(desc != nullptr || n == 0, RBOW_ERR_INVALID_ARG, "null descriptors");
+    RBOW_REQUIRE(n <= Database::kMaxQueryDesc, RBOW_ERR_CAPACITY, "query above 16384 descriptors");
+    Request r;
+    r.n = n;
+    r.desc.assign(desc, desc + uint64_t(n) * kDims);
+    r.bytes = db_.scratchBytes(n) + uint64_t(n) * (kDims + 4);
+    RBOW_REQUIRE(r.bytes <= opt_.memBudget, RBOW_ERR_CAPACITY, "query scratch exceeds the server's memory budget");
+    r.arriveMs = nowMs();
+    uint64_t t;
+    {
+        std::lock_guard<std::mutex> lk(m_);
+        RBOW_REQUIRE(error_.empty(), RBOW_ERR_HIP, "server stopped: " + error_);
+        t = r.ticket = nextTicket_++;
+        pending_.push_back(std::move(r));
+    }
+    cvIn_.notify_one();
+    return t;
+}
+
+Server::Result Server::wait(uint64_t ticket) {
+    std::unique_lock<std::mutex> lk(m_);
+    cvOut_.wait(lk, [&] { return done_.count(ticket) || !error_.empty(); });
+    auto it = done_.find(ticket);
+    RBOW_REQUIRE(it != done_.end(), RBOW_ERR_HIP, "server stopped: " + error_);
+    Result r = std::move(it->second);
+    done_.erase(it);
+    return r;
+}
+
+void Server::reap(Slot& s) {
+    const double doneMs = nowMs();
+    const uint32_t B = uint32_t(s.reqs.size()), K = opt_.K;
+    {
+        std::lock_guard<std::mutex> lk(m_);
+        for (uint32_t q = 0; q < B; q++) {
+            Result r;
+            const uint32_t c = s.hCounts[q];
+            r.docs.assign(s.hDocs + uint64_t(q) * K, s.hDocs + uint64_t(q) * K + c);
+            r.scores.assign(s.hScores + uint64_t(q) * K, s.hScores + uint64_t(q) * K + c);
+            r.arriveMs = s.reqs[q].arriveMs;
+            r.startMs = s.startMs;
+            r.doneMs = doneMs;
+            r.batchSize = B;
+            done_[s.reqs[q].ticket] = std::move(r);
+        }
+        inFlightBytes_ -= s.bytes;
+    }
+    s.reqs.clear();
+    s.busy = false;
+    cvOut_.notify_all();
+}
+
+void Server::launch(Slot& s, std::vector<Request>&& batch) {
+    const uint32_t B = uint32_t(batch.size()), K = opt_.K;
+    s.hOff.assign(B + 1, 0);
+    for (uint32_t q = 0; q < B; q++) s.hOff[q + 1] = s.hOff[q] + batch[q].n;
+    const uint64_t D = s.hOff[B];
+    auto growPinned = [](auto*& p, size_t& cap, size_t need) {
+        if (need <= cap && p) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(need, 1) * sizeof(*p)));
+        cap = need;
+    };
+    growPinned(s.hDesc, s.hDescCap, D * kDims);
+    if (uint64_t(B) * K > s.hOutCap || !s.hDocs) {
+        size_t c1 = 0, c2 = 0, c3 = 0;
+        growPinned(s.hDocs, c1, uint64_t(B) * K);
+        growPinned(s.hScores, c2, uint64_t(B) * K);
+        growPinned(s.hCounts, c3, B);
+        s.hOutCap = uint64_t(B) * K;
+        // counts buffer sized with the results: capacity in queries = hOutCap / K
+    }
+    for (uint32_t q = 0; q < B; q++)
+        if (batch[q].n) std::memcpy(s.hDesc + s.hOff[q] * kDims, batch[q].desc.data(), uint64_t(batch[q].n) * kDims);
+    hipStream_t st = s.stream;
+    void *dDesc = nullptr, *dLeaf = nullptr, *dOff = nullptr, *dDocs = nullptr, *dScores = nullptr, *dCounts = nullptr;
+    RBOW_HIP(hipMallocAsync(&dDesc, std::max<uint64_t>(D, 1) * kDims, st));
+    RBOW_HIP(hipMallocAsync(&dLeaf, std::max<uint64_t>(D, 1) * 4, st));
+    RBOW_HIP(hipMallocAsync(&dOff, 8ull * (B + 1), st));
+    RBOW_HIP(hipMallocAsync(&dDocs, 4ull * B * K, st));
+    RBOW_HIP(hipMallocAsync(&dScores, 4ull * B * K, st));
+    RBOW_HIP(hipMallocAsync(&dCounts, 4ull * B, st));
+    if (D) RBOW_HIP(hipMemcpyAsync(dDesc, s.hDesc, D * kDims, hipMemcpyHostToDevice, st));
+    RBOW_HIP(hipMemcpyAsync(dOff, s.hOff.data(), 8ull * (B + 1), hipMemcpyHostToDevice, st));
+    const TreeView t = voc_.view();
+    if (D) {
+        if (opt_.lookupKernel == RBOW_LOOKUP_PLAIN)
+            launchLookupPlain(t, static_cast<uint8_t*>(dDesc), uint32_t(D), static_cast<uint32_t*>(dLeaf), st);
+        else
+            launchLookupMfma(t, static_cast<uint8_t*>(dDesc), uint32_t(D), static_cast<uint32_t*>(dLeaf), nullptr, st);
+    }
+    db_.scoreAsync(static_cast<uint32_t*>(dLeaf), static_cast<uint64_t*>(dOff), s.hOff.data(), B, K,
+                   static_cast<uint32_t*>(dDocs), static_cast<float*>(dScores), static_cast<uint32_t*>(dCounts), st);
+    RBOW_HIP(hipMemcpyAsync(s.hDocs, dDocs, 4ull * B * K, hipMemcpyDeviceToHost, st));
+    RBOW_HIP(hipMemcpyAsync(s.hScores, dScores, 4ull * B * K, hipMemcpyDeviceToHost, st));
+    RBOW_HIP(hipMemcpyAsync(s.hCounts, dCounts, 4ull * B, hipMemcpyDeviceToHost, st));
+    for (void* p : {dDesc, dLeaf, dOff, dDocs, dScores, dCounts}) RBOW_HIP(hipFreeAsync(p, st));
+    RBOW_HIP(hipEventRecord(s.done, st));
+    s.reqs = std::move(batch);
+    s.startMs = nowMs();
+    s.busy = true;
+}
+
+void Server::loop() {
+    try {
+        for (;;) {
+            for (auto& s : slots_) {
+                if (!s.busy) continue;
+                const hipError_t e = hipEventQuery(s.done);
+                if (e == hipSuccess) reap(s);
+                else if (e != hipErrorNotReady) RBOW_HIP(e);
+            }
+            Slot* free = nullptr;
+            bool anyBusy = false;
+            for (auto& s : slots_) {
+                if (!s.busy && !free) free = &s;
+                anyBusy |= s.busy;
+            }
+            std::vector<Request> batch;
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                if (stop_ && pending_.empty() && !anyBusy) return;
+                if (free) {
+                    // admission: batch slots and the HBM scratch budget
+                    while (!pending_.empty() && batch.size() < opt_.maxBatch &&
+                           inFlightBytes_ + pending_.front().bytes <= opt_.memBudget) {
+                        inFlightBytes_ += pending_.front().bytes;
+                        free->bytes += 0;
+                        batch.push_back(std::move(pending_.front()));
+                        pending_.pop_front();
+                    }
+                }
+                if (batch.empty() && !anyBusy) {
+                    cvIn_.wait(lk, [&] { return stop_ || !pending_.empty(); });
+                    continue;
+                }
+            }
+            if (!batch.empty()) {
+                uint64_t bytes = 0;
+                for (const auto& r : batch) bytes += r.bytes;
+                launch(*free, std::move(batch));
+                free->bytes = bytes;
+            } else {
+                std::this_thread::sleep_for(std::chrono::microseconds(20));  // batches in flight: poll
+            }
+        }
+    } catch (const std::exception& ex) {
+        std::lock_guard<std::mutex> lk(m_);
+        error_ = ex.what();
+        cvOut_.notify_all();
+    }
+}
+
+}  // namespace rbow_amd

@@ -1,0 +1,95 @@
+// server.hpp -- continuous-batching query server over a resident vocabulary
+// and inverted file.  No counterpart in the reference, which answers one host
+// query at a time (DataBase.cpp:71-190).
+//
+// Requests (descriptors on the host) are admitted into a batch while batch
+// slots and the HBM scratch budget allow; a worker thread launches each
+// batch -- H2D, lookup, scoring, top-K, D2H -- on one of several HIP streams,
+// so batches formed from later arrivals run while earlier ones are still on
+// the device.  Every request's result is independent of what it was batched
+// with (per-query accumulators), which the tests check.
+#pragma once
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "database.hpp"
+
+namespace rbow_amd {
+
+struct ServerOptions {
+    uint32_t maxBatch = 64;        // queries per batch
+    uint32_t nbStreams = 2;        // batches in flight
+    uint64_t memBudget = 4ull << 30;  // HBM scratch of in-flight queries (bytes)
+    uint32_t K = 50;
+    int lookupKernel = 2;          // RBOW_LOOKUP_MFMA / RBOW_LOOKUP_PLAIN
+};
+
+class Server {
+public:
+    using Clock = std::chrono::steady_clock;
+    struct Result {
+        std::vector<uint32_t> docs;
+        std::vector<float> scores;
+        double arriveMs = 0, startMs = 0, doneMs = 0;  // since server creation
+        uint32_t batchSize = 0;
+    };
+
+    Server(const Vocabulary& voc, const Database& db, const ServerOptions& o);
+    ~Server();
+    // copies the descriptors; returns the ticket
+    uint64_t submit(const uint8_t* desc, uint32_t n);
+    // blocks until the ticket's result is ready, then removes it
+    Result wait(uint64_t ticket);
+    double nowMs() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0_).count(); }
+    const ServerOptions& options() const { return opt_; }
+
+private:
+    struct Request {
+        uint64_t ticket;
+        std::vector<uint8_t> desc;
+        uint32_t n;
+        double arriveMs;
+        uint64_t bytes;
+    };
+    struct Slot {  // one in-flight batch
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;
+        bool busy = false;
+        std::vector<Request> reqs;
+        std::vector<uint64_t> hOff;
+        uint8_t* hDesc = nullptr;  // pinned staging
+        size_t hDescCap = 0;
+        uint32_t* hDocs = nullptr;
+        float* hScores = nullptr;
+        uint32_t* hCounts = nullptr;
+        size_t hOutCap = 0;
+        uint64_t bytes = 0;
+        double startMs = 0;
+    };
+    void loop();
+    void launch(Slot& s, std::vector<Request>&& batch);
+    void reap(Slot& s);
+
+    const Vocabulary& voc_;
+    const Database& db_;
+    ServerOptions opt_;
+    Clock::time_point t0_;
+    std::vector<Slot> slots_;
+    std::mutex m_;
+    std::condition_variable cvIn_, cvOut_;
+    std::deque<Request> pending_;
+    std::unordered_map<uint64_t, Result> done_;
+    uint64_t nextTicket_ = 1;
+    uint64_t inFlightBytes_ = 0;
+    bool stop_ = false;
+    std::string error_;
+    std::thread worker_;
+};
+
+}  // namespace rbow_amd

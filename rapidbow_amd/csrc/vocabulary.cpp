@@ -30,6 +30,7 @@ void Vocabulary::allocate(uint32_t b_, uint32_t L_, uint32_t ct) {
         normS.resize(nc);
     } else {
         f32rows.resize(nc * kDims);
+        mu.resize(std::max<uint64_t>(nbNodes, 1) * kDims);
         hi.resize(nc * kDims);
         lo.resize(nc * kDims);
         cn2.resize(nc);
@@ -42,7 +43,7 @@ void Vocabulary::allocate(uint32_t b_, uint32_t L_, uint32_t ct) {
 
 void Vocabulary::prepare(hipStream_t s) {
     if (centroidType == 0) launchPrepU8Tree(u8rows.get(), nbChildren(), normU.get(), normS.get(), s);
-    else launchPrepF32Tree(f32rows.get(), nbChildren(), hi.get(), lo.get(), cn2.get(), e1.get(), e2.get(), s);
+    else launchPrepF32Tree(f32rows.get(), nbNodes, b, mu.get(), hi.get(), lo.get(), cn2.get(), e1.get(), e2.get(), s);
 }
 
 TreeView Vocabulary::view() const {
@@ -55,6 +56,7 @@ TreeView Vocabulary::view() const {
     t.normU = normU.get();
     t.normS = normS.get();
     t.f32rows = f32rows.get();
+    t.mu = mu.get();
     t.hi = hi.get();
     t.lo = lo.get();
     t.cn2 = cn2.get();
@@ -141,6 +143,13 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     DevBuf<uint32_t> dTaskBeg, dTaskCnt, dCounts, dChanged, dActive, dPasses, dNbActive(1), dSeed, dMasks;
     DevBuf<long long> dSums;
     DevBuf<KmTile> dTiles;
+    DevBuf<KmChunk> dChunks, dRed;
+    DevBuf<int8_t> dXA, dXT;
+    DevBuf<int32_t> dPartSums;
+    DevBuf<uint32_t> dPartCnt, dPartChanged, dTaskChunkBeg, dTaskChunkCnt;
+    std::vector<KmChunk> chunks, red;
+    std::vector<uint32_t> taskChunkBeg, taskChunkCnt;
+    const bool mfmaPath = ct == 0 && kernel == 0;
     std::vector<uint8_t> hAssign(nbDesc);
     std::vector<uint32_t> hPasses, hMasks;
     std::vector<KmTile> tiles;
@@ -173,7 +182,42 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         h2d(dTaskBeg.get(), taskBeg.data(), nG, s);
         h2d(dTaskCnt.get(), taskCnt.data(), nG, s);
         h2d(dTiles.get(), tiles.data(), tiles.size(), s);
-        launchGatherRows(devDesc, dOrder.get(), nbDesc, dX.get(), s);
+        if (mfmaPath) {
+            // chunks of <= kChunkTiles tiles of one task (one wave each); reduce items of <= 32 chunks
+            chunks.clear();
+            red.clear();
+            taskChunkBeg.assign(nG, 0u);
+            taskChunkCnt.assign(nG, 0u);
+            uint32_t ti = 0;
+            for (uint32_t g = 0; g < nG; g++) {
+                const uint32_t nt = divUp(taskCnt[g], 64);
+                taskChunkBeg[g] = uint32_t(chunks.size());
+                for (uint32_t c = 0; c < nt; c += kChunkTiles)
+                    chunks.push_back(KmChunk{g, ti + c, ti + std::min(nt, c + kChunkTiles)});
+                taskChunkCnt[g] = uint32_t(chunks.size()) - taskChunkBeg[g];
+                if (taskChunkCnt[g] > 32)  // small tasks are reduced inside the update kernel
+                    for (uint32_t c = 0; c < taskChunkCnt[g]; c += 32)
+                        red.push_back(
+                            KmChunk{g, taskChunkBeg[g] + c, taskChunkBeg[g] + std::min(taskChunkCnt[g], c + 32)});
+                ti += nt;
+            }
+            dChunks.reserve(std::max<size_t>(chunks.size(), 1));
+            dRed.reserve(std::max<size_t>(red.size(), 1));
+            dTaskChunkBeg.reserve(nG);
+            dTaskChunkCnt.reserve(nG);
+            dXA.reserve(std::max<size_t>(tiles.size(), 1) * 8192);
+            dXT.reserve(std::max<size_t>(tiles.size(), 1) * 8192);
+            dPartSums.reserve(std::max<size_t>(chunks.size(), 1) * 16 * kDims);
+            dPartCnt.reserve(std::max<size_t>(chunks.size(), 1) * 16);
+            dPartChanged.reserve(std::max<size_t>(chunks.size(), 1));
+            h2d(dChunks.get(), chunks.data(), chunks.size(), s);
+            h2d(dRed.get(), red.data(), red.size(), s);
+            h2d(dTaskChunkBeg.get(), taskChunkBeg.data(), nG, s);
+            h2d(dTaskChunkCnt.get(), taskChunkCnt.data(), nG, s);
+            launchGatherTiles(devDesc, dOrder.get(), dTiles.get(), uint32_t(tiles.size()), dXA.get(), dXT.get(), s);
+        } else {
+            launchGatherRows(devDesc, dOrder.get(), nbDesc, dX.get(), s);
+        }
 
         KmLevel k;
         k.b = b;
@@ -193,6 +237,17 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         k.active = dActive.get();
         k.passes = dPasses.get();
         k.nbActive = dNbActive.get();
+        if (mfmaPath) {
+            k.XA = dXA.get();
+            k.XT = dXT.get();
+            k.chunks = dChunks.get();
+            k.nbChunks = uint32_t(chunks.size());
+            k.taskChunkBeg = dTaskChunkBeg.get();
+            k.taskChunkCnt = dTaskChunkCnt.get();
+            k.partSums = dPartSums.get();
+            k.partCnt = dPartCnt.get();
+            k.partChanged = dPartChanged.get();
+        }
 
         // ---- seeds (kmeans.cpp:170-196 reference picks, or k-means++) ----
         if (opts.seed_mode == 0) {
@@ -234,6 +289,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         for (uint32_t it = 0; it < opts.max_iters; it++) {
             if (stats) mark(4 * it);
             launchKmAssign(k, kernel, s);
+            if (mfmaPath) launchKmReduce(k, dRed.get(), uint32_t(red.size()), s);
             if (stats) mark(4 * it + 1);
             if (poll) launchFillU32(dNbActive.get(), 0u, 1, s);
             if (stats) mark(4 * it + 2);

@@ -12,6 +12,7 @@ struct Vocabulary {
     DevBuf<uint8_t> u8rows;  // [node][b][128]
     DevBuf<int32_t> normU, normS;
     DevBuf<float> f32rows;   // [node][b][128]
+    DevBuf<uint8_t> mu;      // [node][128] centre of the node's children (float trees)
     DevBuf<uint16_t> hi, lo;
     DevBuf<float> cn2, e1, e2;
     std::vector<uint32_t> masks;   // valid-center masks, low bits (kmeans.cpp:189)
