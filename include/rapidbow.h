@@ -152,6 +152,40 @@ rbow_status rbow_db_postings(const rbow_database* db, uint32_t leaf, uint32_t* h
 /* float(1 / L1 norm) of a document (DataBase.cpp:68). */
 rbow_status rbow_db_doc_rcp(const rbow_database* db, uint32_t doc, float* rcp);
 
+/* ------------------------------------------------------------- server --- */
+/* No counterpart in the reference (one host query at a time): a continuous-
+ * batching query server over a resident vocabulary + database.  Requests are
+ * admitted into a batch while batch slots and the HBM scratch budget allow;
+ * a worker thread runs each batch (H2D, lookup, scoring, top-K, D2H) on one of
+ * nb_streams HIP streams, so batches formed from later arrivals run while
+ * earlier ones are on the device.  Results do not depend on batching.  The
+ * database must not receive inserts while a server exists. */
+typedef struct rbow_server rbow_server;
+typedef struct {
+    uint32_t max_batch;    /* queries per batch (default 64) */
+    uint32_t nb_streams;   /* batches in flight (default 2) */
+    uint64_t mem_budget;   /* HBM scratch of admitted queries, bytes (default 4 GiB) */
+    uint32_t max_results;  /* K (default 50, <= 1024) */
+    int32_t lookup_kernel; /* RBOW_LOOKUP_MFMA (default) or RBOW_LOOKUP_PLAIN */
+} rbow_server_options;
+void rbow_server_options_default(rbow_server_options* opts);
+rbow_status rbow_server_create(const rbow_vocabulary* voc, const rbow_database* db, const rbow_server_options* opts,
+                               rbow_server** out);
+void rbow_server_destroy(rbow_server* srv);
+/* Host descriptors uint8[n][128] (copied); returns a ticket. */
+rbow_status rbow_server_submit(rbow_server* srv, const uint8_t* host_desc, uint32_t n, uint64_t* ticket);
+/* Blocks until the ticket completes: ranked docs / scores (K each), count,
+ * latency from submit to completion. */
+rbow_status rbow_server_wait(rbow_server* srv, uint64_t ticket, uint32_t* docs, float* scores, uint32_t* count,
+                             double* latency_ms);
+/* Load generator: submits query i (host_desc rows offsets[i]..offsets[i+1])
+ * at Poisson arrival times of rate qps (seeded), then collects every result:
+ * latency_ms[i], docs[i][K] and counts[i] (optional), batch_sizes[i]
+ * (optional), elapsed_s = first arrival to last completion. */
+rbow_status rbow_server_replay(rbow_server* srv, const uint8_t* host_desc, const uint64_t* offsets,
+                               uint32_t nb_queries, double qps, uint64_t seed, double* latency_ms, uint32_t* docs,
+                               uint32_t* counts, uint32_t* batch_sizes, double* elapsed_s);
+
 /* ------------------------------------------------ kernel-level entries -- */
 /* These expose the batched k-means kernels the build drives, one pass at a
  * time, for callers that run their own Lloyd loop (and for parity tests).

@@ -39,6 +39,11 @@ class BuildStats(ctypes.Structure):
                 ("point_passes", ctypes.c_uint64), ("assign_ops", ctypes.c_uint64)]
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("max_batch", ctypes.c_uint32), ("nb_streams", ctypes.c_uint32), ("mem_budget", ctypes.c_uint64),
+                ("max_results", ctypes.c_uint32), ("lookup_kernel", ctypes.c_int32)]
+
+
 class BuildOptions(ctypes.Structure):
     _fields_ = [("max_iters", ctypes.c_uint32), ("seed_mode", ctypes.c_uint32), ("seed", ctypes.c_uint64),
                 ("centroid_type", ctypes.c_uint32), ("kernel", ctypes.c_uint32)]
@@ -73,6 +78,24 @@ _sig("rbow_kmeans_assign", _i32, [_vp, _vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp
 _sig("rbow_kmeans_update", _i32, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp])
 _sig("rbow_reformat_tiles", _i32, [_vp, _vp, _u32, _u32, _vp, _vp])
 _sig("rbow_gather_rows", _i32, [_vp, _vp, _u64, _vp, _vp])
+_sig("rbow_db_create", _i32, [_vp, ctypes.POINTER(_vp)])
+_sig("rbow_db_destroy", None, [_vp])
+_sig("rbow_db_add_doc", _i32, [_vp, _u32, _vp, _u32])
+_sig("rbow_db_query", _i32, [_vp, _vp, _u32, _u32, _vp, _vp, ctypes.POINTER(_u32)])
+_sig("rbow_db_query_and_add_doc", _i32, [_vp, _u32, _vp, _u32, _u32, _vp, ctypes.POINTER(_u32)])
+_sig("rbow_db_add_docs_device", _i32, [_vp, _vp, _u32, _vp, _vp, _vp])
+_sig("rbow_db_query_batch", _i32, [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp])
+_sig("rbow_db_nb_docs", _u32, [_vp])
+_sig("rbow_db_nb_postings", _u64, [_vp])
+_sig("rbow_db_postings", _i32, [_vp, _u32, _vp, _u32, ctypes.POINTER(_u32)])
+_sig("rbow_db_doc_rcp", _i32, [_vp, _u32, ctypes.POINTER(ctypes.c_float)])
+_sig("rbow_server_options_default", None, [ctypes.POINTER(ServerOptions)])
+_sig("rbow_server_create", _i32, [_vp, _vp, ctypes.POINTER(ServerOptions), ctypes.POINTER(_vp)])
+_sig("rbow_server_destroy", None, [_vp])
+_sig("rbow_server_submit", _i32, [_vp, _vp, _u32, ctypes.POINTER(_u64)])
+_sig("rbow_server_wait", _i32, [_vp, _u64, _vp, _vp, ctypes.POINTER(_u32), ctypes.POINTER(ctypes.c_double)])
+_sig("rbow_server_replay", _i32, [_vp, _vp, _vp, _u32, ctypes.c_double, _u64, _vp, _vp, _vp, _vp,
+                                  ctypes.POINTER(ctypes.c_double)])
 _sig("rbow_synth_centres", None, [_u64, _u32, _vp])
 _sig("rbow_synth_descriptors_device", _i32, [_vp, _vp, _u32, _u64, _u64, _u64, _vp, _vp, _vp])
 _sig("rbow_synth_image_centres_device", _i32, [_vp, _u32, _u64, _u64, _u64, _u32, _u32, _vp, _u64, _vp])
@@ -282,3 +305,101 @@ class DeviceSynth:
                                                    self.nb_centres, subset, _ptr(dev_cloud), cloud_size, out.ptr,
                                                    nb_postings, _stream(stream)))
         return out
+
+
+class DataBase:
+    """IDataBase (RapidBoW.h:92-105) on the GPU; borrows `voc`."""
+
+    def __init__(self, voc):
+        h = _vp()
+        check(lib.rbow_db_create(voc.handle, ctypes.byref(h)))
+        self.handle, self.voc = h.value, voc
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.rbow_db_destroy(self.handle)
+            self.handle = None
+
+    def add_doc(self, doc, leaf):
+        leaf = np.ascontiguousarray(leaf, np.uint32)
+        check(lib.rbow_db_add_doc(self.handle, doc, leaf.ctypes.data if leaf.size else None, leaf.size))
+
+    def query(self, leaf, K):
+        leaf = np.ascontiguousarray(leaf, np.uint32)
+        docs = np.zeros(max(K, 1), np.uint32)
+        sc = np.zeros(max(K, 1), np.float32)
+        n = ctypes.c_uint32()
+        check(lib.rbow_db_query(self.handle, leaf.ctypes.data if leaf.size else None, leaf.size, K, docs.ctypes.data,
+                                sc.ctypes.data, ctypes.byref(n)))
+        return docs[:n.value], sc[:n.value]
+
+    def add_docs_device(self, doc_ids, dev_leaf, offsets, stream=None):
+        ids = np.ascontiguousarray(doc_ids, np.uint32)
+        off = np.ascontiguousarray(offsets, np.uint64)
+        check(lib.rbow_db_add_docs_device(self.handle, ids.ctypes.data, ids.size, _ptr(dev_leaf), off.ctypes.data,
+                                          _stream(stream)))
+
+    def query_batch(self, dev_leaf, offsets, K, dev_docs, dev_scores, dev_counts, stream=None):
+        off = np.ascontiguousarray(offsets, np.uint64)
+        check(lib.rbow_db_query_batch(self.handle, _ptr(dev_leaf), off.ctypes.data, off.size - 1, K, _ptr(dev_docs),
+                                      _ptr(dev_scores), _ptr(dev_counts), _stream(stream)))
+
+    @property
+    def nb_docs(self):
+        return lib.rbow_db_nb_docs(self.handle)
+
+    @property
+    def nb_postings(self):
+        return lib.rbow_db_nb_postings(self.handle)
+
+
+class Server:
+    """Continuous-batching query server (no counterpart in the reference)."""
+
+    def __init__(self, voc, db, max_batch=64, nb_streams=2, mem_budget=4 << 30, max_results=50,
+                 lookup_kernel=LOOKUP_MFMA):
+        o = ServerOptions()
+        lib.rbow_server_options_default(ctypes.byref(o))
+        o.max_batch, o.nb_streams, o.mem_budget, o.max_results, o.lookup_kernel = (
+            max_batch, nb_streams, mem_budget, max_results, lookup_kernel)
+        h = _vp()
+        check(lib.rbow_server_create(voc.handle, db.handle, ctypes.byref(o), ctypes.byref(h)))
+        self.handle, self.K, self._keep = h.value, max_results, (voc, db)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.rbow_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, desc):
+        desc = np.ascontiguousarray(desc, np.uint8)
+        t = ctypes.c_uint64()
+        check(lib.rbow_server_submit(self.handle, desc.ctypes.data if desc.size else None, desc.shape[0],
+                                     ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        docs = np.zeros(self.K, np.uint32)
+        sc = np.zeros(self.K, np.float32)
+        n = ctypes.c_uint32()
+        lat = ctypes.c_double()
+        check(lib.rbow_server_wait(self.handle, ticket, docs.ctypes.data, sc.ctypes.data, ctypes.byref(n),
+                                   ctypes.byref(lat)))
+        return docs[:n.value], sc[:n.value], lat.value
+
+    def replay(self, desc, offsets, qps, seed=4):
+        desc = np.ascontiguousarray(desc, np.uint8)
+        off = np.ascontiguousarray(offsets, np.uint64)
+        nq = off.size - 1
+        lat = np.zeros(nq, np.float64)
+        docs = np.zeros((nq, self.K), np.uint32)
+        cnt = np.zeros(nq, np.uint32)
+        bs = np.zeros(nq, np.uint32)
+        el = ctypes.c_double()
+        check(lib.rbow_server_replay(self.handle, desc.ctypes.data, off.ctypes.data, nq, qps, seed,
+                                     lat.ctypes.data, docs.ctypes.data, cnt.ctypes.data, bs.ctypes.data,
+                                     ctypes.byref(el)))
+        return dict(latency_ms=lat, docs=docs, counts=cnt, batch_sizes=bs, elapsed_s=el.value)

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librapidbow_amd.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["lookup.hip", "kmeans.hip", "database.hip", "vocabulary.cpp", "database.cpp", "capi.cpp", "synth.cpp", "synth.hip"]
+SOURCES = ["lookup.hip", "kmeans.hip", "database.hip", "vocabulary.cpp", "database.cpp", "capi.cpp", "server.cpp", "synth.cpp", "synth.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-Wall", "-Wno-unused-result", f"-I{os.path.join(ROOT, 'include')}"]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

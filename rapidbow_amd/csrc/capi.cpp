@@ -6,7 +6,12 @@
 #include <string>
 #include <vector>
 
+#include <chrono>
+#include <random>
+#include <thread>
+
 #include "database.hpp"
+#include "server.hpp"
 #include "vocabulary.hpp"
 
 using namespace rbow_amd;
@@ -18,6 +23,10 @@ struct rbow_vocabulary {
 
 struct rbow_database {
     std::unique_ptr<Database> db;
+};
+
+struct rbow_server {
+    std::unique_ptr<Server> srv;
 };
 
 namespace {
@@ -423,6 +432,102 @@ rbow_status rbow_db_doc_rcp(const rbow_database* db, uint32_t doc, float* rcp) {
     return guarded([&] {
         RBOW_REQUIRE(db && rcp, RBOW_ERR_INVALID_ARG, "null argument");
         RBOW_REQUIRE(db->db->docRcp(doc, rcp), RBOW_ERR_INVALID_ARG, "unknown document");
+    });
+}
+
+// ------------------------------------------------------------- server --
+
+void rbow_server_options_default(rbow_server_options* o) {
+    if (!o) return;
+    o->max_batch = 64;
+    o->nb_streams = 2;
+    o->mem_budget = 4ull << 30;
+    o->max_results = 50;
+    o->lookup_kernel = RBOW_LOOKUP_MFMA;
+}
+
+rbow_status rbow_server_create(const rbow_vocabulary* voc, const rbow_database* db, const rbow_server_options* opts,
+                               rbow_server** out) {
+    return guarded([&] {
+        RBOW_REQUIRE(voc && db && out, RBOW_ERR_INVALID_ARG, "null argument");
+        RBOW_REQUIRE(&db->db->vocabulary() == voc->v.get(), RBOW_ERR_INVALID_ARG,
+                     "the database was not created on this vocabulary");
+        *out = nullptr;
+        rbow_server_options o;
+        rbow_server_options_default(&o);
+        if (opts) o = *opts;
+        ServerOptions so;
+        so.maxBatch = o.max_batch;
+        so.nbStreams = o.nb_streams;
+        so.memBudget = o.mem_budget;
+        so.K = o.max_results;
+        so.lookupKernel = o.lookup_kernel;
+        std::unique_ptr<rbow_server> h(new rbow_server);
+        h->srv.reset(new Server(*voc->v, *db->db, so));
+        *out = h.release();
+    });
+}
+
+void rbow_server_destroy(rbow_server* srv) { delete srv; }
+
+rbow_status rbow_server_submit(rbow_server* srv, const uint8_t* host_desc, uint32_t n, uint64_t* ticket) {
+    return guarded([&] {
+        RBOW_REQUIRE(srv && ticket, RBOW_ERR_INVALID_ARG, "null argument");
+        *ticket = srv->srv->submit(host_desc, n);
+    });
+}
+
+rbow_status rbow_server_wait(rbow_server* srv, uint64_t ticket, uint32_t* docs, float* scores, uint32_t* count,
+                             double* latency_ms) {
+    return guarded([&] {
+        RBOW_REQUIRE(srv && count, RBOW_ERR_INVALID_ARG, "null argument");
+        const Server::Result r = srv->srv->wait(ticket);
+        *count = uint32_t(r.docs.size());
+        if (docs) std::copy(r.docs.begin(), r.docs.end(), docs);
+        if (scores) std::copy(r.scores.begin(), r.scores.end(), scores);
+        if (latency_ms) *latency_ms = r.doneMs - r.arriveMs;
+    });
+}
+
+rbow_status rbow_server_replay(rbow_server* srv, const uint8_t* host_desc, const uint64_t* offsets,
+                               uint32_t nb_queries, double qps, uint64_t seed, double* latency_ms, uint32_t* docs,
+                               uint32_t* counts, uint32_t* batch_sizes, double* elapsed_s) {
+    return guarded([&] {
+        RBOW_REQUIRE(srv && offsets && latency_ms && qps > 0, RBOW_ERR_INVALID_ARG, "bad argument");
+        Server& S = *srv->srv;
+        const uint32_t K = S.options().K;
+        // Poisson arrivals: exponential gaps at rate qps
+        std::mt19937_64 rng(seed);
+        std::exponential_distribution<double> gap(qps);
+        std::vector<double> at(nb_queries);
+        double t = 0;
+        for (uint32_t i = 0; i < nb_queries; i++) at[i] = (t += gap(rng));
+        std::vector<uint64_t> tickets(nb_queries);
+        std::exception_ptr subErr;
+        const auto start = std::chrono::steady_clock::now();
+        std::thread submitter([&] {
+            try {
+                for (uint32_t i = 0; i < nb_queries; i++) {
+                    std::this_thread::sleep_until(start + std::chrono::duration<double>(at[i]));
+                    tickets[i] = S.submit(host_desc + offsets[i] * kDims, uint32_t(offsets[i + 1] - offsets[i]));
+                }
+            } catch (...) {
+                subErr = std::current_exception();
+            }
+        });
+        submitter.join();  // submissions are quick; waiting afterwards keeps the open-loop schedule
+        if (subErr) std::rethrow_exception(subErr);
+        double first = 1e300, last = 0;
+        for (uint32_t i = 0; i < nb_queries; i++) {
+            const Server::Result r = S.wait(tickets[i]);
+            latency_ms[i] = r.doneMs - r.arriveMs;
+            first = std::min(first, r.arriveMs);
+            last = std::max(last, r.doneMs);
+            if (counts) counts[i] = uint32_t(r.docs.size());
+            if (docs) std::copy(r.docs.begin(), r.docs.end(), docs + uint64_t(i) * K);
+            if (batch_sizes) batch_sizes[i] = r.batchSize;
+        }
+        if (elapsed_s) *elapsed_s = nb_queries ? (last - first) * 1e-3 : 0.0;
     });
 }
 

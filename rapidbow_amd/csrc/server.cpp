@@ -171,7 +171,6 @@ void Server::loop() {
                     while (!pending_.empty() && batch.size() < opt_.maxBatch &&
                            inFlightBytes_ + pending_.front().bytes <= opt_.memBudget) {
                         inFlightBytes_ += pending_.front().bytes;
-                        free->bytes += 0;
                         batch.push_back(std::move(pending_.front()));
                         pending_.pop_front();
                     }
