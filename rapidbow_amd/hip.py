@@ -114,10 +114,19 @@ class DeviceArray:
             check(_hip.hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, D2H), "hipMemcpy D2H")
         return out
 
+    def view(self, first_row, nb_rows):
+        """Non-owning view of rows [first_row, first_row + nb_rows) (leading dimension)."""
+        row = self.nbytes // max(self.shape[0], 1) if self.shape else self.dtype.itemsize
+        assert 0 <= first_row and first_row + nb_rows <= self.shape[0]
+        v = DeviceArray.__new__(DeviceArray)
+        v.shape = (int(nb_rows),) + self.shape[1:]
+        v.dtype, v.nbytes, v.ptr, v._owner = self.dtype, int(nb_rows) * row, self.ptr + int(first_row) * row, self
+        return v
+
     def free(self):
-        if self.ptr:
+        if self.ptr and not hasattr(self, "_owner"):
             _hip.hipFree(self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
