@@ -15,15 +15,25 @@ Beside it on the same JSON line:
     kernel, with the identity check of the leaf ids and the near-tie count;
   * leaf lookups/s on a 4,096-word uint8 tree (branch 16, depth 3) for 50,000
     queries (I8 MFMA vs plain);
+  * batched retrieval: 100,000 images (lognormal descriptor counts, median
+    2,000, clamped to [200, 8,000]) on the 1M-word tree, batches of 64
+    re-noised query images, top-50: batch latency, queries/s, postings/s, and
+    (with the CPU leg) ranked-list identity and max score error vs the oracle;
+  * serving: 1,000,000 images (about 2e9 postings) resident in HBM, Poisson
+    arrivals at 200 / 1,000 / 5,000 QPS of queries with 300-6,000 descriptors,
+    p50/p99 latency and queries/s of the continuous-batching server vs one
+    query at a time with the plain kernels;
   * the CPU baseline: the oracle, on the host cores, on the same build.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--skip-retrieval] [--skip-serving]
 """
 import argparse
 import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -34,6 +44,10 @@ METRIC = "leaf lookups/s & k-means assign TOPS (MFMA vs plain); query p50/p99 la
 I8_PEAK_TOPS = 5000.0     # MI355X dense int8 MFMA, 5.0 POPS
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, 8 TB/s
 LOOKUP_BYTES = 132        # algorithmic bytes per descriptor looked up: 128 in + 4 out
+CPU_THREADS = 16          # host cores given to the CPU legs (the GPU box's share for one GPU)
+
+# seeds of the synthetic inputs
+SEED_TRAIN, SEED_QUERY, SEED_IMAGES, SEED_ARRIVALS = 1, 2, 3, 4
 
 
 def log(*a):
@@ -66,12 +80,208 @@ def cpu_baseline(rb, oracle, desc, threads):
                            f"k-means++ ({dt:.1f} s wall; includes the host regroup)"}
 
 
+def lognormal_counts(n, seed, median, sigma=0.5, lo=200, hi=8000):
+    rng = np.random.default_rng(seed)
+    return np.clip(np.rint(np.exp(rng.normal(np.log(median), sigma, n))), lo, hi).astype(np.int64)
+
+
+def image_descriptors(rb, syn, first_image, counts, desc_seed, noise_seed, noise_first, out):
+    """Descriptors of images first_image.. (one centre subset of 64 per image) into `out` (device rows)."""
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+    n = int(off[-1])
+    doff = rb.DeviceArray.from_numpy(off)
+    ids = syn.image_centre_ids(doff, len(counts), first_image, SEED_IMAGES, desc_seed, n)
+    syn.descriptors(n, noise_seed, first=noise_first, dev_centre_ids=ids, out=out)
+    return off
+
+
+def bench_retrieval(rb, syn, v32, with_cpu, threads):
+    """100k images on the 1M-word tree; batches of 64 re-noised query images, top-50."""
+    from rapidbow_amd import hip
+    NI, K, B, NB = 100_000, 50, 64, 16
+    cnt = lognormal_counts(NI, SEED_IMAGES, 2000)
+    off = np.concatenate([[0], np.cumsum(cnt)]).astype(np.uint64)
+    total = int(off[-1])
+    log(f"bench: retrieval DB: {NI} images, {total} descriptors")
+    db = rb.DataBase(v32)
+    CH = 10_000
+    maxn = max(int(off[min(i + CH, NI)] - off[i]) for i in range(0, NI, CH))
+    dbuf = rb.DeviceArray((maxn, 128), np.uint8)
+    lbuf = rb.DeviceArray((maxn,), np.uint32)
+    host_leaf = np.empty(total, np.uint32) if with_cpu else None
+    t = time.perf_counter()
+    for i0 in range(0, NI, CH):
+        i1 = min(i0 + CH, NI)
+        o = image_descriptors(rb, syn, i0, cnt[i0:i1], desc_seed=31, noise_seed=SEED_IMAGES,
+                              noise_first=int(off[i0]), out=dbuf)
+        n = int(o[-1])
+        v32.lookup(dbuf, n, lbuf, rb.LOOKUP_MFMA)
+        db.add_docs_device(np.arange(i0, i1, dtype=np.uint32), lbuf, o)
+        if with_cpu:
+            host_leaf[int(off[i0]):int(off[i1])] = lbuf.numpy(n)
+    hip.synchronize()
+    build_s = time.perf_counter() - t
+    del dbuf, lbuf
+    # queries: re-noised copies of random DB images (same centre draws, fresh noise)
+    rng = np.random.default_rng(SEED_QUERY)
+    qimg = rng.choice(NI, B * NB, replace=False)
+    qcnt = cnt[qimg]
+    qoff = np.concatenate([[0], np.cumsum(qcnt)]).astype(np.uint64)
+    qdesc = rb.DeviceArray((int(qoff[-1]), 128), np.uint8)
+    for i, g in enumerate(qimg):
+        image_descriptors(rb, syn, int(g), qcnt[i:i + 1], desc_seed=31, noise_seed=SEED_QUERY + 100,
+                          noise_first=int(off[g]), out=qdesc.view(int(qoff[i]), int(qcnt[i])))
+    qleaf = rb.DeviceArray((int(qoff[-1]),), np.uint32)
+    docs = rb.DeviceArray((B, K), np.uint32)
+    sc = rb.DeviceArray((B, K), np.float32)
+    cn = rb.DeviceArray((B,), np.uint32)
+    hip.synchronize()
+    lat, res_docs, res_sc = [], [], []
+    for rep in range(2):  # first pass warms
+        for bi in range(NB):
+            q0, q1 = bi * B, (bi + 1) * B
+            o = qoff[q0:q1 + 1]
+            t0 = time.perf_counter()
+            v32.lookup(qdesc.view(int(o[0]), int(o[-1] - o[0])), int(o[-1] - o[0]),
+                       qleaf.view(int(o[0]), int(o[-1] - o[0])), rb.LOOKUP_MFMA)
+            db.query_batch(qleaf, o, K, docs, sc, cn)  # returns when done
+            dt = time.perf_counter() - t0
+            if rep:
+                lat.append(dt * 1e3)
+                c = cn.numpy()
+                D, S = docs.numpy(), sc.numpy()
+                res_docs += [D[q, :c[q]].copy() for q in range(B)]
+                res_sc += [S[q, :c[q]].copy() for q in range(B)]
+    lat = np.array(lat)
+    out = {"images": NI, "postings": int(db.nb_postings), "db_build_s": round(build_s, 2),
+           "batch": B, "top_k": K, "batches": NB, "query_desc_mean": round(float(qcnt.mean()), 1),
+           "batch_ms_p50": round(float(np.percentile(lat, 50)), 3),
+           "batch_ms_p99": round(float(np.percentile(lat, 99)), 3),
+           "queries_per_s": round(B * NB / (lat.sum() * 1e-3), 1),
+           "self_first": float(np.mean([d.size > 0 and d[0] == g for d, g in zip(res_docs, qimg)]))}
+    if with_cpu:
+        import oracle
+        log("bench: retrieval oracle DB (host)")
+        w = v32.weights()
+        odb = oracle.DataBase(32, 4, w)
+        for i in range(NI):
+            odb.add(i, host_leaf[int(off[i]):int(off[i + 1])])
+            if i % 20_000 == 19_999:
+                log(f"bench:   oracle DB {i + 1} images")
+        ql = qleaf.numpy()
+        sample = range(0, B * NB, 8)
+        qs = [ql[int(qoff[i]):int(qoff[i + 1])] for i in sample]
+        t = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            ref = list(ex.map(lambda q: odb.query(q, K), qs))
+        cpu_s = time.perf_counter() - t
+        with ThreadPoolExecutor(threads) as ex:
+            ref64 = list(ex.map(lambda q: odb.query_fp64(q, K), qs))
+        exact = swaps = 0
+        err = 0.0
+        for (od, os_), (fd, fs), i in zip(ref, ref64, sample):
+            gd, gs = res_docs[i], res_sc[i]
+            if np.array_equal(gd, od):
+                exact += 1
+            elif np.array_equal(np.sort(gd), np.sort(od)) and np.max(np.abs(gs - os_)) < 2e-6:
+                swaps += 1  # neighbours with scores within the tolerance swapped
+            f = {int(d): s for d, s in zip(fd, fs)}
+            err = max(err, max((abs(float(s) - f[int(d)]) for d, s in zip(gd, gs) if int(d) in f), default=0.0))
+        out["oracle_check"] = {"queries": len(qs), "ranked_exact": exact, "tie_swaps_only": swaps,
+                               "max_abs_score_err_vs_fp64": err}
+        out["cpu_queries_per_s"] = {"value": round(len(qs) / cpu_s, 2), "cores": threads, "kind": "port",
+                                    "sample": f"oracle query of {len(qs)} of the leaf-id queries, one per thread"}
+    return out
+
+
+def bench_serving(rb, syn, v32, rates, duration_s):
+    """1M-image inverted file (postings synthesised from leaf clouds); Poisson replay per rate."""
+    from rapidbow_amd import hip
+    NI, K, CLOUD = 1_000_000, 50, 1024
+    # leaf clouds: the looked-up leaves of CLOUD samples around each synthetic centre
+    ids = rb.DeviceArray.from_numpy(np.repeat(np.arange(syn.nb_centres, dtype=np.uint32), CLOUD))
+    cd = syn.descriptors(syn.nb_centres * CLOUD, SEED_IMAGES + 50, dev_centre_ids=ids)
+    cloud = rb.DeviceArray((syn.nb_centres * CLOUD,), np.uint32)
+    v32.lookup(cd, syn.nb_centres * CLOUD, cloud, rb.LOOKUP_MFMA)
+    del cd, ids
+    cnt = lognormal_counts(NI, SEED_IMAGES + 1, 1765)  # mean ~2,000 postings per image
+    off = np.concatenate([[0], np.cumsum(cnt)]).astype(np.uint64)
+    total = int(off[-1])
+    log(f"bench: serving DB: {NI} images, {total} postings")
+    t = time.perf_counter()
+    leaf = rb.DeviceArray((total,), np.uint32)
+    doff = rb.DeviceArray.from_numpy(off)
+    syn.image_postings(doff, NI, 0, SEED_IMAGES, SEED_IMAGES + 2, cloud, CLOUD, total, out=leaf)
+    db = rb.DataBase(v32)
+    db.add_docs_device(np.arange(NI, dtype=np.uint32), leaf, off)
+    hip.synchronize()
+    del leaf, doff
+    build_s = time.perf_counter() - t
+    # query pool: 300-6000 descriptors (uniform), re-observations of random DB images
+    rng = np.random.default_rng(SEED_QUERY + 7)
+    NQ = max(1000, int(max(rates) * duration_s))
+    qimg = rng.integers(0, NI, NQ)
+    qn = rng.integers(300, 6001, NQ)
+    qoff = np.concatenate([[0], np.cumsum(qn)]).astype(np.uint64)
+    qd = rb.DeviceArray((int(qoff[-1]), 128), np.uint8)
+    for i in range(NQ):
+        v = qd.view(int(qoff[i]), int(qn[i]))
+        o = np.array([0, qn[i]], np.uint64)
+        doff = rb.DeviceArray.from_numpy(o)
+        cid = syn.image_centre_ids(doff, 1, int(qimg[i]), SEED_IMAGES, 77, int(qn[i]))
+        syn.descriptors(int(qn[i]), SEED_QUERY + 7, first=int(qoff[i]), dev_centre_ids=cid, out=v)
+    hip.synchronize()
+    qhost = qd.numpy()
+    del qd
+    out = {"images": NI, "postings": int(db.nb_postings), "db_build_s": round(build_s, 1), "top_k": K,
+           "query_desc": "uniform 300-6000", "rates": []}
+
+    def run(srv, n, qps):
+        o = qoff[:n + 1]
+        r = srv.replay(qhost[: int(o[-1])], o, qps, seed=SEED_ARRIVALS)
+        lat = r["latency_ms"]
+        return r, {"queries": n, "achieved_qps": round(n / r["elapsed_s"], 1),
+                   "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                   "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                   "mean_batch": round(float(r["batch_sizes"].mean()), 2)}
+
+    batched = rb.Server(v32, db, max_batch=64, nb_streams=3, mem_budget=8 << 30, max_results=K,
+                        lookup_kernel=rb.LOOKUP_MFMA)
+    single = rb.Server(v32, db, max_batch=1, nb_streams=1, mem_budget=8 << 30, max_results=K,
+                       lookup_kernel=rb.LOOKUP_PLAIN)
+    run(batched, 64, 1000.0)  # warm both
+    _, w1 = run(single, 16, 50.0)
+    # one-at-a-time service time (arrivals 20 ms apart), to bound the overloaded baseline runs to ~10 s each
+    svc_ms = max(w1["p50_ms"], 0.05)
+    out["one_at_a_time_service_ms"] = svc_ms
+    for qps in rates:
+        n = min(NQ, max(1000, int(qps * duration_s)))
+        log(f"bench: serving {qps} QPS, {n} queries")
+        rb_, m = run(batched, n, qps)
+        nb = min(n, max(200, int(10e3 / svc_ms)))
+        rs, s = run(single, nb, qps)
+        same = 0
+        for i in range(nb):
+            c = int(rs["counts"][i])
+            same += int(c == int(rb_["counts"][i]) and np.array_equal(rs["docs"][i, :c], rb_["docs"][i, :c]))
+        out["rates"].append({"offered_qps": qps, "batched": m, "one_at_a_time_plain": s,
+                             "same_ranked_ids": f"{same}/{nb}"})
+        log(f"bench:   batched p50 {m['p50_ms']} ms p99 {m['p99_ms']} ms {m['achieved_qps']} q/s | "
+            f"single p50 {s['p50_ms']} p99 {s['p99_ms']} {s['achieved_qps']} q/s")
+    batched.close()
+    single.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-retrieval", action="store_true")
+    ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=1.5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is design-only (see DESIGN.md)",
@@ -84,7 +294,7 @@ def main():
     syn = rb.DeviceSynth()
     N = 2_000_000
     log(f"bench: generating {N} SIFT-like descriptors in HBM")
-    d = syn.descriptors(N, seed=1)
+    d = syn.descriptors(N, seed=SEED_TRAIN)
     hip.synchronize()
 
     # ---------------- batched vocabulary build (the timed steps) ----------------
@@ -115,7 +325,7 @@ def main():
     v32 = rb.build_vocabulary(d, N, 1000, 4, branch_factor=32, max_iters=10, seed_mode=1, centroid_type=1, kernel=1)
     st32 = v32.build_stats()
     NQ = 10_000_000
-    q = syn.descriptors(NQ, seed=2)
+    q = syn.descriptors(NQ, seed=SEED_QUERY)
     out_m = rb.DeviceArray((NQ,), np.uint32)
     out_p = rb.DeviceArray((NQ,), np.uint32)
     ties = rb.DeviceArray.zeros((1,), np.uint32)
@@ -127,11 +337,13 @@ def main():
     near_ties = int(ties.numpy()[0])
     log(f"bench: b32 lookups mfma {r_m / 1e9:.3f} G/s plain {r_p / 1e9:.3f} G/s identical={identical} "
         f"near_ties={near_ties}")
+    qh = q.view(0, 200_000).numpy() if not args.no_cpu else None
+    del q, out_p
 
     # ---------------- 4,096-word uint8 tree, 50k lookups ----------------
-    d1 = syn.descriptors(200_000, seed=3)
+    d1 = syn.descriptors(200_000, seed=SEED_TRAIN + 10)
     v1 = rb.build_vocabulary(d1, 200_000, 1000, 3, max_iters=8, seed_mode=1)
-    q1 = syn.descriptors(50_000, seed=4)
+    q1 = syn.descriptors(50_000, seed=SEED_QUERY + 10)
     o1 = rb.DeviceArray((50_000,), np.uint32)
     o1p = rb.DeviceArray((50_000,), np.uint32)
     r1_m = timed_lookup(rb, v1, q1, 50_000, o1, rb.LOOKUP_MFMA, 20)
@@ -158,15 +370,19 @@ def main():
         "lookup_b16L3_u8_50k": {"mfma_desc_per_s": round(r1_m, 1), "plain_desc_per_s": round(r1_p, 1),
                                 "identical_leaf_ids": id1},
     }
+    threads = CPU_THREADS
+    if not args.skip_retrieval:
+        line["retrieval_100k_b32L4"] = bench_retrieval(rb, syn, v32, not args.no_cpu, threads)
+        log(f"bench: retrieval {line['retrieval_100k_b32L4']}")
+    if not args.skip_serving:
+        line["serving_1M_b32L4"] = bench_serving(rb, syn, v32, (200.0, 1000.0, 5000.0), args.serving_seconds)
     if not args.no_cpu:
         import oracle
-        threads = min(os.cpu_count() or 1, 16)
         log(f"bench: CPU baseline on {threads} threads")
         host = d.numpy()
         _, cb = cpu_baseline(rb, oracle, host, threads)
         line["cpu_baseline"] = cb
-        nq = 200_000
-        qh = q.numpy()[:nq]
+        nq = qh.shape[0]
         tr = np.zeros((v32.nb_nodes, 32, 128), np.float32)
         for l in range(4):
             b0 = (32 ** l - 1) // 31
