@@ -68,7 +68,7 @@ struct KmScratch {
     DevBuf<uint32_t> beg, cnt, active, passes, nbActive, order, tcBeg, tcCnt, partCnt, partChanged;
     DevBuf<KmTile> tiles;
     DevBuf<KmChunk> chunks, red;
-    DevBuf<int8_t> XA, XT;
+    DevBuf<int8_t> XA;
     DevBuf<int32_t> partSums;
     uint32_t nbRed = 0;
     std::vector<KmTile> ht;
@@ -132,7 +132,6 @@ struct KmScratch {
         tcBeg.resize(std::max(1u, k.nbTasks));
         tcCnt.resize(std::max(1u, k.nbTasks));
         XA.resize(std::max<size_t>(ht.size(), 1) * 8192);
-        XT.resize(std::max<size_t>(ht.size(), 1) * 8192);
         partSums.resize(std::max<size_t>(hc.size(), 1) * 16 * kDims);
         partCnt.resize(std::max<size_t>(hc.size(), 1) * 16);
         partChanged.resize(std::max<size_t>(hc.size(), 1));
@@ -143,9 +142,8 @@ struct KmScratch {
             RBOW_HIP(hipMemcpyAsync(tcBeg.get(), cb.data(), 4ull * k.nbTasks, hipMemcpyHostToDevice, s));
             RBOW_HIP(hipMemcpyAsync(tcCnt.get(), cc.data(), 4ull * k.nbTasks, hipMemcpyHostToDevice, s));
         }
-        launchGatherTiles(points, order.get(), tiles.get(), uint32_t(ht.size()), XA.get(), XT.get(), s);
+        launchGatherTiles(points, order.get(), tiles.get(), uint32_t(ht.size()), XA.get(), s);
         k.XA = XA.get();
-        k.XT = XT.get();
         k.chunks = chunks.get();
         k.nbChunks = uint32_t(hc.size());
         k.taskChunkBeg = tcBeg.get();

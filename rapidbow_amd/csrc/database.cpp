@@ -37,6 +37,8 @@ DbView Database::view() const {
     v.post = post_.get();
     v.rcp = rcp_.get();
     v.docId = docId_.get();
+    v.dir = dirStale_ ? nullptr : dir_.get();  // a stale directory is never handed to a kernel
+    v.R = dirR_;
     return v;
 }
 
@@ -92,17 +94,27 @@ void Database::ensureMerged(hipStream_t s) const {
                                 4ull * (nDocs - docsOnDevice_), hipMemcpyHostToDevice, s));
         docsOnDevice_ = nDocs;
     }
-    if (pendLeaf_.empty()) {
-        RBOW_HIP(hipStreamSynchronize(s));
-        return;
+    if (!pendLeaf_.empty()) {
+        const uint64_t n = pendLeaf_.size();
+        DevBuf<uint32_t> l(n), sl(n);
+        RBOW_HIP(hipMemcpyAsync(l.get(), pendLeaf_.data(), 4 * n, hipMemcpyHostToDevice, s));
+        RBOW_HIP(hipMemcpyAsync(sl.get(), pendSlot_.data(), 4 * n, hipMemcpyHostToDevice, s));
+        mergeDevice(l.get(), sl.get(), n, s);
+        pendLeaf_.clear();
+        pendSlot_.clear();
     }
-    const uint64_t n = pendLeaf_.size();
-    DevBuf<uint32_t> l(n), sl(n);
-    RBOW_HIP(hipMemcpyAsync(l.get(), pendLeaf_.data(), 4 * n, hipMemcpyHostToDevice, s));
-    RBOW_HIP(hipMemcpyAsync(sl.get(), pendSlot_.data(), 4 * n, hipMemcpyHostToDevice, s));
-    mergeDevice(l.get(), sl.get(), n, s);
-    pendLeaf_.clear();
-    pendSlot_.clear();
+    refreshDir(s);
+    RBOW_HIP(hipStreamSynchronize(s));
+}
+
+void Database::refreshDir(hipStream_t s) const {
+    const uint32_t R = divUp(docsOnDevice_, kRangeDocs);
+    if (!dirStale_ && R == dirR_) return;
+    RBOW_REQUIRE(R == 0 || nbLeaves_ * (R + 1) < (1ull << 40), RBOW_ERR_CAPACITY, "range directory too large");
+    dir_.resize(std::max<uint64_t>(nbLeaves_ * (R + 1), 1));
+    dirR_ = R;
+    if (R) launchBuildDir(view(), dir_.get(), s);
+    dirStale_ = false;
 }
 
 void Database::mergeDevice(const uint32_t* dLeaf, const uint32_t* dSlot, uint64_t n, hipStream_t s) const {
@@ -113,6 +125,7 @@ void Database::mergeDevice(const uint32_t* dLeaf, const uint32_t* dSlot, uint64_
     leafOff_ = std::move(off);
     post_ = std::move(post);
     nbPost_ += n;
+    dirStale_ = true;
 }
 
 void Database::addDocsDevice(const uint32_t* docs, uint32_t nbDocs, const uint32_t* dLeaf, const uint64_t* offsets,
@@ -156,6 +169,7 @@ void Database::addDocsDevice(const uint32_t* docs, uint32_t nbDocs, const uint32
         }
         d0 = d1;
     }
+    refreshDir(s);
     RBOW_HIP(hipStreamSynchronize(s));
 }
 

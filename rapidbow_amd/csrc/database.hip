@@ -19,7 +19,7 @@ namespace rbow_amd {
 namespace {
 
 constexpr uint32_t kSortCap = Database::kMaxQueryDesc;  // leaves per query sorted in LDS
-constexpr uint32_t kCandCap = 2048;                     // boundary candidates kept by the top-K select
+constexpr uint32_t kSegBatch = 1024;                    // query leaves staged per scoring round (one per thread)
 
 __device__ inline float ftz(float x) { return fabsf(x) < 1.17549435e-38f ? copysignf(0.0f, x) : x; }
 __device__ inline float fmulz(float a, float b) { return ftz(ftz(a) * ftz(b)); }
@@ -100,10 +100,6 @@ __global__ void k_fill64(uint64_t* p, uint64_t v, uint64_t n) {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
         p[i] = v;
 }
-__global__ void k_fill8(uint8_t* p, uint64_t n) {
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
-        p[i] = 0;
-}
 
 // ------------------------------------------------------------- query ---
 // One 1024-thread block per query: sort the leaves, norm, scored node list.
@@ -112,14 +108,17 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
                                                    QNode* __restrict__ nodes, uint32_t* __restrict__ nodeCount) {
     __shared__ uint32_t s[kSortCap];
     __shared__ double red[16];
-    __shared__ uint32_t nNodes;
+    __shared__ uint32_t nNodes, nInner;
     const uint32_t q = blockIdx.x, tid = threadIdx.x;
     const uint64_t beg = off[q];
     const uint32_t n = uint32_t(off[q + 1] - beg);
     uint32_t n2 = 1;
     while (n2 < n) n2 <<= 1;
     for (uint32_t i = tid; i < n2; i += 1024) s[i] = i < n ? leaf[beg + i] : 0xFFFFFFFFu;
-    if (tid == 0) nNodes = 0;
+    if (tid == 0) {
+        nNodes = 0;
+        nInner = 0;
+    }
     __syncthreads();
     // bitonic sort, ascending
     for (uint32_t k = 2; k <= n2; k <<= 1) {
@@ -146,6 +145,7 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
     const double l1 = blockSumF64(part, red);
     const float qrcp = ftz(float(1.0 / l1));  // DataBase.cpp:77-94
     const uint64_t thr = 16ull + 4ull * K;    // DataBase.h:60-61
+    // leaves at out[0, nLeaf), inner nodes at out[n, n + nInner)
     QNode* out = nodes + uint64_t(v.L) * beg;
     for (uint32_t t = 0; t < v.L; t++) {
         const uint32_t shift = t * v.lg2b;
@@ -174,57 +174,38 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
                 nd.pad = 0;
                 nd.a = a;
                 nd.e = e;
-                out[atomicAdd(&nNodes, 1u)] = nd;
+                if (t == 0) out[atomicAdd(&nNodes, 1u)] = nd;
+                else out[n + atomicAdd(&nInner, 1u)] = nd;
             }
         }
     }
     __syncthreads();
-    if (tid == 0) nodeCount[q] = nNodes;
+    if (tid == 0) {
+        nodeCount[2 * q] = nNodes;
+        nodeCount[2 * q + 1] = nInner;
+    }
 }
 
-__device__ inline void addTerm(const DbView& v, double* acc, uint8_t* touched, uint32_t slot, uint32_t m, float w,
-                               float qE) {
-    const float tE = fmulz(fmulz(float(m), w), v.rcp[slot]);              // DataBase.cpp:128-129
-    const float c = fsubz(fabsf(fsubz(qE, tE)), faddz(qE, tE));            // DataBase.cpp:130
-    atomicAdd(&acc[slot], double(c));
-    touched[slot] = 1;
-}
-
-// Wave per scored node; grid (x, query).
-__global__ __launch_bounds__(256) void k_score(DbView v, const uint64_t* __restrict__ off,
-                                              const QNode* __restrict__ nodes, const uint32_t* __restrict__ nodeCount,
-                                              double* __restrict__ accAll, uint8_t* __restrict__ touchedAll) {
-    const uint32_t q = blockIdx.y, lane = threadIdx.x & 63;
-    const QNode* nd = nodes + uint64_t(v.L) * off[q];
-    const uint32_t nn = nodeCount[q];
-    double* acc = accAll + uint64_t(q) * v.nbDocs;
-    uint8_t* touched = touchedAll + uint64_t(q) * v.nbDocs;
-    for (uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6); k < nn; k += gridDim.x * 4) {
-        const QNode node = nd[k];
-        if (node.t == 0) {
-            // one leaf: duplicates of a doc are adjacent (DataBase.cpp:138-149)
-            for (uint64_t i = node.a + lane; i < node.e; i += 64) {
-                const uint32_t slot = v.post[i];
-                if (i != node.a && v.post[i - 1] == slot) continue;
-                uint32_t m = 1;
-                while (i + m < node.e && v.post[i + m] == slot) m++;
-                addTerm(v, acc, touched, slot, m, node.w, node.qE);
-            }
+// Range directory: dir[(R+1)*leaf + r] = first posting of the leaf whose slot
+// is >= r*kRangeDocs, relative to leafOff[leaf] (lists are sorted by slot).
+__global__ void k_buildDir(DbView v, uint32_t* __restrict__ dir) {
+    const uint64_t n = v.nbLeaves * (v.R + 1);
+    for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t leaf = e / (v.R + 1);
+        const uint32_t r = uint32_t(e - leaf * (v.R + 1));
+        const uint64_t a = v.leafOff[leaf], len = v.leafOff[leaf + 1] - a;
+        uint64_t lo = 0, hi = len;
+        if (r == v.R) {
+            lo = len;
         } else {
-            // virtual inverted file of < 16+4K postings (DataBase.cpp:150-164)
-            for (uint64_t i = node.a + lane; i < node.e; i += 64) {
-                const uint32_t slot = v.post[i];
-                bool first = true;
-                uint32_t m = 0;
-                for (uint64_t j = node.a; j < node.e; j++) {
-                    if (v.post[j] == slot) {
-                        m++;
-                        if (j < i) first = false;
-                    }
-                }
-                if (first) addTerm(v, acc, touched, slot, m, node.w, node.qE);
+            const uint32_t key = r * Database::kRangeDocs;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) / 2;
+                if (v.post[a + mid] < key) lo = mid + 1;
+                else hi = mid;
             }
         }
+        dir[e] = uint32_t(lo);
     }
 }
 
@@ -238,103 +219,288 @@ __device__ inline float keyScore(uint64_t k) {
     return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
 }
 
-// One block per query: exact K smallest (score, doc id) keys by radix select
-// over 12-bit digits, then a bitonic sort of the survivors.
-__global__ __launch_bounds__(1024) void k_topk(DbView v, uint32_t K, const double* __restrict__ accAll,
-                                              const uint8_t* __restrict__ touchedAll, uint32_t* __restrict__ outDocs,
-                                              float* __restrict__ outScores, uint32_t* __restrict__ outCount) {
-    __shared__ uint32_t hist[4096];
-    __shared__ uint64_t cand[kCandCap];
-    __shared__ uint64_t sel[Database::kMaxResults];
-    __shared__ uint32_t nCand, nSel, sDigit, sBefore, sTotal;
-    const uint32_t q = blockIdx.x, tid = threadIdx.x;
-    const double* acc = accAll + uint64_t(q) * v.nbDocs;
-    const uint8_t* touched = touchedAll + uint64_t(q) * v.nbDocs;
-    uint64_t prefix = 0, mask = 0;
-    uint32_t need = K, shiftHi = 64;
-    bool all = false;
-    while (shiftHi > 0) {
-        const uint32_t width = shiftHi >= 12 ? 12 : shiftHi;
-        const uint32_t shift = shiftHi - width;
-        for (uint32_t i = tid; i < 4096; i += 1024) hist[i] = 0;
+// Block-wide scan of one value per thread pair (1024 threads): returns the
+// exclusive prefix of this thread's value; *total gets the block total.
+// `red` holds 16 wave totals.
+__device__ inline uint32_t blockScan(uint32_t val, uint32_t* red, uint32_t* total) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t s = val;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = uint32_t(__shfl_up(int(s), d, 64));
+        if (lane >= uint32_t(d)) s += t;
+    }
+    __syncthreads();  // red is free
+    if (lane == 63) red[wave] = s;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t w = lane < 16 ? red[lane] : 0u;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const uint32_t t = uint32_t(__shfl_up(int(w), d, 64));
+            if (lane >= uint32_t(d)) w += t;
+        }
+        if (lane < 16) red[lane] = w;
+    }
+    __syncthreads();
+    *total = red[15];
+    return (wave ? red[wave - 1] : 0u) + s - val;
+}
+
+// Exact selection, by 8-bit MSB-first radix passes, of the `need` smallest of
+// the unique 64-bit keys given by keyAt(i, key) for i < n (keyAt returns false
+// for empty positions); returns the largest selected key, so the selection is
+// every key <= it.  Requires 1 <= need <= #keys.  hist: 256 words, red: 2 words.
+template <class KeyAt>
+__device__ uint64_t blockSelectKth(KeyAt keyAt, uint32_t n, uint32_t need, uint32_t* hist, uint32_t* red) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    uint64_t prefix = 0;
+    for (int shift = 56; shift >= 0; shift -= 8) {
         __syncthreads();
-        for (uint32_t i = tid; i < v.nbDocs; i += 1024) {
-            if (!touched[i]) continue;
-            const uint64_t key = rankKey(acc[i], v.docId[i]);
-            if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & ((1u << width) - 1)], 1u);
+        if (tid < 256) hist[tid] = 0;
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += blockDim.x) {
+            uint64_t k;
+            if (!keyAt(i, k)) continue;
+            if (shift == 56 || (k >> (shift + 8)) == (prefix >> (shift + 8))) atomicAdd(&hist[(k >> shift) & 255u], 1u);
         }
         __syncthreads();
-        if (tid == 0) {
-            uint32_t before = 0, d = 0, total = 0;
-            for (uint32_t i = 0; i < (1u << width); i++) total += hist[i];
-            if (total <= need) {
-                d = 0xFFFFFFFFu;
-            } else {
-                for (d = 0; before + hist[d] < need; d++) before += hist[d];
+        if (tid < 64) {  // wave 0: the bin holding the need-th key
+            const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2], h3 = hist[4 * lane + 3];
+            const uint32_t sum = h0 + h1 + h2 + h3;
+            uint32_t incl = sum;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t t = uint32_t(__shfl_up(int(incl), d, 64));
+                if (lane >= uint32_t(d)) incl += t;
             }
-            sDigit = d;
-            sBefore = before;
-            sTotal = d == 0xFFFFFFFFu ? total : hist[d];
-        }
-        __syncthreads();
-        if (sDigit == 0xFFFFFFFFu) {  // every matching key is selected
-            all = true;
-            break;
-        }
-        need -= sBefore;
-        prefix |= uint64_t(sDigit) << shift;
-        mask |= uint64_t((1u << width) - 1) << shift;
-        shiftHi = shift;
-        if (sTotal <= kCandCap) break;
-        __syncthreads();
-    }
-    // collect: keys below the boundary prefix are selected; boundary keys are candidates
-    if (tid == 0) {
-        nCand = 0;
-        nSel = 0;
-    }
-    __syncthreads();
-    for (uint32_t i = tid; i < v.nbDocs; i += 1024) {
-        if (!touched[i]) continue;
-        const uint64_t key = rankKey(acc[i], v.docId[i]);
-        const uint64_t hp = key & mask;
-        if (hp == prefix) {
-            const uint32_t c = atomicAdd(&nCand, 1u);
-            if (c < kCandCap) cand[c] = key;
-        } else if (hp < prefix) {
-            const uint32_t c = atomicAdd(&nSel, 1u);
-            if (c < Database::kMaxResults) sel[c] = key;
-        }
-    }
-    __syncthreads();
-    const uint32_t nc = min(nCand, kCandCap);
-    uint32_t n2 = 1;
-    while (n2 < nc) n2 <<= 1;
-    for (uint32_t i = nc + tid; i < n2; i += 1024) cand[i] = ~0ull;
-    __syncthreads();
-    for (uint32_t k = 2; k <= n2; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = tid; i < n2; i += 1024) {
-                const uint32_t p = i ^ j;
-                if (p > i) {
-                    const uint64_t a = cand[i], c = cand[p];
-                    if ((a > c) == ((i & k) == 0)) {
-                        cand[i] = c;
-                        cand[p] = a;
+            const uint32_t excl = incl - sum;
+            if (excl < need && need <= incl) {
+                const uint32_t hs[4] = {h0, h1, h2, h3};
+                uint32_t before = excl, bin = 4 * lane + 3;
+                for (uint32_t j = 0; j < 4; j++) {
+                    if (before + hs[j] >= need) {
+                        bin = 4 * lane + j;
+                        break;
                     }
+                    before += hs[j];
+                }
+                red[0] = bin;
+                red[1] = need - before;
+            }
+        }
+        __syncthreads();
+        prefix |= uint64_t(red[0]) << shift;
+        need = red[1];
+    }
+    __syncthreads();
+    return prefix;
+}
+
+// One workgroup per (document range r, query q).  The binary64 accumulators of
+// the range's kRangeDocs slots live in LDS (ds_add_f64), so a posting costs a
+// coalesced 4-byte read and an LDS atomic instead of a scattered global one.
+//   Leaves, in rounds of kSegBatch: the in-range segment of each query leaf
+//   comes from the range directory; non-empty segments are compacted by a
+//   block scan, and each wave walks a contiguous share of the concatenated
+//   postings 64 at a time (balanced however skewed the leaves are).  A run of
+//   one doc's duplicates (DataBase.cpp:138-149) is scored once, by its first
+//   posting, with m = run length from a ballot of run starts.
+//   Inner nodes (fewer than 16+4K postings, DataBase.cpp:150-164): one wave
+//   per node, multiplicity by scanning the node's slice.
+// The range then keeps its K smallest (score, doc id) keys as candidates for
+// the query's final selection.
+__global__ __launch_bounds__(1024) void k_scoreRange(DbView v, const uint64_t* __restrict__ off, uint32_t q0,
+                                                    const QNode* __restrict__ nodes,
+                                                    const uint32_t* __restrict__ nodeCount, uint32_t K,
+                                                    uint64_t* __restrict__ candKeys, uint32_t* __restrict__ candCount) {
+    constexpr uint32_t S = Database::kRangeDocs;
+    __shared__ double sAcc[S];                 // reused as the keys of the selection
+    __shared__ float sRcp[S];
+    __shared__ uint8_t sTouched[S];
+    __shared__ uint64_t sSegBeg[kSegBatch];    // first posting of the (non-empty) segment
+    __shared__ uint32_t sSegEnd[kSegBatch];    // inclusive prefix of segment lengths
+    __shared__ float sQE[kSegBatch], sW[kSegBatch];
+    __shared__ uint32_t sHist[256], sRed[16], sCount;
+    const uint32_t r = blockIdx.x, qc = blockIdx.y, q = q0 + qc, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t s0 = r * S, ns = min(S, v.nbDocs - s0);
+    for (uint32_t i = tid; i < S; i += 1024) {
+        sAcc[i] = 0.0;
+        sTouched[i] = 0;
+        sRcp[i] = i < ns ? v.rcp[s0 + i] : 0.0f;
+    }
+    if (tid == 0) sCount = 0;
+    __syncthreads();
+    const uint64_t beg = off[qc];  // off points at the launch's first query
+    const uint32_t n = uint32_t(off[qc + 1] - beg);
+    const QNode* nd = nodes + uint64_t(v.L) * beg;
+    const uint32_t nLeaf = nodeCount[2 * q], nInner = nodeCount[2 * q + 1];
+    for (uint32_t k0 = 0; k0 < nLeaf; k0 += kSegBatch) {
+        // ---- stage this round's segments (one query leaf per thread) ----
+        uint32_t len = 0;
+        uint64_t sb = 0;
+        float qE = 0.0f, w = 0.0f;
+        if (k0 + tid < nLeaf) {
+            const QNode node = nd[k0 + tid];
+            const uint32_t* d = v.dir + uint64_t(node.v) * (v.R + 1) + r;
+            const uint32_t a = d[0];
+            len = d[1] - a;
+            sb = node.a + a;
+            qE = node.qE;
+            w = node.w;
+        }
+        uint32_t total, nc;
+        const uint32_t ex = blockScan(len, sRed, &total);
+        const uint32_t cx = blockScan(len ? 1u : 0u, sRed, &nc);
+        if (len) {
+            sSegBeg[cx] = sb;
+            sSegEnd[cx] = ex + len;
+            sQE[cx] = qE;
+            sW[cx] = w;
+        }
+        __syncthreads();
+        // ---- walk the postings: wave-contiguous shares, 64 per step ----
+        const uint32_t per = ((total + 16 * 64 - 1) / (16 * 64)) * 64;
+        const uint32_t wEnd = min(total, wave * per + per);
+        uint32_t seg = 0;
+        if (wave * per < wEnd) {  // first segment whose end exceeds the share's start (wave-uniform)
+            uint32_t lo = 0, hi = nc - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (sSegEnd[mid] > wave * per) hi = mid;
+                else lo = mid + 1;
+            }
+            seg = lo;
+        }
+        for (uint32_t base = wave * per; base < wEnd; base += 64) {
+            const uint32_t idx = base + lane;
+            const bool valid = idx < wEnd;
+            uint32_t sg = seg;
+            if (valid) {
+                uint32_t lo = seg, hi = min(seg + 64u, nc - 1);  // a step spans at most 64 segments
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) / 2;
+                    if (sSegEnd[mid] > idx) hi = mid;
+                    else lo = mid + 1;
+                }
+                sg = lo;
+            }
+            const uint32_t segA = sg ? sSegEnd[sg - 1] : 0u, segLen = sSegEnd[sg] - segA, i = idx - segA;
+            const uint32_t* p = v.post + sSegBeg[sg];
+            const uint32_t slot = valid ? p[i] : 0xFFFFFFFFu;
+            const uint32_t prevSlot = uint32_t(__shfl_up(int(slot), 1, 64));
+            const uint32_t prevSg = uint32_t(__shfl_up(int(sg), 1, 64));
+            bool head = false;
+            if (valid) {
+                if (i == 0) head = true;
+                else if (lane != 0 && prevSg == sg) head = prevSlot != slot;
+                else head = p[i - 1] != slot;
+            }
+            const uint64_t bound = __ballot(head || !valid);
+            if (head) {
+                const uint64_t above = lane == 63 ? 0ull : (bound >> (lane + 1));
+                uint32_t m = above ? uint32_t(__builtin_ctzll(above)) + 1 : 64 - lane;
+                if (lane + m >= 64 || idx + m >= wEnd) {
+                    // no run start follows in this step: the run may go on past it or past the share
+                    while (i + m < segLen && p[i + m] == slot) m++;
+                }
+                const uint32_t j = slot - s0;
+                const float qe = sQE[sg];
+                const float tE = fmulz(fmulz(float(m), sW[sg]), sRcp[j]);      // DataBase.cpp:128-129
+                const float c = fsubz(fabsf(fsubz(qe, tE)), faddz(qe, tE));  // DataBase.cpp:130
+                atomicAdd(&sAcc[j], double(c));
+                sTouched[j] = 1;
+            }
+            seg = uint32_t(__shfl(int(sg), 63, 64));
+        }
+        __syncthreads();  // segment arrays are reused by the next round
+    }
+    for (uint32_t k = wave; k < nInner; k += 16) {
+        const QNode node = nd[n + k];
+        for (uint64_t i = node.a + lane; i < node.e; i += 64) {
+            const uint32_t slot = v.post[i], j = slot - s0;
+            if (j >= S) continue;
+            bool first = true;
+            uint32_t m = 0;
+            for (uint64_t x = node.a; x < node.e; x++) {
+                if (v.post[x] == slot) {
+                    m++;
+                    if (x < i) first = false;
                 }
             }
-            __syncthreads();
+            if (!first) continue;
+            const float tE = fmulz(fmulz(float(m), node.w), sRcp[j]);
+            const float c = fsubz(fabsf(fsubz(node.qE, tE)), faddz(node.qE, tE));
+            atomicAdd(&sAcc[j], double(c));
+            sTouched[j] = 1;
         }
-    // selected (< K keys, unsorted) + the `need` smallest candidates
-    const uint32_t ns = min(nSel, K);
-    const uint32_t take = all ? min(nc, K) : min(need, nc);
-    for (uint32_t i = tid; i < take; i += 1024) sel[ns + i] = cand[i];
+    }
     __syncthreads();
-    const uint32_t tot = ns + take;
-    n2 = 1;
-    while (n2 < tot) n2 <<= 1;
-    for (uint32_t i = tot + tid; i < n2; i += 1024) sel[i] = ~0ull;
+    // keys in place of the accumulators; untouched slots get the largest key
+    uint64_t* sKey = reinterpret_cast<uint64_t*>(sAcc);
+    uint32_t mine = 0;
+    for (uint32_t i = tid; i < S; i += 1024) {
+        const bool t = i < ns && sTouched[i];
+        sKey[i] = t ? rankKey(sAcc[i], v.docId[s0 + i]) : ~0ull;
+        mine += t;
+    }
+    atomicAdd(&sCount, mine);
+    __syncthreads();
+    const uint32_t nT = sCount, need = min(K, nT);
+    uint64_t* out = candKeys + (uint64_t(q) * v.R + r) * K;
+    if (need == 0) {
+        if (tid == 0) candCount[uint64_t(q) * v.R + r] = 0;
+        return;
+    }
+    const uint64_t kth = need == nT ? ~0ull - 1
+                                    : blockSelectKth([&](uint32_t i, uint64_t& k) { k = sKey[i]; return true; }, S,
+                                                     need, sHist, sRed);
+    __syncthreads();
+    if (tid == 0) sCount = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < S; i += 1024) {
+        const uint64_t key = sKey[i];
+        if (key <= kth) out[atomicAdd(&sCount, 1u)] = key;
+    }
+    if (tid == 0) candCount[uint64_t(q) * v.R + r] = need;
+}
+
+// One block per query: the K smallest of the ranges' candidate keys, sorted
+// ascending ((score, doc id): DataBase.cpp:170-176 with ties by doc id).
+__global__ __launch_bounds__(1024) void k_topk(DbView v, uint32_t K, const uint64_t* __restrict__ candKeys,
+                                              const uint32_t* __restrict__ candCount, uint32_t* __restrict__ outDocs,
+                                              float* __restrict__ outScores, uint32_t* __restrict__ outCount) {
+    __shared__ uint64_t sel[Database::kMaxResults];
+    __shared__ uint32_t hist[256], red[16], nSel;
+    const uint32_t q = blockIdx.x, tid = threadIdx.x;
+    const uint64_t* keys = candKeys + uint64_t(q) * v.R * K;
+    const uint32_t* cnt = candCount + uint64_t(q) * v.R;
+    uint32_t mine = 0;
+    for (uint32_t rr = tid; rr < v.R; rr += 1024) mine += cnt[rr];
+    uint32_t tot;
+    (void)blockScan(mine, red, &tot);
+    const uint32_t need = min(K, tot);
+    if (need == 0) {
+        if (tid == 0) outCount[q] = 0;
+        return;
+    }
+    auto keyAt = [&](uint32_t i, uint64_t& key) {  // i over [0, R*K): range i / K, rank i % K
+        const uint32_t rr = i / K;
+        if (i - rr * K >= cnt[rr]) return false;
+        key = keys[i];
+        return true;
+    };
+    const uint64_t kth = need == tot ? ~0ull - 1 : blockSelectKth(keyAt, v.R * K, need, hist, red);
+    if (tid == 0) nSel = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < v.R * K; i += 1024) {
+        uint64_t key;
+        if (keyAt(i, key) && key <= kth) sel[atomicAdd(&nSel, 1u)] = key;
+    }
+    __syncthreads();
+    uint32_t n2 = 1;
+    while (n2 < need) n2 <<= 1;
+    for (uint32_t i = need + tid; i < n2; i += 1024) sel[i] = ~0ull;
     __syncthreads();
     for (uint32_t k = 2; k <= n2; k <<= 1)
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -350,12 +516,11 @@ __global__ __launch_bounds__(1024) void k_topk(DbView v, uint32_t K, const doubl
             }
             __syncthreads();
         }
-    const uint32_t cnt = min(tot, K);
-    for (uint32_t i = tid; i < cnt; i += 1024) {
+    for (uint32_t i = tid; i < need; i += 1024) {
         outDocs[uint64_t(q) * K + i] = uint32_t(sel[i]);
         if (outScores) outScores[uint64_t(q) * K + i] = keyScore(sel[i]);
     }
-    if (tid == 0) outCount[q] = cnt;
+    if (tid == 0) outCount[q] = need;
 }
 
 __global__ void k_expandSlots(const uint64_t* __restrict__ off, uint32_t nbDocs, uint32_t firstSlot,
@@ -371,6 +536,12 @@ uint32_t gridFor(uint64_t work, uint32_t perBlock, uint32_t cap) {
 }
 
 }  // namespace
+
+void launchBuildDir(const DbView& v, uint32_t* dir, hipStream_t s) {
+    if (!v.R) return;
+    k_buildDir<<<gridFor(v.nbLeaves * (v.R + 1), 256, 65536), 256, 0, s>>>(v, dir);
+    RBOW_HIP(hipGetLastError());
+}
 
 void launchExpandSlots(const uint64_t* off, uint32_t nbDocs, uint32_t firstSlot, uint32_t* slots, hipStream_t s) {
     if (!nbDocs) return;
@@ -419,23 +590,24 @@ void launchQueryBatch(const DbView& v, const uint32_t* dLeaf, const uint64_t* dO
                       uint32_t nbQueries, uint32_t K, uint32_t* dDocs, float* dScores, uint32_t* dCounts,
                       hipStream_t s) {
     if (!nbQueries) return;
-    uint64_t maxN = 0;
-    for (uint32_t q = 0; q < nbQueries; q++) maxN = std::max(maxN, hOffsets[q + 1] - hOffsets[q]);
+    RBOW_REQUIRE(v.R == divUp(v.nbDocs, Database::kRangeDocs) && v.R < 65536 && (v.dir || !v.R), RBOW_ERR_HIP,
+                 "stale range directory");
     const uint64_t total = hOffsets[nbQueries];
-    const uint64_t accN = uint64_t(nbQueries) * std::max(v.nbDocs, 1u);
-    const size_t bytes = sizeof(QNode) * std::max<uint64_t>(total * v.L, 1) + 4ull * nbQueries + 8 * accN + accN + 1024;
+    const uint64_t nbCand = uint64_t(nbQueries) * v.R * K;
+    const uint64_t nbNodes = std::max<uint64_t>(total * v.L, 1);
+    const size_t bytes = sizeof(QNode) * nbNodes + 8 * nbCand + 4ull * nbQueries * v.R + 8ull * nbQueries + 1024;
     void* ws = nullptr;
     RBOW_HIP(hipMallocAsync(&ws, bytes, s));
     QNode* nodes = static_cast<QNode*>(ws);
-    double* acc = reinterpret_cast<double*>(nodes + std::max<uint64_t>(total * v.L, 1));
-    uint8_t* touched = reinterpret_cast<uint8_t*>(acc + accN);
-    uint32_t* nodeCount = reinterpret_cast<uint32_t*>((reinterpret_cast<uintptr_t>(touched + accN) + 15) & ~uintptr_t(15));
-    k_fill64<<<gridFor(accN, 256, 8192), 256, 0, s>>>(reinterpret_cast<uint64_t*>(acc), 0, accN);
-    k_fill8<<<gridFor(accN, 256, 8192), 256, 0, s>>>(touched, accN);
+    uint64_t* candKeys = reinterpret_cast<uint64_t*>(nodes + nbNodes);
+    uint32_t* candCount = reinterpret_cast<uint32_t*>(candKeys + nbCand);
+    uint32_t* nodeCount = candCount + uint64_t(nbQueries) * v.R;
     k_queryPrep<<<nbQueries, 1024, 0, s>>>(v, dLeaf, dOffsets, K, nodes, nodeCount);
-    const uint32_t gx = gridFor(maxN * v.L, 4, 256);
-    k_score<<<dim3(gx, nbQueries), 256, 0, s>>>(v, dOffsets, nodes, nodeCount, acc, touched);
-    k_topk<<<nbQueries, 1024, 0, s>>>(v, K, acc, touched, dDocs, dScores, dCounts);
+    // ranges fastest, so the tiles of one query run together and share its node list in L2
+    for (uint32_t q0 = 0; v.R && q0 < nbQueries; q0 += 65535)
+        k_scoreRange<<<dim3(v.R, std::min(nbQueries - q0, 65535u)), 1024, 0, s>>>(v, dOffsets + q0, q0, nodes,
+                                                                                  nodeCount, K, candKeys, candCount);
+    k_topk<<<nbQueries, 1024, 0, s>>>(v, K, candKeys, candCount, dDocs, dScores, dCounts);
     RBOW_HIP(hipGetLastError());
     RBOW_HIP(hipFreeAsync(ws, s));
 }

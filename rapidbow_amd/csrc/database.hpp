@@ -6,6 +6,12 @@
 // Because leaves of one subtree are contiguous, a node's postings are one
 // contiguous slice and its posting count is leafOff[hi] - leafOff[lo], which
 // makes the reference's stop threshold O(1) per node.
+//
+// Scoring is tiled by document range: slots [r*kRangeDocs, (r+1)*kRangeDocs)
+// form range r, whose accumulators fit one workgroup's LDS.  The range
+// directory dir[leaf*(R+1) + r] is the first posting of the leaf with a slot
+// >= r*kRangeDocs (relative to leafOff[leaf]), so a (query, range) tile reads
+// only its own postings of each query leaf.
 #pragma once
 #include <mutex>
 #include <shared_mutex>
@@ -25,6 +31,8 @@ struct DbView {  // device view used by the scoring kernels
     const uint32_t* post;   // slots
     const float* rcp;       // [slot] float(1 / L1 norm)
     const uint32_t* docId;  // [slot]
+    const uint32_t* dir;    // [leaf][R + 1] range directory
+    uint32_t R;             // document ranges
 };
 
 class Database {
@@ -50,8 +58,10 @@ public:
     void prepareForQueries(hipStream_t s) const;
     void scoreAsync(const uint32_t* dLeaf, const uint64_t* dOffsets, const uint64_t* hOffsets, uint32_t nbQueries,
                     uint32_t K, uint32_t* dDocs, float* dScores, uint32_t* dCounts, hipStream_t s) const;
-    // HBM scratch of one query of n descriptors during scoring
-    uint64_t scratchBytes(uint32_t n) const { return uint64_t(nbDocs()) * 9 + uint64_t(n) * (L_ * 40 + 8) + 64; }
+    // HBM scratch of one query of n descriptors, top-K, during scoring
+    uint64_t scratchBytes(uint32_t n, uint32_t K) const {
+        return uint64_t(divUp(nbDocs(), kRangeDocs)) * (8ull * K + 4) + uint64_t(n) * L_ * 40 + 64;
+    }
 
     uint32_t nbDocs() const;
     uint64_t nbPostings() const;
@@ -60,9 +70,11 @@ public:
 
     static constexpr uint32_t kMaxQueryDesc = 16384;
     static constexpr uint32_t kMaxResults = 1024;
+    static constexpr uint32_t kRangeDocs = 4096;  // document slots per scoring tile
 
 private:
     void ensureMerged(hipStream_t s) const;  // folds pending postings into the CSR
+    void refreshDir(hipStream_t s) const;    // rebuilds the range directory if stale
     void mergeDevice(const uint32_t* dLeaf, const uint32_t* dSlot, uint64_t n, hipStream_t s) const;
     void growDocs(uint32_t nb, hipStream_t s) const;
     DbView view() const;
@@ -81,6 +93,9 @@ private:
     mutable DevBuf<float> rcp_;
     mutable DevBuf<uint32_t> docId_;
     mutable uint32_t docsOnDevice_ = 0;
+    mutable DevBuf<uint32_t> dir_;
+    mutable uint32_t dirR_ = 0;
+    mutable bool dirStale_ = true;
     mutable std::vector<uint32_t> pendLeaf_, pendSlot_;
     mutable std::shared_mutex mtx_;
     mutable std::mutex mergeMtx_;
@@ -91,6 +106,8 @@ void launchDocNorms(const DbView& v, const uint32_t* dLeaf, const uint64_t* dOff
                     uint32_t firstSlot, float* rcp, hipStream_t s);
 void launchMergePostings(const DbView& v, uint64_t nbOld, const uint32_t* dLeaf, const uint32_t* dSlot, uint64_t n,
                          uint64_t* newLeafOff, uint32_t* newPost, hipStream_t s);
+// Range directory of the current CSR (v.R ranges): dir[(R+1)*leaf + r].
+void launchBuildDir(const DbView& v, uint32_t* dir, hipStream_t s);
 // slots[i] = firstSlot + d for the postings of doc d (offsets rebased to off[0])
 void launchExpandSlots(const uint64_t* off, uint32_t nbDocs, uint32_t firstSlot, uint32_t* slots, hipStream_t s);
 void launchQueryBatch(const DbView& v, const uint32_t* dLeaf, const uint64_t* dOffsets, const uint64_t* hOffsets,

@@ -75,9 +75,8 @@ struct KmLevel {       // device state of one level's batched k-means
     uint32_t* active = nullptr;      // [task]
     uint32_t* passes = nullptr;      // [task]
     uint32_t* nbActive = nullptr;    // [1]
-    // MFMA path (uint8 centroids): per-tile swizzled operands and per-chunk partials
+    // MFMA path (uint8 centroids): per-tile swizzled operand and per-chunk partials
     const int8_t* XA = nullptr;      // [tile][rb 4][kb 2][lane 64][16]  distance A fragments (x - 128)
-    const int8_t* XT = nullptr;      // [tile][db 8][lane 64][16]        sum B fragments (x - 128), transposed
     const KmChunk* chunks = nullptr;
     uint32_t nbChunks = 0;
     const uint32_t* taskChunkBeg = nullptr;  // [task]
@@ -94,9 +93,9 @@ void launchKmeansPP(const uint8_t* desc, const uint32_t* order, const uint32_t* 
                     uint32_t nbTasks, uint32_t b, uint64_t seed, uint32_t level, uint32_t* seedIdx, uint32_t* masks,
                     hipStream_t s);
 void launchKmLevelReset(const KmLevel& k, uint64_t nbRows, hipStream_t s);
-// Per-tile swizzled operands XA / XT of the MFMA assign from desc[order[row]].
+// Per-tile swizzled operand XA of the MFMA assign from desc[order[row]].
 void launchGatherTiles(const uint8_t* desc, const uint32_t* order, const KmTile* tiles, uint32_t nbTiles, int8_t* XA,
-                       int8_t* XT, hipStream_t s);
+                       hipStream_t s);
 // Adds the per-chunk partials of the MFMA assign into the task totals
 // counts / sums / changed.  red: (task, chunkBeg, chunkEnd) items.
 void launchKmReduce(const KmLevel& k, const KmChunk* red, uint32_t nbRed, hipStream_t s);

@@ -240,19 +240,22 @@ __device__ inline void waveRange(uint32_t nbTiles, uint32_t& beg, uint32_t& end)
 }
 
 // ----------------------------------------------- assign: uint8, I8 MFMA --
-// Per 64-row tile the level gather writes two swizzled copies of x' = x - 128:
+// Per 64-row tile the level gather writes one swizzled copy of x' = x - 128:
 //   XA[tile][rb][kb][lane][16]: A fragments of the distance MFMA, row rb*16 + (lane&15),
-//                               dims kb*64 + (lane>>4)*16 .. +16;
-//   XT[tile][db][lane][16]:     B fragments of the sum MFMA, byte q = point
-//                               16*(q>>2) + 4*(lane>>4) + (q&3), dim 16*db + (lane&15).
+//                               dims kb*64 + (lane>>4)*16 .. +16.
 // Distances: mfma_i32_16x16x64_i8(XA, c') -> ||c'||^2 - 2 x'.c' (exact int32).
 // Sums: mfma_i32_16x16x64_i8(onehot^T, XT) -> per (center, dim) sums of x' over
-// the tile's points (exact int32), i.e. the update's centroid sums on the matrix
-// cores with no atomics; the 4x4 rows a lane owns after the distance MFMA are
-// exactly the 16 points of its K slice in XT, so the one-hot operand is local.
+// the tile's points (exact int32): the update's centroid sums on the matrix
+// cores, with no atomics.  XT is the transposed operand, byte q of lane l =
+// x'[point 16*(q>>2) + 4*(l>>4) + (q&3)][dim 16*db + (l&15)].  It is not read
+// from HBM: it is made on chip by multiplying the XA fragments with 0/1
+// selection matrices (the C layout of a 16x16 MFMA puts rows 4*(l>>4)+i of
+// column l&15 in lane l, which is exactly that byte order), so each point
+// costs 128 bytes of HBM per pass.  The 4x4 rows a lane owns after the
+// distance MFMA are the 16 points of its XT K slice, so the one-hot operand
+// is local.
 __global__ __launch_bounds__(256) void k_gatherTiles(const uint8_t* __restrict__ desc, const uint32_t* __restrict__ order,
-                                                    const KmTile* __restrict__ tiles, int8_t* __restrict__ XA,
-                                                    int8_t* __restrict__ XT) {
+                                                    const KmTile* __restrict__ tiles, int8_t* __restrict__ XA) {
     __shared__ __align__(16) uint8_t sx[64][144];
     const uint32_t t = blockIdx.x, tid = threadIdx.x;
     const KmTile tl = tiles[t];
@@ -270,33 +273,36 @@ __global__ __launch_bounds__(256) void k_gatherTiles(const uint8_t* __restrict__
     }
     __syncthreads();
     int8_t* xa = XA + uint64_t(t) * 8192;
-    int8_t* xt = XT + uint64_t(t) * 8192;
     for (uint32_t c = tid; c < 512; c += 256) {
         const uint32_t rb = c >> 7, kb = (c >> 6) & 1, l = c & 63;
         *reinterpret_cast<uint4*>(xa + c * 16) =
             *reinterpret_cast<const uint4*>(&sx[rb * 16 + (l & 15)][kb * 64 + (l >> 4) * 16]);
     }
-    for (uint32_t c = tid; c < 512; c += 256) {
-        const uint32_t db = c >> 6, l = c & 63;
-        uint32_t w[4];
+}
+
+__device__ inline int packLowBytes(v4i t) {
+    return int((uint32_t(t[0]) & 0xFFu) | ((uint32_t(t[1]) & 0xFFu) << 8) | ((uint32_t(t[2]) & 0xFFu) << 16) |
+               (uint32_t(t[3]) << 24));
+}
+
+__device__ inline void loadTileA(const int8_t* xa, uint32_t lane, v4i (&a)[4][2]) {
 #pragma unroll
-        for (int d = 0; d < 4; d++) {
-            uint32_t v = 0;
+    for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const uint32_t q = 4 * d + e;
-                const uint32_t row = 16 * (q >> 2) + 4 * (l >> 4) + (q & 3);
-                v |= uint32_t(sx[row][16 * db + (l & 15)]) << (8 * e);
-            }
-            w[d] = v;
-        }
-        *reinterpret_cast<uint4*>(xt + c * 16) = make_uint4(w[0], w[1], w[2], w[3]);
-    }
+        for (int kb = 0; kb < 2; kb++) a[rb][kb] = *reinterpret_cast<const v4i*>(xa + ((rb * 2 + kb) * 64 + lane) * 16);
 }
 
 __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 15, grp = lane >> 4;
+    // esel[g]: B operand that selects dims g*16 + col of a 64-dim K chunk
+    v4i esel[4];
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+        for (int d = 0; d < 4; d++)
+            esel[g][d] = (grp == uint32_t(g) && (col >> 2) == uint32_t(d)) ? int(1u << (8 * (col & 3))) : 0;
+    const v4i zero = {0, 0, 0, 0};
     for (uint32_t ch = blockIdx.x * kWaves + wave; ch < k.nbChunks; ch += gridDim.x * kWaves) {
         const KmChunk c = k.chunks[ch];
         if (!k.active[c.task]) continue;
@@ -312,20 +318,20 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
         ns += __shfl_xor(ns, 32, 64);  // ||c - 128||^2 of center `col`
         v4i acc[8];
 #pragma unroll
-        for (int db = 0; db < 8; db++) acc[db] = v4i{0, 0, 0, 0};
+        for (int db = 0; db < 8; db++) acc[db] = zero;
         uint32_t myCnt = 0, myChanged = 0;
+        v4i a[4][2];
+        loadTileA(k.XA + uint64_t(c.tileBeg) * 8192, lane, a);
         for (uint32_t ti = c.tileBeg; ti < c.tileEnd; ti++) {
             const KmTile t = k.tiles[ti];
-            const int8_t* xa = k.XA + uint64_t(ti) * 8192;
-            const int8_t* xt = k.XT + uint64_t(ti) * 8192;
+            v4i an[4][2];  // next tile's fragments, in flight while this one computes
+            if (ti + 1 < c.tileEnd) loadTileA(k.XA + uint64_t(ti + 1) * 8192, lane, an);
             uint32_t key[4][4];
 #pragma unroll
             for (int rb = 0; rb < 4; rb++) {
-                const v4i a0 = *reinterpret_cast<const v4i*>(xa + ((rb * 2 + 0) * 64 + lane) * 16);
-                const v4i a1 = *reinterpret_cast<const v4i*>(xa + ((rb * 2 + 1) * 64 + lane) * 16);
-                v4i d = {0, 0, 0, 0};
-                d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b0, d, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b1, d, 0, 0, 0);
+                v4i d = zero;
+                d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rb][0], b0, d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rb][1], b1, d, 0, 0, 0);
 #pragma unroll
                 for (int i = 0; i < 4; i++) key[rb][i] = (uint32_t(ns - 2 * d[i] + (1 << 25)) << 4) | col;
             }
@@ -340,14 +346,14 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     const uint32_t row = rb * 16 + 4 * grp + i;
-                    const uint32_t a = key[rb][i] & 15u;
+                    const uint32_t as = key[rb][i] & 15u;
                     const bool valid = row < t.cnt;
                     if (valid && col == uint32_t(i)) {  // one writer per row
                         const uint32_t p = t.row + row;
-                        myChanged += k.assign[p] != a;
-                        k.assign[p] = uint8_t(a);
+                        myChanged += k.assign[p] != as;
+                        k.assign[p] = uint8_t(as);
                     }
-                    w |= uint32_t(valid && a == col) << (8 * i);
+                    w |= uint32_t(valid && as == col) << (8 * i);
                 }
                 oh[rb] = int(w);
             }
@@ -355,8 +361,18 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
             for (int rb = 0; rb < 4; rb++) myCnt = __builtin_amdgcn_udot4(uint32_t(oh[rb]), 0x01010101u, myCnt, false);
 #pragma unroll
             for (int db = 0; db < 8; db++) {
-                const v4i bt = *reinterpret_cast<const v4i*>(xt + (db * 64 + lane) * 16);
+                v4i bt;
+#pragma unroll
+                for (int rb = 0; rb < 4; rb++)
+                    bt[rb] = packLowBytes(__builtin_amdgcn_mfma_i32_16x16x64_i8(a[rb][db >> 2], esel[db & 3], zero, 0, 0, 0));
                 acc[db] = __builtin_amdgcn_mfma_i32_16x16x64_i8(oh, bt, acc[db], 0, 0, 0);
+            }
+            if (ti + 1 < c.tileEnd) {
+#pragma unroll
+                for (int rb = 0; rb < 4; rb++) {
+                    a[rb][0] = an[rb][0];
+                    a[rb][1] = an[rb][1];
+                }
             }
         }
         int32_t* ps = k.partSums + uint64_t(ch) * 16 * kDims;
@@ -663,9 +679,9 @@ void launchKmLevelReset(const KmLevel& k, uint64_t nbRows, hipStream_t s) {
 }
 
 void launchGatherTiles(const uint8_t* desc, const uint32_t* order, const KmTile* tiles, uint32_t nbTiles, int8_t* XA,
-                       int8_t* XT, hipStream_t s) {
+                       hipStream_t s) {
     if (!nbTiles) return;
-    k_gatherTiles<<<nbTiles, 256, 0, s>>>(desc, order, tiles, XA, XT);
+    k_gatherTiles<<<nbTiles, 256, 0, s>>>(desc, order, tiles, XA);
     RBOW_HIP(hipGetLastError());
 }
 

@@ -48,7 +48,7 @@ uint64_t Server::submit(const uint8_t* desc, uint32_t n) {
     Request r;
     r.n = n;
     r.desc.assign(desc, desc + uint64_t(n) * kDims);
-    r.bytes = db_.scratchBytes(n) + uint64_t(n) * (kDims + 4);
+    r.bytes = db_.scratchBytes(n, opt_.K) + uint64_t(n) * (kDims + 4);
     RBOW_REQUIRE(r.bytes <= opt_.memBudget, RBOW_ERR_CAPACITY, "query scratch exceeds the server's memory budget");
     r.arriveMs = nowMs();
     uint64_t t;

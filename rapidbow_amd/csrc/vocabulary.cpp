@@ -144,7 +144,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     DevBuf<long long> dSums;
     DevBuf<KmTile> dTiles;
     DevBuf<KmChunk> dChunks, dRed;
-    DevBuf<int8_t> dXA, dXT;
+    DevBuf<int8_t> dXA;
     DevBuf<int32_t> dPartSums;
     DevBuf<uint32_t> dPartCnt, dPartChanged, dTaskChunkBeg, dTaskChunkCnt;
     std::vector<KmChunk> chunks, red;
@@ -206,7 +206,6 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             dTaskChunkBeg.reserve(nG);
             dTaskChunkCnt.reserve(nG);
             dXA.reserve(std::max<size_t>(tiles.size(), 1) * 8192);
-            dXT.reserve(std::max<size_t>(tiles.size(), 1) * 8192);
             dPartSums.reserve(std::max<size_t>(chunks.size(), 1) * 16 * kDims);
             dPartCnt.reserve(std::max<size_t>(chunks.size(), 1) * 16);
             dPartChanged.reserve(std::max<size_t>(chunks.size(), 1));
@@ -214,7 +213,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             h2d(dRed.get(), red.data(), red.size(), s);
             h2d(dTaskChunkBeg.get(), taskChunkBeg.data(), nG, s);
             h2d(dTaskChunkCnt.get(), taskChunkCnt.data(), nG, s);
-            launchGatherTiles(devDesc, dOrder.get(), dTiles.get(), uint32_t(tiles.size()), dXA.get(), dXT.get(), s);
+            launchGatherTiles(devDesc, dOrder.get(), dTiles.get(), uint32_t(tiles.size()), dXA.get(), s);
         } else {
             launchGatherRows(devDesc, dOrder.get(), nbDesc, dX.get(), s);
         }
@@ -239,7 +238,6 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         k.nbActive = dNbActive.get();
         if (mfmaPath) {
             k.XA = dXA.get();
-            k.XT = dXT.get();
             k.chunks = dChunks.get();
             k.nbChunks = uint32_t(chunks.size());
             k.taskChunkBeg = dTaskChunkBeg.get();

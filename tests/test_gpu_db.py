@@ -215,3 +215,46 @@ def test_db_score_error_vs_fp64():
                 worst = max(worst, abs(float(s) - ref[int(d)]))
     print(f"max |GPU score - FP64 ideal| = {worst:.3e}")
     assert worst < 1e-5
+
+
+def test_db_many_document_ranges():
+    """More documents than one scoring tile holds (several document ranges), long
+    skewed posting lists with duplicates, and sparse subtrees whose inner nodes
+    fall under the 16+4K threshold (DataBase.cpp:125-164)."""
+    voc, w = _voc_and_weights(3, 16, seed=8)
+    rng = np.random.default_rng(9)
+    nb = 3 * 4096 + 123  # four ranges, the last one partial
+    hot = rng.integers(0, 1024, 12)
+    docs = {}
+    for d in range(nb):
+        n = int(rng.integers(20, 80))
+        u = rng.random(n)
+        x = np.where(u < 0.15, rng.choice(hot, n),
+                     np.where(u < 0.9, rng.integers(0, 1024, n), rng.integers(1024, 4096, n)))
+        if d % 7 == 0:
+            x[: n // 2] = x[0]  # a run of duplicates of one leaf
+        docs[d] = x.astype(np.uint32)
+    g = GpuDb(voc)
+    o = oracle.DataBase(16, 3, w)
+    ids = np.array([d * 3 + 1 for d in range(nb)], np.uint32)
+    for d in range(0, 1000):
+        g.add(int(ids[d]), docs[d])
+    cat = np.concatenate([docs[d] for d in range(1000, nb)])
+    off = np.concatenate([[0], np.cumsum([docs[d].size for d in range(1000, nb)])]).astype(np.uint64)
+    g.add_batch(ids[1000:], cat, off)
+    for d in range(nb):
+        o.add(int(ids[d]), docs[d])
+    for leaf in list(hot[:3]) + list(rng.integers(0, 4096, 20)):
+        np.testing.assert_array_equal(g.postings(int(leaf)), o.postings(int(leaf)))
+    qs = [docs[int(i)] for i in rng.integers(0, nb, 12)]
+    qs.append(rng.integers(1024, 4096, 40).astype(np.uint32))  # cold region: inner nodes scored
+    qs.append(np.concatenate([rng.choice(hot, 300), rng.integers(0, 4096, 500)]).astype(np.uint32))
+    for qi, q in enumerate(qs):
+        for K in (60, 7, 300):
+            gd, gs = g.query(q, K)
+            od, os_ = o.query(q, K)
+            _assert_ranked_equal(gd, gs, od, os_)
+    res = g.query_batch(np.concatenate(qs), np.concatenate([[0], np.cumsum([x.size for x in qs])]).astype(np.uint64), 60)
+    for q, (gd, gs) in zip(qs, res):
+        od, os_ = o.query(q, 60)
+        _assert_ranked_equal(gd, gs, od, os_)
