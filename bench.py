@@ -4,8 +4,12 @@ Default workload (a "step"): the batched vocabulary build -- hierarchical
 k-means, branch 16, depth 4 (65,536 words), 2,000,000 SIFT-like uint8[128]
 descriptors, 10 Lloyd passes per node, k-means++ seeds, every node of a level
 in the same launches.  `value` is the k-means assign throughput in tera-ops/s
-(2*16*128 int8 multiply-adds per point per pass) of the I8-MFMA kernel, from
-HIP events around the assign launches inside the timed builds.
+(2*16*128 int8 multiply-adds per point per pass) of the I8-MFMA assign step
+(assign kernel + reduction of its per-chunk partials), from HIP events around
+those launches inside the timed builds.  The roofline object is the assign
+kernel's: 129 algorithmic bytes per point-pass (128 in + 1 assignment) over its
+average launch time, against HBM (its intensity, 32 int8 ops per byte, is far
+below the I8 MFMA ridge of ~625 ops/byte).
 
 Beside it on the same JSON line:
   * the same build with the plain scalar kernel (assign TOPS, "MFMA vs plain");
@@ -44,6 +48,8 @@ METRIC = "leaf lookups/s & k-means assign TOPS (MFMA vs plain); query p50/p99 la
 I8_PEAK_TOPS = 5000.0     # MI355X dense int8 MFMA, 5.0 POPS
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, 8 TB/s
 LOOKUP_BYTES = 132        # algorithmic bytes per descriptor looked up: 128 in + 4 out
+ASSIGN_BYTES = 129        # algorithmic bytes per point and Lloyd pass: 128 in + 1 assignment
+COPY_BYTES = 1 << 30      # device-to-device copy that calibrates the achievable HBM rate
 CPU_THREADS = 16          # host cores given to the CPU legs (the GPU box's share for one GPU)
 
 # seeds of the synthetic inputs
@@ -64,6 +70,33 @@ def timed_lookup(rb, voc, q, n, out, kernel, reps, ties=None):
     e1.record()
     e1.synchronize()
     return n * reps / (e0.elapsed_ms(e1) * 1e-3)
+
+
+def copy_rate(rb, reps=5):
+    """Achievable HBM rate: a 1 GiB device-to-device copy (read + write bytes / time)."""
+    src = rb.DeviceArray((COPY_BYTES,), np.uint8)
+    dst = rb.DeviceArray((COPY_BYTES,), np.uint8)
+    dst.copy_from(src)
+    rb.hip.synchronize()
+    e0, e1 = rb.Event(), rb.Event()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_from(src)
+    e1.record()
+    e1.synchronize()
+    return 2.0 * COPY_BYTES * reps / (e0.elapsed_ms(e1) * 1e-3) / 1e9
+
+
+def pmc_traffic():
+    """HBM bytes per assign launch from the committed rocprofv3 counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", "kmeans_assign_b16L4_2M_counters.md")
+    try:
+        for ln in open(path):
+            if ln.startswith("traffic_bytes_per_launch:"):
+                return float(ln.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(rb, oracle, desc, threads):
@@ -246,7 +279,7 @@ def bench_serving(rb, syn, v32, rates, duration_s):
                    "mean_batch": round(float(r["batch_sizes"].mean()), 2)}
 
     batched = rb.Server(v32, db, max_batch=64, nb_streams=3, mem_budget=8 << 30, max_results=K,
-                        lookup_kernel=rb.LOOKUP_MFMA)
+                        lookup_kernel=rb.LOOKUP_AUTO)
     single = rb.Server(v32, db, max_batch=1, nb_streams=1, mem_budget=8 << 30, max_results=K,
                        lookup_kernel=rb.LOOKUP_PLAIN)
     run(batched, 64, 1000.0)  # warm both
@@ -311,13 +344,21 @@ def main():
         log(f"bench: step {i}: build {st['total_ms']:.1f} ms, assign {st['assign_ms']:.2f} ms")
     hip.synchronize()
     wall = time.perf_counter() - t0
-    assign_ms = float(np.median([s["assign_ms"] for s in stats]))
+    # the assign step = the assign kernel + the reduction of its chunk partials
+    step_ms = float(np.median([s["assign_ms"] + s["reduce_ms"] for s in stats]))
     ops = stats[-1]["assign_ops"]
-    tops = ops / (assign_ms * 1e-3) / 1e12
+    tops = ops / (step_ms * 1e-3) / 1e12
+    # roofline of the dominant kernel (the assign kernel alone): algorithmic bytes / its time
+    st = min(stats, key=lambda x: x["assign_ms"])
+    launch_us = st["assign_ms"] * 1e3 / st["assign_launches"]
+    bytes_per_launch = st["point_passes"] * ASSIGN_BYTES / st["assign_launches"]
+    assign_gbs = bytes_per_launch / (launch_us * 1e-6) / 1e9
+    copy_gbs = copy_rate(rb)
+    log(f"bench: assign kernel {launch_us:.1f} us/launch, {assign_gbs:.0f} GB/s algorithmic; copy {copy_gbs:.0f} GB/s")
     log("bench: plain-kernel build")
     vplain = rb.build_vocabulary(d, N, 1000, 4, max_iters=10, seed_mode=1, kernel=1)
     stp = vplain.build_stats()
-    plain_tops = stp["assign_ops"] / (stp["assign_ms"] * 1e-3) / 1e12
+    plain_tops = stp["assign_ops"] / ((stp["assign_ms"] + stp["reduce_ms"]) * 1e-3) / 1e12
     same_tree = vplain.serialize() == voc.serialize()
 
     # ---------------- 1M-word float32 tree, 10M lookups ----------------
@@ -357,8 +398,13 @@ def main():
         "config": {"workload": "batched vocab build: b16 L4 (65,536 words), 2M SIFT-like uint8[128], "
                                "10 Lloyd passes/node, k-means++, every node of a level per launch",
                    "nb_desc": N, "branch": 16, "levels": 4, "passes": 10},
-        "roofline": {"bound": "mfma", "achieved": round(tops, 3), "peak": I8_PEAK_TOPS, "unit": "TFLOP/s",
-                     "frac": round(tops / I8_PEAK_TOPS, 5), "traffic": None},
+        "roofline": {"bound": "hbm", "achieved": round(assign_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(assign_gbs / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(),
+                     "kernel": "k_assignU8Mfma", "avg_launch_us": round(launch_us, 2),
+                     "algorithmic_bytes_per_launch": round(bytes_per_launch), "copy_gbs": round(copy_gbs, 1)},
+        "assign_tops_frac_of_i8_peak": round(tops / I8_PEAK_TOPS, 5),
+        "assign_step_ms": {"assign": round(st["assign_ms"], 3), "reduce": round(st["reduce_ms"], 3),
+                           "update": round(st["update_ms"], 3), "launches": st["assign_launches"]},
         "assign_tops_plain": round(plain_tops, 3),
         "build_ms": {"mfma": round(float(np.median([s["total_ms"] for s in stats])), 2),
                      "plain": round(stp["total_ms"], 2), "identical_trees": same_tree,

@@ -94,7 +94,7 @@ rbow_status rbow_voc_lookup(const rbow_vocabulary* voc, const void* dev_desc, ui
                             rbow_stream stream);
 
 typedef enum {
-    RBOW_LOOKUP_AUTO = 0,  /* MFMA batched form */
+    RBOW_LOOKUP_AUTO = 0,  /* plain below ~128k descriptors (uint8 trees) / ~32k (float trees), else MFMA */
     RBOW_LOOKUP_PLAIN = 1, /* plain scalar per-descriptor descent */
     RBOW_LOOKUP_MFMA = 2   /* level-synchronous bucketed MFMA form */
 } rbow_lookup_kernel;
@@ -166,7 +166,7 @@ typedef struct {
     uint32_t nb_streams;   /* batches in flight (default 2) */
     uint64_t mem_budget;   /* HBM scratch of admitted queries, bytes (default 4 GiB) */
     uint32_t max_results;  /* K (default 50, <= 1024) */
-    int32_t lookup_kernel; /* RBOW_LOOKUP_MFMA (default) or RBOW_LOOKUP_PLAIN */
+    int32_t lookup_kernel; /* RBOW_LOOKUP_AUTO (default), RBOW_LOOKUP_MFMA or RBOW_LOOKUP_PLAIN, per batch */
 } rbow_server_options;
 void rbow_server_options_default(rbow_server_options* opts);
 rbow_status rbow_server_create(const rbow_vocabulary* voc, const rbow_database* db, const rbow_server_options* opts,
@@ -224,11 +224,14 @@ const uint32_t* rbow_voc_build_passes(const rbow_vocabulary* voc);
 /* Timing of the last build (zeros for deserialised vocabularies).
  * assign_ops = point_passes * 2 * b * 128 (the distance contraction). */
 typedef struct {
-    double total_ms;       /* host wall time of the build */
-    double assign_ms;      /* summed assign-kernel time (HIP events) */
-    double update_ms;      /* summed update-kernel time (HIP events) */
-    uint64_t point_passes; /* sum over nodes of points x Lloyd passes */
+    double total_ms;         /* host wall time of the build */
+    double assign_ms;        /* summed assign-kernel time (HIP events around each assign launch) */
+    double update_ms;        /* summed update-kernel time (HIP events) */
+    uint64_t point_passes;   /* sum over nodes of points x Lloyd passes */
     uint64_t assign_ops;
+    double reduce_ms;        /* summed time of the chunk-partial reductions (MFMA path) */
+    uint32_t assign_launches;
+    uint32_t reserved;
 } rbow_build_stats;
 rbow_status rbow_voc_build_stats(const rbow_vocabulary* voc, rbow_build_stats* out);
 

@@ -36,7 +36,8 @@ SIFT_ATTR = DescriptorAttributes(1, 2, 128)  # RapidBoW.h:50 SiftAttr
 
 class BuildStats(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("assign_ms", ctypes.c_double), ("update_ms", ctypes.c_double),
-                ("point_passes", ctypes.c_uint64), ("assign_ops", ctypes.c_uint64)]
+                ("point_passes", ctypes.c_uint64), ("assign_ops", ctypes.c_uint64), ("reduce_ms", ctypes.c_double),
+                ("assign_launches", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class ServerOptions(ctypes.Structure):
@@ -357,7 +358,7 @@ class Server:
     """Continuous-batching query server (no counterpart in the reference)."""
 
     def __init__(self, voc, db, max_batch=64, nb_streams=2, mem_budget=4 << 30, max_results=50,
-                 lookup_kernel=LOOKUP_MFMA):
+                 lookup_kernel=LOOKUP_AUTO):
         o = ServerOptions()
         lib.rbow_server_options_default(ctypes.byref(o))
         o.max_batch, o.nb_streams, o.mem_budget, o.max_results, o.lookup_kernel = (

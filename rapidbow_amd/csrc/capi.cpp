@@ -219,8 +219,7 @@ rbow_status rbow_voc_lookup_ex(const rbow_vocabulary* voc, const void* dev_desc,
         RBOW_REQUIRE(kernel >= RBOW_LOOKUP_AUTO && kernel <= RBOW_LOOKUP_MFMA, RBOW_ERR_INVALID_ARG, "bad kernel id");
         const TreeView t = voc->v->view();
         const auto* d = static_cast<const uint8_t*>(dev_desc);
-        if (kernel == RBOW_LOOKUP_PLAIN || t.L == 0) launchLookupPlain(t, d, nb_desc, dev_leaf, hs(stream));
-        else launchLookupMfma(t, d, nb_desc, dev_leaf, dev_near_ties, hs(stream));
+        launchLookup(t, d, nb_desc, dev_leaf, dev_near_ties, kernel, hs(stream));
     });
 }
 
@@ -265,6 +264,9 @@ rbow_status rbow_voc_build_stats(const rbow_vocabulary* voc, rbow_build_stats* o
         out->update_ms = st.updateMs;
         out->point_passes = st.pointPasses;
         out->assign_ops = st.pointPasses * 2ull * voc->v->b * kDims;
+        out->reduce_ms = st.reduceMs;
+        out->assign_launches = st.assignLaunches;
+        out->reserved = 0;
     });
 }
 
@@ -441,7 +443,7 @@ void rbow_server_options_default(rbow_server_options* o) {
     o->nb_streams = 2;
     o->mem_budget = 4ull << 30;
     o->max_results = 50;
-    o->lookup_kernel = RBOW_LOOKUP_MFMA;
+    o->lookup_kernel = RBOW_LOOKUP_AUTO;
 }
 
 rbow_status rbow_server_create(const rbow_vocabulary* voc, const rbow_database* db, const rbow_server_options* opts,

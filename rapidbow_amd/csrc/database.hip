@@ -359,7 +359,7 @@ __global__ __launch_bounds__(1024) void k_scoreRange(DbView v, const uint64_t* _
             sW[cx] = w;
         }
         __syncthreads();
-        // ---- walk the postings: wave-contiguous shares, 64 per step ----
+        // ---- walk the postings: wave-contiguous shares, 64 per step, U steps per iteration ----
         const uint32_t per = ((total + 16 * 64 - 1) / (16 * 64)) * 64;
         const uint32_t wEnd = min(total, wave * per + per);
         uint32_t seg = 0;
@@ -372,36 +372,63 @@ __global__ __launch_bounds__(1024) void k_scoreRange(DbView v, const uint64_t* _
             }
             seg = lo;
         }
-        for (uint32_t base = wave * per; base < wEnd; base += 64) {
-            const uint32_t idx = base + lane;
-            const bool valid = idx < wEnd;
-            uint32_t sg = seg;
-            if (valid) {
-                uint32_t lo = seg, hi = min(seg + 64u, nc - 1);  // a step spans at most 64 segments
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) / 2;
-                    if (sSegEnd[mid] > idx) hi = mid;
-                    else lo = mid + 1;
+        constexpr int U = 4;  // steps of 64 postings per iteration: U loads in flight per lane
+        for (uint32_t base = wave * per; base < wEnd; base += 64 * U) {
+            uint32_t sgs[U], slots[U], is[U], prev0 = 0;
+            bool valids[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {  // segments and loads of all steps first
+                const uint32_t idx = base + 64 * u + lane;
+                valids[u] = idx < wEnd;
+                uint32_t sg = seg;
+                if (valids[u]) {
+                    uint32_t lo = seg, hi = min(seg + 64u, nc - 1);  // a step spans at most 64 segments
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) / 2;
+                        if (sSegEnd[mid] > idx) hi = mid;
+                        else lo = mid + 1;
+                    }
+                    sg = lo;
                 }
-                sg = lo;
+                sgs[u] = sg;
+                is[u] = idx - (sg ? sSegEnd[sg - 1] : 0u);
+                const uint32_t* p = v.post + sSegBeg[sg];
+                slots[u] = valids[u] ? p[is[u]] : 0xFFFFFFFFu;
+                if (u == 0 && lane == 0 && valids[0] && is[0] > 0) prev0 = p[is[0] - 1];
+                seg = uint32_t(__shfl(int(sg), 63, 64));
             }
-            const uint32_t segA = sg ? sSegEnd[sg - 1] : 0u, segLen = sSegEnd[sg] - segA, i = idx - segA;
-            const uint32_t* p = v.post + sSegBeg[sg];
-            const uint32_t slot = valid ? p[i] : 0xFFFFFFFFu;
-            const uint32_t prevSlot = uint32_t(__shfl_up(int(slot), 1, 64));
-            const uint32_t prevSg = uint32_t(__shfl_up(int(sg), 1, 64));
-            bool head = false;
-            if (valid) {
-                if (i == 0) head = true;
-                else if (lane != 0 && prevSg == sg) head = prevSlot != slot;
-                else head = p[i - 1] != slot;
+            uint64_t bnd[U];
+            bool heads[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {  // run starts: a new segment or a new slot
+                const uint32_t up = uint32_t(__shfl_up(int(slots[u]), 1, 64));
+                const uint32_t last = u ? uint32_t(__shfl(int(slots[u ? u - 1 : 0]), 63, 64)) : prev0;
+                const uint32_t before = lane ? up : last;  // same segment whenever is[u] > 0
+                heads[u] = valids[u] && (is[u] == 0 || before != slots[u]);
+                bnd[u] = __ballot(heads[u] || !valids[u]);
             }
-            const uint64_t bound = __ballot(head || !valid);
-            if (head) {
-                const uint64_t above = lane == 63 ? 0ull : (bound >> (lane + 1));
-                uint32_t m = above ? uint32_t(__builtin_ctzll(above)) + 1 : 64 - lane;
-                if (lane + m >= 64 || idx + m >= wEnd) {
-                    // no run start follows in this step: the run may go on past it or past the share
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (!heads[u]) continue;
+                // run length: distance to the next boundary over this iteration's steps
+                const uint64_t above = lane == 63 ? 0ull : (bnd[u] >> (lane + 1));
+                bool found = above != 0;
+                uint32_t m = found ? uint32_t(__builtin_ctzll(above)) + 1 : 64 - lane;
+#pragma unroll
+                for (int w2 = u + 1; w2 < U; w2++) {
+                    if (found) break;
+                    if (bnd[w2]) {
+                        m += uint32_t(__builtin_ctzll(bnd[w2]));
+                        found = true;
+                    } else {
+                        m += 64;
+                    }
+                }
+                const uint32_t sg = sgs[u], slot = slots[u], i = is[u];
+                if (!found || base + 64 * u + lane + m >= wEnd) {
+                    // no run start follows in this iteration: the run may go on past it or past the share
+                    const uint32_t segLen = sSegEnd[sg] - (sg ? sSegEnd[sg - 1] : 0u);
+                    const uint32_t* p = v.post + sSegBeg[sg];
                     while (i + m < segLen && p[i + m] == slot) m++;
                 }
                 const uint32_t j = slot - s0;
@@ -411,7 +438,6 @@ __global__ __launch_bounds__(1024) void k_scoreRange(DbView v, const uint64_t* _
                 atomicAdd(&sAcc[j], double(c));
                 sTouched[j] = 1;
             }
-            seg = uint32_t(__shfl(int(sg), 63, 64));
         }
         __syncthreads();  // segment arrays are reused by the next round
     }

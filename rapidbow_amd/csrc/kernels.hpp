@@ -39,6 +39,10 @@ void launchLookupPlain(const TreeView& t, const uint8_t* desc, uint32_t n, uint3
 // certified FP32 re-score for float trees).  Allocates its workspace stream-ordered.
 void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
                       hipStream_t s);
+// Kernel choice (RBOW_LOOKUP_*): AUTO takes the plain kernel below the batch
+// size where the bucketed MFMA form's per-level passes start to pay off.
+void launchLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies, int kernel,
+                  hipStream_t s);
 
 // ----------------------------------------------------------------- build --
 // Gather rows: out[p] = desc[idx[p]] (idx == kIdxInvalid -> 0xFF bytes).

@@ -293,6 +293,7 @@ __device__ inline void loadTileA(const int8_t* xa, uint32_t lane, v4i (&a)[4][2]
 }
 
 __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
+    __shared__ uint32_t sNew[kWaves][16];  // a tile's new assignments, one byte per row
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 15, grp = lane >> 4;
     // esel[g]: B operand that selects dims g*16 + col of a 64-dim K chunk
@@ -321,11 +322,19 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
         for (int db = 0; db < 8; db++) acc[db] = zero;
         uint32_t myCnt = 0, myChanged = 0;
         v4i a[4][2];
+        KmTile t = k.tiles[c.tileBeg];
         loadTileA(k.XA + uint64_t(c.tileBeg) * 8192, lane, a);
+        uint32_t prev = lane < t.cnt ? k.assign[t.row + lane] : 0u;  // lane r: previous assignment of row r
         for (uint32_t ti = c.tileBeg; ti < c.tileEnd; ti++) {
-            const KmTile t = k.tiles[ti];
-            v4i an[4][2];  // next tile's fragments, in flight while this one computes
-            if (ti + 1 < c.tileEnd) loadTileA(k.XA + uint64_t(ti + 1) * 8192, lane, an);
+            // next tile's fragments and previous assignments, in flight while this one computes
+            v4i an[4][2];
+            KmTile tn = t;
+            uint32_t prevN = 0;
+            if (ti + 1 < c.tileEnd) {
+                tn = k.tiles[ti + 1];
+                loadTileA(k.XA + uint64_t(ti + 1) * 8192, lane, an);
+                prevN = lane < tn.cnt ? k.assign[tn.row + lane] : 0u;
+            }
             uint32_t key[4][4];
 #pragma unroll
             for (int rb = 0; rb < 4; rb++) {
@@ -339,6 +348,20 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
             for (int rb = 0; rb < 4; rb++)
 #pragma unroll
                 for (int i = 0; i < 4; i++) key[rb][i] = min16u(key[rb][i]);
+            // new assignments to lane-per-row order (rows rb*16 + 4*grp + i live in group grp)
+            if (col == 0) {
+#pragma unroll
+                for (int rb = 0; rb < 4; rb++)
+                    sNew[wave][rb * 4 + grp] = (key[rb][0] & 15u) | ((key[rb][1] & 15u) << 8) |
+                                               ((key[rb][2] & 15u) << 16) | ((key[rb][3] & 15u) << 24);
+            }
+            waveSync();
+            const uint32_t mine = (sNew[wave][lane >> 2] >> (8 * (lane & 3))) & 15u;
+            waveSync();
+            if (lane < t.cnt) {
+                myChanged += prev != mine;
+                k.assign[t.row + lane] = uint8_t(mine);
+            }
             v4i oh;
 #pragma unroll
             for (int rb = 0; rb < 4; rb++) {
@@ -346,14 +369,7 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     const uint32_t row = rb * 16 + 4 * grp + i;
-                    const uint32_t as = key[rb][i] & 15u;
-                    const bool valid = row < t.cnt;
-                    if (valid && col == uint32_t(i)) {  // one writer per row
-                        const uint32_t p = t.row + row;
-                        myChanged += k.assign[p] != as;
-                        k.assign[p] = uint8_t(as);
-                    }
-                    w |= uint32_t(valid && as == col) << (8 * i);
+                    w |= uint32_t(row < t.cnt && (key[rb][i] & 15u) == col) << (8 * i);
                 }
                 oh[rb] = int(w);
             }
@@ -374,6 +390,8 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
                     a[rb][1] = an[rb][1];
                 }
             }
+            t = tn;
+            prev = prevN;
         }
         int32_t* ps = k.partSums + uint64_t(ch) * 16 * kDims;
 #pragma unroll
@@ -390,30 +408,25 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
 }
 
 // Reduce items: (task, chunk range); adds the partials into the task totals.
+// Grid (item, 8): block y owns elements [256y, 256y + 256) of the 16 x 128 sums.
 __global__ __launch_bounds__(256) void k_reduceChunks(KmLevel k, const KmChunk* __restrict__ red, uint32_t nbRed) {
-    __shared__ uint32_t sCnt[16];
     const KmChunk r = red[blockIdx.x];
-    const uint32_t g = r.task, tid = threadIdx.x;
+    const uint32_t g = r.task, tid = threadIdx.x, e = blockIdx.y * 256 + tid, center = e / kDims;
     if (!k.active[g]) return;
-    if (tid < 16) {
+    if (blockIdx.y == 0 && tid <= 16) {
         uint32_t c = 0;
-        for (uint32_t ch = r.tileBeg; ch < r.tileEnd; ch++) c += k.partCnt[ch * 16 + tid];
-        sCnt[tid] = c;
-        if (c) atomicAdd(&k.counts[g * 16 + tid], c);
+        for (uint32_t ch = r.tileBeg; ch < r.tileEnd; ch++) c += tid < 16 ? k.partCnt[ch * 16 + tid] : k.partChanged[ch];
+        if (c) atomicAdd(tid < 16 ? &k.counts[g * 16 + tid] : &k.changed[g], c);
     }
-    if (tid == 0) {
-        uint32_t c = 0;
-        for (uint32_t ch = r.tileBeg; ch < r.tileEnd; ch++) c += k.partChanged[ch];
-        if (c) atomicAdd(&k.changed[g], c);
+    long long s = 0;
+    uint32_t cnt = 0;
+#pragma unroll 8
+    for (uint32_t ch = r.tileBeg; ch < r.tileEnd; ch++) {
+        s += k.partSums[uint64_t(ch) * 16 * kDims + e];
+        cnt += k.partCnt[ch * 16 + center];
     }
-    __syncthreads();
-    unsigned long long* gs = reinterpret_cast<unsigned long long*>(k.sums) + uint64_t(g) * 16 * kDims;
-    for (uint32_t e = tid; e < 16 * kDims; e += 256) {
-        long long s = 0;
-        for (uint32_t ch = r.tileBeg; ch < r.tileEnd; ch++) s += k.partSums[uint64_t(ch) * 16 * kDims + e];
-        s += 128ll * sCnt[e / kDims];  // partials are sums of x - 128
-        if (s) atomicAdd(&gs[e], (unsigned long long)s);
-    }
+    s += 128ll * cnt;  // partials are sums of x - 128
+    if (s) atomicAdd(reinterpret_cast<unsigned long long*>(k.sums) + uint64_t(g) * 16 * kDims + e, (unsigned long long)s);
 }
 
 // ----------------------------------------------- assign: uint8, plain ----
@@ -564,60 +577,52 @@ __global__ __launch_bounds__(256) void k_update(KmLevel k) {
     }
 }
 
-// Update of the MFMA path: tasks with <= 32 chunks reduce their partials here
-// (no atomics); larger tasks were reduced into the totals by k_reduceChunks.
+// Update of the MFMA path, grid (task, 8): block y owns centers 2y, 2y+1, i.e.
+// elements [256y, 256y + 256) of the task's 16 x 128 sums.  Tasks with <= 32
+// chunks reduce their partials here (no atomics); larger tasks were reduced
+// into the totals by k_reduceChunks.  #changed is summed from the per-chunk
+// counts by every block, so no block depends on another's writes.
 __global__ __launch_bounds__(256) void k_updateU8Part(KmLevel k) {
-    __shared__ uint32_t sCnt[16];
-    __shared__ uint32_t sChanged;
-    const uint32_t g = blockIdx.x, tid = threadIdx.x;
+    __shared__ uint32_t sWave[kWaves], sCnt[2];
+    const uint32_t g = blockIdx.x, y = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (!k.active[g]) return;
     const uint32_t cb = k.taskChunkBeg[g], nc = k.taskChunkCnt[g];
     const bool direct = nc <= 32;
-    if (tid < 16) {
+    uint32_t ch = 0;
+    for (uint32_t c = tid; c < nc; c += 256) ch += k.partChanged[cb + c];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) ch += uint32_t(__shfl_xor(int(ch), m, 64));
+    if (lane == 0) sWave[wave] = ch;
+    if (tid < 2) {
+        const uint32_t center = 2 * y + tid;
         uint32_t c = 0;
         if (direct)
-            for (uint32_t ch = cb; ch < cb + nc; ch++) c += k.partCnt[ch * 16 + tid];
+            for (uint32_t x = cb; x < cb + nc; x++) c += k.partCnt[x * 16 + center];
         else
-            c = k.counts[g * 16 + tid];
+            c = k.counts[g * 16 + center];
         sCnt[tid] = c;
     }
-    if (tid == 0) {
-        uint32_t c = 0;
-        if (direct)
-            for (uint32_t ch = cb; ch < cb + nc; ch++) c += k.partChanged[ch];
-        else
-            c = k.changed[g];
-        sChanged = c;
-    }
     __syncthreads();
-    unsigned long long* sums = reinterpret_cast<unsigned long long*>(k.sums) + uint64_t(g) * 16 * kDims;
-    if (sChanged == 0) {  // converged on this pass: no update, task stops
-        if (tid == 0) {
+    if (sWave[0] + sWave[1] + sWave[2] + sWave[3] == 0) {  // converged on this pass: no update, task stops
+        if (y == 0 && tid == 0) {
             k.active[g] = 0;
             k.passes[g]++;
         }
-        if (!direct) {
-            for (uint32_t e = tid; e < 16 * kDims; e += 256) sums[e] = 0;
-            if (tid < 16) k.counts[g * 16 + tid] = 0;
-            if (tid == 0) k.changed[g] = 0;
-        }
         return;
     }
-    for (uint32_t e = tid; e < 16 * kDims; e += 256) {
-        const uint32_t c = sCnt[e / kDims];
-        long long sum;
-        if (direct) {
-            sum = 128ll * c;  // partials are sums of x - 128
-            for (uint32_t ch = cb; ch < cb + nc; ch++) sum += k.partSums[uint64_t(ch) * 16 * kDims + e];
-        } else {
-            sum = (long long)sums[e];
-            sums[e] = 0;
-        }
-        if (c) k.centersU8[uint64_t(g) * 16 * kDims + e] = uint8_t(roundf(float(sum) / float(c)));
+    const uint32_t e = y * 256 + tid, c = sCnt[tid / kDims];
+    long long sum;
+    if (direct) {
+        sum = 128ll * c;  // partials are sums of x - 128
+        for (uint32_t x = cb; x < cb + nc; x++) sum += k.partSums[uint64_t(x) * 16 * kDims + e];
+    } else {
+        unsigned long long* sums = reinterpret_cast<unsigned long long*>(k.sums) + uint64_t(g) * 16 * kDims;
+        sum = (long long)sums[e];
+        sums[e] = 0;
     }
-    if (!direct && tid < 16) k.counts[g * 16 + tid] = 0;
-    if (tid == 0) {
-        if (!direct) k.changed[g] = 0;
+    if (c) k.centersU8[uint64_t(g) * 16 * kDims + e] = uint8_t(roundf(float(sum) / float(c)));
+    if (!direct && tid < 2) k.counts[g * 16 + 2 * y + tid] = 0;  // read above by this thread only
+    if (y == 0 && tid == 0) {
         k.passes[g]++;
         atomicAdd(k.nbActive, 1u);
     }
@@ -687,7 +692,7 @@ void launchGatherTiles(const uint8_t* desc, const uint32_t* order, const KmTile*
 
 void launchKmReduce(const KmLevel& k, const KmChunk* red, uint32_t nbRed, hipStream_t s) {
     if (!nbRed) return;
-    k_reduceChunks<<<nbRed, 256, 0, s>>>(k, red, nbRed);
+    k_reduceChunks<<<dim3(nbRed, 16 * kDims / 256), 256, 0, s>>>(k, red, nbRed);
     RBOW_HIP(hipGetLastError());
 }
 
@@ -707,7 +712,7 @@ void launchKmAssign(const KmLevel& k, int kernel, hipStream_t s) {
 
 void launchKmUpdate(const KmLevel& k, hipStream_t s) {
     if (!k.nbTasks) return;
-    if (k.partSums && k.centroidType == 0) k_updateU8Part<<<k.nbTasks, 256, 0, s>>>(k);
+    if (k.partSums && k.centroidType == 0) k_updateU8Part<<<dim3(k.nbTasks, 16 * kDims / 256), 256, 0, s>>>(k);
     else if (k.b == 32) k_update<32><<<k.nbTasks, 256, 0, s>>>(k);
     else k_update<16><<<k.nbTasks, 256, 0, s>>>(k);
     RBOW_HIP(hipGetLastError());

@@ -720,4 +720,13 @@ void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32
     RBOW_HIP(hipFreeAsync(ws, s));
 }
 
+void launchLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies, int kernel,
+                  hipStream_t s) {
+    // measured crossovers on MI355X: ~150k descriptors for uint8 trees, ~40k for float trees
+    const uint32_t autoMin = t.centroidType == 0 ? 131072u : 32768u;
+    const bool plain = kernel == RBOW_LOOKUP_PLAIN || t.L == 0 || (kernel == RBOW_LOOKUP_AUTO && n < autoMin);
+    if (plain) launchLookupPlain(t, desc, n, leaf, s);
+    else launchLookupMfma(t, desc, n, leaf, nearTies, s);
+}
+
 }  // namespace rbow_amd

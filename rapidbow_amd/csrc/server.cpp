@@ -129,12 +129,9 @@ void Server::launch(Slot& s, std::vector<Request>&& batch) {
     if (D) RBOW_HIP(hipMemcpyAsync(dDesc, s.hDesc, D * kDims, hipMemcpyHostToDevice, st));
     RBOW_HIP(hipMemcpyAsync(dOff, s.hOff.data(), 8ull * (B + 1), hipMemcpyHostToDevice, st));
     const TreeView t = voc_.view();
-    if (D) {
-        if (opt_.lookupKernel == RBOW_LOOKUP_PLAIN)
-            launchLookupPlain(t, static_cast<uint8_t*>(dDesc), uint32_t(D), static_cast<uint32_t*>(dLeaf), st);
-        else
-            launchLookupMfma(t, static_cast<uint8_t*>(dDesc), uint32_t(D), static_cast<uint32_t*>(dLeaf), nullptr, st);
-    }
+    if (D)
+        launchLookup(t, static_cast<uint8_t*>(dDesc), uint32_t(D), static_cast<uint32_t*>(dLeaf), nullptr,
+                     opt_.lookupKernel, st);
     db_.scoreAsync(static_cast<uint32_t*>(dLeaf), static_cast<uint64_t*>(dOff), s.hOff.data(), B, K,
                    static_cast<uint32_t*>(dDocs), static_cast<float*>(dScores), static_cast<uint32_t*>(dCounts), st);
     RBOW_HIP(hipMemcpyAsync(s.hDocs, dDocs, 4ull * B * K, hipMemcpyDeviceToHost, st));

@@ -27,7 +27,7 @@ struct ServerOptions {
     uint32_t nbStreams = 2;        // batches in flight
     uint64_t memBudget = 4ull << 30;  // HBM scratch of in-flight queries (bytes)
     uint32_t K = 50;
-    int lookupKernel = 2;          // RBOW_LOOKUP_MFMA / RBOW_LOOKUP_PLAIN
+    int lookupKernel = RBOW_LOOKUP_AUTO;  // RBOW_LOOKUP_AUTO / _MFMA / _PLAIN, per batch
 };
 
 class Server {

@@ -124,7 +124,8 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         }
         RBOW_HIP(hipEventRecord(ev[i], s));
     };
-    double assignMs = 0, updateMs = 0;
+    double assignMs = 0, updateMs = 0, reduceMs = 0;
+    uint32_t assignLaunches = 0;
     uint64_t pointPasses = 0;
 
     std::unique_ptr<Vocabulary> voc(new Vocabulary);
@@ -285,14 +286,15 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         const bool poll = opts.max_iters > 16;
         uint32_t nbLaunched = 0;
         for (uint32_t it = 0; it < opts.max_iters; it++) {
-            if (stats) mark(4 * it);
+            if (stats) mark(5 * it);
             launchKmAssign(k, kernel, s);
+            if (stats) mark(5 * it + 1);
             if (mfmaPath) launchKmReduce(k, dRed.get(), uint32_t(red.size()), s);
-            if (stats) mark(4 * it + 1);
+            if (stats) mark(5 * it + 2);
             if (poll) launchFillU32(dNbActive.get(), 0u, 1, s);
-            if (stats) mark(4 * it + 2);
+            if (stats) mark(5 * it + 3);
             launchKmUpdate(k, s);
-            if (stats) mark(4 * it + 3);
+            if (stats) mark(5 * it + 4);
             nbLaunched = it + 1;
             if (poll && (it % 4 == 3)) {  // kmeans.cpp:125-131 polls every 4 passes
                 d2h(hNbActive, dNbActive.get(), 1, s);
@@ -312,11 +314,14 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         }
         if (stats)
             for (uint32_t it = 0; it < nbLaunched; it++) {
-                float a = 0, u = 0;
-                RBOW_HIP(hipEventElapsedTime(&a, ev[4 * it], ev[4 * it + 1]));
-                RBOW_HIP(hipEventElapsedTime(&u, ev[4 * it + 2], ev[4 * it + 3]));
+                float a = 0, r = 0, u = 0;
+                RBOW_HIP(hipEventElapsedTime(&a, ev[5 * it], ev[5 * it + 1]));
+                RBOW_HIP(hipEventElapsedTime(&r, ev[5 * it + 1], ev[5 * it + 2]));
+                RBOW_HIP(hipEventElapsedTime(&u, ev[5 * it + 3], ev[5 * it + 4]));
                 assignMs += a;
+                reduceMs += r;
                 updateMs += u;
+                assignLaunches++;
             }
 
         // ---- stable regroup + occurrence (Vocabulary.cpp:83-98) ----
@@ -352,6 +357,8 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         stats->passes = std::move(passesAll);
         stats->assignMs = assignMs;
         stats->updateMs = updateMs;
+        stats->reduceMs = reduceMs;
+        stats->assignLaunches = assignLaunches;
         stats->pointPasses = pointPasses;
         stats->totalMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }

@@ -30,6 +30,8 @@ struct BuildStats {
     double totalMs = 0;            // wall time of the build (host clock)
     double assignMs = 0;           // sum of assign-kernel times (HIP events)
     double updateMs = 0;           // sum of update-kernel times (HIP events)
+    double reduceMs = 0;           // sum of chunk-reduction times (HIP events)
+    uint32_t assignLaunches = 0;
     uint64_t pointPasses = 0;      // sum over nodes of points x Lloyd passes
 };
 

@@ -114,6 +114,12 @@ class DeviceArray:
             check(_hip.hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, D2H), "hipMemcpy D2H")
         return out
 
+    def copy_from(self, src, nbytes=None, stream=None):
+        """Asynchronous device-to-device copy of nbytes (default: all of src)."""
+        n = src.nbytes if nbytes is None else int(nbytes)
+        assert n <= self.nbytes and n <= src.nbytes
+        check(_hip.hipMemcpyAsync(self.ptr, src.ptr, n, D2D, stream.handle if stream else None), "hipMemcpyAsync D2D")
+
     def view(self, first_row, nb_rows):
         """Non-owning view of rows [first_row, first_row + nb_rows) (leading dimension)."""
         row = self.nbytes // max(self.shape[0], 1) if self.shape else self.dtype.itemsize

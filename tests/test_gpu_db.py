@@ -233,6 +233,8 @@ def test_db_many_document_ranges():
                      np.where(u < 0.9, rng.integers(0, 1024, n), rng.integers(1024, 4096, n)))
         if d % 7 == 0:
             x[: n // 2] = x[0]  # a run of duplicates of one leaf
+        if d % 1000 == 5:  # runs longer than a wave step and than an unrolled iteration
+            x = np.concatenate([x, np.full(70 + d // 10, hot[d % 3])])
         docs[d] = x.astype(np.uint32)
     g = GpuDb(voc)
     o = oracle.DataBase(16, 3, w)
