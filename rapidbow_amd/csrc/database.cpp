@@ -14,8 +14,11 @@ inline float ftz(float x) { return std::fabs(x) < FLT_MIN ? std::copysign(0.0f, 
 Database::Database(const Vocabulary& voc) : voc_(voc), b_(voc.b), lg2b_(voc.lg2b), L_(voc.L) {
     RBOW_REQUIRE(L_ >= 1, RBOW_ERR_INVALID_ARG, "vocabulary has no levels");
     nbLeaves_ = levelSize(lg2b_, L_);
-    W_.resize(voc.weights.size());
-    RBOW_HIP(hipMemcpy(W_.get(), voc.weights.data(), voc.weights.size() * 4, hipMemcpyHostToDevice));
+    // IDF weights as the scoring sees them: subnormals read as zero (the reference's DAZ)
+    Wh_ = voc.weights;
+    for (auto& x : Wh_) x = ftz(x);
+    W_.resize(Wh_.size());
+    RBOW_HIP(hipMemcpy(W_.get(), Wh_.data(), Wh_.size() * 4, hipMemcpyHostToDevice));
     leafOff_.resize(nbLeaves_ + 1);
     RBOW_HIP(hipMemset(leafOff_.get(), 0, leafOff_.bytes()));
     post_.resize(1);
@@ -67,7 +70,7 @@ void Database::addDoc(uint32_t doc, const uint32_t* leaf, uint32_t n) {
     std::sort(nxt.begin(), nxt.end());
     double l1 = 0;
     for (uint32_t t = 0; t < L_; t++) {
-        const float* W = voc_.weights.data() + levelBeg(lg2b_, L_ - 1 - t) * b_;
+        const float* W = Wh_.data() + levelBeg(lg2b_, L_ - 1 - t) * b_;
         for (uint32_t i = 0; i < n; i++) {
             l1 += W[nxt[i]];
             nxt[i] >>= lg2b_;

@@ -26,6 +26,17 @@ __device__ inline float fmulz(float a, float b) { return ftz(ftz(a) * ftz(b)); }
 __device__ inline float faddz(float a, float b) { return ftz(ftz(a) + ftz(b)); }
 __device__ inline float fsubz(float a, float b) { return ftz(ftz(a) - ftz(b)); }
 
+// DataBase.cpp:128-130 for one (node, doc) term, when every input is already
+// normal or zero (IDF weights and 1/L1 norms are flushed when stored, qE when
+// computed): then float(m) * w >= FLT_MIN or 0 and qE + tE >= max(qE, tE), so
+// only the product with rcp and the two differences can leave the normal
+// range, and these are the only flushes kept.  Equal to
+// fsubz(fabsf(fsubz(qE, tE)), faddz(qE, tE)) with tE = fmulz(fmulz(m, w), rcp).
+__device__ inline float termNormal(uint32_t m, float w, float rcp, float qE) {
+    const float tE = ftz(float(m) * w * rcp);
+    return ftz(fabsf(ftz(qE - tE)) - (qE + tE));
+}
+
 struct QNode {
     uint32_t v;      // node index within its level
     uint32_t t;      // depth from the leaf level
@@ -310,7 +321,7 @@ __device__ uint64_t blockSelectKth(KeyAt keyAt, uint32_t n, uint32_t need, uint3
 //   per node, multiplicity by scanning the node's slice.
 // The range then keeps its K smallest (score, doc id) keys as candidates for
 // the query's final selection.
-__global__ __launch_bounds__(1024) void k_scoreRange(DbView v, const uint64_t* __restrict__ off, uint32_t q0,
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_scoreRange(DbView v, const uint64_t* __restrict__ off, uint32_t q0,
                                                     const QNode* __restrict__ nodes,
                                                     const uint32_t* __restrict__ nodeCount, uint32_t K,
                                                     uint64_t* __restrict__ candKeys, uint32_t* __restrict__ candCount) {
@@ -381,7 +392,8 @@ __global__ __launch_bounds__(1024) void k_scoreRange(DbView v, const uint64_t* _
                 const uint32_t idx = base + 64 * u + lane;
                 valids[u] = idx < wEnd;
                 uint32_t sg = seg;
-                if (valids[u]) {
+                // most steps lie inside one segment (wave-uniform test); otherwise search
+                if (sSegEnd[seg] < min(base + 64 * (u + 1), wEnd) && valids[u]) {
                     uint32_t lo = seg, hi = min(seg + 64u, nc - 1);  // a step spans at most 64 segments
                     while (lo < hi) {
                         const uint32_t mid = (lo + hi) / 2;
@@ -432,10 +444,7 @@ __global__ __launch_bounds__(1024) void k_scoreRange(DbView v, const uint64_t* _
                     while (i + m < segLen && p[i + m] == slot) m++;
                 }
                 const uint32_t j = slot - s0;
-                const float qe = sQE[sg];
-                const float tE = fmulz(fmulz(float(m), sW[sg]), sRcp[j]);      // DataBase.cpp:128-129
-                const float c = fsubz(fabsf(fsubz(qe, tE)), faddz(qe, tE));  // DataBase.cpp:130
-                atomicAdd(&sAcc[j], double(c));
+                atomicAdd(&sAcc[j], double(termNormal(m, sW[sg], sRcp[j], sQE[sg])));
                 sTouched[j] = 1;
             }
         }

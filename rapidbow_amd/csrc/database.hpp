@@ -82,6 +82,7 @@ private:
     const Vocabulary& voc_;
     uint32_t b_, lg2b_, L_;
     uint64_t nbLeaves_;
+    std::vector<float> Wh_;  // IDF weights, subnormals flushed
     DevBuf<float> W_;
     std::unordered_map<uint32_t, uint32_t> slotOf_;
     std::vector<uint32_t> docIds_;

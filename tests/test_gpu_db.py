@@ -260,3 +260,25 @@ def test_db_many_document_ranges():
     for q, (gd, gs) in zip(qs, res):
         od, os_ = o.query(q, 60)
         _assert_ranked_equal(gd, gs, od, os_)
+
+
+def test_db_subnormal_weights_read_as_zero():
+    """IDF weights below FLT_MIN (possible in a deserialised blob) act as 0, as the
+    reference's DAZ host build reads them (CMakeLists.txt:16)."""
+    rng = np.random.default_rng(11)
+    n = level_beg(16, 2)
+    w = (rng.random((n, 16)) * 4).astype(np.float32)
+    w[rng.random(w.shape) < 0.3] = np.float32(1e-40)  # subnormal
+    rows = rng.integers(0, 256, (n, 16, 128), dtype=np.uint8)
+    voc = rb.deserialize_vocabulary(ref_blob(2, oracle.rows_to_ref_nodes(rows), np.full(n, 0xFFFF), w))
+    g = GpuDb(voc)
+    o = oracle.DataBase(16, 2, w)
+    docs = _random_docs(rng, 120, 256)
+    for d, lf in docs.items():
+        g.add(d, lf)
+        o.add(d, lf)
+        assert g.rcp(d) == o.rcp(d)
+    for qi in (0, 5, 77):
+        gd, gs = g.query(docs[qi], 40)
+        od, os_ = o.query(docs[qi], 40)
+        _assert_ranked_equal(gd, gs, od, os_)
