@@ -390,6 +390,12 @@ def main():
     r1_m = timed_lookup(rb, v1, q1, 50_000, o1, rb.LOOKUP_MFMA, 20)
     r1_p = timed_lookup(rb, v1, q1, 50_000, o1p, rb.LOOKUP_PLAIN, 20)
     id1 = bool(np.array_equal(o1.numpy(), o1p.numpy()))
+    # autotuned AUTO crossovers (plain below, bucketed MFMA from there on)
+    q_tune = syn.descriptors(1 << 20, seed=SEED_QUERY + 20)
+    cross_u8 = v1.tune_lookup(q_tune, 1 << 20)
+    cross_f32 = v32.tune_lookup(q_tune, 1 << 20)
+    del q_tune
+    log(f"bench: tuned AUTO crossovers: b16 L3 uint8 {cross_u8}, b32 L4 float {cross_f32}")
 
     line = {
         "metric": METRIC, "value": round(tops, 3), "unit": "TOPS", "n_gpus": 1, "steps": args.steps,
@@ -415,6 +421,7 @@ def main():
                                  "hbm_frac_mfma": round(r_m * LOOKUP_BYTES / 1e9 / HBM_PEAK_GBS, 4)},
         "lookup_b16L3_u8_50k": {"mfma_desc_per_s": round(r1_m, 1), "plain_desc_per_s": round(r1_p, 1),
                                 "identical_leaf_ids": id1},
+        "lookup_auto_crossover_tuned": {"b16L3_u8": cross_u8, "b32L4_f32": cross_f32},
     }
     threads = CPU_THREADS
     if not args.skip_retrieval:

@@ -103,6 +103,15 @@ typedef enum {
  * candidate set held more than one child and was re-scored in FP32. */
 rbow_status rbow_voc_lookup_ex(const rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc,
                                uint32_t* dev_leaf, int32_t kernel, uint32_t* dev_near_ties, rbow_stream stream);
+/* Autotuner (no reference counterpart): times the plain and MFMA lookups on
+ * prefixes of dev_desc (powers of two from 4096 up to nb_desc) and sets the
+ * vocabulary's AUTO crossover to the smallest size at which MFMA is at least
+ * as fast (nb_desc + 1 if never).  Blocks; does not change any leaf id.
+ * rbow_voc_set_lookup_crossover sets it directly (0 restores the default). */
+rbow_status rbow_voc_tune_lookup(rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc,
+                                 uint32_t* out_crossover, rbow_stream stream);
+rbow_status rbow_voc_set_lookup_crossover(rbow_vocabulary* voc, uint32_t crossover);
+uint32_t rbow_voc_lookup_crossover(const rbow_vocabulary* voc); /* current AUTO crossover */
 
 /* RapidBoW.h:69 IVocabulary::serialize.  Two-call form: blob may be NULL to
  * query *size.  uint8/b16 trees use the reference blob byte layout

@@ -79,6 +79,9 @@ _sig("rbow_kmeans_assign", _i32, [_vp, _vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp
 _sig("rbow_kmeans_update", _i32, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp])
 _sig("rbow_reformat_tiles", _i32, [_vp, _vp, _u32, _u32, _vp, _vp])
 _sig("rbow_gather_rows", _i32, [_vp, _vp, _u64, _vp, _vp])
+_sig("rbow_voc_tune_lookup", _i32, [_vp, _vp, _u32, ctypes.POINTER(_u32), _vp])
+_sig("rbow_voc_set_lookup_crossover", _i32, [_vp, _u32])
+_sig("rbow_voc_lookup_crossover", _u32, [_vp])
 _sig("rbow_db_create", _i32, [_vp, ctypes.POINTER(_vp)])
 _sig("rbow_db_destroy", None, [_vp])
 _sig("rbow_db_add_doc", _i32, [_vp, _u32, _vp, _u32])
@@ -177,6 +180,20 @@ class Vocabulary:
         hip.synchronize()
         leaf = out.numpy(n)
         return (leaf, int(ties.numpy()[0])) if return_near_ties else leaf
+
+    def tune_lookup(self, dev_desc, nb_desc, stream=None):
+        """Autotune the AUTO lookup crossover on these descriptors; returns it."""
+        c = _u32()
+        check(lib.rbow_voc_tune_lookup(self.handle, _ptr(dev_desc), nb_desc, ctypes.byref(c), _stream(stream)))
+        return c.value
+
+    @property
+    def lookup_crossover(self):
+        return lib.rbow_voc_lookup_crossover(self.handle)
+
+    @lookup_crossover.setter
+    def lookup_crossover(self, n):
+        check(lib.rbow_voc_set_lookup_crossover(self.handle, n))
 
     def serialize(self):
         size = _sz(0)

@@ -58,6 +58,11 @@ def build(verbose=False):
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
          ["-fopenmp", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
+    # the command-line tool (main.cpp of the reference), linked against the library
+    tool = os.path.join(HERE, "makevoc")
+    _run([HIPCC] + COMMON + ["-o", tool + ".tmp", os.path.join(HERE, "makevoc.cpp"), f"-L{HERE}", "-lrapidbow_amd",
+                             "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(tool + ".tmp", tool)
     return LIB
 
 

@@ -65,7 +65,7 @@ void checkAligned(const void* p, const char* what) {
 
 // Task table + tiles for the kernel-level k-means entries.
 struct KmScratch {
-    DevBuf<uint32_t> beg, cnt, active, passes, nbActive, order, tcBeg, tcCnt, partCnt, partChanged;
+    DevBuf<uint32_t> beg, cnt, active, passes, nbActive, order, tcBeg, tcCnt, partCnt, partChanged, waveChunkBeg;
     DevBuf<KmTile> tiles;
     DevBuf<KmChunk> chunks, red;
     DevBuf<int8_t> XA;
@@ -115,17 +115,20 @@ struct KmScratch {
     void prepareMfma(const uint8_t* points, const uint32_t* taskCnt, hipStream_t s) {
         std::vector<uint32_t> hOrder(rows);
         for (uint64_t i = 0; i < rows; i++) hOrder[i] = uint32_t(i);
-        std::vector<KmChunk> hc, hr;
-        std::vector<uint32_t> cb(k.nbTasks), cc(k.nbTasks);
-        uint32_t ti = 0;
-        for (uint32_t g = 0; g < k.nbTasks; g++) {
-            const uint32_t nt = divUp(taskCnt[g], 64);
-            cb[g] = uint32_t(hc.size());
-            for (uint32_t c = 0; c < nt; c += kChunkTiles) hc.push_back(KmChunk{g, ti + c, ti + std::min(nt, c + kChunkTiles)});
-            cc[g] = uint32_t(hc.size()) - cb[g];
+        std::vector<uint32_t> taskTiles(k.nbTasks);
+        for (uint32_t g = 0; g < k.nbTasks; g++) taskTiles[g] = divUp(taskCnt[g], 64);
+        KmPartition part = partitionKmTiles(taskTiles);
+        std::vector<KmChunk>& hc = part.chunks;
+        std::vector<uint32_t>& cb = part.taskChunkBeg;
+        std::vector<uint32_t>& cc = part.taskChunkCnt;
+        std::vector<KmChunk> hr;  // every task is reduced into the caller's totals
+        for (uint32_t g = 0; g < k.nbTasks; g++)
             for (uint32_t c = 0; c < cc[g]; c += 32) hr.push_back(KmChunk{g, cb[g] + c, cb[g] + std::min(cc[g], c + 32)});
-            ti += nt;
-        }
+        waveChunkBeg.resize(part.waveChunkBeg.size());
+        RBOW_HIP(hipMemcpyAsync(waveChunkBeg.get(), part.waveChunkBeg.data(), 4 * part.waveChunkBeg.size(),
+                                hipMemcpyHostToDevice, s));
+        k.waveChunkBeg = waveChunkBeg.get();
+        k.nbWaves = part.nbWaves;
         order.resize(std::max<uint64_t>(rows, 1));
         chunks.resize(std::max<size_t>(hc.size(), 1));
         red.resize(std::max<size_t>(hr.size(), 1));
@@ -221,6 +224,33 @@ rbow_status rbow_voc_lookup_ex(const rbow_vocabulary* voc, const void* dev_desc,
         const auto* d = static_cast<const uint8_t*>(dev_desc);
         launchLookup(t, d, nb_desc, dev_leaf, dev_near_ties, kernel, hs(stream));
     });
+}
+
+rbow_status rbow_voc_tune_lookup(rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc,
+                                 uint32_t* out_crossover, rbow_stream stream) {
+    return guarded([&] {
+        RBOW_REQUIRE(voc != nullptr, RBOW_ERR_INVALID_ARG, "null vocabulary");
+        RBOW_REQUIRE(dev_desc != nullptr || nb_desc == 0, RBOW_ERR_INVALID_ARG, "null descriptors");
+        if (nb_desc) checkAligned(dev_desc, "descriptors");
+        DevBuf<uint32_t> leaf(std::max(1u, nb_desc));
+        const uint32_t c = tuneLookupCrossover(voc->v->view(), static_cast<const uint8_t*>(dev_desc), nb_desc,
+                                               leaf.get(), hs(stream));
+        voc->v->lookupAutoMin = c;
+        if (out_crossover) *out_crossover = c;
+    });
+}
+
+rbow_status rbow_voc_set_lookup_crossover(rbow_vocabulary* voc, uint32_t crossover) {
+    return guarded([&] {
+        RBOW_REQUIRE(voc != nullptr, RBOW_ERR_INVALID_ARG, "null vocabulary");
+        voc->v->lookupAutoMin = crossover;
+    });
+}
+
+uint32_t rbow_voc_lookup_crossover(const rbow_vocabulary* voc) {
+    if (!voc) return 0;
+    const Vocabulary& v = *voc->v;
+    return v.lookupAutoMin ? v.lookupAutoMin : (v.centroidType == 0 ? 131072u : 32768u);
 }
 
 rbow_status rbow_voc_lookup(const rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc, uint32_t* dev_leaf,

@@ -1,5 +1,7 @@
 // kernels.hpp -- host-side launchers of the gfx950 kernels.
 #pragma once
+#include <vector>
+
 #include "common.hpp"
 
 namespace rbow_amd {
@@ -26,6 +28,7 @@ struct TreeView {
     const float* cn2 = nullptr;
     const float* e1 = nullptr;
     const float* e2 = nullptr;
+    uint32_t autoMin = 0;  // AUTO lookup: batches below this use the plain kernel (0: built-in default)
 };
 
 // Derived per-child data (norms, bf16 split, bounds) for nodes [0, nbNodes).
@@ -40,9 +43,14 @@ void launchLookupPlain(const TreeView& t, const uint8_t* desc, uint32_t n, uint3
 void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
                       hipStream_t s);
 // Kernel choice (RBOW_LOOKUP_*): AUTO takes the plain kernel below the batch
-// size where the bucketed MFMA form's per-level passes start to pay off.
+// size where the bucketed MFMA form's per-level passes start to pay off
+// (t.autoMin when tuned, else a default measured on MI355X).
 void launchLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies, int kernel,
                   hipStream_t s);
+// Autotuner: times both lookup kernels on prefixes of desc (powers of two from
+// 4096 up to n) and returns the smallest size at which the MFMA form is at
+// least as fast (n + 1 if it never is).  Blocks; uses `leaf` (n words) as output.
+uint32_t tuneLookupCrossover(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s);
 
 // ----------------------------------------------------------------- build --
 // Gather rows: out[p] = desc[idx[p]] (idx == kIdxInvalid -> 0xFF bytes).
@@ -57,12 +65,24 @@ struct KmTile {        // one work item of the assign kernels: <= 64 rows of one
     uint32_t cnt;      // rows in this tile (1..64)
 };
 
-struct KmChunk {       // a run of <= kChunkTiles tiles of one task, processed by one wave
+struct KmChunk {       // a run of tiles of one task, processed by one wave
     uint32_t task;
     uint32_t tileBeg;
     uint32_t tileEnd;
 };
-constexpr uint32_t kChunkTiles = 16;
+
+// Work partition of the MFMA assign: the level's tiles (tasks contiguous) are
+// cut into nbWaves equal ranges, one per resident wave, and each range at task
+// boundaries into chunks (one partial-sum slot each).  Wave w processes chunks
+// [waveChunkBeg[w], waveChunkBeg[w+1]); a task's chunks are contiguous.
+struct KmPartition {
+    uint32_t nbWaves = 0;
+    std::vector<KmChunk> chunks;
+    std::vector<uint32_t> waveChunkBeg;               // [nbWaves + 1]
+    std::vector<uint32_t> taskChunkBeg, taskChunkCnt;  // [task]
+};
+// taskTiles[g] = tiles of task g; tiles are numbered task after task.
+KmPartition partitionKmTiles(const std::vector<uint32_t>& taskTiles);
 
 struct KmLevel {       // device state of one level's batched k-means
     uint32_t b = 16, nbTasks = 0, nbTiles = 0, centroidType = 0;
@@ -83,6 +103,8 @@ struct KmLevel {       // device state of one level's batched k-means
     const int8_t* XA = nullptr;      // [tile][rb 4][kb 2][lane 64][16]  distance A fragments (x - 128)
     const KmChunk* chunks = nullptr;
     uint32_t nbChunks = 0;
+    const uint32_t* waveChunkBeg = nullptr;  // [nbWaves + 1]
+    uint32_t nbWaves = 0;
     const uint32_t* taskChunkBeg = nullptr;  // [task]
     const uint32_t* taskChunkCnt = nullptr;  // [task]
     int32_t* partSums = nullptr;     // [chunk][16][128]  sums of (x - 128)
@@ -103,11 +125,19 @@ void launchGatherTiles(const uint8_t* desc, const uint32_t* order, const KmTile*
 // Adds the per-chunk partials of the MFMA assign into the task totals
 // counts / sums / changed.  red: (task, chunkBeg, chunkEnd) items.
 void launchKmReduce(const KmLevel& k, const KmChunk* red, uint32_t nbRed, hipStream_t s);
+// Blocks of the MFMA assign kernel that the device holds at once (its grid cap).
+uint32_t kmAssignResidentBlocks();
 // One assign pass over all active tasks; kernel 0 = MFMA, 1 = plain.
 void launchKmAssign(const KmLevel& k, int kernel, hipStream_t s);
 // Update pass: converged tasks become inactive, others move their centers
 // and clear their accumulators.  Counts remaining active tasks into nbActive.
 void launchKmUpdate(const KmLevel& k, hipStream_t s);
+
+// Stable regroup of a level's rows by (task, child) (Vocabulary.cpp:83-98):
+// orderOut = orderIn stably sorted by task * b + assignment; hist[task * b + j]
+// = the occurrence counts.  Rows of the level are tasks-contiguous (k.tiles).
+void launchRegroup(const KmLevel& k, uint32_t nbRows, const uint32_t* orderIn, uint32_t* orderOut, uint32_t* hist,
+                   hipStream_t s);
 
 void launchFillU32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 

@@ -11,6 +11,8 @@
 //   loop:   a task stops after the first pass with changed == 0 (no update on
 //           that pass) or after max_iters passes.
 // Sums are exact: int32 per wave in LDS, flushed into int64 per task.
+#include <hipcub/hipcub.hpp>
+
 #include "kernels.hpp"
 
 namespace rbow_amd {
@@ -304,9 +306,31 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
         for (int d = 0; d < 4; d++)
             esel[g][d] = (grp == uint32_t(g) && (col >> 2) == uint32_t(d)) ? int(1u << (8 * (col & 3))) : 0;
     const v4i zero = {0, 0, 0, 0};
-    for (uint32_t ch = blockIdx.x * kWaves + wave; ch < k.nbChunks; ch += gridDim.x * kWaves) {
+    // one wave per resident slot, each with an equal range of tiles cut into
+    // per-task chunks on the host (partitionKmTiles): no wave waits for a
+    // second round and the loads balance to within one tile
+    const uint32_t w = blockIdx.x * kWaves + wave;
+    if (w >= k.nbWaves) return;
+    const uint32_t chEnd = k.waveChunkBeg[w + 1];
+    auto nextActive = [&](uint32_t c0) {
+        while (c0 < chEnd && !k.active[k.chunks[c0].task]) c0++;
+        return c0;
+    };
+    uint32_t ch = nextActive(k.waveChunkBeg[w]);
+    // the first tile of the wave's next chunk is requested while the last tile of
+    // the current chunk computes
+    v4i a[4][2];
+    KmTile t{};
+    uint32_t prev = 0;
+    if (ch < chEnd) {
+        const uint32_t tb = k.chunks[ch].tileBeg;
+        t = k.tiles[tb];
+        loadTileA(k.XA + uint64_t(tb) * 8192, lane, a);
+        prev = lane < t.cnt ? k.assign[t.row + lane] : 0u;  // lane r: previous assignment of row r
+    }
+    while (ch < chEnd) {
         const KmChunk c = k.chunks[ch];
-        if (!k.active[c.task]) continue;
+        const uint32_t chNext = nextActive(ch + 1);
         const uint8_t* crow = k.centersU8 + (uint64_t(c.task) * 16 + col) * kDims;
         const v4i b0 = toSigned(ld16(crow + grp * 16)), b1 = toSigned(ld16(crow + 64 + grp * 16));
         int ns = 0;
@@ -321,18 +345,16 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
 #pragma unroll
         for (int db = 0; db < 8; db++) acc[db] = zero;
         uint32_t myCnt = 0, myChanged = 0;
-        v4i a[4][2];
-        KmTile t = k.tiles[c.tileBeg];
-        loadTileA(k.XA + uint64_t(c.tileBeg) * 8192, lane, a);
-        uint32_t prev = lane < t.cnt ? k.assign[t.row + lane] : 0u;  // lane r: previous assignment of row r
         for (uint32_t ti = c.tileBeg; ti < c.tileEnd; ti++) {
-            // next tile's fragments and previous assignments, in flight while this one computes
+            // next tile's fragments and previous assignments (this chunk's, else the next
+            // chunk's first), in flight while this one computes
             v4i an[4][2];
             KmTile tn = t;
             uint32_t prevN = 0;
-            if (ti + 1 < c.tileEnd) {
-                tn = k.tiles[ti + 1];
-                loadTileA(k.XA + uint64_t(ti + 1) * 8192, lane, an);
+            const uint32_t tiN = ti + 1 < c.tileEnd ? ti + 1 : (chNext < chEnd ? k.chunks[chNext].tileBeg : kIdxInvalid);
+            if (tiN != kIdxInvalid) {
+                tn = k.tiles[tiN];
+                loadTileA(k.XA + uint64_t(tiN) * 8192, lane, an);
                 prevN = lane < tn.cnt ? k.assign[tn.row + lane] : 0u;
             }
             uint32_t key[4][4];
@@ -383,7 +405,7 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
                     bt[rb] = packLowBytes(__builtin_amdgcn_mfma_i32_16x16x64_i8(a[rb][db >> 2], esel[db & 3], zero, 0, 0, 0));
                 acc[db] = __builtin_amdgcn_mfma_i32_16x16x64_i8(oh, bt, acc[db], 0, 0, 0);
             }
-            if (ti + 1 < c.tileEnd) {
+            if (tiN != kIdxInvalid) {
 #pragma unroll
                 for (int rb = 0; rb < 4; rb++) {
                     a[rb][0] = an[rb][0];
@@ -404,6 +426,7 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) myChanged += uint32_t(__shfl_xor(int(myChanged), m, 64));
         if (lane == 0) k.partChanged[ch] = myChanged;
+        ch = chNext;
     }
 }
 
@@ -628,6 +651,18 @@ __global__ __launch_bounds__(256) void k_updateU8Part(KmLevel k) {
     }
 }
 
+// Regroup keys (Vocabulary.cpp:83-98): key of row r = task * b + assignment, and
+// the per-(task, child) histogram (the occurrence counts).
+__global__ __launch_bounds__(64) void k_regroupKeys(KmLevel k, uint32_t* __restrict__ keys, uint32_t* __restrict__ hist) {
+    const KmTile t = k.tiles[blockIdx.x];
+    const uint32_t r = threadIdx.x;
+    if (r < t.cnt) {
+        const uint32_t key = t.task * k.b + k.assign[t.row + r];
+        keys[t.row + r] = key;
+        atomicAdd(&hist[key], 1u);
+    }
+}
+
 __global__ void k_levelReset(KmLevel k, uint64_t nbRows) {
     const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x, stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t i = tid; i < nbRows; i += stride) k.assign[i] = 0xFF;  // "no previous assignment"
@@ -696,18 +731,91 @@ void launchKmReduce(const KmLevel& k, const KmChunk* red, uint32_t nbRed, hipStr
     RBOW_HIP(hipGetLastError());
 }
 
+uint32_t kmAssignResidentBlocks() {
+    static thread_local int device = -1;
+    static thread_local uint32_t blocks = 0;
+    int d = 0;
+    RBOW_HIP(hipGetDevice(&d));
+    if (d != device) {
+        int perCu = 0, cus = 0;
+        RBOW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_assignU8Mfma, 256, 0));
+        RBOW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d));
+        blocks = uint32_t(std::max(1, perCu * cus));
+        device = d;
+    }
+    return blocks;
+}
+
+KmPartition partitionKmTiles(const std::vector<uint32_t>& taskTiles) {
+    KmPartition p;
+    const uint32_t nbTasks = uint32_t(taskTiles.size());
+    uint64_t T = 0;
+    for (uint32_t n : taskTiles) T += n;
+    p.taskChunkBeg.assign(nbTasks, 0u);
+    p.taskChunkCnt.assign(nbTasks, 0u);
+    p.nbWaves = uint32_t(std::min<uint64_t>(uint64_t(kmAssignResidentBlocks()) * kWaves, T));
+    p.waveChunkBeg.assign(p.nbWaves + 1, 0u);
+    uint32_t g = 0;        // current task
+    uint64_t gBeg = 0;     // its first tile
+    while (g < nbTasks && taskTiles[g] == 0) p.taskChunkBeg[g++] = 0;
+    for (uint32_t w = 0; w < p.nbWaves; w++) {
+        p.waveChunkBeg[w] = uint32_t(p.chunks.size());
+        uint64_t lo = T * w / p.nbWaves;
+        const uint64_t hi = T * (w + 1) / p.nbWaves;
+        while (lo < hi) {
+            while (lo >= gBeg + taskTiles[g]) {  // next non-empty task
+                gBeg += taskTiles[g];
+                g++;
+                p.taskChunkBeg[g] = uint32_t(p.chunks.size());
+            }
+            const uint64_t end = std::min<uint64_t>(hi, gBeg + taskTiles[g]);
+            if (p.taskChunkCnt[g] == 0) p.taskChunkBeg[g] = uint32_t(p.chunks.size());
+            p.chunks.push_back(KmChunk{g, uint32_t(lo), uint32_t(end)});
+            p.taskChunkCnt[g]++;
+            lo = end;
+        }
+    }
+    p.waveChunkBeg[p.nbWaves] = uint32_t(p.chunks.size());
+    for (g = 0; g < nbTasks; g++)
+        if (p.taskChunkCnt[g] == 0) p.taskChunkBeg[g] = 0;
+    return p;
+}
+
 void launchKmAssign(const KmLevel& k, int kernel, hipStream_t s) {
     if (!k.nbTiles) return;
     const uint32_t grid = gridFor(k.nbTiles, kWaves, 2048);
     if (k.centroidType == 0) {
         if (kernel == 1) k_assignU8Plain<<<grid, 256, 0, s>>>(k);
-        else k_assignU8Mfma<<<gridFor(k.nbChunks, kWaves, 4096), 256, 0, s>>>(k);
+        else if (k.nbWaves) k_assignU8Mfma<<<divUp(k.nbWaves, kWaves), 256, 0, s>>>(k);
     } else if (k.b == 32) {
         k_assignF32Plain<32><<<grid, 256, 0, s>>>(k);
     } else {
         k_assignF32Plain<16><<<grid, 256, 0, s>>>(k);
     }
     RBOW_HIP(hipGetLastError());
+}
+
+void launchRegroup(const KmLevel& k, uint32_t nbRows, const uint32_t* orderIn, uint32_t* orderOut, uint32_t* hist,
+                   hipStream_t s) {
+    const uint64_t nbKeys = uint64_t(k.nbTasks) * k.b;
+    launchFillU32(hist, 0u, nbKeys, s);
+    if (!nbRows) return;
+    int bits = 1;
+    while ((1ull << bits) < nbKeys) bits++;
+    size_t tempBytes = 0;
+    RBOW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tempBytes, (uint32_t*)nullptr, (uint32_t*)nullptr, orderIn,
+                                                orderOut, int(nbRows), 0, bits, s));
+    void* ws = nullptr;
+    RBOW_HIP(hipMallocAsync(&ws, 8ull * nbRows + tempBytes + 256, s));
+    uint32_t* keys = static_cast<uint32_t*>(ws);
+    uint32_t* keysSorted = keys + nbRows;
+    void* temp = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(keysSorted + nbRows) + 255) & ~uintptr_t(255));
+    k_regroupKeys<<<k.nbTiles, 64, 0, s>>>(k, keys, hist);
+    // LSD radix sort is stable: rows keep their order inside each (task, child)
+    RBOW_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, keys, keysSorted, orderIn, orderOut, int(nbRows), 0,
+                                                bits, s));
+    RBOW_HIP(hipGetLastError());
+    RBOW_HIP(hipFreeAsync(ws, s));
 }
 
 void launchKmUpdate(const KmLevel& k, hipStream_t s) {

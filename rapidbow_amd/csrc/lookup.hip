@@ -22,6 +22,8 @@
 //    FP32 order, so the argmin equals the plain kernel's.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace rbow_amd {
@@ -362,15 +364,25 @@ __global__ __launch_bounds__(256) void k_levelI8(const uint8_t* __restrict__ row
         const uint8_t* crow = rows + child * kDims;
         const v4i b0 = toSigned(ld16(crow + grp * 16)), b1 = toSigned(ld16(crow + 64 + grp * 16));
         const int ns = normS[child];
-        for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
+        // row block rb+1's descriptor bytes are in flight while rb computes
+        auto loadRows = [&](uint32_t rb, v4i& x0, v4i& x1) {
             const uint32_t r = rb * 16 + col;
-            v4i a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
+            x0 = v4i{0, 0, 0, 0};
+            x1 = v4i{0, 0, 0, 0};
             if (r < t.cnt) {
                 const uint32_t q = kIdentity ? t.start + r : perm[t.start + r];
                 const uint8_t* x = desc + uint64_t(q) * kDims;
-                a0 = toSigned(ld16(x + grp * 16));
-                a1 = toSigned(ld16(x + 64 + grp * 16));
+                x0 = ld16(x + grp * 16);
+                x1 = ld16(x + 64 + grp * 16);
             }
+        };
+        v4i n0, n1;
+        loadRows(0, n0, n1);
+        for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
+            const uint32_t r = rb * 16 + col;
+            const v4i c0 = n0, c1 = n1;
+            if ((rb + 1) * 16 < t.cnt) loadRows(rb + 1, n0, n1);
+            const v4i a0 = r < t.cnt ? toSigned(c0) : c0, a1 = r < t.cnt ? toSigned(c1) : c1;
             v4i acc = {0, 0, 0, 0};
             acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b0, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b1, acc, 0, 0, 0);
@@ -434,16 +446,30 @@ __global__ __launch_bounds__(256) void k_levelBf16(
             ea[cb] = e1[child];
             eb[cb] = e2[child];
         }
-        for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
+        // row block rb+1's descriptor bytes are in flight while rb computes
+        auto loadRows = [&](uint32_t rb, uint2 (&w)[4]) {
             const uint32_t r = rb * 16 + col;
-            v8bf ax[4];
-            uint32_t xsq = 0;
             if (r < t.cnt) {
                 const uint32_t q = kIdentity ? t.start + r : perm[t.start + r];
                 const uint8_t* x = desc + uint64_t(q) * kDims;
 #pragma unroll
+                for (int kb = 0; kb < 4; kb++) w[kb] = *reinterpret_cast<const uint2*>(x + kb * 32 + grp * 8);
+            }
+        };
+        uint2 wn[4];
+        loadRows(0, wn);
+        for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
+            const uint32_t r = rb * 16 + col;
+            uint2 wc[4];
+#pragma unroll
+            for (int kb = 0; kb < 4; kb++) wc[kb] = wn[kb];
+            if ((rb + 1) * 16 < t.cnt) loadRows(rb + 1, wn);
+            v8bf ax[4];
+            uint32_t xsq = 0;
+            if (r < t.cnt) {
+#pragma unroll
                 for (int kb = 0; kb < 4; kb++) {
-                    const uint2 w = *reinterpret_cast<const uint2*>(x + kb * 32 + grp * 8);
+                    const uint2 w = wc[kb];
 #pragma unroll
                     for (int e = 0; e < 8; e++) {
                         const uint32_t xw = e < 4 ? w.x : w.y, mw = e < 4 ? muw[kb].x : muw[kb].y;
@@ -722,11 +748,49 @@ void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32
 
 void launchLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies, int kernel,
                   hipStream_t s) {
-    // measured crossovers on MI355X: ~150k descriptors for uint8 trees, ~40k for float trees
-    const uint32_t autoMin = t.centroidType == 0 ? 131072u : 32768u;
+    // defaults measured on MI355X: ~150k descriptors for uint8 trees, ~40k for float trees
+    const uint32_t autoMin = t.autoMin ? t.autoMin : (t.centroidType == 0 ? 131072u : 32768u);
     const bool plain = kernel == RBOW_LOOKUP_PLAIN || t.L == 0 || (kernel == RBOW_LOOKUP_AUTO && n < autoMin);
     if (plain) launchLookupPlain(t, desc, n, leaf, s);
     else launchLookupMfma(t, desc, n, leaf, nearTies, s);
+}
+
+uint32_t tuneLookupCrossover(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s) {
+    hipEvent_t e0, e1;
+    RBOW_HIP(hipEventCreate(&e0));
+    RBOW_HIP(hipEventCreate(&e1));
+    auto timeOne = [&](uint32_t m, int kernel) {  // median of 3 timed runs after one warm run
+        float ms[3];
+        if (kernel == RBOW_LOOKUP_PLAIN) launchLookupPlain(t, desc, m, leaf, s);
+        else launchLookupMfma(t, desc, m, leaf, nullptr, s);
+        for (int r = 0; r < 3; r++) {
+            RBOW_HIP(hipEventRecord(e0, s));
+            if (kernel == RBOW_LOOKUP_PLAIN) launchLookupPlain(t, desc, m, leaf, s);
+            else launchLookupMfma(t, desc, m, leaf, nullptr, s);
+            RBOW_HIP(hipEventRecord(e1, s));
+            RBOW_HIP(hipEventSynchronize(e1));
+            RBOW_HIP(hipEventElapsedTime(&ms[r], e0, e1));
+        }
+        std::sort(ms, ms + 3);
+        return ms[1];
+    };
+    uint32_t cross = n + 1;
+    try {
+        for (uint64_t m = 4096; m <= n; m *= 2) {
+            if (t.L == 0) break;
+            if (timeOne(uint32_t(m), RBOW_LOOKUP_MFMA) <= timeOne(uint32_t(m), RBOW_LOOKUP_PLAIN)) {
+                cross = uint32_t(m);
+                break;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    RBOW_HIP(hipEventDestroy(e0));
+    RBOW_HIP(hipEventDestroy(e1));
+    return cross;
 }
 
 }  // namespace rbow_amd

@@ -62,6 +62,7 @@ TreeView Vocabulary::view() const {
     t.cn2 = cn2.get();
     t.e1 = e1.get();
     t.e2 = e2.get();
+    t.autoMin = lookupAutoMin;
     return t;
 }
 
@@ -135,11 +136,13 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     std::vector<uint32_t> passesAll(voc->nbNodes, 0u);
 
     // host grouping state: descriptor order of the level, tasks contiguous
-    std::vector<uint32_t> order(nbDesc), next(nbDesc);
+    // the level order lives on the device; the host keeps a copy only for the
+    // reference-mode seeds, which index it
+    std::vector<uint32_t> order(nbDesc);
     std::iota(order.begin(), order.end(), 0u);
     std::vector<uint32_t> taskBeg{0u}, taskCnt{nbDesc};
 
-    DevBuf<uint32_t> dOrder(nbDesc);
+    DevBuf<uint32_t> dOrder(nbDesc), dOrderNext(nbDesc), dHist;
     DevBuf<uint8_t> dX(uint64_t(nbDesc) * kDims), dAssign(nbDesc);
     DevBuf<uint32_t> dTaskBeg, dTaskCnt, dCounts, dChanged, dActive, dPasses, dNbActive(1), dSeed, dMasks;
     DevBuf<long long> dSums;
@@ -149,9 +152,10 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     DevBuf<int32_t> dPartSums;
     DevBuf<uint32_t> dPartCnt, dPartChanged, dTaskChunkBeg, dTaskChunkCnt;
     std::vector<KmChunk> chunks, red;
-    std::vector<uint32_t> taskChunkBeg, taskChunkCnt;
+    std::vector<uint32_t> taskChunkBeg, taskChunkCnt, waveChunkBeg;
+    uint32_t nbWaves = 0;
+    DevBuf<uint32_t> dWaveChunkBeg;
     const bool mfmaPath = ct == 0 && kernel == 0;
-    std::vector<uint8_t> hAssign(nbDesc);
     std::vector<uint32_t> hPasses, hMasks;
     std::vector<KmTile> tiles;
     std::mt19937_64 rng(opts.seed);
@@ -179,29 +183,28 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         dCounts.reserve(uint64_t(nG) * b);
         dSums.reserve(uint64_t(nG) * b * kDims);
         dTiles.reserve(std::max<size_t>(tiles.size(), 1));
-        h2d(dOrder.get(), order.data(), nbDesc, s);
+        if (level == 0) h2d(dOrder.get(), order.data(), nbDesc, s);
         h2d(dTaskBeg.get(), taskBeg.data(), nG, s);
         h2d(dTaskCnt.get(), taskCnt.data(), nG, s);
         h2d(dTiles.get(), tiles.data(), tiles.size(), s);
         if (mfmaPath) {
-            // chunks of <= kChunkTiles tiles of one task (one wave each); reduce items of <= 32 chunks
-            chunks.clear();
+            // equal tile ranges per resident wave, cut into per-task chunks; reduce items of <= 32 chunks
+            std::vector<uint32_t> taskTiles(nG);
+            for (uint32_t g = 0; g < nG; g++) taskTiles[g] = divUp(taskCnt[g], 64);
+            KmPartition part = partitionKmTiles(taskTiles);
+            chunks.swap(part.chunks);
+            taskChunkBeg.swap(part.taskChunkBeg);
+            taskChunkCnt.swap(part.taskChunkCnt);
+            waveChunkBeg.swap(part.waveChunkBeg);
+            nbWaves = part.nbWaves;
             red.clear();
-            taskChunkBeg.assign(nG, 0u);
-            taskChunkCnt.assign(nG, 0u);
-            uint32_t ti = 0;
-            for (uint32_t g = 0; g < nG; g++) {
-                const uint32_t nt = divUp(taskCnt[g], 64);
-                taskChunkBeg[g] = uint32_t(chunks.size());
-                for (uint32_t c = 0; c < nt; c += kChunkTiles)
-                    chunks.push_back(KmChunk{g, ti + c, ti + std::min(nt, c + kChunkTiles)});
-                taskChunkCnt[g] = uint32_t(chunks.size()) - taskChunkBeg[g];
+            for (uint32_t g = 0; g < nG; g++)
                 if (taskChunkCnt[g] > 32)  // small tasks are reduced inside the update kernel
                     for (uint32_t c = 0; c < taskChunkCnt[g]; c += 32)
                         red.push_back(
                             KmChunk{g, taskChunkBeg[g] + c, taskChunkBeg[g] + std::min(taskChunkCnt[g], c + 32)});
-                ti += nt;
-            }
+            dWaveChunkBeg.reserve(waveChunkBeg.size());
+            h2d(dWaveChunkBeg.get(), waveChunkBeg.data(), waveChunkBeg.size(), s);
             dChunks.reserve(std::max<size_t>(chunks.size(), 1));
             dRed.reserve(std::max<size_t>(red.size(), 1));
             dTaskChunkBeg.reserve(nG);
@@ -241,6 +244,8 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             k.XA = dXA.get();
             k.chunks = dChunks.get();
             k.nbChunks = uint32_t(chunks.size());
+            k.waveChunkBeg = dWaveChunkBeg.get();
+            k.nbWaves = nbWaves;
             k.taskChunkBeg = dTaskChunkBeg.get();
             k.taskChunkCnt = dTaskChunkCnt.get();
             k.partSums = dPartSums.get();
@@ -303,8 +308,14 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             }
         }
         hPasses.resize(nG);
-        d2h(hAssign.data(), dAssign.get(), nbDesc, s);
         d2h(hPasses.data(), dPasses.get(), nG, s);
+        // ---- stable regroup + occurrence on the device (Vocabulary.cpp:83-98) ----
+        std::vector<uint32_t> childBeg(uint64_t(nG) * b), childCnt(uint64_t(nG) * b, 0u);
+        dHist.reserve(uint64_t(nG) * b);
+        launchRegroup(k, nbDesc, dOrder.get(), dOrderNext.get(), dHist.get(), s);
+        d2h(childCnt.data(), dHist.get(), childCnt.size(), s);
+        std::swap(dOrder, dOrderNext);
+        if (opts.seed_mode == 0 && level + 1 < L) d2h(order.data(), dOrder.get(), nbDesc, s);
         RBOW_HIP(hipStreamSynchronize(s));
         if (opts.seed_mode == 1)
             for (uint32_t g = 0; g < nG; g++) voc->masks[nodeBase + g] = hMasks[g];
@@ -324,23 +335,13 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
                 assignLaunches++;
             }
 
-        // ---- stable regroup + occurrence (Vocabulary.cpp:83-98) ----
-        std::vector<uint32_t> childBeg(uint64_t(nG) * b), childCnt(uint64_t(nG) * b, 0u);
-        for (uint32_t g = 0; g < nG; g++)
-            for (uint32_t i = taskBeg[g]; i < taskBeg[g] + taskCnt[g]; i++) childCnt[uint64_t(g) * b + hAssign[i]]++;
         uint32_t run = 0;
         for (uint64_t c = 0; c < childCnt.size(); c++) {
             childBeg[c] = run;
             run += childCnt[c];
         }
-        {
-            std::vector<uint32_t> cursor = childBeg;
-            for (uint32_t g = 0; g < nG; g++)
-                for (uint32_t i = taskBeg[g]; i < taskBeg[g] + taskCnt[g]; i++)
-                    next[cursor[uint64_t(g) * b + hAssign[i]]++] = order[i];
-        }
+        RBOW_REQUIRE(run == nbDesc, RBOW_ERR_HIP, "regroup lost rows");
         for (uint64_t c = 0; c < childCnt.size(); c++) occ[nodeBase * b + c] = childCnt[c];
-        order.swap(next);
         taskBeg.swap(childBeg);
         taskCnt.swap(childCnt);
     }

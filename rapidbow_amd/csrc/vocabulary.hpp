@@ -17,6 +17,7 @@ struct Vocabulary {
     DevBuf<float> cn2, e1, e2;
     std::vector<uint32_t> masks;   // valid-center masks, low bits (kmeans.cpp:189)
     std::vector<float> weights;    // [node][b] IDF weights (Vocabulary.cpp:103-108), host like the reference
+    uint32_t lookupAutoMin = 0;    // tuned AUTO crossover (0: untuned)
 
     void allocate(uint32_t b_, uint32_t L_, uint32_t centroidType_);
     // derived per-child data used by the kernels (norms, BF16 split, bounds)

@@ -93,3 +93,20 @@ def test_blob_roundtrip_u8():
     np.testing.assert_array_equal(voc.weights().reshape(-1, 16), res["weights"])
     voc2 = rb.deserialize_vocabulary(out)
     assert voc2.serialize() == blob
+
+
+def test_lookup_autotuner_keeps_leaf_ids():
+    desc = rb.synth_descriptors(60000, seed=5)
+    voc, res, _ = _u8_voc_from_oracle(desc, 3)
+    assert voc.lookup_crossover == 131072  # built-in default for uint8 trees
+    q = rb.synth_descriptors(70000, seed=6)
+    expect = oracle.lookup_u8(res["nodes"], 3, q)
+    d = rb.DeviceArray.from_numpy(q)
+    c = voc.tune_lookup(d, q.shape[0])
+    assert c == voc.lookup_crossover and (c == 70001 or (c >= 4096 and c & (c - 1) == 0))
+    np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_AUTO), expect)
+    for c in (1, 1 << 30):  # force either kernel through AUTO
+        voc.lookup_crossover = c
+        np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_AUTO), expect)
+    voc.lookup_crossover = 0
+    assert voc.lookup_crossover == 131072
