@@ -61,6 +61,11 @@ typedef struct {
     uint64_t seed;          /* engine seed; 5489 = std::mt19937_64's default (the reference) */
     uint32_t centroid_type; /* 0: uint8 centroids (reference, b = 16); 1: float32 (b = 16 or 32) */
     uint32_t kernel;        /* 0: MFMA kernels; 1: plain scalar kernels (same results) */
+    uint32_t persist_max_tiles; /* MFMA kernels, uint8 centroids: a node of at most this many 64-point
+                                 * tiles runs its whole Lloyd loop in one launch with its points in
+                                 * LDS (clamped to what one workgroup's LDS holds); 0: every node
+                                 * runs one launch per pass.  Same results either way.  Default 16. */
+    uint32_t reserved;      /* must be 0 */
 } rbow_build_options;
 
 void rbow_build_options_default(rbow_build_options* opts);
@@ -239,8 +244,11 @@ typedef struct {
     uint64_t point_passes;   /* sum over nodes of points x Lloyd passes */
     uint64_t assign_ops;
     double reduce_ms;        /* summed time of the chunk-partial reductions (MFMA path) */
-    uint32_t assign_launches;
+    uint32_t assign_launches; /* per-pass (streamed) MFMA assign launches */
     uint32_t reserved;
+    double stream_ms;        /* time of those launches */
+    uint64_t stream_point_passes; /* point-passes they processed */
+    double persist_ms;       /* persistent launches of small tasks' whole Lloyd loops (in assign_ms) */
 } rbow_build_stats;
 rbow_status rbow_voc_build_stats(const rbow_vocabulary* voc, rbow_build_stats* out);
 

@@ -37,7 +37,8 @@ SIFT_ATTR = DescriptorAttributes(1, 2, 128)  # RapidBoW.h:50 SiftAttr
 class BuildStats(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("assign_ms", ctypes.c_double), ("update_ms", ctypes.c_double),
                 ("point_passes", ctypes.c_uint64), ("assign_ops", ctypes.c_uint64), ("reduce_ms", ctypes.c_double),
-                ("assign_launches", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("assign_launches", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("stream_ms", ctypes.c_double),
+                ("stream_point_passes", ctypes.c_uint64), ("persist_ms", ctypes.c_double)]
 
 
 class ServerOptions(ctypes.Structure):
@@ -47,7 +48,8 @@ class ServerOptions(ctypes.Structure):
 
 class BuildOptions(ctypes.Structure):
     _fields_ = [("max_iters", ctypes.c_uint32), ("seed_mode", ctypes.c_uint32), ("seed", ctypes.c_uint64),
-                ("centroid_type", ctypes.c_uint32), ("kernel", ctypes.c_uint32)]
+                ("centroid_type", ctypes.c_uint32), ("kernel", ctypes.c_uint32),
+                ("persist_max_tiles", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 def _sig(name, res, args):
@@ -228,10 +230,12 @@ class Vocabulary:
         return out.reshape(-1, self.branch_factor, 128)
 
 
-def build_options(max_iters=1000, seed_mode=0, seed=5489, centroid_type=0, kernel=0):
+def build_options(max_iters=1000, seed_mode=0, seed=5489, centroid_type=0, kernel=0, persist_max_tiles=None):
     o = BuildOptions()
     lib.rbow_build_options_default(ctypes.byref(o))
     o.max_iters, o.seed_mode, o.seed, o.centroid_type, o.kernel = max_iters, seed_mode, seed, centroid_type, kernel
+    if persist_max_tiles is not None:
+        o.persist_max_tiles = persist_max_tiles
     return o
 
 

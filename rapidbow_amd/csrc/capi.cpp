@@ -117,7 +117,7 @@ struct KmScratch {
         for (uint64_t i = 0; i < rows; i++) hOrder[i] = uint32_t(i);
         std::vector<uint32_t> taskTiles(k.nbTasks);
         for (uint32_t g = 0; g < k.nbTasks; g++) taskTiles[g] = divUp(taskCnt[g], 64);
-        KmPartition part = partitionKmTiles(taskTiles);
+        KmPartition part = partitionKmTiles(taskTiles, {});
         std::vector<KmChunk>& hc = part.chunks;
         std::vector<uint32_t>& cb = part.taskChunkBeg;
         std::vector<uint32_t>& cc = part.taskChunkCnt;
@@ -170,6 +170,8 @@ void rbow_build_options_default(rbow_build_options* o) {
     o->seed = 5489u;      // std::mt19937_64 default seed (kmeans.cpp:145)
     o->centroid_type = 0;
     o->kernel = 0;
+    o->persist_max_tiles = 16;
+    o->reserved = 0;
 }
 
 rbow_status rbow_build_vocabulary_ex(rbow_descriptor_attributes attr, uint32_t branch_factor, const void* dev_desc,
@@ -183,6 +185,7 @@ rbow_status rbow_build_vocabulary_ex(rbow_descriptor_attributes attr, uint32_t b
         if (opts) o = *opts;
         checkAttributes(attr);
         RBOW_REQUIRE(o.centroid_type <= 1, RBOW_ERR_INVALID_ARG, "centroid_type must be 0 or 1");
+        RBOW_REQUIRE(o.reserved == 0, RBOW_ERR_INVALID_ARG, "reserved option field must be 0");
         RBOW_REQUIRE(branch_factor == 16 || (branch_factor == 32 && o.centroid_type == 1), RBOW_ERR_UNSUPPORTED,
                      "branchFactor must be 16 (or 32 with float32 centroids)");
         RBOW_REQUIRE(dev_desc != nullptr || nb_desc == 0, RBOW_ERR_INVALID_ARG, "null descriptors");
@@ -295,8 +298,11 @@ rbow_status rbow_voc_build_stats(const rbow_vocabulary* voc, rbow_build_stats* o
         out->point_passes = st.pointPasses;
         out->assign_ops = st.pointPasses * 2ull * voc->v->b * kDims;
         out->reduce_ms = st.reduceMs;
-        out->assign_launches = st.assignLaunches;
+        out->assign_launches = st.streamLaunches;
         out->reserved = 0;
+        out->stream_ms = st.streamMs;
+        out->stream_point_passes = st.streamPointPasses;
+        out->persist_ms = st.persistMs;
     });
 }
 

@@ -3,6 +3,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "kmpartition.hpp"
 
 namespace rbow_amd {
 
@@ -65,24 +66,9 @@ struct KmTile {        // one work item of the assign kernels: <= 64 rows of one
     uint32_t cnt;      // rows in this tile (1..64)
 };
 
-struct KmChunk {       // a run of tiles of one task, processed by one wave
-    uint32_t task;
-    uint32_t tileBeg;
-    uint32_t tileEnd;
-};
-
-// Work partition of the MFMA assign: the level's tiles (tasks contiguous) are
-// cut into nbWaves equal ranges, one per resident wave, and each range at task
-// boundaries into chunks (one partial-sum slot each).  Wave w processes chunks
-// [waveChunkBeg[w], waveChunkBeg[w+1]); a task's chunks are contiguous.
-struct KmPartition {
-    uint32_t nbWaves = 0;
-    std::vector<KmChunk> chunks;
-    std::vector<uint32_t> waveChunkBeg;               // [nbWaves + 1]
-    std::vector<uint32_t> taskChunkBeg, taskChunkCnt;  // [task]
-};
-// taskTiles[g] = tiles of task g; tiles are numbered task after task.
-KmPartition partitionKmTiles(const std::vector<uint32_t>& taskTiles);
+// Partition of the level's streamed tiles over the resident waves of the MFMA
+// assign (partitionTiles with the device's resident wave count).
+KmPartition partitionKmTiles(const std::vector<uint32_t>& taskTiles, const std::vector<uint32_t>& tileBeg);
 
 struct KmLevel {       // device state of one level's batched k-means
     uint32_t b = 16, nbTasks = 0, nbTiles = 0, centroidType = 0;
@@ -125,6 +111,16 @@ void launchGatherTiles(const uint8_t* desc, const uint32_t* order, const KmTile*
 // Adds the per-chunk partials of the MFMA assign into the task totals
 // counts / sums / changed.  red: (task, chunkBeg, chunkEnd) items.
 void launchKmReduce(const KmLevel& k, const KmChunk* red, uint32_t nbRed, hipStream_t s);
+// Whole Lloyd loops (assign + update, up to maxIters passes, per-task stop) of
+// the listed uint8 tasks, one workgroup each with the task's points in LDS;
+// leaves them inactive with their pass counts.  Every listed task has 1 to
+// maxTiles tiles (maxTiles <= lloydTasksTileCap(1)); taskTileBeg[g] = first
+// tile of task g in k.XA.
+void launchLloydTasks(const KmLevel& k, const uint32_t* taskList, uint32_t nbList, uint32_t maxTiles,
+                      const uint32_t* taskTileBeg, uint32_t maxIters, hipStream_t s);
+// Largest task (in 64-row tiles) whose points fit one workgroup's LDS when
+// blocksPerCu workgroups of launchLloydTasks share a CU.
+uint32_t lloydTasksTileCap(uint32_t blocksPerCu);
 // Blocks of the MFMA assign kernel that the device holds at once (its grid cap).
 uint32_t kmAssignResidentBlocks();
 // One assign pass over all active tasks; kernel 0 = MFMA, 1 = plain.
@@ -136,9 +132,14 @@ void launchKmUpdate(const KmLevel& k, hipStream_t s);
 // Stable regroup of a level's rows by (task, child) (Vocabulary.cpp:83-98):
 // orderOut = orderIn stably sorted by task * b + assignment; hist[task * b + j]
 // = the occurrence counts.  Rows of the level are tasks-contiguous (k.tiles).
+// ws: device workspace of regroupWorkspaceBytes(>= nbRows, >= k.nbTasks * k.b)
+// bytes, 256-byte aligned; no allocation.
+size_t regroupWorkspaceBytes(uint32_t nbRows, uint64_t nbKeysMax);
 void launchRegroup(const KmLevel& k, uint32_t nbRows, const uint32_t* orderIn, uint32_t* orderOut, uint32_t* hist,
-                   hipStream_t s);
+                   void* ws, size_t wsBytes, hipStream_t s);
 
 void launchFillU32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+// p[i] = i for i < n
+void launchIota(uint32_t* p, uint64_t n, hipStream_t s);
 
 }  // namespace rbow_amd

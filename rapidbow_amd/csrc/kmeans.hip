@@ -56,6 +56,11 @@ __global__ void k_gatherRows(const uint8_t* __restrict__ desc, const uint32_t* _
     }
 }
 
+__global__ void k_iota(uint32_t* __restrict__ p, uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        p[i] = uint32_t(i);
+}
+
 // reformatDesc.cu:38-74: tiles[t][word][slot] = word of desc[idx[t][slot]].
 __global__ void k_reformatTiles(const uint8_t* __restrict__ desc, const uint32_t* __restrict__ idx, uint32_t nbTiles,
                                 uint32_t tileSize, uint32_t* __restrict__ tiles) {
@@ -249,13 +254,14 @@ __device__ inline void waveRange(uint32_t nbTiles, uint32_t& beg, uint32_t& end)
 // Sums: mfma_i32_16x16x64_i8(onehot^T, XT) -> per (center, dim) sums of x' over
 // the tile's points (exact int32): the update's centroid sums on the matrix
 // cores, with no atomics.  XT is the transposed operand, byte q of lane l =
-// x'[point 16*(q>>2) + 4*(l>>4) + (q&3)][dim 16*db + (l&15)].  It is not read
-// from HBM: it is made on chip by multiplying the XA fragments with 0/1
-// selection matrices (the C layout of a 16x16 MFMA puts rows 4*(l>>4)+i of
-// column l&15 in lane l, which is exactly that byte order), so each point
-// costs 128 bytes of HBM per pass.  The 4x4 rows a lane owns after the
-// distance MFMA are the 16 points of its XT K slice, so the one-hot operand
-// is local.
+// x'[point 16*(q>>2) + 4*(l>>4) + (q&3)][dim 16*db + (l&15)] (the C layout of a
+// 16x16 MFMA puts rows 4*(l>>4)+i of column l&15 in lane l, so the 4x4 rows a
+// lane owns after the distance MFMA are the 16 points of its XT K slice and
+// the one-hot operand is local).  XT is not read from HBM: the tile is put in
+// LDS row-major (144-byte pitch) and read back with gfx950's transposing LDS
+// load ds_read_b64_tr_b8, which within each 16-lane group gives output lane i,
+// byte r = byte (i % 8) of the 8 bytes addressed by lane 2r + i/8.  So each
+// point costs 128 bytes of HBM per pass.
 __global__ __launch_bounds__(256) void k_gatherTiles(const uint8_t* __restrict__ desc, const uint32_t* __restrict__ order,
                                                     const KmTile* __restrict__ tiles, int8_t* __restrict__ XA) {
     __shared__ __align__(16) uint8_t sx[64][144];
@@ -282,11 +288,6 @@ __global__ __launch_bounds__(256) void k_gatherTiles(const uint8_t* __restrict__
     }
 }
 
-__device__ inline int packLowBytes(v4i t) {
-    return int((uint32_t(t[0]) & 0xFFu) | ((uint32_t(t[1]) & 0xFFu) << 8) | ((uint32_t(t[2]) & 0xFFu) << 16) |
-               (uint32_t(t[3]) << 24));
-}
-
 __device__ inline void loadTileA(const int8_t* xa, uint32_t lane, v4i (&a)[4][2]) {
 #pragma unroll
     for (int rb = 0; rb < 4; rb++)
@@ -294,17 +295,119 @@ __device__ inline void loadTileA(const int8_t* xa, uint32_t lane, v4i (&a)[4][2]
         for (int kb = 0; kb < 2; kb++) a[rb][kb] = *reinterpret_cast<const v4i*>(xa + ((rb * 2 + kb) * 64 + lane) * 16);
 }
 
+// A 64-row tile of x' in LDS, row-major with a 144-byte pitch (rows 8 words
+// apart in the bank space, so the transposing reads below do not conflict).
+constexpr uint32_t kRowPitch = 144;
+constexpr uint32_t kTileLds = 64 * kRowPitch;
+
+__device__ inline uint32_t fragOffset(int rb, int kb, uint32_t lane) {
+    return (rb * 16 + (lane & 15)) * kRowPitch + kb * 64 + (lane >> 4) * 16;
+}
+// A fragments (XA layout) <-> the row-major LDS tile
+__device__ inline void storeTileLds(uint8_t* t, uint32_t lane, const v4i (&a)[4][2]) {
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int kb = 0; kb < 2; kb++) *reinterpret_cast<v4i*>(t + fragOffset(rb, kb, lane)) = a[rb][kb];
+}
+__device__ inline void loadTileLds(const uint8_t* t, uint32_t lane, v4i (&a)[4][2]) {
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int kb = 0; kb < 2; kb++) a[rb][kb] = *reinterpret_cast<const v4i*>(t + fragOffset(rb, kb, lane));
+}
+
+typedef int v2i __attribute__((ext_vector_type(2)));
+__device__ inline v2i readTr8(const uint8_t* p) {
+    typedef __attribute__((address_space(3))) v2i LdsV2i;  // p points into LDS
+    return __builtin_amdgcn_ds_read_tr8_b64_v2i32((LdsV2i*)(const_cast<uint8_t*>(p)));
+}
+// XT fragment db of an LDS tile.  Lane t = lane & 15 of group grp = lane >> 4
+// addresses, for output bytes q = r (first read) and q = 8 + r (second) with
+// r = t >> 1, dims 16*db + 8*(t & 1) .. +8 of point 16*(q>>2) + 4*grp + (q&3);
+// the transposing read then hands lane i of the group dim 16*db + i of those
+// 8 points.
+__device__ inline v4i loadXt(const uint8_t* t, uint32_t lane, int db) {
+    const uint32_t tt = lane & 15, r = tt >> 1, p0 = 16 * (r >> 2) + 4 * (lane >> 4) + (r & 3);
+    const uint8_t* base = t + p0 * kRowPitch + 16 * db + 8 * (tt & 1);
+    const v2i lo = readTr8(base), hi = readTr8(base + 32 * kRowPitch);  // q = 8 + r: point p0 + 32
+    return v4i{lo.x, lo.y, hi.x, hi.y};
+}
+
+// Center B fragments of `crow` (center `col`'s row) and ||c - 128||^2 of center col.
+__device__ inline int centerOperands(const uint8_t* crow, uint32_t lane, v4i& b0, v4i& b1) {
+    const uint32_t grp = lane >> 4;
+    b0 = toSigned(ld16(crow + grp * 16));
+    b1 = toSigned(ld16(crow + 64 + grp * 16));
+    int ns = 0;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        ns = __builtin_amdgcn_sdot4(b0[e], b0[e], ns, false);
+        ns = __builtin_amdgcn_sdot4(b1[e], b1[e], ns, false);
+    }
+    ns += __shfl_xor(ns, 16, 64);
+    ns += __shfl_xor(ns, 32, 64);
+    return ns;
+}
+
+// One tile of cnt (<= 64) rows: distances by I8 MFMA, argmin (ties to the
+// lowest center), the new assignments compared with `prev` (lane r: previous
+// assignment of row r), and the one-hot sums of x' added into acc (XT from
+// the same tile in LDS, `tile`).  Returns the new assignment of row `lane`
+// (meaningful for lane < cnt).
+__device__ __forceinline__ uint32_t assignTileMfma(const v4i (&a)[4][2], const uint8_t* tile, uint32_t cnt,
+                                                   uint32_t prev, const v4i& b0, const v4i& b1, int ns,
+                                                   uint32_t* sNewWave, uint32_t lane, v4i (&acc)[8], uint32_t& myCnt,
+                                                   uint32_t& myChanged) {
+    const uint32_t col = lane & 15, grp = lane >> 4;
+    const v4i zero = {0, 0, 0, 0};
+    uint32_t key[4][4];
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+        v4i d = zero;
+        d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rb][0], b0, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rb][1], b1, d, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) key[rb][i] = (uint32_t(ns - 2 * d[i] + (1 << 25)) << 4) | col;
+    }
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) key[rb][i] = min16u(key[rb][i]);
+    // new assignments to lane-per-row order (rows rb*16 + 4*grp + i live in group grp)
+    if (col == 0) {
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++)
+            sNewWave[rb * 4 + grp] = (key[rb][0] & 15u) | ((key[rb][1] & 15u) << 8) | ((key[rb][2] & 15u) << 16) |
+                                     ((key[rb][3] & 15u) << 24);
+    }
+    waveSync();
+    const uint32_t mine = (sNewWave[lane >> 2] >> (8 * (lane & 3))) & 15u;
+    waveSync();
+    if (lane < cnt) myChanged += prev != mine;
+    v4i oh;
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t row = rb * 16 + 4 * grp + i;
+            w |= uint32_t(row < cnt && (key[rb][i] & 15u) == col) << (8 * i);
+        }
+        oh[rb] = int(w);
+    }
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) myCnt = __builtin_amdgcn_udot4(uint32_t(oh[rb]), 0x01010101u, myCnt, false);
+#pragma unroll
+    for (int db = 0; db < 8; db++) acc[db] = __builtin_amdgcn_mfma_i32_16x16x64_i8(oh, loadXt(tile, lane, db), acc[db], 0, 0, 0);
+    return mine;
+}
+
 __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
     __shared__ uint32_t sNew[kWaves][16];  // a tile's new assignments, one byte per row
+    __shared__ __align__(16) uint8_t sTile[kWaves][kTileLds];  // the tile, row-major, for the XT reads
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 15, grp = lane >> 4;
-    // esel[g]: B operand that selects dims g*16 + col of a 64-dim K chunk
-    v4i esel[4];
-#pragma unroll
-    for (int g = 0; g < 4; g++)
-#pragma unroll
-        for (int d = 0; d < 4; d++)
-            esel[g][d] = (grp == uint32_t(g) && (col >> 2) == uint32_t(d)) ? int(1u << (8 * (col & 3))) : 0;
     const v4i zero = {0, 0, 0, 0};
     // one wave per resident slot, each with an equal range of tiles cut into
     // per-task chunks on the host (partitionKmTiles): no wave waits for a
@@ -331,16 +434,8 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
     while (ch < chEnd) {
         const KmChunk c = k.chunks[ch];
         const uint32_t chNext = nextActive(ch + 1);
-        const uint8_t* crow = k.centersU8 + (uint64_t(c.task) * 16 + col) * kDims;
-        const v4i b0 = toSigned(ld16(crow + grp * 16)), b1 = toSigned(ld16(crow + 64 + grp * 16));
-        int ns = 0;
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            ns = __builtin_amdgcn_sdot4(b0[e], b0[e], ns, false);
-            ns = __builtin_amdgcn_sdot4(b1[e], b1[e], ns, false);
-        }
-        ns += __shfl_xor(ns, 16, 64);
-        ns += __shfl_xor(ns, 32, 64);  // ||c - 128||^2 of center `col`
+        v4i b0, b1;
+        const int ns = centerOperands(k.centersU8 + (uint64_t(c.task) * 16 + col) * kDims, lane, b0, b1);
         v4i acc[8];
 #pragma unroll
         for (int db = 0; db < 8; db++) acc[db] = zero;
@@ -357,54 +452,12 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
                 loadTileA(k.XA + uint64_t(tiN) * 8192, lane, an);
                 prevN = lane < tn.cnt ? k.assign[tn.row + lane] : 0u;
             }
-            uint32_t key[4][4];
-#pragma unroll
-            for (int rb = 0; rb < 4; rb++) {
-                v4i d = zero;
-                d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rb][0], b0, d, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rb][1], b1, d, 0, 0, 0);
-#pragma unroll
-                for (int i = 0; i < 4; i++) key[rb][i] = (uint32_t(ns - 2 * d[i] + (1 << 25)) << 4) | col;
-            }
-#pragma unroll
-            for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) key[rb][i] = min16u(key[rb][i]);
-            // new assignments to lane-per-row order (rows rb*16 + 4*grp + i live in group grp)
-            if (col == 0) {
-#pragma unroll
-                for (int rb = 0; rb < 4; rb++)
-                    sNew[wave][rb * 4 + grp] = (key[rb][0] & 15u) | ((key[rb][1] & 15u) << 8) |
-                                               ((key[rb][2] & 15u) << 16) | ((key[rb][3] & 15u) << 24);
-            }
+            waveSync();  // the previous tile's transposing reads are done
+            storeTileLds(sTile[wave], lane, a);
             waveSync();
-            const uint32_t mine = (sNew[wave][lane >> 2] >> (8 * (lane & 3))) & 15u;
-            waveSync();
-            if (lane < t.cnt) {
-                myChanged += prev != mine;
-                k.assign[t.row + lane] = uint8_t(mine);
-            }
-            v4i oh;
-#pragma unroll
-            for (int rb = 0; rb < 4; rb++) {
-                uint32_t w = 0;
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const uint32_t row = rb * 16 + 4 * grp + i;
-                    w |= uint32_t(row < t.cnt && (key[rb][i] & 15u) == col) << (8 * i);
-                }
-                oh[rb] = int(w);
-            }
-#pragma unroll
-            for (int rb = 0; rb < 4; rb++) myCnt = __builtin_amdgcn_udot4(uint32_t(oh[rb]), 0x01010101u, myCnt, false);
-#pragma unroll
-            for (int db = 0; db < 8; db++) {
-                v4i bt;
-#pragma unroll
-                for (int rb = 0; rb < 4; rb++)
-                    bt[rb] = packLowBytes(__builtin_amdgcn_mfma_i32_16x16x64_i8(a[rb][db >> 2], esel[db & 3], zero, 0, 0, 0));
-                acc[db] = __builtin_amdgcn_mfma_i32_16x16x64_i8(oh, bt, acc[db], 0, 0, 0);
-            }
+            const uint32_t mine =
+                assignTileMfma(a, sTile[wave], t.cnt, prev, b0, b1, ns, sNew[wave], lane, acc, myCnt, myChanged);
+            if (lane < t.cnt) k.assign[t.row + lane] = uint8_t(mine);
             if (tiN != kIdxInvalid) {
 #pragma unroll
                 for (int rb = 0; rb < 4; rb++) {
@@ -427,6 +480,105 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
         for (int m = 32; m >= 1; m >>= 1) myChanged += uint32_t(__shfl_xor(int(myChanged), m, 64));
         if (lane == 0) k.partChanged[ch] = myChanged;
         ch = chNext;
+    }
+}
+
+// Persistent Lloyd loop with the points on chip, one workgroup per listed
+// task (tasks of at most maxTiles 64-row tiles).  The task's tiles are read
+// from HBM once into LDS, row-major (dynamic: maxTiles x 9 KB, then one
+// assignment byte per row); every pass then takes both MFMA operands from
+// LDS, so the loop is not bound by HBM.  Per pass the 4 waves take the tiles round
+// robin (the next tile's fragments in flight while one computes), their
+// one-hot sums and counts meet in LDS (int32 atomics), and the same rule as
+// k_updateU8Part applies: stop when no assignment changed, else move the
+// centers to round(float(sum)/float(count)).  Results equal those of the
+// one-launch-per-pass path for any pass cap.  At the end the assignments and
+// centers go to HBM and the task is marked inactive with its pass count.
+__global__ __launch_bounds__(256) void k_lloydTasks(KmLevel k, const uint32_t* __restrict__ taskList,
+                                                   const uint32_t* __restrict__ taskTileBeg, uint32_t maxTiles,
+                                                   uint32_t maxIters) {
+    extern __shared__ __align__(16) uint8_t dynLds[];
+    __shared__ uint32_t sNew[kWaves][16];
+    __shared__ int32_t sSum[16 * kDims];  // sums of x - 128
+    __shared__ uint32_t sCnt[16], sChanged;
+    __shared__ __align__(16) uint8_t sCent[16 * kDims];
+    const uint32_t g = taskList[blockIdx.x], tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t col = lane & 15;
+    const uint32_t n = k.taskCnt[g], row0 = k.taskBeg[g], nt = (n + 63) / 64;
+    if (nt == 0 || nt > maxTiles) return;  // the host lists tasks of 1..maxTiles tiles only
+    uint8_t* sA = dynLds;                                   // [nt] row-major tiles of kTileLds bytes
+    uint8_t* sAsg = dynLds + uint64_t(maxTiles) * kTileLds;  // [nt * 64]
+    const uint4* src = reinterpret_cast<const uint4*>(k.XA + uint64_t(taskTileBeg[g]) * 8192);
+    for (uint32_t i = tid; i < nt * 512; i += 256) {  // XA fragment (tile, rb, kb, lane) -> its rows in LDS
+        const uint32_t c = i & 511;
+        *reinterpret_cast<uint4*>(sA + (i >> 9) * kTileLds + fragOffset(c >> 7, (c >> 6) & 1, c & 63)) = src[i];
+    }
+    for (uint32_t i = tid; i < nt * 64; i += 256) sAsg[i] = 0xFF;  // no previous assignment
+    uint8_t* cent = k.centersU8 + uint64_t(g) * 16 * kDims;
+    if (tid < 128) reinterpret_cast<uint4*>(sCent)[tid] = reinterpret_cast<const uint4*>(cent)[tid];
+    for (uint32_t e = tid; e < 16 * kDims; e += 256) sSum[e] = 0;
+    if (tid < 16) sCnt[tid] = 0;
+    if (tid == 0) sChanged = 0;
+    const v4i zero = {0, 0, 0, 0};
+    const uint32_t grp = lane >> 4;
+    __syncthreads();
+    uint32_t pass = 0;
+    while (pass < maxIters) {
+        v4i b0, b1;
+        const int ns = centerOperands(sCent + col * kDims, lane, b0, b1);
+        v4i acc[8];
+#pragma unroll
+        for (int db = 0; db < 8; db++) acc[db] = zero;
+        uint32_t myCnt = 0, myChanged = 0;
+        v4i a[4][2];
+        if (wave < nt) loadTileLds(sA + wave * kTileLds, lane, a);
+        for (uint32_t i = wave; i < nt; i += kWaves) {
+            v4i an[4][2];
+            const bool more = i + kWaves < nt;
+            if (more) loadTileLds(sA + (i + kWaves) * kTileLds, lane, an);
+            const uint32_t cnt = min(64u, n - 64 * i);
+            const uint32_t prev = sAsg[i * 64 + lane];
+            const uint32_t mine =
+                assignTileMfma(a, sA + i * kTileLds, cnt, prev, b0, b1, ns, sNew[wave], lane, acc, myCnt, myChanged);
+            if (lane < cnt) sAsg[i * 64 + lane] = uint8_t(mine);
+            if (more) {
+#pragma unroll
+                for (int rb = 0; rb < 4; rb++) {
+                    a[rb][0] = an[rb][0];
+                    a[rb][1] = an[rb][1];
+                }
+            }
+        }
+        // the waves' partials meet in LDS
+#pragma unroll
+        for (int db = 0; db < 8; db++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (acc[db][i]) atomicAdd(&sSum[(4 * grp + i) * kDims + 16 * db + col], acc[db][i]);
+        myCnt += uint32_t(__shfl_xor(int(myCnt), 16, 64));
+        myCnt += uint32_t(__shfl_xor(int(myCnt), 32, 64));
+        if (lane < 16 && myCnt) atomicAdd(&sCnt[lane], myCnt);
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) myChanged += uint32_t(__shfl_xor(int(myChanged), m, 64));
+        if (lane == 0 && myChanged) atomicAdd(&sChanged, myChanged);
+        __syncthreads();
+        pass++;
+        if (sChanged == 0) break;  // converged on this pass: no update (uniform)
+        for (uint32_t e = tid; e < 16 * kDims; e += 256) {
+            const uint32_t c = sCnt[e / kDims];
+            if (c) sCent[e] = uint8_t(roundf(float((long long)sSum[e] + 128ll * c) / float(c)));
+        }
+        __syncthreads();  // every thread has read sCnt and sSum
+        for (uint32_t e = tid; e < 16 * kDims; e += 256) sSum[e] = 0;
+        if (tid < 16) sCnt[tid] = 0;
+        if (tid == 0) sChanged = 0;
+        __syncthreads();
+    }
+    for (uint32_t i = tid; i < n; i += 256) k.assign[row0 + i] = sAsg[i];
+    if (tid < 128) reinterpret_cast<uint4*>(cent)[tid] = reinterpret_cast<const uint4*>(sCent)[tid];
+    if (tid == 0) {
+        k.passes[g] = pass;
+        k.active[g] = 0;
     }
 }
 
@@ -652,15 +804,37 @@ __global__ __launch_bounds__(256) void k_updateU8Part(KmLevel k) {
 }
 
 // Regroup keys (Vocabulary.cpp:83-98): key of row r = task * b + assignment, and
-// the per-(task, child) histogram (the occurrence counts).
-__global__ __launch_bounds__(64) void k_regroupKeys(KmLevel k, uint32_t* __restrict__ keys, uint32_t* __restrict__ hist) {
-    const KmTile t = k.tiles[blockIdx.x];
-    const uint32_t r = threadIdx.x;
-    if (r < t.cnt) {
-        const uint32_t key = t.task * k.b + k.assign[t.row + r];
-        keys[t.row + r] = key;
-        atomicAdd(&hist[key], 1u);
+// the per-(task, child) histogram (the occurrence counts).  Each wave walks a
+// contiguous range of tiles (tasks are contiguous), keeps the current task's
+// child counts in lanes 0..b-1 and adds them to the histogram when the task
+// changes, so a level costs about one atomic per (wave, child) rather than one
+// per row on a handful of addresses.
+__global__ __launch_bounds__(256) void k_regroupKeys(KmLevel k, uint32_t* __restrict__ keys, uint32_t* __restrict__ hist) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t beg, end;
+    waveRange(k.nbTiles, beg, end);
+    uint32_t task = kIdxInvalid, mine = 0;
+    auto flush = [&]() {
+        if (task != kIdxInvalid && lane < k.b && mine) atomicAdd(&hist[task * k.b + lane], mine);
+        mine = 0;
+    };
+    for (uint32_t ti = beg; ti < end; ti++) {
+        const KmTile t = k.tiles[ti];
+        if (t.task != task) {
+            flush();
+            task = t.task;
+        }
+        // every row holds an assignment < b after a level's first pass; a row that
+        // does not is left out of the histogram (the host checks its total)
+        const bool valid = lane < t.cnt;
+        const uint32_t a = valid ? k.assign[t.row + lane] : 0xFFu;
+        if (valid) keys[t.row + lane] = a < k.b ? t.task * k.b + a : ~0u;
+        for (uint32_t j = 0; j < k.b; j++) {
+            const uint32_t c = uint32_t(__builtin_popcountll(__ballot(a == j)));
+            if (lane == j) mine += c;
+        }
     }
+    flush();
 }
 
 __global__ void k_levelReset(KmLevel k, uint64_t nbRows) {
@@ -685,6 +859,12 @@ uint32_t gridFor(uint64_t work, uint32_t perBlock, uint32_t cap) {
 void launchGatherRows(const uint8_t* desc, const uint32_t* idx, uint64_t n, uint8_t* out, hipStream_t s) {
     if (!n) return;
     k_gatherRows<<<gridFor(n * 8, 256, 8192), 256, 0, s>>>(desc, idx, n, out);
+    RBOW_HIP(hipGetLastError());
+}
+
+void launchIota(uint32_t* p, uint64_t n, hipStream_t s) {
+    if (!n) return;
+    k_iota<<<gridFor(n, 256, 4096), 256, 0, s>>>(p, n);
     RBOW_HIP(hipGetLastError());
 }
 
@@ -746,39 +926,44 @@ uint32_t kmAssignResidentBlocks() {
     return blocks;
 }
 
-KmPartition partitionKmTiles(const std::vector<uint32_t>& taskTiles) {
-    KmPartition p;
-    const uint32_t nbTasks = uint32_t(taskTiles.size());
-    uint64_t T = 0;
-    for (uint32_t n : taskTiles) T += n;
-    p.taskChunkBeg.assign(nbTasks, 0u);
-    p.taskChunkCnt.assign(nbTasks, 0u);
-    p.nbWaves = uint32_t(std::min<uint64_t>(uint64_t(kmAssignResidentBlocks()) * kWaves, T));
-    p.waveChunkBeg.assign(p.nbWaves + 1, 0u);
-    uint32_t g = 0;        // current task
-    uint64_t gBeg = 0;     // its first tile
-    while (g < nbTasks && taskTiles[g] == 0) p.taskChunkBeg[g++] = 0;
-    for (uint32_t w = 0; w < p.nbWaves; w++) {
-        p.waveChunkBeg[w] = uint32_t(p.chunks.size());
-        uint64_t lo = T * w / p.nbWaves;
-        const uint64_t hi = T * (w + 1) / p.nbWaves;
-        while (lo < hi) {
-            while (lo >= gBeg + taskTiles[g]) {  // next non-empty task
-                gBeg += taskTiles[g];
-                g++;
-                p.taskChunkBeg[g] = uint32_t(p.chunks.size());
-            }
-            const uint64_t end = std::min<uint64_t>(hi, gBeg + taskTiles[g]);
-            if (p.taskChunkCnt[g] == 0) p.taskChunkBeg[g] = uint32_t(p.chunks.size());
-            p.chunks.push_back(KmChunk{g, uint32_t(lo), uint32_t(end)});
-            p.taskChunkCnt[g]++;
-            lo = end;
-        }
+KmPartition partitionKmTiles(const std::vector<uint32_t>& taskTiles, const std::vector<uint32_t>& tileBeg) {
+    return partitionTiles(taskTiles, tileBeg, uint64_t(kmAssignResidentBlocks()) * kWaves);
+}
+
+namespace {
+constexpr uint32_t kLloydTileLds = kTileLds + 64;  // LDS per tile: the rows + their assignments
+}
+
+uint32_t lloydTasksTileCap(uint32_t blocksPerCu) {
+    static thread_local int device = -1;
+    static thread_local int64_t perBlock = 0, perCu = 0, fixed = 0;
+    int d = 0;
+    RBOW_HIP(hipGetDevice(&d));
+    if (d != device) {
+        int maxBlock = 0, maxCu = 0;
+        hipFuncAttributes fa{};
+        RBOW_HIP(hipDeviceGetAttribute(&maxBlock, hipDeviceAttributeMaxSharedMemoryPerBlock, d));
+        RBOW_HIP(hipDeviceGetAttribute(&maxCu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, d));
+        RBOW_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_lloydTasks)));
+        perBlock = maxBlock;
+        perCu = maxCu;
+        fixed = int64_t(fa.sharedSizeBytes);
+        device = d;
     }
-    p.waveChunkBeg[p.nbWaves] = uint32_t(p.chunks.size());
-    for (g = 0; g < nbTasks; g++)
-        if (p.taskChunkCnt[g] == 0) p.taskChunkBeg[g] = 0;
-    return p;
+    const int64_t room = std::min(perBlock, perCu / int64_t(std::max(blocksPerCu, 1u))) - fixed;
+    return room > 0 ? uint32_t(room / kLloydTileLds) : 0u;
+}
+
+void launchLloydTasks(const KmLevel& k, const uint32_t* taskList, uint32_t nbList, uint32_t maxTiles,
+                      const uint32_t* taskTileBeg, uint32_t maxIters, hipStream_t s) {
+    if (!nbList) return;
+    RBOW_REQUIRE(maxTiles >= 1 && maxTiles <= lloydTasksTileCap(1), RBOW_ERR_INVALID_ARG,
+                 "persistent k-means tasks exceed the LDS capacity");
+    const int dyn = int(maxTiles * kLloydTileLds);
+    RBOW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_lloydTasks),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    k_lloydTasks<<<nbList, 256, dyn, s>>>(k, taskList, taskTileBeg, maxTiles, maxIters);
+    RBOW_HIP(hipGetLastError());
 }
 
 void launchKmAssign(const KmLevel& k, int kernel, hipStream_t s) {
@@ -795,27 +980,43 @@ void launchKmAssign(const KmLevel& k, int kernel, hipStream_t s) {
     RBOW_HIP(hipGetLastError());
 }
 
+namespace {
+int keyBits(uint64_t nbKeys) {
+    int bits = 1;
+    while ((1ull << bits) < nbKeys) bits++;
+    return bits;
+}
+size_t sortTempBytes(uint32_t nbRows, int bits) {
+    size_t bytes = 0;
+    RBOW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                (uint32_t*)nullptr, (uint32_t*)nullptr, int(nbRows), 0, bits));
+    return bytes;
+}
+}  // namespace
+
+size_t regroupWorkspaceBytes(uint32_t nbRows, uint64_t nbKeysMax) {
+    size_t temp = 0;
+    for (int bits = 1; bits <= keyBits(nbKeysMax); bits++) temp = std::max(temp, sortTempBytes(nbRows, bits));
+    return 8ull * nbRows + temp + 256;
+}
+
 void launchRegroup(const KmLevel& k, uint32_t nbRows, const uint32_t* orderIn, uint32_t* orderOut, uint32_t* hist,
-                   hipStream_t s) {
+                   void* ws, size_t wsBytes, hipStream_t s) {
     const uint64_t nbKeys = uint64_t(k.nbTasks) * k.b;
     launchFillU32(hist, 0u, nbKeys, s);
     if (!nbRows) return;
-    int bits = 1;
-    while ((1ull << bits) < nbKeys) bits++;
-    size_t tempBytes = 0;
-    RBOW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tempBytes, (uint32_t*)nullptr, (uint32_t*)nullptr, orderIn,
-                                                orderOut, int(nbRows), 0, bits, s));
-    void* ws = nullptr;
-    RBOW_HIP(hipMallocAsync(&ws, 8ull * nbRows + tempBytes + 256, s));
+    const int bits = keyBits(nbKeys);
+    size_t tempBytes = sortTempBytes(nbRows, bits);
     uint32_t* keys = static_cast<uint32_t*>(ws);
     uint32_t* keysSorted = keys + nbRows;
     void* temp = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(keysSorted + nbRows) + 255) & ~uintptr_t(255));
-    k_regroupKeys<<<k.nbTiles, 64, 0, s>>>(k, keys, hist);
+    RBOW_REQUIRE(static_cast<uint8_t*>(temp) + tempBytes <= static_cast<uint8_t*>(ws) + wsBytes, RBOW_ERR_HIP,
+                 "regroup workspace too small");
+    k_regroupKeys<<<gridFor(k.nbTiles, kWaves * 8, 1024), 256, 0, s>>>(k, keys, hist);
     // LSD radix sort is stable: rows keep their order inside each (task, child)
     RBOW_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, keys, keysSorted, orderIn, orderOut, int(nbRows), 0,
                                                 bits, s));
     RBOW_HIP(hipGetLastError());
-    RBOW_HIP(hipFreeAsync(ws, s));
 }
 
 void launchKmUpdate(const KmLevel& k, hipStream_t s) {

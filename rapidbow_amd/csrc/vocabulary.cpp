@@ -10,6 +10,8 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <numeric>
 #include <random>
@@ -68,6 +70,13 @@ TreeView Vocabulary::view() const {
 
 namespace {
 
+uint32_t deviceCUs() {
+    int d = 0, cus = 0;
+    RBOW_HIP(hipGetDevice(&d));
+    RBOW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d));
+    return uint32_t(std::max(cus, 1));
+}
+
 // kmeans.cpp:141-168 randPick: pick min(randSpace, nbToPick) distinct members
 // with the persistent engine, sorted; pad with the last pick (0 if none).
 uint32_t randPick(std::mt19937_64& rng, uint32_t* dst, uint32_t randSpace, uint32_t nbToPick) {
@@ -89,13 +98,54 @@ uint32_t randPick(std::mt19937_64& rng, uint32_t* dst, uint32_t randSpace, uint3
 }
 
 template <typename T>
-void h2d(T* dst, const T* src, size_t n, hipStream_t s) {
-    if (n) RBOW_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
-}
-template <typename T>
 void d2h(T* dst, const T* src, size_t n, hipStream_t s) {
     if (n) RBOW_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
 }
+
+// One level's host -> device metadata, laid out in a pinned host arena and sent
+// with a single asynchronous copy into a device arena of the same layout.  The
+// host side is rewritten only after the level's final stream sync.
+class UploadArena {
+public:
+    explicit UploadArena(size_t bytes) : dev_(bytes), cap_(bytes) {
+        RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_), std::max<size_t>(bytes, 1)));
+    }
+    ~UploadArena() {
+        if (host_) (void)hipHostFree(host_);
+    }
+    UploadArena(const UploadArena&) = delete;
+    UploadArena& operator=(const UploadArena&) = delete;
+    void reset() { used_ = 0; }
+    // room for n elements: host pointer to fill, *dev = the same bytes on the device
+    template <typename T>
+    T* take(size_t n, const T** dev) {
+        const size_t off = (used_ + 255) & ~size_t(255);
+        RBOW_REQUIRE(off + n * sizeof(T) <= cap_, RBOW_ERR_HIP, "level upload arena overflow");
+        used_ = off + n * sizeof(T);
+        *dev = reinterpret_cast<const T*>(dev_.get() + off);
+        return reinterpret_cast<T*>(host_ + off);
+    }
+    template <typename T>
+    const T* put(const T* src, size_t n) {
+        const T* dev = nullptr;
+        T* h = take<T>(n, &dev);
+        if (n) std::memcpy(h, src, n * sizeof(T));
+        return dev;
+    }
+    void upload(hipStream_t s) {
+        if (used_) RBOW_HIP(hipMemcpyAsync(dev_.get(), host_, used_, hipMemcpyHostToDevice, s));
+    }
+    static size_t bytesFor(std::initializer_list<size_t> regionBytes) {
+        size_t t = 0;
+        for (size_t r : regionBytes) t += r + 256;
+        return t;
+    }
+
+private:
+    DevBuf<uint8_t> dev_;
+    uint8_t* host_ = nullptr;
+    size_t cap_ = 0, used_ = 0;
+};
 
 }  // namespace
 
@@ -110,7 +160,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     const uint32_t ct = opts.centroid_type;
     const int kernel = int(opts.kernel);
     const auto t0 = std::chrono::steady_clock::now();
-    std::vector<hipEvent_t> ev;  // start/stop pairs around assign and update launches
+    std::vector<hipEvent_t> ev;  // [0, 1]: persistent launch; 2 + 5 * pass + i: assign/reduce/update marks
     struct EventsFree {
         std::vector<hipEvent_t>& v;
         ~EventsFree() {
@@ -125,8 +175,9 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         }
         RBOW_HIP(hipEventRecord(ev[i], s));
     };
-    double assignMs = 0, updateMs = 0, reduceMs = 0;
-    uint32_t assignLaunches = 0;
+    double assignMs = 0, updateMs = 0, reduceMs = 0, streamMs = 0, persistTotalMs = 0;
+    uint32_t streamLaunches = 0;
+    uint64_t streamPointPasses = 0;
     uint64_t pointPasses = 0;
 
     std::unique_ptr<Vocabulary> voc(new Vocabulary);
@@ -135,102 +186,79 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     std::vector<uint32_t> occ(voc->nbChildren(), 0u);
     std::vector<uint32_t> passesAll(voc->nbNodes, 0u);
 
-    // host grouping state: descriptor order of the level, tasks contiguous
-    // the level order lives on the device; the host keeps a copy only for the
-    // reference-mode seeds, which index it
-    std::vector<uint32_t> order(nbDesc);
-    std::iota(order.begin(), order.end(), 0u);
+    // host grouping state: the level order lives on the device; the host keeps a
+    // copy only for the reference-mode seeds, which index it
+    std::vector<uint32_t> order;
+    if (opts.seed_mode == 0) {
+        order.resize(nbDesc);
+        std::iota(order.begin(), order.end(), 0u);
+    }
     std::vector<uint32_t> taskBeg{0u}, taskCnt{nbDesc};
 
-    DevBuf<uint32_t> dOrder(nbDesc), dOrderNext(nbDesc), dHist;
-    DevBuf<uint8_t> dX(uint64_t(nbDesc) * kDims), dAssign(nbDesc);
-    DevBuf<uint32_t> dTaskBeg, dTaskCnt, dCounts, dChanged, dActive, dPasses, dNbActive(1), dSeed, dMasks;
-    DevBuf<long long> dSums;
-    DevBuf<KmTile> dTiles;
-    DevBuf<KmChunk> dChunks, dRed;
-    DevBuf<int8_t> dXA;
-    DevBuf<int32_t> dPartSums;
-    DevBuf<uint32_t> dPartCnt, dPartChanged, dTaskChunkBeg, dTaskChunkCnt;
-    std::vector<KmChunk> chunks, red;
-    std::vector<uint32_t> taskChunkBeg, taskChunkCnt, waveChunkBeg;
-    uint32_t nbWaves = 0;
-    DevBuf<uint32_t> dWaveChunkBeg;
+    // ---- every device buffer sized once for the largest level ----
     const bool mfmaPath = ct == 0 && kernel == 0;
-    std::vector<uint32_t> hPasses, hMasks;
-    std::vector<KmTile> tiles;
-    std::mt19937_64 rng(opts.seed);
-    uint32_t* hNbActive = nullptr;
-    RBOW_HIP(hipHostMalloc(&hNbActive, sizeof(uint32_t)));
+    const uint64_t nGmax = uint64_t(1) << (lg2b * (L - 1));  // tasks of the last level
+    const uint64_t tilesMax = divUp(nbDesc, 64u) + nGmax;   // each task adds at most one partial tile
+    const uint64_t wavesMax = mfmaPath ? uint64_t(kmAssignResidentBlocks()) * 4 : 0;
+    const uint64_t chunksMax = mfmaPath ? wavesMax + nGmax : 0;  // a wave's range splits at task boundaries
+    const uint64_t redMax = mfmaPath ? chunksMax / 32 + nGmax : 0;
+    const uint32_t persistTiles = mfmaPath ? std::min(opts.persist_max_tiles, lloydTasksTileCap(1)) : 0u;
+    const uint32_t persistTiles2 = mfmaPath ? std::min(persistTiles, lloydTasksTileCap(2)) : 0u;  // 2 per CU
+    DevBuf<uint32_t> dOrder(nbDesc), dOrderNext(nbDesc);
+    DevBuf<uint8_t> dX(mfmaPath ? 0 : uint64_t(nbDesc) * kDims), dAssign(nbDesc);
+    DevBuf<uint32_t> dCounts(nGmax * b), dChanged(nGmax), dActive(nGmax), dNbActive(1), dSeed(nGmax * b);
+    DevBuf<long long> dSums(nGmax * b * kDims);
+    DevBuf<int8_t> dXA(mfmaPath ? tilesMax * 8192 : 0);
+    DevBuf<int32_t> dPartSums(chunksMax * 16 * kDims);
+    DevBuf<uint32_t> dPartCnt(chunksMax * 16), dPartChanged(chunksMax);
+    DevBuf<uint8_t> dSortWs(regroupWorkspaceBytes(nbDesc, nGmax * b));
+    // per-level results, downloaded with one copy: [hist nG*b][passes nG][masks nG]
+    DevBuf<uint32_t> dOut(nGmax * b + 2 * nGmax);
+    uint32_t* hOut = nullptr;
+    RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&hOut), 4 * (nGmax * b + 2 * nGmax + 1)));
+    uint32_t* hNbActive = hOut + nGmax * b + 2 * nGmax;
     struct PinnedFree {
         uint32_t* p;
         ~PinnedFree() { (void)hipHostFree(p); }
-    } pinnedGuard{hNbActive};
+    } pinnedGuard{hOut};
+    UploadArena up(UploadArena::bytesFor({4 * nGmax, 4 * nGmax, sizeof(KmTile) * tilesMax, 4 * nGmax, 4 * nGmax,
+                                          4 * (wavesMax + 1), sizeof(KmChunk) * chunksMax, 4 * nGmax, 4 * nGmax,
+                                          sizeof(KmChunk) * redMax, opts.seed_mode == 0 ? 4 * nGmax * b : 0}));
+
+    std::vector<uint32_t> taskTiles, tileBeg, smallTasks;
+    std::vector<KmChunk> red;
+    std::mt19937_64 rng(opts.seed);
+    const uint32_t cus = deviceCUs();
 
     for (uint32_t level = 0; level < L; level++) {
         const uint32_t nG = uint32_t(taskBeg.size());
         const uint64_t nodeBase = levelBeg(lg2b, level);
+        up.reset();
+        const uint32_t* dTaskBeg = up.put(taskBeg.data(), nG);
+        const uint32_t* dTaskCnt = up.put(taskCnt.data(), nG);
         // ---- tiles of <= 64 rows, tasks contiguous (DescGrouping, kmeans.cpp:27-42) ----
-        tiles.clear();
-        for (uint32_t g = 0; g < nG; g++)
-            for (uint32_t r = 0; r < taskCnt[g]; r += 64) tiles.push_back(KmTile{g, taskBeg[g] + r, std::min(64u, taskCnt[g] - r)});
-        dTaskBeg.reserve(nG);
-        dTaskCnt.reserve(nG);
-        dChanged.reserve(nG);
-        dActive.reserve(nG);
-        dPasses.reserve(nG);
-        dMasks.reserve(nG);
-        dSeed.reserve(uint64_t(nG) * b);
-        dCounts.reserve(uint64_t(nG) * b);
-        dSums.reserve(uint64_t(nG) * b * kDims);
-        dTiles.reserve(std::max<size_t>(tiles.size(), 1));
-        if (level == 0) h2d(dOrder.get(), order.data(), nbDesc, s);
-        h2d(dTaskBeg.get(), taskBeg.data(), nG, s);
-        h2d(dTaskCnt.get(), taskCnt.data(), nG, s);
-        h2d(dTiles.get(), tiles.data(), tiles.size(), s);
-        if (mfmaPath) {
-            // equal tile ranges per resident wave, cut into per-task chunks; reduce items of <= 32 chunks
-            std::vector<uint32_t> taskTiles(nG);
-            for (uint32_t g = 0; g < nG; g++) taskTiles[g] = divUp(taskCnt[g], 64);
-            KmPartition part = partitionKmTiles(taskTiles);
-            chunks.swap(part.chunks);
-            taskChunkBeg.swap(part.taskChunkBeg);
-            taskChunkCnt.swap(part.taskChunkCnt);
-            waveChunkBeg.swap(part.waveChunkBeg);
-            nbWaves = part.nbWaves;
-            red.clear();
-            for (uint32_t g = 0; g < nG; g++)
-                if (taskChunkCnt[g] > 32)  // small tasks are reduced inside the update kernel
-                    for (uint32_t c = 0; c < taskChunkCnt[g]; c += 32)
-                        red.push_back(
-                            KmChunk{g, taskChunkBeg[g] + c, taskChunkBeg[g] + std::min(taskChunkCnt[g], c + 32)});
-            dWaveChunkBeg.reserve(waveChunkBeg.size());
-            h2d(dWaveChunkBeg.get(), waveChunkBeg.data(), waveChunkBeg.size(), s);
-            dChunks.reserve(std::max<size_t>(chunks.size(), 1));
-            dRed.reserve(std::max<size_t>(red.size(), 1));
-            dTaskChunkBeg.reserve(nG);
-            dTaskChunkCnt.reserve(nG);
-            dXA.reserve(std::max<size_t>(tiles.size(), 1) * 8192);
-            dPartSums.reserve(std::max<size_t>(chunks.size(), 1) * 16 * kDims);
-            dPartCnt.reserve(std::max<size_t>(chunks.size(), 1) * 16);
-            dPartChanged.reserve(std::max<size_t>(chunks.size(), 1));
-            h2d(dChunks.get(), chunks.data(), chunks.size(), s);
-            h2d(dRed.get(), red.data(), red.size(), s);
-            h2d(dTaskChunkBeg.get(), taskChunkBeg.data(), nG, s);
-            h2d(dTaskChunkCnt.get(), taskChunkCnt.data(), nG, s);
-            launchGatherTiles(devDesc, dOrder.get(), dTiles.get(), uint32_t(tiles.size()), dXA.get(), s);
-        } else {
-            launchGatherRows(devDesc, dOrder.get(), nbDesc, dX.get(), s);
+        taskTiles.assign(nG, 0u);
+        tileBeg.assign(nG, 0u);
+        uint32_t nbTiles = 0;
+        for (uint32_t g = 0; g < nG; g++) {
+            taskTiles[g] = divUp(taskCnt[g], 64u);
+            tileBeg[g] = nbTiles;
+            nbTiles += taskTiles[g];
         }
+        const KmTile* dTiles = nullptr;
+        KmTile* hTiles = up.take<KmTile>(nbTiles, &dTiles);
+        for (uint32_t g = 0, t = 0; g < nG; g++)
+            for (uint32_t r = 0; r < taskCnt[g]; r += 64) hTiles[t++] = KmTile{g, taskBeg[g] + r, std::min(64u, taskCnt[g] - r)};
 
         KmLevel k;
         k.b = b;
         k.nbTasks = nG;
-        k.nbTiles = uint32_t(tiles.size());
+        k.nbTiles = nbTiles;
         k.centroidType = ct;
         k.X = dX.get();
-        k.tiles = dTiles.get();
-        k.taskBeg = dTaskBeg.get();
-        k.taskCnt = dTaskCnt.get();
+        k.tiles = dTiles;
+        k.taskBeg = dTaskBeg;
+        k.taskCnt = dTaskCnt;
         if (ct == 0) k.centersU8 = voc->u8rows.get() + nodeBase * b * kDims;
         else k.centersF32 = voc->f32rows.get() + nodeBase * b * kDims;
         k.assign = dAssign.get();
@@ -238,24 +266,61 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         k.sums = dSums.get();
         k.changed = dChanged.get();
         k.active = dActive.get();
-        k.passes = dPasses.get();
+        uint32_t* dHist = dOut.get();
+        k.passes = dHist + uint64_t(nG) * b;
+        uint32_t* dMasks = k.passes + nG;
         k.nbActive = dNbActive.get();
+
+        // Tasks of <= persistTiles tiles run their whole Lloyd loop in one launch with
+        // their points in LDS (k_lloydTasks), when there are enough of them to fill
+        // the device; the others stream, one launch per pass: equal tile ranges per
+        // resident wave, cut into per-task chunks; reduce items of <= 32 chunks.
+        smallTasks.clear();
+        uint32_t smallTilesMax = 0, nbLarger = 0;  // smallTasks[0, nbLarger): more than persistTiles2 tiles
+        const uint32_t* dSmall = nullptr;
+        const uint32_t* dTileBeg = nullptr;
         if (mfmaPath) {
+            if (persistTiles) {
+                for (uint32_t g = 0; g < nG; g++)
+                    if (taskTiles[g] && taskTiles[g] <= persistTiles) smallTasks.push_back(g);
+                if (smallTasks.size() < cus) smallTasks.clear();
+                // large tasks first in the list: they start early and the short ones fill in
+                std::stable_sort(smallTasks.begin(), smallTasks.end(),
+                                 [&](uint32_t x, uint32_t y) { return taskTiles[x] > taskTiles[y]; });
+                for (uint32_t g : smallTasks) {
+                    smallTilesMax = std::max(smallTilesMax, taskTiles[g]);
+                    nbLarger += taskTiles[g] > persistTiles2;
+                    taskTiles[g] = 0;  // not streamed
+                }
+                dSmall = up.put(smallTasks.data(), smallTasks.size());
+                dTileBeg = up.put(tileBeg.data(), nG);
+            }
+            const KmPartition part = partitionKmTiles(taskTiles, tileBeg);
+            red.clear();
+            for (uint32_t g = 0; g < nG; g++)
+                if (part.taskChunkCnt[g] > 32)  // small tasks are reduced inside the update kernel
+                    for (uint32_t c = 0; c < part.taskChunkCnt[g]; c += 32)
+                        red.push_back(KmChunk{g, part.taskChunkBeg[g] + c,
+                                              part.taskChunkBeg[g] + std::min(part.taskChunkCnt[g], c + 32)});
+            RBOW_REQUIRE(part.chunks.size() <= chunksMax && red.size() <= redMax, RBOW_ERR_HIP,
+                         "k-means work partition exceeds its bound");
             k.XA = dXA.get();
-            k.chunks = dChunks.get();
-            k.nbChunks = uint32_t(chunks.size());
-            k.waveChunkBeg = dWaveChunkBeg.get();
-            k.nbWaves = nbWaves;
-            k.taskChunkBeg = dTaskChunkBeg.get();
-            k.taskChunkCnt = dTaskChunkCnt.get();
+            k.waveChunkBeg = up.put(part.waveChunkBeg.data(), part.waveChunkBeg.size());
+            k.nbWaves = part.nbWaves;
+            k.chunks = up.put(part.chunks.data(), part.chunks.size());
+            k.nbChunks = uint32_t(part.chunks.size());
+            k.taskChunkBeg = up.put(part.taskChunkBeg.data(), nG);
+            k.taskChunkCnt = up.put(part.taskChunkCnt.data(), nG);
             k.partSums = dPartSums.get();
             k.partCnt = dPartCnt.get();
             k.partChanged = dPartChanged.get();
         }
+        const KmChunk* dRed = up.put(red.data(), red.size());
 
         // ---- seeds (kmeans.cpp:170-196 reference picks, or k-means++) ----
+        const uint32_t* dSeedIdx = dSeed.get();
         if (opts.seed_mode == 0) {
-            std::vector<uint32_t> seedIdx(uint64_t(nG) * b);
+            uint32_t* seedIdx = up.take<uint32_t>(uint64_t(nG) * b, &dSeedIdx);
             uint32_t pos[32];
             for (uint32_t g = 0; g < nG; g++) {
                 const uint32_t n = taskCnt[g];
@@ -277,73 +342,98 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
                 }
                 voc->masks[nodeBase + g] = nValid >= 32 ? 0xFFFFFFFFu : ((1u << nValid) - 1u);
             }
-            h2d(dSeed.get(), seedIdx.data(), seedIdx.size(), s);
-        } else {
-            launchKmeansPP(devDesc, dOrder.get(), dTaskBeg.get(), dTaskCnt.get(), nG, b, opts.seed, level,
-                           dSeed.get(), dMasks.get(), s);
-            hMasks.resize(nG);
-            d2h(hMasks.data(), dMasks.get(), nG, s);
         }
-        launchInitCenters(k, devDesc, dSeed.get(), s);
+        up.upload(s);
+        if (level == 0) launchIota(dOrder.get(), nbDesc, s);
+        if (mfmaPath) launchGatherTiles(devDesc, dOrder.get(), dTiles, nbTiles, dXA.get(), s);
+        else launchGatherRows(devDesc, dOrder.get(), nbDesc, dX.get(), s);
+        if (opts.seed_mode == 1)
+            launchKmeansPP(devDesc, dOrder.get(), dTaskBeg, dTaskCnt, nG, b, opts.seed, level, dSeed.get(), dMasks, s);
+        launchInitCenters(k, devDesc, dSeedIdx, s);
         launchKmLevelReset(k, nbDesc, s);
 
         // ---- Lloyd passes (kmeans.cpp:118-133) ----
-        const bool poll = opts.max_iters > 16;
+        if (!smallTasks.empty()) {
+            if (stats) mark(0);
+            // two LDS classes: tasks that allow two workgroups per CU run in a launch of their own
+            launchLloydTasks(k, dSmall, nbLarger, smallTilesMax, dTileBeg, opts.max_iters, s);
+            launchLloydTasks(k, dSmall + nbLarger, uint32_t(smallTasks.size()) - nbLarger,
+                             std::min(smallTilesMax, persistTiles2), dTileBeg, opts.max_iters, s);
+            if (stats) mark(1);
+        }
+        // with no streamed task left, one update pass retires the empty tasks
+        const bool streamed = !mfmaPath || k.nbWaves > 0;
+        const uint32_t iters = streamed ? opts.max_iters : 1u;
+        const bool poll = iters > 16;
         uint32_t nbLaunched = 0;
-        for (uint32_t it = 0; it < opts.max_iters; it++) {
-            if (stats) mark(5 * it);
+        for (uint32_t it = 0; it < iters; it++) {
+            const size_t e = 2 + 5 * size_t(it);
+            if (stats) mark(e);
             launchKmAssign(k, kernel, s);
-            if (stats) mark(5 * it + 1);
-            if (mfmaPath) launchKmReduce(k, dRed.get(), uint32_t(red.size()), s);
-            if (stats) mark(5 * it + 2);
+            if (stats) mark(e + 1);
+            if (mfmaPath) launchKmReduce(k, dRed, uint32_t(red.size()), s);
+            if (stats) mark(e + 2);
             if (poll) launchFillU32(dNbActive.get(), 0u, 1, s);
-            if (stats) mark(5 * it + 3);
+            if (stats) mark(e + 3);
             launchKmUpdate(k, s);
-            if (stats) mark(5 * it + 4);
+            if (stats) mark(e + 4);
             nbLaunched = it + 1;
             if (poll && (it % 4 == 3)) {  // kmeans.cpp:125-131 polls every 4 passes
-                d2h(hNbActive, dNbActive.get(), 1, s);
+                RBOW_HIP(hipMemcpyAsync(hNbActive, dNbActive.get(), 4, hipMemcpyDeviceToHost, s));
                 RBOW_HIP(hipStreamSynchronize(s));
                 if (*hNbActive == 0) break;
             }
         }
-        hPasses.resize(nG);
-        d2h(hPasses.data(), dPasses.get(), nG, s);
         // ---- stable regroup + occurrence on the device (Vocabulary.cpp:83-98) ----
-        std::vector<uint32_t> childBeg(uint64_t(nG) * b), childCnt(uint64_t(nG) * b, 0u);
-        dHist.reserve(uint64_t(nG) * b);
-        launchRegroup(k, nbDesc, dOrder.get(), dOrderNext.get(), dHist.get(), s);
-        d2h(childCnt.data(), dHist.get(), childCnt.size(), s);
+        launchRegroup(k, nbDesc, dOrder.get(), dOrderNext.get(), dHist, dSortWs.get(), dSortWs.size(), s);
         std::swap(dOrder, dOrderNext);
+        const uint64_t outWords = uint64_t(nG) * b + 2 * nG;
+        RBOW_HIP(hipMemcpyAsync(hOut, dOut.get(), 4 * outWords, hipMemcpyDeviceToHost, s));
         if (opts.seed_mode == 0 && level + 1 < L) d2h(order.data(), dOrder.get(), nbDesc, s);
         RBOW_HIP(hipStreamSynchronize(s));
+        const uint32_t* childCnt = hOut;
+        const uint32_t* hPasses = hOut + uint64_t(nG) * b;
+        const uint32_t* hMasks = hPasses + nG;
         if (opts.seed_mode == 1)
             for (uint32_t g = 0; g < nG; g++) voc->masks[nodeBase + g] = hMasks[g];
+        std::vector<uint8_t> isSmall(nG, 0);
+        for (uint32_t g : smallTasks) isSmall[g] = 1;
         for (uint32_t g = 0; g < nG; g++) {
             passesAll[nodeBase + g] = hPasses[g];
             pointPasses += uint64_t(taskCnt[g]) * hPasses[g];
+            if (!isSmall[g]) streamPointPasses += uint64_t(taskCnt[g]) * hPasses[g];
         }
-        if (stats)
+        if (stats) {
+            float pm = 0;
+            if (!smallTasks.empty()) RBOW_HIP(hipEventElapsedTime(&pm, ev[0], ev[1]));
             for (uint32_t it = 0; it < nbLaunched; it++) {
+                const size_t e = 2 + 5 * size_t(it);
                 float a = 0, r = 0, u = 0;
-                RBOW_HIP(hipEventElapsedTime(&a, ev[5 * it], ev[5 * it + 1]));
-                RBOW_HIP(hipEventElapsedTime(&r, ev[5 * it + 1], ev[5 * it + 2]));
-                RBOW_HIP(hipEventElapsedTime(&u, ev[5 * it + 3], ev[5 * it + 4]));
+                RBOW_HIP(hipEventElapsedTime(&a, ev[e], ev[e + 1]));
+                RBOW_HIP(hipEventElapsedTime(&r, ev[e + 1], ev[e + 2]));
+                RBOW_HIP(hipEventElapsedTime(&u, ev[e + 3], ev[e + 4]));
                 assignMs += a;
                 reduceMs += r;
                 updateMs += u;
-                assignLaunches++;
+                if (streamed) {
+                    streamMs += a;
+                    streamLaunches++;
+                }
             }
+            assignMs += pm;
+            persistTotalMs += pm;
+        }
 
+        std::vector<uint32_t> childBeg(uint64_t(nG) * b);
         uint32_t run = 0;
-        for (uint64_t c = 0; c < childCnt.size(); c++) {
+        for (uint64_t c = 0; c < childBeg.size(); c++) {
             childBeg[c] = run;
             run += childCnt[c];
         }
         RBOW_REQUIRE(run == nbDesc, RBOW_ERR_HIP, "regroup lost rows");
-        for (uint64_t c = 0; c < childCnt.size(); c++) occ[nodeBase * b + c] = childCnt[c];
+        for (uint64_t c = 0; c < childBeg.size(); c++) occ[nodeBase * b + c] = childCnt[c];
         taskBeg.swap(childBeg);
-        taskCnt.swap(childCnt);
+        taskCnt.assign(childCnt, childCnt + uint64_t(nG) * b);
     }
 
     // ---- IDF weights (Vocabulary.cpp:103-108): log of the binary32 quotient
@@ -359,7 +449,10 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         stats->assignMs = assignMs;
         stats->updateMs = updateMs;
         stats->reduceMs = reduceMs;
-        stats->assignLaunches = assignLaunches;
+        stats->streamMs = streamMs;
+        stats->streamLaunches = streamLaunches;
+        stats->streamPointPasses = streamPointPasses;
+        stats->persistMs = persistTotalMs;
         stats->pointPasses = pointPasses;
         stats->totalMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }

@@ -29,10 +29,13 @@ struct Vocabulary {
 struct BuildStats {
     std::vector<uint32_t> passes;  // Lloyd passes per node
     double totalMs = 0;            // wall time of the build (host clock)
-    double assignMs = 0;           // sum of assign-kernel times (HIP events)
+    double assignMs = 0;           // sum of assign-kernel times (HIP events), persistent launches included
     double updateMs = 0;           // sum of update-kernel times (HIP events)
     double reduceMs = 0;           // sum of chunk-reduction times (HIP events)
-    uint32_t assignLaunches = 0;
+    double streamMs = 0;           // per-pass MFMA assign launches only
+    uint32_t streamLaunches = 0;
+    uint64_t streamPointPasses = 0;
+    double persistMs = 0;          // persistent small-task launches (whole Lloyd loops)
     uint64_t pointPasses = 0;      // sum over nodes of points x Lloyd passes
 };
 

@@ -1,0 +1,112 @@
+// Host-only check of the MFMA assign's work partition (kmpartition.hpp), built
+// with g++ by test_partition.py.  Random levels with tasks of every size,
+// including tasks whose tiles are not streamed (0 streamed tiles between
+// streamed ones), against the invariants the assign and update kernels rely on:
+// every streamed tile of every task is covered exactly once, chunk tile indices
+// are global and stay inside their task's tiles, a task's chunks are
+// contiguous and listed by taskChunkBeg/taskChunkCnt, and each wave gets an
+// equal share of the streamed tiles (to within one).
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+#include "kmpartition.hpp"
+
+using namespace rbow_amd;
+
+static int fails = 0;
+#define CHECK(c, ...)                                      \
+    do {                                                   \
+        if (!(c)) {                                        \
+            if (fails++ < 20) {                            \
+                std::fprintf(stderr, "FAIL %s: ", #c);     \
+                std::fprintf(stderr, __VA_ARGS__);         \
+                std::fprintf(stderr, "\n");                \
+            }                                              \
+        }                                                  \
+    } while (0)
+
+static void checkCase(const std::vector<uint32_t>& realTiles, const std::vector<uint8_t>& streamed, uint64_t maxWaves,
+                      bool passBase, int id) {
+    const uint32_t n = uint32_t(realTiles.size());
+    std::vector<uint32_t> taskTiles(n), base(n);
+    uint32_t g0 = 0;
+    uint64_t T = 0;
+    for (uint32_t g = 0; g < n; g++) {
+        base[g] = g0;
+        g0 += realTiles[g];
+        taskTiles[g] = streamed[g] ? realTiles[g] : 0u;
+        T += taskTiles[g];
+    }
+    if (!passBase)  // prefix-sum numbering: only meaningful when every task streams
+        for (uint32_t g = 0; g < n; g++) CHECK(streamed[g], "case %d: test setup", id);
+    const KmPartition p = partitionTiles(taskTiles, passBase ? base : std::vector<uint32_t>{}, maxWaves);
+    const uint64_t W = std::min<uint64_t>(std::max<uint64_t>(maxWaves, 1), T);
+    CHECK(p.nbWaves == W, "case %d: nbWaves %u expected %llu", id, p.nbWaves, (unsigned long long)W);
+    CHECK(p.waveChunkBeg.size() == size_t(p.nbWaves) + 1, "case %d: waveChunkBeg size", id);
+    CHECK(p.waveChunkBeg.front() == 0 && p.waveChunkBeg.back() == p.chunks.size(), "case %d: waveChunkBeg ends", id);
+    for (uint32_t w = 0; w < p.nbWaves; w++) {
+        CHECK(p.waveChunkBeg[w] <= p.waveChunkBeg[w + 1], "case %d: wave %u order", id, w);
+        uint64_t tilesOfWave = 0;
+        for (uint32_t c = p.waveChunkBeg[w]; c < p.waveChunkBeg[w + 1]; c++)
+            tilesOfWave += p.chunks[c].tileEnd - p.chunks[c].tileBeg;
+        const uint64_t want = T * (w + 1) / W - T * w / W;
+        CHECK(tilesOfWave == want, "case %d: wave %u has %llu tiles, expected %llu", id, w,
+              (unsigned long long)tilesOfWave, (unsigned long long)want);
+    }
+    std::vector<uint32_t> covered(g0, 0u);
+    for (size_t c = 0; c < p.chunks.size(); c++) {
+        const KmChunk& k = p.chunks[c];
+        CHECK(k.task < n, "case %d: chunk %zu task", id, c);
+        if (k.task >= n) continue;
+        CHECK(streamed[k.task] && taskTiles[k.task] > 0, "case %d: chunk %zu of non-streamed task %u", id, c, k.task);
+        CHECK(k.tileBeg < k.tileEnd, "case %d: chunk %zu empty", id, c);
+        CHECK(k.tileBeg >= base[k.task] && k.tileEnd <= base[k.task] + realTiles[k.task],
+              "case %d: chunk %zu [%u,%u) outside task %u tiles [%u,%u)", id, c, k.tileBeg, k.tileEnd, k.task,
+              base[k.task], base[k.task] + realTiles[k.task]);
+        for (uint32_t t = k.tileBeg; t < k.tileEnd && t < g0; t++) covered[t]++;
+        if (c) CHECK(p.chunks[c - 1].tileEnd <= k.tileBeg, "case %d: chunks %zu not increasing", id, c);
+    }
+    for (uint32_t g = 0; g < n; g++) {
+        for (uint32_t t = base[g]; t < base[g] + realTiles[g]; t++)
+            CHECK(covered[t] == (streamed[g] ? 1u : 0u), "case %d: tile %u of task %u covered %u times", id, t, g,
+                  covered[t]);
+        CHECK((p.taskChunkCnt[g] > 0) == (taskTiles[g] > 0), "case %d: task %u chunk count %u", id, g,
+              p.taskChunkCnt[g]);
+        for (uint32_t c = p.taskChunkBeg[g]; c < p.taskChunkBeg[g] + p.taskChunkCnt[g]; c++)
+            CHECK(c < p.chunks.size() && p.chunks[c].task == g, "case %d: task %u chunk list", id, g);
+        if (!p.taskChunkCnt[g]) CHECK(p.taskChunkBeg[g] == 0, "case %d: task %u chunk begin", id, g);
+    }
+}
+
+int main() {
+    std::mt19937_64 rng(2024);
+    int cases = 0;
+    // hand-picked shapes: leading, inner and trailing non-streamed tasks, empty tasks
+    const std::vector<std::vector<uint32_t>> shapes = {{5}, {1, 1, 1}, {0, 3, 0, 2, 0}, {40, 2, 2, 40}, {7, 0, 0, 9}};
+    const std::vector<std::vector<uint8_t>> masks = {{1}, {1, 0, 1}, {1, 1, 1, 1, 1}, {1, 0, 0, 1}, {0, 1, 1, 1}};
+    for (size_t i = 0; i < shapes.size(); i++)
+        for (uint64_t mw : {1ull, 2ull, 3ull, 64ull, 4096ull}) {
+            checkCase(shapes[i], masks[i], mw, true, cases++);
+            std::vector<uint8_t> all(shapes[i].size(), 1);
+            checkCase(shapes[i], all, mw, false, cases++);
+        }
+    // random levels: sizes from a heavy tail, about half the tasks not streamed
+    for (int r = 0; r < 400; r++) {
+        const uint32_t n = 1 + uint32_t(rng() % (r < 200 ? 40 : 4096));
+        std::vector<uint32_t> tiles(n);
+        std::vector<uint8_t> st(n);
+        for (uint32_t g = 0; g < n; g++) {
+            const uint32_t pick = uint32_t(rng() % 10);
+            tiles[g] = pick == 0 ? 0u : pick < 6 ? 1u + uint32_t(rng() % 32) : 1u + uint32_t(rng() % 3000);
+            st[g] = (tiles[g] > 32 || rng() % 2) ? 1 : 0;
+        }
+        const uint64_t mw = std::vector<uint64_t>{1, 5, 256, 1024, 8192}[rng() % 5];
+        checkCase(tiles, st, mw, true, cases++);
+        std::vector<uint8_t> all(n, 1);
+        checkCase(tiles, all, mw, rng() % 2 == 0, cases++);
+    }
+    std::printf("%d cases, %d failures\n", cases, fails);
+    return fails ? EXIT_FAILURE : EXIT_SUCCESS;
+}

@@ -84,3 +84,25 @@ def test_build_errors():
     with pytest.raises(rb.RbowError) as e:
         rb.build_vocabulary(d, 15, 1, 2)
     assert str(e.value) == "Insufficient descriptors"
+
+
+@pytest.mark.parametrize("iters", [3, 40])
+def test_build_u8_persistent_and_streamed_tasks(iters):
+    # 400k points, L=4: most of the 4,096 tasks of level 3 are small enough for the
+    # on-chip persistent loop, in both LDS classes (<= 8 and 9..16 tiles of 64
+    # points), while the larger ones and levels 0-2 stream; the pass cap stops
+    # some tasks early (3) or lets most converge (40)
+    desc = rb.synth_descriptors(400000, seed=7, nb_centres=30)
+    L, nb_doc = 4, 500
+    res = oracle.build(desc, nb_doc, L, max_iters=iters, seed_mode=1)
+    tiles = (res["occ"][level_beg(16, 2):level_beg(16, 3)].ravel().astype(np.int64) + 63) // 64  # level-3 tasks
+    assert ((tiles >= 1) & (tiles <= 8)).sum() >= 256 and ((tiles > 8) & (tiles <= 16)).any()
+    assert iters != 3 or (tiles > 16).any()  # with the cap at 3, level 3 also streams some tasks
+    voc = _gpu_build(desc, nb_doc, L, max_iters=iters, seed_mode=1, kernel=0)
+    _check_u8(voc, res, L)
+    st = voc.build_stats()
+    assert st["persist_ms"] > 0 and st["assign_launches"] > 0  # both paths ran
+    # the same tree with every task streamed, one launch per pass
+    voc0 = _gpu_build(desc, nb_doc, L, max_iters=iters, seed_mode=1, kernel=0, persist_max_tiles=0)
+    assert voc0.build_stats()["persist_ms"] == 0
+    assert voc0.serialize() == voc.serialize()

@@ -1,0 +1,25 @@
+"""Work partition of the MFMA k-means assign (rapidbow_amd/csrc/kmpartition.hpp).
+
+The partition is plain host C++; partition_check.cpp is compiled here with g++
+and checks it on hand-picked and random levels (tasks of every size, tasks
+whose Lloyd loop is not streamed between streamed ones, any resident wave
+count) against the invariants the assign and update kernels index by.
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "..", "rapidbow_amd", "csrc")
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_partition_invariants(tmp_path):
+    exe = tmp_path / "partition_check"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", CSRC,
+                    os.path.join(HERE, "partition_check.cpp"), "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert "0 failures" in r.stdout
