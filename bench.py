@@ -356,6 +356,14 @@ def main():
     assign_gbs = bytes_per_launch / (launch_us * 1e-6) / 1e9
     copy_gbs = copy_rate(rb)
     log(f"bench: assign kernel {launch_us:.1f} us/launch, {assign_gbs:.0f} GB/s algorithmic; copy {copy_gbs:.0f} GB/s")
+    # the same MFMA build with every node streamed (no persistent on-chip Lloyd loops)
+    vstream = rb.build_vocabulary(d, N, 1000, 4, max_iters=10, seed_mode=1, kernel=0, persist_max_tiles=0)
+    sts = vstream.build_stats()
+    stream_tops = sts["assign_ops"] / ((sts["assign_ms"] + sts["reduce_ms"]) * 1e-3) / 1e12
+    same_stream = vstream.serialize() == voc.serialize()
+    del vstream
+    log(f"bench: stream-only build {sts['total_ms']:.1f} ms, assign {sts['assign_ms']:.2f} ms, "
+        f"{stream_tops:.1f} TOPS, identical={same_stream}")
     log("bench: plain-kernel build")
     vplain = rb.build_vocabulary(d, N, 1000, 4, max_iters=10, seed_mode=1, kernel=1)
     stp = vplain.build_stats()
@@ -414,6 +422,9 @@ def main():
                            "reduce": round(st["reduce_ms"], 3), "update": round(st["update_ms"], 3),
                            "streamed_launches": st["assign_launches"]},
         "assign_tops_plain": round(plain_tops, 3),
+        "assign_stream_only": {"tops": round(stream_tops, 3), "assign_ms": round(sts["assign_ms"], 3),
+                               "reduce_ms": round(sts["reduce_ms"], 3), "build_ms": round(sts["total_ms"], 2),
+                               "identical_tree": same_stream},
         "build_ms": {"mfma": round(float(np.median([s["total_ms"] for s in stats])), 2),
                      "plain": round(stp["total_ms"], 2), "identical_trees": same_tree,
                      "b32L4_f32": round(st32["total_ms"], 1)},

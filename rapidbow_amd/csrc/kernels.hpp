@@ -96,6 +96,8 @@ struct KmLevel {       // device state of one level's batched k-means
     int32_t* partSums = nullptr;     // [chunk][16][128]  sums of (x - 128)
     uint32_t* partCnt = nullptr;     // [chunk][16]
     uint32_t* partChanged = nullptr; // [chunk]
+    const uint32_t* updTasks = nullptr;  // tasks the MFMA update visits (null: all); the others run elsewhere
+    uint32_t nbUpdTasks = 0;
 };
 
 // centers[task][j] = desc[seedIdx[task*b+j]] (invalid -> 0xFF / 255.0f)
@@ -114,10 +116,10 @@ void launchKmReduce(const KmLevel& k, const KmChunk* red, uint32_t nbRed, hipStr
 // Whole Lloyd loops (assign + update, up to maxIters passes, per-task stop) of
 // the listed uint8 tasks, one workgroup each with the task's points in LDS;
 // leaves them inactive with their pass counts.  Every listed task has 1 to
-// maxTiles tiles (maxTiles <= lloydTasksTileCap(1)); taskTileBeg[g] = first
-// tile of task g in k.XA.
+// maxTiles tiles (maxTiles <= lloydTasksTileCap(blocksPerCu), blocksPerCu = 1
+// or 2 workgroups sharing a CU); taskTileBeg[g] = first tile of task g in k.XA.
 void launchLloydTasks(const KmLevel& k, const uint32_t* taskList, uint32_t nbList, uint32_t maxTiles,
-                      const uint32_t* taskTileBeg, uint32_t maxIters, hipStream_t s);
+                      uint32_t blocksPerCu, const uint32_t* taskTileBeg, uint32_t maxIters, hipStream_t s);
 // Largest task (in 64-row tiles) whose points fit one workgroup's LDS when
 // blocksPerCu workgroups of launchLloydTasks share a CU.
 uint32_t lloydTasksTileCap(uint32_t blocksPerCu);

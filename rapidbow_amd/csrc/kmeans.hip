@@ -322,15 +322,15 @@ __device__ inline v2i readTr8(const uint8_t* p) {
     typedef __attribute__((address_space(3))) v2i LdsV2i;  // p points into LDS
     return __builtin_amdgcn_ds_read_tr8_b64_v2i32((LdsV2i*)(const_cast<uint8_t*>(p)));
 }
-// XT fragment db of an LDS tile.  Lane t = lane & 15 of group grp = lane >> 4
-// addresses, for output bytes q = r (first read) and q = 8 + r (second) with
-// r = t >> 1, dims 16*db + 8*(t & 1) .. +8 of point 16*(q>>2) + 4*grp + (q&3);
-// the transposing read then hands lane i of the group dim 16*db + i of those
-// 8 points.
+// XT fragment db of an LDS tile: lane l of group grp = l >> 4 holds dim
+// 16*db + (l & 15) of points 16*grp + q, q = 0..15 (byte q).  Lane t = l & 15
+// addresses dims 16*db + 8*(t & 1) .. +8 of point 16*grp + (t >> 1) (first
+// read, bytes 0..7) and of the point 8 rows further (second read, bytes 8..15);
+// the transposing read hands lane i of the group dim 16*db + i of those points.
 __device__ inline v4i loadXt(const uint8_t* t, uint32_t lane, int db) {
-    const uint32_t tt = lane & 15, r = tt >> 1, p0 = 16 * (r >> 2) + 4 * (lane >> 4) + (r & 3);
-    const uint8_t* base = t + p0 * kRowPitch + 16 * db + 8 * (tt & 1);
-    const v2i lo = readTr8(base), hi = readTr8(base + 32 * kRowPitch);  // q = 8 + r: point p0 + 32
+    const uint32_t tt = lane & 15;
+    const uint8_t* base = t + (16 * (lane >> 4) + (tt >> 1)) * kRowPitch + 16 * db + 8 * (tt & 1);
+    const v2i lo = readTr8(base), hi = readTr8(base + 8 * kRowPitch);
     return v4i{lo.x, lo.y, hi.x, hi.y};
 }
 
@@ -350,61 +350,82 @@ __device__ inline int centerOperands(const uint8_t* crow, uint32_t lane, v4i& b0
     return ns;
 }
 
-// One tile of cnt (<= 64) rows: distances by I8 MFMA, argmin (ties to the
-// lowest center), the new assignments compared with `prev` (lane r: previous
-// assignment of row r), and the one-hot sums of x' added into acc (XT from
-// the same tile in LDS, `tile`).  Returns the new assignment of row `lane`
-// (meaningful for lane < cnt).
+// Argmin keys ((||c'||^2 - 2 x'.c' + 2^25) << 4) | center: unique per center,
+// the smallest is the nearest center with ties to the lowest index, and
+// |2 x'.c'| < 2^23, ||c'||^2 <= 2^21 keep them in [0, 2^30).  base[i] is the
+// part without x' of center 4*(lane >> 4) + i; a key is base - (x'.c' << 5).
+struct CenterKeys {
+    v4i b0, b1;         // A operand of the distance MFMA: lane l holds center l & 15
+    uint32_t base[4];
+};
+__device__ inline void centerKeys(const uint8_t* cent, uint32_t lane, CenterKeys& ck) {
+    const int ns = centerOperands(cent + (lane & 15) * kDims, lane, ck.b0, ck.b1);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint32_t c = 4 * (lane >> 4) + i;
+        ck.base[i] = (uint32_t(__shfl(ns, int(c), 64) + (1 << 25)) << 4) | c;
+    }
+}
+
+// Minimum over lanes l, l ^ 16, l ^ 32, l ^ 48 (the four 16-lane rows), in every lane.
+__device__ inline uint32_t minAcrossRows(uint32_t m) {
+    const auto p = __builtin_amdgcn_permlane16_swap(m, m, false, false);  // rows 0<->1, 2<->3
+    m = min(uint32_t(p[0]), uint32_t(p[1]));
+    const auto q = __builtin_amdgcn_permlane32_swap(m, m, false, false);  // halves
+    return min(uint32_t(q[0]), uint32_t(q[1]));
+}
+
+// 0x01 in each byte of x that is zero, else 0x00 (exact: no carries cross bytes).
+__device__ inline uint32_t zeroBytes(uint32_t x) {
+    const uint32_t t = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
+    return (~(t | x) & 0x80808080u) >> 7;
+}
+
+// One tile of cnt (<= 64) rows: distances by I8 MFMA with the centers as the
+// A operand (lane l then holds centers 4*(l>>4) + i of point 16*rb + (l&15)),
+// argmin (ties to the lowest center), the new assignments compared with `prev`
+// (lane r: previous assignment of row r), and the one-hot sums of x' added
+// into acc with XT from the same tile in LDS (`tile`).  sAsgWave: the wave's
+// 64 bytes of LDS.  Returns the new assignment of row `lane` (meaningful for
+// lane < cnt).
 __device__ __forceinline__ uint32_t assignTileMfma(const v4i (&a)[4][2], const uint8_t* tile, uint32_t cnt,
-                                                   uint32_t prev, const v4i& b0, const v4i& b1, int ns,
-                                                   uint32_t* sNewWave, uint32_t lane, v4i (&acc)[8], uint32_t& myCnt,
+                                                   uint32_t prev, const CenterKeys& ck, uint8_t* sAsgWave,
+                                                   uint32_t lane, v4i (&acc)[8], uint32_t& myCnt,
                                                    uint32_t& myChanged) {
-    const uint32_t col = lane & 15, grp = lane >> 4;
+    const uint32_t grp = lane >> 4;
     const v4i zero = {0, 0, 0, 0};
-    uint32_t key[4][4];
+    uint32_t best[4];
 #pragma unroll
     for (int rb = 0; rb < 4; rb++) {
-        v4i d = zero;
-        d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rb][0], b0, d, 0, 0, 0);
-        d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rb][1], b1, d, 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; i++) key[rb][i] = (uint32_t(ns - 2 * d[i] + (1 << 25)) << 4) | col;
+        v4i d = __builtin_amdgcn_mfma_i32_16x16x64_i8(ck.b0, a[rb][0], zero, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_i32_16x16x64_i8(ck.b1, a[rb][1], d, 0, 0, 0);
+        const uint32_t m01 = min(ck.base[0] - (uint32_t(d[0]) << 5), ck.base[1] - (uint32_t(d[1]) << 5));
+        const uint32_t m23 = min(ck.base[2] - (uint32_t(d[2]) << 5), ck.base[3] - (uint32_t(d[3]) << 5));
+        best[rb] = minAcrossRows(min(m01, m23));
     }
-#pragma unroll
-    for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) key[rb][i] = min16u(key[rb][i]);
-    // new assignments to lane-per-row order (rows rb*16 + 4*grp + i live in group grp)
-    if (col == 0) {
-#pragma unroll
-        for (int rb = 0; rb < 4; rb++)
-            sNewWave[rb * 4 + grp] = (key[rb][0] & 15u) | ((key[rb][1] & 15u) << 8) | ((key[rb][2] & 15u) << 16) |
-                                     ((key[rb][3] & 15u) << 24);
-    }
+    // row `lane` is point 16*grp + (lane & 15): the block rb = grp
+    const uint32_t mine = (grp == 0 ? best[0] : grp == 1 ? best[1] : grp == 2 ? best[2] : best[3]) & 15u;
+    const bool valid = lane < cnt;
+    if (valid) myChanged += prev != mine;
+    waveSync();  // the previous tile's reads of sAsgWave are done
+    sAsgWave[lane] = valid ? uint8_t(mine) : uint8_t(0xFF);  // padding rows match no center
     waveSync();
-    const uint32_t mine = (sNewWave[lane >> 2] >> (8 * (lane & 3))) & 15u;
-    waveSync();
-    if (lane < cnt) myChanged += prev != mine;
+    // one-hot operand: lane l, byte q = [point 16*grp + q belongs to center l & 15]
+    const v4i w = *reinterpret_cast<const v4i*>(sAsgWave + 16 * grp);
+    const uint32_t rep = (lane & 15) * 0x01010101u;
     v4i oh;
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++) {
-        uint32_t w = 0;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const uint32_t row = rb * 16 + 4 * grp + i;
-            w |= uint32_t(row < cnt && (key[rb][i] & 15u) == col) << (8 * i);
-        }
-        oh[rb] = int(w);
+    for (int q = 0; q < 4; q++) {
+        oh[q] = int(zeroBytes(uint32_t(w[q]) ^ rep));
+        myCnt = __builtin_amdgcn_udot4(uint32_t(oh[q]), 0x01010101u, myCnt, false);
     }
-#pragma unroll
-    for (int rb = 0; rb < 4; rb++) myCnt = __builtin_amdgcn_udot4(uint32_t(oh[rb]), 0x01010101u, myCnt, false);
 #pragma unroll
     for (int db = 0; db < 8; db++) acc[db] = __builtin_amdgcn_mfma_i32_16x16x64_i8(oh, loadXt(tile, lane, db), acc[db], 0, 0, 0);
     return mine;
 }
 
 __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
-    __shared__ uint32_t sNew[kWaves][16];  // a tile's new assignments, one byte per row
+    __shared__ __align__(16) uint8_t sAsg[kWaves][64];          // a tile's new assignments, one byte per row
     __shared__ __align__(16) uint8_t sTile[kWaves][kTileLds];  // the tile, row-major, for the XT reads
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 15, grp = lane >> 4;
@@ -434,8 +455,8 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
     while (ch < chEnd) {
         const KmChunk c = k.chunks[ch];
         const uint32_t chNext = nextActive(ch + 1);
-        v4i b0, b1;
-        const int ns = centerOperands(k.centersU8 + (uint64_t(c.task) * 16 + col) * kDims, lane, b0, b1);
+        CenterKeys ck;
+        centerKeys(k.centersU8 + uint64_t(c.task) * 16 * kDims, lane, ck);
         v4i acc[8];
 #pragma unroll
         for (int db = 0; db < 8; db++) acc[db] = zero;
@@ -456,7 +477,7 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
             storeTileLds(sTile[wave], lane, a);
             waveSync();
             const uint32_t mine =
-                assignTileMfma(a, sTile[wave], t.cnt, prev, b0, b1, ns, sNew[wave], lane, acc, myCnt, myChanged);
+                assignTileMfma(a, sTile[wave], t.cnt, prev, ck, sAsg[wave], lane, acc, myCnt, myChanged);
             if (lane < t.cnt) k.assign[t.row + lane] = uint8_t(mine);
             if (tiN != kIdxInvalid) {
 #pragma unroll
@@ -483,63 +504,67 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
     }
 }
 
-// Persistent Lloyd loop with the points on chip, one workgroup per listed
-// task (tasks of at most maxTiles 64-row tiles).  The task's tiles are read
-// from HBM once into LDS, row-major (dynamic: maxTiles x 9 KB, then one
+// Persistent Lloyd loop with the points on chip, one workgroup of W waves per
+// listed task (tasks of at most maxTiles 64-row tiles).  The task's tiles are
+// read from HBM once into LDS, row-major (dynamic: maxTiles x 9 KB, then one
 // assignment byte per row); every pass then takes both MFMA operands from
-// LDS, so the loop is not bound by HBM.  Per pass the 4 waves take the tiles round
-// robin (the next tile's fragments in flight while one computes), their
+// LDS, so the loop is not bound by HBM.  Per pass the waves take the tiles
+// round robin (the next tile's fragments in flight while one computes), their
 // one-hot sums and counts meet in LDS (int32 atomics), and the same rule as
 // k_updateU8Part applies: stop when no assignment changed, else move the
-// centers to round(float(sum)/float(count)).  Results equal those of the
-// one-launch-per-pass path for any pass cap.  At the end the assignments and
-// centers go to HBM and the task is marked inactive with its pass count.
-__global__ __launch_bounds__(256) void k_lloydTasks(KmLevel k, const uint32_t* __restrict__ taskList,
-                                                   const uint32_t* __restrict__ taskTileBeg, uint32_t maxTiles,
-                                                   uint32_t maxIters) {
+// centers to round(float(sum)/float(count)).  Two barriers per pass: each
+// thread clears the sums it has just read, and the counters alternate between
+// two buffers.  Results equal those of the one-launch-per-pass path for any
+// pass cap.  At the end the assignments and centers go to HBM and the task is
+// marked inactive with its pass count.
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_lloydTasks(KmLevel k, const uint32_t* __restrict__ taskList,
+                                                      const uint32_t* __restrict__ taskTileBeg, uint32_t maxTiles,
+                                                      uint32_t maxIters) {
+    constexpr uint32_t kThreads = 64 * W;
     extern __shared__ __align__(16) uint8_t dynLds[];
-    __shared__ uint32_t sNew[kWaves][16];
+    __shared__ __align__(16) uint8_t sNew[W][64];
     __shared__ int32_t sSum[16 * kDims];  // sums of x - 128
-    __shared__ uint32_t sCnt[16], sChanged;
+    __shared__ uint32_t sCnt[2][16], sChanged[2];
     __shared__ __align__(16) uint8_t sCent[16 * kDims];
     const uint32_t g = taskList[blockIdx.x], tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t col = lane & 15;
+    const uint32_t col = lane & 15, grp = lane >> 4;
     const uint32_t n = k.taskCnt[g], row0 = k.taskBeg[g], nt = (n + 63) / 64;
     if (nt == 0 || nt > maxTiles) return;  // the host lists tasks of 1..maxTiles tiles only
     uint8_t* sA = dynLds;                                   // [nt] row-major tiles of kTileLds bytes
     uint8_t* sAsg = dynLds + uint64_t(maxTiles) * kTileLds;  // [nt * 64]
     const uint4* src = reinterpret_cast<const uint4*>(k.XA + uint64_t(taskTileBeg[g]) * 8192);
-    for (uint32_t i = tid; i < nt * 512; i += 256) {  // XA fragment (tile, rb, kb, lane) -> its rows in LDS
+    for (uint32_t i = tid; i < nt * 512; i += kThreads) {  // XA fragment (tile, rb, kb, lane) -> its rows in LDS
         const uint32_t c = i & 511;
         *reinterpret_cast<uint4*>(sA + (i >> 9) * kTileLds + fragOffset(c >> 7, (c >> 6) & 1, c & 63)) = src[i];
     }
-    for (uint32_t i = tid; i < nt * 64; i += 256) sAsg[i] = 0xFF;  // no previous assignment
+    for (uint32_t i = tid; i < nt * 64; i += kThreads) sAsg[i] = 0xFF;  // no previous assignment
     uint8_t* cent = k.centersU8 + uint64_t(g) * 16 * kDims;
     if (tid < 128) reinterpret_cast<uint4*>(sCent)[tid] = reinterpret_cast<const uint4*>(cent)[tid];
-    for (uint32_t e = tid; e < 16 * kDims; e += 256) sSum[e] = 0;
-    if (tid < 16) sCnt[tid] = 0;
-    if (tid == 0) sChanged = 0;
+    for (uint32_t e = tid; e < 16 * kDims; e += kThreads) sSum[e] = 0;
+    if (tid < 32) sCnt[tid >> 4][tid & 15] = 0;
+    if (tid < 2) sChanged[tid] = 0;
     const v4i zero = {0, 0, 0, 0};
-    const uint32_t grp = lane >> 4;
     __syncthreads();
     uint32_t pass = 0;
     while (pass < maxIters) {
-        v4i b0, b1;
-        const int ns = centerOperands(sCent + col * kDims, lane, b0, b1);
+        const uint32_t par = pass & 1;
+        CenterKeys ck;
+        centerKeys(sCent, lane, ck);
         v4i acc[8];
 #pragma unroll
         for (int db = 0; db < 8; db++) acc[db] = zero;
         uint32_t myCnt = 0, myChanged = 0;
         v4i a[4][2];
         if (wave < nt) loadTileLds(sA + wave * kTileLds, lane, a);
-        for (uint32_t i = wave; i < nt; i += kWaves) {
+        for (uint32_t i = wave; i < nt; i += W) {
             v4i an[4][2];
-            const bool more = i + kWaves < nt;
-            if (more) loadTileLds(sA + (i + kWaves) * kTileLds, lane, an);
+            const bool more = i + W < nt;
+            if (more) loadTileLds(sA + (i + W) * kTileLds, lane, an);
             const uint32_t cnt = min(64u, n - 64 * i);
             const uint32_t prev = sAsg[i * 64 + lane];
             const uint32_t mine =
-                assignTileMfma(a, sA + i * kTileLds, cnt, prev, b0, b1, ns, sNew[wave], lane, acc, myCnt, myChanged);
+                assignTileMfma(a, sA + i * kTileLds, cnt, prev, ck, sNew[wave], lane, acc, myCnt, myChanged);
             if (lane < cnt) sAsg[i * 64 + lane] = uint8_t(mine);
             if (more) {
 #pragma unroll
@@ -553,28 +578,28 @@ __global__ __launch_bounds__(256) void k_lloydTasks(KmLevel k, const uint32_t* _
 #pragma unroll
         for (int db = 0; db < 8; db++)
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                if (acc[db][i]) atomicAdd(&sSum[(4 * grp + i) * kDims + 16 * db + col], acc[db][i]);
+            for (int i = 0; i < 4; i++) atomicAdd(&sSum[(4 * grp + i) * kDims + 16 * db + col], acc[db][i]);
         myCnt += uint32_t(__shfl_xor(int(myCnt), 16, 64));
         myCnt += uint32_t(__shfl_xor(int(myCnt), 32, 64));
-        if (lane < 16 && myCnt) atomicAdd(&sCnt[lane], myCnt);
+        if (lane < 16) atomicAdd(&sCnt[par][lane], myCnt);
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) myChanged += uint32_t(__shfl_xor(int(myChanged), m, 64));
-        if (lane == 0 && myChanged) atomicAdd(&sChanged, myChanged);
-        __syncthreads();
+        if (lane == 0) atomicAdd(&sChanged[par], myChanged);
+        __syncthreads();  // every partial is in
         pass++;
-        if (sChanged == 0) break;  // converged on this pass: no update (uniform)
-        for (uint32_t e = tid; e < 16 * kDims; e += 256) {
-            const uint32_t c = sCnt[e / kDims];
-            if (c) sCent[e] = uint8_t(roundf(float((long long)sSum[e] + 128ll * c) / float(c)));
+        const bool stop = sChanged[par] == 0;  // converged on this pass: no update (uniform)
+        for (uint32_t e = tid; e < 16 * kDims; e += kThreads) {
+            const int32_t sv = sSum[e];
+            sSum[e] = 0;  // this thread's own elements: cleared for the next pass
+            const uint32_t c = sCnt[par][e / kDims];
+            if (!stop && c) sCent[e] = uint8_t(roundf(float((long long)sv + 128ll * c) / float(c)));
         }
-        __syncthreads();  // every thread has read sCnt and sSum
-        for (uint32_t e = tid; e < 16 * kDims; e += 256) sSum[e] = 0;
-        if (tid < 16) sCnt[tid] = 0;
-        if (tid == 0) sChanged = 0;
-        __syncthreads();
+        if (tid < 16) sCnt[par ^ 1][tid] = 0;  // counters of the next pass
+        if (tid == 0) sChanged[par ^ 1] = 0;
+        __syncthreads();  // new centers and cleared accumulators are visible
+        if (stop) break;
     }
-    for (uint32_t i = tid; i < n; i += 256) k.assign[row0 + i] = sAsg[i];
+    for (uint32_t i = tid; i < n; i += kThreads) k.assign[row0 + i] = sAsg[i];
     if (tid < 128) reinterpret_cast<uint4*>(cent)[tid] = reinterpret_cast<const uint4*>(sCent)[tid];
     if (tid == 0) {
         k.passes[g] = pass;
@@ -595,7 +620,7 @@ __global__ __launch_bounds__(256) void k_reduceChunks(KmLevel k, const KmChunk* 
     }
     long long s = 0;
     uint32_t cnt = 0;
-#pragma unroll 8
+#pragma unroll 16
     for (uint32_t ch = r.tileBeg; ch < r.tileEnd; ch++) {
         s += k.partSums[uint64_t(ch) * 16 * kDims + e];
         cnt += k.partCnt[ch * 16 + center];
@@ -759,7 +784,8 @@ __global__ __launch_bounds__(256) void k_update(KmLevel k) {
 // counts by every block, so no block depends on another's writes.
 __global__ __launch_bounds__(256) void k_updateU8Part(KmLevel k) {
     __shared__ uint32_t sWave[kWaves], sCnt[2];
-    const uint32_t g = blockIdx.x, y = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t g = k.updTasks ? k.updTasks[blockIdx.x] : blockIdx.x;
+    const uint32_t y = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (!k.active[g]) return;
     const uint32_t cb = k.taskChunkBeg[g], nc = k.taskChunkCnt[g];
     const bool direct = nc <= 32;
@@ -771,10 +797,12 @@ __global__ __launch_bounds__(256) void k_updateU8Part(KmLevel k) {
     if (tid < 2) {
         const uint32_t center = 2 * y + tid;
         uint32_t c = 0;
-        if (direct)
+        if (direct) {
+#pragma unroll 8
             for (uint32_t x = cb; x < cb + nc; x++) c += k.partCnt[x * 16 + center];
-        else
+        } else {
             c = k.counts[g * 16 + center];
+        }
         sCnt[tid] = c;
     }
     __syncthreads();
@@ -789,6 +817,7 @@ __global__ __launch_bounds__(256) void k_updateU8Part(KmLevel k) {
     long long sum;
     if (direct) {
         sum = 128ll * c;  // partials are sums of x - 128
+#pragma unroll 8
         for (uint32_t x = cb; x < cb + nc; x++) sum += k.partSums[uint64_t(x) * 16 * kDims + e];
     } else {
         unsigned long long* sums = reinterpret_cast<unsigned long long*>(k.sums) + uint64_t(g) * 16 * kDims;
@@ -941,13 +970,14 @@ uint32_t lloydTasksTileCap(uint32_t blocksPerCu) {
     RBOW_HIP(hipGetDevice(&d));
     if (d != device) {
         int maxBlock = 0, maxCu = 0;
-        hipFuncAttributes fa{};
+        hipFuncAttributes f4{}, f8{};
         RBOW_HIP(hipDeviceGetAttribute(&maxBlock, hipDeviceAttributeMaxSharedMemoryPerBlock, d));
         RBOW_HIP(hipDeviceGetAttribute(&maxCu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, d));
-        RBOW_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_lloydTasks)));
+        RBOW_HIP(hipFuncGetAttributes(&f4, reinterpret_cast<const void*>(k_lloydTasks<4>)));
+        RBOW_HIP(hipFuncGetAttributes(&f8, reinterpret_cast<const void*>(k_lloydTasks<8>)));
         perBlock = maxBlock;
         perCu = maxCu;
-        fixed = int64_t(fa.sharedSizeBytes);
+        fixed = int64_t(std::max(f4.sharedSizeBytes, f8.sharedSizeBytes));
         device = d;
     }
     const int64_t room = std::min(perBlock, perCu / int64_t(std::max(blocksPerCu, 1u))) - fixed;
@@ -955,14 +985,21 @@ uint32_t lloydTasksTileCap(uint32_t blocksPerCu) {
 }
 
 void launchLloydTasks(const KmLevel& k, const uint32_t* taskList, uint32_t nbList, uint32_t maxTiles,
-                      const uint32_t* taskTileBeg, uint32_t maxIters, hipStream_t s) {
+                      uint32_t blocksPerCu, const uint32_t* taskTileBeg, uint32_t maxIters, hipStream_t s) {
     if (!nbList) return;
-    RBOW_REQUIRE(maxTiles >= 1 && maxTiles <= lloydTasksTileCap(1), RBOW_ERR_INVALID_ARG,
+    RBOW_REQUIRE(maxTiles >= 1 && maxTiles <= lloydTasksTileCap(blocksPerCu), RBOW_ERR_INVALID_ARG,
                  "persistent k-means tasks exceed the LDS capacity");
     const int dyn = int(maxTiles * kLloydTileLds);
-    RBOW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_lloydTasks),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
-    k_lloydTasks<<<nbList, 256, dyn, s>>>(k, taskList, taskTileBeg, maxTiles, maxIters);
+    // one workgroup per CU: 8 waves (2 per SIMD); two per CU: 4 waves each
+    if (blocksPerCu <= 1) {
+        RBOW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_lloydTasks<8>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+        k_lloydTasks<8><<<nbList, 512, dyn, s>>>(k, taskList, taskTileBeg, maxTiles, maxIters);
+    } else {
+        RBOW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_lloydTasks<4>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+        k_lloydTasks<4><<<nbList, 256, dyn, s>>>(k, taskList, taskTileBeg, maxTiles, maxIters);
+    }
     RBOW_HIP(hipGetLastError());
 }
 
@@ -1021,7 +1058,10 @@ void launchRegroup(const KmLevel& k, uint32_t nbRows, const uint32_t* orderIn, u
 
 void launchKmUpdate(const KmLevel& k, hipStream_t s) {
     if (!k.nbTasks) return;
-    if (k.partSums && k.centroidType == 0) k_updateU8Part<<<dim3(k.nbTasks, 16 * kDims / 256), 256, 0, s>>>(k);
+    if (k.partSums && k.centroidType == 0) {
+        const uint32_t n = k.updTasks ? k.nbUpdTasks : k.nbTasks;
+        if (n) k_updateU8Part<<<dim3(n, 16 * kDims / 256), 256, 0, s>>>(k);
+    }
     else if (k.b == 32) k_update<32><<<k.nbTasks, 256, 0, s>>>(k);
     else k_update<16><<<k.nbTasks, 256, 0, s>>>(k);
     RBOW_HIP(hipGetLastError());

@@ -221,7 +221,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         uint32_t* p;
         ~PinnedFree() { (void)hipHostFree(p); }
     } pinnedGuard{hOut};
-    UploadArena up(UploadArena::bytesFor({4 * nGmax, 4 * nGmax, sizeof(KmTile) * tilesMax, 4 * nGmax, 4 * nGmax,
+    UploadArena up(UploadArena::bytesFor({4 * nGmax, 4 * nGmax, sizeof(KmTile) * tilesMax, 4 * nGmax, 4 * nGmax, 4 * nGmax,
                                           4 * (wavesMax + 1), sizeof(KmChunk) * chunksMax, 4 * nGmax, 4 * nGmax,
                                           sizeof(KmChunk) * redMax, opts.seed_mode == 0 ? 4 * nGmax * b : 0}));
 
@@ -314,6 +314,16 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             k.partSums = dPartSums.get();
             k.partCnt = dPartCnt.get();
             k.partChanged = dPartChanged.get();
+            if (!smallTasks.empty()) {  // the update visits the tasks that stream (and the empty ones)
+                const uint32_t* dUpd = nullptr;
+                uint32_t* hUpd = up.take<uint32_t>(nG - smallTasks.size(), &dUpd);
+                uint32_t nbUpd = 0;
+                for (uint32_t g = 0; g < nG; g++)
+                    if (taskTiles[g] || !taskCnt[g]) hUpd[nbUpd++] = g;
+                RBOW_REQUIRE(nbUpd == nG - smallTasks.size(), RBOW_ERR_HIP, "update task list size");
+                k.updTasks = dUpd;
+                k.nbUpdTasks = nbUpd;
+            }
         }
         const KmChunk* dRed = up.put(red.data(), red.size());
 
@@ -356,9 +366,9 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         if (!smallTasks.empty()) {
             if (stats) mark(0);
             // two LDS classes: tasks that allow two workgroups per CU run in a launch of their own
-            launchLloydTasks(k, dSmall, nbLarger, smallTilesMax, dTileBeg, opts.max_iters, s);
+            launchLloydTasks(k, dSmall, nbLarger, smallTilesMax, 1, dTileBeg, opts.max_iters, s);
             launchLloydTasks(k, dSmall + nbLarger, uint32_t(smallTasks.size()) - nbLarger,
-                             std::min(smallTilesMax, persistTiles2), dTileBeg, opts.max_iters, s);
+                             std::min(smallTilesMax, persistTiles2), 2, dTileBeg, opts.max_iters, s);
             if (stats) mark(1);
         }
         // with no streamed task left, one update pass retires the empty tasks
