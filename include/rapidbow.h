@@ -181,6 +181,8 @@ typedef struct {
     uint64_t mem_budget;   /* HBM scratch of admitted queries, bytes (default 4 GiB) */
     uint32_t max_results;  /* K (default 50, <= 1024) */
     int32_t lookup_kernel; /* RBOW_LOOKUP_AUTO (default), RBOW_LOOKUP_MFMA or RBOW_LOOKUP_PLAIN, per batch */
+    uint64_t staging_bytes; /* pinned host ring that submit copies descriptors into (default 256 MiB,
+                             * at least 2 MiB); a submit waits while it is full of queries in flight */
 } rbow_server_options;
 void rbow_server_options_default(rbow_server_options* opts);
 rbow_status rbow_server_create(const rbow_vocabulary* voc, const rbow_database* db, const rbow_server_options* opts,

@@ -43,7 +43,8 @@ class BuildStats(ctypes.Structure):
 
 class ServerOptions(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_uint32), ("nb_streams", ctypes.c_uint32), ("mem_budget", ctypes.c_uint64),
-                ("max_results", ctypes.c_uint32), ("lookup_kernel", ctypes.c_int32)]
+                ("max_results", ctypes.c_uint32), ("lookup_kernel", ctypes.c_int32),
+                ("staging_bytes", ctypes.c_uint64)]
 
 
 class BuildOptions(ctypes.Structure):
@@ -379,11 +380,13 @@ class Server:
     """Continuous-batching query server (no counterpart in the reference)."""
 
     def __init__(self, voc, db, max_batch=64, nb_streams=2, mem_budget=4 << 30, max_results=50,
-                 lookup_kernel=LOOKUP_AUTO):
+                 lookup_kernel=LOOKUP_AUTO, staging_bytes=None):
         o = ServerOptions()
         lib.rbow_server_options_default(ctypes.byref(o))
         o.max_batch, o.nb_streams, o.mem_budget, o.max_results, o.lookup_kernel = (
             max_batch, nb_streams, mem_budget, max_results, lookup_kernel)
+        if staging_bytes is not None:
+            o.staging_bytes = staging_bytes
         h = _vp()
         check(lib.rbow_server_create(voc.handle, db.handle, ctypes.byref(o), ctypes.byref(h)))
         self.handle, self.K, self._keep = h.value, max_results, (voc, db)

@@ -480,6 +480,7 @@ void rbow_server_options_default(rbow_server_options* o) {
     o->mem_budget = 4ull << 30;
     o->max_results = 50;
     o->lookup_kernel = RBOW_LOOKUP_AUTO;
+    o->staging_bytes = 256ull << 20;
 }
 
 rbow_status rbow_server_create(const rbow_vocabulary* voc, const rbow_database* db, const rbow_server_options* opts,
@@ -498,6 +499,7 @@ rbow_status rbow_server_create(const rbow_vocabulary* voc, const rbow_database* 
         so.memBudget = o.mem_budget;
         so.K = o.max_results;
         so.lookupKernel = o.lookup_kernel;
+        so.stagingBytes = o.staging_bytes;
         std::unique_ptr<rbow_server> h(new rbow_server);
         h->srv.reset(new Server(*voc->v, *db->db, so));
         *out = h.release();

@@ -383,31 +383,70 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
             }
             seg = lo;
         }
-        constexpr int U = 4;  // steps of 64 postings per iteration: U loads in flight per lane
+        constexpr int U = 6;  // steps of 64 postings per iteration: U loads in flight per lane
         for (uint32_t base = wave * per; base < wEnd; base += 64 * U) {
             uint32_t sgs[U], slots[U], is[U], prev0 = 0;
             bool valids[U];
+            const uint32_t itEnd = min(base + 64 * U, wEnd);
+            while (sSegEnd[seg] <= base) seg++;  // the segment holding `base` (wave-uniform)
+            // The segments that end inside this iteration are seg, seg + 1, ...: one LDS read
+            // per lane finds them (lane j: segment seg + j), and each position's segment is
+            // then a wave-uniform walk over those boundaries, with no dependent LDS reads.
+            const uint32_t jSeg = seg + lane;
+            const uint32_t endJ = jSeg < nc ? sSegEnd[jSeg] : 0xFFFFFFFFu;
+            const uint64_t ends = __ballot(endJ < itEnd);  // a prefix of the lanes
+            const uint32_t nb = uint32_t(__builtin_popcountll(ends));
+            if (nb < 64) {
+                const uint32_t start0 = seg ? sSegEnd[seg - 1] : 0u;
+                const uint64_t beg0 = sSegBeg[seg];
+                const uint64_t begNext = lane < nb ? sSegBeg[jSeg + 1] : 0ull;  // segment after boundary `lane`
+                uint32_t idx[U], st[U];
+                uint64_t sb[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {  // segments and loads of all steps first
-                const uint32_t idx = base + 64 * u + lane;
-                valids[u] = idx < wEnd;
-                uint32_t sg = seg;
-                // most steps lie inside one segment (wave-uniform test); otherwise search
-                if (sSegEnd[seg] < min(base + 64 * (u + 1), wEnd) && valids[u]) {
-                    uint32_t lo = seg, hi = min(seg + 64u, nc - 1);  // a step spans at most 64 segments
+                for (int u = 0; u < U; u++) {
+                    idx[u] = base + 64 * u + lane;
+                    sgs[u] = seg;
+                    st[u] = start0;
+                    sb[u] = beg0;
+                }
+                for (uint32_t b = 0; b < nb; b++) {
+                    const uint32_t e = uint32_t(__builtin_amdgcn_readlane(int(endJ), int(b)));
+                    const uint64_t nbeg = uint64_t(uint32_t(__builtin_amdgcn_readlane(int(uint32_t(begNext)), int(b)))) |
+                                          (uint64_t(uint32_t(__builtin_amdgcn_readlane(int(uint32_t(begNext >> 32)), int(b)))) << 32);
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        if (idx[u] >= e) {
+                            sgs[u] = seg + b + 1;
+                            st[u] = e;
+                            sb[u] = nbeg;
+                        }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    valids[u] = idx[u] < wEnd;
+                    is[u] = idx[u] - st[u];
+                    const uint32_t* p = v.post + sb[u];
+                    slots[u] = valids[u] ? p[is[u]] : 0xFFFFFFFFu;
+                    if (u == 0 && lane == 0 && valids[0] && is[0] > 0) prev0 = p[is[0] - 1];
+                }
+            } else {  // more than 63 segments end in this iteration: search per position
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const uint32_t idx = base + 64 * u + lane;
+                    valids[u] = idx < wEnd;
+                    uint32_t lo = seg, hi = nc - 1;
                     while (lo < hi) {
                         const uint32_t mid = (lo + hi) / 2;
                         if (sSegEnd[mid] > idx) hi = mid;
                         else lo = mid + 1;
                     }
-                    sg = lo;
+                    const uint32_t sg = valids[u] ? lo : seg;
+                    sgs[u] = sg;
+                    is[u] = idx - (sg ? sSegEnd[sg - 1] : 0u);
+                    const uint32_t* p = v.post + sSegBeg[sg];
+                    slots[u] = valids[u] ? p[is[u]] : 0xFFFFFFFFu;
+                    if (u == 0 && lane == 0 && valids[0] && is[0] > 0) prev0 = p[is[0] - 1];
                 }
-                sgs[u] = sg;
-                is[u] = idx - (sg ? sSegEnd[sg - 1] : 0u);
-                const uint32_t* p = v.post + sSegBeg[sg];
-                slots[u] = valids[u] ? p[is[u]] : 0xFFFFFFFFu;
-                if (u == 0 && lane == 0 && valids[0] && is[0] > 0) prev0 = p[is[0] - 1];
-                seg = uint32_t(__shfl(int(sg), 63, 64));
             }
             uint64_t bnd[U];
             bool heads[U];

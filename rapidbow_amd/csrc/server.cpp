@@ -13,12 +13,20 @@ Server::Server(const Vocabulary& voc, const Database& db, const ServerOptions& o
     RBOW_REQUIRE(o.lookupKernel == RBOW_LOOKUP_PLAIN || o.lookupKernel == RBOW_LOOKUP_MFMA ||
                      o.lookupKernel == RBOW_LOOKUP_AUTO,
                  RBOW_ERR_INVALID_ARG, "bad lookup kernel");
+    RBOW_REQUIRE(o.stagingBytes >= uint64_t(Database::kMaxQueryDesc) * kDims, RBOW_ERR_INVALID_ARG,
+                 "staging_bytes must hold one query of 16384 descriptors (2 MiB)");
     db_.prepareForQueries(nullptr);
     RBOW_HIP(hipDeviceSynchronize());
+    ringCap_ = o.stagingBytes;
+    RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&ring_), ringCap_));
     slots_.resize(o.nbStreams);
     for (auto& s : slots_) {
         RBOW_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));  // default priority
         RBOW_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+        const uint64_t outs = uint64_t(o.maxBatch) * o.K;
+        RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.hDocs), 4 * outs));
+        RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.hScores), 4 * outs));
+        RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.hCounts), 4ull * o.maxBatch));
     }
     t0_ = Clock::now();
     worker_ = std::thread([this] { loop(); });
@@ -35,28 +43,43 @@ Server::~Server() {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         if (s.done) (void)hipEventDestroy(s.done);
         if (s.stream) (void)hipStreamDestroy(s.stream);
-        if (s.hDesc) (void)hipHostFree(s.hDesc);
         if (s.hDocs) (void)hipHostFree(s.hDocs);
         if (s.hScores) (void)hipHostFree(s.hScores);
         if (s.hCounts) (void)hipHostFree(s.hCounts);
     }
+    if (ring_) (void)hipHostFree(ring_);
 }
 
 uint64_t Server::submit(const uint8_t* desc, uint32_t n) {
     RBOW_REQUIRE(desc != nullptr || n == 0, RBOW_ERR_INVALID_ARG, "null descriptors");
     RBOW_REQUIRE(n <= Database::kMaxQueryDesc, RBOW_ERR_CAPACITY, "query above 16384 descriptors");
     Request r;
+    r.arriveMs = nowMs();
     r.n = n;
-    r.desc.assign(desc, desc + uint64_t(n) * kDims);
     r.bytes = db_.scratchBytes(n, opt_.K) + uint64_t(n) * (kDims + 4);
     RBOW_REQUIRE(r.bytes <= opt_.memBudget, RBOW_ERR_CAPACITY, "query scratch exceeds the server's memory budget");
-    r.arriveMs = nowMs();
+    const uint64_t len = uint64_t(n) * kDims;
+    std::lock_guard<std::mutex> sub(subM_);
+    {
+        // reserve ring bytes; a request never wraps (the bytes skipped at the
+        // end of the ring belong to its span and are released with it)
+        std::unique_lock<std::mutex> lk(m_);
+        RBOW_REQUIRE(error_.empty(), RBOW_ERR_HIP, "server stopped: " + error_);
+        uint64_t beg = ringHead_;
+        if (len && beg % ringCap_ + len > ringCap_) beg += ringCap_ - beg % ringCap_;
+        cvRing_.wait(lk, [&] { return beg + len - ringTail_ <= ringCap_ || !error_.empty(); });
+        RBOW_REQUIRE(error_.empty(), RBOW_ERR_HIP, "server stopped: " + error_);
+        ringHead_ = beg + len;
+        r.ringOff = beg;
+        r.span = spanBase_ + spans_.size();
+        spans_.push_back(Span{beg + len, false});
+    }
+    if (len) std::memcpy(ring_ + r.ringOff % ringCap_, desc, len);  // outside the lock: the bytes are ours
     uint64_t t;
     {
         std::lock_guard<std::mutex> lk(m_);
-        RBOW_REQUIRE(error_.empty(), RBOW_ERR_HIP, "server stopped: " + error_);
         t = r.ticket = nextTicket_++;
-        pending_.push_back(std::move(r));
+        pending_.push_back(r);
     }
     cvIn_.notify_one();
     return t;
@@ -87,12 +110,19 @@ void Server::reap(Slot& s) {
             r.doneMs = doneMs;
             r.batchSize = B;
             done_[s.reqs[q].ticket] = std::move(r);
+            spans_[s.reqs[q].span - spanBase_].done = true;
         }
         inFlightBytes_ -= s.bytes;
+        while (!spans_.empty() && spans_.front().done) {  // release the ring in reservation order
+            ringTail_ = spans_.front().end;
+            spans_.pop_front();
+            spanBase_++;
+        }
     }
     s.reqs.clear();
     s.busy = false;
     cvOut_.notify_all();
+    cvRing_.notify_all();
 }
 
 void Server::launch(Slot& s, std::vector<Request>&& batch) {
@@ -100,24 +130,6 @@ void Server::launch(Slot& s, std::vector<Request>&& batch) {
     s.hOff.assign(B + 1, 0);
     for (uint32_t q = 0; q < B; q++) s.hOff[q + 1] = s.hOff[q] + batch[q].n;
     const uint64_t D = s.hOff[B];
-    auto growPinned = [](auto*& p, size_t& cap, size_t need) {
-        if (need <= cap && p) return;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(need, 1) * sizeof(*p)));
-        cap = need;
-    };
-    growPinned(s.hDesc, s.hDescCap, D * kDims);
-    if (uint64_t(B) * K > s.hOutCap || !s.hDocs) {
-        size_t c1 = 0, c2 = 0, c3 = 0;
-        growPinned(s.hDocs, c1, uint64_t(B) * K);
-        growPinned(s.hScores, c2, uint64_t(B) * K);
-        growPinned(s.hCounts, c3, B);
-        s.hOutCap = uint64_t(B) * K;
-        // counts buffer sized with the results: capacity in queries = hOutCap / K
-    }
-    for (uint32_t q = 0; q < B; q++)
-        if (batch[q].n) std::memcpy(s.hDesc + s.hOff[q] * kDims, batch[q].desc.data(), uint64_t(batch[q].n) * kDims);
     hipStream_t st = s.stream;
     void *dDesc = nullptr, *dLeaf = nullptr, *dOff = nullptr, *dDocs = nullptr, *dScores = nullptr, *dCounts = nullptr;
     RBOW_HIP(hipMallocAsync(&dDesc, std::max<uint64_t>(D, 1) * kDims, st));
@@ -126,7 +138,18 @@ void Server::launch(Slot& s, std::vector<Request>&& batch) {
     RBOW_HIP(hipMallocAsync(&dDocs, 4ull * B * K, st));
     RBOW_HIP(hipMallocAsync(&dScores, 4ull * B * K, st));
     RBOW_HIP(hipMallocAsync(&dCounts, 4ull * B, st));
-    if (D) RBOW_HIP(hipMemcpyAsync(dDesc, s.hDesc, D * kDims, hipMemcpyHostToDevice, st));
+    // H2D straight from the staging ring: one copy per run of adjacent requests
+    for (uint32_t q = 0; q < B;) {
+        uint32_t e = q + 1;
+        while (e < B && batch[e].ringOff == batch[e - 1].ringOff + uint64_t(batch[e - 1].n) * kDims &&
+               batch[e].ringOff % ringCap_ != 0)
+            e++;
+        const uint64_t bytes = (s.hOff[e] - s.hOff[q]) * kDims;
+        if (bytes)
+            RBOW_HIP(hipMemcpyAsync(static_cast<uint8_t*>(dDesc) + s.hOff[q] * kDims, ring_ + batch[q].ringOff % ringCap_,
+                                    bytes, hipMemcpyHostToDevice, st));
+        q = e;
+    }
     RBOW_HIP(hipMemcpyAsync(dOff, s.hOff.data(), 8ull * (B + 1), hipMemcpyHostToDevice, st));
     const TreeView t = voc_.view();
     if (D)
@@ -190,6 +213,7 @@ void Server::loop() {
         std::lock_guard<std::mutex> lk(m_);
         error_ = ex.what();
         cvOut_.notify_all();
+        cvRing_.notify_all();
     }
 }
 

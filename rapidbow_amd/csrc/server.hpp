@@ -2,12 +2,14 @@
 // and inverted file.  No counterpart in the reference, which answers one host
 // query at a time (DataBase.cpp:71-190).
 //
-// Requests (descriptors on the host) are admitted into a batch while batch
-// slots and the HBM scratch budget allow; a worker thread launches each
-// batch -- H2D, lookup, scoring, top-K, D2H -- on one of several HIP streams,
-// so batches formed from later arrivals run while earlier ones are still on
-// the device.  Every request's result is independent of what it was batched
-// with (per-query accumulators), which the tests check.
+// Requests (descriptors on the host) are copied by the submitting thread into
+// a pinned staging ring, then admitted into a batch while batch slots and the
+// HBM scratch budget allow; a worker thread launches each batch -- H2D
+// straight from the ring, lookup, scoring, top-K, D2H -- on one of several HIP
+// streams, so batches formed from later arrivals run while earlier ones are
+// still on the device, and the worker itself copies no descriptors.  Every
+// request's result is independent of what it was batched with (per-query
+// accumulators), which the tests check.
 #pragma once
 #include <atomic>
 #include <chrono>
@@ -28,6 +30,7 @@ struct ServerOptions {
     uint64_t memBudget = 4ull << 30;  // HBM scratch of in-flight queries (bytes)
     uint32_t K = 50;
     int lookupKernel = RBOW_LOOKUP_AUTO;  // RBOW_LOOKUP_AUTO / _MFMA / _PLAIN, per batch
+    uint64_t stagingBytes = 256ull << 20;  // pinned ring of submitted descriptors
 };
 
 class Server {
@@ -42,7 +45,8 @@ public:
 
     Server(const Vocabulary& voc, const Database& db, const ServerOptions& o);
     ~Server();
-    // copies the descriptors; returns the ticket
+    // copies the descriptors into the staging ring (waiting for room if it is
+    // full of queries still in flight); returns the ticket
     uint64_t submit(const uint8_t* desc, uint32_t n);
     // blocks until the ticket's result is ready, then removes it
     Result wait(uint64_t ticket);
@@ -52,10 +56,15 @@ public:
 private:
     struct Request {
         uint64_t ticket;
-        std::vector<uint8_t> desc;
+        uint64_t span;     // index of its staging span
+        uint64_t ringOff;  // its descriptors: ring bytes [ringOff, ringOff + n*128) (monotonic offsets)
         uint32_t n;
         double arriveMs;
-        uint64_t bytes;
+        uint64_t bytes;    // HBM scratch
+    };
+    struct Span {          // staging ring bytes up to `end` (monotonic), owned by one request
+        uint64_t end;
+        bool done;
     };
     struct Slot {  // one in-flight batch
         hipStream_t stream = nullptr;
@@ -63,12 +72,9 @@ private:
         bool busy = false;
         std::vector<Request> reqs;
         std::vector<uint64_t> hOff;
-        uint8_t* hDesc = nullptr;  // pinned staging
-        size_t hDescCap = 0;
-        uint32_t* hDocs = nullptr;
+        uint32_t* hDocs = nullptr;  // pinned results, max_batch x K
         float* hScores = nullptr;
         uint32_t* hCounts = nullptr;
-        size_t hOutCap = 0;
         uint64_t bytes = 0;
         double startMs = 0;
     };
@@ -84,6 +90,12 @@ private:
     std::mutex m_;
     std::condition_variable cvIn_, cvOut_;
     std::deque<Request> pending_;
+    std::mutex subM_;             // submitters one at a time: ring order = queue order
+    std::condition_variable cvRing_;
+    uint8_t* ring_ = nullptr;     // pinned staging ring of ringCap_ bytes
+    uint64_t ringCap_ = 0, ringHead_ = 0, ringTail_ = 0;  // monotonic offsets: next free, oldest live
+    std::deque<Span> spans_;      // live spans in reservation order
+    uint64_t spanBase_ = 0;       // index of spans_.front()
     std::unordered_map<uint64_t, Result> done_;
     uint64_t nextTicket_ = 1;
     uint64_t inFlightBytes_ = 0;

@@ -90,6 +90,28 @@ def test_server_alone_vs_mix_vs_oracle(b, L, kernel):
     alone_srv.close()
 
 
+def test_server_staging_ring_wraps_and_waits():
+    # a 2 MiB staging ring (the minimum) for ~4 MiB of queries submitted at once:
+    # submits wait for batches to complete, requests wrap around the ring, and
+    # every result still equals the query alone
+    voc, db, odb, ref_leaf, images = _setup(16, 3, nb_docs=60, seed=4)
+    rng = np.random.default_rng(13)
+    qs = _queries(images, rng, 60)
+    K = 10
+    srv = rb.Server(voc, db, max_batch=8, nb_streams=2, max_results=K, staging_bytes=2 << 20)
+    tickets = [srv.submit(q) for q in qs]
+    mix = [srv.wait(t) for t in tickets]
+    srv.close()
+    assert sum(q.nbytes for q in qs) > 3 << 20
+    for i, q in enumerate(qs):
+        od, os_ = odb.query(ref_leaf(q), K)
+        md, ms, _ = mix[i]
+        _assert_ranked_equal(md, ms, od, os_)
+    with pytest.raises(rb.RbowError) as e:
+        rb.Server(voc, db, staging_bytes=(2 << 20) - 1)
+    assert e.value.status == rb.ERR_INVALID_ARG
+
+
 def test_server_replay_poisson():
     voc, db, odb, ref_leaf, images = _setup(16, 3, nb_docs=80, seed=3)
     rng = np.random.default_rng(9)
