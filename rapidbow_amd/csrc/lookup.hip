@@ -34,7 +34,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr int kScoreBias = 1 << 25;   // makes integer scores positive before key packing
-constexpr uint32_t kTQ = 128;         // descriptors per MFMA work item (8 row blocks of 16)
+constexpr uint32_t kTQ = 256;         // descriptors per MFMA work item (16 row blocks of 16)
 constexpr float kGammaPlain = 1.0f / 65536.0f;  // 2^-16 >= 131 * 2^-24: relative error of the plain FP32 chain
 
 // A work item: up to kTQ descriptors that sit at the same node.  Items of a
@@ -62,16 +62,6 @@ __device__ inline uint16_t bf16Rne(float f) {
 }
 __device__ inline float bf16ToF(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
-// 8 uint8 (as two dwords) -> 8 exact bf16 values
-__device__ inline v8bf bytesToBf16(uint2 w) {
-    v8bf r;
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-        r[e] = __builtin_bit_cast(__bf16, uint16_t(__float_as_uint(float((w.x >> (8 * e)) & 0xFFu)) >> 16));
-        r[4 + e] = __builtin_bit_cast(__bf16, uint16_t(__float_as_uint(float((w.y >> (8 * e)) & 0xFFu)) >> 16));
-    }
-    return r;
-}
 
 // ------------------------------------------------------------ tree prep --
 __global__ void k_prepU8(const uint8_t* rows, uint64_t nbChildren, int32_t* normU, int32_t* normS) {
@@ -219,15 +209,6 @@ __global__ __launch_bounds__(256) void k_lookupPlainF32(const float* __restrict_
 }
 
 // ---------------------------------------------------- bucketing kernels --
-__device__ inline uint32_t waveInclusiveScan(uint32_t v, uint32_t lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t o = uint32_t(__shfl_up(int(v), d, 64));
-        if (lane >= uint32_t(d)) v += o;
-    }
-    return v;
-}
-
 __device__ inline void waveSync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -446,26 +427,36 @@ __global__ __launch_bounds__(256) void k_levelBf16(
             ea[cb] = e1[child];
             eb[cb] = e2[child];
         }
-        // row block rb+1's descriptor bytes are in flight while rb computes
-        auto loadRows = [&](uint32_t rb, uint2 (&w)[4]) {
+        // the descriptor bytes of row blocks rb+1 and rb+2, and the permutation
+        // entries of blocks rb+3 and rb+4, are in flight while rb computes
+        auto permAt = [&](uint32_t rb) {
             const uint32_t r = rb * 16 + col;
-            if (r < t.cnt) {
-                const uint32_t q = kIdentity ? t.start + r : perm[t.start + r];
+            return r < t.cnt ? (kIdentity ? t.start + r : perm[t.start + r]) : 0u;
+        };
+        auto loadRows = [&](uint32_t rb, uint32_t q, uint2 (&w)[4]) {
+            if (rb * 16 + col < t.cnt) {
                 const uint8_t* x = desc + uint64_t(q) * kDims;
 #pragma unroll
                 for (int kb = 0; kb < 4; kb++) w[kb] = *reinterpret_cast<const uint2*>(x + kb * 32 + grp * 8);
             }
         };
-        uint2 wn[4];
-        loadRows(0, wn);
+        uint2 wn[4], wn2[4];
+        uint32_t q2 = permAt(2), q3 = permAt(3);
+        loadRows(0, permAt(0), wn);
+        loadRows(1, permAt(1), wn2);
         for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
             const uint32_t r = rb * 16 + col;
             uint2 wc[4];
 #pragma unroll
-            for (int kb = 0; kb < 4; kb++) wc[kb] = wn[kb];
-            if ((rb + 1) * 16 < t.cnt) loadRows(rb + 1, wn);
+            for (int kb = 0; kb < 4; kb++) {
+                wc[kb] = wn[kb];
+                wn[kb] = wn2[kb];
+            }
+            if ((rb + 2) * 16 < t.cnt) loadRows(rb + 2, q2, wn2);
+            q2 = q3;
+            if ((rb + 4) * 16 < t.cnt) q3 = permAt(rb + 4);
             v8bf ax[4];
-            uint32_t xsq = 0;
+            float xsq = 0.0f;  // ||x - mu||^2 of this lane's dims: integers < 2^24, exact
             if (r < t.cnt) {
 #pragma unroll
                 for (int kb = 0; kb < 4; kb++) {
@@ -473,17 +464,19 @@ __global__ __launch_bounds__(256) void k_levelBf16(
 #pragma unroll
                     for (int e = 0; e < 8; e++) {
                         const uint32_t xw = e < 4 ? w.x : w.y, mw = e < 4 ? muw[kb].x : muw[kb].y;
-                        const int dv = int((xw >> (8 * (e & 3))) & 0xFFu) - int((mw >> (8 * (e & 3))) & 0xFFu);
-                        ax[kb][e] = __builtin_bit_cast(__bf16, uint16_t(__float_as_uint(float(dv)) >> 16));  // exact
-                        xsq += uint32_t(dv * dv);
+                        // byte -> float conversions, the difference in [-255, 255] is exact in
+                        // float and in bf16 (packed two at a time)
+                        const float dv = float((xw >> (8 * (e & 3))) & 0xFFu) - float((mw >> (8 * (e & 3))) & 0xFFu);
+                        ax[kb][e] = __bf16(dv);
+                        xsq = fmaf(dv, dv, xsq);
                     }
                 }
             } else {
 #pragma unroll
                 for (int kb = 0; kb < 4; kb++) ax[kb] = v8bf{};
             }
-            xsq += uint32_t(__shfl_xor(int(xsq), 16, 64));
-            xsq += uint32_t(__shfl_xor(int(xsq), 32, 64));
+            xsq += __shfl_xor(xsq, 16, 64);
+            xsq += __shfl_xor(xsq, 32, 64);
             v4f acc[CB];
 #pragma unroll
             for (int cb = 0; cb < CB; cb++) {
@@ -498,7 +491,7 @@ __global__ __launch_bounds__(256) void k_levelBf16(
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const uint32_t rowLocal = 4 * grp + i;
-                const float xn2 = float(uint32_t(__shfl(int(xsq), int(rowLocal), 64)));  // ||x - mu||^2
+                const float xn2 = __shfl(xsq, int(rowLocal), 64);  // ||x - mu||^2
                 const float xn = sqrtf(xn2) * (1.0f + 1.0f / 1048576.0f);
                 float sc[CB], E[CB];
                 float U = INFINITY;
