@@ -310,15 +310,18 @@ __global__ void k_writeItems(const uint32_t* __restrict__ O, const uint32_t* __r
                              const uint32_t* __restrict__ pN, uint32_t nItems0, uint32_t nb, uint32_t lg2b,
                              const uint32_t* __restrict__ cBase, LkItem* __restrict__ items,
                              uint32_t* __restrict__ nbItems) {
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= nb; v += gridDim.x * blockDim.x) {
+    // one node per block: its items are written by the block's threads (a node
+    // near the root holds thousands of items)
+    for (uint32_t v = blockIdx.x; v <= nb; v += gridDim.x) {
         if (v == nb) {
-            *nbItems = cBase[nb];
+            if (threadIdx.x == 0) *nbItems = cBase[nb];
             continue;
         }
         uint32_t s0, c0;
         childRange(O, pBase, pN, nItems0, lg2b, v, s0, c0);
         const uint32_t base = cBase[v], ni = (c0 + kTQ - 1) / kTQ;
-        for (uint32_t t = 0; t < ni; t++) items[base + t] = LkItem{v, s0 + t * kTQ, min(kTQ, c0 - t * kTQ), base, ni, 0u};
+        for (uint32_t t = threadIdx.x; t < ni; t += blockDim.x)
+            items[base + t] = LkItem{v, s0 + t * kTQ, min(kTQ, c0 - t * kTQ), base, ni, 0u};
     }
 }
 
@@ -712,8 +715,8 @@ void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32
         k_nodeItemCounts<<<gridFor(nbChildren + 1, 256, 4096), 256, 0, s>>>(O, pBase, pN, nItems0, nbChildren, lg2b,
                                                                           nCnt[dst]);
         RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, nCnt[dst], nBase[dst], int(nbChildren + 1), s));
-        k_writeItems<<<gridFor(nbChildren + 1, 256, 4096), 256, 0, s>>>(O, pBase, pN, nItems0, nbChildren, lg2b,
-                                                                      nBase[dst], items[dst], nbItems);
+        k_writeItems<<<std::min(nbChildren + 1, 65536u), 64, 0, s>>>(O, pBase, pN, nItems0, nbChildren, lg2b,
+                                                                    nBase[dst], items[dst], nbItems);
         RBOW_HIP(hipGetLastError());
     };
     buildItems(nullptr, nullptr, items0, b, 0);
