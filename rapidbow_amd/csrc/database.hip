@@ -397,35 +397,31 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
             const uint64_t ends = __ballot(endJ < itEnd);  // a prefix of the lanes
             const uint32_t nb = uint32_t(__builtin_popcountll(ends));
             if (nb < 64) {
-                const uint32_t start0 = seg ? sSegEnd[seg - 1] : 0u;
-                const uint64_t beg0 = sSegBeg[seg];
-                const uint64_t begNext = lane < nb ? sSegBeg[jSeg + 1] : 0ull;  // segment after boundary `lane`
-                uint32_t idx[U], st[U];
-                uint64_t sb[U];
+                // lane j also holds the start and the posting base of the segment after
+                // boundary j - 1 (lane 0: of segment seg itself)
+                const uint32_t prevEnd = uint32_t(__shfl(int(endJ), int(lane ? lane - 1 : 0), 64));  // all lanes
+                const uint32_t startJ = lane == 0 ? (seg ? sSegEnd[seg - 1] : 0u) : prevEnd;
+                const uint64_t begJ = lane <= nb ? sSegBeg[jSeg] : 0ull;
+                uint32_t idx[U], k[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     idx[u] = base + 64 * u + lane;
-                    sgs[u] = seg;
-                    st[u] = start0;
-                    sb[u] = beg0;
+                    k[u] = 0;
                 }
-                for (uint32_t b = 0; b < nb; b++) {
+                for (uint32_t b = 0; b < nb; b++) {  // k[u]: boundaries at or below the position
                     const uint32_t e = uint32_t(__builtin_amdgcn_readlane(int(endJ), int(b)));
-                    const uint64_t nbeg = uint64_t(uint32_t(__builtin_amdgcn_readlane(int(uint32_t(begNext)), int(b)))) |
-                                          (uint64_t(uint32_t(__builtin_amdgcn_readlane(int(uint32_t(begNext >> 32)), int(b)))) << 32);
 #pragma unroll
-                    for (int u = 0; u < U; u++)
-                        if (idx[u] >= e) {
-                            sgs[u] = seg + b + 1;
-                            st[u] = e;
-                            sb[u] = nbeg;
-                        }
+                    for (int u = 0; u < U; u++) k[u] += idx[u] >= e;
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
+                    sgs[u] = seg + k[u];
+                    const uint32_t st = uint32_t(__shfl(int(startJ), int(k[u]), 64));
+                    const uint64_t sb = uint64_t(uint32_t(__shfl(int(uint32_t(begJ)), int(k[u]), 64))) |
+                                        (uint64_t(uint32_t(__shfl(int(uint32_t(begJ >> 32)), int(k[u]), 64))) << 32);
                     valids[u] = idx[u] < wEnd;
-                    is[u] = idx[u] - st[u];
-                    const uint32_t* p = v.post + sb[u];
+                    is[u] = idx[u] - st;
+                    const uint32_t* p = v.post + sb;
                     slots[u] = valids[u] ? p[is[u]] : 0xFFFFFFFFu;
                     if (u == 0 && lane == 0 && valids[0] && is[0] > 0) prev0 = p[is[0] - 1];
                 }
