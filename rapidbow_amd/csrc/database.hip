@@ -454,23 +454,22 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
                 heads[u] = valids[u] && (is[u] == 0 || before != slots[u]);
                 bnd[u] = __ballot(heads[u] || !valids[u]);
             }
+            // first boundary after step u, as a position of the iteration (64 * U: none);
+            // wave-uniform, so the run lengths below need no per-lane loop over steps
+            uint32_t after[U];
+            uint32_t nxt = 64 * U;
+#pragma unroll
+            for (int u = U - 1; u >= 0; u--) {
+                after[u] = nxt;
+                if (bnd[u]) nxt = 64 * u + uint32_t(__builtin_ctzll(bnd[u]));
+            }
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 if (!heads[u]) continue;
                 // run length: distance to the next boundary over this iteration's steps
                 const uint64_t above = lane == 63 ? 0ull : (bnd[u] >> (lane + 1));
-                bool found = above != 0;
-                uint32_t m = found ? uint32_t(__builtin_ctzll(above)) + 1 : 64 - lane;
-#pragma unroll
-                for (int w2 = u + 1; w2 < U; w2++) {
-                    if (found) break;
-                    if (bnd[w2]) {
-                        m += uint32_t(__builtin_ctzll(bnd[w2]));
-                        found = true;
-                    } else {
-                        m += 64;
-                    }
-                }
+                const bool found = above != 0 || after[u] < 64 * U;
+                uint32_t m = above ? uint32_t(__builtin_ctzll(above)) + 1 : after[u] - (64 * u + lane);
                 const uint32_t sg = sgs[u], slot = slots[u], i = is[u];
                 if (!found || base + 64 * u + lane + m >= wEnd) {
                     // no run start follows in this iteration: the run may go on past it or past the share
