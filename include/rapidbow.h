@@ -99,7 +99,7 @@ rbow_status rbow_voc_lookup(const rbow_vocabulary* voc, const void* dev_desc, ui
                             rbow_stream stream);
 
 typedef enum {
-    RBOW_LOOKUP_AUTO = 0,  /* plain below ~128k descriptors (uint8 trees) / ~32k (float trees), else MFMA */
+    RBOW_LOOKUP_AUTO = 0,  /* plain below 131,072 descriptors (uint8 trees) / 49,152 (float b16) / 32,768 (float b32), else MFMA */
     RBOW_LOOKUP_PLAIN = 1, /* plain scalar per-descriptor descent */
     RBOW_LOOKUP_MFMA = 2   /* level-synchronous bucketed MFMA form */
 } rbow_lookup_kernel;

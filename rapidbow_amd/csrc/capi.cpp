@@ -253,7 +253,7 @@ rbow_status rbow_voc_set_lookup_crossover(rbow_vocabulary* voc, uint32_t crossov
 uint32_t rbow_voc_lookup_crossover(const rbow_vocabulary* voc) {
     if (!voc) return 0;
     const Vocabulary& v = *voc->v;
-    return v.lookupAutoMin ? v.lookupAutoMin : (v.centroidType == 0 ? 131072u : 32768u);
+    return v.lookupAutoMin ? v.lookupAutoMin : lookupAutoDefault(v.centroidType, v.b);
 }
 
 rbow_status rbow_voc_lookup(const rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc, uint32_t* dev_leaf,

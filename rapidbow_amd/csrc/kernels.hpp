@@ -46,6 +46,8 @@ void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32
 // Kernel choice (RBOW_LOOKUP_*): AUTO takes the plain kernel below the batch
 // size where the bucketed MFMA form's per-level passes start to pay off
 // (t.autoMin when tuned, else a default measured on MI355X).
+// AUTO crossover of a tree kind when no tuned value is set (measured on MI355X)
+uint32_t lookupAutoDefault(uint32_t centroidType, uint32_t b);
 void launchLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies, int kernel,
                   hipStream_t s);
 // Autotuner: times both lookup kernels on prefixes of desc (powers of two from

@@ -742,10 +742,16 @@ void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32
     RBOW_HIP(hipFreeAsync(ws, s));
 }
 
+uint32_t lookupAutoDefault(uint32_t centroidType, uint32_t b) {
+    // measured crossovers on MI355X (256-descriptor items): uint8 trees ~140k
+    // descriptors, float b16 ~48k, float b32 below 32k
+    return centroidType == 0 ? 131072u : b == 16 ? 49152u : 32768u;
+}
+
 void launchLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies, int kernel,
                   hipStream_t s) {
     // defaults measured on MI355X: ~150k descriptors for uint8 trees, ~40k for float trees
-    const uint32_t autoMin = t.autoMin ? t.autoMin : (t.centroidType == 0 ? 131072u : 32768u);
+    const uint32_t autoMin = t.autoMin ? t.autoMin : lookupAutoDefault(t.centroidType, t.b);
     const bool plain = kernel == RBOW_LOOKUP_PLAIN || t.L == 0 || (kernel == RBOW_LOOKUP_AUTO && n < autoMin);
     if (plain) launchLookupPlain(t, desc, n, leaf, s);
     else launchLookupMfma(t, desc, n, leaf, nearTies, s);
