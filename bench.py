@@ -14,8 +14,10 @@ below the I8 MFMA ridge of ~625 ops/byte).
 
 Beside it on the same JSON line:
   * the same build with the plain scalar kernel (assign TOPS, "MFMA vs plain");
-  * leaf lookups/s on a 1,048,576-word tree (branch 32, depth 4, float32
-    centroids built here by k-means on the 2M descriptors) for 10,000,000
+  * the build of a 1,048,576-word tree (branch 32, depth 4, float32
+    centroids) on the 2M descriptors, BF16-MFMA-filtered k-means vs plain
+    FP32 chains, with the identity check of the trees;
+  * leaf lookups/s on that tree for 10,000,000
     queries: split-BF16 MFMA candidate filter + FP32 re-score vs the plain FP32
     kernel, with the identity check of the leaf ids and the near-tie count;
   * leaf lookups/s on a 4,096-word uint8 tree (branch 16, depth 3) for 50,000
@@ -372,8 +374,14 @@ def main():
 
     # ---------------- 1M-word float32 tree, 10M lookups ----------------
     log("bench: building the b32 L4 float32 tree (1,048,576 words) on the 2M descriptors")
-    v32 = rb.build_vocabulary(d, N, 1000, 4, branch_factor=32, max_iters=10, seed_mode=1, centroid_type=1, kernel=1)
+    v32 = rb.build_vocabulary(d, N, 1000, 4, branch_factor=32, max_iters=10, seed_mode=1, centroid_type=1, kernel=0)
     st32 = v32.build_stats()
+    v32p = rb.build_vocabulary(d, N, 1000, 4, branch_factor=32, max_iters=10, seed_mode=1, centroid_type=1, kernel=1)
+    st32p = v32p.build_stats()
+    same32 = v32p.serialize() == v32.serialize()
+    del v32p
+    log(f"bench: b32 L4 float build: BF16-MFMA filter {st32['total_ms']:.1f} ms, plain {st32p['total_ms']:.1f} ms, "
+        f"identical={same32}")
     NQ = 10_000_000
     q = syn.descriptors(NQ, seed=SEED_QUERY)
     out_m = rb.DeviceArray((NQ,), np.uint32)
@@ -427,7 +435,8 @@ def main():
                                "identical_tree": same_stream},
         "build_ms": {"mfma": round(float(np.median([s["total_ms"] for s in stats])), 2),
                      "plain": round(stp["total_ms"], 2), "identical_trees": same_tree,
-                     "b32L4_f32": round(st32["total_ms"], 1)},
+                     "b32L4_f32": round(st32["total_ms"], 1), "b32L4_f32_plain": round(st32p["total_ms"], 1),
+                     "b32L4_f32_identical": same32},
         "lookup_b32L4_f32_10M": {"mfma_desc_per_s": round(r_m, 1), "plain_desc_per_s": round(r_p, 1),
                                  "identical_leaf_ids": identical, "near_ties": near_ties,
                                  "hbm_gbs_mfma": round(r_m * LOOKUP_BYTES / 1e9, 1),

@@ -98,6 +98,15 @@ struct KmLevel {       // device state of one level's batched k-means
     int32_t* partSums = nullptr;     // [chunk][16][128]  sums of (x - 128)
     uint32_t* partCnt = nullptr;     // [chunk][16]
     uint32_t* partChanged = nullptr; // [chunk]
+    // float centroids, BF16 MFMA filter: per task k_prepF32 of its centroids (integer
+    // centre [task][128], bf16 split [task][b][128], norms and bounds [task][b])
+    const uint8_t* fMu = nullptr;
+    const uint16_t* fHi = nullptr;
+    const uint16_t* fLo = nullptr;
+    const float* fCn2 = nullptr;
+    const float* fE1 = nullptr;
+    const float* fE2 = nullptr;
+    uint32_t* nearTies = nullptr;  // optional: rows re-scored in FP32
     const uint32_t* updTasks = nullptr;  // tasks the MFMA update visits (null: all); the others run elsewhere
     uint32_t nbUpdTasks = 0;
 };

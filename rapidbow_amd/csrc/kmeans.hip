@@ -13,6 +13,7 @@
 // Sums are exact: int32 per wave in LDS, flushed into int64 per task.
 #include <hipcub/hipcub.hpp>
 
+#include "bf16filter.hpp"
 #include "kernels.hpp"
 
 namespace rbow_amd {
@@ -745,6 +746,105 @@ __global__ __launch_bounds__(256) void k_assignF32Plain(KmLevel k) {
     if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
 }
 
+// ---------------------------------------- assign: float, BF16 MFMA filter --
+// Same results as k_assignF32Plain (the plain FP32 chain, ties to the lowest
+// center): per 16-row block, x' = x - mu (exact in BF16) against the task's
+// split centroids (k_prepF32 of this pass's centers) by 2 x 4 x B/16 MFMAs, the
+// certified candidate filter, and the plain chain for near ties (filterBest).
+// Work split and epilogue as the plain kernel: each wave a contiguous tile
+// range, the task operands reloaded when the task changes, exact sums in LDS.
+template <int B>
+__global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
+    constexpr int CB = B / 16;
+    __shared__ uint32_t sSums[kWaves][B * kDims];
+    __shared__ uint32_t sCnt[kWaves][B];
+    __shared__ uint8_t sBest[kWaves][64];  // a tile's assignments, one byte per row
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t col = lane & 15, grp = lane >> 4;
+    WaveAcc<B> acc{sSums[wave], sCnt[wave]};
+    acc.clear(lane);
+    uint32_t beg, end;
+    waveRange(k.nbTiles, beg, end);
+    uint32_t curTask = kIdxInvalid, changedLocal = 0, ties = 0;
+    v8bf bh[CB][4], bl[CB][4];
+    float cn[CB], ea[CB], eb[CB];
+    uint2 muw[4];  // the task's centre on this lane's dims
+    for (uint32_t ti = beg; ti < end; ti++) {
+        const KmTile t = k.tiles[ti];
+        if (!k.active[t.task]) continue;
+        if (t.task != curTask) {
+            if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
+            changedLocal = 0;
+            curTask = t.task;
+            const uint64_t childBase = uint64_t(curTask) * B;
+#pragma unroll
+            for (int kb = 0; kb < 4; kb++)
+                muw[kb] = *reinterpret_cast<const uint2*>(k.fMu + uint64_t(curTask) * kDims + kb * 32 + grp * 8);
+#pragma unroll
+            for (int cb = 0; cb < CB; cb++) {
+                const uint64_t child = childBase + cb * 16 + col;
+#pragma unroll
+                for (int kb = 0; kb < 4; kb++) {
+                    bh[cb][kb] = *reinterpret_cast<const v8bf*>(k.fHi + child * kDims + kb * 32 + grp * 8);
+                    bl[cb][kb] = *reinterpret_cast<const v8bf*>(k.fLo + child * kDims + kb * 32 + grp * 8);
+                }
+                cn[cb] = k.fCn2[child];
+                ea[cb] = k.fE1[child];
+                eb[cb] = k.fE2[child];
+            }
+        }
+        const float* centers = k.centersF32 + uint64_t(curTask) * B * kDims;
+        for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
+            const uint32_t r = rb * 16 + col;
+            v8bf ax[4];
+            float xsq = 0.0f;  // ||x - mu||^2 of this lane's dims: integers < 2^24, exact
+            if (r < t.cnt) {
+                const uint8_t* x = k.X + uint64_t(t.row + r) * kDims;
+#pragma unroll
+                for (int kb = 0; kb < 4; kb++) {
+                    const uint2 w = *reinterpret_cast<const uint2*>(x + kb * 32 + grp * 8);
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        const uint32_t xw = e < 4 ? w.x : w.y, mw = e < 4 ? muw[kb].x : muw[kb].y;
+                        const float dv = float((xw >> (8 * (e & 3))) & 0xFFu) - float((mw >> (8 * (e & 3))) & 0xFFu);
+                        ax[kb][e] = __bf16(dv);
+                        xsq = fmaf(dv, dv, xsq);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int kb = 0; kb < 4; kb++) ax[kb] = v8bf{};
+            }
+            xsq += __shfl_xor(xsq, 16, 64);
+            xsq += __shfl_xor(xsq, 32, 64);
+            v4f d[CB];
+#pragma unroll
+            for (int cb = 0; cb < CB; cb++) {
+                d[cb] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kb = 0; kb < 4; kb++) {
+                    d[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[kb], bh[cb][kb], d[cb], 0, 0, 0);
+                    d[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[kb], bl[cb][kb], d[cb], 0, 0, 0);
+                }
+            }
+            uint32_t best[4];
+            filterBest<CB>(
+                d, xsq, cn, ea, eb, lane, [&](uint32_t rr) { return rb * 16 + rr < t.cnt; },
+                [&](uint32_t rr) { return k.X + uint64_t(t.row + rb * 16 + rr) * kDims; },
+                [&](uint32_t c) { return centers + uint64_t(c) * kDims; }, best, ties);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (col == uint32_t(i)) sBest[wave][rb * 16 + 4 * grp + i] = uint8_t(best[i]);
+        }
+        waveSync();
+        const uint32_t myA = sBest[wave][lane];  // row `lane` (meaningful for lane < cnt)
+        waveSync();
+        changedLocal += tileEpilogue<B>(k, t, lane, myA, acc);
+    }
+    if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
+    if (k.nearTies && ties) atomicAdd(k.nearTies, ties);
+}
+
 // ------------------------------------------------------------ update -----
 template <int B>
 __global__ __launch_bounds__(256) void k_update(KmLevel k) {
@@ -1009,6 +1109,9 @@ void launchKmAssign(const KmLevel& k, int kernel, hipStream_t s) {
     if (k.centroidType == 0) {
         if (kernel == 1) k_assignU8Plain<<<grid, 256, 0, s>>>(k);
         else if (k.nbWaves) k_assignU8Mfma<<<divUp(k.nbWaves, kWaves), 256, 0, s>>>(k);
+    } else if (kernel == 0 && k.fHi) {
+        if (k.b == 32) k_assignF32Mfma<32><<<grid, 256, 0, s>>>(k);
+        else k_assignF32Mfma<16><<<grid, 256, 0, s>>>(k);
     } else if (k.b == 32) {
         k_assignF32Plain<32><<<grid, 256, 0, s>>>(k);
     } else {

@@ -24,18 +24,16 @@
 
 #include <algorithm>
 
+#include "bf16filter.hpp"
 #include "kernels.hpp"
 
 namespace rbow_amd {
 namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr int kScoreBias = 1 << 25;   // makes integer scores positive before key packing
 constexpr uint32_t kTQ = 256;         // descriptors per MFMA work item (16 row blocks of 16)
-constexpr float kGammaPlain = 1.0f / 65536.0f;  // 2^-16 >= 131 * 2^-24: relative error of the plain FP32 chain
 
 // A work item: up to kTQ descriptors that sit at the same node.  Items of a
 // node are consecutive; itemBase/nItems locate the node's items so that the
@@ -490,97 +488,14 @@ __global__ __launch_bounds__(256) void k_levelBf16(
                     acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[kb], bl[cb][kb], acc[cb], 0, 0, 0);
                 }
             }
-            uint32_t cm[4];  // candidate children of rows 4*grp + i (same in all 16 lanes of the group)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint32_t rowLocal = 4 * grp + i;
-                const float xn2 = __shfl(xsq, int(rowLocal), 64);  // ||x - mu||^2
-                const float xn = sqrtf(xn2) * (1.0f + 1.0f / 1048576.0f);
-                float sc[CB], E[CB];
-                float U = INFINITY;
-#pragma unroll
-                for (int cb = 0; cb < CB; cb++) {
-                    sc[cb] = cn[cb] - 2.0f * acc[cb][i];
-                    const float e1v = ea[cb] * xn + eb[cb];  // MFMA score vs exact distance
-                    // plus the plain chain's relative error on the distance itself
-                    E[cb] = (e1v + kGammaPlain * (fmaxf(xn2 + sc[cb], 0.0f) + e1v)) * (1.0f + 1.0f / 1024.0f);
-                    U = fminf(U, sc[cb] + E[cb]);
-                }
-                U = min16f(U);
-                uint32_t m = 0;
-#pragma unroll
-                for (int cb = 0; cb < CB; cb++) {
-                    const uint64_t bal = __ballot(sc[cb] - E[cb] <= U);
-                    m |= uint32_t((bal >> (16 * grp)) & 0xFFFFu) << (16 * cb);
-                }
-                cm[i] = m;
-            }
-            // Near ties: the candidates of the 4 rows of a group are re-scored in
-            // parallel, lane 4*i + s taking candidate s (+4 per extra pass) of row i,
-            // each lane running the plain FP32 chain; a quad minimum picks the child.
             uint32_t best[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) best[i] = __builtin_ctz(cm[i]);
-            const bool tieHere = __builtin_popcount(cm[0]) > 1 || __builtin_popcount(cm[1]) > 1 ||
-                                 __builtin_popcount(cm[2]) > 1 || __builtin_popcount(cm[3]) > 1;
-            if (__ballot(tieHere)) {
-                const uint32_t i = col >> 2, slot = col & 3;
-                const uint32_t mine = i == 0 ? cm[0] : i == 1 ? cm[1] : i == 2 ? cm[2] : cm[3];
-                const uint32_t row = rb * 16 + 4 * grp + i;
-                const bool rowTie = __builtin_popcount(mine) > 1 && row < t.cnt;
-                float bd = INFINITY;
-                uint32_t bj = 0xFFFFFFFFu;
-                const uint32_t npass = (32u + 3u) / 4u;
-                for (uint32_t pass = 0; pass < npass; pass++) {
-                    const uint32_t need = __ballot(rowTie && uint32_t(__builtin_popcount(mine)) > 4 * pass) != 0;
-                    if (!need) break;
-                    // candidate number slot + 4*pass of this row
-                    uint32_t mm = mine, k = slot + 4 * pass, c = 0xFFFFFFFFu;
-                    for (; mm; mm &= mm - 1, k--)
-                        if (k == 0) {
-                            c = __builtin_ctz(mm);
-                            break;
-                        }
-                    float a = INFINITY;
-                    if (rowTie && c != 0xFFFFFFFFu) {
-                        const uint32_t q = kIdentity ? t.start + row : perm[t.start + row];
-                        const uint4* x4 = reinterpret_cast<const uint4*>(desc + uint64_t(q) * kDims);
-                        const float4* cr = reinterpret_cast<const float4*>(rows32 + (childBase + c) * kDims);
-                        a = 0.0f;
-#pragma unroll 4
-                        for (uint32_t w4 = 0; w4 < 8; w4++) {  // 16 dims per step, k ascending
-                            const uint4 xv = x4[w4];
-                            const uint32_t xw[4] = {xv.x, xv.y, xv.z, xv.w};
-#pragma unroll
-                            for (int e = 0; e < 4; e++) {
-                                const float4 cv = cr[4 * w4 + e];
-                                float d = float(xw[e] & 0xFFu) - cv.x;
-                                a = fmaf(d, d, a);
-                                d = float((xw[e] >> 8) & 0xFFu) - cv.y;
-                                a = fmaf(d, d, a);
-                                d = float((xw[e] >> 16) & 0xFFu) - cv.z;
-                                a = fmaf(d, d, a);
-                                d = float(xw[e] >> 24) - cv.w;
-                                a = fmaf(d, d, a);
-                            }
-                        }
-                    }
-                    // keep (distance, child) minimum per quad; lowest child among equal distances
-                    const float qa = min4f(a);
-                    const uint32_t qj = min4u(a == qa && c != 0xFFFFFFFFu ? c : 0xFFFFFFFFu);
-                    if (qa < bd || (qa == bd && qj < bj)) {
-                        bd = qa;
-                        bj = qj;
-                    }
-                }
-                // lane 4*i of the group now holds row i's plain-order argmin
-#pragma unroll
-                for (int ii = 0; ii < 4; ii++) {
-                    const uint32_t v = uint32_t(__shfl(int(bj), int(16 * grp + 4 * ii), 64));
-                    if (__builtin_popcount(cm[ii]) > 1) best[ii] = v;
-                }
-                if (slot == 0 && rowTie) ties++;
-            }
+            filterBest<CB>(
+                acc, xsq, cn, ea, eb, lane, [&](uint32_t rr) { return rb * 16 + rr < t.cnt; },
+                [&](uint32_t rr) {
+                    const uint32_t row = rb * 16 + rr;
+                    return desc + uint64_t(kIdentity ? t.start + row : perm[t.start + row]) * kDims;
+                },
+                [&](uint32_t c) { return rows32 + (childBase + c) * kDims; }, best, ties);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const uint32_t row = rb * 16 + 4 * grp + i;

@@ -212,6 +212,11 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     DevBuf<int32_t> dPartSums(chunksMax * 16 * kDims);
     DevBuf<uint32_t> dPartCnt(chunksMax * 16), dPartChanged(chunksMax);
     DevBuf<uint8_t> dSortWs(regroupWorkspaceBytes(nbDesc, nGmax * b));
+    // float centroids with kernel 0: the BF16 MFMA filter's per-task operands, redone every pass
+    const bool f32Mfma = ct == 1 && kernel == 0;
+    DevBuf<uint8_t> dFMu(f32Mfma ? nGmax * kDims : 0);
+    DevBuf<uint16_t> dFHi(f32Mfma ? nGmax * b * kDims : 0), dFLo(f32Mfma ? nGmax * b * kDims : 0);
+    DevBuf<float> dFCn2(f32Mfma ? nGmax * b : 0), dFE1(f32Mfma ? nGmax * b : 0), dFE2(f32Mfma ? nGmax * b : 0);
     // per-level results, downloaded with one copy: [hist nG*b][passes nG][masks nG]
     DevBuf<uint32_t> dOut(nGmax * b + 2 * nGmax);
     uint32_t* hOut = nullptr;
@@ -270,6 +275,14 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         k.passes = dHist + uint64_t(nG) * b;
         uint32_t* dMasks = k.passes + nG;
         k.nbActive = dNbActive.get();
+        if (f32Mfma) {
+            k.fMu = dFMu.get();
+            k.fHi = dFHi.get();
+            k.fLo = dFLo.get();
+            k.fCn2 = dFCn2.get();
+            k.fE1 = dFE1.get();
+            k.fE2 = dFE2.get();
+        }
 
         // Tasks of <= persistTiles tiles run their whole Lloyd loop in one launch with
         // their points in LDS (k_lloydTasks), when there are enough of them to fill
@@ -379,6 +392,9 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         for (uint32_t it = 0; it < iters; it++) {
             const size_t e = 2 + 5 * size_t(it);
             if (stats) mark(e);
+            if (f32Mfma)  // the filter's operands of this pass's centers
+                launchPrepF32Tree(k.centersF32, nG, b, dFMu.get(), dFHi.get(), dFLo.get(), dFCn2.get(), dFE1.get(),
+                                  dFE2.get(), s);
             launchKmAssign(k, kernel, s);
             if (stats) mark(e + 1);
             if (mfmaPath) launchKmReduce(k, dRed, uint32_t(red.size()), s);

@@ -59,11 +59,13 @@ def test_build_voc_tests_invariants():
         _check_u8(voc, res, L)
 
 
+@pytest.mark.parametrize("kernel", [0, 1])
 @pytest.mark.parametrize("b,L", [(32, 2), (16, 2)])
-def test_build_f32_parity(b, L):
+def test_build_f32_parity(b, L, kernel):
+    # kernel 0: BF16 MFMA candidate filter + FP32 re-score of near ties; 1: plain FP32 chains
     desc = rb.synth_descriptors(20000, seed=4)
     res = oracle.build(desc, 64, L, b=b, max_iters=10, seed_mode=1, centroid_type=1)
-    voc = _gpu_build(desc, 64, L, branch_factor=b, max_iters=10, seed_mode=1, centroid_type=1, kernel=1)
+    voc = _gpu_build(desc, 64, L, branch_factor=b, max_iters=10, seed_mode=1, centroid_type=1, kernel=kernel)
     for l in range(L):
         got = voc.centroids(l)
         exp = res["nodes"][level_beg(b, l):level_beg(b, l + 1)]
@@ -77,6 +79,20 @@ def test_build_f32_parity(b, L):
     np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_PLAIN), exp)
     blob = voc.serialize()
     assert rb.deserialize_vocabulary(blob).serialize() == blob
+
+
+@pytest.mark.parametrize("b", [32, 16])
+def test_build_f32_mfma_equals_plain_three_levels(b):
+    # 120k points, 3 levels (uneven tasks, many passes): the MFMA-filtered float
+    # build must give the plain build's centroids bit for bit
+    desc = rb.synth_descriptors(120000, seed=9, nb_centres=500)
+    kw = dict(branch_factor=b, max_iters=12, seed_mode=1, centroid_type=1)
+    vm = _gpu_build(desc, 100, 3, kernel=0, **kw)
+    vp = _gpu_build(desc, 100, 3, kernel=1, **kw)
+    for l in range(3):
+        np.testing.assert_array_equal(vm.centroids(l), vp.centroids(l))
+    np.testing.assert_array_equal(vm.build_passes(), vp.build_passes())
+    assert vm.serialize() == vp.serialize()
 
 
 def test_build_errors():
