@@ -32,10 +32,11 @@ struct TreeView {
     uint32_t autoMin = 0;  // AUTO lookup: batches below this use the plain kernel (0: built-in default)
 };
 
-// Derived per-child data (norms, bf16 split, bounds) for nodes [0, nbNodes).
+// Derived per-child data (norms, bf16 split, bounds) for nodes [0, nbNodes); the
+// float form skips the nodes whose flag in `active` (optional) is zero.
 void launchPrepU8Tree(const uint8_t* rows, uint64_t nbChildren, int32_t* normU, int32_t* normS, hipStream_t s);
 void launchPrepF32Tree(const float* rows, uint64_t nbNodes, uint32_t b, uint8_t* mu, uint16_t* hi, uint16_t* lo,
-                       float* cn2, float* e1, float* e2, hipStream_t s);
+                       float* cn2, float* e1, float* e2, hipStream_t s, const uint32_t* active = nullptr);
 
 // Plain scalar per-descriptor descent (vocTreeLookUp.cu:48-160 semantics).
 void launchLookupPlain(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s);

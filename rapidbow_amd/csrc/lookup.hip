@@ -85,9 +85,11 @@ __global__ void k_prepU8(const uint8_t* rows, uint64_t nbChildren, int32_t* norm
 //               sequential FP32 accumulation of the 256 products, rounding of the score
 //   e2   = 2^-22 ||c'||^2 + 1e-4
 __global__ __launch_bounds__(128) void k_prepF32(const float* rows, uint32_t b, uint8_t* mu, uint16_t* hi,
-                                                uint16_t* lo, float* cn2, float* e1, float* e2) {
+                                                uint16_t* lo, float* cn2, float* e1, float* e2,
+                                                const uint32_t* active) {
     __shared__ double red[4][2][32];
     const uint64_t node = blockIdx.x;
+    if (active && !active[node]) return;  // a converged k-means task keeps its operands
     const uint32_t k = threadIdx.x, lane = k & 63, wave = k >> 6;
     const float* c = rows + node * b * kDims;
     double m = 0;
@@ -552,9 +554,9 @@ void launchPrepU8Tree(const uint8_t* rows, uint64_t nbChildren, int32_t* normU, 
 }
 
 void launchPrepF32Tree(const float* rows, uint64_t nbNodes, uint32_t b, uint8_t* mu, uint16_t* hi, uint16_t* lo,
-                       float* cn2, float* e1, float* e2, hipStream_t s) {
+                       float* cn2, float* e1, float* e2, hipStream_t s, const uint32_t* active) {
     if (!nbNodes) return;
-    k_prepF32<<<uint32_t(nbNodes), 128, 0, s>>>(rows, b, mu, hi, lo, cn2, e1, e2);
+    k_prepF32<<<uint32_t(nbNodes), 128, 0, s>>>(rows, b, mu, hi, lo, cn2, e1, e2, active);
     RBOW_HIP(hipGetLastError());
 }
 

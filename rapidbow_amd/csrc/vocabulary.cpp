@@ -394,7 +394,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             if (stats) mark(e);
             if (f32Mfma)  // the filter's operands of this pass's centers
                 launchPrepF32Tree(k.centersF32, nG, b, dFMu.get(), dFHi.get(), dFLo.get(), dFCn2.get(), dFE1.get(),
-                                  dFE2.get(), s);
+                                  dFE2.get(), s, k.active);
             launchKmAssign(k, kernel, s);
             if (stats) mark(e + 1);
             if (mfmaPath) launchKmReduce(k, dRed, uint32_t(red.size()), s);
@@ -464,11 +464,22 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
 
     // ---- IDF weights (Vocabulary.cpp:103-108): log of the binary32 quotient
     // evaluated in binary64, rounded to binary32 ----
-    for (uint64_t i = 0; i < occ.size(); i++) {
-        const uint32_t o = occ[i];
-        voc->weights[i] = o == 0 ? FLT_MIN : std::max(FLT_MIN, float(std::log(double(float(nbDoc) / float(o)))));
+    // (one evaluation per distinct count: a level holds ~1M children but few distinct counts)
+    voc->prepare(s);  // the tree's derived data on the device meanwhile
+    auto idf = [&](uint32_t o) {
+        return o == 0 ? FLT_MIN : std::max(FLT_MIN, float(std::log(double(float(nbDoc) / float(o)))));
+    };
+    const uint32_t maxOcc = occ.empty() ? 0u : *std::max_element(occ.begin(), occ.end());
+    if (maxOcc < (1u << 22)) {
+        std::vector<float> memo(size_t(maxOcc) + 1, -1.0f);  // weights are >= FLT_MIN > 0
+        for (uint64_t i = 0; i < occ.size(); i++) {
+            float& w = memo[occ[i]];
+            if (w < 0.0f) w = idf(occ[i]);
+            voc->weights[i] = w;
+        }
+    } else {
+        for (uint64_t i = 0; i < occ.size(); i++) voc->weights[i] = idf(occ[i]);
     }
-    voc->prepare(s);
     RBOW_HIP(hipStreamSynchronize(s));
     if (stats) {
         stats->passes = std::move(passesAll);
