@@ -65,9 +65,11 @@ void checkAligned(const void* p, const char* what) {
 
 // Task table + tiles for the kernel-level k-means entries.
 struct KmScratch {
-    DevBuf<uint32_t> beg, cnt, active, passes, nbActive, order, tcBeg, tcCnt, partCnt, partChanged, waveChunkBeg;
+    DevBuf<uint32_t> beg, cnt, active, passes, nbActive, order, tpBeg, tpCnt, partCnt, partChanged, waveChunkBeg;
     DevBuf<KmTile> tiles;
-    DevBuf<KmChunk> chunks, red;
+    DevBuf<KmChunk> chunks;
+    DevBuf<KmSlot> slots;
+    DevBuf<KmPartRange> red;
     DevBuf<int8_t> XA;
     DevBuf<int32_t> partSums;
     uint32_t nbRed = 0;
@@ -119,11 +121,12 @@ struct KmScratch {
         for (uint32_t g = 0; g < k.nbTasks; g++) taskTiles[g] = divUp(taskCnt[g], 64);
         KmPartition part = partitionKmTiles(taskTiles, {});
         std::vector<KmChunk>& hc = part.chunks;
-        std::vector<uint32_t>& cb = part.taskChunkBeg;
-        std::vector<uint32_t>& cc = part.taskChunkCnt;
-        std::vector<KmChunk> hr;  // every task is reduced into the caller's totals
+        std::vector<uint32_t>& cb = part.taskPartBeg;
+        std::vector<uint32_t>& cc = part.taskPartCnt;
+        std::vector<KmPartRange> hr;  // every task is reduced into the caller's totals
         for (uint32_t g = 0; g < k.nbTasks; g++)
-            for (uint32_t c = 0; c < cc[g]; c += 32) hr.push_back(KmChunk{g, cb[g] + c, cb[g] + std::min(cc[g], c + 32)});
+            for (uint32_t c = 0; c < cc[g]; c += 32)
+                hr.push_back(KmPartRange{g, cb[g] + c, cb[g] + std::min(cc[g], c + 32)});
         waveChunkBeg.resize(part.waveChunkBeg.size());
         RBOW_HIP(hipMemcpyAsync(waveChunkBeg.get(), part.waveChunkBeg.data(), 4 * part.waveChunkBeg.size(),
                                 hipMemcpyHostToDevice, s));
@@ -132,25 +135,32 @@ struct KmScratch {
         order.resize(std::max<uint64_t>(rows, 1));
         chunks.resize(std::max<size_t>(hc.size(), 1));
         red.resize(std::max<size_t>(hr.size(), 1));
-        tcBeg.resize(std::max(1u, k.nbTasks));
-        tcCnt.resize(std::max(1u, k.nbTasks));
+        slots.resize(std::max<size_t>(part.blockSlots.size(), 1));
+        tpBeg.resize(std::max(1u, k.nbTasks));
+        tpCnt.resize(std::max(1u, k.nbTasks));
         XA.resize(std::max<size_t>(ht.size(), 1) * 8192);
-        partSums.resize(std::max<size_t>(hc.size(), 1) * 16 * kDims);
-        partCnt.resize(std::max<size_t>(hc.size(), 1) * 16);
-        partChanged.resize(std::max<size_t>(hc.size(), 1));
+        partSums.resize(std::max<size_t>(part.nbParts, 1) * 16 * kDims);
+        partCnt.resize(std::max<size_t>(part.nbParts, 1) * 16);
+        partChanged.resize(std::max<size_t>(part.nbParts, 1));
         if (rows) RBOW_HIP(hipMemcpyAsync(order.get(), hOrder.data(), 4 * rows, hipMemcpyHostToDevice, s));
         if (!hc.empty()) RBOW_HIP(hipMemcpyAsync(chunks.get(), hc.data(), sizeof(KmChunk) * hc.size(), hipMemcpyHostToDevice, s));
-        if (!hr.empty()) RBOW_HIP(hipMemcpyAsync(red.get(), hr.data(), sizeof(KmChunk) * hr.size(), hipMemcpyHostToDevice, s));
+        if (!hr.empty())
+            RBOW_HIP(hipMemcpyAsync(red.get(), hr.data(), sizeof(KmPartRange) * hr.size(), hipMemcpyHostToDevice, s));
+        if (!part.blockSlots.empty())
+            RBOW_HIP(hipMemcpyAsync(slots.get(), part.blockSlots.data(), sizeof(KmSlot) * part.blockSlots.size(),
+                                    hipMemcpyHostToDevice, s));
         if (k.nbTasks) {
-            RBOW_HIP(hipMemcpyAsync(tcBeg.get(), cb.data(), 4ull * k.nbTasks, hipMemcpyHostToDevice, s));
-            RBOW_HIP(hipMemcpyAsync(tcCnt.get(), cc.data(), 4ull * k.nbTasks, hipMemcpyHostToDevice, s));
+            RBOW_HIP(hipMemcpyAsync(tpBeg.get(), cb.data(), 4ull * k.nbTasks, hipMemcpyHostToDevice, s));
+            RBOW_HIP(hipMemcpyAsync(tpCnt.get(), cc.data(), 4ull * k.nbTasks, hipMemcpyHostToDevice, s));
         }
         launchGatherTiles(points, order.get(), tiles.get(), uint32_t(ht.size()), XA.get(), s);
         k.XA = XA.get();
         k.chunks = chunks.get();
         k.nbChunks = uint32_t(hc.size());
-        k.taskChunkBeg = tcBeg.get();
-        k.taskChunkCnt = tcCnt.get();
+        k.blockSlots = slots.get();
+        k.nbParts = part.nbParts;
+        k.taskPartBeg = tpBeg.get();
+        k.taskPartCnt = tpCnt.get();
         k.partSums = partSums.get();
         k.partCnt = partCnt.get();
         k.partChanged = partChanged.get();

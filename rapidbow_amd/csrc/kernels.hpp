@@ -70,8 +70,10 @@ struct KmTile {        // one work item of the assign kernels: <= 64 rows of one
 };
 
 // Partition of the level's streamed tiles over the resident waves of the MFMA
-// assign (partitionTiles with the device's resident wave count).
+// assign (partitionTiles with the device's resident wave count and the
+// kernel's waves per workgroup, kmAssignWavesPerBlock()).
 KmPartition partitionKmTiles(const std::vector<uint32_t>& taskTiles, const std::vector<uint32_t>& tileBeg);
+uint32_t kmAssignWavesPerBlock();
 
 struct KmLevel {       // device state of one level's batched k-means
     uint32_t b = 16, nbTasks = 0, nbTiles = 0, centroidType = 0;
@@ -88,17 +90,20 @@ struct KmLevel {       // device state of one level's batched k-means
     uint32_t* active = nullptr;      // [task]
     uint32_t* passes = nullptr;      // [task]
     uint32_t* nbActive = nullptr;    // [1]
-    // MFMA path (uint8 centroids): per-tile swizzled operand and per-chunk partials
+    // MFMA path (uint8 centroids): per-tile swizzled operand and partial sums
+    // (one per chunk, or one per run of a task's chunks merged in a workgroup)
     const int8_t* XA = nullptr;      // [tile][rb 4][kb 2][lane 64][16]  distance A fragments (x - 128)
     const KmChunk* chunks = nullptr;
     uint32_t nbChunks = 0;
     const uint32_t* waveChunkBeg = nullptr;  // [nbWaves + 1]
     uint32_t nbWaves = 0;
-    const uint32_t* taskChunkBeg = nullptr;  // [task]
-    const uint32_t* taskChunkCnt = nullptr;  // [task]
-    int32_t* partSums = nullptr;     // [chunk][16][128]  sums of (x - 128)
-    uint32_t* partCnt = nullptr;     // [chunk][16]
-    uint32_t* partChanged = nullptr; // [chunk]
+    const KmSlot* blockSlots = nullptr;      // [workgroup][waves per workgroup - 1]
+    uint32_t nbParts = 0;
+    const uint32_t* taskPartBeg = nullptr;   // [task]
+    const uint32_t* taskPartCnt = nullptr;   // [task]
+    int32_t* partSums = nullptr;     // [part][16][128]  sums of (x - 128)
+    uint32_t* partCnt = nullptr;     // [part][16]
+    uint32_t* partChanged = nullptr; // [part]
     // float centroids, BF16 MFMA filter: per task k_prepF32 of its centroids (integer
     // centre [task][128], bf16 split [task][b][128], norms and bounds [task][b])
     const uint8_t* fMu = nullptr;
@@ -122,9 +127,9 @@ void launchKmLevelReset(const KmLevel& k, uint64_t nbRows, hipStream_t s);
 // Per-tile swizzled operand XA of the MFMA assign from desc[order[row]].
 void launchGatherTiles(const uint8_t* desc, const uint32_t* order, const KmTile* tiles, uint32_t nbTiles, int8_t* XA,
                        hipStream_t s);
-// Adds the per-chunk partials of the MFMA assign into the task totals
-// counts / sums / changed.  red: (task, chunkBeg, chunkEnd) items.
-void launchKmReduce(const KmLevel& k, const KmChunk* red, uint32_t nbRed, hipStream_t s);
+// Adds the partials of the MFMA assign into the task totals counts / sums /
+// changed.  red: (task, partBeg, partEnd) items.
+void launchKmReduce(const KmLevel& k, const KmPartRange* red, uint32_t nbRed, hipStream_t s);
 // Whole Lloyd loops (assign + update, up to maxIters passes, per-task stop) of
 // the listed uint8 tasks, one workgroup each with the task's points in LDS;
 // leaves them inactive with their pass counts.  Every listed task has 1 to

@@ -428,20 +428,27 @@ __device__ __forceinline__ uint32_t assignTileMfma(const v4i (&a)[4][2], const u
 __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
     __shared__ __align__(16) uint8_t sAsg[kWaves][64];          // a tile's new assignments, one byte per row
     __shared__ __align__(16) uint8_t sTile[kWaves][kTileLds];  // the tile, row-major, for the XT reads
+    // merge slots: a task's chunks on neighbouring waves of this workgroup add
+    // their sums, counts and #changed here and leave as one partial
+    __shared__ __align__(16) int32_t sMergeSums[kWaves - 1][16 * kDims];
+    __shared__ uint32_t sMergeCnt[kWaves - 1][17];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 15, grp = lane >> 4;
     const v4i zero = {0, 0, 0, 0};
+    for (uint32_t i = threadIdx.x; i < (kWaves - 1) * 16 * kDims / 4; i += 256)
+        reinterpret_cast<v4i*>(&sMergeSums[0][0])[i] = zero;
+    if (threadIdx.x < (kWaves - 1) * 17) (&sMergeCnt[0][0])[threadIdx.x] = 0u;
+    __syncthreads();
     // one wave per resident slot, each with an equal range of tiles cut into
     // per-task chunks on the host (partitionKmTiles): no wave waits for a
     // second round and the loads balance to within one tile
     const uint32_t w = blockIdx.x * kWaves + wave;
-    if (w >= k.nbWaves) return;
-    const uint32_t chEnd = k.waveChunkBeg[w + 1];
+    const uint32_t chEnd = w < k.nbWaves ? k.waveChunkBeg[w + 1] : 0u;
     auto nextActive = [&](uint32_t c0) {
         while (c0 < chEnd && !k.active[k.chunks[c0].task]) c0++;
         return c0;
     };
-    uint32_t ch = nextActive(k.waveChunkBeg[w]);
+    uint32_t ch = w < k.nbWaves ? nextActive(k.waveChunkBeg[w]) : 0u;
     // the first tile of the wave's next chunk is requested while the last tile of
     // the current chunk computes
     v4i a[4][2];
@@ -490,18 +497,38 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
             t = tn;
             prev = prevN;
         }
-        int32_t* ps = k.partSums + uint64_t(ch) * 16 * kDims;
-#pragma unroll
-        for (int db = 0; db < 8; db++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) ps[(4 * grp + i) * kDims + 16 * db + col] = acc[db][i];
         myCnt += uint32_t(__shfl_xor(int(myCnt), 16, 64));
         myCnt += uint32_t(__shfl_xor(int(myCnt), 32, 64));
-        if (lane < 16) k.partCnt[ch * 16 + lane] = myCnt;
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) myChanged += uint32_t(__shfl_xor(int(myChanged), m, 64));
-        if (lane == 0) k.partChanged[ch] = myChanged;
+        if (c.slot == kNoSlot) {  // the chunk's own partial
+            int32_t* ps = k.partSums + uint64_t(c.part) * 16 * kDims;
+#pragma unroll
+            for (int db = 0; db < 8; db++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) ps[(4 * grp + i) * kDims + 16 * db + col] = acc[db][i];
+            if (lane < 16) k.partCnt[c.part * 16 + lane] = myCnt;
+            if (lane == 0) k.partChanged[c.part] = myChanged;
+        } else {  // shared with a neighbouring wave: add into the workgroup's slot
+            int32_t* ms = sMergeSums[c.slot];
+#pragma unroll
+            for (int db = 0; db < 8; db++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) atomicAdd(&ms[(4 * grp + i) * kDims + 16 * db + col], acc[db][i]);
+            if (lane < 16) atomicAdd(&sMergeCnt[c.slot][lane], myCnt);
+            if (lane == 0) atomicAdd(&sMergeCnt[c.slot][16], myChanged);
+        }
         ch = chNext;
+    }
+    __syncthreads();
+    // the merged partials leave once (a slot of a converged task was never written)
+    for (uint32_t sl = 0; sl < kWaves - 1; sl++) {
+        const KmSlot slot = k.blockSlots[blockIdx.x * (kWaves - 1) + sl];
+        if (slot.part == kNoSlot || !k.active[slot.task]) continue;
+        v4i* ps = reinterpret_cast<v4i*>(k.partSums + uint64_t(slot.part) * 16 * kDims);
+        for (uint32_t i = threadIdx.x; i < 16 * kDims / 4; i += 256) ps[i] = reinterpret_cast<const v4i*>(sMergeSums[sl])[i];
+        if (threadIdx.x < 16) k.partCnt[slot.part * 16 + threadIdx.x] = sMergeCnt[sl][threadIdx.x];
+        if (threadIdx.x == 16) k.partChanged[slot.part] = sMergeCnt[sl][16];
     }
 }
 
@@ -608,23 +635,23 @@ __global__ __launch_bounds__(64 * W) void k_lloydTasks(KmLevel k, const uint32_t
     }
 }
 
-// Reduce items: (task, chunk range); adds the partials into the task totals.
+// Reduce items: (task, partial range); adds the partials into the task totals.
 // Grid (item, 8): block y owns elements [256y, 256y + 256) of the 16 x 128 sums.
-__global__ __launch_bounds__(256) void k_reduceChunks(KmLevel k, const KmChunk* __restrict__ red, uint32_t nbRed) {
-    const KmChunk r = red[blockIdx.x];
+__global__ __launch_bounds__(256) void k_reduceChunks(KmLevel k, const KmPartRange* __restrict__ red, uint32_t nbRed) {
+    const KmPartRange r = red[blockIdx.x];
     const uint32_t g = r.task, tid = threadIdx.x, e = blockIdx.y * 256 + tid, center = e / kDims;
     if (!k.active[g]) return;
     if (blockIdx.y == 0 && tid <= 16) {
         uint32_t c = 0;
-        for (uint32_t ch = r.tileBeg; ch < r.tileEnd; ch++) c += tid < 16 ? k.partCnt[ch * 16 + tid] : k.partChanged[ch];
+        for (uint32_t x = r.partBeg; x < r.partEnd; x++) c += tid < 16 ? k.partCnt[x * 16 + tid] : k.partChanged[x];
         if (c) atomicAdd(tid < 16 ? &k.counts[g * 16 + tid] : &k.changed[g], c);
     }
     long long s = 0;
     uint32_t cnt = 0;
 #pragma unroll 16
-    for (uint32_t ch = r.tileBeg; ch < r.tileEnd; ch++) {
-        s += k.partSums[uint64_t(ch) * 16 * kDims + e];
-        cnt += k.partCnt[ch * 16 + center];
+    for (uint32_t x = r.partBeg; x < r.partEnd; x++) {
+        s += k.partSums[uint64_t(x) * 16 * kDims + e];
+        cnt += k.partCnt[x * 16 + center];
     }
     s += 128ll * cnt;  // partials are sums of x - 128
     if (s) atomicAdd(reinterpret_cast<unsigned long long*>(k.sums) + uint64_t(g) * 16 * kDims + e, (unsigned long long)s);
@@ -879,15 +906,15 @@ __global__ __launch_bounds__(256) void k_update(KmLevel k) {
 
 // Update of the MFMA path, grid (task, 8): block y owns centers 2y, 2y+1, i.e.
 // elements [256y, 256y + 256) of the task's 16 x 128 sums.  Tasks with <= 32
-// chunks reduce their partials here (no atomics); larger tasks were reduced
-// into the totals by k_reduceChunks.  #changed is summed from the per-chunk
-// counts by every block, so no block depends on another's writes.
+// partials reduce them here (no atomics); larger tasks were reduced into the
+// totals by k_reduceChunks.  #changed is summed from the partials' counts by
+// every block, so no block depends on another's writes.
 __global__ __launch_bounds__(256) void k_updateU8Part(KmLevel k) {
     __shared__ uint32_t sWave[kWaves], sCnt[2];
     const uint32_t g = k.updTasks ? k.updTasks[blockIdx.x] : blockIdx.x;
     const uint32_t y = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (!k.active[g]) return;
-    const uint32_t cb = k.taskChunkBeg[g], nc = k.taskChunkCnt[g];
+    const uint32_t cb = k.taskPartBeg[g], nc = k.taskPartCnt[g];
     const bool direct = nc <= 32;
     uint32_t ch = 0;
     for (uint32_t c = tid; c < nc; c += 256) ch += k.partChanged[cb + c];
@@ -1034,7 +1061,7 @@ void launchGatherTiles(const uint8_t* desc, const uint32_t* order, const KmTile*
     RBOW_HIP(hipGetLastError());
 }
 
-void launchKmReduce(const KmLevel& k, const KmChunk* red, uint32_t nbRed, hipStream_t s) {
+void launchKmReduce(const KmLevel& k, const KmPartRange* red, uint32_t nbRed, hipStream_t s) {
     if (!nbRed) return;
     k_reduceChunks<<<dim3(nbRed, 16 * kDims / 256), 256, 0, s>>>(k, red, nbRed);
     RBOW_HIP(hipGetLastError());
@@ -1056,8 +1083,10 @@ uint32_t kmAssignResidentBlocks() {
 }
 
 KmPartition partitionKmTiles(const std::vector<uint32_t>& taskTiles, const std::vector<uint32_t>& tileBeg) {
-    return partitionTiles(taskTiles, tileBeg, uint64_t(kmAssignResidentBlocks()) * kWaves);
+    return partitionTiles(taskTiles, tileBeg, uint64_t(kmAssignResidentBlocks()) * kWaves, kWaves);
 }
+
+uint32_t kmAssignWavesPerBlock() { return kWaves; }
 
 namespace {
 constexpr uint32_t kLloydTileLds = kTileLds + 64;  // LDS per tile: the rows + their assignments

@@ -199,8 +199,10 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     const bool mfmaPath = ct == 0 && kernel == 0;
     const uint64_t nGmax = uint64_t(1) << (lg2b * (L - 1));  // tasks of the last level
     const uint64_t tilesMax = divUp(nbDesc, 64u) + nGmax;   // each task adds at most one partial tile
-    const uint64_t wavesMax = mfmaPath ? uint64_t(kmAssignResidentBlocks()) * 4 : 0;
+    const uint64_t blocksMax = mfmaPath ? uint64_t(kmAssignResidentBlocks()) : 0;
+    const uint64_t wavesMax = blocksMax * kmAssignWavesPerBlock();
     const uint64_t chunksMax = mfmaPath ? wavesMax + nGmax : 0;  // a wave's range splits at task boundaries
+    const uint64_t slotsMax = blocksMax * (kmAssignWavesPerBlock() - 1);
     const uint64_t redMax = mfmaPath ? chunksMax / 32 + nGmax : 0;
     const uint32_t persistTiles = mfmaPath ? std::min(opts.persist_max_tiles, lloydTasksTileCap(1)) : 0u;
     const uint32_t persistTiles2 = mfmaPath ? std::min(persistTiles, lloydTasksTileCap(2)) : 0u;  // 2 per CU
@@ -227,11 +229,12 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         ~PinnedFree() { (void)hipHostFree(p); }
     } pinnedGuard{hOut};
     UploadArena up(UploadArena::bytesFor({4 * nGmax, 4 * nGmax, sizeof(KmTile) * tilesMax, 4 * nGmax, 4 * nGmax, 4 * nGmax,
-                                          4 * (wavesMax + 1), sizeof(KmChunk) * chunksMax, 4 * nGmax, 4 * nGmax,
-                                          sizeof(KmChunk) * redMax, opts.seed_mode == 0 ? 4 * nGmax * b : 0}));
+                                          4 * (wavesMax + 1), sizeof(KmChunk) * chunksMax, sizeof(KmSlot) * slotsMax,
+                                          4 * nGmax, 4 * nGmax, sizeof(KmPartRange) * redMax,
+                                          opts.seed_mode == 0 ? 4 * nGmax * b : 0}));
 
     std::vector<uint32_t> taskTiles, tileBeg, smallTasks;
-    std::vector<KmChunk> red;
+    std::vector<KmPartRange> red;
     std::mt19937_64 rng(opts.seed);
     const uint32_t cus = deviceCUs();
 
@@ -311,19 +314,21 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             const KmPartition part = partitionKmTiles(taskTiles, tileBeg);
             red.clear();
             for (uint32_t g = 0; g < nG; g++)
-                if (part.taskChunkCnt[g] > 32)  // small tasks are reduced inside the update kernel
-                    for (uint32_t c = 0; c < part.taskChunkCnt[g]; c += 32)
-                        red.push_back(KmChunk{g, part.taskChunkBeg[g] + c,
-                                              part.taskChunkBeg[g] + std::min(part.taskChunkCnt[g], c + 32)});
-            RBOW_REQUIRE(part.chunks.size() <= chunksMax && red.size() <= redMax, RBOW_ERR_HIP,
-                         "k-means work partition exceeds its bound");
+                if (part.taskPartCnt[g] > 32)  // tasks with few partials are reduced inside the update kernel
+                    for (uint32_t c = 0; c < part.taskPartCnt[g]; c += 32)
+                        red.push_back(KmPartRange{g, part.taskPartBeg[g] + c,
+                                                  part.taskPartBeg[g] + std::min(part.taskPartCnt[g], c + 32)});
+            RBOW_REQUIRE(part.chunks.size() <= chunksMax && part.blockSlots.size() <= slotsMax && red.size() <= redMax,
+                         RBOW_ERR_HIP, "k-means work partition exceeds its bound");
             k.XA = dXA.get();
             k.waveChunkBeg = up.put(part.waveChunkBeg.data(), part.waveChunkBeg.size());
             k.nbWaves = part.nbWaves;
             k.chunks = up.put(part.chunks.data(), part.chunks.size());
             k.nbChunks = uint32_t(part.chunks.size());
-            k.taskChunkBeg = up.put(part.taskChunkBeg.data(), nG);
-            k.taskChunkCnt = up.put(part.taskChunkCnt.data(), nG);
+            k.blockSlots = up.put(part.blockSlots.data(), part.blockSlots.size());
+            k.nbParts = part.nbParts;
+            k.taskPartBeg = up.put(part.taskPartBeg.data(), nG);
+            k.taskPartCnt = up.put(part.taskPartCnt.data(), nG);
             k.partSums = dPartSums.get();
             k.partCnt = dPartCnt.get();
             k.partChanged = dPartChanged.get();
@@ -338,7 +343,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
                 k.nbUpdTasks = nbUpd;
             }
         }
-        const KmChunk* dRed = up.put(red.data(), red.size());
+        const KmPartRange* dRed = up.put(red.data(), red.size());
 
         // ---- seeds (kmeans.cpp:170-196 reference picks, or k-means++) ----
         const uint32_t* dSeedIdx = dSeed.get();

@@ -5,7 +5,12 @@
 // every streamed tile of every task is covered exactly once, chunk tile indices
 // are global and stay inside their task's tiles, a task's chunks are
 // contiguous and listed by taskChunkBeg/taskChunkCnt, and each wave gets an
-// equal share of the streamed tiles (to within one).
+// equal share of the streamed tiles (to within one).  Partials: chunks share
+// one exactly when they are consecutive chunks of one task in one workgroup
+// (then through an LDS slot of that workgroup, at most wavesPerBlock - 1 per
+// workgroup), partials are numbered in chunk order, and taskPartBeg/Cnt list
+// each task's partials.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -27,8 +32,60 @@ static int fails = 0;
         }                                                  \
     } while (0)
 
+static void checkParts(const KmPartition& p, uint32_t nbTasks, uint32_t wpb, int id) {
+    const uint32_t W = std::max(wpb, 1u), S = W - 1, nbBlocks = (p.nbWaves + W - 1) / W;
+    CHECK(p.wavesPerBlock == W, "case %d: wavesPerBlock", id);
+    CHECK(p.blockSlots.size() == size_t(nbBlocks) * S, "case %d: blockSlots size", id);
+    std::vector<uint32_t> blockOf(p.chunks.size(), 0u);
+    for (uint32_t w = 0; w < p.nbWaves; w++)
+        for (uint32_t c = p.waveChunkBeg[w]; c < p.waveChunkBeg[w + 1]; c++) blockOf[c] = w / W;
+    std::vector<uint32_t> slotUses(p.blockSlots.size(), 0u);
+    for (size_t c = 0; c < p.chunks.size(); c++) {
+        const KmChunk& k = p.chunks[c];
+        CHECK(k.part < p.nbParts, "case %d: chunk %zu part %u of %u", id, c, k.part, p.nbParts);
+        if (c == 0) CHECK(k.part == 0, "case %d: first part %u", id, k.part);
+        const bool sameRun = c > 0 && blockOf[c] == blockOf[c - 1] && p.chunks[c - 1].task == k.task;
+        if (c > 0) {
+            if (sameRun) CHECK(k.part == p.chunks[c - 1].part, "case %d: chunk %zu not merged with its run", id, c);
+            else CHECK(k.part == p.chunks[c - 1].part + 1, "case %d: chunk %zu part %u after %u", id, c, k.part,
+                       p.chunks[c - 1].part);
+        }
+        const bool runNext = c + 1 < p.chunks.size() && blockOf[c + 1] == blockOf[c] && p.chunks[c + 1].task == k.task;
+        if (sameRun || runNext) {
+            CHECK(k.slot < S, "case %d: merged chunk %zu slot %u", id, c, k.slot);
+            if (k.slot < S) {
+                const KmSlot& sl = p.blockSlots[size_t(blockOf[c]) * S + k.slot];
+                CHECK(sl.part == k.part && sl.task == k.task, "case %d: chunk %zu slot record", id, c);
+                slotUses[size_t(blockOf[c]) * S + k.slot]++;
+            }
+        } else {
+            CHECK(k.slot == kNoSlot, "case %d: lone chunk %zu has slot %u", id, c, k.slot);
+        }
+    }
+    if (!p.chunks.empty()) CHECK(p.chunks.back().part + 1 == p.nbParts, "case %d: nbParts %u", id, p.nbParts);
+    if (p.chunks.empty()) CHECK(p.nbParts == 0, "case %d: nbParts %u without chunks", id, p.nbParts);
+    for (size_t i = 0; i < p.blockSlots.size(); i++) {
+        const bool used = p.blockSlots[i].part != kNoSlot;
+        CHECK(used ? slotUses[i] >= 2 : (slotUses[i] == 0 && p.blockSlots[i].task == kNoSlot),
+              "case %d: slot %zu used by %u chunks", id, i, slotUses[i]);
+    }
+    if (W == 1) CHECK(p.nbParts == p.chunks.size(), "case %d: one wave per block merges nothing", id);
+    std::vector<uint32_t> partTask(p.nbParts, kNoSlot);
+    for (const KmChunk& k : p.chunks)
+        if (k.part < p.nbParts) partTask[k.part] = k.task;
+    for (uint32_t g = 0; g < nbTasks; g++) {
+        CHECK((p.taskPartCnt[g] > 0) == (p.taskChunkCnt[g] > 0), "case %d: task %u part count", id, g);
+        if (!p.taskPartCnt[g]) CHECK(p.taskPartBeg[g] == 0, "case %d: task %u part begin", id, g);
+        for (uint32_t x = p.taskPartBeg[g]; x < p.taskPartBeg[g] + p.taskPartCnt[g]; x++)
+            CHECK(x < p.nbParts && partTask[x] == g, "case %d: task %u part list", id, g);
+    }
+    uint64_t listed = 0;
+    for (uint32_t g = 0; g < nbTasks; g++) listed += p.taskPartCnt[g];
+    CHECK(listed == p.nbParts, "case %d: %llu parts listed of %u", id, (unsigned long long)listed, p.nbParts);
+}
+
 static void checkCase(const std::vector<uint32_t>& realTiles, const std::vector<uint8_t>& streamed, uint64_t maxWaves,
-                      bool passBase, int id) {
+                      bool passBase, int id, uint32_t wpb = 4) {
     const uint32_t n = uint32_t(realTiles.size());
     std::vector<uint32_t> taskTiles(n), base(n);
     uint32_t g0 = 0;
@@ -41,7 +98,7 @@ static void checkCase(const std::vector<uint32_t>& realTiles, const std::vector<
     }
     if (!passBase)  // prefix-sum numbering: only meaningful when every task streams
         for (uint32_t g = 0; g < n; g++) CHECK(streamed[g], "case %d: test setup", id);
-    const KmPartition p = partitionTiles(taskTiles, passBase ? base : std::vector<uint32_t>{}, maxWaves);
+    const KmPartition p = partitionTiles(taskTiles, passBase ? base : std::vector<uint32_t>{}, maxWaves, wpb);
     const uint64_t W = std::min<uint64_t>(std::max<uint64_t>(maxWaves, 1), T);
     CHECK(p.nbWaves == W, "case %d: nbWaves %u expected %llu", id, p.nbWaves, (unsigned long long)W);
     CHECK(p.waveChunkBeg.size() == size_t(p.nbWaves) + 1, "case %d: waveChunkBeg size", id);
@@ -78,6 +135,7 @@ static void checkCase(const std::vector<uint32_t>& realTiles, const std::vector<
             CHECK(c < p.chunks.size() && p.chunks[c].task == g, "case %d: task %u chunk list", id, g);
         if (!p.taskChunkCnt[g]) CHECK(p.taskChunkBeg[g] == 0, "case %d: task %u chunk begin", id, g);
     }
+    checkParts(p, n, wpb, id);
 }
 
 int main() {
@@ -87,11 +145,12 @@ int main() {
     const std::vector<std::vector<uint32_t>> shapes = {{5}, {1, 1, 1}, {0, 3, 0, 2, 0}, {40, 2, 2, 40}, {7, 0, 0, 9}};
     const std::vector<std::vector<uint8_t>> masks = {{1}, {1, 0, 1}, {1, 1, 1, 1, 1}, {1, 0, 0, 1}, {0, 1, 1, 1}};
     for (size_t i = 0; i < shapes.size(); i++)
-        for (uint64_t mw : {1ull, 2ull, 3ull, 64ull, 4096ull}) {
-            checkCase(shapes[i], masks[i], mw, true, cases++);
-            std::vector<uint8_t> all(shapes[i].size(), 1);
-            checkCase(shapes[i], all, mw, false, cases++);
-        }
+        for (uint64_t mw : {1ull, 2ull, 3ull, 5ull, 64ull, 4096ull})
+            for (uint32_t wpb : {1u, 2u, 4u, 8u}) {
+                checkCase(shapes[i], masks[i], mw, true, cases++, wpb);
+                std::vector<uint8_t> all(shapes[i].size(), 1);
+                checkCase(shapes[i], all, mw, false, cases++, wpb);
+            }
     // random levels: sizes from a heavy tail, about half the tasks not streamed
     for (int r = 0; r < 400; r++) {
         const uint32_t n = 1 + uint32_t(rng() % (r < 200 ? 40 : 4096));
@@ -103,9 +162,10 @@ int main() {
             st[g] = (tiles[g] > 32 || rng() % 2) ? 1 : 0;
         }
         const uint64_t mw = std::vector<uint64_t>{1, 5, 256, 1024, 8192}[rng() % 5];
-        checkCase(tiles, st, mw, true, cases++);
+        const uint32_t wpb = std::vector<uint32_t>{1, 2, 4, 4, 8}[rng() % 5];
+        checkCase(tiles, st, mw, true, cases++, wpb);
         std::vector<uint8_t> all(n, 1);
-        checkCase(tiles, all, mw, rng() % 2 == 0, cases++);
+        checkCase(tiles, all, mw, rng() % 2 == 0, cases++, wpb);
     }
     std::printf("%d cases, %d failures\n", cases, fails);
     return fails ? EXIT_FAILURE : EXIT_SUCCESS;
