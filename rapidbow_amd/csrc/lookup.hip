@@ -391,7 +391,10 @@ __global__ __launch_bounds__(256) void k_levelI8(const uint8_t* __restrict__ row
 }
 
 // ----------------------------------------------- split-BF16 level step --
-// mfma_f32_16x16x32_bf16: lane l feeds 8 dims [8*(l>>4) + 32*kb, +8), kb=0..3.
+// mfma_f32_16x16x32_bf16: lane l feeds 8 dims [8*(l>>4) + 32*kb, +8), kb=0..3,
+// of A row (l&15) and of B column (l&15).  The node's children are A (16 per
+// block cb) and 16 descriptor rows are B, so the result lane l holds children
+// cb*16 + 4*(l>>4) + i of row (l&15): each lane owns one row (filterBestT).
 template <bool kIdentity, bool kLast, int B>
 __global__ __launch_bounds__(256) void k_levelBf16(
     const float* __restrict__ rows32, const uint8_t* __restrict__ mu, const uint16_t* __restrict__ hi,
@@ -411,13 +414,12 @@ __global__ __launch_bounds__(256) void k_levelBf16(
     uint32_t ties = 0;
     for (uint32_t it = blockIdx.x * 4 + wave; it < nbItems; it += gridDim.x * 4) {
         const LkItem t = itemAt<kIdentity>(it, n, items);
-        const uint64_t childBase = (levelBase + t.node) * B;
+        const uint64_t node = levelBase + t.node, childBase = node * B;
         v8bf bh[CB][4], bl[CB][4];
-        float cn[CB], ea[CB], eb[CB];
+        float cn[CB][4];
         uint2 muw[4];  // the node's centre on this lane's dims
 #pragma unroll
-        for (int kb = 0; kb < 4; kb++)
-            muw[kb] = *reinterpret_cast<const uint2*>(mu + (levelBase + t.node) * kDims + kb * 32 + grp * 8);
+        for (int kb = 0; kb < 4; kb++) muw[kb] = *reinterpret_cast<const uint2*>(mu + node * kDims + kb * 32 + grp * 8);
 #pragma unroll
         for (int cb = 0; cb < CB; cb++) {
             const uint64_t child = childBase + cb * 16 + col;
@@ -426,9 +428,25 @@ __global__ __launch_bounds__(256) void k_levelBf16(
                 bh[cb][kb] = *reinterpret_cast<const v8bf*>(hi + child * kDims + kb * 32 + grp * 8);
                 bl[cb][kb] = *reinterpret_cast<const v8bf*>(lo + child * kDims + kb * 32 + grp * 8);
             }
-            cn[cb] = cn2[child];
-            ea[cb] = e1[child];
-            eb[cb] = e2[child];
+            const float4 c4 = *reinterpret_cast<const float4*>(cn2 + childBase + cb * 16 + 4 * grp);
+            cn[cb][0] = c4.x;
+            cn[cb][1] = c4.y;
+            cn[cb][2] = c4.z;
+            cn[cb][3] = c4.w;
+        }
+        // the node's largest bound coefficients (lane l reads child l % B)
+        float e1m = e1[childBase + lane % B], e2m = e2[childBase + lane % B];
+#pragma unroll
+        for (int d = B / 2; d >= 1; d >>= 1) {
+            e1m = fmaxf(e1m, __shfl_xor(e1m, d, 64));
+            e2m = fmaxf(e2m, __shfl_xor(e2m, d, 64));
+        }
+        // ||mu||^2 over this lane's dims (the rest of ||x - mu||^2 comes from byte dot products)
+        uint32_t mm = 0;
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++) {
+            mm = __builtin_amdgcn_udot4(muw[kb].x, muw[kb].x, mm, false);
+            mm = __builtin_amdgcn_udot4(muw[kb].y, muw[kb].y, mm, false);
         }
         // the descriptor bytes of row blocks rb+1 and rb+2, and the permutation
         // entries of blocks rb+3 and rb+4, are in flight while rb computes
@@ -444,11 +462,12 @@ __global__ __launch_bounds__(256) void k_levelBf16(
             }
         };
         uint2 wn[4], wn2[4];
-        uint32_t q2 = permAt(2), q3 = permAt(3);
-        loadRows(0, permAt(0), wn);
-        loadRows(1, permAt(1), wn2);
+        uint32_t qc = permAt(0), qn = permAt(1), q2 = permAt(2), q3 = permAt(3);
+        loadRows(0, qc, wn);
+        loadRows(1, qn, wn2);
         for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
             const uint32_t r = rb * 16 + col;
+            const bool valid = r < t.cnt;
             uint2 wc[4];
 #pragma unroll
             for (int kb = 0; kb < 4; kb++) {
@@ -456,59 +475,53 @@ __global__ __launch_bounds__(256) void k_levelBf16(
                 wn[kb] = wn2[kb];
             }
             if ((rb + 2) * 16 < t.cnt) loadRows(rb + 2, q2, wn2);
+            const uint32_t q = qc;  // this block's row (lane & 15)
+            qc = qn;
+            qn = q2;
             q2 = q3;
             if ((rb + 4) * 16 < t.cnt) q3 = permAt(rb + 4);
             v8bf ax[4];
-            float xsq = 0.0f;  // ||x - mu||^2 of this lane's dims: integers < 2^24, exact
-            if (r < t.cnt) {
+            uint32_t xx = 0, xm = 0;
+            if (valid) {
 #pragma unroll
                 for (int kb = 0; kb < 4; kb++) {
                     const uint2 w = wc[kb];
+                    xx = __builtin_amdgcn_udot4(w.x, w.x, xx, false);
+                    xx = __builtin_amdgcn_udot4(w.y, w.y, xx, false);
+                    xm = __builtin_amdgcn_udot4(w.x, muw[kb].x, xm, false);
+                    xm = __builtin_amdgcn_udot4(w.y, muw[kb].y, xm, false);
 #pragma unroll
                     for (int e = 0; e < 8; e++) {
                         const uint32_t xw = e < 4 ? w.x : w.y, mw = e < 4 ? muw[kb].x : muw[kb].y;
-                        // byte -> float conversions, the difference in [-255, 255] is exact in
-                        // float and in bf16 (packed two at a time)
+                        // the difference in [-255, 255] is exact in float and in bf16
                         const float dv = float((xw >> (8 * (e & 3))) & 0xFFu) - float((mw >> (8 * (e & 3))) & 0xFFu);
                         ax[kb][e] = __bf16(dv);
-                        xsq = fmaf(dv, dv, xsq);
                     }
                 }
             } else {
 #pragma unroll
                 for (int kb = 0; kb < 4; kb++) ax[kb] = v8bf{};
             }
-            xsq += __shfl_xor(xsq, 16, 64);
-            xsq += __shfl_xor(xsq, 32, 64);
+            // ||x - mu||^2 of row (lane & 15): exact integers (< 2^24 after the sum over the lanes)
+            int xs = valid ? int(xx) - 2 * int(xm) + int(mm) : 0;
+            xs += __shfl_xor(xs, 16, 64);
+            xs += __shfl_xor(xs, 32, 64);
             v4f acc[CB];
 #pragma unroll
             for (int cb = 0; cb < CB; cb++) {
                 acc[cb] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int kb = 0; kb < 4; kb++) {
-                    acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[kb], bh[cb][kb], acc[cb], 0, 0, 0);
-                    acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[kb], bl[cb][kb], acc[cb], 0, 0, 0);
+                    acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[cb][kb], ax[kb], acc[cb], 0, 0, 0);
+                    acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[cb][kb], ax[kb], acc[cb], 0, 0, 0);
                 }
             }
-            uint32_t best[4];
-            filterBest<CB>(
-                acc, xsq, cn, ea, eb, lane, [&](uint32_t rr) { return rb * 16 + rr < t.cnt; },
-                [&](uint32_t rr) {
-                    const uint32_t row = rb * 16 + rr;
-                    return desc + uint64_t(kIdentity ? t.start + row : perm[t.start + row]) * kDims;
-                },
-                [&](uint32_t c) { return rows32 + (childBase + c) * kDims; }, best, ties);
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint32_t row = rb * 16 + 4 * grp + i;
-                if (col == uint32_t(i) && row < t.cnt) {
-                    if (kLast) {
-                        const uint32_t q = kIdentity ? t.start + row : perm[t.start + row];
-                        out[q] = t.node * B + best[i];
-                    } else {
-                        sj[row] = uint8_t(best[i]);
-                    }
-                }
+            const uint32_t best = filterBestT<CB>(
+                acc, float(xs), cn, e1m, e2m, lane, valid, [&]() { return desc + uint64_t(q) * kDims; },
+                [&](uint32_t c) { return rows32 + (childBase + c) * kDims; }, ties);
+            if (grp == 0 && valid) {
+                if (kLast) out[q] = t.node * B + best;
+                else sj[r] = uint8_t(best);
             }
         }
         if (!kLast) itemCounts(lane, LG2B, it, t, sj, scAll[wave], out, C);

@@ -83,6 +83,37 @@ def test_lookup_f32_parity_and_near_ties(b, L):
     assert ties > 0
 
 
+@pytest.mark.parametrize("b", [32, 16])
+def test_lookup_f32_candidate_band(b):
+    """Siblings that differ from child 0 by perturbations from 1e-3 to 30 (in
+    norm) put the plain argmin inside, at and outside the filter's certified
+    band; queries sit on and between them.  MFMA leaf ids equal the plain
+    kernel's and the oracle's."""
+    rng = np.random.default_rng(11)
+    L = 2
+    desc = rb.synth_descriptors(4000, seed=9)
+    rows = _f32_tree(rng, b, L, desc, near_ties=False)
+    n = rows.shape[0]
+    scales = np.logspace(-3, 1.5, b - 1).astype(np.float32)
+    for v in range(n):
+        u = rng.normal(0, 1, (b - 1, 128)).astype(np.float32)
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        rows[v, 1:] = rows[v, 0] + scales[:, None] * u
+    voc = rb.deserialize_vocabulary(rbw2_blob(b, L, 1, rows, np.full(n, 0xFFFFFFFF), np.ones((n, b))))
+    # queries: rounded centroids and points between child 0 and each sibling
+    q = [np.clip(np.rint(rows[v, j]), 0, 255) for v in range(n) for j in range(b)]
+    for v in range(n):
+        for j in range(1, b):
+            t = rng.uniform(0.3, 0.7)
+            q.append(np.clip(np.rint((1 - t) * rows[v, 0] + t * rows[v, j] + rng.normal(0, 0.5, 128)), 0, 255))
+    q = np.array(q, np.uint8)
+    expect = oracle.lookup_f32(rows, b, L, q)
+    np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_PLAIN), expect)
+    mfma, ties = voc.lookup_host(q, rb.LOOKUP_MFMA, return_near_ties=True)
+    np.testing.assert_array_equal(mfma, expect)
+    print(f"b={b}: {q.shape[0]} queries, near ties re-scored: {ties}")
+
+
 def test_blob_roundtrip_u8():
     desc = rb.synth_descriptors(8000, seed=9)
     voc, res, blob = _u8_voc_from_oracle(desc, 2)
