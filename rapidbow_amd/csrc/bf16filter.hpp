@@ -13,8 +13,9 @@
 //   E_j = (E1 + 2^-16 (max(||x'||^2 + s_j, 0) + E1)) (1 + 2^-10),
 //   E1  = e1_j ||x'|| + e2_j,
 // e1/e2 from k_prepF32 (operand split, FP32 accumulation, stored norms) and the
-// 2^-16 term (>= 131 * 2^-24) the plain chain's own relative error.  Child j is
-// a candidate when s_j - E_j <= min_k (s_k + E_k); one candidate is the plain
+// 2^-16 term (>= 131 * 2^-24) the plain chain's own relative error.  A child
+// that can still be the argmin under these bounds is a candidate (filterBestT
+// tests a superset with one threshold per row); one candidate is the plain
 // argmin, several ("near ties") are re-scored with the plain chain in child
 // order, so the result equals the plain kernel's exactly (ties to the lowest).
 #pragma once
@@ -52,91 +53,7 @@ __device__ inline float plainF32Distance(const uint8_t* x, const float* c) {
     return a;
 }
 
-// One block of 16 rows.  acc[cb]: the MFMA scores' dot parts, lane l holding
-// rows 4*(l>>4)+i of child cb*16 + (l&15); xsq: ||x'||^2 of row (l & 15),
-// valid in every lane.  cn/ea/eb: ||c'||^2, e1, e2 of child cb*16 + (l&15).
-// rowValid(r) / rowPtr(r): row r (0..15) of the block exists / its 128 bytes;
-// childPtr(j): centroid j's float row.  Returns in best[i] the plain-chain
-// argmin of row 4*(l>>4)+i (ties to the lowest child) and adds the rows that
-// needed a re-score to ties (counted once per row).
-template <int CB, class RowValid, class RowPtr, class ChildPtr>
-__device__ __forceinline__ void filterBest(const v4f (&acc)[CB], float xsq, const float (&cn)[CB],
-                                           const float (&ea)[CB], const float (&eb)[CB], uint32_t lane,
-                                           RowValid rowValid, RowPtr rowPtr, ChildPtr childPtr, uint32_t (&best)[4],
-                                           uint32_t& ties) {
-    constexpr uint32_t B = 16 * CB;
-    const uint32_t col = lane & 15, grp = lane >> 4;
-    uint32_t cm[4];  // candidate children of rows 4*grp + i (same in all 16 lanes of the group)
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const uint32_t rowLocal = 4 * grp + i;
-        const float xn2 = __shfl(xsq, int(rowLocal), 64);  // ||x - mu||^2
-        const float xn = sqrtf(xn2) * (1.0f + 1.0f / 1048576.0f);
-        float sc[CB], E[CB];
-        float U = INFINITY;
-#pragma unroll
-        for (int cb = 0; cb < CB; cb++) {
-            sc[cb] = cn[cb] - 2.0f * acc[cb][i];
-            const float e1v = ea[cb] * xn + eb[cb];  // MFMA score vs exact distance
-            // plus the plain chain's relative error on the distance itself
-            E[cb] = (e1v + kGammaPlain * (fmaxf(xn2 + sc[cb], 0.0f) + e1v)) * (1.0f + 1.0f / 1024.0f);
-            U = fminf(U, sc[cb] + E[cb]);
-        }
-        U = min16f(U);
-        uint32_t m = 0;
-#pragma unroll
-        for (int cb = 0; cb < CB; cb++) {
-            const uint64_t bal = __ballot(sc[cb] - E[cb] <= U);
-            m |= uint32_t((bal >> (16 * grp)) & 0xFFFFu) << (16 * cb);
-        }
-        cm[i] = m;
-    }
-    // Near ties: the candidates of the 4 rows of a group are re-scored in
-    // parallel, lane 4*i + s taking candidate s (+4 per extra pass) of row i,
-    // each lane running the plain FP32 chain; a quad minimum picks the child.
-#pragma unroll
-    for (int i = 0; i < 4; i++) best[i] = __builtin_ctz(cm[i]);
-    const bool tieHere = __builtin_popcount(cm[0]) > 1 || __builtin_popcount(cm[1]) > 1 ||
-                         __builtin_popcount(cm[2]) > 1 || __builtin_popcount(cm[3]) > 1;
-    if (__ballot(tieHere)) {
-        const uint32_t i = col >> 2, slot = col & 3;
-        const uint32_t mine = i == 0 ? cm[0] : i == 1 ? cm[1] : i == 2 ? cm[2] : cm[3];
-        const uint32_t row = 4 * grp + i;
-        const bool rowTie = __builtin_popcount(mine) > 1 && rowValid(row);
-        float bd = INFINITY;
-        uint32_t bj = 0xFFFFFFFFu;
-        const uint32_t npass = (B + 3u) / 4u;
-        for (uint32_t pass = 0; pass < npass; pass++) {
-            const uint32_t need = __ballot(rowTie && uint32_t(__builtin_popcount(mine)) > 4 * pass) != 0;
-            if (!need) break;
-            // candidate number slot + 4*pass of this row
-            uint32_t mm = mine, k = slot + 4 * pass, c = 0xFFFFFFFFu;
-            for (; mm; mm &= mm - 1, k--)
-                if (k == 0) {
-                    c = __builtin_ctz(mm);
-                    break;
-                }
-            float a = INFINITY;
-            if (rowTie && c != 0xFFFFFFFFu) a = plainF32Distance(rowPtr(row), childPtr(c));
-            // keep (distance, child) minimum per quad; lowest child among equal distances
-            const float qa = min4f(a);
-            const uint32_t qj = min4u(a == qa && c != 0xFFFFFFFFu ? c : 0xFFFFFFFFu);
-            if (qa < bd || (qa == bd && qj < bj)) {
-                bd = qa;
-                bj = qj;
-            }
-        }
-        // lane 4*i of the group now holds row i's plain-order argmin
-#pragma unroll
-        for (int ii = 0; ii < 4; ii++) {
-            const uint32_t v = uint32_t(__shfl(int(bj), int(16 * grp + 4 * ii), 64));
-            if (__builtin_popcount(cm[ii]) > 1) best[ii] = v;
-        }
-        if (slot == 0 && rowTie) ties++;
-    }
-}
-
-// ---- transposed layout: children as the MFMA A operand, rows as B ----
+// ---- children as the MFMA A operand, rows as B ----
 //
 // With the node's children as A and 16 descriptor rows as B, the accumulator
 // of lane l holds child cb*16 + 4*(l>>4) + i of row (l & 15): every lane owns

@@ -59,22 +59,6 @@ __device__ inline uint32_t min16u(uint32_t v) {
     v = min(v, dppU<0x140>(v));  // row_mirror
     return v;
 }
-__device__ inline float min16f(float v) {
-    v = fminf(v, __uint_as_float(dppU<0xB1>(__float_as_uint(v))));
-    v = fminf(v, __uint_as_float(dppU<0x4E>(__float_as_uint(v))));
-    v = fminf(v, __uint_as_float(dppU<0x141>(__float_as_uint(v))));
-    v = fminf(v, __uint_as_float(dppU<0x140>(__float_as_uint(v))));
-    return v;
-}
-__device__ inline float min4f(float v) {  // within each quad
-    v = fminf(v, __uint_as_float(dppU<0xB1>(__float_as_uint(v))));
-    return fminf(v, __uint_as_float(dppU<0x4E>(__float_as_uint(v))));
-}
-__device__ inline uint32_t min4u(uint32_t v) {
-    v = min(v, dppU<0xB1>(v));
-    return min(v, dppU<0x4E>(v));
-}
-
 
 // Owning device allocation.
 template <typename T>

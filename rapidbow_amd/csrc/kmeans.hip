@@ -776,10 +776,12 @@ __global__ __launch_bounds__(256) void k_assignF32Plain(KmLevel k) {
 // ---------------------------------------- assign: float, BF16 MFMA filter --
 // Same results as k_assignF32Plain (the plain FP32 chain, ties to the lowest
 // center): per 16-row block, x' = x - mu (exact in BF16) against the task's
-// split centroids (k_prepF32 of this pass's centers) by 2 x 4 x B/16 MFMAs, the
-// certified candidate filter, and the plain chain for near ties (filterBest).
-// Work split and epilogue as the plain kernel: each wave a contiguous tile
-// range, the task operands reloaded when the task changes, exact sums in LDS.
+// split centroids (k_prepF32 of this pass's centers) by 2 x 4 x B/16 MFMAs with
+// the centroids as the A operand (each lane then owns one row), the certified
+// one-threshold-per-row candidate filter, and the plain chain for near ties
+// (filterBestT).  Work split and epilogue as the plain kernel: each wave a
+// contiguous tile range, the task operands reloaded when the task changes,
+// exact sums in LDS; the next block's rows are in flight while one computes.
 template <int B>
 __global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
     constexpr int CB = B / 16;
@@ -794,8 +796,9 @@ __global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
     waveRange(k.nbTiles, beg, end);
     uint32_t curTask = kIdxInvalid, changedLocal = 0, ties = 0;
     v8bf bh[CB][4], bl[CB][4];
-    float cn[CB], ea[CB], eb[CB];
-    uint2 muw[4];  // the task's centre on this lane's dims
+    float cn[CB][4], e1m = 0.0f, e2m = 0.0f;
+    uint2 muw[4];     // the task's centre on this lane's dims
+    uint32_t mm = 0;  // ||mu||^2 over this lane's dims
     for (uint32_t ti = beg; ti < end; ti++) {
         const KmTile t = k.tiles[ti];
         if (!k.active[t.task]) continue;
@@ -804,9 +807,13 @@ __global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
             changedLocal = 0;
             curTask = t.task;
             const uint64_t childBase = uint64_t(curTask) * B;
+            mm = 0;
 #pragma unroll
-            for (int kb = 0; kb < 4; kb++)
+            for (int kb = 0; kb < 4; kb++) {
                 muw[kb] = *reinterpret_cast<const uint2*>(k.fMu + uint64_t(curTask) * kDims + kb * 32 + grp * 8);
+                mm = __builtin_amdgcn_udot4(muw[kb].x, muw[kb].x, mm, false);
+                mm = __builtin_amdgcn_udot4(muw[kb].y, muw[kb].y, mm, false);
+            }
 #pragma unroll
             for (int cb = 0; cb < CB; cb++) {
                 const uint64_t child = childBase + cb * 16 + col;
@@ -815,53 +822,77 @@ __global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
                     bh[cb][kb] = *reinterpret_cast<const v8bf*>(k.fHi + child * kDims + kb * 32 + grp * 8);
                     bl[cb][kb] = *reinterpret_cast<const v8bf*>(k.fLo + child * kDims + kb * 32 + grp * 8);
                 }
-                cn[cb] = k.fCn2[child];
-                ea[cb] = k.fE1[child];
-                eb[cb] = k.fE2[child];
+                const float4 c4 = *reinterpret_cast<const float4*>(k.fCn2 + childBase + cb * 16 + 4 * grp);
+                cn[cb][0] = c4.x;
+                cn[cb][1] = c4.y;
+                cn[cb][2] = c4.z;
+                cn[cb][3] = c4.w;
+            }
+            e1m = k.fE1[childBase + lane % B];
+            e2m = k.fE2[childBase + lane % B];
+#pragma unroll
+            for (int d = B / 2; d >= 1; d >>= 1) {
+                e1m = fmaxf(e1m, __shfl_xor(e1m, d, 64));
+                e2m = fmaxf(e2m, __shfl_xor(e2m, d, 64));
             }
         }
         const float* centers = k.centersF32 + uint64_t(curTask) * B * kDims;
+        auto loadRows = [&](uint32_t rb, uint2 (&w)[4]) {
+            if (rb * 16 + col < t.cnt) {
+                const uint8_t* x = k.X + uint64_t(t.row + rb * 16 + col) * kDims;
+#pragma unroll
+                for (int kb = 0; kb < 4; kb++) w[kb] = *reinterpret_cast<const uint2*>(x + kb * 32 + grp * 8);
+            }
+        };
+        uint2 wn[4];
+        loadRows(0, wn);
         for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
             const uint32_t r = rb * 16 + col;
+            const bool valid = r < t.cnt;
+            uint2 wc[4];
+#pragma unroll
+            for (int kb = 0; kb < 4; kb++) wc[kb] = wn[kb];
+            if ((rb + 1) * 16 < t.cnt) loadRows(rb + 1, wn);
             v8bf ax[4];
-            float xsq = 0.0f;  // ||x - mu||^2 of this lane's dims: integers < 2^24, exact
-            if (r < t.cnt) {
-                const uint8_t* x = k.X + uint64_t(t.row + r) * kDims;
+            uint32_t xx = 0, xm = 0;
+            if (valid) {
 #pragma unroll
                 for (int kb = 0; kb < 4; kb++) {
-                    const uint2 w = *reinterpret_cast<const uint2*>(x + kb * 32 + grp * 8);
+                    const uint2 w = wc[kb];
+                    xx = __builtin_amdgcn_udot4(w.x, w.x, xx, false);
+                    xx = __builtin_amdgcn_udot4(w.y, w.y, xx, false);
+                    xm = __builtin_amdgcn_udot4(w.x, muw[kb].x, xm, false);
+                    xm = __builtin_amdgcn_udot4(w.y, muw[kb].y, xm, false);
 #pragma unroll
                     for (int e = 0; e < 8; e++) {
                         const uint32_t xw = e < 4 ? w.x : w.y, mw = e < 4 ? muw[kb].x : muw[kb].y;
                         const float dv = float((xw >> (8 * (e & 3))) & 0xFFu) - float((mw >> (8 * (e & 3))) & 0xFFu);
                         ax[kb][e] = __bf16(dv);
-                        xsq = fmaf(dv, dv, xsq);
                     }
                 }
             } else {
 #pragma unroll
                 for (int kb = 0; kb < 4; kb++) ax[kb] = v8bf{};
             }
-            xsq += __shfl_xor(xsq, 16, 64);
-            xsq += __shfl_xor(xsq, 32, 64);
+            // ||x - mu||^2 of row (lane & 15): exact integers
+            int xs = valid ? int(xx) - 2 * int(xm) + int(mm) : 0;
+            xs += __shfl_xor(xs, 16, 64);
+            xs += __shfl_xor(xs, 32, 64);
             v4f d[CB];
 #pragma unroll
             for (int cb = 0; cb < CB; cb++) {
                 d[cb] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int kb = 0; kb < 4; kb++) {
-                    d[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[kb], bh[cb][kb], d[cb], 0, 0, 0);
-                    d[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[kb], bl[cb][kb], d[cb], 0, 0, 0);
+                    d[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[cb][kb], ax[kb], d[cb], 0, 0, 0);
+                    d[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[cb][kb], ax[kb], d[cb], 0, 0, 0);
                 }
             }
-            uint32_t best[4];
-            filterBest<CB>(
-                d, xsq, cn, ea, eb, lane, [&](uint32_t rr) { return rb * 16 + rr < t.cnt; },
-                [&](uint32_t rr) { return k.X + uint64_t(t.row + rb * 16 + rr) * kDims; },
-                [&](uint32_t c) { return centers + uint64_t(c) * kDims; }, best, ties);
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                if (col == uint32_t(i)) sBest[wave][rb * 16 + 4 * grp + i] = uint8_t(best[i]);
+            const uint32_t best = filterBestT<CB>(
+                d, float(xs), cn, e1m, e2m, lane, valid,
+                [&]() { return k.X + uint64_t(t.row + r) * kDims; },
+                [&](uint32_t c) { return centers + uint64_t(c) * kDims; }, ties);
+            if (grp == 0) sBest[wave][r] = uint8_t(best);
         }
         waveSync();
         const uint32_t myA = sBest[wave][lane];  // row `lane` (meaningful for lane < cnt)
