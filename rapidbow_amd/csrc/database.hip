@@ -47,6 +47,12 @@ struct QNode {
     uint64_t a, e;   // posting slice
 };
 
+// Orders LDS accesses of one wave (fence + wave barrier).
+__device__ inline void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 __device__ inline double blockSumF64(double v, double* red) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -57,6 +63,34 @@ __device__ inline double blockSumF64(double v, double* red) {
     double s = 0;
     for (uint32_t w = 0; w < blockDim.x / 64; w++) s += red[w];
     return s;
+}
+
+// Block-wide scan of one value per thread pair (1024 threads): returns the
+// exclusive prefix of this thread's value; *total gets the block total.
+// `red` holds 16 wave totals.
+__device__ inline uint32_t blockScan(uint32_t val, uint32_t* red, uint32_t* total) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t s = val;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = uint32_t(__shfl_up(int(s), d, 64));
+        if (lane >= uint32_t(d)) s += t;
+    }
+    __syncthreads();  // red is free
+    if (lane == 63) red[wave] = s;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t w = lane < 16 ? red[lane] : 0u;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const uint32_t t = uint32_t(__shfl_up(int(w), d, 64));
+            if (lane >= uint32_t(d)) w += t;
+        }
+        if (lane < 16) red[lane] = w;
+    }
+    __syncthreads();
+    *total = red[15];
+    return (wave ? red[wave - 1] : 0u) + s - val;
 }
 
 // ------------------------------------------------------------- norms ---
@@ -113,14 +147,18 @@ __global__ void k_fill64(uint64_t* p, uint64_t v, uint64_t n) {
 }
 
 // ------------------------------------------------------------- query ---
-// One 1024-thread block per query: sort the leaves, norm, scored node list.
+// One 1024-thread block per query (DataBase.cpp:71-130): sort the leaves in
+// LDS, binary64 L1 norm, then per level the distinct nodes with their
+// multiplicities: every leaf, and inner nodes under the posting threshold.
 __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __restrict__ leaf,
                                                    const uint64_t* __restrict__ off, uint32_t K,
                                                    QNode* __restrict__ nodes, uint32_t* __restrict__ nodeCount) {
     __shared__ uint32_t s[kSortCap];
+    __shared__ uint16_t sStart[kSortCap + 1];  // run starts of one level (positions in s), then n
     __shared__ double red[16];
+    __shared__ uint32_t redU[16];
     __shared__ uint32_t nNodes, nInner;
-    const uint32_t q = blockIdx.x, tid = threadIdx.x;
+    const uint32_t q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t beg = off[q];
     const uint32_t n = uint32_t(off[q + 1] - beg);
     uint32_t n2 = 1;
@@ -131,23 +169,33 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
         nInner = 0;
     }
     __syncthreads();
-    // bitonic sort, ascending
-    for (uint32_t k = 2; k <= n2; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = tid; i < n2; i += 1024) {
-                const uint32_t p = i ^ j;
-                if (p > i) {
-                    const uint32_t a = s[i], c = s[p];
-                    const bool up = (i & k) == 0;
-                    if ((a > c) == up) {
-                        s[i] = c;
-                        s[p] = a;
-                    }
-                }
+    // bitonic sort, ascending.  Wave w owns the segment [w*seg, (w+1)*seg): a
+    // step whose partners i ^ j lie in one segment (j < seg) runs wave-locally;
+    // only the steps across segments take a block barrier.
+    const uint32_t seg = n2 >= 16 ? n2 / 16 : 1;
+    auto cmpx = [&](uint32_t i, uint32_t j, uint32_t k) {
+        const uint32_t p = i ^ j;
+        if (p > i) {
+            const uint32_t a = s[i], c = s[p];
+            if ((a > c) == ((i & k) == 0)) {
+                s[i] = c;
+                s[p] = a;
             }
-            __syncthreads();
+        }
+    };
+    for (uint32_t k = 2; k <= n2; k <<= 1) {
+        if ((k >> 1) >= seg) __syncthreads();  // the wave-local steps before are visible block-wide
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            if (j >= seg) {
+                for (uint32_t i = tid; i < n2; i += 1024) cmpx(i, j, k);
+                __syncthreads();
+            } else {
+                for (uint32_t i = wave * seg + lane; i < (wave + 1) * seg; i += 64) cmpx(i, j, k);
+                waveSync();
+            }
         }
     }
+    __syncthreads();
     double part = 0;
     for (uint32_t t = 0; t < v.L; t++) {
         const float* W = v.W + levelBeg(v.lg2b, v.L - 1 - t) * v.b;
@@ -158,39 +206,61 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
     const uint64_t thr = 16ull + 4ull * K;    // DataBase.h:60-61
     // leaves at out[0, nLeaf), inner nodes at out[n, n + nInner)
     QNode* out = nodes + uint64_t(v.L) * beg;
+    // per level: the runs of equal node ids (leaf >> shift) in the sorted leaves;
+    // thread tid scans the contiguous chunk [c0, c1) for run starts
+    const uint32_t E = (n + 1023) / 1024, c0 = min(n, tid * E), c1 = min(n, c0 + E);
     for (uint32_t t = 0; t < v.L; t++) {
         const uint32_t shift = t * v.lg2b;
         const float* W = v.W + levelBeg(v.lg2b, v.L - 1 - t) * v.b;
-        for (uint32_t i = tid; i < n; i += 1024) {
-            const uint32_t x = s[i] >> shift;
-            if (i != 0 && (s[i - 1] >> shift) == x) continue;
-            // run end: first index whose node differs (binary search in the sorted leaves)
-            const uint64_t nextLeaf = (uint64_t(x) + 1) << shift;
-            uint32_t lo = i, hi = n;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) / 2;
-                if (uint64_t(s[mid]) < nextLeaf) lo = mid + 1;
-                else hi = mid;
+        auto isStart = [&](uint32_t i) { return i == 0 || (s[i - 1] >> shift) != (s[i] >> shift); };
+        uint32_t mine = 0;
+        for (uint32_t i = c0; i < c1; i++) mine += isStart(i);
+        uint32_t nRuns;
+        uint32_t pos = blockScan(mine, redU, &nRuns);
+        for (uint32_t i = c0; i < c1; i++)
+            if (isStart(i)) sStart[pos++] = uint16_t(i);
+        if (tid == 0) sStart[nRuns] = uint16_t(n);
+        __syncthreads();
+        // runs tid, tid + 1024, ... four at a time: their posting-offset loads are in flight together
+        for (uint32_t k0 = tid; k0 < nRuns; k0 += 4 * 1024) {
+            uint32_t x[4], cnt[4];
+            uint64_t a[4], e[4];
+            float w[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t k = k0 + u * 1024;
+                x[u] = 0;
+                cnt[u] = 0;
+                a[u] = 0;
+                e[u] = 0;
+                w[u] = 0.0f;
+                if (k < nRuns) {
+                    const uint32_t st = sStart[k];
+                    cnt[u] = uint32_t(sStart[k + 1]) - st;
+                    x[u] = s[st] >> shift;
+                    a[u] = v.leafOff[uint64_t(x[u]) << shift];
+                    e[u] = v.leafOff[(uint64_t(x[u]) + 1) << shift];
+                    w[u] = W[x[u]];
+                }
             }
-            const uint32_t cnt = lo - i;
-            const uint64_t a = v.leafOff[uint64_t(x) << shift], e = v.leafOff[nextLeaf];
-            if (t == 0 || e - a < thr) {
-                const float w = W[x];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (k0 + u * 1024 >= nRuns || (t != 0 && e[u] - a[u] >= thr)) continue;
                 QNode nd;
-                nd.v = x;
+                nd.v = x[u];
                 nd.t = t;
-                nd.cnt = cnt;
-                nd.qE = fmulz(fmulz(float(cnt), w), qrcp);  // DataBase.cpp:117
-                nd.w = w;
+                nd.cnt = cnt[u];
+                nd.qE = fmulz(fmulz(float(cnt[u]), w[u]), qrcp);  // DataBase.cpp:117
+                nd.w = w[u];
                 nd.pad = 0;
-                nd.a = a;
-                nd.e = e;
+                nd.a = a[u];
+                nd.e = e[u];
                 if (t == 0) out[atomicAdd(&nNodes, 1u)] = nd;
                 else out[n + atomicAdd(&nInner, 1u)] = nd;
             }
         }
+        __syncthreads();  // sStart is rewritten by the next level
     }
-    __syncthreads();
     if (tid == 0) {
         nodeCount[2 * q] = nNodes;
         nodeCount[2 * q + 1] = nInner;
@@ -230,51 +300,34 @@ __device__ inline float keyScore(uint64_t k) {
     return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
 }
 
-// Block-wide scan of one value per thread pair (1024 threads): returns the
-// exclusive prefix of this thread's value; *total gets the block total.
-// `red` holds 16 wave totals.
-__device__ inline uint32_t blockScan(uint32_t val, uint32_t* red, uint32_t* total) {
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t s = val;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = uint32_t(__shfl_up(int(s), d, 64));
-        if (lane >= uint32_t(d)) s += t;
-    }
-    __syncthreads();  // red is free
-    if (lane == 63) red[wave] = s;
-    __syncthreads();
-    if (wave == 0) {
-        uint32_t w = lane < 16 ? red[lane] : 0u;
-#pragma unroll
-        for (int d = 1; d < 16; d <<= 1) {
-            const uint32_t t = uint32_t(__shfl_up(int(w), d, 64));
-            if (lane >= uint32_t(d)) w += t;
-        }
-        if (lane < 16) red[lane] = w;
-    }
-    __syncthreads();
-    *total = red[15];
-    return (wave ? red[wave - 1] : 0u) + s - val;
-}
-
 // Exact selection, by 8-bit MSB-first radix passes, of the `need` smallest of
-// the unique 64-bit keys given by keyAt(i, key) for i < n (keyAt returns false
-// for empty positions); returns the largest selected key, so the selection is
-// every key <= it.  Requires 1 <= need <= #keys.  hist: 256 words, red: 2 words.
-template <class KeyAt>
-__device__ uint64_t blockSelectKth(KeyAt keyAt, uint32_t n, uint32_t need, uint32_t* hist, uint32_t* red) {
+// a block's unique 64-bit keys; returns the largest selected key, so the
+// selection is every key <= it.  Requires 1 <= need <= #keys.  forEach(f)
+// calls f(key, valid) over each thread's share of the keys, wave-uniformly (a
+// lane with nothing at a step passes valid = false).  Lanes of a wave whose
+// keys fall in one bin add to the histogram once (the first passes put nearly
+// every key in the same bin), and the passes stop as soon as the bin holding
+// the need-th key is selected whole.  hist: 256 words, red: 3 words.
+template <class ForEach>
+__device__ uint64_t blockSelectKth(ForEach forEach, uint32_t need, uint32_t* hist, uint32_t* red) {
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     uint64_t prefix = 0;
     for (int shift = 56; shift >= 0; shift -= 8) {
         __syncthreads();
         if (tid < 256) hist[tid] = 0;
         __syncthreads();
-        for (uint32_t i = tid; i < n; i += blockDim.x) {
-            uint64_t k;
-            if (!keyAt(i, k)) continue;
-            if (shift == 56 || (k >> (shift + 8)) == (prefix >> (shift + 8))) atomicAdd(&hist[(k >> shift) & 255u], 1u);
-        }
+        forEach([&](uint64_t k, bool valid) {
+            const bool in = valid && (shift == 56 || (k >> (shift + 8)) == (prefix >> (shift + 8)));
+            const uint64_t act = __ballot(in);
+            if (!act) return;
+            const uint32_t bin = uint32_t(k >> shift) & 255u;
+            const uint32_t first = uint32_t(__builtin_ctzll(act));
+            const uint32_t fb = uint32_t(__shfl(int(bin), int(first), 64));
+            const bool same = in && bin == fb;
+            const uint64_t sm = __ballot(same);
+            if (lane == first) atomicAdd(&hist[fb], uint32_t(__builtin_popcountll(sm)));
+            else if (in && !same) atomicAdd(&hist[bin], 1u);
+        });
         __syncthreads();
         if (tid < 64) {  // wave 0: the bin holding the need-th key
             const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2], h3 = hist[4 * lane + 3];
@@ -288,21 +341,26 @@ __device__ uint64_t blockSelectKth(KeyAt keyAt, uint32_t n, uint32_t need, uint3
             const uint32_t excl = incl - sum;
             if (excl < need && need <= incl) {
                 const uint32_t hs[4] = {h0, h1, h2, h3};
-                uint32_t before = excl, bin = 4 * lane + 3;
-                for (uint32_t j = 0; j < 4; j++) {
-                    if (before + hs[j] >= need) {
-                        bin = 4 * lane + j;
+                uint32_t before = excl, j = 3;
+                for (uint32_t jj = 0; jj < 4; jj++) {
+                    if (before + hs[jj] >= need) {
+                        j = jj;
                         break;
                     }
-                    before += hs[j];
+                    before += hs[jj];
                 }
-                red[0] = bin;
+                red[0] = 4 * lane + j;
                 red[1] = need - before;
+                red[2] = hs[j];
             }
         }
         __syncthreads();
         prefix |= uint64_t(red[0]) << shift;
         need = red[1];
+        if (need == red[2]) {  // every key of the bin is selected: the lower digits do not matter
+            __syncthreads();
+            return prefix | ((1ull << shift) - 1ull);
+        }
     }
     __syncthreads();
     return prefix;
@@ -522,8 +580,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
         return;
     }
     const uint64_t kth = need == nT ? ~0ull - 1
-                                    : blockSelectKth([&](uint32_t i, uint64_t& k) { k = sKey[i]; return true; }, S,
-                                                     need, sHist, sRed);
+                                    : blockSelectKth(
+                                          [&](auto f) {
+                                              for (uint32_t i = tid; i < S; i += 1024) f(sKey[i], true);  // S = 4 x 1024
+                                          },
+                                          need, sHist, sRed);
     __syncthreads();
     if (tid == 0) sCount = 0;
     __syncthreads();
@@ -541,7 +602,7 @@ __global__ __launch_bounds__(1024) void k_topk(DbView v, uint32_t K, const uint6
                                               float* __restrict__ outScores, uint32_t* __restrict__ outCount) {
     __shared__ uint64_t sel[Database::kMaxResults];
     __shared__ uint32_t hist[256], red[16], nSel;
-    const uint32_t q = blockIdx.x, tid = threadIdx.x;
+    const uint32_t q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t* keys = candKeys + uint64_t(q) * v.R * K;
     const uint32_t* cnt = candCount + uint64_t(q) * v.R;
     uint32_t mine = 0;
@@ -553,19 +614,22 @@ __global__ __launch_bounds__(1024) void k_topk(DbView v, uint32_t K, const uint6
         if (tid == 0) outCount[q] = 0;
         return;
     }
-    auto keyAt = [&](uint32_t i, uint64_t& key) {  // i over [0, R*K): range i / K, rank i % K
-        const uint32_t rr = i / K;
-        if (i - rr * K >= cnt[rr]) return false;
-        key = keys[i];
-        return true;
+    // the candidates: range rr holds cnt[rr] keys at keys[rr*K ...]; one wave per range at a time
+    auto forEach = [&](auto f) {
+        for (uint32_t rr = wave; rr < v.R; rr += 16) {
+            const uint32_t c = cnt[rr];
+            for (uint32_t r0 = 0; r0 < c; r0 += 64) {
+                const bool valid = r0 + lane < c;
+                f(valid ? keys[uint64_t(rr) * K + r0 + lane] : 0ull, valid);
+            }
+        }
     };
-    const uint64_t kth = need == tot ? ~0ull - 1 : blockSelectKth(keyAt, v.R * K, need, hist, red);
+    const uint64_t kth = need == tot ? ~0ull - 1 : blockSelectKth(forEach, need, hist, red);
     if (tid == 0) nSel = 0;
     __syncthreads();
-    for (uint32_t i = tid; i < v.R * K; i += 1024) {
-        uint64_t key;
-        if (keyAt(i, key) && key <= kth) sel[atomicAdd(&nSel, 1u)] = key;
-    }
+    forEach([&](uint64_t key, bool valid) {
+        if (valid && key <= kth) sel[atomicAdd(&nSel, 1u)] = key;
+    });
     __syncthreads();
     uint32_t n2 = 1;
     while (n2 < need) n2 <<= 1;
