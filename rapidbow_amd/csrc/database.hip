@@ -47,12 +47,6 @@ struct QNode {
     uint64_t a, e;   // posting slice
 };
 
-// Orders LDS accesses of one wave (fence + wave barrier).
-__device__ inline void waveSync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
 __device__ inline double blockSumF64(double v, double* red) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -146,6 +140,57 @@ __global__ void k_fill64(uint64_t* p, uint64_t v, uint64_t n) {
         p[i] = v;
 }
 
+// Bitonic sort, ascending, of s[0, n2), n2 = 1024 * E, by a 1024-thread block
+// with E consecutive keys per thread in registers: steps whose partners i ^ j
+// are in the same thread (j < E) swap registers, steps within one wave
+// (j < 64 E) exchange with the partner lane, and only steps across waves pass
+// through LDS (two barriers each).  s is complete and visible on return.
+template <int E>
+__device__ void bitonicSortBlock(uint32_t* s, uint32_t n2) {
+    const uint32_t tid = threadIdx.x, i0 = tid * E;
+    uint32_t r[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) r[e] = s[i0 + e];
+    for (uint32_t k = 2; k <= n2; k <<= 1) {
+        uint32_t j = k >> 1;
+        for (; j >= uint32_t(64 * E); j >>= 1) {  // across waves
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; e++) s[i0 + e] = r[e];
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const uint32_t i = i0 + e, o = s[i ^ j];
+                r[e] = ((i & j) == 0) == ((i & k) == 0) ? min(r[e], o) : max(r[e], o);
+            }
+        }
+        for (; j >= uint32_t(E); j >>= 1) {  // across lanes of one wave
+            const int lm = int(j / E);
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const uint32_t i = i0 + e, o = uint32_t(__shfl_xor(int(r[e]), lm, 64));
+                r[e] = ((i & j) == 0) == ((i & k) == 0) ? min(r[e], o) : max(r[e], o);
+            }
+        }
+#pragma unroll
+        for (int jj = E / 2; jj > 0; jj >>= 1) {  // within the thread
+            if (uint32_t(jj) > (k >> 1)) continue;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                if (e & jj) continue;
+                const uint32_t a = r[e], c = r[e | jj];
+                const bool up = ((i0 + e) & k) == 0;
+                r[e] = up ? min(a, c) : max(a, c);
+                r[e | jj] = up ? max(a, c) : min(a, c);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; e++) s[i0 + e] = r[e];
+    __syncthreads();
+}
+
 // ------------------------------------------------------------- query ---
 // One 1024-thread block per query (DataBase.cpp:71-130): sort the leaves in
 // LDS, binary64 L1 norm, then per level the distinct nodes with their
@@ -158,10 +203,10 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
     __shared__ double red[16];
     __shared__ uint32_t redU[16];
     __shared__ uint32_t nNodes, nInner;
-    const uint32_t q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t q = blockIdx.x, tid = threadIdx.x;
     const uint64_t beg = off[q];
     const uint32_t n = uint32_t(off[q + 1] - beg);
-    uint32_t n2 = 1;
+    uint32_t n2 = 1024;
     while (n2 < n) n2 <<= 1;
     for (uint32_t i = tid; i < n2; i += 1024) s[i] = i < n ? leaf[beg + i] : 0xFFFFFFFFu;
     if (tid == 0) {
@@ -169,33 +214,14 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
         nInner = 0;
     }
     __syncthreads();
-    // bitonic sort, ascending.  Wave w owns the segment [w*seg, (w+1)*seg): a
-    // step whose partners i ^ j lie in one segment (j < seg) runs wave-locally;
-    // only the steps across segments take a block barrier.
-    const uint32_t seg = n2 >= 16 ? n2 / 16 : 1;
-    auto cmpx = [&](uint32_t i, uint32_t j, uint32_t k) {
-        const uint32_t p = i ^ j;
-        if (p > i) {
-            const uint32_t a = s[i], c = s[p];
-            if ((a > c) == ((i & k) == 0)) {
-                s[i] = c;
-                s[p] = a;
-            }
-        }
-    };
-    for (uint32_t k = 2; k <= n2; k <<= 1) {
-        if ((k >> 1) >= seg) __syncthreads();  // the wave-local steps before are visible block-wide
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            if (j >= seg) {
-                for (uint32_t i = tid; i < n2; i += 1024) cmpx(i, j, k);
-                __syncthreads();
-            } else {
-                for (uint32_t i = wave * seg + lane; i < (wave + 1) * seg; i += 64) cmpx(i, j, k);
-                waveSync();
-            }
-        }
+    // bitonic sort, ascending, of n2 >= 1024 keys (padded with 0xFFFFFFFF)
+    switch (n2 >> 10) {
+    case 1: bitonicSortBlock<1>(s, n2); break;
+    case 2: bitonicSortBlock<2>(s, n2); break;
+    case 4: bitonicSortBlock<4>(s, n2); break;
+    case 8: bitonicSortBlock<8>(s, n2); break;
+    default: bitonicSortBlock<16>(s, n2); break;
     }
-    __syncthreads();
     double part = 0;
     for (uint32_t t = 0; t < v.L; t++) {
         const float* W = v.W + levelBeg(v.lg2b, v.L - 1 - t) * v.b;
