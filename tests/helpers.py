@@ -35,3 +35,19 @@ def brute_descent(rows, b, L, desc, exact_int=True):
         d = ((x[:, None, :] - c) ** 2).sum(-1)
         node = node * b + np.argmin(d, axis=1)  # argmin returns the first minimum
     return node.astype(np.uint32)
+
+
+# |GPU float score - reference-semantics float score|: both accumulate in binary64,
+# the GPU with atomics in any order
+SCORE_TOL = 2e-6
+
+
+def assert_ranked_equal(gd, gs, od, os_, tol=SCORE_TOL):
+    """Ranked ids must be identical; the only allowed difference is an order swap between
+    neighbours whose scores differ by less than tol."""
+    assert len(gd) == len(od)
+    np.testing.assert_allclose(gs, os_, atol=tol, rtol=0)
+    if not np.array_equal(gd, od):
+        for i in np.nonzero(gd != od)[0]:
+            j = int(np.nonzero(od == gd[i])[0][0]) if gd[i] in od else -1
+            assert j >= 0 and abs(float(os_[j]) - float(gs[i])) < tol, (i, gd[i], od[i])

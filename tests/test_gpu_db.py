@@ -6,11 +6,10 @@ import pytest
 
 import oracle
 import rapidbow_amd as rb
-from tests.helpers import level_beg, ref_blob, rbw2_blob
+from tests.helpers import SCORE_TOL, assert_ranked_equal, level_beg, ref_blob, rbw2_blob
 
 pytestmark = pytest.mark.gpu
 
-SCORE_TOL = 2e-6  # |GPU float score - reference-semantics float score|; both accumulate in binary64
 
 
 class GpuDb:
@@ -99,17 +98,6 @@ def _random_docs(rng, nb, nleaves, lo=1, hi=300, hot=None):
     return docs
 
 
-def _assert_ranked_equal(gd, gs, od, os_, tol=SCORE_TOL):
-    """Ranked ids must be identical; the only allowed difference is an order swap between
-    neighbours whose scores differ by less than tol (reported)."""
-    assert len(gd) == len(od)
-    np.testing.assert_allclose(gs, os_, atol=tol, rtol=0)
-    if not np.array_equal(gd, od):
-        for i in np.nonzero(gd != od)[0]:
-            j = int(np.nonzero(od == gd[i])[0][0]) if gd[i] in od else -1
-            assert j >= 0 and abs(float(os_[j]) - float(gs[i])) < tol, (i, gd[i], od[i])
-
-
 @pytest.mark.parametrize("b,L", [(16, 2), (16, 3), (32, 2)])
 def test_db_reference_api_parity(b, L):
     voc, w = _voc_and_weights(L, b, seed=L + b)
@@ -129,7 +117,7 @@ def test_db_reference_api_parity(b, L):
         q = docs[qi]
         gd, gs = g.query(q, K)
         od, os_ = o.query(q, K)
-        _assert_ranked_equal(gd, gs, od, os_)
+        assert_ranked_equal(gd, gs, od, os_)
         assert gd[0] == qi * 7 + 3  # self-query first (ApiTests.cpp:136)
     # a query sharing nothing with the DB, and an empty query
     gd, _ = g.query(np.array([], np.uint32), 10)
@@ -188,7 +176,7 @@ def test_db_batch_add_and_batch_query():
     res = g.query_batch(qcat, qoff, 50)
     for q, (gd, gs) in zip(qs, res):
         od, os_ = o.query(q, 50)
-        _assert_ranked_equal(gd, gs, od, os_)
+        assert_ranked_equal(gd, gs, od, os_)
     # each result is the same alone as in the batch
     for i in (0, 7, 63):
         gd, gs = g.query(qs[i], 50)
@@ -255,11 +243,11 @@ def test_db_many_document_ranges():
         for K in ((60, 7, 300, 1024) if qi % 4 == 0 else (60, 7, 300)):
             gd, gs = g.query(q, K)
             od, os_ = o.query(q, K)
-            _assert_ranked_equal(gd, gs, od, os_)
+            assert_ranked_equal(gd, gs, od, os_)
     res = g.query_batch(np.concatenate(qs), np.concatenate([[0], np.cumsum([x.size for x in qs])]).astype(np.uint64), 60)
     for q, (gd, gs) in zip(qs, res):
         od, os_ = o.query(q, 60)
-        _assert_ranked_equal(gd, gs, od, os_)
+        assert_ranked_equal(gd, gs, od, os_)
 
 
 def test_db_subnormal_weights_read_as_zero():
@@ -281,4 +269,4 @@ def test_db_subnormal_weights_read_as_zero():
     for qi in (0, 5, 77):
         gd, gs = g.query(docs[qi], 40)
         od, os_ = o.query(docs[qi], 40)
-        _assert_ranked_equal(gd, gs, od, os_)
+        assert_ranked_equal(gd, gs, od, os_)

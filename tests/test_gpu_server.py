@@ -6,11 +6,10 @@ import pytest
 
 import oracle
 import rapidbow_amd as rb
-from tests.helpers import level_beg, ref_blob, rbw2_blob
+from tests.helpers import SCORE_TOL, assert_ranked_equal, level_beg, ref_blob, rbw2_blob
 
 pytestmark = pytest.mark.gpu
 
-SCORE_TOL = 2e-6  # float scores from binary64 accumulators; atomics may add in any order
 
 
 def _tree(b, L, seed):
@@ -56,15 +55,6 @@ def _queries(images, rng, nq):
     return qs
 
 
-def _assert_ranked_equal(gd, gs, od, os_, tol=SCORE_TOL):
-    assert len(gd) == len(od)
-    np.testing.assert_allclose(gs, os_, atol=tol, rtol=0)
-    if not np.array_equal(gd, od):
-        for i in np.nonzero(gd != od)[0]:
-            j = int(np.nonzero(od == gd[i])[0][0]) if gd[i] in od else -1
-            assert j >= 0 and abs(float(os_[j]) - float(gs[i])) < tol, (i, gd[i], od[i])
-
-
 @pytest.mark.parametrize("b,L,kernel", [(16, 3, rb.LOOKUP_MFMA), (16, 3, rb.LOOKUP_PLAIN), (32, 2, rb.LOOKUP_MFMA)])
 def test_server_alone_vs_mix_vs_oracle(b, L, kernel):
     voc, db, odb, ref_leaf, images = _setup(b, L, seed=b + L)
@@ -81,10 +71,10 @@ def test_server_alone_vs_mix_vs_oracle(b, L, kernel):
     for i, q in enumerate(qs):
         ad, as_, _ = alone_srv.wait(alone_srv.submit(q))
         md, ms, lat = mix[i]
-        _assert_ranked_equal(md, ms, ad, as_)
+        assert_ranked_equal(md, ms, ad, as_)
         assert lat >= 0
         od, os_ = odb.query(ref_leaf(q), K)
-        _assert_ranked_equal(md, ms, od, os_)
+        assert_ranked_equal(md, ms, od, os_)
         if q.shape[0] == 0:
             assert md.size == 0
     alone_srv.close()
@@ -106,7 +96,7 @@ def test_server_staging_ring_wraps_and_waits():
     for i, q in enumerate(qs):
         od, os_ = odb.query(ref_leaf(q), K)
         md, ms, _ = mix[i]
-        _assert_ranked_equal(md, ms, od, os_)
+        assert_ranked_equal(md, ms, od, os_)
     with pytest.raises(rb.RbowError) as e:
         rb.Server(voc, db, staging_bytes=(2 << 20) - 1)
     assert e.value.status == rb.ERR_INVALID_ARG
@@ -131,7 +121,7 @@ def test_server_replay_poisson():
         if not np.array_equal(gd, od):  # only swaps of neighbours within the score tolerance
             gd2, gs2 = db.query(ref_leaf(q), K)
             np.testing.assert_array_equal(np.sort(gd), np.sort(od))
-            _assert_ranked_equal(gd2, gs2, od, os_)
+            assert_ranked_equal(gd2, gs2, od, os_)
 
 
 def test_server_errors():
