@@ -72,12 +72,15 @@ private:
         bool busy = false;
         std::vector<Request> reqs;
         std::vector<uint64_t> hOff;
-        uint32_t* hDocs = nullptr;  // pinned results, max_batch x K
-        float* hScores = nullptr;
-        uint32_t* hCounts = nullptr;
+        // pinned results of one batch of B queries, one copy from the device:
+        // docs [B][K] u32, scores [B][K] f32, counts [B] u32
+        uint8_t* hOut = nullptr;
         uint64_t bytes = 0;
         double startMs = 0;
     };
+    // the worker spins this long for a new request before it blocks (a thread
+    // wake-up costs tens of microseconds)
+    static constexpr int kIdleSpinUs = 500;
     void loop();
     void launch(Slot& s, std::vector<Request>&& batch);
     void reap(Slot& s);
@@ -98,6 +101,7 @@ private:
     uint64_t spanBase_ = 0;       // index of spans_.front()
     std::unordered_map<uint64_t, Result> done_;
     uint64_t nextTicket_ = 1;
+    std::atomic<uint32_t> nPending_{0};  // pending_.size(), read by the idle worker without the lock
     uint64_t inFlightBytes_ = 0;
     bool stop_ = false;
     std::string error_;
