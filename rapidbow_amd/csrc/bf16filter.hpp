@@ -14,10 +14,11 @@
 //   E1  = e1_j ||x'|| + e2_j,
 // e1/e2 from k_prepF32 (operand split, FP32 accumulation, stored norms) and the
 // 2^-16 term (>= 131 * 2^-24) the plain chain's own relative error.  A child
-// that can still be the argmin under these bounds is a candidate (filterBestT
-// tests a superset with one threshold per row); one candidate is the plain
-// argmin, several ("near ties") are re-scored with the plain chain in child
-// order, so the result equals the plain kernel's exactly (ties to the lowest).
+// that can still be the argmin under these bounds is a candidate
+// (filterCandidates tests a superset with one threshold per row); one
+// candidate is the plain argmin, several ("near ties") are re-scored with the
+// plain chain in child order (resolveTies), so the result equals the plain
+// kernel's exactly (ties to the lowest).
 #pragma once
 #include "common.hpp"
 
@@ -87,45 +88,16 @@ __device__ inline uint32_t orAcrossGroups(uint32_t v) {
     const auto q = __builtin_amdgcn_permlane32_swap(v, v, false, false);
     return q[0] | q[1];
 }
-// Lexicographic minimum of (distance, child) over lanes l, l ^ 16, l ^ 32, l ^ 48.
-__device__ inline void argminAcrossGroups(float& d, uint32_t& j) {
-    auto step = [&](float od, uint32_t oj) {
-        if (od < d || (od == d && oj < j)) {
-            d = od;
-            j = oj;
-        }
-    };
-    {
-        const auto pd = __builtin_amdgcn_permlane16_swap(__float_as_uint(d), __float_as_uint(d), false, false);
-        const auto pj = __builtin_amdgcn_permlane16_swap(j, j, false, false);
-        // lane l now sees both its own value and the swapped partner's in [0] / [1]
-        const float d0 = __uint_as_float(pd[0]), d1 = __uint_as_float(pd[1]);
-        d = d0;
-        j = pj[0];
-        step(d1, pj[1]);
-    }
-    {
-        const auto pd = __builtin_amdgcn_permlane32_swap(__float_as_uint(d), __float_as_uint(d), false, false);
-        const auto pj = __builtin_amdgcn_permlane32_swap(j, j, false, false);
-        const float d0 = __uint_as_float(pd[0]), d1 = __uint_as_float(pd[1]);
-        d = d0;
-        j = pj[0];
-        step(d1, pj[1]);
-    }
-}
-
 // One block of 16 rows in the transposed layout.  acc[cb][i]: dot part of
 // child cb*16 + 4*(lane>>4) + i with row (lane & 15); xn2: ||x'||^2 of that row
 // (an exact integer); cn[cb][i]: ||c'||^2 of the same children; e1max, e2max:
-// the node's largest e1 / e2.  rowValid / rowPtr: the lane's row exists / its
-// 128 bytes; childPtr(j): centroid j's float row.  Returns the plain-chain
-// argmin of the lane's row (ties to the lowest child), the same in the row's
-// four lanes, and counts a row that needed the plain chain in ties (once, in
-// its group-0 lane).
-template <int CB, class RowPtr, class ChildPtr>
-__device__ __forceinline__ uint32_t filterBestT(const v4f (&acc)[CB], float xn2, const float (&cn)[CB][4],
-                                                float e1max, float e2max, uint32_t lane, bool rowValid,
-                                                RowPtr rowPtr, ChildPtr childPtr, uint32_t& ties) {
+// the node's largest e1 / e2; rowValid: the lane's row exists.  Returns the
+// candidate mask of the lane's row (bit j: child j), the same in the row's four
+// lanes.  A single bit is the plain-chain argmin; a row with several
+// candidates (a near tie) is resolved by resolveTies.
+template <int CB>
+__device__ __forceinline__ uint32_t filterCandidates(const v4f (&acc)[CB], float xn2, const float (&cn)[CB][4],
+                                                     float e1max, float e2max, uint32_t lane, bool rowValid) {
     const uint32_t grp = lane >> 4;
     float s[CB][4];
     float m = INFINITY;
@@ -148,30 +120,81 @@ __device__ __forceinline__ uint32_t filterBestT(const v4f (&acc)[CB], float xn2,
         for (int i = 0; i < 4; i++)
             if (s[cb][i] - m <= tau) mask |= 1u << (cb * 16 + 4 * grp + i);
     mask = orAcrossGroups(mask);
-    uint32_t best = __builtin_ctz(mask);
-    const bool multi = rowValid && (mask & (mask - 1u)) != 0;
-    if (__ballot(multi)) {
-        // the row's four lanes take candidates grp, grp + 4, ... in turn
+    return rowValid ? mask : 1u;
+}
+
+// Near ties of one wave, queued in LDS and resolved together so that the
+// loads of many rows overlap (a tie resolved where it appears stalls the wave
+// on one row's dependent chain).  Entry e: tag[e] (the descriptor row), aux[e]
+// (the caller's handle for the result) and mask[e] (the row's candidates).
+// Capacity kTieCap; a caller pushes one block (<= 16 rows) at a time and
+// flushes once more than kTieCap - 16 wait.
+constexpr uint32_t kTieCap = 64;
+struct TieQueue {
+    uint32_t tag[kTieCap], aux[kTieCap], mask[kTieCap];
+};
+
+// Appends the rows of this block with several candidates: the group-0 lane of
+// a row with tie == true adds (tag, aux, mask).  n: the wave's queue length
+// (uniform).  Returns the number of rows added.
+__device__ inline uint32_t tiePush(TieQueue& tq, uint32_t& n, uint32_t lane, bool tie, uint32_t tag, uint32_t aux,
+                                   uint32_t mask) {
+    const uint64_t b = __ballot(tie && (lane >> 4) == 0);
+    if (tie && (lane >> 4) == 0) {
+        const uint32_t pos = n + __builtin_amdgcn_mbcnt_hi(uint32_t(b >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(b), 0u));
+        tq.tag[pos] = tag;
+        tq.aux[pos] = aux;
+        tq.mask[pos] = mask;
+    }
+    const uint32_t added = uint32_t(__builtin_popcountll(b));
+    n += added;
+    return added;
+}
+
+// Resolves the n queued rows: each row's candidates are re-scored with the
+// plain FP32 chain, four lanes per row taking every fourth candidate, 16 rows
+// per round; done(aux, best) runs in the row's first lane with the plain
+// argmin (ties to the lowest child).  rowPtr(tag): the row's 128 bytes;
+// childPtr(j): centroid j's float row.  Leaves n = 0.
+template <class RowPtr, class ChildPtr, class Done>
+__device__ __forceinline__ void resolveTies(const TieQueue& tq, uint32_t& n, uint32_t lane, RowPtr rowPtr,
+                                            ChildPtr childPtr, Done done) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the queue writes of this wave are visible
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t r0 = 0; r0 < n; r0 += 16) {
+        const uint32_t e = r0 + (lane >> 2), sub = lane & 3;
+        const bool live = e < n;
+        const uint32_t mask = live ? tq.mask[e] : 0u, tag = live ? tq.tag[e] : 0u;
         const uint32_t nc = __builtin_popcount(mask);
         float bd = INFINITY;
         uint32_t bj = 0xFFFFFFFFu;
-        for (uint32_t k = grp; __ballot(multi && k < nc); k += 4) {
-            if (multi && k < nc) {
+        for (uint32_t k = sub; __ballot(live && k < nc); k += 4) {
+            if (live && k < nc) {
                 uint32_t mm = mask;
                 for (uint32_t r = 0; r < k; r++) mm &= mm - 1u;
                 const uint32_t c = __builtin_ctz(mm);
-                const float a = plainF32Distance(rowPtr(), childPtr(c));
+                const float a = plainF32Distance(rowPtr(tag), childPtr(c));
                 if (a < bd || (a == bd && c < bj)) {
                     bd = a;
                     bj = c;
                 }
             }
         }
-        argminAcrossGroups(bd, bj);
-        if (multi) best = bj;
-        if (multi && grp == 0) ties++;
+        // lexicographic (distance, child) minimum over the row's four lanes
+#pragma unroll
+        for (int d = 1; d <= 2; d <<= 1) {
+            const float od = __shfl_xor(bd, d, 64);
+            const uint32_t oj = uint32_t(__shfl_xor(int(bj), d, 64));
+            if (od < bd || (od == bd && oj < bj)) {
+                bd = od;
+                bj = oj;
+            }
+        }
+        if (live && sub == 0) done(tq.aux[e], bj);
     }
-    return best;
+    n = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // results and the emptied queue are visible
+    __builtin_amdgcn_wave_barrier();
 }
 
 }  // namespace rbow_amd

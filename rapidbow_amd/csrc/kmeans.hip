@@ -778,8 +778,8 @@ __global__ __launch_bounds__(256) void k_assignF32Plain(KmLevel k) {
 // center): per 16-row block, x' = x - mu (exact in BF16) against the task's
 // split centroids (k_prepF32 of this pass's centers) by 2 x 4 x B/16 MFMAs with
 // the centroids as the A operand (each lane then owns one row), the certified
-// one-threshold-per-row candidate filter, and the plain chain for near ties
-// (filterBestT).  Work split and epilogue as the plain kernel: each wave a
+// one-threshold-per-row candidate filter (filterCandidates), and the plain
+// chain for the tile's near ties, resolved together (resolveTies).  Work split and epilogue as the plain kernel: each wave a
 // contiguous tile range, the task operands reloaded when the task changes,
 // exact sums in LDS; the next block's rows are in flight while one computes.
 template <int B>
@@ -788,13 +788,15 @@ __global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
     __shared__ uint32_t sSums[kWaves][B * kDims];
     __shared__ uint32_t sCnt[kWaves][B];
     __shared__ uint8_t sBest[kWaves][64];  // a tile's assignments, one byte per row
+    __shared__ TieQueue tqAll[kWaves];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 15, grp = lane >> 4;
     WaveAcc<B> acc{sSums[wave], sCnt[wave]};
     acc.clear(lane);
+    TieQueue& tq = tqAll[wave];
     uint32_t beg, end;
     waveRange(k.nbTiles, beg, end);
-    uint32_t curTask = kIdxInvalid, changedLocal = 0, ties = 0;
+    uint32_t curTask = kIdxInvalid, changedLocal = 0, ties = 0, nTie = 0;
     v8bf bh[CB][4], bl[CB][4];
     float cn[CB][4], e1m = 0.0f, e2m = 0.0f;
     uint2 muw[4];     // the task's centre on this lane's dims
@@ -888,19 +890,23 @@ __global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
                     d[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[cb][kb], ax[kb], d[cb], 0, 0, 0);
                 }
             }
-            const uint32_t best = filterBestT<CB>(
-                d, float(xs), cn, e1m, e2m, lane, valid,
-                [&]() { return k.X + uint64_t(t.row + r) * kDims; },
-                [&](uint32_t c) { return centers + uint64_t(c) * kDims; }, ties);
-            if (grp == 0) sBest[wave][r] = uint8_t(best);
+            const uint32_t mask = filterCandidates<CB>(d, float(xs), cn, e1m, e2m, lane, valid);
+            const bool tie = (mask & (mask - 1u)) != 0;  // only valid rows can have several candidates
+            if (grp == 0 && !tie) sBest[wave][r] = uint8_t(__builtin_ctz(mask));
+            ties += tiePush(tq, nTie, lane, tie, t.row + r, r, mask);  // a tile's <= 64 rows fit the queue
         }
+        if (nTie)  // the tile's near ties: the plain chain over their candidates
+            resolveTies(
+                tq, nTie, lane, [&](uint32_t row) { return k.X + uint64_t(row) * kDims; },
+                [&](uint32_t c) { return centers + uint64_t(c) * kDims; },
+                [&](uint32_t r, uint32_t best) { sBest[wave][r] = uint8_t(best); });
         waveSync();
         const uint32_t myA = sBest[wave][lane];  // row `lane` (meaningful for lane < cnt)
         waveSync();
         changedLocal += tileEpilogue<B>(k, t, lane, myA, acc);
     }
     if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
-    if (k.nearTies && ties) atomicAdd(k.nearTies, ties);
+    if (k.nearTies && ties && lane == 0) atomicAdd(k.nearTies, ties);  // ties: rows of this wave (uniform)
 }
 
 // ------------------------------------------------------------ update -----

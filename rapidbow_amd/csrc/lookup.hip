@@ -394,7 +394,8 @@ __global__ __launch_bounds__(256) void k_levelI8(const uint8_t* __restrict__ row
 // mfma_f32_16x16x32_bf16: lane l feeds 8 dims [8*(l>>4) + 32*kb, +8), kb=0..3,
 // of A row (l&15) and of B column (l&15).  The node's children are A (16 per
 // block cb) and 16 descriptor rows are B, so the result lane l holds children
-// cb*16 + 4*(l>>4) + i of row (l&15): each lane owns one row (filterBestT).
+// cb*16 + 4*(l>>4) + i of row (l&15): each lane owns one row (filterCandidates);
+// near ties are queued per wave and resolved at the item's end (resolveTies).
 template <bool kIdentity, bool kLast, int B>
 __global__ __launch_bounds__(256) void k_levelBf16(
     const float* __restrict__ rows32, const uint8_t* __restrict__ mu, const uint16_t* __restrict__ hi,
@@ -407,14 +408,26 @@ __global__ __launch_bounds__(256) void k_levelBf16(
     constexpr uint32_t LG2B = B == 32 ? 5 : 4;
     __shared__ uint8_t sjAll[4][kTQ];
     __shared__ uint32_t scAll[4][32];
+    __shared__ TieQueue tqAll[4];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 15, grp = lane >> 4;
     uint8_t* sj = sjAll[wave];
+    TieQueue& tq = tqAll[wave];
     const uint32_t nbItems = kIdentity ? nbItemsImplicit : *nbItemsPtr;
-    uint32_t ties = 0;
+    uint32_t ties = 0, nTie = 0;
     for (uint32_t it = blockIdx.x * 4 + wave; it < nbItems; it += gridDim.x * 4) {
         const LkItem t = itemAt<kIdentity>(it, n, items);
         const uint64_t node = levelBase + t.node, childBase = node * B;
+        // near ties of this item: the plain chain over the queued rows' candidates
+        auto flushTies = [&]() {
+            resolveTies(
+                tq, nTie, lane, [&](uint32_t q) { return desc + uint64_t(q) * kDims; },
+                [&](uint32_t c) { return rows32 + (childBase + c) * kDims; },
+                [&](uint32_t aux, uint32_t best) {
+                    if (kLast) out[aux] = t.node * B + best;  // aux: the descriptor
+                    else sj[aux] = uint8_t(best);             // aux: the row in the item
+                });
+        };
         v8bf bh[CB][4], bl[CB][4];
         float cn[CB][4];
         uint2 muw[4];  // the node's centre on this lane's dims
@@ -516,17 +529,20 @@ __global__ __launch_bounds__(256) void k_levelBf16(
                     acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[cb][kb], ax[kb], acc[cb], 0, 0, 0);
                 }
             }
-            const uint32_t best = filterBestT<CB>(
-                acc, float(xs), cn, e1m, e2m, lane, valid, [&]() { return desc + uint64_t(q) * kDims; },
-                [&](uint32_t c) { return rows32 + (childBase + c) * kDims; }, ties);
-            if (grp == 0 && valid) {
+            const uint32_t mask = filterCandidates<CB>(acc, float(xs), cn, e1m, e2m, lane, valid);
+            const bool tie = (mask & (mask - 1u)) != 0;  // only valid rows can have several candidates
+            if (grp == 0 && valid && !tie) {
+                const uint32_t best = __builtin_ctz(mask);
                 if (kLast) out[q] = t.node * B + best;
                 else sj[r] = uint8_t(best);
             }
+            ties += tiePush(tq, nTie, lane, tie, q, kLast ? q : r, mask);
+            if (nTie > kTieCap - 16) flushTies();
         }
+        if (nTie) flushTies();
         if (!kLast) itemCounts(lane, LG2B, it, t, sj, scAll[wave], out, C);
     }
-    if (nearTies && ties) atomicAdd(nearTies, ties);
+    if (nearTies && ties && lane == 0) atomicAdd(nearTies, ties);  // ties: rows of this wave (uniform)
 }
 
 uint32_t gridFor(uint64_t work, uint32_t perBlock, uint32_t cap) {
