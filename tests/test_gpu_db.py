@@ -124,6 +124,31 @@ def test_db_reference_api_parity(b, L):
     assert gd.size == 0
 
 
+def test_db_query_sizes_up_to_the_maximum():
+    """Queries of 1 to 16,384 leaves: every size class of the query preparation's
+    register sort (1,024 x E keys, E = 1..16), one past each power of two and the
+    maximum, at K = 50 and K = 1,024, against the oracle; 16,385 leaves are rejected."""
+    voc, w = _voc_and_weights(3, 16, seed=12)
+    rng = np.random.default_rng(13)
+    nleaves = 16 ** 3
+    docs = _random_docs(rng, 300, nleaves, lo=50, hi=2000, hot=rng.integers(0, nleaves, 16))
+    g = GpuDb(voc)
+    o = oracle.DataBase(16, 3, w)
+    for d, lf in docs.items():
+        g.add(d, lf)
+        o.add(d, lf)
+    for n in (1, 1023, 1024, 1025, 2048, 4097, 8192, 8193, 16384):
+        q = np.concatenate([docs[int(i)] for i in rng.integers(0, 300, 40)])
+        q = np.resize(q, n).astype(np.uint32)  # duplicates included
+        for K in (50, 1024):
+            gd, gs = g.query(q, K)
+            od, os_ = o.query(q, K)
+            assert_ranked_equal(gd, gs, od, os_)
+    with pytest.raises(rb.RbowError) as e:
+        g.query(np.zeros(16385, np.uint32), 10)
+    assert e.value.status == rb.ERR_CAPACITY
+
+
 def test_db_errors_and_query_and_add():
     voc, w = _voc_and_weights(2, 16, seed=3)
     g = GpuDb(voc)
