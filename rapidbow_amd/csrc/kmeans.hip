@@ -532,6 +532,21 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
     }
 }
 
+// uint8(roundf(float(s) / float(c))), the update rule (kmeans.cu:367), for
+// 0 < c < 2^16 and 0 <= s <= 255 c: the quotient q = s / c lies in [0, 255],
+// where float's half-ulp is at most 2^-17, and unless q is exactly k + 1/2 it
+// is at least 1/(2c) > 2^-17 away from every k + 1/2, so the float division
+// never rounds q onto or across a half and roundf gives floor(q + 1/2) =
+// floor((2s + c) / (2c)).  That is computed with a reciprocal estimate and one
+// integer correction (the estimate is within one of the floor).
+__device__ inline uint32_t roundedMean(uint32_t s, uint32_t c) {
+    const uint32_t num = 2 * s + c, den = 2 * c;
+    uint32_t q = uint32_t(float(num) * __builtin_amdgcn_rcpf(float(den)));
+    if (q * den > num) q--;
+    else if ((q + 1) * den <= num) q++;
+    return q;
+}
+
 // Persistent Lloyd loop with the points on chip, one workgroup of W waves per
 // listed task (tasks of at most maxTiles 64-row tiles).  The task's tiles are
 // read from HBM once into LDS, row-major (dynamic: maxTiles x 9 KB, then one
@@ -620,7 +635,7 @@ __global__ __launch_bounds__(64 * W) void k_lloydTasks(KmLevel k, const uint32_t
             const int32_t sv = sSum[e];
             sSum[e] = 0;  // this thread's own elements: cleared for the next pass
             const uint32_t c = sCnt[par][e / kDims];
-            if (!stop && c) sCent[e] = uint8_t(roundf(float((long long)sv + 128ll * c) / float(c)));
+            if (!stop && c) sCent[e] = uint8_t(roundedMean(uint32_t(sv + 128 * int32_t(c)), c));
         }
         if (tid < 16) sCnt[par ^ 1][tid] = 0;  // counters of the next pass
         if (tid == 0) sChanged[par ^ 1] = 0;

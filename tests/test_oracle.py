@@ -133,6 +133,27 @@ def test_ref_update_known_answer(nb_desc):
     np.testing.assert_array_equal(upd, expect)
 
 
+def test_update_rounding_identity():
+    """uint8(roundf(float(s)/float(c))) (kmeans.cu:367, IEEE binary32 division, half away
+    from zero) equals floor((2s + c) / (2c)) for 0 < c < 2^16 and 0 <= s <= 255 c: the
+    persistent on-chip Lloyd kernel updates its centers with the integer form.  Every s
+    for c <= 64, and for every c < 2^16 the sums next to each k + 1/2 boundary."""
+    def ref(sv, cv):
+        q = (sv.astype(np.float32) / cv.astype(np.float32)).astype(np.float64)  # binary32 quotient
+        return np.floor(q + 0.5)  # roundf for q >= 0 (q + 0.5 is exact in binary64)
+
+    for c in range(1, 65):
+        sv = np.arange(0, 255 * c + 1, dtype=np.int64)
+        cv = np.full_like(sv, c)
+        np.testing.assert_array_equal((2 * sv + cv) // (2 * cv), ref(sv, cv))
+    cv = np.arange(1, 1 << 16, dtype=np.int64)
+    for k in range(0, 255):
+        mid = (2 * k + 1) * cv // 2  # floor((k + 1/2) c)
+        for d in (-1, 0, 1, 2):
+            sv = np.clip(mid + d, 0, 255 * cv)
+            np.testing.assert_array_equal((2 * sv + cv) // (2 * cv), ref(sv, cv), err_msg=f"k={k} d={d}")
+
+
 @pytest.mark.parametrize("tile_size", [32, 16])  # ReformatTests.cpp:105-107,115-117
 def test_ref_reformat_known_answer(tile_size):
     desc, idx = oracle.ref_reformat_test_inputs(136, 3, tile_size)
