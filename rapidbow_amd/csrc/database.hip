@@ -20,6 +20,7 @@ namespace {
 
 constexpr uint32_t kSortCap = Database::kMaxQueryDesc;  // leaves per query sorted in LDS
 constexpr uint32_t kSegBatch = 1024;                    // query leaves staged per scoring round (one per thread)
+constexpr uint32_t kTopkRegs = 16;                      // candidate keys per thread kept in registers by k_topk
 
 __device__ inline float ftz(float x) { return fabsf(x) < 1.17549435e-38f ? copysignf(0.0f, x) : x; }
 __device__ inline float fmulz(float a, float b) { return ftz(ftz(a) * ftz(b)); }
@@ -365,19 +366,23 @@ __device__ uint64_t blockSelectKth(ForEach forEach, uint32_t need, uint32_t* his
                 if (lane >= uint32_t(d)) incl += t;
             }
             const uint32_t excl = incl - sum;
-            if (excl < need && need <= incl) {
-                const uint32_t hs[4] = {h0, h1, h2, h3};
-                uint32_t before = excl, j = 3;
-                for (uint32_t jj = 0; jj < 4; jj++) {
-                    if (before + hs[jj] >= need) {
-                        j = jj;
-                        break;
-                    }
-                    before += hs[jj];
+            if (excl < need && need <= incl) {  // one of this lane's four bins (selects, no indexed array)
+                uint32_t before = excl, j = 3, hj = h3;
+                if (before + h0 >= need) {
+                    j = 0;
+                    hj = h0;
+                } else if (before += h0, before + h1 >= need) {
+                    j = 1;
+                    hj = h1;
+                } else if (before += h1, before + h2 >= need) {
+                    j = 2;
+                    hj = h2;
+                } else {
+                    before += h2;
                 }
                 red[0] = 4 * lane + j;
                 red[1] = need - before;
-                red[2] = hs[j];
+                red[2] = hj;
             }
         }
         __syncthreads();
@@ -640,8 +645,30 @@ __global__ __launch_bounds__(1024) void k_topk(DbView v, uint32_t K, const uint6
         if (tid == 0) outCount[q] = 0;
         return;
     }
-    // the candidates: range rr holds cnt[rr] keys at keys[rr*K ...]; one wave per range at a time
+    // the candidates: range rr holds cnt[rr] keys at keys[rr*K ...].  Wave w takes
+    // ranges w, w + 16, ..., lane l positions l, l + 64, ... of each.  When that is
+    // at most kTopkRegs keys per thread they are loaded once, all loads in flight
+    // together, and the selection passes run on registers; otherwise every pass
+    // walks the ranges in memory.
+    const uint32_t nR = (v.R + 15) / 16, nK = (K + 63) / 64;
+    uint64_t kr[kTopkRegs];
+    uint32_t kv = 0;  // bit i: kr[i] holds a key
+    const bool inRegs = nR * nK <= kTopkRegs;
+    if (inRegs) {
+#pragma unroll
+        for (uint32_t i = 0; i < kTopkRegs; i++) {
+            const uint32_t rr = wave + 16 * (i / nK), pos = lane + 64 * (i % nK);
+            const bool valid = i < nR * nK && rr < v.R && pos < cnt[rr];
+            kr[i] = valid ? keys[uint64_t(rr) * K + pos] : 0ull;
+            kv |= uint32_t(valid) << i;
+        }
+    }
     auto forEach = [&](auto f) {
+        if (inRegs) {
+#pragma unroll
+            for (uint32_t i = 0; i < kTopkRegs; i++) f(kr[i], ((kv >> i) & 1u) != 0);
+            return;
+        }
         for (uint32_t rr = wave; rr < v.R; rr += 16) {
             const uint32_t c = cnt[rr];
             for (uint32_t r0 = 0; r0 < c; r0 += 64) {
