@@ -74,6 +74,7 @@ struct KmTile {        // one work item of the assign kernels: <= 64 rows of one
 // kernel's waves per workgroup, kmAssignWavesPerBlock()).
 KmPartition partitionKmTiles(const std::vector<uint32_t>& taskTiles, const std::vector<uint32_t>& tileBeg);
 uint32_t kmAssignWavesPerBlock();
+uint32_t kmAssignMergeSlots();  // LDS merge slots per workgroup (KmPartition::blockSlots per block)
 
 struct KmLevel {       // device state of one level's batched k-means
     uint32_t b = 16, nbTasks = 0, nbTiles = 0, centroidType = 0;
@@ -97,7 +98,7 @@ struct KmLevel {       // device state of one level's batched k-means
     uint32_t nbChunks = 0;
     const uint32_t* waveChunkBeg = nullptr;  // [nbWaves + 1]
     uint32_t nbWaves = 0;
-    const KmSlot* blockSlots = nullptr;      // [workgroup][waves per workgroup - 1]
+    const KmSlot* blockSlots = nullptr;      // [workgroup][kmAssignMergeSlots()]
     uint32_t nbParts = 0;
     const uint32_t* taskPartBeg = nullptr;   // [task]
     const uint32_t* taskPartCnt = nullptr;   // [task]

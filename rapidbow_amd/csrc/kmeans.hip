@@ -22,6 +22,11 @@ namespace {
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr uint32_t kWaves = 4;  // waves per 256-thread block
+// LDS merge slots per workgroup of the MFMA assign: one per wave boundary, so
+// every same-task run of a workgroup leaves as one partial.  (Two slots let a
+// third workgroup fit per CU by LDS, three waves per SIMD: measured no faster,
+// with more partials to reduce.)
+constexpr uint32_t kMergeSlots = kWaves - 1;
 
 __device__ inline v4i ld16(const uint8_t* p) { return *reinterpret_cast<const v4i*>(p); }
 __device__ inline v4i toSigned(v4i v) {
@@ -430,14 +435,14 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
     __shared__ __align__(16) uint8_t sTile[kWaves][kTileLds];  // the tile, row-major, for the XT reads
     // merge slots: a task's chunks on neighbouring waves of this workgroup add
     // their sums, counts and #changed here and leave as one partial
-    __shared__ __align__(16) int32_t sMergeSums[kWaves - 1][16 * kDims];
-    __shared__ uint32_t sMergeCnt[kWaves - 1][17];
+    __shared__ __align__(16) int32_t sMergeSums[kMergeSlots][16 * kDims];
+    __shared__ uint32_t sMergeCnt[kMergeSlots][17];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 15, grp = lane >> 4;
     const v4i zero = {0, 0, 0, 0};
-    for (uint32_t i = threadIdx.x; i < (kWaves - 1) * 16 * kDims / 4; i += 256)
+    for (uint32_t i = threadIdx.x; i < kMergeSlots * 16 * kDims / 4; i += 256)
         reinterpret_cast<v4i*>(&sMergeSums[0][0])[i] = zero;
-    if (threadIdx.x < (kWaves - 1) * 17) (&sMergeCnt[0][0])[threadIdx.x] = 0u;
+    if (threadIdx.x < kMergeSlots * 17) (&sMergeCnt[0][0])[threadIdx.x] = 0u;
     __syncthreads();
     // one wave per resident slot, each with an equal range of tiles cut into
     // per-task chunks on the host (partitionKmTiles): no wave waits for a
@@ -522,8 +527,8 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
     }
     __syncthreads();
     // the merged partials leave once (a slot of a converged task was never written)
-    for (uint32_t sl = 0; sl < kWaves - 1; sl++) {
-        const KmSlot slot = k.blockSlots[blockIdx.x * (kWaves - 1) + sl];
+    for (uint32_t sl = 0; sl < kMergeSlots; sl++) {
+        const KmSlot slot = k.blockSlots[blockIdx.x * kMergeSlots + sl];
         if (slot.part == kNoSlot || !k.active[slot.task]) continue;
         v4i* ps = reinterpret_cast<v4i*>(k.partSums + uint64_t(slot.part) * 16 * kDims);
         for (uint32_t i = threadIdx.x; i < 16 * kDims / 4; i += 256) ps[i] = reinterpret_cast<const v4i*>(sMergeSums[sl])[i];
@@ -1135,8 +1140,10 @@ uint32_t kmAssignResidentBlocks() {
 }
 
 KmPartition partitionKmTiles(const std::vector<uint32_t>& taskTiles, const std::vector<uint32_t>& tileBeg) {
-    return partitionTiles(taskTiles, tileBeg, uint64_t(kmAssignResidentBlocks()) * kWaves, kWaves);
+    return partitionTiles(taskTiles, tileBeg, uint64_t(kmAssignResidentBlocks()) * kWaves, kWaves, kMergeSlots);
 }
+
+uint32_t kmAssignMergeSlots() { return kMergeSlots; }
 
 uint32_t kmAssignWavesPerBlock() { return kWaves; }
 

@@ -14,9 +14,10 @@
 // workgroup, consecutive chunks of the same task (they necessarily belong to
 // different waves: the last chunk of one wave and the first of the next) are
 // merged in LDS and leave as ONE partial; every other chunk writes its own.
-// A workgroup has at most wavesPerBlock - 1 merged runs (each crosses a wave
-// boundary of its own), so it needs that many LDS slots.  Partials are
-// numbered in chunk order, so the partials of a task are contiguous
+// A workgroup has at most wavesPerBlock - 1 such runs (each crosses a wave
+// boundary of its own); the first slotsPerBlock of them get an LDS slot and
+// any further one keeps a partial per chunk.  Partials are numbered in chunk
+// order, so the partials of a task are contiguous
 // [taskPartBeg[g], taskPartBeg[g] + taskPartCnt[g]).
 #pragma once
 #include <algorithm>
@@ -47,19 +48,20 @@ struct KmPartRange {   // a reduce item: partials [partBeg, partEnd) of one task
 };
 
 struct KmPartition {
-    uint32_t nbWaves = 0, wavesPerBlock = 1, nbParts = 0;
+    uint32_t nbWaves = 0, wavesPerBlock = 1, slotsPerBlock = 0, nbParts = 0;
     std::vector<KmChunk> chunks;
     std::vector<uint32_t> waveChunkBeg;               // [nbWaves + 1]
     std::vector<uint32_t> taskChunkBeg, taskChunkCnt;  // [task]; 0, 0 for tasks with no streamed tile
     std::vector<uint32_t> taskPartBeg, taskPartCnt;    // [task]; 0, 0 for tasks with no streamed tile
-    std::vector<KmSlot> blockSlots;                    // [block][wavesPerBlock - 1]
+    std::vector<KmSlot> blockSlots;                    // [block][slotsPerBlock]
 };
 
 // taskTiles[g]: streamed tiles of task g (its first taskTiles[g] tiles);
 // tileBeg[g]: global first tile of task g (empty: the prefix sums of taskTiles);
-// maxWaves: resident waves of the assign kernel; wavesPerBlock: its waves per workgroup.
+// maxWaves: resident waves of the assign kernel; wavesPerBlock: its waves per
+// workgroup; slotsPerBlock: its LDS merge slots (at most wavesPerBlock - 1 are used).
 inline KmPartition partitionTiles(const std::vector<uint32_t>& taskTiles, const std::vector<uint32_t>& tileBeg,
-                                  uint64_t maxWaves, uint32_t wavesPerBlock = 1) {
+                                  uint64_t maxWaves, uint32_t wavesPerBlock = 1, uint32_t slotsPerBlock = 0) {
     KmPartition p;
     const uint32_t nbTasks = uint32_t(taskTiles.size());
     std::vector<uint32_t> base(tileBeg);
@@ -96,7 +98,8 @@ inline KmPartition partitionTiles(const std::vector<uint32_t>& taskTiles, const 
 
     // partials: runs of same-task chunks inside one workgroup share one
     p.wavesPerBlock = std::max(wavesPerBlock, 1u);
-    const uint32_t W = p.wavesPerBlock, S = W - 1;
+    p.slotsPerBlock = std::min(slotsPerBlock, p.wavesPerBlock - 1);
+    const uint32_t W = p.wavesPerBlock, S = p.slotsPerBlock;
     const uint32_t nbBlocks = (p.nbWaves + W - 1) / W;
     p.blockSlots.assign(uint64_t(nbBlocks) * S, KmSlot{kNoSlot, kNoSlot});
     p.taskPartBeg.assign(nbTasks, 0u);
@@ -110,8 +113,7 @@ inline KmPartition partitionTiles(const std::vector<uint32_t>& taskTiles, const 
             uint32_t e = c + 1;
             while (e < ce && p.chunks[e].task == task) e++;
             if (p.taskPartCnt[task] == 0) p.taskPartBeg[task] = part;
-            // a run crosses a wave boundary of its own, so nextSlot < S always holds
-            // here; were it not, each chunk would simply keep a partial of its own
+            // runs beyond the workgroup's slots keep a partial per chunk
             if (e - c >= 2 && nextSlot < S) {
                 const uint32_t slot = nextSlot++;
                 p.blockSlots[uint64_t(blk) * S + slot] = KmSlot{part, task};

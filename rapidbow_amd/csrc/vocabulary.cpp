@@ -202,7 +202,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     const uint64_t blocksMax = mfmaPath ? uint64_t(kmAssignResidentBlocks()) : 0;
     const uint64_t wavesMax = blocksMax * kmAssignWavesPerBlock();
     const uint64_t chunksMax = mfmaPath ? wavesMax + nGmax : 0;  // a wave's range splits at task boundaries
-    const uint64_t slotsMax = blocksMax * (kmAssignWavesPerBlock() - 1);
+    const uint64_t slotsMax = blocksMax * kmAssignMergeSlots();
     const uint64_t redMax = mfmaPath ? chunksMax / 32 + nGmax : 0;
     const uint32_t persistTiles = mfmaPath ? std::min(opts.persist_max_tiles, lloydTasksTileCap(1)) : 0u;
     const uint32_t persistTiles2 = mfmaPath ? std::min(persistTiles, lloydTasksTileCap(2)) : 0u;  // 2 per CU

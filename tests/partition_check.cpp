@@ -32,26 +32,32 @@ static int fails = 0;
         }                                                  \
     } while (0)
 
-static void checkParts(const KmPartition& p, uint32_t nbTasks, uint32_t wpb, int id) {
-    const uint32_t W = std::max(wpb, 1u), S = W - 1, nbBlocks = (p.nbWaves + W - 1) / W;
-    CHECK(p.wavesPerBlock == W, "case %d: wavesPerBlock", id);
+static void checkParts(const KmPartition& p, uint32_t nbTasks, uint32_t wpb, uint32_t spb, int id) {
+    const uint32_t W = std::max(wpb, 1u), S = std::min(spb, W - 1), nbBlocks = (p.nbWaves + W - 1) / W;
+    CHECK(p.wavesPerBlock == W && p.slotsPerBlock == S, "case %d: wavesPerBlock / slotsPerBlock", id);
     CHECK(p.blockSlots.size() == size_t(nbBlocks) * S, "case %d: blockSlots size", id);
     std::vector<uint32_t> blockOf(p.chunks.size(), 0u);
     for (uint32_t w = 0; w < p.nbWaves; w++)
         for (uint32_t c = p.waveChunkBeg[w]; c < p.waveChunkBeg[w + 1]; c++) blockOf[c] = w / W;
+    // runs of same-task chunks within a workgroup, numbered per workgroup in order:
+    // the first S runs share a part through their slot, later ones keep a part per chunk
     std::vector<uint32_t> slotUses(p.blockSlots.size(), 0u);
+    uint32_t runsInBlock = 0;
     for (size_t c = 0; c < p.chunks.size(); c++) {
         const KmChunk& k = p.chunks[c];
         CHECK(k.part < p.nbParts, "case %d: chunk %zu part %u of %u", id, c, k.part, p.nbParts);
         if (c == 0) CHECK(k.part == 0, "case %d: first part %u", id, k.part);
+        if (c == 0 || blockOf[c] != blockOf[c - 1]) runsInBlock = 0;
         const bool sameRun = c > 0 && blockOf[c] == blockOf[c - 1] && p.chunks[c - 1].task == k.task;
+        const bool runNext = c + 1 < p.chunks.size() && blockOf[c + 1] == blockOf[c] && p.chunks[c + 1].task == k.task;
+        if (!sameRun && runNext) runsInBlock++;  // chunk c starts a run of two or more
+        const bool slotted = (sameRun || runNext) && runsInBlock <= S;
         if (c > 0) {
-            if (sameRun) CHECK(k.part == p.chunks[c - 1].part, "case %d: chunk %zu not merged with its run", id, c);
+            if (sameRun && slotted) CHECK(k.part == p.chunks[c - 1].part, "case %d: chunk %zu not merged with its run", id, c);
             else CHECK(k.part == p.chunks[c - 1].part + 1, "case %d: chunk %zu part %u after %u", id, c, k.part,
                        p.chunks[c - 1].part);
         }
-        const bool runNext = c + 1 < p.chunks.size() && blockOf[c + 1] == blockOf[c] && p.chunks[c + 1].task == k.task;
-        if (sameRun || runNext) {
+        if (slotted) {
             CHECK(k.slot < S, "case %d: merged chunk %zu slot %u", id, c, k.slot);
             if (k.slot < S) {
                 const KmSlot& sl = p.blockSlots[size_t(blockOf[c]) * S + k.slot];
@@ -59,7 +65,7 @@ static void checkParts(const KmPartition& p, uint32_t nbTasks, uint32_t wpb, int
                 slotUses[size_t(blockOf[c]) * S + k.slot]++;
             }
         } else {
-            CHECK(k.slot == kNoSlot, "case %d: lone chunk %zu has slot %u", id, c, k.slot);
+            CHECK(k.slot == kNoSlot, "case %d: chunk %zu without a run slot has slot %u", id, c, k.slot);
         }
     }
     if (!p.chunks.empty()) CHECK(p.chunks.back().part + 1 == p.nbParts, "case %d: nbParts %u", id, p.nbParts);
@@ -69,7 +75,7 @@ static void checkParts(const KmPartition& p, uint32_t nbTasks, uint32_t wpb, int
         CHECK(used ? slotUses[i] >= 2 : (slotUses[i] == 0 && p.blockSlots[i].task == kNoSlot),
               "case %d: slot %zu used by %u chunks", id, i, slotUses[i]);
     }
-    if (W == 1) CHECK(p.nbParts == p.chunks.size(), "case %d: one wave per block merges nothing", id);
+    if (S == 0) CHECK(p.nbParts == p.chunks.size(), "case %d: no slots merge nothing", id);
     std::vector<uint32_t> partTask(p.nbParts, kNoSlot);
     for (const KmChunk& k : p.chunks)
         if (k.part < p.nbParts) partTask[k.part] = k.task;
@@ -85,7 +91,7 @@ static void checkParts(const KmPartition& p, uint32_t nbTasks, uint32_t wpb, int
 }
 
 static void checkCase(const std::vector<uint32_t>& realTiles, const std::vector<uint8_t>& streamed, uint64_t maxWaves,
-                      bool passBase, int id, uint32_t wpb = 4) {
+                      bool passBase, int id, uint32_t wpb = 4, uint32_t spb = 3) {
     const uint32_t n = uint32_t(realTiles.size());
     std::vector<uint32_t> taskTiles(n), base(n);
     uint32_t g0 = 0;
@@ -98,7 +104,7 @@ static void checkCase(const std::vector<uint32_t>& realTiles, const std::vector<
     }
     if (!passBase)  // prefix-sum numbering: only meaningful when every task streams
         for (uint32_t g = 0; g < n; g++) CHECK(streamed[g], "case %d: test setup", id);
-    const KmPartition p = partitionTiles(taskTiles, passBase ? base : std::vector<uint32_t>{}, maxWaves, wpb);
+    const KmPartition p = partitionTiles(taskTiles, passBase ? base : std::vector<uint32_t>{}, maxWaves, wpb, spb);
     const uint64_t W = std::min<uint64_t>(std::max<uint64_t>(maxWaves, 1), T);
     CHECK(p.nbWaves == W, "case %d: nbWaves %u expected %llu", id, p.nbWaves, (unsigned long long)W);
     CHECK(p.waveChunkBeg.size() == size_t(p.nbWaves) + 1, "case %d: waveChunkBeg size", id);
@@ -135,7 +141,7 @@ static void checkCase(const std::vector<uint32_t>& realTiles, const std::vector<
             CHECK(c < p.chunks.size() && p.chunks[c].task == g, "case %d: task %u chunk list", id, g);
         if (!p.taskChunkCnt[g]) CHECK(p.taskChunkBeg[g] == 0, "case %d: task %u chunk begin", id, g);
     }
-    checkParts(p, n, wpb, id);
+    checkParts(p, n, wpb, spb, id);
 }
 
 int main() {
@@ -146,11 +152,12 @@ int main() {
     const std::vector<std::vector<uint8_t>> masks = {{1}, {1, 0, 1}, {1, 1, 1, 1, 1}, {1, 0, 0, 1}, {0, 1, 1, 1}};
     for (size_t i = 0; i < shapes.size(); i++)
         for (uint64_t mw : {1ull, 2ull, 3ull, 5ull, 64ull, 4096ull})
-            for (uint32_t wpb : {1u, 2u, 4u, 8u}) {
-                checkCase(shapes[i], masks[i], mw, true, cases++, wpb);
-                std::vector<uint8_t> all(shapes[i].size(), 1);
-                checkCase(shapes[i], all, mw, false, cases++, wpb);
-            }
+            for (uint32_t wpb : {1u, 2u, 4u, 8u})
+                for (uint32_t spb : {0u, 1u, 2u, 7u}) {
+                    checkCase(shapes[i], masks[i], mw, true, cases++, wpb, spb);
+                    std::vector<uint8_t> all(shapes[i].size(), 1);
+                    checkCase(shapes[i], all, mw, false, cases++, wpb, spb);
+                }
     // random levels: sizes from a heavy tail, about half the tasks not streamed
     for (int r = 0; r < 400; r++) {
         const uint32_t n = 1 + uint32_t(rng() % (r < 200 ? 40 : 4096));
@@ -163,9 +170,10 @@ int main() {
         }
         const uint64_t mw = std::vector<uint64_t>{1, 5, 256, 1024, 8192}[rng() % 5];
         const uint32_t wpb = std::vector<uint32_t>{1, 2, 4, 4, 8}[rng() % 5];
-        checkCase(tiles, st, mw, true, cases++, wpb);
+        const uint32_t spb = std::vector<uint32_t>{0, 1, 2, 2, 3, 7}[rng() % 6];
+        checkCase(tiles, st, mw, true, cases++, wpb, spb);
         std::vector<uint8_t> all(n, 1);
-        checkCase(tiles, all, mw, rng() % 2 == 0, cases++, wpb);
+        checkCase(tiles, all, mw, rng() % 2 == 0, cases++, wpb, spb);
     }
     std::printf("%d cases, %d failures\n", cases, fails);
     return fails ? EXIT_FAILURE : EXIT_SUCCESS;
