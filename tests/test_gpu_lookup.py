@@ -114,6 +114,27 @@ def test_lookup_f32_candidate_band(b):
     print(f"b={b}: {q.shape[0]} queries, near ties re-scored: {ties}")
 
 
+@pytest.mark.parametrize("b", [32, 16])
+def test_lookup_f32_all_rows_tie(b):
+    """Every query sits on a duplicated child, so every row of every work item is a
+    near tie: the per-wave tie queue fills and is resolved several times within an
+    item.  Leaf ids equal the plain kernel's (the lower of the equal children)."""
+    rng = np.random.default_rng(21)
+    desc = rb.synth_descriptors(2000, seed=22)
+    rows = _f32_tree(rng, b, 2, desc, near_ties=False)
+    n = rows.shape[0]
+    for v in range(n):
+        rows[v, b - 1] = rows[v, 3]  # child 3 and child b-1 are equal in every node
+    voc = rb.deserialize_vocabulary(rbw2_blob(b, 2, 1, rows, np.full(n, 0xFFFFFFFF), np.ones((n, b))))
+    target = np.clip(np.rint(rows[0, 3]), 0, 255).astype(np.uint8)
+    q = np.repeat(target[None], 1000, axis=0)
+    expect = oracle.lookup_f32(rows, b, 2, q)
+    np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_PLAIN), expect)
+    mfma, ties = voc.lookup_host(q, rb.LOOKUP_MFMA, return_near_ties=True)
+    np.testing.assert_array_equal(mfma, expect)
+    assert ties >= 1000  # at least the root level ties for every row
+
+
 def test_blob_roundtrip_u8():
     desc = rb.synth_descriptors(8000, seed=9)
     voc, res, blob = _u8_voc_from_oracle(desc, 2)
