@@ -474,7 +474,8 @@ __global__ __launch_bounds__(256) void k_levelBf16(
                 for (int kb = 0; kb < 4; kb++) w[kb] = *reinterpret_cast<const uint2*>(x + kb * 32 + grp * 8);
             }
         };
-        uint2 wn[4], wn2[4];
+        // rows past the item's end keep the previous (or zero) bytes: their scores are discarded
+        uint2 wn[4] = {}, wn2[4] = {};
         uint32_t qc = permAt(0), qn = permAt(1), q2 = permAt(2), q3 = permAt(3);
         loadRows(0, qc, wn);
         loadRows(1, qn, wn2);
@@ -495,25 +496,20 @@ __global__ __launch_bounds__(256) void k_levelBf16(
             if ((rb + 4) * 16 < t.cnt) q3 = permAt(rb + 4);
             v8bf ax[4];
             uint32_t xx = 0, xm = 0;
-            if (valid) {
 #pragma unroll
-                for (int kb = 0; kb < 4; kb++) {
-                    const uint2 w = wc[kb];
-                    xx = __builtin_amdgcn_udot4(w.x, w.x, xx, false);
-                    xx = __builtin_amdgcn_udot4(w.y, w.y, xx, false);
-                    xm = __builtin_amdgcn_udot4(w.x, muw[kb].x, xm, false);
-                    xm = __builtin_amdgcn_udot4(w.y, muw[kb].y, xm, false);
+            for (int kb = 0; kb < 4; kb++) {
+                const uint2 w = wc[kb];
+                xx = __builtin_amdgcn_udot4(w.x, w.x, xx, false);
+                xx = __builtin_amdgcn_udot4(w.y, w.y, xx, false);
+                xm = __builtin_amdgcn_udot4(w.x, muw[kb].x, xm, false);
+                xm = __builtin_amdgcn_udot4(w.y, muw[kb].y, xm, false);
 #pragma unroll
-                    for (int e = 0; e < 8; e++) {
-                        const uint32_t xw = e < 4 ? w.x : w.y, mw = e < 4 ? muw[kb].x : muw[kb].y;
-                        // the difference in [-255, 255] is exact in float and in bf16
-                        const float dv = float((xw >> (8 * (e & 3))) & 0xFFu) - float((mw >> (8 * (e & 3))) & 0xFFu);
-                        ax[kb][e] = __bf16(dv);
-                    }
+                for (int e = 0; e < 8; e++) {
+                    const uint32_t xw = e < 4 ? w.x : w.y, mw = e < 4 ? muw[kb].x : muw[kb].y;
+                    // the difference in [-255, 255] is exact in float and in bf16
+                    const float dv = float((xw >> (8 * (e & 3))) & 0xFFu) - float((mw >> (8 * (e & 3))) & 0xFFu);
+                    ax[kb][e] = __bf16(dv);
                 }
-            } else {
-#pragma unroll
-                for (int kb = 0; kb < 4; kb++) ax[kb] = v8bf{};
             }
             // ||x - mu||^2 of row (lane & 15): exact integers (< 2^24 after the sum over the lanes)
             int xs = valid ? int(xx) - 2 * int(xm) + int(mm) : 0;
