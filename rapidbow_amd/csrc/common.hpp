@@ -60,6 +60,14 @@ __device__ inline uint32_t min16u(uint32_t v) {
     return v;
 }
 
+// Minimum over lanes l, l ^ 16, l ^ 32, l ^ 48 (the four 16-lane rows of a wave), in every lane.
+__device__ inline uint32_t minAcrossGroupsU(uint32_t m) {
+    const auto p = __builtin_amdgcn_permlane16_swap(m, m, false, false);  // rows 0<->1, 2<->3
+    m = min(uint32_t(p[0]), uint32_t(p[1]));
+    const auto q = __builtin_amdgcn_permlane32_swap(m, m, false, false);  // halves
+    return min(uint32_t(q[0]), uint32_t(q[1]));
+}
+
 // Owning device allocation.
 template <typename T>
 class DevBuf {

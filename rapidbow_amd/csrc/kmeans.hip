@@ -373,14 +373,6 @@ __device__ inline void centerKeys(const uint8_t* cent, uint32_t lane, CenterKeys
     }
 }
 
-// Minimum over lanes l, l ^ 16, l ^ 32, l ^ 48 (the four 16-lane rows), in every lane.
-__device__ inline uint32_t minAcrossRows(uint32_t m) {
-    const auto p = __builtin_amdgcn_permlane16_swap(m, m, false, false);  // rows 0<->1, 2<->3
-    m = min(uint32_t(p[0]), uint32_t(p[1]));
-    const auto q = __builtin_amdgcn_permlane32_swap(m, m, false, false);  // halves
-    return min(uint32_t(q[0]), uint32_t(q[1]));
-}
-
 // 0x01 in each byte of x that is zero, else 0x00 (exact: no carries cross bytes).
 __device__ inline uint32_t zeroBytes(uint32_t x) {
     const uint32_t t = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
@@ -407,7 +399,7 @@ __device__ __forceinline__ uint32_t assignTileMfma(const v4i (&a)[4][2], const u
         d = __builtin_amdgcn_mfma_i32_16x16x64_i8(ck.b1, a[rb][1], d, 0, 0, 0);
         const uint32_t m01 = min(ck.base[0] - (uint32_t(d[0]) << 5), ck.base[1] - (uint32_t(d[1]) << 5));
         const uint32_t m23 = min(ck.base[2] - (uint32_t(d[2]) << 5), ck.base[3] - (uint32_t(d[3]) << 5));
-        best[rb] = minAcrossRows(min(m01, m23));
+        best[rb] = minAcrossGroupsU(min(m01, m23));
     }
     // row `lane` is point 16*grp + (lane & 15): the block rb = grp
     const uint32_t mine = (grp == 0 ? best[0] : grp == 1 ? best[1] : grp == 2 ? best[2] : best[3]) & 15u;

@@ -327,9 +327,11 @@ __global__ void k_writeItems(const uint32_t* __restrict__ O, const uint32_t* __r
 
 // --------------------------------------------------- I8 MFMA level step --
 // mfma_i32_16x16x64_i8: lane l feeds A row (l&15) / B column (l&15) with the
-// 16 bytes of dims [16*(l>>4) + 64*kb, +16), kb = 0,1 (same placement for A
-// and B, so the sum runs over all 128 dims); the result lane l holds rows
-// 4*(l>>4)+i, column l&15.
+// 16 bytes of dims [16*(l>>4) + 64*kb, +16) (same placement for A and B, so
+// the sum runs over all 128 dims); the result lane l holds rows 4*(l>>4)+i,
+// column l&15.  The node's 16 children are A and 16 descriptor rows are B, so
+// lane l holds children 4*(l>>4)+i of row (l&15): each lane owns one row, whose
+// argmin is an in-lane minimum over four keys and two permlane swaps.
 template <bool kIdentity, bool kLast>
 __global__ __launch_bounds__(256) void k_levelI8(const uint8_t* __restrict__ rows, const int32_t* __restrict__ normS,
                                                 uint64_t levelBase, const uint8_t* __restrict__ desc, uint32_t n,
@@ -344,46 +346,47 @@ __global__ __launch_bounds__(256) void k_levelI8(const uint8_t* __restrict__ row
     const uint32_t nbItems = kIdentity ? nbItemsImplicit : *nbItemsPtr;
     for (uint32_t it = blockIdx.x * 4 + wave; it < nbItems; it += gridDim.x * 4) {
         const LkItem t = itemAt<kIdentity>(it, n, items);
-        const uint64_t child = (levelBase + t.node) * 16 + col;
-        const uint8_t* crow = rows + child * kDims;
+        const uint64_t childBase = (levelBase + t.node) * 16;
+        const uint8_t* crow = rows + (childBase + col) * kDims;
         const v4i b0 = toSigned(ld16(crow + grp * 16)), b1 = toSigned(ld16(crow + 64 + grp * 16));
-        const int ns = normS[child];
-        // row block rb+1's descriptor bytes are in flight while rb computes
-        auto loadRows = [&](uint32_t rb, v4i& x0, v4i& x1) {
+        const v4i ns = *reinterpret_cast<const v4i*>(normS + childBase + 4 * grp);  // ||c - 128||^2 of children 4*grp + i
+        // the lane's row of a block: its descriptor (input index) and bytes; the next
+        // block's bytes and the permutation entry of the one after are in flight
+        auto qAt = [&](uint32_t rb) {
             const uint32_t r = rb * 16 + col;
-            x0 = v4i{0, 0, 0, 0};
+            return r < t.cnt ? (kIdentity ? t.start + r : perm[t.start + r]) : 0u;
+        };
+        auto loadRows = [&](uint32_t rb, uint32_t q, v4i& x0, v4i& x1) {
+            x0 = v4i{0, 0, 0, 0};  // rows past the item's end contribute nothing (and are not written)
             x1 = v4i{0, 0, 0, 0};
-            if (r < t.cnt) {
-                const uint32_t q = kIdentity ? t.start + r : perm[t.start + r];
+            if (rb * 16 + col < t.cnt) {
                 const uint8_t* x = desc + uint64_t(q) * kDims;
-                x0 = ld16(x + grp * 16);
-                x1 = ld16(x + 64 + grp * 16);
+                x0 = toSigned(ld16(x + grp * 16));
+                x1 = toSigned(ld16(x + 64 + grp * 16));
             }
         };
+        uint32_t qc = qAt(0), qn = qAt(1);
         v4i n0, n1;
-        loadRows(0, n0, n1);
+        loadRows(0, qc, n0, n1);
         for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
-            const uint32_t r = rb * 16 + col;
-            const v4i c0 = n0, c1 = n1;
-            if ((rb + 1) * 16 < t.cnt) loadRows(rb + 1, n0, n1);
-            const v4i a0 = r < t.cnt ? toSigned(c0) : c0, a1 = r < t.cnt ? toSigned(c1) : c1;
+            const uint32_t r = rb * 16 + col, q = qc;
+            const v4i a0 = n0, a1 = n1;
+            qc = qn;
+            if ((rb + 1) * 16 < t.cnt) loadRows(rb + 1, qc, n0, n1);
+            if ((rb + 2) * 16 < t.cnt) qn = qAt(rb + 2);
             v4i acc = {0, 0, 0, 0};
-            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b0, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b1, acc, 0, 0, 0);
-            uint32_t key[4];
+            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(b0, a0, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(b1, a1, acc, 0, 0, 0);
+            // keys ((||c'||^2 - 2 x'.c' + bias) << 4) | child: unique, the smallest is the
+            // nearest child with ties to the lowest index
+            uint32_t key = 0xFFFFFFFFu;
 #pragma unroll
-            for (int i = 0; i < 4; i++) key[i] = min16u((uint32_t(ns - 2 * acc[i] + kScoreBias) << 4) | col);
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint32_t row = rb * 16 + 4 * grp + i;
-                if (col == uint32_t(i) && row < t.cnt) {
-                    if (kLast) {
-                        const uint32_t q = kIdentity ? t.start + row : perm[t.start + row];
-                        out[q] = t.node * 16 + (key[i] & 15u);
-                    } else {
-                        sj[row] = uint8_t(key[i] & 15u);
-                    }
-                }
+            for (int i = 0; i < 4; i++)
+                key = min(key, (uint32_t(ns[i] - 2 * acc[i] + kScoreBias) << 4) | (4 * grp + uint32_t(i)));
+            key = minAcrossGroupsU(key);
+            if (grp == 0 && r < t.cnt) {
+                if (kLast) out[q] = t.node * 16 + (key & 15u);
+                else sj[r] = uint8_t(key & 15u);
             }
         }
         if (!kLast) itemCounts(lane, 4, it, t, sj, scAll[wave], out, C);
