@@ -65,7 +65,9 @@ typedef struct {
                                  * tiles runs its whole Lloyd loop in one launch with its points in
                                  * LDS (clamped to what one workgroup's LDS holds); 0: every node
                                  * runs one launch per pass.  Same results either way.  Default 16. */
-    uint32_t reserved;      /* must be 0 */
+    uint32_t timing;        /* 1: time every assign, reduce and update launch with HIP events for
+                             * rbow_voc_build_stats (each event takes a few microseconds of queue
+                             * time); 0 (default): total_ms, point and launch counts only */
 } rbow_build_options;
 
 void rbow_build_options_default(rbow_build_options* opts);
@@ -238,7 +240,8 @@ rbow_status rbow_gather_rows(const void* dev_desc, const uint32_t* dev_idx, uint
 const uint32_t* rbow_voc_build_passes(const rbow_vocabulary* voc);
 
 /* Timing of the last build (zeros for deserialised vocabularies).
- * assign_ops = point_passes * 2 * b * 128 (the distance contraction). */
+ * assign_ops = point_passes * 2 * b * 128 (the distance contraction).  The
+ * *_ms fields other than total_ms need rbow_build_options.timing = 1 (else 0). */
 typedef struct {
     double total_ms;         /* host wall time of the build */
     double assign_ms;        /* summed assign-kernel time (HIP events around each assign launch) */

@@ -50,7 +50,7 @@ class ServerOptions(ctypes.Structure):
 class BuildOptions(ctypes.Structure):
     _fields_ = [("max_iters", ctypes.c_uint32), ("seed_mode", ctypes.c_uint32), ("seed", ctypes.c_uint64),
                 ("centroid_type", ctypes.c_uint32), ("kernel", ctypes.c_uint32),
-                ("persist_max_tiles", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("persist_max_tiles", ctypes.c_uint32), ("timing", ctypes.c_uint32)]
 
 
 def _sig(name, res, args):
@@ -231,10 +231,12 @@ class Vocabulary:
         return out.reshape(-1, self.branch_factor, 128)
 
 
-def build_options(max_iters=1000, seed_mode=0, seed=5489, centroid_type=0, kernel=0, persist_max_tiles=None):
+def build_options(max_iters=1000, seed_mode=0, seed=5489, centroid_type=0, kernel=0, persist_max_tiles=None,
+                  timing=False):
     o = BuildOptions()
     lib.rbow_build_options_default(ctypes.byref(o))
     o.max_iters, o.seed_mode, o.seed, o.centroid_type, o.kernel = max_iters, seed_mode, seed, centroid_type, kernel
+    o.timing = int(bool(timing))
     if persist_max_tiles is not None:
         o.persist_max_tiles = persist_max_tiles
     return o

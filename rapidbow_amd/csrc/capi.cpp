@@ -181,7 +181,7 @@ void rbow_build_options_default(rbow_build_options* o) {
     o->centroid_type = 0;
     o->kernel = 0;
     o->persist_max_tiles = 16;
-    o->reserved = 0;
+    o->timing = 0;
 }
 
 rbow_status rbow_build_vocabulary_ex(rbow_descriptor_attributes attr, uint32_t branch_factor, const void* dev_desc,
@@ -195,7 +195,7 @@ rbow_status rbow_build_vocabulary_ex(rbow_descriptor_attributes attr, uint32_t b
         if (opts) o = *opts;
         checkAttributes(attr);
         RBOW_REQUIRE(o.centroid_type <= 1, RBOW_ERR_INVALID_ARG, "centroid_type must be 0 or 1");
-        RBOW_REQUIRE(o.reserved == 0, RBOW_ERR_INVALID_ARG, "reserved option field must be 0");
+        RBOW_REQUIRE(o.timing <= 1, RBOW_ERR_INVALID_ARG, "timing must be 0 or 1");
         RBOW_REQUIRE(branch_factor == 16 || (branch_factor == 32 && o.centroid_type == 1), RBOW_ERR_UNSUPPORTED,
                      "branchFactor must be 16 (or 32 with float32 centroids)");
         RBOW_REQUIRE(dev_desc != nullptr || nb_desc == 0, RBOW_ERR_INVALID_ARG, "null descriptors");

@@ -160,7 +160,10 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     const uint32_t ct = opts.centroid_type;
     const int kernel = int(opts.kernel);
     const auto t0 = std::chrono::steady_clock::now();
-    std::vector<hipEvent_t> ev;  // [0, 1]: persistent launch; 2 + 5 * pass + i: assign/reduce/update marks
+    // per-launch timing (opts.timing): [0, 1] bracket the persistent launch; 2 + 4 * pass + i: before the
+    // assign, after it, after the reduce (only when one is launched), after the update
+    const bool timing = stats && opts.timing;
+    std::vector<hipEvent_t> ev;
     struct EventsFree {
         std::vector<hipEvent_t>& v;
         ~EventsFree() {
@@ -382,32 +385,34 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
 
         // ---- Lloyd passes (kmeans.cpp:118-133) ----
         if (!smallTasks.empty()) {
-            if (stats) mark(0);
+            if (timing) mark(0);
             // two LDS classes: tasks that allow two workgroups per CU run in a launch of their own
             launchLloydTasks(k, dSmall, nbLarger, smallTilesMax, 1, dTileBeg, opts.max_iters, s);
             launchLloydTasks(k, dSmall + nbLarger, uint32_t(smallTasks.size()) - nbLarger,
                              std::min(smallTilesMax, persistTiles2), 2, dTileBeg, opts.max_iters, s);
-            if (stats) mark(1);
+            if (timing) mark(1);
         }
         // with no streamed task left, one update pass retires the empty tasks
         const bool streamed = !mfmaPath || k.nbWaves > 0;
         const uint32_t iters = streamed ? opts.max_iters : 1u;
         const bool poll = iters > 16;
         uint32_t nbLaunched = 0;
+        const bool reduced = mfmaPath && !red.empty();
         for (uint32_t it = 0; it < iters; it++) {
-            const size_t e = 2 + 5 * size_t(it);
-            if (stats) mark(e);
+            const size_t e = 2 + 4 * size_t(it);
+            if (timing) mark(e);
             if (f32Mfma)  // the filter's operands of this pass's centers
                 launchPrepF32Tree(k.centersF32, nG, b, dFMu.get(), dFHi.get(), dFLo.get(), dFCn2.get(), dFE1.get(),
                                   dFE2.get(), s, k.active);
             launchKmAssign(k, kernel, s);
-            if (stats) mark(e + 1);
-            if (mfmaPath) launchKmReduce(k, dRed, uint32_t(red.size()), s);
-            if (stats) mark(e + 2);
+            if (timing) mark(e + 1);
+            if (reduced) {
+                launchKmReduce(k, dRed, uint32_t(red.size()), s);
+                if (timing) mark(e + 2);
+            }
             if (poll) launchFillU32(dNbActive.get(), 0u, 1, s);
-            if (stats) mark(e + 3);
             launchKmUpdate(k, s);
-            if (stats) mark(e + 4);
+            if (timing) mark(e + 3);
             nbLaunched = it + 1;
             if (poll && (it % 4 == 3)) {  // kmeans.cpp:125-131 polls every 4 passes
                 RBOW_HIP(hipMemcpyAsync(hNbActive, dNbActive.get(), 4, hipMemcpyDeviceToHost, s));
@@ -434,22 +439,20 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             pointPasses += uint64_t(taskCnt[g]) * hPasses[g];
             if (!isSmall[g]) streamPointPasses += uint64_t(taskCnt[g]) * hPasses[g];
         }
-        if (stats) {
+        if (streamed) streamLaunches += nbLaunched;
+        if (timing) {
             float pm = 0;
             if (!smallTasks.empty()) RBOW_HIP(hipEventElapsedTime(&pm, ev[0], ev[1]));
             for (uint32_t it = 0; it < nbLaunched; it++) {
-                const size_t e = 2 + 5 * size_t(it);
+                const size_t e = 2 + 4 * size_t(it);
                 float a = 0, r = 0, u = 0;
                 RBOW_HIP(hipEventElapsedTime(&a, ev[e], ev[e + 1]));
-                RBOW_HIP(hipEventElapsedTime(&r, ev[e + 1], ev[e + 2]));
-                RBOW_HIP(hipEventElapsedTime(&u, ev[e + 3], ev[e + 4]));
+                if (reduced) RBOW_HIP(hipEventElapsedTime(&r, ev[e + 1], ev[e + 2]));
+                RBOW_HIP(hipEventElapsedTime(&u, ev[reduced ? e + 2 : e + 1], ev[e + 3]));
                 assignMs += a;
                 reduceMs += r;
                 updateMs += u;
-                if (streamed) {
-                    streamMs += a;
-                    streamLaunches++;
-                }
+                if (streamed) streamMs += a;
             }
             assignMs += pm;
             persistTotalMs += pm;

@@ -114,11 +114,11 @@ def test_build_u8_persistent_and_streamed_tasks(iters):
     tiles = (res["occ"][level_beg(16, 2):level_beg(16, 3)].ravel().astype(np.int64) + 63) // 64  # level-3 tasks
     assert ((tiles >= 1) & (tiles <= 8)).sum() >= 256 and ((tiles > 8) & (tiles <= 16)).any()
     assert iters != 3 or (tiles > 16).any()  # with the cap at 3, level 3 also streams some tasks
-    voc = _gpu_build(desc, nb_doc, L, max_iters=iters, seed_mode=1, kernel=0)
+    voc = _gpu_build(desc, nb_doc, L, max_iters=iters, seed_mode=1, kernel=0, timing=True)
     _check_u8(voc, res, L)
     st = voc.build_stats()
     assert st["persist_ms"] > 0 and st["assign_launches"] > 0  # both paths ran
     # the same tree with every task streamed, one launch per pass
-    voc0 = _gpu_build(desc, nb_doc, L, max_iters=iters, seed_mode=1, kernel=0, persist_max_tiles=0)
+    voc0 = _gpu_build(desc, nb_doc, L, max_iters=iters, seed_mode=1, kernel=0, persist_max_tiles=0, timing=True)
     assert voc0.build_stats()["persist_ms"] == 0
     assert voc0.serialize() == voc.serialize()
