@@ -564,7 +564,10 @@ __global__ __launch_bounds__(64 * W) void k_lloydTasks(KmLevel k, const uint32_t
     constexpr uint32_t kThreads = 64 * W;
     extern __shared__ __align__(16) uint8_t dynLds[];
     __shared__ __align__(16) uint8_t sNew[W][64];
-    __shared__ int32_t sSum[16 * kDims];  // sums of x - 128
+    // sums of x - 128, center rows kSumPitch words apart: the 16-lane groups of an
+    // atomic add (rows 4*grp + i) fall on disjoint banks
+    constexpr uint32_t kSumPitch = kDims + 4;
+    __shared__ __align__(16) int32_t sSum[16 * kSumPitch];
     __shared__ uint32_t sCnt[2][16], sChanged[2];
     __shared__ __align__(16) uint8_t sCent[16 * kDims];
     const uint32_t g = taskList[blockIdx.x], tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -574,14 +577,26 @@ __global__ __launch_bounds__(64 * W) void k_lloydTasks(KmLevel k, const uint32_t
     uint8_t* sA = dynLds;                                   // [nt] row-major tiles of kTileLds bytes
     uint8_t* sAsg = dynLds + uint64_t(maxTiles) * kTileLds;  // [nt * 64]
     const uint4* src = reinterpret_cast<const uint4*>(k.XA + uint64_t(taskTileBeg[g]) * 8192);
-    for (uint32_t i = tid; i < nt * 512; i += kThreads) {  // XA fragment (tile, rb, kb, lane) -> its rows in LDS
-        const uint32_t c = i & 511;
-        *reinterpret_cast<uint4*>(sA + (i >> 9) * kTileLds + fragOffset(c >> 7, (c >> 6) & 1, c & 63)) = src[i];
+    // XA fragment (tile, rb, kb, lane) -> its rows in LDS; 8 loads in flight per thread
+    constexpr uint32_t kLoadBatch = 8;
+    for (uint32_t i0 = tid; i0 < nt * 512; i0 += kLoadBatch * kThreads) {
+        uint4 v[kLoadBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kLoadBatch; j++) {
+            const uint32_t i = i0 + j * kThreads;
+            v[j] = i < nt * 512 ? src[i] : uint4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kLoadBatch; j++) {
+            const uint32_t i = i0 + j * kThreads, c = i & 511;
+            if (i < nt * 512)
+                *reinterpret_cast<uint4*>(sA + (i >> 9) * kTileLds + fragOffset(c >> 7, (c >> 6) & 1, c & 63)) = v[j];
+        }
     }
     for (uint32_t i = tid; i < nt * 64; i += kThreads) sAsg[i] = 0xFF;  // no previous assignment
     uint8_t* cent = k.centersU8 + uint64_t(g) * 16 * kDims;
     if (tid < 128) reinterpret_cast<uint4*>(sCent)[tid] = reinterpret_cast<const uint4*>(cent)[tid];
-    for (uint32_t e = tid; e < 16 * kDims; e += kThreads) sSum[e] = 0;
+    for (uint32_t e = tid; e < 16 * kSumPitch; e += kThreads) sSum[e] = 0;
     if (tid < 32) sCnt[tid >> 4][tid & 15] = 0;
     if (tid < 2) sChanged[tid] = 0;
     const v4i zero = {0, 0, 0, 0};
@@ -614,25 +629,35 @@ __global__ __launch_bounds__(64 * W) void k_lloydTasks(KmLevel k, const uint32_t
                 }
             }
         }
-        // the waves' partials meet in LDS
+        if (wave < nt) {  // the waves' partials meet in LDS (a wave without a tile has none)
 #pragma unroll
-        for (int db = 0; db < 8; db++)
+            for (int db = 0; db < 8; db++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) atomicAdd(&sSum[(4 * grp + i) * kDims + 16 * db + col], acc[db][i]);
-        myCnt += uint32_t(__shfl_xor(int(myCnt), 16, 64));
-        myCnt += uint32_t(__shfl_xor(int(myCnt), 32, 64));
-        if (lane < 16) atomicAdd(&sCnt[par][lane], myCnt);
+                for (int i = 0; i < 4; i++) atomicAdd(&sSum[(4 * grp + i) * kSumPitch + 16 * db + col], acc[db][i]);
+            myCnt += uint32_t(__shfl_xor(int(myCnt), 16, 64));
+            myCnt += uint32_t(__shfl_xor(int(myCnt), 32, 64));
+            if (lane < 16) atomicAdd(&sCnt[par][lane], myCnt);
 #pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) myChanged += uint32_t(__shfl_xor(int(myChanged), m, 64));
-        if (lane == 0) atomicAdd(&sChanged[par], myChanged);
+            for (int m = 32; m >= 1; m >>= 1) myChanged += uint32_t(__shfl_xor(int(myChanged), m, 64));
+            if (lane == 0) atomicAdd(&sChanged[par], myChanged);
+        }
         __syncthreads();  // every partial is in
         pass++;
         const bool stop = sChanged[par] == 0;  // converged on this pass: no update (uniform)
-        for (uint32_t e = tid; e < 16 * kDims; e += kThreads) {
-            const int32_t sv = sSum[e];
-            sSum[e] = 0;  // this thread's own elements: cleared for the next pass
-            const uint32_t c = sCnt[par][e / kDims];
-            if (!stop && c) sCent[e] = uint8_t(roundedMean(uint32_t(sv + 128 * int32_t(c)), c));
+        // four dims of one center per thread and step
+#pragma unroll 2
+        for (uint32_t e4 = tid; e4 < 16 * kDims / 4; e4 += kThreads) {
+            const uint32_t row = e4 / (kDims / 4), d4 = e4 % (kDims / 4);
+            v4i* sp = reinterpret_cast<v4i*>(sSum + row * kSumPitch) + d4;
+            const v4i sv = *sp;
+            *sp = zero;  // this thread's own elements: cleared for the next pass
+            const uint32_t c = sCnt[par][row];
+            if (!stop && c) {
+                uint32_t packed = 0;
+#pragma unroll
+                for (int j = 0; j < 4; j++) packed |= roundedMean(uint32_t(sv[j] + 128 * int32_t(c)), c) << (8 * j);
+                reinterpret_cast<uint32_t*>(sCent)[e4] = packed;
+            }
         }
         if (tid < 16) sCnt[par ^ 1][tid] = 0;  // counters of the next pass
         if (tid == 0) sChanged[par ^ 1] = 0;
