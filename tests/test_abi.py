@@ -48,6 +48,15 @@ def test_unsupported_attributes():
     assert st == rb.ERR_UNSUPPORTED
 
 
+def test_build_option_timing_is_checked():
+    o = rb.build_options(max_iters=6)
+    o.timing = 2
+    h = ctypes.c_void_p()
+    st = rb.lib.rbow_build_vocabulary_ex(rb.SIFT_ATTR, 16, 0x1000, 100, 1, 2, ctypes.byref(o), None, ctypes.byref(h))
+    assert st == rb.ERR_INVALID_ARG and h.value is None
+    assert rb.lib.rbow_last_error() == b"timing must be 0 or 1"
+
+
 def test_build_input_errors_match_reference_messages():
     with pytest.raises(rb.RbowError) as e:
         rb.build_vocabulary(0x1000, 15, 1, 2)

@@ -122,3 +122,17 @@ def test_build_u8_persistent_and_streamed_tasks(iters):
     voc0 = _gpu_build(desc, nb_doc, L, max_iters=iters, seed_mode=1, kernel=0, persist_max_tiles=0, timing=True)
     assert voc0.build_stats()["persist_ms"] == 0
     assert voc0.serialize() == voc.serialize()
+
+
+def test_build_timing_option():
+    # per-launch events only with timing=1; the tree is the same either way
+    desc = rb.synth_descriptors(120000, seed=11, nb_centres=40)
+    timed = _gpu_build(desc, 300, 3, max_iters=6, seed_mode=1, kernel=0, timing=True)
+    plain = _gpu_build(desc, 300, 3, max_iters=6, seed_mode=1, kernel=0)
+    assert timed.serialize() == plain.serialize()
+    st, s0 = timed.build_stats(), plain.build_stats()
+    assert st["assign_ms"] > 0 and st["update_ms"] > 0 and st["total_ms"] > 0
+    assert s0["assign_ms"] == 0 and s0["update_ms"] == 0 and s0["reduce_ms"] == 0 and s0["total_ms"] > 0
+    # counts do not depend on timing
+    for f in ("point_passes", "assign_ops", "assign_launches", "stream_point_passes"):
+        assert st[f] == s0[f], f
