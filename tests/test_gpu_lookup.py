@@ -33,7 +33,8 @@ def test_lookup_u8_ties_and_edges(kernel):
         n = level_beg(16, L)
         rows = rng.integers(0, 256, (n, 16, 128), dtype=np.uint8)
         rows[:, 11] = rows[:, 4]  # duplicate children: ties must go to the lower index
-        rows[:, 14] = rows[:, 1]
+        rows[:, 14] = rows[:, 1]  # (across the MFMA's 4-child lane groups: 4 / 11, 1 / 14)
+        rows[:, 6] = rows[:, 5]   # (within one group: 5 / 6)
         blob = ref_blob(L, oracle.rows_to_ref_nodes(rows), np.full(n, 0xFFFF), np.ones((n, 16), np.float32))
         voc = rb.deserialize_vocabulary(blob)
         for nq in (1, 15, 16, 17, 129, 1000):
