@@ -65,9 +65,9 @@ typedef struct {
                                  * tiles runs its whole Lloyd loop in one launch with its points in
                                  * LDS (clamped to what one workgroup's LDS holds); 0: every node
                                  * runs one launch per pass.  Same results either way.  Default 16. */
-    uint32_t timing;        /* 1: time every assign, reduce and update launch with HIP events for
-                             * rbow_voc_build_stats (each event takes a few microseconds of queue
-                             * time); 0 (default): total_ms, point and launch counts only */
+    uint32_t timing;        /* 1: time every assign, reduce and update launch for rbow_voc_build_stats
+                             * with HIP events stamped by the dispatches (kernel start / end);
+                             * 0 (default): total_ms, point and launch counts only */
 } rbow_build_options;
 
 void rbow_build_options_default(rbow_build_options* opts);

@@ -128,26 +128,37 @@ void launchKmLevelReset(const KmLevel& k, uint64_t nbRows, hipStream_t s);
 // Per-tile swizzled operand XA of the MFMA assign from desc[order[row]].
 void launchGatherTiles(const uint8_t* desc, const uint32_t* order, const KmTile* tiles, uint32_t nbTiles, int8_t* XA,
                        hipStream_t s);
+// Optional timing of one launch (or of a run of launches): HIP events recorded
+// by the dispatch itself when the kernel starts and when it ends, so their
+// interval is the kernel's own duration.  A null event is not recorded.
+struct KernelTimer {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
 // Adds the partials of the MFMA assign into the task totals counts / sums /
-// changed.  red: (task, partBeg, partEnd) items.
-void launchKmReduce(const KmLevel& k, const KmPartRange* red, uint32_t nbRed, hipStream_t s);
+// changed.  red: (task, partBeg, partEnd) items.  Returns whether a kernel ran.
+bool launchKmReduce(const KmLevel& k, const KmPartRange* red, uint32_t nbRed, hipStream_t s,
+                    const KernelTimer& t = {});
 // Whole Lloyd loops (assign + update, up to maxIters passes, per-task stop) of
 // the listed uint8 tasks, one workgroup each with the task's points in LDS;
 // leaves them inactive with their pass counts.  Every listed task has 1 to
 // maxTiles tiles (maxTiles <= lloydTasksTileCap(blocksPerCu), blocksPerCu = 1
 // or 2 workgroups sharing a CU); taskTileBeg[g] = first tile of task g in k.XA.
 void launchLloydTasks(const KmLevel& k, const uint32_t* taskList, uint32_t nbList, uint32_t maxTiles,
-                      uint32_t blocksPerCu, const uint32_t* taskTileBeg, uint32_t maxIters, hipStream_t s);
+                      uint32_t blocksPerCu, const uint32_t* taskTileBeg, uint32_t maxIters, hipStream_t s,
+                      const KernelTimer& t = {});
 // Largest task (in 64-row tiles) whose points fit one workgroup's LDS when
 // blocksPerCu workgroups of launchLloydTasks share a CU.
 uint32_t lloydTasksTileCap(uint32_t blocksPerCu);
 // Blocks of the MFMA assign kernel that the device holds at once (its grid cap).
 uint32_t kmAssignResidentBlocks();
-// One assign pass over all active tasks; kernel 0 = MFMA, 1 = plain.
-void launchKmAssign(const KmLevel& k, int kernel, hipStream_t s);
+// One assign pass over all active tasks; kernel 0 = MFMA, 1 = plain.  Returns
+// whether a kernel ran.
+bool launchKmAssign(const KmLevel& k, int kernel, hipStream_t s, const KernelTimer& t = {});
 // Update pass: converged tasks become inactive, others move their centers
 // and clear their accumulators.  Counts remaining active tasks into nbActive.
-void launchKmUpdate(const KmLevel& k, hipStream_t s);
+// Returns whether a kernel ran.
+bool launchKmUpdate(const KmLevel& k, hipStream_t s, const KernelTimer& t = {});
 
 // Stable regroup of a level's rows by (task, child) (Vocabulary.cpp:83-98):
 // orderOut = orderIn stably sorted by task * b + assignment; hist[task * b + j]

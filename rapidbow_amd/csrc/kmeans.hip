@@ -11,7 +11,9 @@
 //   loop:   a task stops after the first pass with changed == 0 (no update on
 //           that pass) or after max_iters passes.
 // Sums are exact: int32 per wave in LDS, flushed into int64 per task.
+#include <hip/hip_ext.h>
 #include <hipcub/hipcub.hpp>
+#include <type_traits>
 
 #include "bf16filter.hpp"
 #include "kernels.hpp"
@@ -1084,6 +1086,23 @@ uint32_t gridFor(uint64_t work, uint32_t perBlock, uint32_t cap) {
     return uint32_t(g == 0 ? 1 : (g > cap ? cap : g));
 }
 
+// kernel<<<grid, block, lds, s>>>(args...), with t's events recorded by the
+// dispatch (hipExtLaunchKernel) when it has any.  Args are the kernel's
+// parameter types exactly: their addresses are the argument array.
+template <typename... Params, typename... Args>
+void launch(void (*kernel)(Params...), dim3 grid, dim3 block, uint32_t lds, hipStream_t s, const KernelTimer& t,
+            Args... args) {
+    static_assert(sizeof...(Params) == sizeof...(Args), "argument count");
+    static_assert((std::is_same<Params, Args>::value && ...), "argument types must match the kernel's");
+    if (!t.start && !t.stop) {
+        kernel<<<grid, block, lds, s>>>(args...);
+        RBOW_HIP(hipGetLastError());
+        return;
+    }
+    void* a[] = {static_cast<void*>(&args)...};
+    RBOW_HIP(hipExtLaunchKernel(reinterpret_cast<const void*>(kernel), grid, block, a, lds, s, t.start, t.stop, 0));
+}
+
 }  // namespace
 
 void launchGatherRows(const uint8_t* desc, const uint32_t* idx, uint64_t n, uint8_t* out, hipStream_t s) {
@@ -1135,10 +1154,10 @@ void launchGatherTiles(const uint8_t* desc, const uint32_t* order, const KmTile*
     RBOW_HIP(hipGetLastError());
 }
 
-void launchKmReduce(const KmLevel& k, const KmPartRange* red, uint32_t nbRed, hipStream_t s) {
-    if (!nbRed) return;
-    k_reduceChunks<<<dim3(nbRed, 16 * kDims / 256), 256, 0, s>>>(k, red, nbRed);
-    RBOW_HIP(hipGetLastError());
+bool launchKmReduce(const KmLevel& k, const KmPartRange* red, uint32_t nbRed, hipStream_t s, const KernelTimer& t) {
+    if (!nbRed) return false;
+    launch(k_reduceChunks, dim3(nbRed, 16 * kDims / 256), 256, 0, s, t, k, red, nbRed);
+    return true;
 }
 
 uint32_t kmAssignResidentBlocks() {
@@ -1190,7 +1209,8 @@ uint32_t lloydTasksTileCap(uint32_t blocksPerCu) {
 }
 
 void launchLloydTasks(const KmLevel& k, const uint32_t* taskList, uint32_t nbList, uint32_t maxTiles,
-                      uint32_t blocksPerCu, const uint32_t* taskTileBeg, uint32_t maxIters, hipStream_t s) {
+                      uint32_t blocksPerCu, const uint32_t* taskTileBeg, uint32_t maxIters, hipStream_t s,
+                      const KernelTimer& t) {
     if (!nbList) return;
     RBOW_REQUIRE(maxTiles >= 1 && maxTiles <= lloydTasksTileCap(blocksPerCu), RBOW_ERR_INVALID_ARG,
                  "persistent k-means tasks exceed the LDS capacity");
@@ -1199,30 +1219,30 @@ void launchLloydTasks(const KmLevel& k, const uint32_t* taskList, uint32_t nbLis
     if (blocksPerCu <= 1) {
         RBOW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_lloydTasks<8>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
-        k_lloydTasks<8><<<nbList, 512, dyn, s>>>(k, taskList, taskTileBeg, maxTiles, maxIters);
+        launch(k_lloydTasks<8>, nbList, 512, uint32_t(dyn), s, t, k, taskList, taskTileBeg, maxTiles, maxIters);
     } else {
         RBOW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_lloydTasks<4>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
-        k_lloydTasks<4><<<nbList, 256, dyn, s>>>(k, taskList, taskTileBeg, maxTiles, maxIters);
+        launch(k_lloydTasks<4>, nbList, 256, uint32_t(dyn), s, t, k, taskList, taskTileBeg, maxTiles, maxIters);
     }
-    RBOW_HIP(hipGetLastError());
 }
 
-void launchKmAssign(const KmLevel& k, int kernel, hipStream_t s) {
-    if (!k.nbTiles) return;
+bool launchKmAssign(const KmLevel& k, int kernel, hipStream_t s, const KernelTimer& t) {
+    if (!k.nbTiles) return false;
     const uint32_t grid = gridFor(k.nbTiles, kWaves, 2048);
     if (k.centroidType == 0) {
-        if (kernel == 1) k_assignU8Plain<<<grid, 256, 0, s>>>(k);
-        else if (k.nbWaves) k_assignU8Mfma<<<divUp(k.nbWaves, kWaves), 256, 0, s>>>(k);
+        if (kernel == 1) launch(k_assignU8Plain, grid, 256, 0, s, t, k);
+        else if (k.nbWaves) launch(k_assignU8Mfma, divUp(k.nbWaves, kWaves), 256, 0, s, t, k);
+        else return false;
     } else if (kernel == 0 && k.fHi) {
-        if (k.b == 32) k_assignF32Mfma<32><<<grid, 256, 0, s>>>(k);
-        else k_assignF32Mfma<16><<<grid, 256, 0, s>>>(k);
+        if (k.b == 32) launch(k_assignF32Mfma<32>, grid, 256, 0, s, t, k);
+        else launch(k_assignF32Mfma<16>, grid, 256, 0, s, t, k);
     } else if (k.b == 32) {
-        k_assignF32Plain<32><<<grid, 256, 0, s>>>(k);
+        launch(k_assignF32Plain<32>, grid, 256, 0, s, t, k);
     } else {
-        k_assignF32Plain<16><<<grid, 256, 0, s>>>(k);
+        launch(k_assignF32Plain<16>, grid, 256, 0, s, t, k);
     }
-    RBOW_HIP(hipGetLastError());
+    return true;
 }
 
 namespace {
@@ -1264,15 +1284,18 @@ void launchRegroup(const KmLevel& k, uint32_t nbRows, const uint32_t* orderIn, u
     RBOW_HIP(hipGetLastError());
 }
 
-void launchKmUpdate(const KmLevel& k, hipStream_t s) {
-    if (!k.nbTasks) return;
+bool launchKmUpdate(const KmLevel& k, hipStream_t s, const KernelTimer& t) {
+    if (!k.nbTasks) return false;
     if (k.partSums && k.centroidType == 0) {
         const uint32_t n = k.updTasks ? k.nbUpdTasks : k.nbTasks;
-        if (n) k_updateU8Part<<<dim3(n, 16 * kDims / 256), 256, 0, s>>>(k);
+        if (!n) return false;
+        launch(k_updateU8Part, dim3(n, 16 * kDims / 256), 256, 0, s, t, k);
+    } else if (k.b == 32) {
+        launch(k_update<32>, k.nbTasks, 256, 0, s, t, k);
+    } else {
+        launch(k_update<16>, k.nbTasks, 256, 0, s, t, k);
     }
-    else if (k.b == 32) k_update<32><<<k.nbTasks, 256, 0, s>>>(k);
-    else k_update<16><<<k.nbTasks, 256, 0, s>>>(k);
-    RBOW_HIP(hipGetLastError());
+    return true;
 }
 
 }  // namespace rbow_amd

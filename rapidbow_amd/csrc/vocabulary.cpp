@@ -160,8 +160,10 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     const uint32_t ct = opts.centroid_type;
     const int kernel = int(opts.kernel);
     const auto t0 = std::chrono::steady_clock::now();
-    // per-launch timing (opts.timing): [0, 1] bracket the persistent launch; 2 + 4 * pass + i: before the
-    // assign, after it, after the reduce (only when one is launched), after the update
+    // per-launch timing (opts.timing), with HIP events recorded by the dispatches themselves: [0, 1] the
+    // start of the first persistent launch and the end of the last; 2 + 6 * pass + i: start / end of the
+    // assign (from a marker before the filter operands' preparation on the float path), of the reduce,
+    // of the update
     const bool timing = stats && opts.timing;
     std::vector<hipEvent_t> ev;
     struct EventsFree {
@@ -170,14 +172,16 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             for (auto e : v) (void)hipEventDestroy(e);
         }
     } evGuard{ev};
-    auto mark = [&](size_t i) {
+    auto evAt = [&](size_t i) {
         while (ev.size() <= i) {
             hipEvent_t e;
             RBOW_HIP(hipEventCreate(&e));
             ev.push_back(e);
         }
-        RBOW_HIP(hipEventRecord(ev[i], s));
+        return ev[i];
     };
+    auto timer = [&](size_t i) { return timing ? KernelTimer{evAt(i), evAt(i + 1)} : KernelTimer{}; };
+    std::vector<uint8_t> ranAssign, ranUpdate;  // per pass of the current level
     double assignMs = 0, updateMs = 0, reduceMs = 0, streamMs = 0, persistTotalMs = 0;
     uint32_t streamLaunches = 0;
     uint64_t streamPointPasses = 0;
@@ -385,12 +389,14 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
 
         // ---- Lloyd passes (kmeans.cpp:118-133) ----
         if (!smallTasks.empty()) {
-            if (timing) mark(0);
-            // two LDS classes: tasks that allow two workgroups per CU run in a launch of their own
-            launchLloydTasks(k, dSmall, nbLarger, smallTilesMax, 1, dTileBeg, opts.max_iters, s);
-            launchLloydTasks(k, dSmall + nbLarger, uint32_t(smallTasks.size()) - nbLarger,
-                             std::min(smallTilesMax, persistTiles2), 2, dTileBeg, opts.max_iters, s);
-            if (timing) mark(1);
+            // two LDS classes: tasks that allow two workgroups per CU run in a launch of their own;
+            // timed from the start of the first launch to the end of the second
+            const uint32_t nbSmaller = uint32_t(smallTasks.size()) - nbLarger;
+            const KernelTimer t = timer(0);
+            launchLloydTasks(k, dSmall, nbLarger, smallTilesMax, 1, dTileBeg, opts.max_iters, s,
+                             KernelTimer{t.start, nbSmaller ? nullptr : t.stop});
+            launchLloydTasks(k, dSmall + nbLarger, nbSmaller, std::min(smallTilesMax, persistTiles2), 2, dTileBeg,
+                             opts.max_iters, s, KernelTimer{nbLarger ? nullptr : t.start, t.stop});
         }
         // with no streamed task left, one update pass retires the empty tasks
         const bool streamed = !mfmaPath || k.nbWaves > 0;
@@ -398,21 +404,21 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         const bool poll = iters > 16;
         uint32_t nbLaunched = 0;
         const bool reduced = mfmaPath && !red.empty();
+        ranAssign.assign(iters, 0);
+        ranUpdate.assign(iters, 0);
         for (uint32_t it = 0; it < iters; it++) {
-            const size_t e = 2 + 4 * size_t(it);
-            if (timing) mark(e);
-            if (f32Mfma)  // the filter's operands of this pass's centers
+            const size_t e = 2 + 6 * size_t(it);
+            KernelTimer ta = timer(e);
+            if (f32Mfma) {  // the filter's operands of this pass's centers, timed with the assign
+                if (timing) RBOW_HIP(hipEventRecord(ta.start, s));
+                ta.start = nullptr;
                 launchPrepF32Tree(k.centersF32, nG, b, dFMu.get(), dFHi.get(), dFLo.get(), dFCn2.get(), dFE1.get(),
                                   dFE2.get(), s, k.active);
-            launchKmAssign(k, kernel, s);
-            if (timing) mark(e + 1);
-            if (reduced) {
-                launchKmReduce(k, dRed, uint32_t(red.size()), s);
-                if (timing) mark(e + 2);
             }
+            ranAssign[it] = launchKmAssign(k, kernel, s, ta);
+            if (reduced) launchKmReduce(k, dRed, uint32_t(red.size()), s, timer(e + 2));
             if (poll) launchFillU32(dNbActive.get(), 0u, 1, s);
-            launchKmUpdate(k, s);
-            if (timing) mark(e + 3);
+            ranUpdate[it] = launchKmUpdate(k, s, timer(e + 4));
             nbLaunched = it + 1;
             if (poll && (it % 4 == 3)) {  // kmeans.cpp:125-131 polls every 4 passes
                 RBOW_HIP(hipMemcpyAsync(hNbActive, dNbActive.get(), 4, hipMemcpyDeviceToHost, s));
@@ -444,11 +450,11 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             float pm = 0;
             if (!smallTasks.empty()) RBOW_HIP(hipEventElapsedTime(&pm, ev[0], ev[1]));
             for (uint32_t it = 0; it < nbLaunched; it++) {
-                const size_t e = 2 + 4 * size_t(it);
+                const size_t e = 2 + 6 * size_t(it);
                 float a = 0, r = 0, u = 0;
-                RBOW_HIP(hipEventElapsedTime(&a, ev[e], ev[e + 1]));
-                if (reduced) RBOW_HIP(hipEventElapsedTime(&r, ev[e + 1], ev[e + 2]));
-                RBOW_HIP(hipEventElapsedTime(&u, ev[reduced ? e + 2 : e + 1], ev[e + 3]));
+                if (ranAssign[it]) RBOW_HIP(hipEventElapsedTime(&a, ev[e], ev[e + 1]));
+                if (reduced) RBOW_HIP(hipEventElapsedTime(&r, ev[e + 2], ev[e + 3]));
+                if (ranUpdate[it]) RBOW_HIP(hipEventElapsedTime(&u, ev[e + 4], ev[e + 5]));
                 assignMs += a;
                 reduceMs += r;
                 updateMs += u;
