@@ -125,8 +125,8 @@ struct KmScratch {
         std::vector<uint32_t>& cc = part.taskPartCnt;
         std::vector<KmPartRange> hr;  // every task is reduced into the caller's totals
         for (uint32_t g = 0; g < k.nbTasks; g++)
-            for (uint32_t c = 0; c < cc[g]; c += 32)
-                hr.push_back(KmPartRange{g, cb[g] + c, cb[g] + std::min(cc[g], c + 32)});
+            for (uint32_t c = 0; c < cc[g]; c += kReduceItemParts)
+                hr.push_back(KmPartRange{g, cb[g] + c, cb[g] + std::min(cc[g], c + kReduceItemParts)});
         waveChunkBeg.resize(part.waveChunkBeg.size());
         RBOW_HIP(hipMemcpyAsync(waveChunkBeg.get(), part.waveChunkBeg.data(), 4 * part.waveChunkBeg.size(),
                                 hipMemcpyHostToDevice, s));

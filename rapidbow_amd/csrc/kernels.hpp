@@ -76,6 +76,12 @@ KmPartition partitionKmTiles(const std::vector<uint32_t>& taskTiles, const std::
 uint32_t kmAssignWavesPerBlock();
 uint32_t kmAssignMergeSlots();  // LDS merge slots per workgroup (KmPartition::blockSlots per block)
 
+// Partials of the MFMA assign: a task with at most kDirectParts of them is
+// reduced inside the update kernel; larger ones go through launchKmReduce in
+// items of at most kReduceItemParts partials.
+constexpr uint32_t kDirectParts = 32;
+constexpr uint32_t kReduceItemParts = 32;
+
 struct KmLevel {       // device state of one level's batched k-means
     uint32_t b = 16, nbTasks = 0, nbTiles = 0, centroidType = 0;
     const uint8_t* X = nullptr;      // gathered rows [N][128]

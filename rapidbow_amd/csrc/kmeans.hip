@@ -991,7 +991,7 @@ __global__ __launch_bounds__(256) void k_updateU8Part(KmLevel k) {
     const uint32_t y = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (!k.active[g]) return;
     const uint32_t cb = k.taskPartBeg[g], nc = k.taskPartCnt[g];
-    const bool direct = nc <= 32;
+    const bool direct = nc <= kDirectParts;
     uint32_t ch = 0;
     for (uint32_t c = tid; c < nc; c += 256) ch += k.partChanged[cb + c];
 #pragma unroll

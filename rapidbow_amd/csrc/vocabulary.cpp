@@ -210,7 +210,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     const uint64_t wavesMax = blocksMax * kmAssignWavesPerBlock();
     const uint64_t chunksMax = mfmaPath ? wavesMax + nGmax : 0;  // a wave's range splits at task boundaries
     const uint64_t slotsMax = blocksMax * kmAssignMergeSlots();
-    const uint64_t redMax = mfmaPath ? chunksMax / 32 + nGmax : 0;
+    const uint64_t redMax = mfmaPath ? chunksMax / kReduceItemParts + nGmax : 0;
     const uint32_t persistTiles = mfmaPath ? std::min(opts.persist_max_tiles, lloydTasksTileCap(1)) : 0u;
     const uint32_t persistTiles2 = mfmaPath ? std::min(persistTiles, lloydTasksTileCap(2)) : 0u;  // 2 per CU
     DevBuf<uint32_t> dOrder(nbDesc), dOrderNext(nbDesc);
@@ -321,10 +321,10 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             const KmPartition part = partitionKmTiles(taskTiles, tileBeg);
             red.clear();
             for (uint32_t g = 0; g < nG; g++)
-                if (part.taskPartCnt[g] > 32)  // tasks with few partials are reduced inside the update kernel
-                    for (uint32_t c = 0; c < part.taskPartCnt[g]; c += 32)
+                if (part.taskPartCnt[g] > kDirectParts)  // tasks with few partials are reduced inside the update kernel
+                    for (uint32_t c = 0; c < part.taskPartCnt[g]; c += kReduceItemParts)
                         red.push_back(KmPartRange{g, part.taskPartBeg[g] + c,
-                                                  part.taskPartBeg[g] + std::min(part.taskPartCnt[g], c + 32)});
+                                                  part.taskPartBeg[g] + std::min(part.taskPartCnt[g], c + kReduceItemParts)});
             RBOW_REQUIRE(part.chunks.size() <= chunksMax && part.blockSlots.size() <= slotsMax && red.size() <= redMax,
                          RBOW_ERR_HIP, "k-means work partition exceeds its bound");
             k.XA = dXA.get();
