@@ -6,8 +6,9 @@ descriptors, 10 Lloyd passes per node, k-means++ seeds, every node of a level
 in the same launches.  `value` is the k-means assign throughput in tera-ops/s
 (2*16*128 int8 multiply-adds per point per pass) of the I8-MFMA assign step
 (assign kernel + reduction of its per-chunk partials, and the persistent
-launches that run small tasks' whole Lloyd loops), from HIP events around those
-launches inside the timed builds.  The roofline object is the per-pass assign
+launches that run small tasks' whole Lloyd loops), from HIP events that those
+launches' dispatches stamp at kernel start and end inside the timed builds
+(build option timing=1).  The roofline object is the per-pass assign
 kernel's: 129 algorithmic bytes per point-pass (128 in + 1 assignment) over its
 average launch time, against HBM (its intensity, 32 int8 ops per byte, is far
 below the I8 MFMA ridge of ~625 ops/byte).
