@@ -321,7 +321,7 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=1.5)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding is design-only (see DESIGN.md)",
+        print("bench.py: this library runs on one GPU only; sharding over several GPUs is not implemented",
               file=sys.stderr)
         sys.exit(2)
 

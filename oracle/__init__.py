@@ -3,8 +3,10 @@
 TEST INFRASTRUCTURE ONLY.  Used by tests/, the smoke check and the CPU
 baseline leg of bench.py as the checker; the product never imports it.
 
-Parity status of each oracle function (how it is pinned) is listed in
-DESIGN.md; every C function cites the reference file:line it restates.
+Each C function cites the reference file:line it restates; the tests that
+pin it are the golden fixtures (tests/golden), the known-answer and
+closed-form checks of tests/test_oracle.py and the brute-force restatements
+in tests/helpers.py.
 """
 import ctypes
 import os
