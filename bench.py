@@ -31,7 +31,14 @@ Beside it on the same JSON line:
     arrivals at 200 / 1,000 / 5,000 QPS of queries with 300-6,000 descriptors,
     p50/p99 latency and queries/s of the continuous-batching server vs one
     query at a time with the plain kernels;
-  * the CPU baseline: the oracle, on the host cores, on the same build.
+  * the CPU baseline: the oracle, on the host cores, on the same build (median
+    of 5 runs), whose tree is also compared with the timed GPU build
+    (tree_equals_oracle); with it the error report: centroids vs the binary64
+    means of their points, lookup distances vs binary64, scores vs the FP64
+    ideal;
+  * the queryAndAddDoc loop (query, then insert) on the 100k- and 1M-image
+    files: per-call latency, which the delta segment keeps independent of the
+    file's size, and the cost of folding the delta into the main CSR.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                        [--skip-retrieval] [--skip-serving]
@@ -54,7 +61,7 @@ HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, 8 TB/s
 LOOKUP_BYTES = 132        # algorithmic bytes per descriptor looked up: 128 in + 4 out
 ASSIGN_BYTES = 129        # algorithmic bytes per point and Lloyd pass: 128 in + 1 assignment
 COPY_BYTES = 1 << 30      # device-to-device copy that calibrates the achievable HBM rate
-CPU_THREADS = 16          # host cores given to the CPU legs (the GPU box's share for one GPU)
+CPU_RUNS = 5              # timed runs of the CPU baseline (median reported)
 
 # seeds of the synthetic inputs
 SEED_TRAIN, SEED_QUERY, SEED_IMAGES, SEED_ARRIVALS = 1, 2, 3, 4
@@ -103,18 +110,94 @@ def pmc_traffic():
     return None
 
 
+def host_info():
+    """The host the CPU legs ran on: logical CPUs, the ones this process may use, the model."""
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0)), "model": model}
+
+
+def cpu_threads():
+    """Every host core this process may use (capped by OMP_NUM_THREADS when set)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
+
+
 def cpu_baseline(rb, oracle, desc, threads):
-    """Oracle build of the same workload (b16 L4, 10 passes, k-means++) on the host cores."""
+    """Oracle build of the same workload (b16 L4, 10 passes, k-means++) on the host cores:
+    one warm-up run, then the median of CPU_RUNS timed runs."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     n = desc.shape[0]
-    t = time.perf_counter()
     res = oracle.build(desc, 1000, 4, max_iters=10, seed_mode=1)
-    dt = time.perf_counter() - t
+    times = []
+    for _ in range(CPU_RUNS):
+        t = time.perf_counter()
+        oracle.build(desc, 1000, 4, max_iters=10, seed_mode=1)
+        times.append(time.perf_counter() - t)
+    dt = float(np.median(times))
     sizes = np.concatenate([[n], res["occ"].reshape(-1)[: res["passes"].size - 1]])
     ops = float((sizes.astype(np.float64) * res["passes"]).sum()) * 2 * 16 * 128
     return res, {"value": round(ops / dt / 1e12, 5), "unit": "TOPS", "cores": threads, "kind": "port",
+                 "runs": CPU_RUNS, "run_s": [round(x, 3) for x in times], "host": host_info(),
                  "sample": f"oracle hierarchical k-means b16 L4 on the same {n} descriptors, 10 passes, "
-                           f"k-means++ ({dt:.1f} s wall; includes the host regroup)"}
+                           f"k-means++, median of {CPU_RUNS} runs ({dt:.2f} s wall each; includes the host "
+                           f"regroup)"}
+
+
+def tree_vs_oracle_u8(voc, res):
+    """Nodes, masks, IDF weights and pass counts of a uint8 b16 tree against the oracle's build."""
+    blob = voc.serialize()
+    n = res["nodes"].shape[0]
+    nodes = np.frombuffer(blob, np.uint8, n * 2048, 4).reshape(n, 2048)
+    masks = np.frombuffer(blob, np.uint16, n, 4 + n * 2048)
+    w = np.frombuffer(blob, np.float32, n * 16, 4 + n * 2050).reshape(n, 16)
+    out = {"nodes": bool(np.array_equal(nodes, res["nodes"])),
+           "masks": bool(np.array_equal(masks, res["masks"].astype(np.uint16))),
+           "weights": bool(np.array_equal(w.view(np.uint32), res["weights"].view(np.uint32))),
+           "passes": bool(np.array_equal(voc.build_passes(), res["passes"])),
+           "differing_nodes": int((nodes != res["nodes"]).any(1).sum())}
+    out["all"] = out["nodes"] and out["masks"] and out["weights"] and out["passes"]
+    return out
+
+
+def tree_vs_oracle_f32(voc, res, b, L):
+    out = {"differing_nodes": 0}
+    for l in range(L):
+        b0 = (b ** l - 1) // (b - 1)
+        out["differing_nodes"] += int((voc.centroids(l) != res["nodes"][b0:b0 + b ** l]).any(axis=(1, 2)).sum())
+    out["nodes"] = out["differing_nodes"] == 0
+    out["weights"] = bool(np.array_equal(voc.weights().reshape(-1, b).view(np.uint32),
+                                         res["weights"].view(np.uint32)))
+    out["passes"] = bool(np.array_equal(voc.build_passes(), res["passes"]))
+    out["all"] = out["nodes"] and out["weights"] and out["passes"]
+    return out
+
+
+def query_and_add_loop(db, leaf_sets, first_doc, K, odb=None, check=0):
+    """queryAndAddDoc over new documents: per-call latency; the first `check` calls against the oracle."""
+    lat, same = [], 0
+    for i, lf in enumerate(leaf_sets):
+        t = time.perf_counter()
+        got = db.query_and_add(first_doc + i, lf, K)
+        lat.append((time.perf_counter() - t) * 1e3)
+        if i < check:
+            od, _ = odb.query(lf, K)
+            odb.add(first_doc + i, lf)
+            same += int(np.array_equal(got, od))
+    lat = np.array(lat)
+    out = {"calls": len(leaf_sets), "postings_per_call_mean": round(float(np.mean([x.size for x in leaf_sets])), 1),
+           "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+           "delta_postings_after": int(db.nb_delta_postings)}
+    if check:
+        out["ranked_equal_oracle"] = f"{same}/{check}"
+    return out
 
 
 def lognormal_counts(n, seed, median, sigma=0.5, lo=200, hi=8000):
@@ -228,6 +311,14 @@ def bench_retrieval(rb, syn, v32, with_cpu, threads):
                                "max_abs_score_err_vs_fp64": err}
         out["cpu_queries_per_s"] = {"value": round(len(qs) / cpu_s, 2), "cores": threads, "kind": "port",
                                     "sample": f"oracle query of {len(qs)} of the leaf-id queries, one per thread"}
+    # queryAndAddDoc (DataBase.cpp:192-197): the query images' leaves come back as new documents
+    ql = qleaf.numpy()
+    sets = [ql[int(qoff[i]):int(qoff[i + 1])].copy() for i in range(B * NB)][:200]
+    out["query_and_add"] = query_and_add_loop(db, sets, NI, K, odb if with_cpu else None, 20 if with_cpu else 0)
+    t = time.perf_counter()
+    db.compact()
+    out["query_and_add"]["compact_ms"] = round((time.perf_counter() - t) * 1e3, 2)
+    log(f"bench: queryAndAddDoc at {NI} images {out['query_and_add']}")
     return out
 
 
@@ -307,6 +398,17 @@ def bench_serving(rb, syn, v32, rates, duration_s):
             f"single p50 {s['p50_ms']} p99 {s['p99_ms']} {s['achieved_qps']} q/s")
     batched.close()
     single.close()
+    # queryAndAddDoc on the 1M-image file: new documents are re-observations of pool queries
+    lk = rb.DeviceArray((int(qoff[200]),), np.uint32)
+    qd = rb.DeviceArray.from_numpy(qhost[: int(qoff[200])])
+    v32.lookup(qd, int(qoff[200]), lk, rb.LOOKUP_MFMA)
+    hl = lk.numpy()
+    sets = [hl[int(qoff[i]):int(qoff[i + 1])].copy() for i in range(200)]
+    out["query_and_add"] = query_and_add_loop(db, sets, NI, K)
+    t = time.perf_counter()
+    db.compact()
+    out["query_and_add"]["compact_ms"] = round((time.perf_counter() - t) * 1e3, 2)
+    log(f"bench: queryAndAddDoc at {NI} images {out['query_and_add']}")
     return out
 
 
@@ -352,6 +454,7 @@ def main():
     step_ms = float(np.median([s["assign_ms"] + s["reduce_ms"] for s in stats]))
     ops = stats[-1]["assign_ops"]
     tops = ops / (step_ms * 1e-3) / 1e12
+    log(f"bench: assign step {step_ms:.3f} ms, {tops:.1f} TOPS")
     # roofline of the dominant kernel (the assign kernel alone): algorithmic bytes / its time
     st = min(stats, key=lambda x: x["stream_ms"])
     launch_us = st["stream_ms"] * 1e3 / st["assign_launches"]
@@ -436,6 +539,7 @@ def main():
                      "kernel": "k_assignU8Mfma", "avg_launch_us": round(launch_us, 2),
                      "algorithmic_bytes_per_launch": round(bytes_per_launch), "copy_gbs": round(copy_gbs, 1)},
         "assign_tops_frac_of_i8_peak": round(tops / I8_PEAK_TOPS, 5),
+        "build_tops_whole": round(ops / (float(np.median(prod_ms)) * 1e-3) / 1e12, 3),
         "assign_step_ms": {"assign": round(st["assign_ms"], 3), "of_which_persistent": round(st["persist_ms"], 3),
                            "reduce": round(st["reduce_ms"], 3), "update": round(st["update_ms"], 3),
                            "streamed_launches": st["assign_launches"]},
@@ -456,7 +560,7 @@ def main():
                                 "identical_leaf_ids": id1},
         "lookup_auto_crossover_tuned": {"b16L3_u8": cross_u8, "b32L4_f32": cross_f32},
     }
-    threads = CPU_THREADS
+    threads = cpu_threads()
     if not args.skip_retrieval:
         line["retrieval_100k_b32L4"] = bench_retrieval(rb, syn, v32, not args.no_cpu, threads)
         log(f"bench: retrieval {line['retrieval_100k_b32L4']}")
@@ -466,8 +570,14 @@ def main():
         import oracle
         log(f"bench: CPU baseline on {threads} threads")
         host = d.numpy()
-        _, cb = cpu_baseline(rb, oracle, host, threads)
+        res, cb = cpu_baseline(rb, oracle, host, threads)
         line["cpu_baseline"] = cb
+        line["tree_equals_oracle"] = tree_vs_oracle_u8(voc, res)
+        log(f"bench: timed b16 L4 build vs oracle: {line['tree_equals_oracle']}")
+        log("bench: oracle b32 L4 float build on the same descriptors")
+        res32 = oracle.build(host, 1000, 4, b=32, max_iters=10, seed_mode=1, centroid_type=1)
+        line["tree_equals_oracle_b32L4_f32"] = tree_vs_oracle_f32(v32, res32, 32, 4)
+        log(f"bench: b32 L4 float build vs oracle: {line['tree_equals_oracle_b32L4_f32']}")
         nq = qh.shape[0]
         tr = np.zeros((v32.nb_nodes, 32, 128), np.float32)
         for l in range(4):
@@ -479,6 +589,16 @@ def main():
         line["cpu_lookup_b32L4_f32"] = {"desc_per_s": round(nq / dt, 1), "cores": threads, "kind": "port",
                                         "sample": f"oracle descent of the first {nq} queries",
                                         "matches_gpu": bool(np.array_equal(lc, lm[:nq]))}
+        d32, d64 = oracle.path_dist_f32(tr, 32, 4, qh, lm[:nq], threads)
+        derr = np.abs(d32.astype(np.float64) - d64)
+        line["error_report_vs_fp64"] = {
+            "centroid_max_abs_b16L4_u8": res["centroid_err"],
+            "centroid_max_abs_b32L4_f32": res32["centroid_err"],
+            "lookup_distance_max_abs_b32L4_f32": float(derr.max()),
+            "lookup_distance_max_rel_b32L4_f32": float((derr / np.maximum(d64, 1.0)).max()),
+            "lookup_distance_sample": f"the winning child at each of the 4 levels of the first {nq} queries",
+            "score_max_abs": (line.get("retrieval_100k_b32L4", {}).get("oracle_check", {})
+                              .get("max_abs_score_err_vs_fp64"))}
     print(json.dumps(line), flush=True)
 
 

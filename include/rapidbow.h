@@ -151,7 +151,11 @@ rbow_status rbow_db_query_and_add_doc(rbow_database* db, uint32_t doc, const uin
                                       uint32_t max_results, uint32_t* out_docs, uint32_t* out_count);
 /* Batched forms (no counterpart in the reference, which has one host query
  * at a time): doc host_doc_ids[i] owns dev_leaf[host_offsets[i] ..
- * host_offsets[i+1]) (offsets are indices into dev_leaf). */
+ * host_offsets[i+1]) (offsets are indices into dev_leaf).  Every device leaf
+ * id must be below b^L: the batch insert checks them all on the device before
+ * it changes anything, and a query batch holding one out of range returns
+ * RBOW_ERR_INVALID_ARG "leaf id out of range" (its other queries' results are
+ * still written). */
 rbow_status rbow_db_add_docs_device(rbow_database* db, const uint32_t* host_doc_ids, uint32_t nb_docs,
                                     const uint32_t* dev_leaf, const uint64_t* host_offsets, rbow_stream stream);
 /* Scores nb_queries queries in one pass: results dev_docs[q][max_results],
@@ -161,6 +165,14 @@ rbow_status rbow_db_query_batch(const rbow_database* db, const uint32_t* dev_lea
                                 uint32_t* dev_counts, rbow_stream stream);
 uint32_t rbow_db_nb_docs(const rbow_database* db);
 uint64_t rbow_db_nb_postings(const rbow_database* db);
+/* Incremental inserts (no counterpart in the reference, whose addDoc inserts
+ * into sorted per-leaf vectors, DataBase.cpp:44-53): recent documents' postings
+ * live in a delta segment beside the main CSR and are scored from both; the
+ * delta is folded into the main CSR once it holds more than 2^24 postings.
+ * rbow_db_nb_delta_postings = postings not yet in the main CSR;
+ * rbow_db_compact folds them in now (blocks). */
+uint64_t rbow_db_nb_delta_postings(const rbow_database* db);
+rbow_status rbow_db_compact(rbow_database* db, rbow_stream stream);
 /* Posting list of one leaf as sorted doc ids (duplicates kept), the
  * reference's InvertedFile (DataBase.h:58-59); *length = full length. */
 rbow_status rbow_db_postings(const rbow_database* db, uint32_t leaf, uint32_t* host_out, uint32_t capacity,
@@ -198,11 +210,11 @@ rbow_status rbow_server_wait(rbow_server* srv, uint64_t ticket, uint32_t* docs, 
                              double* latency_ms);
 /* Load generator: submits query i (host_desc rows offsets[i]..offsets[i+1])
  * at Poisson arrival times of rate qps (seeded), then collects every result:
- * latency_ms[i], docs[i][K] and counts[i] (optional), batch_sizes[i]
- * (optional), elapsed_s = first arrival to last completion. */
+ * latency_ms[i], docs[i][K], scores[i][K] and counts[i] (optional),
+ * batch_sizes[i] (optional), elapsed_s = first arrival to last completion. */
 rbow_status rbow_server_replay(rbow_server* srv, const uint8_t* host_desc, const uint64_t* offsets,
                                uint32_t nb_queries, double qps, uint64_t seed, double* latency_ms, uint32_t* docs,
-                               uint32_t* counts, uint32_t* batch_sizes, double* elapsed_s);
+                               float* scores, uint32_t* counts, uint32_t* batch_sizes, double* elapsed_s);
 
 /* ------------------------------------------------ kernel-level entries -- */
 /* These expose the batched k-means kernels the build drives, one pass at a
@@ -232,6 +244,12 @@ rbow_status rbow_kmeans_update(const uint32_t* task_cnt, uint32_t nb_tasks, uint
  * descriptor, tile_size 32 or 16); idx 0xFFFFFFFF -> 0xFFFFFFFF. */
 rbow_status rbow_reformat_tiles(const void* dev_desc, const uint32_t* dev_idx, uint32_t nb_tiles, uint32_t tile_size,
                                 uint32_t* dev_tiles, rbow_stream stream);
+/* Vocabulary.cpp:103-108 IDF weights, as the build computes them on the device:
+ * dev_weights[i] = dev_occ[i] == 0 ? FLT_MIN
+ *                : max(FLT_MIN, float(log(double(float(nb_doc) / float(dev_occ[i])))))
+ * (binary32 quotient, binary64 logarithm).  Asynchronous on stream. */
+rbow_status rbow_idf_weights(const uint32_t* dev_occ, uint64_t n, uint32_t nb_doc, float* dev_weights,
+                             rbow_stream stream);
 /* Row gather used by the build: dev_out[p] = desc[idx[p]] (invalid -> 0xFF). */
 rbow_status rbow_gather_rows(const void* dev_desc, const uint32_t* dev_idx, uint64_t n, uint8_t* dev_out,
                              rbow_stream stream);

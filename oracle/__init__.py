@@ -43,7 +43,9 @@ _sig("orc_rng_next", _u64, [_vp])
 _sig("orc_rand_pick", _u32, [_vp, _vp, _u32, _u32])
 _sig("orc_kmeanspp_pick", _u32, [_vp, _vp, _u32, _u32, _u64, _u32, _u32, _vp])
 _sig("orc_hash", _u64, [_u64, _u64, _u64, _u64])
-_sig("orc_build", _i32, [_vp, _u32, _u32, _u32, _u32, _u32, _i32, _u64, _i32, _vp, _vp, _vp, _vp, _vp])
+_sig("orc_build", _i32, [_vp, _u32, _u32, _u32, _u32, _u32, _i32, _u64, _i32, _vp, _vp, _vp, _vp, _vp, _vp])
+_sig("orc_idf_weights", None, [_vp, _u64, _u32, _vp])
+_sig("orc_path_dist_f32", None, [_vp, _u32, _u32, _vp, _u64, _vp, _vp, _vp, _i32])
 _sig("orc_db_new", _vp, [_u32, _u32, _vp])
 _sig("orc_db_free", None, [_vp])
 _sig("orc_db_add", _i32, [_vp, _u32, _vp, _u32])
@@ -168,8 +170,8 @@ def hash4(seed, a, b, c):
 
 
 def build(desc, nb_doc, L, b=16, max_iters=1000, seed_mode=0, seed=5489, centroid_type=0):
-    """Returns dict(nodes, masks, weights, occ, passes); u8 nodes in the reference layout
-    [node][2048], float nodes [node][b][128]."""
+    """Returns dict(nodes, masks, weights, occ, passes, centroid_err); u8 nodes in the reference
+    layout [node][2048], float nodes [node][b][128]; centroid_err = max |centroid - binary64 mean|."""
     desc = _c(desc, np.uint8)
     n = nb_nodes(b, L)
     nodes = np.zeros((n, 2048), np.uint8) if centroid_type == 0 else np.zeros((n, b, 128), np.float32)
@@ -177,12 +179,32 @@ def build(desc, nb_doc, L, b=16, max_iters=1000, seed_mode=0, seed=5489, centroi
     weights = np.zeros((n, b), np.float32)
     occ = np.zeros((n, b), np.uint32)
     passes = np.zeros(n, np.uint32)
+    err = ctypes.c_double(0)
     r = lib.orc_build(desc.ctypes.data, desc.shape[0], nb_doc, b, L, max_iters, seed_mode, seed, centroid_type,
                       nodes.ctypes.data, masks.ctypes.data, weights.ctypes.data, occ.ctypes.data,
-                      passes.ctypes.data)
+                      passes.ctypes.data, ctypes.byref(err))
     if r != 0:
         raise ValueError("orc_build rejected its arguments")
-    return dict(nodes=nodes, masks=masks, weights=weights, occ=occ, passes=passes)
+    return dict(nodes=nodes, masks=masks, weights=weights, occ=occ, passes=passes, centroid_err=err.value)
+
+
+def idf_weights(occ, nb_doc):
+    """Vocabulary.cpp:103-108 weights of the given occurrence counts."""
+    occ = _c(occ, np.uint32)
+    w = np.zeros(occ.size, np.float32)
+    lib.orc_idf_weights(occ.ctypes.data, occ.size, nb_doc, w.ctypes.data)
+    return w
+
+
+def path_dist_f32(nodes, b, L, desc, leaf, nthreads=0):
+    """(d32, d64) [n][L]: FP32-chain and binary64 distances to the child on each descriptor's path."""
+    nodes, desc, leaf = _c(nodes, np.float32), _c(desc, np.uint8), _c(leaf, np.uint32)
+    n = desc.shape[0]
+    d32 = np.zeros((n, L), np.float32)
+    d64 = np.zeros((n, L), np.float64)
+    lib.orc_path_dist_f32(nodes.ctypes.data, b, L, desc.ctypes.data, n, leaf.ctypes.data, d32.ctypes.data,
+                          d64.ctypes.data, nthreads)
+    return d32, d64
 
 
 def ref_nodes_to_rows(nodes_ref):

@@ -289,7 +289,7 @@ uint32_t orc_kmeanspp_pick(const uint8_t* desc, const uint32_t* grpIdx, uint32_t
 
 int orc_build(const uint8_t* desc, uint32_t nbDesc, uint32_t nbDoc, uint32_t b, uint32_t nbLevels, uint32_t maxIters,
               int seedMode, uint64_t seed, int centroidType, void* nodesOut, uint32_t* masks, float* weights,
-              uint32_t* occ, uint32_t* lloydPasses) {
+              uint32_t* occ, uint32_t* lloydPasses, double* centroidErr) {
     if (centroidType == 0 && b != 16) return -1;
     if (centroidType == 1 && b != 16 && b != 32) return -1;
     if (nbLevels == 0 || maxIters == 0 || (seedMode != 0 && seedMode != 1)) return -1;
@@ -300,6 +300,7 @@ int orc_build(const uint8_t* desc, uint32_t nbDesc, uint32_t nbDoc, uint32_t b, 
     std::mt19937_64 rng(seed);
 
     std::vector<std::vector<uint32_t>> groups{std::vector<uint32_t>(nbDesc)};  // Vocabulary.cpp:51-52
+    double maxErr = 0;  // max |centroid - binary64 mean of its points| over children with points
     std::iota(groups[0].begin(), groups[0].end(), 0u);
 
     for (uint32_t level = 0; level < nbLevels; level++) {
@@ -334,6 +335,7 @@ int orc_build(const uint8_t* desc, uint32_t nbDesc, uint32_t nbDoc, uint32_t b, 
         }
         // ---- Lloyd loop per group (kmeans.cpp:118-133 with per-task stop) ----
         std::vector<std::vector<uint8_t>> assigns(nG);
+        std::vector<double> taskErr(nG, 0.0);
 #pragma omp parallel for schedule(dynamic, 1)
         for (int64_t gi = 0; gi < int64_t(nG); gi++) {
             const uint32_t g = uint32_t(gi);
@@ -360,6 +362,11 @@ int orc_build(const uint8_t* desc, uint32_t nbDesc, uint32_t nbDoc, uint32_t b, 
                     orc_update_u8(c, counts.data(), sums.data());
                 }
                 rowsToRefNode(c, static_cast<uint8_t*>(nodesOut) + (nodeBase + g) * 2048);
+                // the final centers are the update of the final assignment, whose totals the last pass left
+                for (uint32_t j = 0; j < 16; j++)
+                    for (uint32_t d = 0; counts[j] && d < kDims; d++)
+                        taskErr[g] = std::max(taskErr[g], std::fabs(double(c[j * kDims + d]) -
+                                                                    double(sums[j * kDims + d]) / counts[j]));
             } else {
                 std::vector<float> c(size_t(b) * kDims);
                 for (uint32_t j = 0; j < b; j++) {
@@ -376,9 +383,14 @@ int orc_build(const uint8_t* desc, uint32_t nbDesc, uint32_t nbDoc, uint32_t b, 
                 }
                 std::memcpy(static_cast<float*>(nodesOut) + (nodeBase + g) * b * kDims, c.data(),
                             sizeof(float) * b * kDims);
+                for (uint32_t j = 0; j < b; j++)
+                    for (uint32_t d = 0; counts[j] && d < kDims; d++)
+                        taskErr[g] = std::max(taskErr[g], std::fabs(double(c[j * kDims + d]) -
+                                                                    double(sums[j * kDims + d]) / counts[j]));
             }
             if (lloydPasses) lloydPasses[nodeBase + g] = passes;
         }
+        for (double e : taskErr) maxErr = std::max(maxErr, e);
         // ---- stable regroup + occurrence (Vocabulary.cpp:83-98) ----
         std::vector<std::vector<uint32_t>> next(size_t(nG) * b);
         for (uint32_t g = 0; g < nG; g++) {
@@ -387,9 +399,15 @@ int orc_build(const uint8_t* desc, uint32_t nbDesc, uint32_t nbDoc, uint32_t b, 
         }
         groups.swap(next);
     }
-    // ---- IDF weights (Vocabulary.cpp:103-108); log evaluated in binary64 of the
-    // binary32 quotient, then rounded to binary32 (builder rule for std::log(float)).
-    for (uint64_t i = 0; i < nbNodes * b; i++) {
+    orc_idf_weights(occ, nbNodes * b, nbDoc, weights);
+    if (centroidErr) *centroidErr = maxErr;
+    return 0;
+}
+
+// Vocabulary.cpp:103-108 IDF weights; log evaluated in binary64 of the binary32
+// quotient, then rounded to binary32 (builder rule for std::log(float)).
+void orc_idf_weights(const uint32_t* occ, uint64_t n, uint32_t nbDoc, float* weights) {
+    for (uint64_t i = 0; i < n; i++) {
         const uint32_t o = occ[i];
         if (o == 0) {
             weights[i] = FLT_MIN;
@@ -398,7 +416,31 @@ int orc_build(const uint8_t* desc, uint32_t nbDesc, uint32_t nbDoc, uint32_t b, 
             weights[i] = std::max(FLT_MIN, float(std::log(double(q))));
         }
     }
-    return 0;
+}
+
+// Distances along each descriptor's path: for level l, the child of the path's
+// level-l node that the leaf id names; d32 = orc_dist_f32 (the FP32 chain),
+// d64 = the same sum in binary64 of the binary32 centroid values.
+void orc_path_dist_f32(const float* nodes, uint32_t b, uint32_t nbLevels, const uint8_t* desc, uint64_t nbDesc,
+                       const uint32_t* leaf, float* d32, double* d64, int nthreads) {
+    uint32_t lg2b = 0;
+    while ((1u << lg2b) < b) lg2b++;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+    for (int64_t i = 0; i < int64_t(nbDesc); i++) {
+        const uint8_t* x = desc + uint64_t(i) * 128;
+        for (uint32_t l = 0; l < nbLevels; l++) {
+            const uint64_t path = leaf[i] >> (lg2b * (nbLevels - 1 - l));  // node index within level l+1
+            const uint64_t node = orc_level_beg(b, l) + (path >> lg2b);
+            const float* c = nodes + (node * b + (path & (b - 1))) * 128;
+            double s = 0;
+            for (int k = 0; k < 128; k++) {
+                const double d = double(x[k]) - double(c[k]);
+                s += d * d;
+            }
+            d32[uint64_t(i) * nbLevels + l] = orc_dist_f32(x, c);
+            d64[uint64_t(i) * nbLevels + l] = s;
+        }
+    }
 }
 
 }  // extern "C"

@@ -78,6 +78,8 @@ uint32_t orc_kmeanspp_pick(const uint8_t* desc, const uint32_t* grpIdx, uint32_t
 uint64_t orc_hash(uint64_t seed, uint64_t a, uint64_t b, uint64_t c);
 
 /* Vocabulary.cpp:32-121 + kmeans.cpp:27-224: hierarchical k-means build.
+ * centroidErr (optional): max over children with points and over dims of
+ * |final centroid - binary64 mean of the child's points| (the error report).
  *   centroidType 0: uint8 centroids (b must be 16); nodes in reference layout.
  *   centroidType 1: float centroids [node][b][128]; b = 16 or 32.
  *   seedMode 0: reference randPick with a fresh std::mt19937_64(seed)
@@ -89,7 +91,16 @@ uint64_t orc_hash(uint64_t seed, uint64_t a, uint64_t b, uint64_t c);
  * reference's input errors (nbDesc < b, nbDoc == 0) or bad arguments. */
 int orc_build(const uint8_t* desc, uint32_t nbDesc, uint32_t nbDoc, uint32_t b, uint32_t nbLevels,
               uint32_t maxIters, int seedMode, uint64_t seed, int centroidType, void* nodesOut,
-              uint32_t* masks, float* weights, uint32_t* occ, uint32_t* lloydPasses);
+              uint32_t* masks, float* weights, uint32_t* occ, uint32_t* lloydPasses, double* centroidErr);
+
+/* Vocabulary.cpp:103-108 IDF weights from occurrence counts:
+ * occ == 0 ? FLT_MIN : max(FLT_MIN, float(log(double(float(nbDoc) / float(occ))))). */
+void orc_idf_weights(const uint32_t* occ, uint64_t n, uint32_t nbDoc, float* weights);
+/* Error report along lookup paths on a float tree: for descriptor i and level
+ * l, the child on leaf[i]'s path: d32[i][l] = orc_dist_f32, d64[i][l] = the
+ * binary64 sum of squared differences. */
+void orc_path_dist_f32(const float* nodes, uint32_t b, uint32_t nbLevels, const uint8_t* desc, uint64_t nbDesc,
+                       const uint32_t* leaf, float* d32, double* d64, int nthreads);
 
 /* DataBase.cpp:39-197 inverted-file database. lg2b = log2(branch factor). */
 void* orc_db_new(uint32_t b, uint32_t nbLevels, const float* weights);

@@ -94,6 +94,9 @@ _sig("rbow_db_add_docs_device", _i32, [_vp, _vp, _u32, _vp, _vp, _vp])
 _sig("rbow_db_query_batch", _i32, [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp])
 _sig("rbow_db_nb_docs", _u32, [_vp])
 _sig("rbow_db_nb_postings", _u64, [_vp])
+_sig("rbow_db_nb_delta_postings", _u64, [_vp])
+_sig("rbow_db_compact", _i32, [_vp, _vp])
+_sig("rbow_idf_weights", _i32, [_vp, _u64, _u32, _vp, _vp])
 _sig("rbow_db_postings", _i32, [_vp, _u32, _vp, _u32, ctypes.POINTER(_u32)])
 _sig("rbow_db_doc_rcp", _i32, [_vp, _u32, ctypes.POINTER(ctypes.c_float)])
 _sig("rbow_server_options_default", None, [ctypes.POINTER(ServerOptions)])
@@ -101,7 +104,7 @@ _sig("rbow_server_create", _i32, [_vp, _vp, ctypes.POINTER(ServerOptions), ctype
 _sig("rbow_server_destroy", None, [_vp])
 _sig("rbow_server_submit", _i32, [_vp, _vp, _u32, ctypes.POINTER(_u64)])
 _sig("rbow_server_wait", _i32, [_vp, _u64, _vp, _vp, ctypes.POINTER(_u32), ctypes.POINTER(ctypes.c_double)])
-_sig("rbow_server_replay", _i32, [_vp, _vp, _vp, _u32, ctypes.c_double, _u64, _vp, _vp, _vp, _vp,
+_sig("rbow_server_replay", _i32, [_vp, _vp, _vp, _u32, ctypes.c_double, _u64, _vp, _vp, _vp, _vp, _vp,
                                   ctypes.POINTER(ctypes.c_double)])
 _sig("rbow_synth_centres", None, [_u64, _u32, _vp])
 _sig("rbow_synth_descriptors_device", _i32, [_vp, _vp, _u32, _u64, _u64, _u64, _vp, _vp, _vp])
@@ -278,6 +281,10 @@ def reformat_tiles(dev_desc, dev_idx, nb_tiles, tile_size, dev_tiles, stream=Non
                                   _stream(stream)))
 
 
+def idf_weights(dev_occ, n, nb_doc, dev_weights, stream=None):
+    check(lib.rbow_idf_weights(_ptr(dev_occ), n, nb_doc, _ptr(dev_weights), _stream(stream)))
+
+
 def gather_rows(dev_desc, dev_idx, n, dev_out, stream=None):
     check(lib.rbow_gather_rows(_ptr(dev_desc), _ptr(dev_idx), n, _ptr(dev_out), _stream(stream)))
 
@@ -358,6 +365,15 @@ class DataBase:
                                 sc.ctypes.data, ctypes.byref(n)))
         return docs[:n.value], sc[:n.value]
 
+    def query_and_add(self, doc, leaf, K):
+        """queryAndAddDoc (RapidBoW.h:102-103): ranked doc ids of the query, then the insert."""
+        leaf = np.ascontiguousarray(leaf, np.uint32)
+        docs = np.zeros(max(K, 1), np.uint32)
+        n = ctypes.c_uint32()
+        check(lib.rbow_db_query_and_add_doc(self.handle, doc, leaf.ctypes.data if leaf.size else None, leaf.size, K,
+                                            docs.ctypes.data, ctypes.byref(n)))
+        return docs[:n.value]
+
     def add_docs_device(self, doc_ids, dev_leaf, offsets, stream=None):
         ids = np.ascontiguousarray(doc_ids, np.uint32)
         off = np.ascontiguousarray(offsets, np.uint64)
@@ -376,6 +392,13 @@ class DataBase:
     @property
     def nb_postings(self):
         return lib.rbow_db_nb_postings(self.handle)
+
+    @property
+    def nb_delta_postings(self):
+        return lib.rbow_db_nb_delta_postings(self.handle)
+
+    def compact(self, stream=None):
+        check(lib.rbow_db_compact(self.handle, _stream(stream)))
 
 
 class Server:
@@ -423,10 +446,11 @@ class Server:
         nq = off.size - 1
         lat = np.zeros(nq, np.float64)
         docs = np.zeros((nq, self.K), np.uint32)
+        scores = np.zeros((nq, self.K), np.float32)
         cnt = np.zeros(nq, np.uint32)
         bs = np.zeros(nq, np.uint32)
         el = ctypes.c_double()
         check(lib.rbow_server_replay(self.handle, desc.ctypes.data, off.ctypes.data, nq, qps, seed,
-                                     lat.ctypes.data, docs.ctypes.data, cnt.ctypes.data, bs.ctypes.data,
-                                     ctypes.byref(el)))
-        return dict(latency_ms=lat, docs=docs, counts=cnt, batch_sizes=bs, elapsed_s=el.value)
+                                     lat.ctypes.data, docs.ctypes.data, scores.ctypes.data, cnt.ctypes.data,
+                                     bs.ctypes.data, ctypes.byref(el)))
+        return dict(latency_ms=lat, docs=docs, scores=scores, counts=cnt, batch_sizes=bs, elapsed_s=el.value)

@@ -404,6 +404,15 @@ rbow_status rbow_gather_rows(const void* dev_desc, const uint32_t* dev_idx, uint
     });
 }
 
+rbow_status rbow_idf_weights(const uint32_t* dev_occ, uint64_t n, uint32_t nb_doc, float* dev_weights,
+                             rbow_stream stream) {
+    return guarded([&] {
+        RBOW_REQUIRE((dev_occ && dev_weights) || n == 0, RBOW_ERR_INVALID_ARG, "null argument");
+        RBOW_REQUIRE(nb_doc > 0, RBOW_ERR_NB_DOC, "nbDoc must be positive");
+        launchIdfWeights(dev_occ, n, nb_doc, dev_weights, hs(stream));
+    });
+}
+
 // ------------------------------------------------------------ database --
 
 rbow_status rbow_db_create(const rbow_vocabulary* voc, rbow_database** out) {
@@ -465,6 +474,14 @@ rbow_status rbow_db_query_batch(const rbow_database* db, const uint32_t* dev_lea
 
 uint32_t rbow_db_nb_docs(const rbow_database* db) { return db ? db->db->nbDocs() : 0u; }
 uint64_t rbow_db_nb_postings(const rbow_database* db) { return db ? db->db->nbPostings() : 0u; }
+uint64_t rbow_db_nb_delta_postings(const rbow_database* db) { return db ? db->db->nbDeltaPostings() : 0u; }
+
+rbow_status rbow_db_compact(rbow_database* db, rbow_stream stream) {
+    return guarded([&] {
+        RBOW_REQUIRE(db != nullptr, RBOW_ERR_INVALID_ARG, "null database");
+        db->db->compact(hs(stream));
+    });
+}
 
 rbow_status rbow_db_postings(const rbow_database* db, uint32_t leaf, uint32_t* host_out, uint32_t capacity,
                              uint32_t* length) {
@@ -539,7 +556,7 @@ rbow_status rbow_server_wait(rbow_server* srv, uint64_t ticket, uint32_t* docs, 
 
 rbow_status rbow_server_replay(rbow_server* srv, const uint8_t* host_desc, const uint64_t* offsets,
                                uint32_t nb_queries, double qps, uint64_t seed, double* latency_ms, uint32_t* docs,
-                               uint32_t* counts, uint32_t* batch_sizes, double* elapsed_s) {
+                               float* scores, uint32_t* counts, uint32_t* batch_sizes, double* elapsed_s) {
     return guarded([&] {
         RBOW_REQUIRE(srv && offsets && latency_ms && qps > 0, RBOW_ERR_INVALID_ARG, "bad argument");
         Server& S = *srv->srv;
@@ -573,6 +590,7 @@ rbow_status rbow_server_replay(rbow_server* srv, const uint8_t* host_desc, const
             last = std::max(last, r.doneMs);
             if (counts) counts[i] = uint32_t(r.docs.size());
             if (docs) std::copy(r.docs.begin(), r.docs.end(), docs + uint64_t(i) * K);
+            if (scores) std::copy(r.scores.begin(), r.scores.end(), scores + uint64_t(i) * K);
             if (batch_sizes) batch_sizes[i] = r.batchSize;
         }
         if (elapsed_s) *elapsed_s = nb_queries ? (last - first) * 1e-3 : 0.0;

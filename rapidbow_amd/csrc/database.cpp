@@ -9,6 +9,12 @@ namespace rbow_amd {
 
 namespace {
 inline float ftz(float x) { return std::fabs(x) < FLT_MIN ? std::copysign(0.0f, x) : x; }
+
+// capacity growth of reused device buffers
+template <typename T>
+void ensureCap(DevBuf<T>& b, size_t n) {
+    if (n > b.size()) b.resize(std::max(n, b.size() + b.size() / 2));
+}
 }  // namespace
 
 Database::Database(const Vocabulary& voc) : voc_(voc), b_(voc.b), lg2b_(voc.lg2b), L_(voc.L) {
@@ -19,9 +25,13 @@ Database::Database(const Vocabulary& voc) : voc_(voc), b_(voc.b), lg2b_(voc.lg2b
     for (auto& x : Wh_) x = ftz(x);
     W_.resize(Wh_.size());
     RBOW_HIP(hipMemcpy(W_.get(), Wh_.data(), Wh_.size() * 4, hipMemcpyHostToDevice));
-    leafOff_.resize(nbLeaves_ + 1);
-    RBOW_HIP(hipMemset(leafOff_.get(), 0, leafOff_.bytes()));
-    post_.resize(1);
+    for (DevCsr* c : {&main_, &delta_}) {
+        c->off.resize(nbLeaves_ + 1);
+        RBOW_HIP(hipMemset(c->off.get(), 0, c->off.bytes()));
+        c->post.resize(1);
+    }
+    spare_.off.resize(nbLeaves_ + 1);
+    flag_.resize(1);
     growDocs(1024, nullptr);
     RBOW_HIP(hipDeviceSynchronize());
 }
@@ -36,12 +46,17 @@ DbView Database::view() const {
     v.nbLeaves = nbLeaves_;
     v.nbDocs = docsOnDevice_;
     v.W = W_.get();
-    v.leafOff = leafOff_.get();
-    v.post = post_.get();
+    v.leafOff = main_.off.get();
+    v.post = main_.post.get();
+    v.leafOffD = delta_.off.get();
+    v.postD = delta_.post.get();
+    v.nbPostD = delta_.n;
+    v.slotD0 = mainSlots_;
     v.rcp = rcp_.get();
     v.docId = docId_.get();
     v.dir = dirStale_ ? nullptr : dir_.get();  // a stale directory is never handed to a kernel
-    v.R = dirR_;
+    v.Rm = dirR_;
+    v.R = divUp(docsOnDevice_, kRangeDocs);
     return v;
 }
 
@@ -86,7 +101,53 @@ void Database::addDoc(uint32_t doc, const uint32_t* leaf, uint32_t n) {
     }
 }
 
-void Database::ensureMerged(hipStream_t s) const {
+// delta <- delta ++ batch, leaf by leaf, into the spare buffers (whose capacity
+// is kept), then the two swap: O(delta + batch + leaves)
+void Database::addToDelta(DevCsr& batch, hipStream_t s) const {
+    if (batch.n == 0) return;
+    ensureCap(spare_.post, delta_.n + batch.n);
+    launchMergeCsr(nbLeaves_, delta_, batch, spare_, s);
+    std::swap(delta_, spare_);
+}
+
+void Database::foldPending(hipStream_t s) const {
+    if (pendLeaf_.empty()) {
+        foldedSlots_ = uint32_t(docIds_.size());
+        return;
+    }
+    const uint64_t n = pendLeaf_.size();
+    DevBuf<uint32_t> pairs(2 * n);
+    RBOW_HIP(hipMemcpyAsync(pairs.get(), pendLeaf_.data(), 4 * n, hipMemcpyHostToDevice, s));
+    RBOW_HIP(hipMemcpyAsync(pairs.get() + n, pendSlot_.data(), 4 * n, hipMemcpyHostToDevice, s));
+    DevCsr batch;
+    batch.off.resize(nbLeaves_ + 1);
+    batch.post.resize(n);
+    launchCsrFromPairs(nbLeaves_, pairs.get(), pairs.get() + n, n, batch, s);
+    addToDelta(batch, s);
+    RBOW_HIP(hipStreamSynchronize(s));  // the host lists and the batch buffers are released below
+    pendLeaf_.clear();
+    pendSlot_.clear();
+    foldedSlots_ = uint32_t(docIds_.size());
+}
+
+void Database::mergeDeltaIntoMain(hipStream_t s) const {
+    if (delta_.n == 0) {
+        mainSlots_ = foldedSlots_;
+        return;
+    }
+    DevCsr m;
+    m.off.resize(nbLeaves_ + 1);
+    m.post.resize(main_.n + delta_.n);
+    launchMergeCsr(nbLeaves_, main_, delta_, m, s);
+    RBOW_HIP(hipStreamSynchronize(s));  // no query still reads the old main CSR
+    main_ = std::move(m);
+    RBOW_HIP(hipMemsetAsync(delta_.off.get(), 0, delta_.off.bytes(), s));
+    delta_.n = 0;
+    mainSlots_ = foldedSlots_;
+    dirStale_ = true;
+}
+
+void Database::syncDevice(hipStream_t s) const {
     std::lock_guard<std::mutex> g(mergeMtx_);
     const uint32_t nDocs = uint32_t(docIds_.size());
     if (nDocs > docsOnDevice_) {
@@ -97,44 +158,37 @@ void Database::ensureMerged(hipStream_t s) const {
                                 4ull * (nDocs - docsOnDevice_), hipMemcpyHostToDevice, s));
         docsOnDevice_ = nDocs;
     }
-    if (!pendLeaf_.empty()) {
-        const uint64_t n = pendLeaf_.size();
-        DevBuf<uint32_t> l(n), sl(n);
-        RBOW_HIP(hipMemcpyAsync(l.get(), pendLeaf_.data(), 4 * n, hipMemcpyHostToDevice, s));
-        RBOW_HIP(hipMemcpyAsync(sl.get(), pendSlot_.data(), 4 * n, hipMemcpyHostToDevice, s));
-        mergeDevice(l.get(), sl.get(), n, s);
-        pendLeaf_.clear();
-        pendSlot_.clear();
-    }
+    foldPending(s);
+    if (delta_.n > kDeltaMax) mergeDeltaIntoMain(s);
+    if (delta_.n == 0) mainSlots_ = foldedSlots_;  // empty delta: its documents count as main ones
+    refreshDir(s);
+    RBOW_HIP(hipStreamSynchronize(s));
+}
+
+void Database::compact(hipStream_t s) {
+    std::unique_lock<std::shared_mutex> lk(mtx_);
+    syncDevice(s);
+    std::lock_guard<std::mutex> g(mergeMtx_);
+    mergeDeltaIntoMain(s);
     refreshDir(s);
     RBOW_HIP(hipStreamSynchronize(s));
 }
 
 void Database::refreshDir(hipStream_t s) const {
-    const uint32_t R = divUp(docsOnDevice_, kRangeDocs);
-    if (!dirStale_ && R == dirR_) return;
-    RBOW_REQUIRE(R == 0 || nbLeaves_ * (R + 1) < (1ull << 40), RBOW_ERR_CAPACITY, "range directory too large");
-    dir_.resize(std::max<uint64_t>(nbLeaves_ * (R + 1), 1));
-    dirR_ = R;
-    if (R) launchBuildDir(view(), dir_.get(), s);
+    const uint32_t Rm = divUp(mainSlots_, kRangeDocs);
+    if (!dirStale_ && Rm == dirR_) return;
+    RBOW_REQUIRE(Rm == 0 || nbLeaves_ * (Rm + 1) < (1ull << 40), RBOW_ERR_CAPACITY, "range directory too large");
+    dir_.resize(std::max<uint64_t>(nbLeaves_ * (Rm + 1), 1));
+    dirR_ = Rm;
     dirStale_ = false;
-}
-
-void Database::mergeDevice(const uint32_t* dLeaf, const uint32_t* dSlot, uint64_t n, hipStream_t s) const {
-    DevBuf<uint64_t> off(nbLeaves_ + 1);
-    DevBuf<uint32_t> post(nbPost_ + n);
-    launchMergePostings(view(), nbPost_, dLeaf, dSlot, n, off.get(), post.get(), s);
-    RBOW_HIP(hipDeviceSynchronize());  // no query still reads the old CSR
-    leafOff_ = std::move(off);
-    post_ = std::move(post);
-    nbPost_ += n;
-    dirStale_ = true;
+    if (Rm) launchBuildDir(view(), dir_.get(), s);
 }
 
 void Database::addDocsDevice(const uint32_t* docs, uint32_t nbDocs, const uint32_t* dLeaf, const uint64_t* offsets,
                              hipStream_t s) {
     std::unique_lock<std::shared_mutex> lk(mtx_);
-    RBOW_REQUIRE(docs && offsets, RBOW_ERR_INVALID_ARG, "null argument");
+    RBOW_REQUIRE((docs && offsets) || nbDocs == 0, RBOW_ERR_INVALID_ARG, "null argument");
+    if (nbDocs == 0) return;
     {
         std::unordered_map<uint32_t, uint32_t> seen;
         for (uint32_t d = 0; d < nbDocs; d++) {
@@ -143,14 +197,26 @@ void Database::addDocsDevice(const uint32_t* docs, uint32_t nbDocs, const uint32
                          "Duplicate document index");
         }
     }
-    ensureMerged(s);  // host-added postings keep their (smaller) slots first in every leaf
+    const uint64_t total = offsets[nbDocs] - offsets[0];
+    RBOW_REQUIRE(dLeaf != nullptr || total == 0, RBOW_ERR_INVALID_ARG, "null leaf ids");
+    // every leaf id is checked on the device before anything reads through it
+    if (total) {
+        RBOW_HIP(hipMemsetAsync(flag_.get(), 0, 4, s));
+        launchCountBadLeaves(dLeaf + offsets[0], total, nbLeaves_, flag_.get(), s);
+        uint32_t bad = 0;
+        RBOW_HIP(hipMemcpyAsync(&bad, flag_.get(), 4, hipMemcpyDeviceToHost, s));
+        RBOW_HIP(hipStreamSynchronize(s));
+        RBOW_REQUIRE(bad == 0, RBOW_ERR_INVALID_ARG, "leaf id out of range");
+    }
+    syncDevice(s);  // host-added postings keep their (smaller) slots first in every leaf
+    std::lock_guard<std::mutex> g(mergeMtx_);
     const uint32_t first = uint32_t(docIds_.size());
     for (uint32_t d = 0; d < nbDocs; d++) {
         slotOf_[docs[d]] = first + d;
         docIds_.push_back(docs[d]);
     }
     growDocs(first + nbDocs, s);
-    // norms on the device, chunked merges of <= 2^30 postings
+    // norms on the device, then the postings in chunks of <= 2^30 through the delta
     DevBuf<uint64_t> dOff(nbDocs + 1);
     RBOW_HIP(hipMemcpyAsync(dOff.get(), offsets, 8ull * (nbDocs + 1), hipMemcpyHostToDevice, s));
     DbView v = view();
@@ -168,10 +234,26 @@ void Database::addDocsDevice(const uint32_t* docs, uint32_t nbDocs, const uint32
         if (n) {
             DevBuf<uint32_t> slots(n);
             launchExpandSlots(dOff.get() + d0, d1 - d0, first + d0, slots.get(), s);
-            mergeDevice(dLeaf + offsets[d0], slots.get(), n, s);
+            DevCsr batch;
+            batch.off.resize(nbLeaves_ + 1);
+            batch.post.resize(n);
+            launchCsrFromPairs(nbLeaves_, dLeaf + offsets[d0], slots.get(), n, batch, s);
+            if (delta_.n == 0 && n > kDeltaMax) {  // a bulk chunk goes straight into the main CSR
+                foldedSlots_ = first + d1;
+                std::swap(delta_, batch);
+                mergeDeltaIntoMain(s);
+                std::swap(delta_, batch);  // the (zeroed) empty delta comes back
+            } else {
+                addToDelta(batch, s);
+                foldedSlots_ = first + d1;
+                if (delta_.n > kDeltaMax) mergeDeltaIntoMain(s);
+            }
+            RBOW_HIP(hipStreamSynchronize(s));  // the chunk's buffers are released here
         }
+        foldedSlots_ = first + d1;
         d0 = d1;
     }
+    if (delta_.n == 0) mainSlots_ = foldedSlots_;
     refreshDir(s);
     RBOW_HIP(hipStreamSynchronize(s));
 }
@@ -188,32 +270,39 @@ void Database::queryBatch(const uint32_t* dLeaf, const uint64_t* offsets, uint32
     if (nbQueries == 0) return;
     {
         std::unique_lock<std::shared_mutex> lk(mtx_);
-        ensureMerged(s);
+        syncDevice(s);
     }
     std::shared_lock<std::shared_mutex> lk(mtx_);  // DataBase.cpp:73: queries share the lock
     // rebase offsets so that the kernels index dLeaf from 0
     std::vector<uint64_t> h(nbQueries + 1);
     for (uint32_t q = 0; q <= nbQueries; q++) h[q] = offsets[q] - offsets[0];
+    RBOW_REQUIRE(dLeaf != nullptr || h[nbQueries] == 0, RBOW_ERR_INVALID_ARG, "null leaf ids");
     void* dOff = nullptr;
-    RBOW_HIP(hipMallocAsync(&dOff, 8ull * (nbQueries + 1), s));
+    RBOW_HIP(hipMallocAsync(&dOff, 8ull * (nbQueries + 1) + 8, s));
+    uint32_t* bad = reinterpret_cast<uint32_t*>(static_cast<uint64_t*>(dOff) + nbQueries + 1);
     RBOW_HIP(hipMemcpyAsync(dOff, h.data(), 8ull * (nbQueries + 1), hipMemcpyHostToDevice, s));
-    launchQueryBatch(view(), dLeaf + offsets[0], static_cast<uint64_t*>(dOff), h.data(), nbQueries, K, dDocs,
-                     dScores, dCounts, s);
+    RBOW_HIP(hipMemsetAsync(bad, 0, 4, s));
+    launchQueryBatch(view(), dLeaf ? dLeaf + offsets[0] : nullptr, static_cast<uint64_t*>(dOff), h.data(),
+                     nbQueries, K, dDocs, dScores, dCounts, bad, s);
+    uint32_t hBad = 0;
+    RBOW_HIP(hipMemcpyAsync(&hBad, bad, 4, hipMemcpyDeviceToHost, s));
     RBOW_HIP(hipFreeAsync(dOff, s));
     RBOW_HIP(hipStreamSynchronize(s));  // h (host) must outlive the copy; results ready on return
+    RBOW_REQUIRE(hBad == 0, RBOW_ERR_INVALID_ARG, "leaf id out of range");
 }
 
 void Database::prepareForQueries(hipStream_t s) const {
     std::unique_lock<std::shared_mutex> lk(mtx_);
-    ensureMerged(s);
+    syncDevice(s);
 }
 
 void Database::scoreAsync(const uint32_t* dLeaf, const uint64_t* dOffsets, const uint64_t* hOffsets,
                           uint32_t nbQueries, uint32_t K, uint32_t* dDocs, float* dScores, uint32_t* dCounts,
                           hipStream_t s) const {
     std::shared_lock<std::shared_mutex> lk(mtx_);
-    RBOW_REQUIRE(pendLeaf_.empty(), RBOW_ERR_INVALID_ARG, "database has unmerged inserts");
-    launchQueryBatch(view(), dLeaf, dOffsets, hOffsets, nbQueries, K, dDocs, dScores, dCounts, s);
+    RBOW_REQUIRE(pendLeaf_.empty() && docsOnDevice_ == docIds_.size(), RBOW_ERR_INVALID_ARG,
+                 "database has unmerged inserts");
+    launchQueryBatch(view(), dLeaf, dOffsets, hOffsets, nbQueries, K, dDocs, dScores, dCounts, nullptr, s);
 }
 
 uint32_t Database::query(const uint32_t* leaf, uint32_t n, uint32_t K, uint32_t* docs, float* scores) const {
@@ -241,7 +330,12 @@ uint32_t Database::nbDocs() const {
 
 uint64_t Database::nbPostings() const {
     std::shared_lock<std::shared_mutex> lk(mtx_);
-    return nbPost_ + pendLeaf_.size();
+    return main_.n + delta_.n + pendLeaf_.size();
+}
+
+uint64_t Database::nbDeltaPostings() const {
+    std::shared_lock<std::shared_mutex> lk(mtx_);
+    return delta_.n + pendLeaf_.size();
 }
 
 bool Database::docRcp(uint32_t doc, float* rcp) const {
@@ -256,14 +350,18 @@ uint32_t Database::postings(uint32_t leaf, uint32_t* out, uint32_t cap) const {
     RBOW_REQUIRE(leaf < nbLeaves_, RBOW_ERR_INVALID_ARG, "leaf id out of range");
     {
         std::unique_lock<std::shared_mutex> lk(mtx_);
-        ensureMerged(nullptr);
+        syncDevice(nullptr);
     }
     std::shared_lock<std::shared_mutex> lk(mtx_);
-    uint64_t o[2];
-    RBOW_HIP(hipMemcpy(o, leafOff_.get() + leaf, 16, hipMemcpyDeviceToHost));
-    std::vector<uint32_t> slots(o[1] - o[0]);
-    if (!slots.empty())
-        RBOW_HIP(hipMemcpy(slots.data(), post_.get() + o[0], 4 * slots.size(), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> slots;
+    for (const DevCsr* c : {&main_, &delta_}) {
+        uint64_t o[2];
+        RBOW_HIP(hipMemcpy(o, c->off.get() + leaf, 16, hipMemcpyDeviceToHost));
+        const size_t at = slots.size();
+        slots.resize(at + (o[1] - o[0]));
+        if (o[1] > o[0])
+            RBOW_HIP(hipMemcpy(slots.data() + at, c->post.get() + o[0], 4 * (o[1] - o[0]), hipMemcpyDeviceToHost));
+    }
     for (auto& x : slots) x = docIds_[x];
     std::sort(slots.begin(), slots.end());  // the reference keeps each list sorted by doc id
     if (out) std::copy_n(slots.begin(), std::min<size_t>(cap, slots.size()), out);

@@ -45,8 +45,10 @@ struct QNode {
     float qE;
     float w;
     uint32_t pad;
-    uint64_t a, e;   // posting slice
+    uint64_t a, e;   // posting slice in the main CSR
+    uint64_t aD, eD; // posting slice in the delta CSR
 };
+static_assert(sizeof(QNode) == Database::kQNodeBytes, "QNode size");
 
 __device__ inline double blockSumF64(double v, double* red) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -112,24 +114,26 @@ __global__ void k_hist(const uint32_t* __restrict__ leaf, uint64_t n, uint64_t* 
         atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[leaf[i]]), 1ull);
 }
 
-// new CSR = for each leaf: old postings, then the batch's postings in slot order
-__global__ void k_placeOld(const uint64_t* __restrict__ leafOff, uint64_t nbLeaves, const uint32_t* __restrict__ post,
-                           uint64_t nbOld, const uint64_t* __restrict__ batchOff, uint32_t* __restrict__ newPost) {
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nbOld;
-         i += uint64_t(gridDim.x) * blockDim.x) {
-        uint64_t lo = 0, hi = nbLeaves;  // leaf v with leafOff[v] <= i < leafOff[v+1]
-        while (hi - lo > 1) {
-            const uint64_t mid = (lo + hi) / 2;
-            if (leafOff[mid] <= i) lo = mid;
-            else hi = mid;
-        }
-        newPost[i + batchOff[lo]] = post[i];
+// out = a ++ b, leaf by leaf (one wave per leaf, coalesced copies); offC[x] =
+// offA[x] + offB[x] is written by k_newOff beforehand.
+__global__ __launch_bounds__(256) void k_mergeLeaves(const uint64_t* __restrict__ offA, const uint32_t* __restrict__ postA,
+                                                    const uint64_t* __restrict__ offB, const uint32_t* __restrict__ postB,
+                                                    uint64_t nbLeaves, const uint64_t* __restrict__ offC,
+                                                    uint32_t* __restrict__ postC) {
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint64_t x = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); x < nbLeaves; x += uint64_t(gridDim.x) * 4) {
+        const uint64_t a0 = offA[x], na = offA[x + 1] - a0, b0 = offB[x], nb = offB[x + 1] - b0;
+        uint32_t* dst = postC + offC[x];
+        for (uint64_t i = lane; i < na; i += 64) dst[i] = postA[a0 + i];
+        dst += na;
+        for (uint64_t i = lane; i < nb; i += 64) dst[i] = postB[b0 + i];
     }
 }
-__global__ void k_placeNew(const uint64_t* __restrict__ leafOff, const uint32_t* __restrict__ sortedLeaf,
-                           const uint32_t* __restrict__ sortedSlot, uint64_t n, uint32_t* __restrict__ newPost) {
-    for (uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += uint64_t(gridDim.x) * blockDim.x)
-        newPost[leafOff[sortedLeaf[k] + 1] + k] = sortedSlot[k];
+__global__ void k_countBad(const uint32_t* __restrict__ leaf, uint64_t n, uint64_t nbLeaves, uint32_t* __restrict__ flag) {
+    uint32_t bad = 0;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        bad += leaf[i] >= nbLeaves;
+    if (bad) atomicAdd(flag, bad);
 }
 __global__ void k_newOff(const uint64_t* __restrict__ leafOff, const uint64_t* __restrict__ batchOff, uint64_t n1,
                          uint64_t* __restrict__ out) {
@@ -195,10 +199,13 @@ __device__ void bitonicSortBlock(uint32_t* s, uint32_t n2) {
 // ------------------------------------------------------------- query ---
 // One 1024-thread block per query (DataBase.cpp:71-130): sort the leaves in
 // LDS, binary64 L1 norm, then per level the distinct nodes with their
-// multiplicities: every leaf, and inner nodes under the posting threshold.
+// multiplicities: every leaf, and inner nodes under the posting threshold
+// (postings of the main and the delta CSR together).  A query holding a leaf
+// id >= nbLeaves scores nothing and sets *bad (when given).
 __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __restrict__ leaf,
                                                    const uint64_t* __restrict__ off, uint32_t K,
-                                                   QNode* __restrict__ nodes, uint32_t* __restrict__ nodeCount) {
+                                                   QNode* __restrict__ nodes, uint32_t* __restrict__ nodeCount,
+                                                   uint32_t* __restrict__ bad) {
     __shared__ uint32_t s[kSortCap];
     __shared__ uint16_t sStart[kSortCap + 1];  // run starts of one level (positions in s), then n
     __shared__ double red[16];
@@ -222,6 +229,15 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
     case 4: bitonicSortBlock<4>(s, n2); break;
     case 8: bitonicSortBlock<8>(s, n2); break;
     default: bitonicSortBlock<16>(s, n2); break;
+    }
+    // the largest leaf id is s[n - 1]: nothing below reads W or the CSRs through an invalid one
+    if (n && s[n - 1] >= v.nbLeaves) {
+        if (tid == 0) {
+            nodeCount[2 * q] = 0;
+            nodeCount[2 * q + 1] = 0;
+            if (bad) atomicOr(bad, 1u);
+        }
+        return;
     }
     double part = 0;
     for (uint32_t t = 0; t < v.L; t++) {
@@ -251,7 +267,7 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
         // runs tid, tid + 1024, ... four at a time: their posting-offset loads are in flight together
         for (uint32_t k0 = tid; k0 < nRuns; k0 += 4 * 1024) {
             uint32_t x[4], cnt[4];
-            uint64_t a[4], e[4];
+            uint64_t a[4], e[4], aD[4], eD[4];
             float w[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -260,6 +276,8 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
                 cnt[u] = 0;
                 a[u] = 0;
                 e[u] = 0;
+                aD[u] = 0;
+                eD[u] = 0;
                 w[u] = 0.0f;
                 if (k < nRuns) {
                     const uint32_t st = sStart[k];
@@ -267,12 +285,14 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
                     x[u] = s[st] >> shift;
                     a[u] = v.leafOff[uint64_t(x[u]) << shift];
                     e[u] = v.leafOff[(uint64_t(x[u]) + 1) << shift];
+                    aD[u] = v.leafOffD[uint64_t(x[u]) << shift];
+                    eD[u] = v.leafOffD[(uint64_t(x[u]) + 1) << shift];
                     w[u] = W[x[u]];
                 }
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                if (k0 + u * 1024 >= nRuns || (t != 0 && e[u] - a[u] >= thr)) continue;
+                if (k0 + u * 1024 >= nRuns || (t != 0 && (e[u] - a[u]) + (eD[u] - aD[u]) >= thr)) continue;
                 QNode nd;
                 nd.v = x[u];
                 nd.t = t;
@@ -282,6 +302,8 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
                 nd.pad = 0;
                 nd.a = a[u];
                 nd.e = e[u];
+                nd.aD = aD[u];
+                nd.eD = eD[u];
                 if (t == 0) out[atomicAdd(&nNodes, 1u)] = nd;
                 else out[n + atomicAdd(&nInner, 1u)] = nd;
             }
@@ -294,26 +316,27 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
     }
 }
 
-// Range directory: dir[(R+1)*leaf + r] = first posting of the leaf whose slot
-// is >= r*kRangeDocs, relative to leafOff[leaf] (lists are sorted by slot).
-__global__ void k_buildDir(DbView v, uint32_t* __restrict__ dir) {
-    const uint64_t n = v.nbLeaves * (v.R + 1);
-    for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += uint64_t(gridDim.x) * blockDim.x) {
-        const uint64_t leaf = e / (v.R + 1);
-        const uint32_t r = uint32_t(e - leaf * (v.R + 1));
+// Range directory of the main CSR: dir[(Rm+1)*leaf + r] = first posting of the
+// leaf whose slot is >= r*kRangeDocs, relative to leafOff[leaf] (lists are
+// sorted by slot, and every main slot is below Rm*kRangeDocs, so dir[..+Rm] is
+// the list length).  One wave per leaf reads the list once: posting i writes
+// the entries of the ranges after its predecessor's, up to its own.
+__global__ __launch_bounds__(256) void k_buildDir(DbView v, uint32_t* __restrict__ dir) {
+    constexpr uint32_t S = Database::kRangeDocs;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t R1 = v.Rm + 1;
+    for (uint64_t leaf = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); leaf < v.nbLeaves;
+         leaf += uint64_t(gridDim.x) * 4) {
         const uint64_t a = v.leafOff[leaf], len = v.leafOff[leaf + 1] - a;
-        uint64_t lo = 0, hi = len;
-        if (r == v.R) {
-            lo = len;
-        } else {
-            const uint32_t key = r * Database::kRangeDocs;
-            while (lo < hi) {
-                const uint64_t mid = (lo + hi) / 2;
-                if (v.post[a + mid] < key) lo = mid + 1;
-                else hi = mid;
-            }
+        uint32_t* d = dir + leaf * R1;
+        const uint32_t* p = v.post + a;
+        for (uint64_t i = lane; i < len; i += 64) {
+            const uint32_t rc = p[i] / S;
+            const uint32_t r0 = i ? p[i - 1] / S + 1 : 0u;
+            for (uint32_t r = r0; r <= rc; r++) d[r] = uint32_t(i);
         }
-        dir[e] = uint32_t(lo);
+        const uint32_t rl = len ? p[len - 1] / S + 1 : 0u;  // ranges past the last posting
+        for (uint32_t r = rl + lane; r < R1; r += 64) d[r] = uint32_t(len);
     }
 }
 
@@ -397,17 +420,32 @@ __device__ uint64_t blockSelectKth(ForEach forEach, uint32_t need, uint32_t* his
     return prefix;
 }
 
+// First position in p[0, n) whose slot is >= key (p sorted ascending).
+__device__ inline uint32_t lowerBound(const uint32_t* p, uint32_t n, uint32_t key) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (p[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // One workgroup per (document range r, query q).  The binary64 accumulators of
 // the range's kRangeDocs slots live in LDS (ds_add_f64), so a posting costs a
 // coalesced 4-byte read and an LDS atomic instead of a scattered global one.
 //   Leaves, in rounds of kSegBatch: the in-range segment of each query leaf
-//   comes from the range directory; non-empty segments are compacted by a
-//   block scan, and each wave walks a contiguous share of the concatenated
-//   postings 64 at a time (balanced however skewed the leaves are).  A run of
-//   one doc's duplicates (DataBase.cpp:138-149) is scored once, by its first
-//   posting, with m = run length from a ballot of run starts.
+//   comes from the main CSR's range directory, and (for ranges that hold
+//   delta slots) from a binary search in the leaf's delta slice, in rounds of
+//   their own; a segment's start is kept as a pointer, so the walk does not
+//   care which CSR it reads.  Non-empty segments are compacted by a block
+//   scan, and each wave walks a contiguous share of the concatenated postings
+//   64 at a time (balanced however skewed the leaves are).  A run of one doc's
+//   duplicates (DataBase.cpp:138-149) is scored once, by its first posting,
+//   with m = run length from a ballot of run starts; a document's postings are
+//   all in one CSR, so no run spans two segments.
 //   Inner nodes (fewer than 16+4K postings, DataBase.cpp:150-164): one wave
-//   per node, multiplicity by scanning the node's slice.
+//   per node, multiplicity by scanning the node's slice in each CSR.
 // The range then keeps its K smallest (score, doc id) keys as candidates for
 // the query's final selection.
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_scoreRange(DbView v, const uint64_t* __restrict__ off, uint32_t q0,
@@ -435,17 +473,29 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     const uint32_t n = uint32_t(off[qc + 1] - beg);
     const QNode* nd = nodes + uint64_t(v.L) * beg;
     const uint32_t nLeaf = nodeCount[2 * q], nInner = nodeCount[2 * q + 1];
-    for (uint32_t k0 = 0; k0 < nLeaf; k0 += kSegBatch) {
-        // ---- stage this round's segments (one query leaf per thread) ----
+    const bool hasMain = r < v.Rm;                                      // block-uniform
+    const bool hasDelta = v.nbPostD != 0 && s0 + S > v.slotD0;
+    const uint32_t nStage = (hasMain ? nLeaf : 0u) + (hasDelta ? nLeaf : 0u);
+    for (uint32_t k0 = 0; k0 < nStage; k0 += kSegBatch) {
+        // ---- stage this round's segments (one query leaf and CSR per thread) ----
         uint32_t len = 0;
-        uint64_t sb = 0;
+        uint64_t sb = 0;  // address of the segment's first posting
         float qE = 0.0f, w = 0.0f;
-        if (k0 + tid < nLeaf) {
-            const QNode node = nd[k0 + tid];
-            const uint32_t* d = v.dir + uint64_t(node.v) * (v.R + 1) + r;
-            const uint32_t a = d[0];
-            len = d[1] - a;
-            sb = node.a + a;
+        if (k0 + tid < nStage) {
+            const bool main = hasMain && k0 + tid < nLeaf;
+            const QNode node = nd[k0 + tid - (main || !hasMain ? 0u : nLeaf)];
+            if (main) {
+                const uint32_t* d = v.dir + uint64_t(node.v) * (v.Rm + 1) + r;
+                const uint32_t a = d[0];
+                len = d[1] - a;
+                sb = reinterpret_cast<uint64_t>(v.post + node.a + a);
+            } else {
+                const uint32_t* p = v.postD + node.aD;
+                const uint32_t m = uint32_t(node.eD - node.aD);
+                const uint32_t lo = s0 <= v.slotD0 ? 0u : lowerBound(p, m, s0);
+                len = lowerBound(p + lo, m - lo, s0 + S);
+                sb = reinterpret_cast<uint64_t>(p + lo);
+            }
             qE = node.qE;
             w = node.w;
         }
@@ -510,7 +560,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
                                         (uint64_t(uint32_t(__shfl(int(uint32_t(begJ >> 32)), int(k[u]), 64))) << 32);
                     valids[u] = idx[u] < wEnd;
                     is[u] = idx[u] - st;
-                    const uint32_t* p = v.post + sb;
+                    const uint32_t* p = reinterpret_cast<const uint32_t*>(sb);
                     slots[u] = valids[u] ? p[is[u]] : 0xFFFFFFFFu;
                     if (u == 0 && lane == 0 && valids[0] && is[0] > 0) prev0 = p[is[0] - 1];
                 }
@@ -528,7 +578,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
                     const uint32_t sg = valids[u] ? lo : seg;
                     sgs[u] = sg;
                     is[u] = idx - (sg ? sSegEnd[sg - 1] : 0u);
-                    const uint32_t* p = v.post + sSegBeg[sg];
+                    const uint32_t* p = reinterpret_cast<const uint32_t*>(sSegBeg[sg]);
                     slots[u] = valids[u] ? p[is[u]] : 0xFFFFFFFFu;
                     if (u == 0 && lane == 0 && valids[0] && is[0] > 0) prev0 = p[is[0] - 1];
                 }
@@ -563,7 +613,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
                 if (!found || base + 64 * u + lane + m >= wEnd) {
                     // no run start follows in this iteration: the run may go on past it or past the share
                     const uint32_t segLen = sSegEnd[sg] - (sg ? sSegEnd[sg - 1] : 0u);
-                    const uint32_t* p = v.post + sSegBeg[sg];
+                    const uint32_t* p = reinterpret_cast<const uint32_t*>(sSegBeg[sg]);
                     while (i + m < segLen && p[i + m] == slot) m++;
                 }
                 const uint32_t j = slot - s0;
@@ -575,22 +625,27 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     }
     for (uint32_t k = wave; k < nInner; k += 16) {
         const QNode node = nd[n + k];
-        for (uint64_t i = node.a + lane; i < node.e; i += 64) {
-            const uint32_t slot = v.post[i], j = slot - s0;
-            if (j >= S) continue;
-            bool first = true;
-            uint32_t m = 0;
-            for (uint64_t x = node.a; x < node.e; x++) {
-                if (v.post[x] == slot) {
-                    m++;
-                    if (x < i) first = false;
+        for (int c = 0; c < 2; c++) {  // the node's slice in the main CSR, then in the delta
+            const uint32_t* p = c ? v.postD + node.aD : v.post + node.a;
+            const uint32_t len = uint32_t(c ? node.eD - node.aD : node.e - node.a);
+            if (c ? !hasDelta : !hasMain) continue;
+            for (uint32_t i = lane; i < len; i += 64) {
+                const uint32_t slot = p[i], j = slot - s0;
+                if (j >= S) continue;
+                bool first = true;
+                uint32_t m = 0;
+                for (uint32_t x = 0; x < len; x++) {
+                    if (p[x] == slot) {
+                        m++;
+                        if (x < i) first = false;
+                    }
                 }
+                if (!first) continue;
+                const float tE = fmulz(fmulz(float(m), node.w), sRcp[j]);
+                const float e = fsubz(fabsf(fsubz(node.qE, tE)), faddz(node.qE, tE));
+                atomicAdd(&sAcc[j], double(e));
+                sTouched[j] = 1;
             }
-            if (!first) continue;
-            const float tE = fmulz(fmulz(float(m), node.w), sRcp[j]);
-            const float c = fsubz(fabsf(fsubz(node.qE, tE)), faddz(node.qE, tE));
-            atomicAdd(&sAcc[j], double(c));
-            sTouched[j] = 1;
         }
     }
     __syncthreads();
@@ -724,8 +779,8 @@ uint32_t gridFor(uint64_t work, uint32_t perBlock, uint32_t cap) {
 }  // namespace
 
 void launchBuildDir(const DbView& v, uint32_t* dir, hipStream_t s) {
-    if (!v.R) return;
-    k_buildDir<<<gridFor(v.nbLeaves * (v.R + 1), 256, 65536), 256, 0, s>>>(v, dir);
+    if (!v.Rm) return;
+    k_buildDir<<<gridFor(v.nbLeaves, 4, 16384), 256, 0, s>>>(v, dir);
     RBOW_HIP(hipGetLastError());
 }
 
@@ -742,42 +797,58 @@ void launchDocNorms(const DbView& v, const uint32_t* dLeaf, const uint64_t* dOff
     RBOW_HIP(hipGetLastError());
 }
 
-void launchMergePostings(const DbView& v, uint64_t nbOld, const uint32_t* dLeaf, const uint32_t* dSlot, uint64_t n,
-                         uint64_t* newLeafOff, uint32_t* newPost, hipStream_t s) {
-    const uint64_t n1 = v.nbLeaves + 1;
-    RBOW_REQUIRE(n < (1ull << 31), RBOW_ERR_CAPACITY, "merge batch too large");
+void launchCountBadLeaves(const uint32_t* leaf, uint64_t n, uint64_t nbLeaves, uint32_t* flag, hipStream_t s) {
+    if (!n) return;
+    k_countBad<<<gridFor(n, 256, 4096), 256, 0, s>>>(leaf, n, nbLeaves, flag);
+    RBOW_HIP(hipGetLastError());
+}
+
+void launchCsrFromPairs(uint64_t nbLeaves, const uint32_t* dLeaf, const uint32_t* dSlot, uint64_t n, DevCsr& out,
+                        hipStream_t s) {
+    const uint64_t n1 = nbLeaves + 1;
+    RBOW_REQUIRE(n < (1ull << 31), RBOW_ERR_CAPACITY, "insert batch too large");
+    RBOW_REQUIRE(out.off.size() >= n1 && out.post.size() >= n, RBOW_ERR_HIP, "CSR buffers too small");
     uint32_t bits = 1;
-    while ((1ull << bits) < v.nbLeaves) bits++;
+    while ((1ull << bits) < nbLeaves) bits++;
     size_t tempBytes = 0;
     RBOW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tempBytes, dLeaf, (uint32_t*)nullptr, dSlot,
                                                 (uint32_t*)nullptr, int(n), 0, int(bits), s));
     size_t scanBytes = 0;
     RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, (uint64_t*)nullptr, (uint64_t*)nullptr, int(n1), s));
-    const size_t bytes = 8 * n1 * 2 + 8 * n + std::max(tempBytes, scanBytes) + 1024;
+    const size_t bytes = 8 * n1 + 4 * n + std::max(tempBytes, scanBytes) + 512;
     void* ws = nullptr;
     RBOW_HIP(hipMallocAsync(&ws, bytes, s));
     uint64_t* cnt = static_cast<uint64_t*>(ws);
-    uint64_t* batchOff = cnt + n1;
-    uint32_t* sLeaf = reinterpret_cast<uint32_t*>(batchOff + n1);
-    uint32_t* sSlot = sLeaf + n;
-    void* temp = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(sSlot + n) + 255) & ~uintptr_t(255));
+    uint32_t* sLeaf = reinterpret_cast<uint32_t*>(cnt + n1);
+    void* temp = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(sLeaf + n) + 255) & ~uintptr_t(255));
     k_fill64<<<gridFor(n1, 256, 4096), 256, 0, s>>>(cnt, 0, n1);
-    k_hist<<<gridFor(n, 256, 4096), 256, 0, s>>>(dLeaf, n, cnt);
-    RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, cnt, batchOff, int(n1), s));
-    RBOW_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, dLeaf, sLeaf, dSlot, sSlot, int(n), 0, int(bits), s));
-    k_placeOld<<<gridFor(nbOld, 256, 8192), 256, 0, s>>>(v.leafOff, v.nbLeaves, v.post, nbOld, batchOff, newPost);
-    k_placeNew<<<gridFor(n, 256, 8192), 256, 0, s>>>(v.leafOff, sLeaf, sSlot, n, newPost);
-    k_newOff<<<gridFor(n1, 256, 4096), 256, 0, s>>>(v.leafOff, batchOff, n1, newLeafOff);
+    if (n) k_hist<<<gridFor(n, 256, 4096), 256, 0, s>>>(dLeaf, n, cnt);
+    RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, cnt, out.off.get(), int(n1), s));
+    // stable: the slots of one leaf keep their ascending input order
+    if (n)
+        RBOW_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, dLeaf, sLeaf, dSlot, out.post.get(), int(n), 0,
+                                                    int(bits), s));
     RBOW_HIP(hipGetLastError());
     RBOW_HIP(hipFreeAsync(ws, s));
+    out.n = n;
+}
+
+void launchMergeCsr(uint64_t nbLeaves, const DevCsr& a, const DevCsr& b, DevCsr& out, hipStream_t s) {
+    const uint64_t n1 = nbLeaves + 1;
+    RBOW_REQUIRE(out.off.size() >= n1 && out.post.size() >= a.n + b.n, RBOW_ERR_HIP, "CSR buffers too small");
+    k_newOff<<<gridFor(n1, 256, 4096), 256, 0, s>>>(a.off.get(), b.off.get(), n1, out.off.get());
+    k_mergeLeaves<<<gridFor(nbLeaves, 4, 16384), 256, 0, s>>>(a.off.get(), a.post.get(), b.off.get(), b.post.get(),
+                                                             nbLeaves, out.off.get(), out.post.get());
+    RBOW_HIP(hipGetLastError());
+    out.n = a.n + b.n;
 }
 
 void launchQueryBatch(const DbView& v, const uint32_t* dLeaf, const uint64_t* dOffsets, const uint64_t* hOffsets,
                       uint32_t nbQueries, uint32_t K, uint32_t* dDocs, float* dScores, uint32_t* dCounts,
-                      hipStream_t s) {
+                      uint32_t* badFlag, hipStream_t s) {
     if (!nbQueries) return;
-    RBOW_REQUIRE(v.R == divUp(v.nbDocs, Database::kRangeDocs) && v.R < 65536 && (v.dir || !v.R), RBOW_ERR_HIP,
-                 "stale range directory");
+    RBOW_REQUIRE(v.R == divUp(v.nbDocs, Database::kRangeDocs) && v.R < 65536 && v.Rm <= v.R && (v.dir || !v.Rm),
+                 RBOW_ERR_HIP, "stale range directory");
     const uint64_t total = hOffsets[nbQueries];
     const uint64_t nbCand = uint64_t(nbQueries) * v.R * K;
     const uint64_t nbNodes = std::max<uint64_t>(total * v.L, 1);
@@ -788,7 +859,7 @@ void launchQueryBatch(const DbView& v, const uint32_t* dLeaf, const uint64_t* dO
     uint64_t* candKeys = reinterpret_cast<uint64_t*>(nodes + nbNodes);
     uint32_t* candCount = reinterpret_cast<uint32_t*>(candKeys + nbCand);
     uint32_t* nodeCount = candCount + uint64_t(nbQueries) * v.R;
-    k_queryPrep<<<nbQueries, 1024, 0, s>>>(v, dLeaf, dOffsets, K, nodes, nodeCount);
+    k_queryPrep<<<nbQueries, 1024, 0, s>>>(v, dLeaf, dOffsets, K, nodes, nodeCount, badFlag);
     // ranges fastest, so the tiles of one query run together and share its node list in L2
     for (uint32_t q0 = 0; v.R && q0 < nbQueries; q0 += 65535)
         k_scoreRange<<<dim3(v.R, std::min(nbQueries - q0, 65535u)), 1024, 0, s>>>(v, dOffsets + q0, q0, nodes,

@@ -175,6 +175,10 @@ size_t regroupWorkspaceBytes(uint32_t nbRows, uint64_t nbKeysMax);
 void launchRegroup(const KmLevel& k, uint32_t nbRows, const uint32_t* orderIn, uint32_t* orderOut, uint32_t* hist,
                    void* ws, size_t wsBytes, hipStream_t s);
 
+// IDF weights (Vocabulary.cpp:103-108) of n children from their occurrence
+// counts: w[i] = occ[i] == 0 ? FLT_MIN : max(FLT_MIN, float(log(double(float(nbDoc) / float(occ[i]))))).
+void launchIdfWeights(const uint32_t* occ, uint64_t n, uint32_t nbDoc, float* w, hipStream_t s);
+
 void launchFillU32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 // p[i] = i for i < n
 void launchIota(uint32_t* p, uint64_t n, hipStream_t s);

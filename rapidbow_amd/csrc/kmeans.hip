@@ -1069,6 +1069,20 @@ __global__ __launch_bounds__(256) void k_regroupKeys(KmLevel k, uint32_t* __rest
     flush();
 }
 
+// Vocabulary.cpp:103-108 IDF weight of one child from its occurrence count:
+// occ == 0 -> FLT_MIN, else max(FLT_MIN, log(nbDoc / occ)) with the quotient
+// in binary32 (IEEE division of the two binary32 conversions), its logarithm
+// in binary64 and the result rounded to binary32.
+__global__ void k_idfWeights(const uint32_t* __restrict__ occ, uint64_t n, uint32_t nbDoc, float* __restrict__ w) {
+    constexpr float kFltMin = 1.17549435e-38f;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t o = occ[i];
+        float r = kFltMin;
+        if (o) r = fmaxf(kFltMin, float(log(double(__fdiv_rn(float(nbDoc), float(o))))));
+        w[i] = r;
+    }
+}
+
 __global__ void k_levelReset(KmLevel k, uint64_t nbRows) {
     const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x, stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t i = tid; i < nbRows; i += stride) k.assign[i] = 0xFF;  // "no previous assignment"
@@ -1281,6 +1295,12 @@ void launchRegroup(const KmLevel& k, uint32_t nbRows, const uint32_t* orderIn, u
     // LSD radix sort is stable: rows keep their order inside each (task, child)
     RBOW_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, keys, keysSorted, orderIn, orderOut, int(nbRows), 0,
                                                 bits, s));
+    RBOW_HIP(hipGetLastError());
+}
+
+void launchIdfWeights(const uint32_t* occ, uint64_t n, uint32_t nbDoc, float* w, hipStream_t s) {
+    if (!n) return;
+    k_idfWeights<<<gridFor(n, 256, 4096), 256, 0, s>>>(occ, n, nbDoc, w);
     RBOW_HIP(hipGetLastError());
 }
 

@@ -190,7 +190,6 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     std::unique_ptr<Vocabulary> voc(new Vocabulary);
     voc->allocate(b, L, ct);
     const uint32_t lg2b = voc->lg2b;
-    std::vector<uint32_t> occ(voc->nbChildren(), 0u);
     std::vector<uint32_t> passesAll(voc->nbNodes, 0u);
 
     // host grouping state: the level order lives on the device; the host keeps a
@@ -226,11 +225,11 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     DevBuf<uint8_t> dFMu(f32Mfma ? nGmax * kDims : 0);
     DevBuf<uint16_t> dFHi(f32Mfma ? nGmax * b * kDims : 0), dFLo(f32Mfma ? nGmax * b * kDims : 0);
     DevBuf<float> dFCn2(f32Mfma ? nGmax * b : 0), dFE1(f32Mfma ? nGmax * b : 0), dFE2(f32Mfma ? nGmax * b : 0);
-    // per-level results, downloaded with one copy: [hist nG*b][passes nG][masks nG]
-    DevBuf<uint32_t> dOut(nGmax * b + 2 * nGmax);
+    // per-level results, downloaded with one copy: [hist nG*b][passes nG][masks nG][IDF weights nG*b]
+    DevBuf<uint32_t> dOut(2 * nGmax * b + 2 * nGmax);
     uint32_t* hOut = nullptr;
-    RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&hOut), 4 * (nGmax * b + 2 * nGmax + 1)));
-    uint32_t* hNbActive = hOut + nGmax * b + 2 * nGmax;
+    RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&hOut), 4 * (2 * nGmax * b + 2 * nGmax + 1)));
+    uint32_t* hNbActive = hOut + 2 * nGmax * b + 2 * nGmax;
     struct PinnedFree {
         uint32_t* p;
         ~PinnedFree() { (void)hipHostFree(p); }
@@ -426,16 +425,20 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
                 if (*hNbActive == 0) break;
             }
         }
-        // ---- stable regroup + occurrence on the device (Vocabulary.cpp:83-98) ----
+        // ---- stable regroup + occurrence on the device (Vocabulary.cpp:83-98), and the
+        // children's IDF weights from the occurrence counts (Vocabulary.cpp:103-108) ----
         launchRegroup(k, nbDesc, dOrder.get(), dOrderNext.get(), dHist, dSortWs.get(), dSortWs.size(), s);
         std::swap(dOrder, dOrderNext);
-        const uint64_t outWords = uint64_t(nG) * b + 2 * nG;
+        float* dW = reinterpret_cast<float*>(dMasks + nG);
+        launchIdfWeights(dHist, uint64_t(nG) * b, nbDoc, dW, s);
+        const uint64_t outWords = 2 * uint64_t(nG) * b + 2 * nG;
         RBOW_HIP(hipMemcpyAsync(hOut, dOut.get(), 4 * outWords, hipMemcpyDeviceToHost, s));
         if (opts.seed_mode == 0 && level + 1 < L) d2h(order.data(), dOrder.get(), nbDesc, s);
         RBOW_HIP(hipStreamSynchronize(s));
         const uint32_t* childCnt = hOut;
         const uint32_t* hPasses = hOut + uint64_t(nG) * b;
         const uint32_t* hMasks = hPasses + nG;
+        std::memcpy(voc->weights.data() + nodeBase * b, hMasks + nG, 4ull * nG * b);
         if (opts.seed_mode == 1)
             for (uint32_t g = 0; g < nG; g++) voc->masks[nodeBase + g] = hMasks[g];
         std::vector<uint8_t> isSmall(nG, 0);
@@ -471,29 +474,11 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             run += childCnt[c];
         }
         RBOW_REQUIRE(run == nbDesc, RBOW_ERR_HIP, "regroup lost rows");
-        for (uint64_t c = 0; c < childBeg.size(); c++) occ[nodeBase * b + c] = childCnt[c];
         taskBeg.swap(childBeg);
         taskCnt.assign(childCnt, childCnt + uint64_t(nG) * b);
     }
 
-    // ---- IDF weights (Vocabulary.cpp:103-108): log of the binary32 quotient
-    // evaluated in binary64, rounded to binary32 ----
-    // (one evaluation per distinct count: a level holds ~1M children but few distinct counts)
-    voc->prepare(s);  // the tree's derived data on the device meanwhile
-    auto idf = [&](uint32_t o) {
-        return o == 0 ? FLT_MIN : std::max(FLT_MIN, float(std::log(double(float(nbDoc) / float(o)))));
-    };
-    const uint32_t maxOcc = occ.empty() ? 0u : *std::max_element(occ.begin(), occ.end());
-    if (maxOcc < (1u << 22)) {
-        std::vector<float> memo(size_t(maxOcc) + 1, -1.0f);  // weights are >= FLT_MIN > 0
-        for (uint64_t i = 0; i < occ.size(); i++) {
-            float& w = memo[occ[i]];
-            if (w < 0.0f) w = idf(occ[i]);
-            voc->weights[i] = w;
-        }
-    } else {
-        for (uint64_t i = 0; i < occ.size(); i++) voc->weights[i] = idf(occ[i]);
-    }
+    voc->prepare(s);  // the tree's derived data on the device
     RBOW_HIP(hipStreamSynchronize(s));
     if (stats) {
         stats->passes = std::move(passesAll);

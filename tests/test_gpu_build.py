@@ -136,3 +136,42 @@ def test_build_timing_option():
     # counts do not depend on timing
     for f in ("point_passes", "assign_ops", "assign_launches", "stream_point_passes"):
         assert st[f] == s0[f], f
+
+
+@pytest.mark.parametrize("seed_mode", [0, 1])
+def test_build_and_lookup_small_tree_full_size(seed_mode):
+    """The small-tree workload at its own size: 200,000 SIFT-like descriptors,
+    b16 L3 (4,096 words), 8 passes per node, both seed modes; then 50,000 queries
+    through both lookup kernels, all against the oracle."""
+    desc = rb.synth_descriptors(200_000, seed=1)
+    L, nb_doc = 3, 1000
+    res = oracle.build(desc, nb_doc, L, max_iters=8, seed_mode=seed_mode)
+    voc = _gpu_build(desc, nb_doc, L, max_iters=8, seed_mode=seed_mode, kernel=0)
+    _check_u8(voc, res, L)
+    q = rb.synth_descriptors(50_000, seed=2)
+    exp = oracle.lookup_u8(res["nodes"], L, q)
+    for k in (rb.LOOKUP_MFMA, rb.LOOKUP_PLAIN, rb.LOOKUP_AUTO):
+        np.testing.assert_array_equal(voc.lookup_host(q, k), exp)
+
+
+def test_build_and_lookup_b32_L4_float_tree():
+    """A 1,048,576-word tree (b32 L4, float centroids, k-means++, 10 passes) on
+    300,000 descriptors: every level's centroids, the weights and the pass
+    counts against the oracle; 200,000 queries through both lookup kernels."""
+    desc = rb.synth_descriptors(300_000, seed=3)
+    b, L, nb_doc = 32, 4, 1000
+    res = oracle.build(desc, nb_doc, L, b=b, max_iters=10, seed_mode=1, centroid_type=1)
+    voc = _gpu_build(desc, nb_doc, L, branch_factor=b, max_iters=10, seed_mode=1, centroid_type=1, kernel=0)
+    for l in range(L):
+        got = voc.centroids(l)
+        exp = res["nodes"][level_beg(b, l):level_beg(b, l + 1)]
+        bad = np.nonzero((got != exp).any(axis=(1, 2)))[0]
+        assert bad.size == 0, f"level {l}: nodes differ {bad[:10]}"
+    np.testing.assert_array_equal(voc.weights().reshape(-1, b), res["weights"])
+    np.testing.assert_array_equal(voc.build_passes(), res["passes"])
+    q = rb.synth_descriptors(200_000, seed=4)
+    exp = oracle.lookup_f32(res["nodes"], b, L, q)
+    got, ties = voc.lookup_host(q, rb.LOOKUP_MFMA, return_near_ties=True)
+    np.testing.assert_array_equal(got, exp)
+    np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_PLAIN), exp)
+    print(f"near ties re-scored in FP32: {ties}")

@@ -6,7 +6,7 @@ import pytest
 
 import oracle
 import rapidbow_amd as rb
-from tests.helpers import SCORE_TOL, assert_ranked_equal, level_beg, ref_blob, rbw2_blob
+from tests.helpers import assert_ranked_equal, level_beg, ref_blob, rbw2_blob
 
 pytestmark = pytest.mark.gpu
 

@@ -6,7 +6,7 @@ import pytest
 
 import oracle
 import rapidbow_amd as rb
-from tests.helpers import SCORE_TOL, assert_ranked_equal, level_beg, ref_blob, rbw2_blob
+from tests.helpers import assert_ranked_equal, level_beg, ref_blob, rbw2_blob
 
 pytestmark = pytest.mark.gpu
 
@@ -114,14 +114,11 @@ def test_server_replay_poisson():
     srv.close()
     assert r["elapsed_s"] > 0 and np.all(r["latency_ms"] > 0)
     assert np.all(r["batch_sizes"] >= 1) and np.all(r["batch_sizes"] <= 8)
-    for i, q in enumerate(qs):
+    for i, q in enumerate(qs):  # the replayed ranking itself, against the oracle
         od, os_ = odb.query(ref_leaf(q), K)
-        assert r["counts"][i] == od.size
-        gd = r["docs"][i, :od.size]
-        if not np.array_equal(gd, od):  # only swaps of neighbours within the score tolerance
-            gd2, gs2 = db.query(ref_leaf(q), K)
-            np.testing.assert_array_equal(np.sort(gd), np.sort(od))
-            assert_ranked_equal(gd2, gs2, od, os_)
+        c = int(r["counts"][i])
+        assert c == od.size
+        assert_ranked_equal(r["docs"][i, :c], r["scores"][i, :c], od, os_)
 
 
 def test_server_errors():

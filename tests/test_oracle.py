@@ -400,3 +400,92 @@ def test_db_threshold_and_postings():
             assert abs(s64[d] - (acc + 2)) < 1e-9, d
         else:
             assert np.isnan(s64[d])
+
+
+def test_idf_weights_pinned():
+    """Vocabulary.cpp:103-108 weights: the closed forms at occ = 0, occ >= nbDoc
+    (log <= 0 -> FLT_MIN), and numpy's binary64 log of the binary32 quotient over
+    every count to 2^16; a binary64 quotient or a binary32 log would differ."""
+    tiny = np.finfo(np.float32).tiny
+    occ = np.arange(1 << 16, dtype=np.uint32)
+    for nb_doc in (1, 3, 1000, 65535, 1_000_000):
+        w = oracle.idf_weights(occ, nb_doc)
+        assert w[0] == tiny
+        assert np.all(w[occ >= nb_doc] == tiny)
+        o = occ[1:].astype(np.float32)
+        q32 = (np.float32(nb_doc) / o).astype(np.float32)
+        exp = np.maximum(tiny, np.log(q32.astype(np.float64)).astype(np.float32))
+        np.testing.assert_array_equal(w[1:], exp)
+        if nb_doc == 1000:
+            q64 = np.float64(nb_doc) / o.astype(np.float64)
+            assert np.any(np.maximum(tiny, np.log(q64).astype(np.float32)) != exp)  # the quotient's precision shows
+            assert np.any(np.maximum(tiny, np.log(q32)) != exp)  # and so does the log's
+    assert oracle.idf_weights(np.array([1], np.uint32), 1000)[0] == np.float32(np.log(1000.0))
+
+
+def test_path_dist_pinned():
+    """Distances along lookup paths: binary64 equals numpy's (to summation order), binary32 equals the
+    FP32 chain of orc_dist_f32, and the child on the path is the one the leaf id
+    names (level-l node leaf >> lg2b*(L-1-l), child its low bits)."""
+    rng = np.random.default_rng(5)
+    b, L = 4, 3
+    n_nodes = (b ** L - 1) // (b - 1)
+    nodes = (rng.random((n_nodes, b, 128)) * 255).astype(np.float32)
+    desc = rng.integers(0, 256, (50, 128), dtype=np.uint8)
+    leaf = rng.integers(0, b ** L, 50).astype(np.uint32)
+    d32, d64 = oracle.path_dist_f32(nodes, b, L, desc, leaf)
+    for i in range(50):
+        for l in range(L):
+            path = int(leaf[i]) >> (2 * (L - 1 - l))
+            c = nodes[level_beg(b, l) + (path >> 2), path & 3]
+            ref = float(((desc[i].astype(np.float64) - c.astype(np.float64)) ** 2).sum())  # pairwise order
+            assert abs(d64[i, l] - ref) <= 1e-13 * ref
+            assert d32[i, l] == oracle.dist_f32(desc[i], c)
+
+
+def test_assert_ranked_equal_is_strict():
+    """The ranking check accepts one-ulp score differences and the swaps they
+    cause, and rejects a two-ulp error, a swap of distinct scores, an unranked
+    list and a foreign document."""
+    from tests.helpers import assert_ranked_equal
+    od = np.array([5, 9, 2, 7], np.uint32)
+    os_ = np.array([0.25, 0.5, 0.75, 1.0], np.float32)
+    up = lambda x, k=1: np.float32(x) + np.float32(k) * np.spacing(np.float32(x))
+    assert_ranked_equal(od, os_, od, os_)
+    assert_ranked_equal(od, np.array([0.25, up(0.5), 0.75, 1.0], np.float32), od, os_)
+    tied = np.array([0.25, 0.5, up(0.5), 1.0], np.float32)  # oracle: docs 9 and 2 one ulp apart
+    assert_ranked_equal(np.array([5, 2, 9, 7], np.uint32), np.array([0.25, 0.5, 0.5, 1.0], np.float32),
+                        np.array([5, 9, 2, 7], np.uint32), tied)
+    bad = [
+        (od, np.array([0.25, up(0.5, 2), 0.75, 1.0], np.float32)),                 # two ulps off
+        (np.array([5, 2, 9, 7], np.uint32), np.array([0.25, 0.5, 0.75, 1.0], np.float32)),  # swap of distinct scores
+        (np.array([9, 5, 2, 7], np.uint32), np.array([0.5, 0.25, 0.75, 1.0], np.float32)),  # not ranked
+        (np.array([5, 9, 8, 7], np.uint32), os_),                                  # foreign doc at a distinct score
+    ]
+    for gd, gs in bad:
+        with pytest.raises(AssertionError):
+            assert_ranked_equal(gd, gs, od, os_)
+
+
+@pytest.mark.parametrize("centroid_type", [0, 1])
+def test_build_centroid_error_pinned(centroid_type):
+    """centroid_err = max |centroid - binary64 mean of its points|: on a converged
+    one-level build the members are the points the final centers' descent sends
+    to each child, so numpy recomputes it from exact int64 sums."""
+    rng = np.random.default_rng(31)
+    desc = rng.integers(0, 256, (3000, 128), dtype=np.uint8)
+    res = oracle.build(desc, 10, 1, max_iters=1000, seed_mode=1, centroid_type=centroid_type)
+    assert res["passes"][0] < 1000  # converged: the assignment is the descent on the final centers
+    if centroid_type == 0:
+        cent = oracle.ref_nodes_to_rows(res["nodes"])[0].astype(np.float64)
+        leaf = oracle.lookup_u8(res["nodes"], 1, desc)
+    else:
+        cent = res["nodes"][0].astype(np.float64)
+        leaf = oracle.lookup_f32(res["nodes"], 16, 1, desc)
+    err = 0.0
+    for j in range(16):
+        m = desc[leaf == j].astype(np.int64)
+        if len(m):
+            err = max(err, float(np.abs(cent[j] - m.sum(0) / len(m)).max()))
+    assert res["centroid_err"] == err
+    assert (err <= 0.5) if centroid_type == 0 else (0 < err < 1e-4)
