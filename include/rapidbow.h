@@ -110,6 +110,18 @@ typedef enum {
  * candidate set held more than one child and was re-scored in FP32. */
 rbow_status rbow_voc_lookup_ex(const rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc,
                                uint32_t* dev_leaf, int32_t kernel, uint32_t* dev_near_ties, rbow_stream stream);
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the same lookup as rbow_voc_lookup_ex, same leaf ids, with no
+ * allocation of any kind.  The MFMA form's workspace is a static 64 MiB
+ * device array instead of a stream-ordered allocation: each GPU the library
+ * runs on holds its own copy until the process ends.  A batch whose workspace
+ * does not fit it returns RBOW_ERR_CAPACITY before anything is launched (about
+ * 2 million descriptors per call on a b32 L4 tree, 4 million on b16 L4; the
+ * plain form needs none).  Every call on one GPU shares that array, so calls
+ * that overlap in time on one GPU race: issue them on one stream. */
+rbow_status rbow_voc_lookup_noalloc(const rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc,
+                                    uint32_t* dev_leaf, int32_t kernel, uint32_t* dev_near_ties, rbow_stream stream);
 /* Autotuner (no reference counterpart): times the plain and MFMA lookups on
  * prefixes of dev_desc (powers of two from 4096 up to nb_desc) and sets the
  * vocabulary's AUTO crossover to the smallest size at which MFMA is at least

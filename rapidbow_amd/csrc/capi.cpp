@@ -239,6 +239,19 @@ rbow_status rbow_voc_lookup_ex(const rbow_vocabulary* voc, const void* dev_desc,
     });
 }
 
+rbow_status rbow_voc_lookup_noalloc(const rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc,
+                                    uint32_t* dev_leaf, int32_t kernel, uint32_t* dev_near_ties, rbow_stream stream) {
+    return guarded([&] {
+        RBOW_REQUIRE(voc != nullptr, RBOW_ERR_INVALID_ARG, "null vocabulary");
+        if (nb_desc == 0) return;
+        RBOW_REQUIRE(dev_desc && dev_leaf, RBOW_ERR_INVALID_ARG, "null descriptors or output");
+        checkAligned(dev_desc, "descriptors");
+        RBOW_REQUIRE(kernel >= RBOW_LOOKUP_AUTO && kernel <= RBOW_LOOKUP_MFMA, RBOW_ERR_INVALID_ARG, "bad kernel id");
+        launchLookupNoAlloc(voc->v->view(), static_cast<const uint8_t*>(dev_desc), nb_desc, dev_leaf, dev_near_ties,
+                            kernel, hs(stream));
+    });
+}
+
 rbow_status rbow_voc_tune_lookup(rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc,
                                  uint32_t* out_crossover, rbow_stream stream) {
     return guarded([&] {

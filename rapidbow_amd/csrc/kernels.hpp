@@ -51,6 +51,13 @@ void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32
 uint32_t lookupAutoDefault(uint32_t centroidType, uint32_t b);
 void launchLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies, int kernel,
                   hipStream_t s);
+// The same lookup without any allocation: the MFMA form's workspace is a static
+// device array of lookupScratchBytes() per GPU, shared by every call on that GPU
+// (calls that overlap in time on one GPU race).  Batches whose workspace does
+// not fit it are rejected with RBOW_ERR_CAPACITY before anything is launched.
+void launchLookupNoAlloc(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
+                         int kernel, hipStream_t s);
+size_t lookupScratchBytes();
 // Autotuner: times both lookup kernels on prefixes of desc (powers of two from
 // 4096 up to n) and returns the smallest size at which the MFMA form is at
 // least as fast (n + 1 if it never is).  Blocks; uses `leaf` (n words) as output.

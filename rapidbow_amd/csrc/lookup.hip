@@ -23,6 +23,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <mutex>
 
 #include "bf16filter.hpp"
 #include "kernels.hpp"
@@ -603,39 +604,41 @@ void launchLookupPlain(const TreeView& t, const uint8_t* desc, uint32_t n, uint3
 // Workspace of one bucketed lookup of at most kChunk descriptors.
 namespace {
 constexpr uint32_t kChunk = 1u << 26;  // descriptors per pass (item ids stay below 2^20)
+
+// Layout of the bucketed lookup's workspace for n descriptors (n <= kChunk, L >= 2):
+// perm[2][n], code[n], C[cMax], O[cMax], node arrays [2][2][maxNodes+1], nbItems,
+// then (256-aligned) items[2][itemsMax], then the scan's temporary storage.
+struct LookupWs {
+    uint64_t maxNodes, itemsMax, cMax;
+    size_t scanBytes, bytes;
+};
+
+LookupWs lookupWsLayout(const TreeView& t, uint32_t n, hipStream_t s) {
+    LookupWs w;
+    const uint32_t items0 = (n + kTQ - 1) / kTQ;
+    w.maxNodes = levelSize(t.lg2b, t.L - 1);
+    w.itemsMax = items0 + w.maxNodes;
+    w.cMax = w.itemsMax * t.b + 1;
+    size_t scanBytes2 = 0;
+    w.scanBytes = 0;
+    RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, w.scanBytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                              int(w.cMax), s));
+    RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes2, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                              int(w.maxNodes + 1), s));
+    w.scanBytes = std::max(w.scanBytes, scanBytes2);
+    const size_t words = 3ull * n + 2 * w.cMax + 4 * (w.maxNodes + 1) + 1;
+    w.bytes = 4 * words + 256 + 2 * sizeof(LkItem) * w.itemsMax + 256 + w.scanBytes + 256;
+    return w;
 }
 
-void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
-                      hipStream_t s) {
-    if (!n) return;
-    if (t.L == 0) {
-        launchFillU32(leaf, 0u, n, s);
-        return;
-    }
-    if (n > kChunk) {  // independent chunks
-        for (uint64_t o = 0; o < n; o += kChunk)
-            launchLookupMfma(t, desc + o * kDims, uint32_t(std::min<uint64_t>(kChunk, n - o)), leaf + o, nearTies, s);
-        return;
-    }
+// The bucketed descent of n <= kChunk descriptors (L >= 2) in the given
+// workspace (256-byte aligned, at least lookupWsLayout(t, n).bytes).  Allocates nothing.
+void lookupMfmaInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
+                    const LookupWs& w, void* ws, hipStream_t s) {
     const uint32_t b = t.b, lg2b = t.lg2b;
     const uint32_t items0 = (n + kTQ - 1) / kTQ;
-    if (t.L == 1) {
-        launchLevel<true, true>(t, 0, desc, n, nullptr, nullptr, nullptr, items0, leaf, nullptr, nearTies, s);
-        return;
-    }
-    const uint64_t maxNodes = levelSize(lg2b, t.L - 1);
-    const uint64_t itemsMax = items0 + maxNodes;
-    const uint64_t cMax = itemsMax * b + 1;
-    size_t scanBytes = 0, scanBytes2 = 0;
-    RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, (uint32_t*)nullptr, (uint32_t*)nullptr, int(cMax), s));
-    RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes2, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                              int(maxNodes + 1), s));
-    scanBytes = std::max(scanBytes, scanBytes2);
-    // workspace: perm[2][n], code[n], C[cMax], O[cMax], node arrays [2][2][maxNodes+1], nbItems, items[2][itemsMax], scan temp
-    const size_t words = 3ull * n + 2 * cMax + 4 * (maxNodes + 1) + 1;
-    const size_t bytes = 4 * words + 256 + 2 * sizeof(LkItem) * itemsMax + 256 + scanBytes + 256;
-    void* ws = nullptr;
-    RBOW_HIP(hipMallocAsync(&ws, bytes, s));
+    const uint64_t maxNodes = w.maxNodes, itemsMax = w.itemsMax, cMax = w.cMax;
+    size_t scanBytes = w.scanBytes;
     uint32_t* perm[2] = {static_cast<uint32_t*>(ws), static_cast<uint32_t*>(ws) + n};
     uint32_t* code = perm[1] + n;
     uint32_t* C = code + n;
@@ -684,7 +687,72 @@ void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32
         buildItems(nBase[cur], nCnt[cur], 0, uint32_t(nodesL * b), cur ^ 1);
         cur ^= 1;
     }
+}
+
+// Static workspace of the allocation-free lookup: one copy per GPU (each device
+// holds its own instance of this array for the life of the process).  Host code
+// never uses the array's name as an address: the device address comes from
+// hipGetSymbolAddress, once per device.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+__device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+void* scratchOfCurrentDevice() {
+    static std::mutex m;
+    static void* ptr[256] = {};
+    int d = 0;
+    RBOW_HIP(hipGetDevice(&d));
+    RBOW_REQUIRE(d >= 0 && d < 256, RBOW_ERR_HIP, "device index out of range");
+    std::lock_guard<std::mutex> g(m);
+    if (!ptr[d]) {
+        void* p = nullptr;
+        RBOW_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        RBOW_REQUIRE(p != nullptr, RBOW_ERR_HIP, "no device address for the lookup scratch");
+        ptr[d] = p;
+    }
+    return ptr[d];
+}
+}  // namespace
+
+void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
+                      hipStream_t s) {
+    if (!n) return;
+    if (t.L == 0) {
+        launchFillU32(leaf, 0u, n, s);
+        return;
+    }
+    if (n > kChunk) {  // independent chunks
+        for (uint64_t o = 0; o < n; o += kChunk)
+            launchLookupMfma(t, desc + o * kDims, uint32_t(std::min<uint64_t>(kChunk, n - o)), leaf + o, nearTies, s);
+        return;
+    }
+    if (t.L == 1) {
+        launchLevel<true, true>(t, 0, desc, n, nullptr, nullptr, nullptr, (n + kTQ - 1) / kTQ, leaf, nullptr,
+                                nearTies, s);
+        return;
+    }
+    const LookupWs w = lookupWsLayout(t, n, s);
+    void* ws = nullptr;
+    RBOW_HIP(hipMallocAsync(&ws, w.bytes, s));
+    lookupMfmaInWs(t, desc, n, leaf, nearTies, w, ws, s);
     RBOW_HIP(hipFreeAsync(ws, s));
+}
+
+size_t lookupScratchBytes() { return kScratchBytes; }
+
+void launchLookupNoAlloc(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
+                         int kernel, hipStream_t s) {
+    const uint32_t autoMin = t.autoMin ? t.autoMin : lookupAutoDefault(t.centroidType, t.b);
+    const bool plain = kernel == RBOW_LOOKUP_PLAIN || t.L == 0 || (kernel == RBOW_LOOKUP_AUTO && n < autoMin);
+    if (!n) return;
+    if (plain || t.L == 1) {  // neither form needs a workspace here
+        launchLookup(t, desc, n, leaf, nearTies, kernel, s);
+        return;
+    }
+    RBOW_REQUIRE(n <= kChunk, RBOW_ERR_CAPACITY, "lookup batch above the allocation-free scratch");
+    const LookupWs w = lookupWsLayout(t, n, s);
+    RBOW_REQUIRE(w.bytes <= kScratchBytes, RBOW_ERR_CAPACITY,
+                 "lookup batch needs more than the 64 MiB allocation-free scratch");
+    lookupMfmaInWs(t, desc, n, leaf, nearTies, w, scratchOfCurrentDevice(), s);
 }
 
 uint32_t lookupAutoDefault(uint32_t centroidType, uint32_t b) {
