@@ -9,11 +9,17 @@
 // one exactly when they are consecutive chunks of one task in one workgroup
 // (then through an LDS slot of that workgroup, at most wavesPerBlock - 1 per
 // workgroup), partials are numbered in chunk order, and taskPartBeg/Cnt list
-// each task's partials.
+// each task's partials.  Then a simulation of one assign pass's partial-sum
+// bookkeeping (below): the current layout, one tile range per wave, must never
+// lose a tile's contribution; the layout of a reverted experiment, one range
+// per workgroup whose waves take the tiles in turn, must be caught losing them
+// (chunks without a merge slot are written with plain stores, which is only
+// sound when one wave processes all of the chunk's tiles).
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "kmpartition.hpp"
@@ -144,6 +150,189 @@ static void checkCase(const std::vector<uint32_t>& realTiles, const std::vector<
     checkParts(p, n, wpb, spb, id);
 }
 
+// ---------------------------------------------------------------------------
+// Simulation of one MFMA assign pass's partial-sum bookkeeping (k_assignU8Mfma,
+// k_reduceChunks, k_updateU8Part) under a given tile -> wave layout.
+//
+// A tile contributes (1, h(t)) -- a count and a hashed weight -- to its task.
+// Each wave accumulates its tiles of one chunk and, when it leaves the chunk,
+// flushes the accumulator: a chunk without a merge slot STORES it into its
+// partial (the kernel's plain stores), a chunk with a slot ADDS it into the
+// workgroup's LDS slot (atomics), and after the workgroup's barrier every slot
+// of an active task is stored into its partial.  Stores of different waves land
+// in an order fixed by the simulated timing (last writer wins).  Partials of
+// inactive (converged) tasks are never written and keep stale values from
+// earlier passes, as on the device.  The reduction then sums each active task's
+// partials -- directly for at most kDirectParts, else through reduce items of at
+// most kReduceItemParts that must tile the task's partial range -- and the
+// result must equal the sum over the task's streamed tiles.
+//
+// Layouts:
+//   OwnRanges: the current kernel; wave w processes its own chunks in order.
+//   InTurn:    the reverted variant -- one contiguous tile range per
+//              workgroup (the union of its waves' ranges), its waves taking the
+//              tiles in turn; which wave takes which tile depends on timing
+//              (modelled by a random pick per tile), and so does store order.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kDirectPartsSim = 32, kReduceItemPartsSim = 32;
+
+struct SimAcc {
+    uint64_t cnt = 0, sum = 0;
+    SimAcc& operator+=(const SimAcc& o) {
+        cnt += o.cnt;
+        sum += o.sum;
+        return *this;
+    }
+    bool operator==(const SimAcc& o) const { return cnt == o.cnt && sum == o.sum; }
+};
+static SimAcc tileValue(uint32_t t) { return SimAcc{1, (uint64_t(t) + 1) * 0x9E3779B97F4A7C15ull}; }
+
+enum class SimLayout { OwnRanges, InTurn };
+
+// Returns the number of active tasks whose reduced totals are wrong after this pass.
+static int simulatePass(const KmPartition& p, const std::vector<uint8_t>& active, std::vector<SimAcc>& partials,
+                        SimLayout layout, std::mt19937_64& rng, uint32_t* firstBadChunk) {
+    const uint32_t W = p.wavesPerBlock, S = p.slotsPerBlock, nbBlocks = (p.nbWaves + W - 1) / W;
+    const uint32_t nbTasks = uint32_t(p.taskPartBeg.size());
+    struct Store {
+        uint64_t time;
+        uint32_t part;
+        SimAcc v;
+    };
+    std::vector<Store> stores;
+    std::vector<SimAcc> truth(nbTasks);
+    for (uint32_t blk = 0; blk < nbBlocks; blk++) {
+        const uint32_t w0 = blk * W, w1 = std::min(w0 + W, p.nbWaves);
+        std::vector<SimAcc> slot(S);
+        // the tiles each wave of this workgroup processes, in order, with their chunk
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> work(W);
+        for (uint32_t w = w0; w < w1; w++)
+            for (uint32_t c = p.waveChunkBeg[w]; c < p.waveChunkBeg[w + 1]; c++) {
+                if (!active[p.chunks[c].task]) continue;  // nextActive: converged tasks' chunks are skipped
+                for (uint32_t t = p.chunks[c].tileBeg; t < p.chunks[c].tileEnd; t++) {
+                    truth[p.chunks[c].task] += tileValue(t);
+                    const uint32_t who = layout == SimLayout::OwnRanges ? w - w0 : uint32_t(rng() % (w1 - w0));
+                    work[who].push_back({t, c});
+                }
+            }
+        uint64_t clock = 0;
+        for (uint32_t wv = 0; wv < W; wv++) {
+            const auto& wk = work[wv];
+            uint64_t time = rng() % 64;  // the wave's start
+            for (size_t i = 0; i < wk.size();) {
+                const uint32_t c = wk[i].second;
+                SimAcc acc;
+                for (; i < wk.size() && wk[i].second == c; i++) {
+                    acc += tileValue(wk[i].first);
+                    time += 1 + rng() % 8;
+                }
+                const KmChunk& k = p.chunks[c];
+                if (k.slot == kNoSlot) stores.push_back(Store{time, k.part, acc});
+                else slot[k.slot] += acc;
+            }
+            clock = std::max(clock, time);
+        }
+        for (uint32_t sl = 0; sl < S; sl++) {  // after the barrier: slots of active tasks leave once
+            const KmSlot& ks = p.blockSlots[uint64_t(blk) * S + sl];
+            if (ks.part != kNoSlot && active[ks.task]) stores.push_back(Store{clock + 1, ks.part, slot[sl]});
+        }
+    }
+    std::stable_sort(stores.begin(), stores.end(), [](const Store& a, const Store& b) { return a.time < b.time; });
+    for (const Store& st : stores) partials[st.part] = st.v;
+    int bad = 0;
+    for (uint32_t g = 0; g < nbTasks; g++) {
+        if (!active[g] || p.taskPartCnt[g] == 0) continue;
+        const uint32_t cb = p.taskPartBeg[g], nc = p.taskPartCnt[g];
+        SimAcc tot;
+        if (nc <= kDirectPartsSim) {
+            for (uint32_t x = cb; x < cb + nc; x++) tot += partials[x];
+        } else {  // reduce items as the host lists them: they must tile [cb, cb + nc)
+            uint32_t next = cb;
+            for (uint32_t c = 0; c < nc; c += kReduceItemPartsSim) {
+                const uint32_t b = cb + c, e = cb + std::min(nc, c + kReduceItemPartsSim);
+                CHECK(b == next && e > b, "reduce items of task %u do not tile its partials", g);
+                for (uint32_t x = b; x < e; x++) tot += partials[x];
+                next = e;
+            }
+            CHECK(next == cb + nc, "reduce items of task %u stop early", g);
+        }
+        if (!(tot == truth[g])) {
+            if (bad++ == 0 && firstBadChunk)
+                for (uint32_t c = p.taskChunkBeg[g]; c < p.taskChunkBeg[g] + p.taskChunkCnt[g]; c++)
+                    if (p.chunks[c].slot == kNoSlot && p.chunks[c].tileEnd - p.chunks[c].tileBeg >= 2) {
+                        *firstBadChunk = c;
+                        break;
+                    }
+        }
+    }
+    return bad;
+}
+
+// Several passes over one level with tasks converging at random; returns the
+// number of (pass, task) pairs with wrong totals, and the number of passes with any.
+static std::pair<uint64_t, uint32_t> simulateLevel(const std::vector<uint32_t>& taskTiles, uint64_t maxWaves,
+                                                   uint32_t wpb, uint32_t spb, SimLayout layout, uint32_t passes,
+                                                   std::mt19937_64& rng, uint32_t* firstBadChunk = nullptr,
+                                                   KmPartition* out = nullptr) {
+    const KmPartition p = partitionTiles(taskTiles, {}, maxWaves, wpb, spb);
+    std::vector<uint8_t> active(taskTiles.size(), 1);
+    std::vector<SimAcc> partials(p.nbParts, SimAcc{0xDEADull, 0xBEEFull});  // never-written partials are garbage
+    uint64_t bad = 0;
+    uint32_t badPasses = 0;
+    for (uint32_t it = 0; it < passes; it++) {
+        const int b = simulatePass(p, active, partials, layout, rng, firstBadChunk);
+        bad += uint64_t(b);
+        badPasses += b > 0;
+        for (auto& a : active)
+            if (a && rng() % 4 == 0) a = 0;  // converged: its partials go stale, its slots are not written
+    }
+    if (out) *out = p;
+    return {bad, badPasses};
+}
+
+static void simulateLayouts(std::mt19937_64& rng) {
+    // a hand-sized level: one workgroup of four waves (the kernel's shape), three
+    // tasks whose middle one lies inside one wave's range of the per-wave layout
+    {
+        const std::vector<uint32_t> tiles = {5, 2, 9};  // 16 tiles, 4 per wave: task 1 = tiles 5..6, inside wave 1
+        KmPartition p;
+        uint32_t badChunk = kNoSlot;
+        const auto own = simulateLevel(tiles, 4, 4, 3, SimLayout::OwnRanges, 1, rng, nullptr, &p);
+        CHECK(own.first == 0, "per-wave layout loses sums on the hand-sized level");
+        uint64_t turnBad = 0;
+        for (int r = 0; r < 64; r++) turnBad += simulateLevel(tiles, 4, 4, 3, SimLayout::InTurn, 1, rng, &badChunk).first;
+        std::printf("hand-sized level {5,2,9} tiles, 4 waves, 3 slots: chunks");
+        for (const KmChunk& c : p.chunks)
+            std::printf(" [task %u tiles %u-%u slot %s]", c.task, c.tileBeg, c.tileEnd - 1,
+                        c.slot == kNoSlot ? "none" : std::to_string(c.slot).c_str());
+        std::printf("\n  per-wave ranges: 0 wrong task totals; in-turn: %llu wrong in 64 runs\n",
+                    (unsigned long long)turnBad);
+        CHECK(turnBad > 0, "the in-turn layout should lose the sums of task 1 (a chunk without a slot)");
+    }
+    // random levels with the kernel's shape (4 waves, 3 slots), several passes each
+    uint64_t ownBad = 0, turnBad = 0;
+    uint32_t levels = 0, turnLevelsBad = 0;
+    for (int r = 0; r < 300; r++) {
+        const uint32_t n = 1 + uint32_t(rng() % (r < 150 ? 60 : 3000));
+        std::vector<uint32_t> tiles(n);
+        for (uint32_t g = 0; g < n; g++) {
+            const uint32_t pick = uint32_t(rng() % 10);
+            tiles[g] = pick == 0 ? 0u : pick < 7 ? 1u + uint32_t(rng() % 24) : 17u + uint32_t(rng() % 400);
+        }
+        const uint64_t mw = std::vector<uint64_t>{4, 8, 64, 512, 2048}[rng() % 5];
+        ownBad += simulateLevel(tiles, mw, 4, 3, SimLayout::OwnRanges, 6, rng).first;
+        const auto t = simulateLevel(tiles, mw, 4, 3, SimLayout::InTurn, 6, rng);
+        turnBad += t.first;
+        turnLevelsBad += t.first > 0;
+        levels++;
+    }
+    std::printf("random levels (4 waves, 3 slots, 6 passes each): per-wave ranges %llu wrong task totals in %u "
+                "levels; in-turn %llu wrong in %u of %u levels\n",
+                (unsigned long long)ownBad, levels, (unsigned long long)turnBad, turnLevelsBad, levels);
+    CHECK(ownBad == 0, "the current layout lost sums");
+    CHECK(turnBad > 0, "the simulation does not detect the in-turn layout's lost stores");
+}
+
 int main() {
     std::mt19937_64 rng(2024);
     int cases = 0;
@@ -175,6 +364,7 @@ int main() {
         std::vector<uint8_t> all(n, 1);
         checkCase(tiles, all, mw, rng() % 2 == 0, cases++, wpb, spb);
     }
+    simulateLayouts(rng);
     std::printf("%d cases, %d failures\n", cases, fails);
     return fails ? EXIT_FAILURE : EXIT_SUCCESS;
 }
