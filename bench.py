@@ -25,12 +25,15 @@ Beside it on the same JSON line:
     queries (I8 MFMA vs plain);
   * batched retrieval: 100,000 images (lognormal descriptor counts, median
     2,000, clamped to [200, 8,000]) on the 1M-word tree, batches of 64
-    re-noised query images, top-50: batch latency, queries/s, postings/s, and
-    (with the CPU leg) ranked-list identity and max score error vs the oracle;
+    re-noised query images, top-50: batch latency and queries/s as a separate
+    lookup + batch query, and through the query pipeline as direct launches and
+    as ONE captured HIP graph launch per batch; with the CPU leg, ranked-list
+    identity and max score error vs the oracle;
   * serving: 1,000,000 images (about 2e9 postings) resident in HBM, Poisson
     arrivals at 200 / 1,000 / 5,000 QPS of queries with 300-6,000 descriptors,
-    p50/p99 latency and queries/s of the continuous-batching server vs one
-    query at a time with the plain kernels;
+    p50/p99 latency and queries/s of the continuous-batching server (one graph
+    launch per batch, and with direct launches) vs one query at a time with the
+    plain kernels;
   * the CPU baseline: the oracle, on the host cores, on the same build (median
     of 5 runs), whose tree is also compared with the timed GPU build
     (tree_equals_oracle); with it the error report: centroids vs the binary64
@@ -255,9 +258,13 @@ def bench_retrieval(rb, syn, v32, with_cpu, threads):
     docs = rb.DeviceArray((B, K), np.uint32)
     sc = rb.DeviceArray((B, K), np.float32)
     cn = rb.DeviceArray((B,), np.uint32)
+    maxb = max(int(qoff[(bi + 1) * B] - qoff[bi * B]) for bi in range(NB))
+    pipe = rb.QueryPipeline(v32, db, B, maxb, K)
     hip.synchronize()
     lat, res_docs, res_sc = [], [], []
-    for rep in range(2):  # first pass warms
+    lat_p = {False: [], True: []}
+    same_pipe = True
+    for rep in range(2):  # first pass warms (and captures the graphs)
         for bi in range(NB):
             q0, q1 = bi * B, (bi + 1) * B
             o = qoff[q0:q1 + 1]
@@ -266,18 +273,33 @@ def bench_retrieval(rb, syn, v32, with_cpu, threads):
                        qleaf.view(int(o[0]), int(o[-1] - o[0])), rb.LOOKUP_MFMA)
             db.query_batch(qleaf, o, K, docs, sc, cn)  # returns when done
             dt = time.perf_counter() - t0
+            c = cn.numpy()
+            D, S = docs.numpy(), sc.numpy()
             if rep:
                 lat.append(dt * 1e3)
-                c = cn.numpy()
-                D, S = docs.numpy(), sc.numpy()
                 res_docs += [D[q, :c[q]].copy() for q in range(B)]
                 res_sc += [S[q, :c[q]].copy() for q in range(B)]
+            # the same batch through the pipeline: direct launches, then one graph launch
+            for g in (False, True):
+                t0 = time.perf_counter()
+                pipe.run(qdesc, o, docs, sc, cn, lookup_kernel=rb.LOOKUP_MFMA, use_graph=g)  # returns when done
+                dt = time.perf_counter() - t0
+                if rep:
+                    lat_p[g].append(dt * 1e3)
+                    same_pipe &= bool(np.array_equal(cn.numpy(), c) and np.array_equal(docs.numpy(), D)
+                                      and np.array_equal(sc.numpy(), S))
     lat = np.array(lat)
     out = {"images": NI, "postings": int(db.nb_postings), "db_build_s": round(build_s, 2),
            "batch": B, "top_k": K, "batches": NB, "query_desc_mean": round(float(qcnt.mean()), 1),
            "batch_ms_p50": round(float(np.percentile(lat, 50)), 3),
            "batch_ms_p99": round(float(np.percentile(lat, 99)), 3),
            "queries_per_s": round(B * NB / (lat.sum() * 1e-3), 1),
+           "pipeline": {"direct_batch_ms_p50": round(float(np.percentile(lat_p[False], 50)), 3),
+                        "direct_batch_ms_p99": round(float(np.percentile(lat_p[False], 99)), 3),
+                        "graph_batch_ms_p50": round(float(np.percentile(lat_p[True], 50)), 3),
+                        "graph_batch_ms_p99": round(float(np.percentile(lat_p[True], 99)), 3),
+                        "graph_queries_per_s": round(B * NB / (np.sum(lat_p[True]) * 1e-3), 1),
+                        "graphs_captured": int(pipe.graphs), "same_results": same_pipe},
            "self_first": float(np.mean([d.size > 0 and d[0] == g for d, g in zip(res_docs, qimg)]))}
     if with_cpu:
         import oracle
@@ -375,9 +397,12 @@ def bench_serving(rb, syn, v32, rates, duration_s):
 
     batched = rb.Server(v32, db, max_batch=64, nb_streams=3, mem_budget=8 << 30, max_results=K,
                         lookup_kernel=rb.LOOKUP_AUTO)
+    direct = rb.Server(v32, db, max_batch=64, nb_streams=3, mem_budget=8 << 30, max_results=K,
+                       lookup_kernel=rb.LOOKUP_AUTO, use_graphs=False)
     single = rb.Server(v32, db, max_batch=1, nb_streams=1, mem_budget=8 << 30, max_results=K,
                        lookup_kernel=rb.LOOKUP_PLAIN)
-    run(batched, 64, 1000.0)  # warm both
+    run(batched, 64, 1000.0)  # warm them
+    run(direct, 64, 1000.0)
     _, w1 = run(single, 16, 50.0)
     # one-at-a-time service time (arrivals 20 ms apart), to bound the overloaded baseline runs to ~10 s each
     svc_ms = max(w1["p50_ms"], 0.05)
@@ -386,17 +411,21 @@ def bench_serving(rb, syn, v32, rates, duration_s):
         n = min(NQ, max(1000, int(qps * duration_s)))
         log(f"bench: serving {qps} QPS, {n} queries")
         rb_, m = run(batched, n, qps)
+        _, m_direct = run(direct, n, qps)
         nb = min(n, max(200, int(10e3 / svc_ms)))
         rs, s = run(single, nb, qps)
         same = 0
         for i in range(nb):
             c = int(rs["counts"][i])
             same += int(c == int(rb_["counts"][i]) and np.array_equal(rs["docs"][i, :c], rb_["docs"][i, :c]))
-        out["rates"].append({"offered_qps": qps, "batched": m, "one_at_a_time_plain": s,
+        out["rates"].append({"offered_qps": qps, "batched": m, "batched_without_graphs": m_direct,
+                             "one_at_a_time_plain": s,
                              "same_ranked_ids": f"{same}/{nb}"})
-        log(f"bench:   batched p50 {m['p50_ms']} ms p99 {m['p99_ms']} ms {m['achieved_qps']} q/s | "
+        log(f"bench:   batched p50 {m['p50_ms']} ms p99 {m['p99_ms']} ms {m['achieved_qps']} q/s "
+            f"(no graphs p50 {m_direct['p50_ms']} p99 {m_direct['p99_ms']}) | "
             f"single p50 {s['p50_ms']} p99 {s['p99_ms']} {s['achieved_qps']} q/s")
     batched.close()
+    direct.close()
     single.close()
     # queryAndAddDoc on the 1M-image file: new documents are re-observations of pool queries
     lk = rb.DeviceArray((int(qoff[200]),), np.uint32)

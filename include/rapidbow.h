@@ -101,9 +101,10 @@ rbow_status rbow_voc_lookup(const rbow_vocabulary* voc, const void* dev_desc, ui
                             rbow_stream stream);
 
 typedef enum {
-    RBOW_LOOKUP_AUTO = 0,  /* plain below 131,072 descriptors (uint8 trees) / 49,152 (float b16) / 32,768 (float b32), else MFMA */
+    RBOW_LOOKUP_AUTO = 0,  /* plain below 12,288 descriptors (uint8 trees) / 49,152 (float b16) / 32,768 (float b32), else MFMA */
     RBOW_LOOKUP_PLAIN = 1, /* plain scalar per-descriptor descent */
-    RBOW_LOOKUP_MFMA = 2   /* level-synchronous bucketed MFMA form */
+    RBOW_LOOKUP_MFMA = 2   /* MFMA form: fused single launch for small batches on uint8 b16 trees,
+                            * else the level-synchronous bucketed descent */
 } rbow_lookup_kernel;
 /* Lookup with an explicit kernel.  near_ties (optional, device uint32[1],
  * accumulated, not cleared) counts (descriptor, level) pairs whose MFMA
@@ -131,6 +132,11 @@ rbow_status rbow_voc_tune_lookup(rbow_vocabulary* voc, const void* dev_desc, uin
                                  uint32_t* out_crossover, rbow_stream stream);
 rbow_status rbow_voc_set_lookup_crossover(rbow_vocabulary* voc, uint32_t crossover);
 uint32_t rbow_voc_lookup_crossover(const rbow_vocabulary* voc); /* current AUTO crossover */
+/* uint8 b16 trees: the MFMA form runs a fused single-launch descent for batches
+ * of at most fused_max descriptors and the level-synchronous bucketed descent
+ * above (same leaf ids); 0 restores the default (196,608); 0 on other trees. */
+rbow_status rbow_voc_set_lookup_fused_max(rbow_vocabulary* voc, uint32_t fused_max);
+uint32_t rbow_voc_lookup_fused_max(const rbow_vocabulary* voc);
 
 /* RapidBoW.h:69 IVocabulary::serialize.  Two-call form: blob may be NULL to
  * query *size.  uint8/b16 trees use the reference blob byte layout
@@ -209,6 +215,11 @@ typedef struct {
     int32_t lookup_kernel; /* RBOW_LOOKUP_AUTO (default), RBOW_LOOKUP_MFMA or RBOW_LOOKUP_PLAIN, per batch */
     uint64_t staging_bytes; /* pinned host ring that submit copies descriptors into (default 256 MiB,
                              * at least 2 MiB); a submit waits while it is full of queries in flight */
+    uint32_t use_graphs;    /* 1 (default): each batch is one launch of a HIP graph captured per size
+                             * class (lookup, scoring, top-K, result copy); 0: direct launches */
+    uint32_t reserved;
+    uint64_t batch_max_desc; /* descriptors per batch (0 = default min(max_batch * 16384, 2^20)); every
+                              * stream's device buffers are sized for it once, at creation */
 } rbow_server_options;
 void rbow_server_options_default(rbow_server_options* opts);
 rbow_status rbow_server_create(const rbow_vocabulary* voc, const rbow_database* db, const rbow_server_options* opts,
@@ -227,6 +238,27 @@ rbow_status rbow_server_wait(rbow_server* srv, uint64_t ticket, uint32_t* docs, 
 rbow_status rbow_server_replay(rbow_server* srv, const uint8_t* host_desc, const uint64_t* offsets,
                                uint32_t nb_queries, double qps, uint64_t seed, double* latency_ms, uint32_t* docs,
                                float* scores, uint32_t* counts, uint32_t* batch_sizes, double* elapsed_s);
+
+/* ---------------------------------------------------------- pipeline --- */
+/* No counterpart in the reference (Vocabulary.cpp:162-173 lookUp, then
+ * DataBase.cpp:71-190 query, per image): a batch of query images whose
+ * descriptors are already in HBM, through lookup, scoring and top-K, in device
+ * buffers sized once (max_queries images, max_desc descriptors).  With
+ * use_graph the batch is ONE launch of a HIP graph captured for its size class
+ * (descriptors rounded up to a quarter octave, images to a power of two);
+ * graphs are kept and reused, and recaptured after the database changes.
+ * Results as rbow_db_query_batch; returns when done.  A pipeline runs one
+ * batch at a time; the database must not change while a run is in progress. */
+typedef struct rbow_query_pipeline rbow_query_pipeline;
+rbow_status rbow_pipeline_create(const rbow_vocabulary* voc, const rbow_database* db, uint32_t max_queries,
+                                 uint64_t max_desc, uint32_t max_results, rbow_query_pipeline** out);
+void rbow_pipeline_destroy(rbow_query_pipeline* p);
+/* image q owns dev_desc rows host_offsets[q] .. host_offsets[q+1] */
+rbow_status rbow_pipeline_run(rbow_query_pipeline* p, const void* dev_desc, const uint64_t* host_offsets,
+                              uint32_t nb_queries, int32_t lookup_kernel, int32_t use_graph, uint32_t* dev_docs,
+                              float* dev_scores, uint32_t* dev_counts, rbow_stream stream);
+/* graphs captured so far by this pipeline */
+uint32_t rbow_pipeline_graphs(const rbow_query_pipeline* p);
 
 /* ------------------------------------------------ kernel-level entries -- */
 /* These expose the batched k-means kernels the build drives, one pass at a

@@ -44,7 +44,8 @@ class BuildStats(ctypes.Structure):
 class ServerOptions(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_uint32), ("nb_streams", ctypes.c_uint32), ("mem_budget", ctypes.c_uint64),
                 ("max_results", ctypes.c_uint32), ("lookup_kernel", ctypes.c_int32),
-                ("staging_bytes", ctypes.c_uint64)]
+                ("staging_bytes", ctypes.c_uint64), ("use_graphs", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("batch_max_desc", ctypes.c_uint64)]
 
 
 class BuildOptions(ctypes.Structure):
@@ -85,6 +86,8 @@ _sig("rbow_gather_rows", _i32, [_vp, _vp, _u64, _vp, _vp])
 _sig("rbow_voc_tune_lookup", _i32, [_vp, _vp, _u32, ctypes.POINTER(_u32), _vp])
 _sig("rbow_voc_set_lookup_crossover", _i32, [_vp, _u32])
 _sig("rbow_voc_lookup_crossover", _u32, [_vp])
+_sig("rbow_voc_set_lookup_fused_max", _i32, [_vp, _u32])
+_sig("rbow_voc_lookup_fused_max", _u32, [_vp])
 _sig("rbow_db_create", _i32, [_vp, ctypes.POINTER(_vp)])
 _sig("rbow_db_destroy", None, [_vp])
 _sig("rbow_db_add_doc", _i32, [_vp, _u32, _vp, _u32])
@@ -106,6 +109,10 @@ _sig("rbow_server_submit", _i32, [_vp, _vp, _u32, ctypes.POINTER(_u64)])
 _sig("rbow_server_wait", _i32, [_vp, _u64, _vp, _vp, ctypes.POINTER(_u32), ctypes.POINTER(ctypes.c_double)])
 _sig("rbow_server_replay", _i32, [_vp, _vp, _vp, _u32, ctypes.c_double, _u64, _vp, _vp, _vp, _vp, _vp,
                                   ctypes.POINTER(ctypes.c_double)])
+_sig("rbow_pipeline_create", _i32, [_vp, _vp, _u32, _u64, _u32, ctypes.POINTER(_vp)])
+_sig("rbow_pipeline_destroy", None, [_vp])
+_sig("rbow_pipeline_run", _i32, [_vp, _vp, _vp, _u32, _i32, _i32, _vp, _vp, _vp, _vp])
+_sig("rbow_pipeline_graphs", _u32, [_vp])
 _sig("rbow_synth_centres", None, [_u64, _u32, _vp])
 _sig("rbow_synth_descriptors_device", _i32, [_vp, _vp, _u32, _u64, _u64, _u64, _vp, _vp, _vp])
 _sig("rbow_synth_image_centres_device", _i32, [_vp, _u32, _u64, _u64, _u64, _u32, _u32, _vp, _u64, _vp])
@@ -200,6 +207,15 @@ class Vocabulary:
     @lookup_crossover.setter
     def lookup_crossover(self, n):
         check(lib.rbow_voc_set_lookup_crossover(self.handle, n))
+
+    @property
+    def lookup_fused_max(self):
+        """uint8 b16 trees: MFMA-form batches up to this size run the fused single-launch descent."""
+        return lib.rbow_voc_lookup_fused_max(self.handle)
+
+    @lookup_fused_max.setter
+    def lookup_fused_max(self, n):
+        check(lib.rbow_voc_set_lookup_fused_max(self.handle, n))
 
     def serialize(self):
         size = _sz(0)
@@ -401,15 +417,45 @@ class DataBase:
         check(lib.rbow_db_compact(self.handle, _stream(stream)))
 
 
+class QueryPipeline:
+    """A batch of query images already in HBM through lookup, scoring and top-K, as one
+    launch of a captured HIP graph (use_graph) or as direct launches."""
+
+    def __init__(self, voc, db, max_queries, max_desc, max_results=50):
+        h = _vp()
+        check(lib.rbow_pipeline_create(voc.handle, db.handle, max_queries, max_desc, max_results, ctypes.byref(h)))
+        self.handle, self.K, self._keep = h.value, max_results, (voc, db)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.rbow_pipeline_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, dev_desc, offsets, dev_docs, dev_scores, dev_counts, lookup_kernel=LOOKUP_AUTO, use_graph=True,
+            stream=None):
+        off = np.ascontiguousarray(offsets, np.uint64)
+        check(lib.rbow_pipeline_run(self.handle, _ptr(dev_desc), off.ctypes.data, off.size - 1, lookup_kernel,
+                                    int(bool(use_graph)), _ptr(dev_docs), _ptr(dev_scores), _ptr(dev_counts),
+                                    _stream(stream)))
+
+    @property
+    def graphs(self):
+        return lib.rbow_pipeline_graphs(self.handle)
+
+
 class Server:
     """Continuous-batching query server (no counterpart in the reference)."""
 
     def __init__(self, voc, db, max_batch=64, nb_streams=2, mem_budget=4 << 30, max_results=50,
-                 lookup_kernel=LOOKUP_AUTO, staging_bytes=None):
+                 lookup_kernel=LOOKUP_AUTO, staging_bytes=None, use_graphs=True, batch_max_desc=0):
         o = ServerOptions()
         lib.rbow_server_options_default(ctypes.byref(o))
         o.max_batch, o.nb_streams, o.mem_budget, o.max_results, o.lookup_kernel = (
             max_batch, nb_streams, mem_budget, max_results, lookup_kernel)
+        o.use_graphs, o.batch_max_desc = int(bool(use_graphs)), batch_max_desc
         if staging_bytes is not None:
             o.staging_bytes = staging_bytes
         h = _vp()

@@ -1,8 +1,8 @@
 """Builds the gfx950 shared library librapidbow_amd.so in-tree with hipcc.
 
-Every HIP source is compiled for gfx950 only; packed-FP32 instruction
-selection is disabled for all device code.  Host-only sources go through the
-same driver.  The oracle under oracle/ is built separately by its Makefile.
+Every source is compiled for gfx950 only (hipcc treats the host-only .cpp
+files as HIP too, so they get the same target); packed-FP32 instruction
+selection is disabled for all device code.  The oracle under oracle/ is built separately by its Makefile.
 """
 import os
 import subprocess
@@ -14,7 +14,8 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librapidbow_amd.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["lookup.hip", "kmeans.hip", "database.hip", "vocabulary.cpp", "database.cpp", "capi.cpp", "server.cpp", "synth.cpp", "synth.hip"]
+SOURCES = ["lookup.hip", "kmeans.hip", "database.hip", "vocabulary.cpp", "database.cpp", "capi.cpp", "server.cpp",
+           "pipeline.cpp", "synth.cpp", "synth.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-Wall", "-Wno-unused-result", f"-I{os.path.join(ROOT, 'include')}"]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -38,11 +39,9 @@ def build(verbose=False):
         src = os.path.join(CSRC, s)
         obj = os.path.join(BUILD_DIR, s.replace(".", "_") + ".o")
         objs.append(obj)
-        cmd = [HIPCC] + COMMON + ["-c", src, "-o", obj]
-        if s.endswith(".hip"):
-            cmd[1:1] = DEVICE
-        else:
-            cmd += ["-fopenmp"] if s == "synth.cpp" else []
+        cmd = [HIPCC] + DEVICE + COMMON + ["-c", src, "-o", obj]  # host sources too: gfx950 is the only target
+        if s == "synth.cpp":
+            cmd += ["-fopenmp"]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
     errors = []
     for cmd, p in procs:

@@ -29,6 +29,10 @@ struct rbow_server {
     std::unique_ptr<Server> srv;
 };
 
+struct rbow_query_pipeline {
+    std::unique_ptr<QueryPipeline> p;
+};
+
 namespace {
 thread_local std::string g_lastError;
 
@@ -271,6 +275,19 @@ rbow_status rbow_voc_set_lookup_crossover(rbow_vocabulary* voc, uint32_t crossov
         RBOW_REQUIRE(voc != nullptr, RBOW_ERR_INVALID_ARG, "null vocabulary");
         voc->v->lookupAutoMin = crossover;
     });
+}
+
+rbow_status rbow_voc_set_lookup_fused_max(rbow_vocabulary* voc, uint32_t fused_max) {
+    return guarded([&] {
+        RBOW_REQUIRE(voc != nullptr, RBOW_ERR_INVALID_ARG, "null vocabulary");
+        voc->v->lookupFusedMax = fused_max;
+    });
+}
+
+uint32_t rbow_voc_lookup_fused_max(const rbow_vocabulary* voc) {
+    if (!voc) return 0;
+    const TreeView t = voc->v->view();
+    return t.fusedMax ? t.fusedMax : lookupFusedDefault(t);
 }
 
 uint32_t rbow_voc_lookup_crossover(const rbow_vocabulary* voc) {
@@ -521,6 +538,9 @@ void rbow_server_options_default(rbow_server_options* o) {
     o->max_results = 50;
     o->lookup_kernel = RBOW_LOOKUP_AUTO;
     o->staging_bytes = 256ull << 20;
+    o->use_graphs = 1;
+    o->reserved = 0;
+    o->batch_max_desc = 0;
 }
 
 rbow_status rbow_server_create(const rbow_vocabulary* voc, const rbow_database* db, const rbow_server_options* opts,
@@ -540,6 +560,8 @@ rbow_status rbow_server_create(const rbow_vocabulary* voc, const rbow_database* 
         so.K = o.max_results;
         so.lookupKernel = o.lookup_kernel;
         so.stagingBytes = o.staging_bytes;
+        so.useGraphs = o.use_graphs != 0;
+        so.batchMaxDesc = o.batch_max_desc;
         std::unique_ptr<rbow_server> h(new rbow_server);
         h->srv.reset(new Server(*voc->v, *db->db, so));
         *out = h.release();
@@ -547,6 +569,61 @@ rbow_status rbow_server_create(const rbow_vocabulary* voc, const rbow_database* 
 }
 
 void rbow_server_destroy(rbow_server* srv) { delete srv; }
+
+// ------------------------------------------------------------ pipeline --
+
+rbow_status rbow_pipeline_create(const rbow_vocabulary* voc, const rbow_database* db, uint32_t max_queries,
+                                 uint64_t max_desc, uint32_t max_results, rbow_query_pipeline** out) {
+    return guarded([&] {
+        RBOW_REQUIRE(voc && db && out, RBOW_ERR_INVALID_ARG, "null argument");
+        RBOW_REQUIRE(&db->db->vocabulary() == voc->v.get(), RBOW_ERR_INVALID_ARG,
+                     "the database was not created on this vocabulary");
+        *out = nullptr;
+        db->db->prepareForQueries(nullptr);
+        std::unique_ptr<rbow_query_pipeline> h(new rbow_query_pipeline);
+        h->p.reset(new QueryPipeline(*voc->v, *db->db, max_queries, max_desc, max_results));
+        *out = h.release();
+    });
+}
+
+void rbow_pipeline_destroy(rbow_query_pipeline* p) { delete p; }
+
+rbow_status rbow_pipeline_run(rbow_query_pipeline* p, const void* dev_desc, const uint64_t* host_offsets,
+                              uint32_t nb_queries, int32_t lookup_kernel, int32_t use_graph, uint32_t* dev_docs,
+                              float* dev_scores, uint32_t* dev_counts, rbow_stream stream) {
+    return guarded([&] {
+        RBOW_REQUIRE(p && host_offsets && dev_docs && dev_counts, RBOW_ERR_INVALID_ARG, "null argument");
+        RBOW_REQUIRE(lookup_kernel >= RBOW_LOOKUP_AUTO && lookup_kernel <= RBOW_LOOKUP_MFMA, RBOW_ERR_INVALID_ARG,
+                     "bad kernel id");
+        if (!nb_queries) return;
+        QueryPipeline& P = *p->p;
+        RBOW_REQUIRE(nb_queries <= P.maxQueries(), RBOW_ERR_CAPACITY, "batch above the pipeline's query capacity");
+        std::vector<uint64_t> off(nb_queries + 1);
+        for (uint32_t q = 0; q <= nb_queries; q++) {
+            RBOW_REQUIRE(host_offsets[q] >= host_offsets[0], RBOW_ERR_INVALID_ARG, "offsets must be non-decreasing");
+            off[q] = host_offsets[q] - host_offsets[0];
+        }
+        const uint64_t n = off[nb_queries];
+        RBOW_REQUIRE(n <= P.maxDesc(), RBOW_ERR_CAPACITY, "batch above the pipeline's descriptor capacity");
+        RBOW_REQUIRE(dev_desc || n == 0, RBOW_ERR_INVALID_ARG, "null descriptors");
+        hipStream_t s = hs(stream);
+        if (n) {
+            checkAligned(dev_desc, "descriptors");
+            RBOW_HIP(hipMemcpyAsync(P.descBuffer(), static_cast<const uint8_t*>(dev_desc) + host_offsets[0] * kDims,
+                                    n * kDims, hipMemcpyDeviceToDevice, s));
+        }
+        P.run(off.data(), nb_queries, lookup_kernel, use_graph != 0, false, s);
+        const uint8_t* res = P.deviceResults();
+        const uint64_t K = P.K();
+        RBOW_HIP(hipMemcpyAsync(dev_docs, P.docs(res, 0), 4 * K * nb_queries, hipMemcpyDeviceToDevice, s));
+        if (dev_scores)
+            RBOW_HIP(hipMemcpyAsync(dev_scores, P.scores(res, 0), 4 * K * nb_queries, hipMemcpyDeviceToDevice, s));
+        RBOW_HIP(hipMemcpyAsync(dev_counts, P.counts(res), 4ull * nb_queries, hipMemcpyDeviceToDevice, s));
+        RBOW_HIP(hipStreamSynchronize(s));  // the pipeline's buffers are free again on return
+    });
+}
+
+uint32_t rbow_pipeline_graphs(const rbow_query_pipeline* p) { return p ? p->p->graphsCaptured() : 0u; }
 
 rbow_status rbow_server_submit(rbow_server* srv, const uint8_t* host_desc, uint32_t n, uint64_t* ticket) {
     return guarded([&] {

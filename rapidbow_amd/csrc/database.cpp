@@ -150,6 +150,7 @@ void Database::mergeDeltaIntoMain(hipStream_t s) const {
 void Database::syncDevice(hipStream_t s) const {
     std::lock_guard<std::mutex> g(mergeMtx_);
     const uint32_t nDocs = uint32_t(docIds_.size());
+    if (nDocs > docsOnDevice_ || !pendLeaf_.empty()) version_++;
     if (nDocs > docsOnDevice_) {
         growDocs(nDocs, s);
         RBOW_HIP(hipMemcpyAsync(rcp_.get() + docsOnDevice_, rcpHost_.data() + docsOnDevice_,
@@ -169,6 +170,7 @@ void Database::compact(hipStream_t s) {
     std::unique_lock<std::shared_mutex> lk(mtx_);
     syncDevice(s);
     std::lock_guard<std::mutex> g(mergeMtx_);
+    version_++;
     mergeDeltaIntoMain(s);
     refreshDir(s);
     RBOW_HIP(hipStreamSynchronize(s));
@@ -210,6 +212,7 @@ void Database::addDocsDevice(const uint32_t* docs, uint32_t nbDocs, const uint32
     }
     syncDevice(s);  // host-added postings keep their (smaller) slots first in every leaf
     std::lock_guard<std::mutex> g(mergeMtx_);
+    version_++;
     const uint32_t first = uint32_t(docIds_.size());
     for (uint32_t d = 0; d < nbDocs; d++) {
         slotOf_[docs[d]] = first + d;
@@ -289,6 +292,23 @@ void Database::queryBatch(const uint32_t* dLeaf, const uint64_t* offsets, uint32
     RBOW_HIP(hipFreeAsync(dOff, s));
     RBOW_HIP(hipStreamSynchronize(s));  // h (host) must outlive the copy; results ready on return
     RBOW_REQUIRE(hBad == 0, RBOW_ERR_INVALID_ARG, "leaf id out of range");
+}
+
+DbView Database::scoringView() const {
+    std::shared_lock<std::shared_mutex> lk(mtx_);
+    RBOW_REQUIRE(pendLeaf_.empty() && docsOnDevice_ == docIds_.size(), RBOW_ERR_INVALID_ARG,
+                 "database has unmerged inserts");
+    return view();
+}
+
+bool Database::hasPendingInserts() const {
+    std::shared_lock<std::shared_mutex> lk(mtx_);
+    return !pendLeaf_.empty() || docsOnDevice_ != docIds_.size();
+}
+
+uint64_t Database::version() const {
+    std::shared_lock<std::shared_mutex> lk(mtx_);
+    return version_;
 }
 
 void Database::prepareForQueries(hipStream_t s) const {

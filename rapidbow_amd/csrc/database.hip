@@ -214,6 +214,13 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
     const uint32_t q = blockIdx.x, tid = threadIdx.x;
     const uint64_t beg = off[q];
     const uint32_t n = uint32_t(off[q + 1] - beg);
+    if (n == 0) {  // an empty query (also the padding of a fixed-size batch) scores nothing
+        if (tid == 0) {
+            nodeCount[2 * q] = 0;
+            nodeCount[2 * q + 1] = 0;
+        }
+        return;
+    }
     uint32_t n2 = 1024;
     while (n2 < n) n2 <<= 1;
     for (uint32_t i = tid; i < n2; i += 1024) s[i] = i < n ? leaf[beg + i] : 0xFFFFFFFFu;
@@ -461,6 +468,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     __shared__ float sQE[kSegBatch], sW[kSegBatch];
     __shared__ uint32_t sHist[256], sRed[16], sCount;
     const uint32_t r = blockIdx.x, qc = blockIdx.y, q = q0 + qc, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (nodeCount[2 * q] == 0 && nodeCount[2 * q + 1] == 0) {  // nothing to score (an empty query)
+        if (tid == 0) candCount[uint64_t(q) * v.R + r] = 0;
+        return;
+    }
     const uint32_t s0 = r * S, ns = min(S, v.nbDocs - s0);
     for (uint32_t i = tid; i < S; i += 1024) {
         sAcc[i] = 0.0;
@@ -843,18 +854,21 @@ void launchMergeCsr(uint64_t nbLeaves, const DevCsr& a, const DevCsr& b, DevCsr&
     out.n = a.n + b.n;
 }
 
-void launchQueryBatch(const DbView& v, const uint32_t* dLeaf, const uint64_t* dOffsets, const uint64_t* hOffsets,
-                      uint32_t nbQueries, uint32_t K, uint32_t* dDocs, float* dScores, uint32_t* dCounts,
-                      uint32_t* badFlag, hipStream_t s) {
+size_t queryWorkspaceBytes(const DbView& v, uint64_t totalDesc, uint32_t nbQueries, uint32_t K) {
+    const uint64_t nbCand = uint64_t(nbQueries) * v.R * K;
+    const uint64_t nbNodes = std::max<uint64_t>(totalDesc * v.L, 1);
+    return sizeof(QNode) * nbNodes + 8 * nbCand + 4ull * nbQueries * v.R + 8ull * nbQueries + 1024;
+}
+
+void launchQueryBatchInWs(const DbView& v, const uint32_t* dLeaf, const uint64_t* dOffsets, uint64_t totalDesc,
+                          uint32_t nbQueries, uint32_t K, uint32_t* dDocs, float* dScores, uint32_t* dCounts,
+                          uint32_t* badFlag, void* ws, size_t wsBytes, hipStream_t s) {
     if (!nbQueries) return;
     RBOW_REQUIRE(v.R == divUp(v.nbDocs, Database::kRangeDocs) && v.R < 65536 && v.Rm <= v.R && (v.dir || !v.Rm),
                  RBOW_ERR_HIP, "stale range directory");
-    const uint64_t total = hOffsets[nbQueries];
+    RBOW_REQUIRE(wsBytes >= queryWorkspaceBytes(v, totalDesc, nbQueries, K), RBOW_ERR_HIP, "query workspace too small");
     const uint64_t nbCand = uint64_t(nbQueries) * v.R * K;
-    const uint64_t nbNodes = std::max<uint64_t>(total * v.L, 1);
-    const size_t bytes = sizeof(QNode) * nbNodes + 8 * nbCand + 4ull * nbQueries * v.R + 8ull * nbQueries + 1024;
-    void* ws = nullptr;
-    RBOW_HIP(hipMallocAsync(&ws, bytes, s));
+    const uint64_t nbNodes = std::max<uint64_t>(totalDesc * v.L, 1);
     QNode* nodes = static_cast<QNode*>(ws);
     uint64_t* candKeys = reinterpret_cast<uint64_t*>(nodes + nbNodes);
     uint32_t* candCount = reinterpret_cast<uint32_t*>(candKeys + nbCand);
@@ -866,6 +880,17 @@ void launchQueryBatch(const DbView& v, const uint32_t* dLeaf, const uint64_t* dO
                                                                                   nodeCount, K, candKeys, candCount);
     k_topk<<<nbQueries, 1024, 0, s>>>(v, K, candKeys, candCount, dDocs, dScores, dCounts);
     RBOW_HIP(hipGetLastError());
+}
+
+void launchQueryBatch(const DbView& v, const uint32_t* dLeaf, const uint64_t* dOffsets, const uint64_t* hOffsets,
+                      uint32_t nbQueries, uint32_t K, uint32_t* dDocs, float* dScores, uint32_t* dCounts,
+                      uint32_t* badFlag, hipStream_t s) {
+    if (!nbQueries) return;
+    const uint64_t total = hOffsets[nbQueries] - hOffsets[0];
+    const size_t bytes = queryWorkspaceBytes(v, total, nbQueries, K);
+    void* ws = nullptr;
+    RBOW_HIP(hipMallocAsync(&ws, bytes, s));
+    launchQueryBatchInWs(v, dLeaf, dOffsets, total, nbQueries, K, dDocs, dScores, dCounts, badFlag, ws, bytes, s);
     RBOW_HIP(hipFreeAsync(ws, s));
 }
 

@@ -84,6 +84,11 @@ public:
     void prepareForQueries(hipStream_t s) const;
     void scoreAsync(const uint32_t* dLeaf, const uint64_t* dOffsets, const uint64_t* hOffsets, uint32_t nbQueries,
                     uint32_t K, uint32_t* dDocs, float* dScores, uint32_t* dCounts, hipStream_t s) const;
+    // The device view the scoring kernels read (after prepareForQueries, with no
+    // insert pending), and a counter that changes whenever that view does.
+    DbView scoringView() const;
+    uint64_t version() const;
+    bool hasPendingInserts() const;  // inserts not yet folded into the device state
     // HBM scratch of one query of n descriptors, top-K, during scoring
     uint64_t scratchBytes(uint32_t n, uint32_t K) const {
         return uint64_t(divUp(nbDocs(), kRangeDocs)) * (8ull * K + 4) + uint64_t(n) * L_ * kQNodeBytes + 64;
@@ -130,6 +135,7 @@ private:
     mutable uint32_t dirR_ = 0;
     mutable bool dirStale_ = true;
     mutable DevBuf<uint32_t> flag_;  // device error flag of the leaf-id checks
+    mutable uint64_t version_ = 0;   // changes with every change of the scoring view
     mutable std::vector<uint32_t> pendLeaf_, pendSlot_;
     mutable std::shared_mutex mtx_;
     mutable std::mutex mergeMtx_;
@@ -150,6 +156,12 @@ void launchBuildDir(const DbView& v, uint32_t* dir, hipStream_t s);
 void launchExpandSlots(const uint64_t* off, uint32_t nbDocs, uint32_t firstSlot, uint32_t* slots, hipStream_t s);
 // flag[0] += number of ids in leaf[0, n) that are >= nbLeaves (flag zeroed by the caller)
 void launchCountBadLeaves(const uint32_t* leaf, uint64_t n, uint64_t nbLeaves, uint32_t* flag, hipStream_t s);
+// Scoring workspace of a batch of nbQueries queries holding totalDesc leaf ids,
+// and the batch launched in a caller's workspace (no allocation).
+size_t queryWorkspaceBytes(const DbView& v, uint64_t totalDesc, uint32_t nbQueries, uint32_t K);
+void launchQueryBatchInWs(const DbView& v, const uint32_t* dLeaf, const uint64_t* dOffsets, uint64_t totalDesc,
+                          uint32_t nbQueries, uint32_t K, uint32_t* dDocs, float* dScores, uint32_t* dCounts,
+                          uint32_t* badFlag, void* ws, size_t wsBytes, hipStream_t s);
 // badFlag (optional, device u32, zeroed by the caller): set to nonzero when a
 // query holds a leaf id >= nbLeaves; such a query scores nothing.
 void launchQueryBatch(const DbView& v, const uint32_t* dLeaf, const uint64_t* dOffsets, const uint64_t* hOffsets,

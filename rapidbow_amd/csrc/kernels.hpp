@@ -30,6 +30,8 @@ struct TreeView {
     const float* e1 = nullptr;
     const float* e2 = nullptr;
     uint32_t autoMin = 0;  // AUTO lookup: batches below this use the plain kernel (0: built-in default)
+    uint32_t fusedMax = 0; // uint8 b16 trees: MFMA-form batches of at most this many descriptors run the
+                           // fused single-launch descent, larger ones the bucketed one (0: built-in default)
 };
 
 // Derived per-child data (norms, bf16 split, bounds) for nodes [0, nbNodes); the
@@ -49,6 +51,10 @@ void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32
 // (t.autoMin when tuned, else a default measured on MI355X).
 // AUTO crossover of a tree kind when no tuned value is set (measured on MI355X)
 uint32_t lookupAutoDefault(uint32_t centroidType, uint32_t b);
+// Whether an MFMA-form lookup of n descriptors runs the fused single-launch
+// descent (uint8 b16 trees, n <= t.fusedMax or its default), and that default.
+bool lookupUsesFused(const TreeView& t, uint32_t n);
+uint32_t lookupFusedDefault(const TreeView& t);
 void launchLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies, int kernel,
                   hipStream_t s);
 // The same lookup without any allocation: the MFMA form's workspace is a static
@@ -58,6 +64,13 @@ void launchLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* 
 void launchLookupNoAlloc(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
                          int kernel, hipStream_t s);
 size_t lookupScratchBytes();
+// The kernel a lookup of n descriptors runs (AUTO resolved), the workspace that
+// form needs (0 for the plain one), and the lookup in a caller's workspace
+// (256-byte aligned), allocating nothing.
+bool lookupUsesPlain(const TreeView& t, uint32_t n, int kernel);
+size_t lookupWorkspaceBytes(const TreeView& t, uint32_t n, bool plain, hipStream_t s);
+void launchLookupInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
+                      bool plain, void* ws, size_t wsBytes, hipStream_t s);
 // Autotuner: times both lookup kernels on prefixes of desc (powers of two from
 // 4096 up to n) and returns the smallest size at which the MFMA form is at
 // least as fast (n + 1 if it never is).  Blocks; uses `leaf` (n words) as output.

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <utility>
 
 #include "bf16filter.hpp"
 #include "kernels.hpp"
@@ -394,6 +395,140 @@ __global__ __launch_bounds__(256) void k_levelI8(const uint8_t* __restrict__ row
     }
 }
 
+// ------------------------------------------- fused I8 descent (b = 16) --
+template <int... R, class F>
+__device__ inline void unrollSeq(F&& f, std::integer_sequence<int, R...>) {
+    (f(std::integral_constant<int, R>{}), ...);
+}
+template <class F>
+__device__ inline void unroll16(F&& f) {
+    unrollSeq(f, std::make_integer_sequence<int, 16>{});
+}
+
+// Every level of a uint8 b16 tree in one launch, for batches too small to
+// bucket.  A wave takes 16 descriptors at a time.  Level 0: the root's 16 children as the MFMA A operand against the
+// 16 rows as B (as k_levelI8).  Every later level, row by row: the row's node's
+// 16 children as A and the row broadcast to all 16 columns as B (DPP, from the
+// registers that hold it), so every lane
+// holds four children's scores of that row; the same in-lane minimum and
+// permlane swaps give the row's child (ties to the lowest index).  The child
+// rows of levels 0 and 1 (17 nodes, 38 KB at the 144-byte pitch) are staged in
+// LDS once per workgroup; deeper levels read them from L2, the next row's
+// operands in flight while one computes.  Same exact integer keys as the plain
+// and bucketed kernels, hence the same leaf ids.
+constexpr uint32_t kFusedPitch = 144;   // LDS bytes per 128-byte row: conflict-free 16-byte reads
+constexpr uint32_t kFusedNodes = 17;    // nodes of levels 0 and 1 staged in LDS
+static_assert(kFusedNodes * 16 <= 512, "the staging loads at most two child norms per thread");
+
+__device__ inline uint32_t rowKey(const v4i& acc, const v4i& ns, uint32_t grp) {
+    uint32_t key = 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        key = min(key, (uint32_t(ns[i] - 2 * acc[i] + kScoreBias) << 4) | (4 * grp + uint32_t(i)));
+    return minAcrossGroupsU(key);
+}
+
+// Lane r of each 16-lane row of the wave, to every lane of that row (DPP row_newbcast).
+template <int R>
+__device__ inline v4i rowBcast(const v4i& x) {
+    v4i y;
+#pragma unroll
+    for (int i = 0; i < 4; i++) y[i] = __builtin_amdgcn_update_dpp(0, x[i], 0x150 + R, 0xF, 0xF, false);
+    return y;
+}
+
+// One per-row level of the fused descent: row r (0..15) of the wave's block sits
+// at node (lane r's value) of this level; returns, for this lane's row col, the
+// index within the next level.  Row r's B operand (its bytes at dims 16*grp and
+// 64 + 16*grp, held by lane r of each 16-lane group) is broadcast from registers.
+// The A operands of D rows are in flight at once (a register ring; the loop is
+// unrolled, so every index is a constant).
+template <bool kLds, int D>
+__device__ inline uint32_t fusedLevel(const uint8_t* sRows, const int32_t* sNorm, const uint8_t* __restrict__ rows,
+                                      const int32_t* __restrict__ normS, uint64_t lb, const v4i& x0, const v4i& x1,
+                                      uint32_t node, uint32_t col, uint32_t grp) {
+    v4i a0[D], a1[D], ns[D];
+    auto loadA = [&](uint32_t v, v4i& x0, v4i& x1, v4i& n4) {
+        if (kLds) {  // level 1: node v is staged node 1 + v
+            const uint8_t* c = sRows + ((1 + v) * 16 + col) * kFusedPitch;
+            x0 = *reinterpret_cast<const v4i*>(c + grp * 16);
+            x1 = *reinterpret_cast<const v4i*>(c + 64 + grp * 16);
+            n4 = *reinterpret_cast<const v4i*>(sNorm + (1 + v) * 16 + 4 * grp);
+        } else {
+            const uint64_t child = (lb + v) * 16;
+            const uint8_t* c = rows + (child + col) * kDims;
+            x0 = toSigned(ld16(c + grp * 16));
+            x1 = toSigned(ld16(c + 64 + grp * 16));
+            n4 = *reinterpret_cast<const v4i*>(normS + child + 4 * grp);
+        }
+    };
+#pragma unroll
+    for (int r = 0; r < D; r++) loadA(uint32_t(__builtin_amdgcn_readlane(int(node), r)), a0[r], a1[r], ns[r]);
+    uint32_t next = 0;
+    auto step = [&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        const uint32_t v = uint32_t(__builtin_amdgcn_readlane(int(node), r));  // row r's node
+        const v4i b0 = rowBcast<r>(x0), b1 = rowBcast<r>(x1);                   // row r in every column
+        v4i d = {0, 0, 0, 0};
+        d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[r % D], b0, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[r % D], b1, d, 0, 0, 0);
+        const uint32_t key = rowKey(d, ns[r % D], grp);
+        if (r + D < 16) loadA(uint32_t(__builtin_amdgcn_readlane(int(node), r + D)), a0[r % D], a1[r % D], ns[r % D]);
+        if (col == uint32_t(r)) next = v * 16 + (key & 15u);
+    };
+    unroll16(step);
+    return next;
+}
+
+__global__ __launch_bounds__(256) void k_lookupFusedI8(const uint8_t* __restrict__ rows,
+                                                      const int32_t* __restrict__ normS, uint32_t L,
+                                                      const uint8_t* __restrict__ desc, uint32_t n,
+                                                      uint32_t* __restrict__ leaf) {
+    __shared__ __align__(16) uint8_t sRows[kFusedNodes * 16 * kFusedPitch];  // children (c - 128), node-major
+    __shared__ __align__(16) int32_t sNorm[kFusedNodes * 16];                // ||c - 128||^2
+    // stage the children of levels 0-1 (16-byte chunks: child i / 8, chunk i % 8), every load in flight at once
+    const uint32_t staged = L >= 2 ? kFusedNodes : 1u;
+    constexpr int kIt = (kFusedNodes * 16 * 8 + 255) / 256;
+    v4i st[kIt];
+#pragma unroll
+    for (int j = 0; j < kIt; j++) {
+        const uint32_t i = threadIdx.x + 256 * j;
+        if (i < staged * 128) st[j] = ld16(rows + uint64_t(i >> 3) * kDims + 16 * (i & 7));
+    }
+    const int32_t nrm0 = threadIdx.x < staged * 16 ? normS[threadIdx.x] : 0;  // 272 norms: two per thread
+    const int32_t nrm1 = threadIdx.x + 256 < staged * 16 ? normS[threadIdx.x + 256] : 0;
+#pragma unroll
+    for (int j = 0; j < kIt; j++) {
+        const uint32_t i = threadIdx.x + 256 * j;
+        if (i < staged * 128) *reinterpret_cast<v4i*>(sRows + (i >> 3) * kFusedPitch + 16 * (i & 7)) = toSigned(st[j]);
+    }
+    if (threadIdx.x < staged * 16) sNorm[threadIdx.x] = nrm0;
+    if (threadIdx.x + 256 < staged * 16) sNorm[threadIdx.x + 256] = nrm1;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t col = lane & 15, grp = lane >> 4;
+    for (uint32_t blk = blockIdx.x * 4 + wave; blk * 16 < n; blk += gridDim.x * 4) {
+        const uint32_t row = blk * 16 + col;  // this lane's row of the block
+        v4i x0 = {0, 0, 0, 0}, x1 = {0, 0, 0, 0};  // rows past n score garbage and are not written
+        if (row < n) {
+            x0 = toSigned(ld16(desc + uint64_t(row) * kDims + grp * 16));
+            x1 = toSigned(ld16(desc + uint64_t(row) * kDims + 64 + grp * 16));
+        }
+        // level 0: the root's children (A) against the 16 rows (B)
+        v4i acc = {0, 0, 0, 0};
+        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(*reinterpret_cast<const v4i*>(sRows + col * kFusedPitch + grp * 16),
+                                                    x0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+            *reinterpret_cast<const v4i*>(sRows + col * kFusedPitch + 64 + grp * 16), x1, acc, 0, 0, 0);
+        uint32_t node = rowKey(acc, *reinterpret_cast<const v4i*>(sNorm + 4 * grp), grp) & 15u;  // in level 1
+        for (uint32_t l = 1; l < L; l++) {
+            if (l == 1) node = fusedLevel<true, 2>(sRows, sNorm, rows, normS, 0, x0, x1, node, col, grp);
+            else node = fusedLevel<false, 4>(sRows, sNorm, rows, normS, levelBeg(4, l), x0, x1, node, col, grp);
+        }
+        if (grp == 0 && row < n) leaf[row] = node;
+    }
+}
+
 // ----------------------------------------------- split-BF16 level step --
 // mfma_f32_16x16x32_bf16: lane l feeds 8 dims [8*(l>>4) + 32*kb, +8), kb=0..3,
 // of A row (l&15) and of B column (l&15).  The node's children are A (16 per
@@ -713,11 +848,47 @@ void* scratchOfCurrentDevice() {
 }
 }  // namespace
 
+uint32_t lookupFusedDefault(const TreeView& t) { return t.centroidType == 0 && t.b == 16 ? 196608u : 0u; }
+
+bool lookupUsesFused(const TreeView& t, uint32_t n) {
+    if (t.centroidType != 0 || t.b != 16 || t.L == 0) return false;
+    return n <= (t.fusedMax ? t.fusedMax : lookupFusedDefault(t));
+}
+
+namespace {
+// Workgroups of the fused descent the device holds at once (4 per CU: 40 KB of
+// LDS and at most 128 VGPRs each); the grid never exceeds them, so each
+// workgroup stages the upper levels once and then walks its share of the blocks.
+uint32_t fusedResidentBlocks() {
+    static std::mutex m;
+    static uint32_t cached[256] = {};
+    int d = 0;
+    RBOW_HIP(hipGetDevice(&d));
+    std::lock_guard<std::mutex> g(m);
+    if (d < 0 || d >= 256) return 1024;
+    if (!cached[d]) {
+        int cus = 0;
+        RBOW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d));
+        cached[d] = 4u * uint32_t(std::max(cus, 1));
+    }
+    return cached[d];
+}
+
+void launchLookupFused(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s) {
+    k_lookupFusedI8<<<gridFor(n, 64, fusedResidentBlocks()), 256, 0, s>>>(t.u8rows, t.normS, t.L, desc, n, leaf);
+    RBOW_HIP(hipGetLastError());
+}
+}  // namespace
+
 void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
                       hipStream_t s) {
     if (!n) return;
     if (t.L == 0) {
         launchFillU32(leaf, 0u, n, s);
+        return;
+    }
+    if (lookupUsesFused(t, n)) {
+        launchLookupFused(t, desc, n, leaf, s);
         return;
     }
     if (n > kChunk) {  // independent chunks
@@ -739,12 +910,35 @@ void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32
 
 size_t lookupScratchBytes() { return kScratchBytes; }
 
+bool lookupUsesPlain(const TreeView& t, uint32_t n, int kernel) {
+    const uint32_t autoMin = t.autoMin ? t.autoMin : lookupAutoDefault(t.centroidType, t.b);
+    return kernel == RBOW_LOOKUP_PLAIN || t.L == 0 || (kernel == RBOW_LOOKUP_AUTO && n < autoMin);
+}
+
+size_t lookupWorkspaceBytes(const TreeView& t, uint32_t n, bool plain, hipStream_t s) {
+    if (plain || n == 0 || t.L <= 1) return 0;
+    RBOW_REQUIRE(n <= kChunk, RBOW_ERR_CAPACITY, "lookup batch above 2^26 descriptors");
+    return lookupWsLayout(t, n, s).bytes;
+}
+
+void launchLookupInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
+                      bool plain, void* ws, size_t wsBytes, hipStream_t s) {
+    if (!n) return;
+    if (plain || t.L <= 1 || lookupUsesFused(t, n)) {
+        launchLookup(t, desc, n, leaf, nearTies, plain ? RBOW_LOOKUP_PLAIN : RBOW_LOOKUP_MFMA, s);
+        return;
+    }
+    const LookupWs w = lookupWsLayout(t, n, s);
+    RBOW_REQUIRE(n <= kChunk && w.bytes <= wsBytes, RBOW_ERR_HIP, "lookup workspace too small");
+    lookupMfmaInWs(t, desc, n, leaf, nearTies, w, ws, s);
+}
+
 void launchLookupNoAlloc(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
                          int kernel, hipStream_t s) {
     const uint32_t autoMin = t.autoMin ? t.autoMin : lookupAutoDefault(t.centroidType, t.b);
     const bool plain = kernel == RBOW_LOOKUP_PLAIN || t.L == 0 || (kernel == RBOW_LOOKUP_AUTO && n < autoMin);
     if (!n) return;
-    if (plain || t.L == 1) {  // neither form needs a workspace here
+    if (plain || t.L == 1 || lookupUsesFused(t, n)) {  // none of these forms needs a workspace
         launchLookup(t, desc, n, leaf, nearTies, kernel, s);
         return;
     }
@@ -758,7 +952,7 @@ void launchLookupNoAlloc(const TreeView& t, const uint8_t* desc, uint32_t n, uin
 uint32_t lookupAutoDefault(uint32_t centroidType, uint32_t b) {
     // measured crossovers on MI355X (256-descriptor items): uint8 trees ~140k
     // descriptors, float b16 ~48k, float b32 below 32k
-    return centroidType == 0 ? 131072u : b == 16 ? 49152u : 32768u;
+    return centroidType == 0 ? 12288u : b == 16 ? 49152u : 32768u;
 }
 
 void launchLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies, int kernel,

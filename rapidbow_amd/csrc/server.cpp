@@ -19,11 +19,16 @@ Server::Server(const Vocabulary& voc, const Database& db, const ServerOptions& o
     RBOW_HIP(hipDeviceSynchronize());
     ringCap_ = o.stagingBytes;
     RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&ring_), ringCap_));
+    if (!opt_.batchMaxDesc)
+        opt_.batchMaxDesc = std::max<uint64_t>(Database::kMaxQueryDesc,
+                                               std::min<uint64_t>(uint64_t(o.maxBatch) * Database::kMaxQueryDesc, 1u << 20));
+    RBOW_REQUIRE(opt_.batchMaxDesc >= Database::kMaxQueryDesc && opt_.batchMaxDesc < (1ull << 26),
+                 RBOW_ERR_INVALID_ARG, "batch_max_desc must be in [16384, 2^26)");
     slots_.resize(o.nbStreams);
     for (auto& s : slots_) {
         RBOW_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));  // default priority
         RBOW_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
-        RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.hOut), 8ull * o.maxBatch * o.K + 4ull * o.maxBatch));
+        s.pipe.reset(new QueryPipeline(voc_, db_, o.maxBatch, opt_.batchMaxDesc, o.K));
     }
     t0_ = Clock::now();
     worker_ = std::thread([this] { loop(); });
@@ -40,7 +45,7 @@ Server::~Server() {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         if (s.done) (void)hipEventDestroy(s.done);
         if (s.stream) (void)hipStreamDestroy(s.stream);
-        if (s.hOut) (void)hipHostFree(s.hOut);
+        s.pipe.reset();
     }
     if (ring_) (void)hipHostFree(ring_);
 }
@@ -93,17 +98,16 @@ Server::Result Server::wait(uint64_t ticket) {
 
 void Server::reap(Slot& s) {
     const double doneMs = nowMs();
-    const uint32_t B = uint32_t(s.reqs.size()), K = opt_.K;
-    const uint32_t* hDocs = reinterpret_cast<const uint32_t*>(s.hOut);
-    const float* hScores = reinterpret_cast<const float*>(s.hOut + 4ull * B * K);
-    const uint32_t* hCounts = reinterpret_cast<const uint32_t*>(s.hOut + 8ull * B * K);
+    const uint32_t B = uint32_t(s.reqs.size());
+    const QueryPipeline& P = *s.pipe;
+    const uint8_t* res = P.hostResults();
     {
         std::lock_guard<std::mutex> lk(m_);
         for (uint32_t q = 0; q < B; q++) {
             Result r;
-            const uint32_t c = hCounts[q];
-            r.docs.assign(hDocs + uint64_t(q) * K, hDocs + uint64_t(q) * K + c);
-            r.scores.assign(hScores + uint64_t(q) * K, hScores + uint64_t(q) * K + c);
+            const uint32_t c = P.count(res, q);
+            r.docs.assign(P.docs(res, q), P.docs(res, q) + c);
+            r.scores.assign(P.scores(res, q), P.scores(res, q) + c);
             r.arriveMs = s.reqs[q].arriveMs;
             r.startMs = s.startMs;
             r.doneMs = doneMs;
@@ -125,25 +129,11 @@ void Server::reap(Slot& s) {
 }
 
 void Server::launch(Slot& s, std::vector<Request>&& batch) {
-    const uint32_t B = uint32_t(batch.size()), K = opt_.K;
+    const uint32_t B = uint32_t(batch.size());
     s.hOff.assign(B + 1, 0);
     for (uint32_t q = 0; q < B; q++) s.hOff[q + 1] = s.hOff[q] + batch[q].n;
-    const uint64_t D = s.hOff[B];
     hipStream_t st = s.stream;
-    // one stream-ordered block: descriptors, leaf ids, offsets, then the results
-    // laid out as the slot's pinned buffer (one D2H copy)
-    auto up = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
-    const uint64_t descB = up(std::max<uint64_t>(D, 1) * kDims), leafB = up(std::max<uint64_t>(D, 1) * 4);
-    const uint64_t offB = up(8ull * (B + 1)), outB = 8ull * B * K + 4ull * B;
-    uint8_t* block = nullptr;
-    RBOW_HIP(hipMallocAsync(reinterpret_cast<void**>(&block), descB + leafB + offB + outB, st));
-    uint8_t* dDesc = block;
-    uint32_t* dLeaf = reinterpret_cast<uint32_t*>(block + descB);
-    uint64_t* dOff = reinterpret_cast<uint64_t*>(block + descB + leafB);
-    uint8_t* dOut = block + descB + leafB + offB;
-    uint32_t* dDocs = reinterpret_cast<uint32_t*>(dOut);
-    float* dScores = reinterpret_cast<float*>(dOut + 4ull * B * K);
-    uint32_t* dCounts = reinterpret_cast<uint32_t*>(dOut + 8ull * B * K);
+    uint8_t* dDesc = s.pipe->descBuffer();
     // H2D straight from the staging ring: one copy per run of adjacent requests
     for (uint32_t q = 0; q < B;) {
         uint32_t e = q + 1;
@@ -156,12 +146,8 @@ void Server::launch(Slot& s, std::vector<Request>&& batch) {
                                     hipMemcpyHostToDevice, st));
         q = e;
     }
-    RBOW_HIP(hipMemcpyAsync(dOff, s.hOff.data(), 8ull * (B + 1), hipMemcpyHostToDevice, st));
-    const TreeView t = voc_.view();
-    if (D) launchLookup(t, dDesc, uint32_t(D), dLeaf, nullptr, opt_.lookupKernel, st);
-    db_.scoreAsync(dLeaf, dOff, s.hOff.data(), B, K, dDocs, dScores, dCounts, st);
-    RBOW_HIP(hipMemcpyAsync(s.hOut, dOut, outB, hipMemcpyDeviceToHost, st));
-    RBOW_HIP(hipFreeAsync(block, st));
+    // lookup, scoring, top-K and the results' download: one graph launch per batch
+    s.pipe->run(s.hOff.data(), B, opt_.lookupKernel, opt_.useGraphs, true, st);
     RBOW_HIP(hipEventRecord(s.done, st));
     s.reqs = std::move(batch);
     s.startMs = nowMs();
@@ -188,10 +174,13 @@ void Server::loop() {
                 std::unique_lock<std::mutex> lk(m_);
                 if (stop_ && pending_.empty() && !anyBusy) return;
                 if (free) {
-                    // admission: batch slots and the HBM scratch budget
+                    // admission: batch slots, the slot's descriptor capacity and the HBM scratch budget
+                    uint64_t desc = 0;
                     while (!pending_.empty() && batch.size() < opt_.maxBatch &&
+                           desc + pending_.front().n <= opt_.batchMaxDesc &&
                            inFlightBytes_ + pending_.front().bytes <= opt_.memBudget) {
                         inFlightBytes_ += pending_.front().bytes;
+                        desc += pending_.front().n;
                         batch.push_back(std::move(pending_.front()));
                         pending_.pop_front();
                         nPending_.fetch_sub(1, std::memory_order_relaxed);

@@ -5,9 +5,11 @@
 // Requests (descriptors on the host) are copied by the submitting thread into
 // a pinned staging ring, then admitted into a batch while batch slots and the
 // HBM scratch budget allow; a worker thread launches each batch -- H2D
-// straight from the ring, lookup, scoring, top-K, D2H -- on one of several HIP
-// streams, so batches formed from later arrivals run while earlier ones are
-// still on the device, and the worker itself copies no descriptors.  Every
+// straight from the ring, then lookup, scoring, top-K and D2H through the
+// slot's QueryPipeline (one captured HIP graph launch per batch, or direct
+// launches) -- on one of several HIP streams, so batches formed from later
+// arrivals run while earlier ones are still on the device, and the worker
+// itself copies no descriptors.  Every
 // request's result is independent of what it was batched with (per-query
 // accumulators), which the tests check.
 #pragma once
@@ -20,7 +22,9 @@
 #include <unordered_map>
 #include <vector>
 
-#include "database.hpp"
+#include <memory>
+
+#include "pipeline.hpp"
 
 namespace rbow_amd {
 
@@ -31,6 +35,8 @@ struct ServerOptions {
     uint32_t K = 50;
     int lookupKernel = RBOW_LOOKUP_AUTO;  // RBOW_LOOKUP_AUTO / _MFMA / _PLAIN, per batch
     uint64_t stagingBytes = 256ull << 20;  // pinned ring of submitted descriptors
+    bool useGraphs = true;         // each batch as one launch of a captured graph (per size class)
+    uint64_t batchMaxDesc = 0;     // descriptors per batch (0: min(maxBatch * 16384, 2^20), at least 16384)
 };
 
 class Server {
@@ -72,9 +78,7 @@ private:
         bool busy = false;
         std::vector<Request> reqs;
         std::vector<uint64_t> hOff;
-        // pinned results of one batch of B queries, one copy from the device:
-        // docs [B][K] u32, scores [B][K] f32, counts [B] u32
-        uint8_t* hOut = nullptr;
+        std::unique_ptr<QueryPipeline> pipe;  // device buffers, graphs, pinned results of this slot's batches
         uint64_t bytes = 0;
         double startMs = 0;
     };

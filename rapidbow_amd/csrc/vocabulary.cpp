@@ -65,6 +65,7 @@ TreeView Vocabulary::view() const {
     t.e1 = e1.get();
     t.e2 = e2.get();
     t.autoMin = lookupAutoMin;
+    t.fusedMax = lookupFusedMax;
     return t;
 }
 

@@ -151,7 +151,7 @@ def test_blob_roundtrip_u8():
 def test_lookup_autotuner_keeps_leaf_ids():
     desc = rb.synth_descriptors(60000, seed=5)
     voc, res, _ = _u8_voc_from_oracle(desc, 3)
-    assert voc.lookup_crossover == 131072  # built-in default for uint8 trees
+    assert voc.lookup_crossover == 12288  # built-in default for uint8 trees
     q = rb.synth_descriptors(70000, seed=6)
     expect = oracle.lookup_u8(res["nodes"], 3, q)
     d = rb.DeviceArray.from_numpy(q)
@@ -162,4 +162,32 @@ def test_lookup_autotuner_keeps_leaf_ids():
         voc.lookup_crossover = c
         np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_AUTO), expect)
     voc.lookup_crossover = 0
-    assert voc.lookup_crossover == 131072
+    assert voc.lookup_crossover == 12288
+
+
+def test_lookup_u8_fused_bucketed_plain_agree():
+    """The MFMA form's two descents on uint8 b16 trees -- the fused single launch
+    (small batches) and the level-synchronous bucketed one -- and the plain
+    kernel give the oracle's leaf ids, with planted duplicate children, on trees
+    of 1 to 4 levels and ragged batch sizes; 0 restores the default crossover."""
+    rng = np.random.default_rng(31)
+    for L in (1, 2, 3, 4):
+        n = level_beg(16, L)
+        rows = rng.integers(0, 256, (n, 16, 128), dtype=np.uint8)
+        rows[:, 9] = rows[:, 3]   # ties across lane groups
+        rows[:, 7] = rows[:, 6]   # and within one
+        nodes = oracle.rows_to_ref_nodes(rows)
+        voc = rb.deserialize_vocabulary(ref_blob(L, nodes, np.full(n, 0xFFFF), np.ones((n, 16), np.float32)))
+        assert voc.lookup_fused_max == 196608
+        for nq in (1, 16, 33, 1000, 40_001):
+            q = rng.integers(0, 256, (nq, 128), dtype=np.uint8)
+            q[: min(nq, 64)] = rows[rng.integers(0, n, min(nq, 64)), rng.integers(0, 16, min(nq, 64))]  # on a child
+            exp = oracle.lookup_u8(nodes, L, q)
+            voc.lookup_fused_max = 0
+            fused = voc.lookup_host(q, rb.LOOKUP_MFMA)
+            voc.lookup_fused_max = 1 if nq > 1 else 0  # every batch above one descriptor bucketed
+            bucketed = voc.lookup_host(q, rb.LOOKUP_MFMA)
+            voc.lookup_fused_max = 0
+            np.testing.assert_array_equal(fused, exp, err_msg=f"fused L={L} n={nq}")
+            np.testing.assert_array_equal(bucketed, exp, err_msg=f"bucketed L={L} n={nq}")
+            np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_PLAIN), exp)
