@@ -188,10 +188,10 @@ bool launchKmUpdate(const KmLevel& k, hipStream_t s, const KernelTimer& t = {});
 
 // Stable regroup of a level's rows by (task, child) (Vocabulary.cpp:83-98):
 // orderOut = orderIn stably sorted by task * b + assignment; hist[task * b + j]
-// = the occurrence counts.  Rows of the level are tasks-contiguous (k.tiles).
-// ws: device workspace of regroupWorkspaceBytes(>= nbRows, >= k.nbTasks * k.b)
-// bytes, 256-byte aligned; no allocation.
-size_t regroupWorkspaceBytes(uint32_t nbRows, uint64_t nbKeysMax);
+// = the occurrence counts.  Rows of the level are tasks-contiguous (k.tiles);
+// orderOut == nullptr computes the histogram only.  ws: device workspace of
+// regroupWorkspaceBytes(>= k.nbTiles, k.b) bytes, 256-byte aligned; no allocation.
+size_t regroupWorkspaceBytes(uint64_t nbTilesMax, uint32_t b);
 void launchRegroup(const KmLevel& k, uint32_t nbRows, const uint32_t* orderIn, uint32_t* orderOut, uint32_t* hist,
                    void* ws, size_t wsBytes, hipStream_t s);
 

@@ -12,7 +12,6 @@
 //           that pass) or after max_iters passes.
 // Sums are exact: int32 per wave in LDS, flushed into int64 per task.
 #include <hip/hip_ext.h>
-#include <hipcub/hipcub.hpp>
 #include <type_traits>
 
 #include "bf16filter.hpp"
@@ -96,99 +95,149 @@ __global__ void k_initCenters(uint32_t b, uint32_t nbTasks, const uint8_t* __res
     }
 }
 
-__device__ inline uint64_t blockSumU64(uint64_t v, uint64_t* red) {
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += (uint64_t)__shfl_xor((unsigned long long)v, d, 64);
-    __syncthreads();
-    if (lane == 0) red[wave] = v;
-    __syncthreads();
-    uint64_t s = 0;
-    for (uint32_t w = 0; w < blockDim.x / 64; w++) s += red[w];
-    return s;
-}
-
 // Builder-defined k-means++ (same definition as the oracle's):
 // candidates = one hashed member per stratum of the group (<= 256), then
-// D^2-weighted picks with integer weights and hashed draws.
+// D^2-weighted picks with integer weights and hashed draws.  One wave per
+// task: lane l holds candidates 4l .. 4l+3 (rows in registers, D^2 and norms);
+// a chosen center's row is broadcast with readlane, the weighted draw is a wave
+// prefix sum.  No LDS and no barrier, so the many small tasks of a deep level
+// run side by side.
+// Inclusive prefix sum over the 64 lanes of a wave: DPP row shifts within each
+// row of 16 lanes (lanes with no source add 0), then the earlier rows' totals.
+__device__ inline uint32_t waveInclusiveSum(uint32_t x) {
+    x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x111, 0xF, 0xF, false));  // row_shr:1
+    x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x112, 0xF, 0xF, false));  // row_shr:2
+    x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x114, 0xF, 0xF, false));  // row_shr:4
+    x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x118, 0xF, 0xF, false));  // row_shr:8
+    const uint32_t r0 = __builtin_amdgcn_readlane(x, 15), r1 = __builtin_amdgcn_readlane(x, 31),
+                   r2 = __builtin_amdgcn_readlane(x, 47);
+    const uint32_t row = (threadIdx.x & 63) >> 4;
+    return x + (row == 0 ? 0u : row == 1 ? r0 : row == 2 ? r0 + r1 : r0 + r1 + r2);
+}
+
+constexpr uint32_t kPPSlots = 4;  // candidates per lane (64 * 4 = 256)
+
+template <uint32_t S>
+__device__ inline void ppBroadcast(const uint32_t (&x)[kPPSlots][32], uint32_t owner, uint32_t (&c)[32]) {
+#pragma unroll
+    for (int w = 0; w < 32; w++) c[w] = __builtin_amdgcn_readlane(x[S][w], owner);
+}
+
 __global__ __launch_bounds__(256) void k_kmeansPP(const uint8_t* __restrict__ desc, const uint32_t* __restrict__ order,
                                                  const uint32_t* __restrict__ taskBeg,
-                                                 const uint32_t* __restrict__ taskCnt, uint32_t b, uint64_t seed,
-                                                 uint32_t level, uint32_t* __restrict__ seedIdx,
+                                                 const uint32_t* __restrict__ taskCnt, uint32_t nbTasks, uint32_t b,
+                                                 uint64_t seed, uint32_t level, uint32_t* __restrict__ seedIdx,
                                                  uint32_t* __restrict__ masks) {
-    __shared__ uint32_t sx[256][33];
-    __shared__ uint32_t sD[256], sNorm[256], sCand[256];
-    __shared__ uint64_t red[8], scanW[4];
-    __shared__ uint32_t sChosen[32], sPick;
-    const uint32_t g = blockIdx.x, tid = threadIdx.x;
+    const uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + (threadIdx.x >> 6));
+    const uint32_t lane = threadIdx.x & 63;
+    if (g >= nbTasks) return;
     const uint32_t n = taskCnt[g];
     if (n == 0) {
-        if (tid < b) seedIdx[g * b + tid] = kIdxInvalid;
-        if (tid == 0) masks[g] = 0;
+        if (lane < b) seedIdx[g * b + lane] = kIdxInvalid;
+        if (lane == 0) masks[g] = 0;
         return;
     }
     const uint32_t m = n < 256 ? n : 256;
-    if (tid < m) {
-        uint32_t pos = tid;
-        if (n > 256) {
-            const uint64_t lo = uint64_t(tid) * n / m, hi = uint64_t(tid + 1) * n / m;
-            pos = uint32_t(lo + hash4(seed, level, g, 0x100000000ull + tid) % (hi - lo));
+    uint32_t x[kPPSlots][32], nrm[kPPSlots], dd[kPPSlots], cand[kPPSlots];
+#pragma unroll
+    for (uint32_t i = 0; i < kPPSlots; i++) {
+        const uint32_t c = lane * kPPSlots + i;
+        cand[i] = kIdxInvalid;
+        nrm[i] = 0;
+        dd[i] = 0;  // candidates >= m weigh nothing
+        if (c < m) {
+            uint32_t pos = c;
+            if (n > 256) {
+                const uint64_t lo = uint64_t(c) * n / m, hi = uint64_t(c + 1) * n / m;
+                pos = uint32_t(lo + hash4(seed, level, g, 0x100000000ull + c) % (hi - lo));
+            }
+            cand[i] = order[taskBeg[g] + pos];
+            dd[i] = 0xFFFFFFFFu;
         }
-        const uint32_t di = order[taskBeg[g] + pos];
-        sCand[tid] = di;
-        uint32_t nrm = 0;
-        for (uint32_t w = 0; w < 32; w++) {
-            const uint32_t v = reinterpret_cast<const uint32_t*>(desc + uint64_t(di) * kDims)[w];
-            sx[tid][w] = v;
-            nrm = __builtin_amdgcn_udot4(v, v, nrm, false);
-        }
-        sNorm[tid] = nrm;
-        sD[tid] = 0xFFFFFFFFu;
     }
-    if (tid == 0) sChosen[0] = uint32_t(hash4(seed, level, g, 0x200000000ull) % m);
-    __syncthreads();
-    auto relax = [&](uint32_t c) {
-        if (tid < m) {
-            uint32_t dot = 0;
-            for (uint32_t w = 0; w < 32; w++) dot = __builtin_amdgcn_udot4(sx[tid][w], sx[c][w], dot, false);
-            const uint32_t d = sNorm[tid] + sNorm[c] - 2 * dot;
-            sD[tid] = min(sD[tid], d);
+#pragma unroll
+    for (uint32_t i = 0; i < kPPSlots; i++) {
+        const v4i* row = reinterpret_cast<const v4i*>(desc + uint64_t(cand[i] == kIdxInvalid ? 0u : cand[i]) * kDims);
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const v4i v = lane * kPPSlots + i < m ? row[q] : v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int e = 0; e < 4; e++) x[i][4 * q + e] = uint32_t(v[e]);
         }
-        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < 32; w++) nrm[i] = __builtin_amdgcn_udot4(x[i][w], x[i][w], nrm[i], false);
+    }
+    // D^2 of every candidate to its nearest chosen one, after choosing candidate c
+    auto relax = [&](uint32_t c) {
+        c = __builtin_amdgcn_readfirstlane(c);  // the same in every lane
+        const uint32_t owner = c / kPPSlots, slot = c % kPPSlots;
+        uint32_t cw[32];
+        switch (slot) {
+            case 0: ppBroadcast<0>(x, owner, cw); break;
+            case 1: ppBroadcast<1>(x, owner, cw); break;
+            case 2: ppBroadcast<2>(x, owner, cw); break;
+            default: ppBroadcast<3>(x, owner, cw); break;
+        }
+        const uint32_t nSel = slot == 0 ? nrm[0] : slot == 1 ? nrm[1] : slot == 2 ? nrm[2] : nrm[3];
+        const uint32_t nc = __builtin_amdgcn_readlane(nSel, owner);
+#pragma unroll
+        for (uint32_t i = 0; i < kPPSlots; i++) {
+            uint32_t dot = 0;
+#pragma unroll
+            for (int w = 0; w < 32; w++) dot = __builtin_amdgcn_udot4(x[i][w], cw[w], dot, false);
+            if (lane * kPPSlots + i < m) dd[i] = min(dd[i], nrm[i] + nc - 2 * dot);
+        }
     };
-    relax(sChosen[0]);
+    uint32_t prev = uint32_t(hash4(seed, level, g, 0x200000000ull) % m);
+    uint32_t myChosen = lane == 0 ? prev : 0u;  // lane r < b: the r-th chosen candidate
+    // lane r: the draw of round r, hashed up front
+    const uint64_t myDraw = lane < b ? hash4(seed, level, g, 0x300000000ull + lane) : 0ull;
+    relax(prev);
     uint32_t nValid = 1;
     for (uint32_t r = 1; r < b; r++) {
-        const uint64_t dv = tid < m ? sD[tid] : 0;
-        const uint64_t total = blockSumU64(dv, red);
-        if (total == 0) {
-            if (tid == 0) sChosen[r] = sChosen[r - 1];
-            __syncthreads();
-            continue;
-        }
-        const uint64_t t = hash4(seed, level, g, 0x300000000ull + r) % total;
-        // inclusive prefix of D over candidates
-        const uint32_t lane = tid & 63, wave = tid >> 6;
-        uint64_t inc = dv;
+        // inclusive prefix of D^2 in candidate order: lane-local, then across lanes.  Exact in
+        // 32 bits: 256 candidates * 128 * 255^2 < 2^31.
+        uint32_t inc[kPPSlots];
+        uint32_t run = 0;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint64_t o = (uint64_t)__shfl_up((unsigned long long)inc, d, 64);
-            if (lane >= uint32_t(d)) inc += o;
+        for (uint32_t i = 0; i < kPPSlots; i++) inc[i] = run += dd[i];
+        const uint32_t lanePre = waveInclusiveSum(run);
+        const uint32_t total = __builtin_amdgcn_readlane(lanePre, 63);
+        uint32_t pick = prev;
+        if (total != 0) {
+            // (readlane returns int: each half goes through uint32_t, never sign-extended)
+            const uint64_t draw = (uint64_t(uint32_t(__builtin_amdgcn_readlane(uint32_t(myDraw >> 32), r))) << 32) |
+                                  uint32_t(__builtin_amdgcn_readlane(uint32_t(myDraw), r));
+            const uint32_t t = uint32_t(draw % total);
+            const uint32_t excl = lanePre - run;
+            uint32_t mine = 0;
+            bool hit = false;
+#pragma unroll
+            for (uint32_t i = 0; i < kPPSlots; i++) {
+                const uint32_t hiEnd = excl + inc[i];
+                if (hiEnd > t && hiEnd - dd[i] <= t) {
+                    hit = true;
+                    mine = lane * kPPSlots + i;
+                }
+            }
+            const uint64_t who = __ballot(hit);
+            pick = __builtin_amdgcn_readlane(mine, int(__builtin_ctzll(who)));
+            nValid++;
         }
-        if (lane == 63) scanW[wave] = inc;
-        __syncthreads();
-        uint64_t off = 0;
-        for (uint32_t w = 0; w < wave; w++) off += scanW[w];
-        inc += off;
-        if (tid < m && inc > t && inc - dv <= t) sPick = tid;
-        __syncthreads();
-        if (tid == 0) sChosen[r] = sPick;
-        nValid++;
-        __syncthreads();
-        relax(sChosen[r]);
+        if (lane == r) myChosen = pick;
+        if (total != 0) relax(pick);
+        prev = pick;
     }
-    if (tid < b) seedIdx[g * b + tid] = sCand[sChosen[tid]];
-    if (tid == 0) masks[g] = nValid >= 32 ? 0xFFFFFFFFu : ((1u << nValid) - 1u);
+    // seeds: the chosen candidates' descriptor ids
+    const uint32_t owner = myChosen / kPPSlots, slot = myChosen % kPPSlots;
+    uint32_t id = kIdxInvalid;
+#pragma unroll
+    for (uint32_t i = 0; i < kPPSlots; i++) {
+        const uint32_t v = __shfl(cand[i], int(owner), 64);
+        if (slot == i) id = v;
+    }
+    if (lane < b) seedIdx[g * b + lane] = id;
+    if (lane == 0) masks[g] = nValid >= 32 ? 0xFFFFFFFFu : ((1u << nValid) - 1u);
 }
 
 // ------------------------------------------------------- accumulators ----
@@ -1035,38 +1084,172 @@ __global__ __launch_bounds__(256) void k_updateU8Part(KmLevel k) {
     }
 }
 
-// Regroup keys (Vocabulary.cpp:83-98): key of row r = task * b + assignment, and
-// the per-(task, child) histogram (the occurrence counts).  Each wave walks a
-// contiguous range of tiles (tasks are contiguous), keeps the current task's
-// child counts in lanes 0..b-1 and adds them to the histogram when the task
-// changes, so a level costs about one atomic per (wave, child) rather than one
-// per row on a handful of addresses.
-__global__ __launch_bounds__(256) void k_regroupKeys(KmLevel k, uint32_t* __restrict__ keys, uint32_t* __restrict__ hist) {
-    const uint32_t lane = threadIdx.x & 63;
-    uint32_t beg, end;
-    waveRange(k.nbTiles, beg, end);
-    uint32_t task = kIdxInvalid, mine = 0;
-    auto flush = [&]() {
-        if (task != kIdxInvalid && lane < k.b && mine) atomicAdd(&hist[task * k.b + lane], mine);
-        mine = 0;
-    };
-    for (uint32_t ti = beg; ti < end; ti++) {
-        const KmTile t = k.tiles[ti];
-        if (t.task != task) {
-            flush();
-            task = t.task;
-        }
-        // every row holds an assignment < b after a level's first pass; a row that
-        // does not is left out of the histogram (the host checks its total)
-        const bool valid = lane < t.cnt;
-        const uint32_t a = valid ? k.assign[t.row + lane] : 0xFFu;
-        if (valid) keys[t.row + lane] = a < k.b ? t.task * k.b + a : ~0u;
-        for (uint32_t j = 0; j < k.b; j++) {
-            const uint32_t c = uint32_t(__builtin_popcountll(__ballot(a == j)));
-            if (lane == j) mine += c;
+// ------------------------------------------------------------ regroup ----
+// Stable regroup of a level's rows by (task, child) (Vocabulary.cpp:83-98 sorts
+// the same keys stably) as a counting sort over the level's tiles.  A
+// workgroup owns kRgTiles consecutive tiles (tiles are in row order and tasks
+// are contiguous, so a task's tiles form one run):
+//   count    per tile and child the rows (ballots), scanned tile by tile inside
+//            the workgroup and restarted at every task boundary; the tile's
+//            offsets are stored and each task run's totals added to the
+//            histogram (one atomic per run and child);
+//   carry    the workgroups' trailing-run totals chained across workgroups;
+//   scatter  row -> taskBeg[task] + rows of the task's lower children + rows of
+//            its child in earlier tiles + rows of its child earlier in its tile.
+constexpr uint32_t kRgTiles = 256;  // tiles per workgroup (one per thread of the scan)
+
+struct RgWs {
+    uint32_t* tileOff;   // [tile][b] rows of the child in earlier tiles of the task within the workgroup
+    uint32_t* blkTot;    // [workgroup][b] rows of the workgroup's last task in the workgroup
+    uint32_t* blkCarry;  // [workgroup][b] rows of the workgroup's first task in earlier workgroups
+    uint32_t* blkHead;   // [workgroup] task of the first tile
+    uint32_t* blkTail;   // [workgroup] task of the last tile
+};
+
+template <uint32_t B>
+__global__ __launch_bounds__(256) void k_regroupCount(KmLevel k, RgWs ws, uint32_t* __restrict__ hist) {
+    constexpr uint32_t kGroups = kRgTiles / B;  // scan groups of B tiles, one thread per (group, child)
+    __shared__ uint32_t sCnt[kRgTiles][B + 1], sOff[kRgTiles][B + 1];
+    __shared__ uint32_t sTask[kRgTiles + 1];
+    __shared__ uint32_t sGrpTot[kGroups][B], sGrpCarry[kGroups][B];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t blk = blockIdx.x, t0 = blk * kRgTiles;
+    const uint32_t nT = min(kRgTiles, k.nbTiles - t0);
+    for (uint32_t j = 0; j < B; j++) sCnt[tid][j] = 0;
+    // lane l: tile wave * 64 + l; then every row of the wave's tiles loaded before counting
+    const uint32_t myTl = wave * 64 + lane;
+    const KmTile mine = myTl < nT ? k.tiles[t0 + myTl] : KmTile{kIdxInvalid, 0u, 0u};
+    sTask[myTl] = mine.task;
+    uint32_t a[64];
+#pragma unroll
+    for (int i = 0; i < 64; i++) {
+        const uint32_t row = __builtin_amdgcn_readlane(mine.row, i), cnt = __builtin_amdgcn_readlane(mine.cnt, i);
+        a[i] = lane < cnt ? k.assign[row + lane] : 0xFFu;
+    }
+    __syncthreads();  // sCnt cleared
+    // a row whose assignment is not < b counts nowhere (the caller checks the total)
+#pragma unroll
+    for (int i = 0; i < 64; i++)
+        if (a[i] < B) atomicAdd(&sCnt[wave * 64 + i][a[i]], 1u);
+    if (tid == 0) sTask[kRgTiles] = kIdxInvalid;
+    __syncthreads();
+    const uint32_t j = tid % B, grp = tid / B, g0 = grp * B;
+    uint32_t run = 0;
+    for (uint32_t i = 0; i < B; i++) {
+        const uint32_t tl = g0 + i;
+        if (i > 0 && sTask[tl] != sTask[tl - 1]) run = 0;
+        sOff[tl][j] = run;
+        run += sCnt[tl][j];
+    }
+    sGrpTot[grp][j] = run;
+    __syncthreads();
+    if (tid < B) {  // thread j: carries between the groups
+        uint32_t carry = 0;
+        sGrpCarry[0][j] = 0;
+        for (uint32_t q = 1; q < kGroups; q++) {
+            const uint32_t head = sTask[q * B], prevTail = sTask[q * B - 1], prevHead = sTask[(q - 1) * B];
+            carry = prevTail == head ? (prevHead == prevTail ? carry : 0u) + sGrpTot[q - 1][j] : 0u;
+            sGrpCarry[q][j] = carry;
         }
     }
-    flush();
+    __syncthreads();
+    const uint32_t carry = sGrpCarry[grp][j], head = sTask[g0];
+    for (uint32_t i = 0; i < B; i++) {
+        const uint32_t tl = g0 + i, task = sTask[tl];
+        if (task == kIdxInvalid) break;
+        const uint32_t off = sOff[tl][j] + (task == head ? carry : 0u);  // the group's leading run gets the carry
+        ws.tileOff[uint64_t(t0 + tl) * B + j] = off;
+        if (sTask[tl + 1] != task) {  // the task's run in this workgroup ends here
+            const uint32_t tot = off + sCnt[tl][j];
+            if (tot) atomicAdd(&hist[uint64_t(task) * B + j], tot);
+            if (tl + 1 == nT) ws.blkTot[uint64_t(blk) * B + j] = tot;
+        }
+    }
+    if (tid == 0) {
+        ws.blkHead[blk] = sTask[0];
+        ws.blkTail[blk] = sTask[nT - 1];
+    }
+}
+
+// One workgroup: blkCarry[w][j] = rows of child j of workgroup w's first task
+// in workgroups before w (a task's run may span many workgroups).
+template <uint32_t B>
+__global__ __launch_bounds__(256) void k_regroupCarry(RgWs ws, uint32_t nbBlk) {
+    __shared__ uint32_t sHead[256], sTail[256], sTot[256][B];
+    const uint32_t tid = threadIdx.x;
+    uint32_t carry = 0, prevHead = kIdxInvalid, prevTail = kIdxInvalid, prevTot = 0;
+    for (uint32_t c0 = 0; c0 < nbBlk; c0 += 256) {
+        const uint32_t n = min(256u, nbBlk - c0);
+        __syncthreads();
+        for (uint32_t e = tid; e < n; e += 256) {
+            sHead[e] = ws.blkHead[c0 + e];
+            sTail[e] = ws.blkTail[c0 + e];
+        }
+        for (uint32_t e = tid; e < n * B; e += 256) sTot[e / B][e % B] = ws.blkTot[uint64_t(c0) * B + e];
+        __syncthreads();
+        if (tid < B) {
+            for (uint32_t i = 0; i < n; i++) {
+                const uint32_t h = sHead[i];
+                carry = prevTail == h ? (prevHead == prevTail ? carry : 0u) + prevTot : 0u;
+                ws.blkCarry[uint64_t(c0 + i) * B + tid] = carry;
+                prevHead = h;
+                prevTail = sTail[i];
+                prevTot = sTot[i][tid];
+            }
+        }
+    }
+}
+
+constexpr uint32_t kScTiles = 4;  // tiles per wave of the scatter (kRgTiles is a multiple)
+
+template <uint32_t B>
+__global__ __launch_bounds__(256) void k_regroupScatter(KmLevel k, RgWs ws, const uint32_t* __restrict__ hist,
+                                                       uint32_t nbRows, const uint32_t* __restrict__ orderIn,
+                                                       uint32_t* __restrict__ orderOut) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t tBeg = __builtin_amdgcn_readfirstlane((blockIdx.x * kWaves + (threadIdx.x >> 6)) * kScTiles);
+    if (tBeg >= k.nbTiles) return;
+    const uint32_t blk = tBeg / kRgTiles;  // the wave's tiles share a workgroup of the count pass
+    const uint32_t head = ws.blkHead[blk];
+    KmTile t[kScTiles];
+#pragma unroll
+    for (uint32_t i = 0; i < kScTiles; i++)
+        t[i] = tBeg + i < k.nbTiles ? k.tiles[tBeg + i] : KmTile{kIdxInvalid, 0u, 0u};
+    uint32_t a[kScTiles], src[kScTiles], h[kScTiles], tb[kScTiles], cr[kScTiles], to[kScTiles];
+#pragma unroll
+    for (uint32_t i = 0; i < kScTiles; i++) {
+        const bool valid = lane < t[i].cnt, any = t[i].cnt != 0;
+        a[i] = valid ? k.assign[t[i].row + lane] : 0xFFu;
+        src[i] = valid ? orderIn[t[i].row + lane] : 0u;
+        h[i] = any && lane < B ? hist[uint64_t(t[i].task) * B + lane] : 0u;
+        tb[i] = any ? k.taskBeg[t[i].task] : 0u;
+        cr[i] = any && t[i].task == head && lane < B ? ws.blkCarry[uint64_t(blk) * B + lane] : 0u;
+        to[i] = a[i] < B ? ws.tileOff[uint64_t(tBeg + i) * B + a[i]] : 0u;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kScTiles; i++) {
+        // lane c < B: first destination row of child c of the tile's task in this workgroup
+        uint32_t inc = h[i];
+#pragma unroll
+        for (int d = 1; d < int(B); d <<= 1) {
+            const uint32_t o = __shfl_up(inc, d, 64);
+            if (lane >= uint32_t(d)) inc += o;
+        }
+        const uint32_t base = tb[i] + inc - h[i] + cr[i];
+        uint64_t same = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < B; j++) {
+            const uint64_t m = __ballot(a[i] == j);
+            if (a[i] == j) same = m;
+        }
+        const uint32_t rank =
+            __builtin_amdgcn_mbcnt_hi(uint32_t(same >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(same), 0u));
+        const uint32_t dst = __shfl(base, int(a[i] < B ? a[i] : 0u), 64);
+        if (a[i] < B) {
+            const uint32_t pos = dst + to[i] + rank;
+            if (pos < nbRows) orderOut[pos] = src[i];
+        }
+    }
 }
 
 // Vocabulary.cpp:103-108 IDF weight of one child from its occurrence count:
@@ -1151,7 +1334,8 @@ void launchKmeansPP(const uint8_t* desc, const uint32_t* order, const uint32_t* 
                     uint32_t nbTasks, uint32_t b, uint64_t seed, uint32_t level, uint32_t* seedIdx, uint32_t* masks,
                     hipStream_t s) {
     if (!nbTasks) return;
-    k_kmeansPP<<<nbTasks, 256, 0, s>>>(desc, order, taskBeg, taskCnt, b, seed, level, seedIdx, masks);
+    k_kmeansPP<<<divUp(nbTasks, kWaves), 256, 0, s>>>(desc, order, taskBeg, taskCnt, nbTasks, b, seed, level, seedIdx,
+                                                       masks);
     RBOW_HIP(hipGetLastError());
 }
 
@@ -1260,41 +1444,49 @@ bool launchKmAssign(const KmLevel& k, int kernel, hipStream_t s, const KernelTim
 }
 
 namespace {
-int keyBits(uint64_t nbKeys) {
-    int bits = 1;
-    while ((1ull << bits) < nbKeys) bits++;
-    return bits;
-}
-size_t sortTempBytes(uint32_t nbRows, int bits) {
-    size_t bytes = 0;
-    RBOW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                (uint32_t*)nullptr, (uint32_t*)nullptr, int(nbRows), 0, bits));
-    return bytes;
+RgWs regroupWs(void* ws, uint64_t nbTiles, uint32_t b) {
+    const uint64_t nbBlk = divUp(nbTiles, uint64_t(kRgTiles));
+    auto at = [&](uint64_t off) { return reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws) + off); };
+    auto up256 = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
+    RgWs w;
+    uint64_t o = 0;
+    w.tileOff = at(o);
+    o = up256(o + 4 * nbTiles * b);
+    w.blkTot = at(o);
+    o = up256(o + 4 * nbBlk * b);
+    w.blkCarry = at(o);
+    o = up256(o + 4 * nbBlk * b);
+    w.blkHead = at(o);
+    o = up256(o + 4 * nbBlk);
+    w.blkTail = at(o);
+    return w;
 }
 }  // namespace
 
-size_t regroupWorkspaceBytes(uint32_t nbRows, uint64_t nbKeysMax) {
-    size_t temp = 0;
-    for (int bits = 1; bits <= keyBits(nbKeysMax); bits++) temp = std::max(temp, sortTempBytes(nbRows, bits));
-    return 8ull * nbRows + temp + 256;
+size_t regroupWorkspaceBytes(uint64_t nbTilesMax, uint32_t b) {
+    const uint64_t nbBlk = divUp(nbTilesMax, uint64_t(kRgTiles));
+    return 4 * nbTilesMax * b + 8 * nbBlk * b + 8 * nbBlk + 5 * 256;
 }
 
 void launchRegroup(const KmLevel& k, uint32_t nbRows, const uint32_t* orderIn, uint32_t* orderOut, uint32_t* hist,
                    void* ws, size_t wsBytes, hipStream_t s) {
-    const uint64_t nbKeys = uint64_t(k.nbTasks) * k.b;
-    launchFillU32(hist, 0u, nbKeys, s);
-    if (!nbRows) return;
-    const int bits = keyBits(nbKeys);
-    size_t tempBytes = sortTempBytes(nbRows, bits);
-    uint32_t* keys = static_cast<uint32_t*>(ws);
-    uint32_t* keysSorted = keys + nbRows;
-    void* temp = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(keysSorted + nbRows) + 255) & ~uintptr_t(255));
-    RBOW_REQUIRE(static_cast<uint8_t*>(temp) + tempBytes <= static_cast<uint8_t*>(ws) + wsBytes, RBOW_ERR_HIP,
-                 "regroup workspace too small");
-    k_regroupKeys<<<gridFor(k.nbTiles, kWaves * 8, 1024), 256, 0, s>>>(k, keys, hist);
-    // LSD radix sort is stable: rows keep their order inside each (task, child)
-    RBOW_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, keys, keysSorted, orderIn, orderOut, int(nbRows), 0,
-                                                bits, s));
+    RBOW_REQUIRE(k.b == 16 || k.b == 32, RBOW_ERR_INVALID_ARG, "branch factor must be 16 or 32");
+    launchFillU32(hist, 0u, uint64_t(k.nbTasks) * k.b, s);
+    if (!nbRows || !k.nbTiles) return;
+    RBOW_REQUIRE(regroupWorkspaceBytes(k.nbTiles, k.b) <= wsBytes, RBOW_ERR_HIP, "regroup workspace too small");
+    const RgWs w = regroupWs(ws, k.nbTiles, k.b);
+    const uint32_t nbBlk = uint32_t(divUp(uint64_t(k.nbTiles), uint64_t(kRgTiles)));
+    if (k.b == 16) k_regroupCount<16><<<nbBlk, 256, 0, s>>>(k, w, hist);
+    else k_regroupCount<32><<<nbBlk, 256, 0, s>>>(k, w, hist);
+    RBOW_HIP(hipGetLastError());
+    if (!orderOut) return;  // the histogram only
+    if (k.b == 16) {
+        k_regroupCarry<16><<<1, 256, 0, s>>>(w, nbBlk);
+        k_regroupScatter<16><<<divUp(k.nbTiles, kWaves * kScTiles), 256, 0, s>>>(k, w, hist, nbRows, orderIn, orderOut);
+    } else {
+        k_regroupCarry<32><<<1, 256, 0, s>>>(w, nbBlk);
+        k_regroupScatter<32><<<divUp(k.nbTiles, kWaves * kScTiles), 256, 0, s>>>(k, w, hist, nbRows, orderIn, orderOut);
+    }
     RBOW_HIP(hipGetLastError());
 }
 

@@ -220,7 +220,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     DevBuf<int8_t> dXA(mfmaPath ? tilesMax * 8192 : 0);
     DevBuf<int32_t> dPartSums(chunksMax * 16 * kDims);
     DevBuf<uint32_t> dPartCnt(chunksMax * 16), dPartChanged(chunksMax);
-    DevBuf<uint8_t> dSortWs(regroupWorkspaceBytes(nbDesc, nGmax * b));
+    DevBuf<uint8_t> dSortWs(regroupWorkspaceBytes(tilesMax, b));
     // float centroids with kernel 0: the BF16 MFMA filter's per-task operands, redone every pass
     const bool f32Mfma = ct == 1 && kernel == 0;
     DevBuf<uint8_t> dFMu(f32Mfma ? nGmax * kDims : 0);
@@ -428,8 +428,11 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         }
         // ---- stable regroup + occurrence on the device (Vocabulary.cpp:83-98), and the
         // children's IDF weights from the occurrence counts (Vocabulary.cpp:103-108) ----
-        launchRegroup(k, nbDesc, dOrder.get(), dOrderNext.get(), dHist, dSortWs.get(), dSortWs.size(), s);
-        std::swap(dOrder, dOrderNext);
+        // (the last level needs the occurrence counts only)
+        const bool last = level + 1 == L;
+        launchRegroup(k, nbDesc, dOrder.get(), last ? nullptr : dOrderNext.get(), dHist, dSortWs.get(), dSortWs.size(),
+                      s);
+        if (!last) std::swap(dOrder, dOrderNext);
         float* dW = reinterpret_cast<float*>(dMasks + nG);
         launchIdfWeights(dHist, uint64_t(nG) * b, nbDoc, dW, s);
         const uint64_t outWords = 2 * uint64_t(nG) * b + 2 * nG;
