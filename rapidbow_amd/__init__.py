@@ -67,6 +67,7 @@ _sig("rbow_build_vocabulary", _i32, [DescriptorAttributes, _u32, _vp, _u32, _u32
 _sig("rbow_build_vocabulary_ex", _i32,
      [DescriptorAttributes, _u32, _vp, _u32, _u32, _u32, ctypes.POINTER(BuildOptions), _vp, ctypes.POINTER(_vp)])
 _sig("rbow_voc_destroy", None, [_vp])
+_sig("rbow_release_build_workspaces", _i32, [])
 _sig("rbow_voc_get_attributes", _i32, [_vp, ctypes.POINTER(DescriptorAttributes)])
 _sig("rbow_voc_branch_factor", _u32, [_vp])
 _sig("rbow_voc_nb_levels", _u32, [_vp])
@@ -268,6 +269,11 @@ def build_vocabulary(dev_desc, nb_desc, nb_doc, nb_levels, branch_factor=16, str
     check(lib.rbow_build_vocabulary_ex(attr, branch_factor, _ptr(dev_desc), nb_desc, nb_doc, nb_levels,
                                        ctypes.byref(o), _stream(stream), ctypes.byref(h)))
     return Vocabulary(h.value)
+
+
+def release_build_workspaces():
+    """Frees the work buffers that completed builds keep per device for the next build."""
+    check(lib.rbow_release_build_workspaces())
 
 
 def deserialize_vocabulary(blob):

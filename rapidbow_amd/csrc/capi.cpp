@@ -157,7 +157,7 @@ struct KmScratch {
             RBOW_HIP(hipMemcpyAsync(tpBeg.get(), cb.data(), 4ull * k.nbTasks, hipMemcpyHostToDevice, s));
             RBOW_HIP(hipMemcpyAsync(tpCnt.get(), cc.data(), 4ull * k.nbTasks, hipMemcpyHostToDevice, s));
         }
-        launchGatherTiles(points, order.get(), tiles.get(), uint32_t(ht.size()), XA.get(), s);
+        launchGatherTiles(points, uint32_t(rows), order.get(), tiles.get(), uint32_t(ht.size()), XA.get(), s);
         k.XA = XA.get();
         k.chunks = chunks.get();
         k.nbChunks = uint32_t(hc.size());
@@ -254,6 +254,10 @@ rbow_status rbow_voc_lookup_noalloc(const rbow_vocabulary* voc, const void* dev_
         launchLookupNoAlloc(voc->v->view(), static_cast<const uint8_t*>(dev_desc), nb_desc, dev_leaf, dev_near_ties,
                             kernel, hs(stream));
     });
+}
+
+rbow_status rbow_release_build_workspaces(void) {
+    return guarded([&] { releaseBuildWorkspaces(); });
 }
 
 rbow_status rbow_voc_tune_lookup(rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc,

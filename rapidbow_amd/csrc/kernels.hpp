@@ -78,7 +78,9 @@ uint32_t tuneLookupCrossover(const TreeView& t, const uint8_t* desc, uint32_t n,
 
 // ----------------------------------------------------------------- build --
 // Gather rows: out[p] = desc[idx[p]] (idx == kIdxInvalid -> 0xFF bytes).
-void launchGatherRows(const uint8_t* desc, const uint32_t* idx, uint64_t n, uint8_t* out, hipStream_t s);
+// (ids >= nbSrc, like kIdxInvalid, give 0xFF bytes)
+void launchGatherRows(const uint8_t* desc, const uint32_t* idx, uint64_t n, uint8_t* out, hipStream_t s,
+                      uint64_t nbSrc = ~0ull);
 // reformatDesc.cu:38-74 tile layout: tiles[t][word][slot] (tileSize 32 or 16).
 void launchReformatTiles(const uint8_t* desc, const uint32_t* idx, uint32_t nbTiles, uint32_t tileSize,
                          uint32_t* tiles, hipStream_t s);
@@ -147,13 +149,24 @@ struct KmLevel {       // device state of one level's batched k-means
 // centers[task][j] = desc[seedIdx[task*b+j]] (invalid -> 0xFF / 255.0f)
 void launchInitCenters(const KmLevel& k, const uint8_t* desc, const uint32_t* seedIdx, hipStream_t s);
 // Builder-defined k-means++ picks: seedIdx[task*b+j], masks[task].
-void launchKmeansPP(const uint8_t* desc, const uint32_t* order, const uint32_t* taskBeg, const uint32_t* taskCnt,
-                    uint32_t nbTasks, uint32_t b, uint64_t seed, uint32_t level, uint32_t* seedIdx, uint32_t* masks,
-                    hipStream_t s);
+// desc holds nbSrc rows (order entries are row ids below it).
+void launchKmeansPP(const uint8_t* desc, uint32_t nbSrc, const uint32_t* order, const uint32_t* taskBeg,
+                    const uint32_t* taskCnt, uint32_t nbTasks, uint32_t b, uint64_t seed, uint32_t level,
+                    uint32_t* seedIdx, uint32_t* masks, hipStream_t s);
 void launchKmLevelReset(const KmLevel& k, uint64_t nbRows, hipStream_t s);
-// Per-tile swizzled operand XA of the MFMA assign from desc[order[row]].
-void launchGatherTiles(const uint8_t* desc, const uint32_t* order, const KmTile* tiles, uint32_t nbTiles, int8_t* XA,
-                       hipStream_t s);
+// Per-tile swizzled operand XA of the MFMA assign from desc[order[row]] (desc
+// holds nbSrc rows; an order entry outside them gives a zero row).  With
+// nbTilesDev, nbTiles is a bound and the device count decides.
+void launchGatherTiles(const uint8_t* desc, uint32_t nbSrc, const uint32_t* order, const KmTile* tiles, uint32_t nbTiles,
+                       int8_t* XA, hipStream_t s, const uint32_t* nbTilesDev = nullptr);
+// Next level's tasks planned on the device from a level's histogram hist[nbTasks]
+// (the child counts): taskBeg, taskCnt, tileBeg [nbTasks], *nbTiles and the
+// tile list tiles[*nbTiles] (tileBound >= the tile count sizes the launch), as
+// the host would derive them from the same counts; tiles == nullptr skips the
+// tile list.  nbTasks <= kPlanMaxTasks.
+constexpr uint32_t kPlanMaxTasks = 1u << 20;
+void launchPlanLevel(const uint32_t* hist, uint32_t nbTasks, uint32_t tileBound, uint32_t* taskBeg, uint32_t* taskCnt,
+                     uint32_t* tileBeg, uint32_t* nbTiles, KmTile* tiles, hipStream_t s);
 // Optional timing of one launch (or of a run of launches): HIP events recorded
 // by the dispatch itself when the kernel starts and when it ends, so their
 // interval is the kernel's own duration.  A null event is not recorded.

@@ -52,13 +52,13 @@ __device__ inline uint64_t hash4(uint64_t seed, uint64_t a, uint64_t b, uint64_t
 
 // ------------------------------------------------------------- gathers --
 __global__ void k_gatherRows(const uint8_t* __restrict__ desc, const uint32_t* __restrict__ idx, uint64_t n,
-                             uint8_t* __restrict__ out) {
+                             uint8_t* __restrict__ out, uint64_t nbSrc) {
     for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n * 8;
          t += uint64_t(gridDim.x) * blockDim.x) {
         const uint64_t p = t >> 3, part = t & 7;
         const uint32_t i = idx[p];
         uint4 v = make_uint4(~0u, ~0u, ~0u, ~0u);
-        if (i != kIdxInvalid) v = reinterpret_cast<const uint4*>(desc + uint64_t(i) * kDims)[part];
+        if (i != kIdxInvalid && i < nbSrc) v = reinterpret_cast<const uint4*>(desc + uint64_t(i) * kDims)[part];
         reinterpret_cast<uint4*>(out + p * kDims)[part] = v;
     }
 }
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_kmeansPP(const uint8_t* __restrict__ de
                                                  const uint32_t* __restrict__ taskBeg,
                                                  const uint32_t* __restrict__ taskCnt, uint32_t nbTasks, uint32_t b,
                                                  uint64_t seed, uint32_t level, uint32_t* __restrict__ seedIdx,
-                                                 uint32_t* __restrict__ masks) {
+                                                 uint32_t* __restrict__ masks, uint32_t nbSrc) {
     const uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + (threadIdx.x >> 6));
     const uint32_t lane = threadIdx.x & 63;
     if (g >= nbTasks) return;
@@ -151,7 +151,8 @@ __global__ __launch_bounds__(256) void k_kmeansPP(const uint8_t* __restrict__ de
                 const uint64_t lo = uint64_t(c) * n / m, hi = uint64_t(c + 1) * n / m;
                 pos = uint32_t(lo + hash4(seed, level, g, 0x100000000ull + c) % (hi - lo));
             }
-            cand[i] = order[taskBeg[g] + pos];
+            const uint32_t id = order[taskBeg[g] + pos];
+            cand[i] = id < nbSrc ? id : kIdxInvalid;  // (an order entry is always a row id < nbSrc)
             dd[i] = 0xFFFFFFFFu;
         }
     }
@@ -320,15 +321,18 @@ __device__ inline void waveRange(uint32_t nbTiles, uint32_t& beg, uint32_t& end)
 // byte r = byte (i % 8) of the 8 bytes addressed by lane 2r + i/8.  So each
 // point costs 128 bytes of HBM per pass.
 __global__ __launch_bounds__(256) void k_gatherTiles(const uint8_t* __restrict__ desc, const uint32_t* __restrict__ order,
-                                                    const KmTile* __restrict__ tiles, int8_t* __restrict__ XA) {
+                                                    const KmTile* __restrict__ tiles, int8_t* __restrict__ XA,
+                                                    const uint32_t* __restrict__ nbTilesDev, uint32_t nbSrc) {
     __shared__ __align__(16) uint8_t sx[64][144];
     const uint32_t t = blockIdx.x, tid = threadIdx.x;
+    if (nbTilesDev && t >= *nbTilesDev) return;  // a grid sized by a bound: tiles past the count
     const KmTile tl = tiles[t];
     for (uint32_t c = tid; c < 512; c += 256) {
         const uint32_t r = c >> 3, part = c & 7;
         uint4 v = make_uint4(0u, 0u, 0u, 0u);  // padding rows: x' = 0
-        if (r < tl.cnt) {
-            v = reinterpret_cast<const uint4*>(desc + uint64_t(order[tl.row + r]) * kDims)[part];
+        const uint32_t i = r < tl.cnt ? order[tl.row + r] : kIdxInvalid;
+        if (i < nbSrc) {
+            v = reinterpret_cast<const uint4*>(desc + uint64_t(i) * kDims)[part];
             v.x ^= 0x80808080u;
             v.y ^= 0x80808080u;
             v.z ^= 0x80808080u;
@@ -1084,6 +1088,63 @@ __global__ __launch_bounds__(256) void k_updateU8Part(KmLevel k) {
     }
 }
 
+// --------------------------------------------------------- level plan ----
+// The next level's tasks from a level's (task, child) histogram, on the device
+// (the host derives the same from the downloaded histogram): taskCnt = hist,
+// taskBeg / tileBeg = exclusive prefix sums of the rows and of the 64-row
+// tiles, *nbTiles = the tile count; then the tile list, tasks in order, each
+// task's rows cut into 64-row tiles.  One workgroup scans (nbTasks <=
+// kPlanMaxTasks), one thread per tile writes the list.
+__global__ __launch_bounds__(1024) void k_planLevel(const uint32_t* __restrict__ cnt, uint32_t n,
+                                                   uint32_t* __restrict__ taskBeg, uint32_t* __restrict__ taskCnt,
+                                                   uint32_t* __restrict__ tileBeg, uint32_t* __restrict__ nbTiles) {
+    __shared__ uint32_t sRows[1024], sTiles[1024];
+    const uint32_t tid = threadIdx.x, per = (n + 1023) / 1024;
+    const uint32_t beg = min(n, tid * per), end = min(n, beg + per);
+    uint32_t rows = 0, tiles = 0;
+    for (uint32_t g = beg; g < end; g++) {
+        const uint32_t c = cnt[g];
+        rows += c;
+        tiles += (c + 63) / 64;
+    }
+    sRows[tid] = rows;
+    sTiles[tid] = tiles;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const uint32_t r = tid >= d ? sRows[tid - d] : 0u, t = tid >= d ? sTiles[tid - d] : 0u;
+        __syncthreads();
+        sRows[tid] += r;
+        sTiles[tid] += t;
+        __syncthreads();
+    }
+    uint32_t r0 = sRows[tid] - rows, t0 = sTiles[tid] - tiles;
+    for (uint32_t g = beg; g < end; g++) {
+        const uint32_t c = cnt[g];
+        taskCnt[g] = c;
+        taskBeg[g] = r0;
+        tileBeg[g] = t0;
+        r0 += c;
+        t0 += (c + 63) / 64;
+    }
+    if (tid == 1023) *nbTiles = sTiles[1023];
+}
+
+__global__ void k_planTiles(const uint32_t* __restrict__ taskBeg, const uint32_t* __restrict__ taskCnt,
+                            const uint32_t* __restrict__ tileBeg, uint32_t n, const uint32_t* __restrict__ nbTiles,
+                            KmTile* __restrict__ tiles, uint32_t bound) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= bound || t >= *nbTiles) return;
+    // the last task whose first tile is <= t: empty tasks just before it share its first tile
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (tileBeg[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t r = t - tileBeg[lo];
+    tiles[t] = KmTile{lo, taskBeg[lo] + 64 * r, min(64u, taskCnt[lo] - 64 * r)};
+}
+
 // ------------------------------------------------------------ regroup ----
 // Stable regroup of a level's rows by (task, child) (Vocabulary.cpp:83-98 sorts
 // the same keys stably) as a counting sort over the level's tiles.  A
@@ -1302,9 +1363,9 @@ void launch(void (*kernel)(Params...), dim3 grid, dim3 block, uint32_t lds, hipS
 
 }  // namespace
 
-void launchGatherRows(const uint8_t* desc, const uint32_t* idx, uint64_t n, uint8_t* out, hipStream_t s) {
+void launchGatherRows(const uint8_t* desc, const uint32_t* idx, uint64_t n, uint8_t* out, hipStream_t s, uint64_t nbSrc) {
     if (!n) return;
-    k_gatherRows<<<gridFor(n * 8, 256, 8192), 256, 0, s>>>(desc, idx, n, out);
+    k_gatherRows<<<gridFor(n * 8, 256, 8192), 256, 0, s>>>(desc, idx, n, out, nbSrc);
     RBOW_HIP(hipGetLastError());
 }
 
@@ -1330,12 +1391,12 @@ void launchInitCenters(const KmLevel& k, const uint8_t* desc, const uint32_t* se
     RBOW_HIP(hipGetLastError());
 }
 
-void launchKmeansPP(const uint8_t* desc, const uint32_t* order, const uint32_t* taskBeg, const uint32_t* taskCnt,
-                    uint32_t nbTasks, uint32_t b, uint64_t seed, uint32_t level, uint32_t* seedIdx, uint32_t* masks,
-                    hipStream_t s) {
+void launchKmeansPP(const uint8_t* desc, uint32_t nbSrc, const uint32_t* order, const uint32_t* taskBeg,
+                    const uint32_t* taskCnt, uint32_t nbTasks, uint32_t b, uint64_t seed, uint32_t level,
+                    uint32_t* seedIdx, uint32_t* masks, hipStream_t s) {
     if (!nbTasks) return;
     k_kmeansPP<<<divUp(nbTasks, kWaves), 256, 0, s>>>(desc, order, taskBeg, taskCnt, nbTasks, b, seed, level, seedIdx,
-                                                       masks);
+                                                       masks, nbSrc);
     RBOW_HIP(hipGetLastError());
 }
 
@@ -1345,10 +1406,20 @@ void launchKmLevelReset(const KmLevel& k, uint64_t nbRows, hipStream_t s) {
     RBOW_HIP(hipGetLastError());
 }
 
-void launchGatherTiles(const uint8_t* desc, const uint32_t* order, const KmTile* tiles, uint32_t nbTiles, int8_t* XA,
-                       hipStream_t s) {
+void launchGatherTiles(const uint8_t* desc, uint32_t nbSrc, const uint32_t* order, const KmTile* tiles, uint32_t nbTiles,
+                       int8_t* XA, hipStream_t s, const uint32_t* nbTilesDev) {
     if (!nbTiles) return;
-    k_gatherTiles<<<nbTiles, 256, 0, s>>>(desc, order, tiles, XA);
+    k_gatherTiles<<<nbTiles, 256, 0, s>>>(desc, order, tiles, XA, nbTilesDev, nbSrc);
+    RBOW_HIP(hipGetLastError());
+}
+
+void launchPlanLevel(const uint32_t* hist, uint32_t nbTasks, uint32_t tileBound, uint32_t* taskBeg, uint32_t* taskCnt,
+                     uint32_t* tileBeg, uint32_t* nbTiles, KmTile* tiles, hipStream_t s) {
+    RBOW_REQUIRE(nbTasks >= 1 && nbTasks <= kPlanMaxTasks, RBOW_ERR_INVALID_ARG, "level plan: task count out of range");
+    k_planLevel<<<1, 1024, 0, s>>>(hist, nbTasks, taskBeg, taskCnt, tileBeg, nbTiles);
+    if (tiles)  // only the MFMA path's gather reads the tile list
+        k_planTiles<<<gridFor(tileBound, 256, 1u << 30), 256, 0, s>>>(taskBeg, taskCnt, tileBeg, nbTasks, nbTiles,
+                                                                      tiles, tileBound);
     RBOW_HIP(hipGetLastError());
 }
 

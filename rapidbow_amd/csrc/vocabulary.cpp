@@ -13,8 +13,10 @@
 #include <cstring>
 #include <initializer_list>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <random>
+#include <unordered_map>
 #include <unordered_set>
 
 namespace rbow_amd {
@@ -103,19 +105,41 @@ void d2h(T* dst, const T* src, size_t n, hipStream_t s) {
     if (n) RBOW_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
 }
 
+// Owning pinned host allocation that only grows.
+class PinnedBuf {
+public:
+    PinnedBuf() = default;
+    ~PinnedBuf() {
+        if (p_) (void)hipHostFree(p_);
+    }
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    uint8_t* reserve(size_t bytes) {
+        if (bytes > n_) {
+            if (p_) (void)hipHostFree(p_);
+            p_ = nullptr;
+            n_ = 0;
+            RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&p_), bytes));
+            n_ = bytes;
+        }
+        return p_;
+    }
+
+private:
+    uint8_t* p_ = nullptr;
+    size_t n_ = 0;
+};
+
 // One level's host -> device metadata, laid out in a pinned host arena and sent
 // with a single asynchronous copy into a device arena of the same layout.  The
-// host side is rewritten only after the level's final stream sync.
+// host side is rewritten only once the previous level's copy has completed.
 class UploadArena {
 public:
-    explicit UploadArena(size_t bytes) : dev_(bytes), cap_(bytes) {
-        RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_), std::max<size_t>(bytes, 1)));
+    UploadArena(DevBuf<uint8_t>& dev, PinnedBuf& host, size_t bytes) : cap_(bytes) {
+        dev.reserve(std::max<size_t>(bytes, 1));
+        dev_ = dev.get();
+        host_ = host.reserve(std::max<size_t>(bytes, 1));
     }
-    ~UploadArena() {
-        if (host_) (void)hipHostFree(host_);
-    }
-    UploadArena(const UploadArena&) = delete;
-    UploadArena& operator=(const UploadArena&) = delete;
     void reset() { used_ = 0; }
     // room for n elements: host pointer to fill, *dev = the same bytes on the device
     template <typename T>
@@ -123,7 +147,7 @@ public:
         const size_t off = (used_ + 255) & ~size_t(255);
         RBOW_REQUIRE(off + n * sizeof(T) <= cap_, RBOW_ERR_HIP, "level upload arena overflow");
         used_ = off + n * sizeof(T);
-        *dev = reinterpret_cast<const T*>(dev_.get() + off);
+        *dev = reinterpret_cast<const T*>(dev_ + off);
         return reinterpret_cast<T*>(host_ + off);
     }
     template <typename T>
@@ -134,7 +158,7 @@ public:
         return dev;
     }
     void upload(hipStream_t s) {
-        if (used_) RBOW_HIP(hipMemcpyAsync(dev_.get(), host_, used_, hipMemcpyHostToDevice, s));
+        if (used_) RBOW_HIP(hipMemcpyAsync(dev_, host_, used_, hipMemcpyHostToDevice, s));
     }
     static size_t bytesFor(std::initializer_list<size_t> regionBytes) {
         size_t t = 0;
@@ -143,12 +167,73 @@ public:
     }
 
 private:
-    DevBuf<uint8_t> dev_;
+    uint8_t* dev_ = nullptr;
     uint8_t* host_ = nullptr;
     size_t cap_ = 0, used_ = 0;
 };
 
+// The device and pinned buffers of a build.  After a build that completes, they
+// stay with the device for the next build (grown when a build needs more), so
+// a repeated build allocates nothing but the tree itself;
+// releaseBuildWorkspaces() frees them.
+struct BuildWs {
+    DevBuf<uint32_t> order, orderNext, counts, changed, active, nbActive, seed, partCnt, partChanged, out;
+    DevBuf<uint32_t> planTaskBeg, planTaskCnt, planTileBeg, planNbTiles;
+    DevBuf<KmTile> planTiles;
+    DevBuf<uint8_t> X, assign, sortWs, fMu, upDev;
+    DevBuf<long long> sums;
+    DevBuf<int8_t> XA;
+    DevBuf<int32_t> partSums;
+    DevBuf<uint16_t> fHi, fLo;
+    DevBuf<float> fCn2, fE1, fE2;
+    PinnedBuf hOut, upHost;
+};
+
+struct BuildWsCache {
+    std::mutex mtx;
+    std::unordered_map<int, std::unique_ptr<BuildWs>> byDevice;
+};
+// never destroyed: freeing device memory from a static destructor may run after the runtime's teardown
+BuildWsCache& buildWsCache() {
+    static BuildWsCache* c = new BuildWsCache;
+    return *c;
+}
+
+// The calling thread's device's cached workspace, or a new one when there is
+// none (or another build on the device holds it).
+std::unique_ptr<BuildWs> takeBuildWs(int device) {
+    BuildWsCache& c = buildWsCache();
+    std::lock_guard<std::mutex> lk(c.mtx);
+    auto it = c.byDevice.find(device);
+    if (it != c.byDevice.end() && it->second) return std::move(it->second);
+    return std::unique_ptr<BuildWs>(new BuildWs);
+}
+
+// Back into the cache after a build whose work on the device has completed.
+void returnBuildWs(int device, std::unique_ptr<BuildWs> w) {
+    BuildWsCache& c = buildWsCache();
+    std::lock_guard<std::mutex> lk(c.mtx);
+    std::unique_ptr<BuildWs>& slot = c.byDevice[device];
+    if (!slot) slot = std::move(w);
+}
+
 }  // namespace
+
+void releaseBuildWorkspaces() {
+    std::unordered_map<int, std::unique_ptr<BuildWs>> drop;
+    {
+        BuildWsCache& c = buildWsCache();
+        std::lock_guard<std::mutex> lk(c.mtx);
+        drop.swap(c.byDevice);
+    }
+    int cur = 0;
+    RBOW_HIP(hipGetDevice(&cur));
+    for (auto& e : drop) {
+        RBOW_HIP(hipSetDevice(e.first));
+        e.second.reset();
+    }
+    RBOW_HIP(hipSetDevice(cur));
+}
 
 Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nbDoc, uint32_t b, uint32_t L,
                             const rbow_build_options& opts, hipStream_t s, BuildStats* stats) {
@@ -213,32 +298,60 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     const uint64_t redMax = mfmaPath ? chunksMax / kReduceItemParts + nGmax : 0;
     const uint32_t persistTiles = mfmaPath ? std::min(opts.persist_max_tiles, lloydTasksTileCap(1)) : 0u;
     const uint32_t persistTiles2 = mfmaPath ? std::min(persistTiles, lloydTasksTileCap(2)) : 0u;  // 2 per CU
-    DevBuf<uint32_t> dOrder(nbDesc), dOrderNext(nbDesc);
-    DevBuf<uint8_t> dX(mfmaPath ? 0 : uint64_t(nbDesc) * kDims), dAssign(nbDesc);
-    DevBuf<uint32_t> dCounts(nGmax * b), dChanged(nGmax), dActive(nGmax), dNbActive(1), dSeed(nGmax * b);
-    DevBuf<long long> dSums(nGmax * b * kDims);
-    DevBuf<int8_t> dXA(mfmaPath ? tilesMax * 8192 : 0);
-    DevBuf<int32_t> dPartSums(chunksMax * 16 * kDims);
-    DevBuf<uint32_t> dPartCnt(chunksMax * 16), dPartChanged(chunksMax);
-    DevBuf<uint8_t> dSortWs(regroupWorkspaceBytes(tilesMax, b));
+    int device = 0;
+    RBOW_HIP(hipGetDevice(&device));
+    std::unique_ptr<BuildWs> wsOwner = takeBuildWs(device);
+    BuildWs& ws = *wsOwner;
+    ws.order.reserve(nbDesc);
+    ws.orderNext.reserve(nbDesc);
+    ws.X.reserve(mfmaPath ? 0 : uint64_t(nbDesc) * kDims);
+    ws.assign.reserve(nbDesc);
+    ws.counts.reserve(nGmax * b);
+    ws.changed.reserve(nGmax);
+    ws.active.reserve(nGmax);
+    ws.nbActive.reserve(1);
+    ws.seed.reserve(nGmax * b);
+    ws.sums.reserve(nGmax * b * kDims);
+    ws.XA.reserve(mfmaPath ? tilesMax * 8192 : 0);
+    ws.partSums.reserve(chunksMax * 16 * kDims);
+    ws.partCnt.reserve(chunksMax * 16);
+    ws.partChanged.reserve(chunksMax);
+    ws.sortWs.reserve(regroupWorkspaceBytes(tilesMax, b));
     // float centroids with kernel 0: the BF16 MFMA filter's per-task operands, redone every pass
     const bool f32Mfma = ct == 1 && kernel == 0;
-    DevBuf<uint8_t> dFMu(f32Mfma ? nGmax * kDims : 0);
-    DevBuf<uint16_t> dFHi(f32Mfma ? nGmax * b * kDims : 0), dFLo(f32Mfma ? nGmax * b * kDims : 0);
-    DevBuf<float> dFCn2(f32Mfma ? nGmax * b : 0), dFE1(f32Mfma ? nGmax * b : 0), dFE2(f32Mfma ? nGmax * b : 0);
+    ws.fMu.reserve(f32Mfma ? nGmax * kDims : 0);
+    ws.fHi.reserve(f32Mfma ? nGmax * b * kDims : 0);
+    ws.fLo.reserve(f32Mfma ? nGmax * b * kDims : 0);
+    ws.fCn2.reserve(f32Mfma ? nGmax * b : 0);
+    ws.fE1.reserve(f32Mfma ? nGmax * b : 0);
+    ws.fE2.reserve(f32Mfma ? nGmax * b : 0);
     // per-level results, downloaded with one copy: [hist nG*b][passes nG][masks nG][IDF weights nG*b]
-    DevBuf<uint32_t> dOut(2 * nGmax * b + 2 * nGmax);
-    uint32_t* hOut = nullptr;
-    RBOW_HIP(hipHostMalloc(reinterpret_cast<void**>(&hOut), 4 * (2 * nGmax * b + 2 * nGmax + 1)));
+    ws.out.reserve(2 * nGmax * b + 2 * nGmax);
+    uint32_t* hOut = reinterpret_cast<uint32_t*>(ws.hOut.reserve(4 * (2 * nGmax * b + 2 * nGmax + 1)));
     uint32_t* hNbActive = hOut + 2 * nGmax * b + 2 * nGmax;
-    struct PinnedFree {
-        uint32_t* p;
-        ~PinnedFree() { (void)hipHostFree(p); }
-    } pinnedGuard{hOut};
-    UploadArena up(UploadArena::bytesFor({4 * nGmax, 4 * nGmax, sizeof(KmTile) * tilesMax, 4 * nGmax, 4 * nGmax, 4 * nGmax,
-                                          4 * (wavesMax + 1), sizeof(KmChunk) * chunksMax, sizeof(KmSlot) * slotsMax,
-                                          4 * nGmax, 4 * nGmax, sizeof(KmPartRange) * redMax,
+    // the next level's plan on the device (k-means++ seeding only: the reference picks index the host order)
+    const bool devicePlan = opts.seed_mode == 1 && L > 1 && nGmax <= kPlanMaxTasks;
+    if (devicePlan) {
+        ws.planTaskBeg.reserve(nGmax);
+        ws.planTaskCnt.reserve(nGmax);
+        ws.planTileBeg.reserve(nGmax);
+        ws.planNbTiles.reserve(1);
+        ws.planTiles.reserve(mfmaPath ? tilesMax : 0);
+    }
+    DevBuf<uint32_t>& dOrder = ws.order;
+    DevBuf<uint32_t>& dOrderNext = ws.orderNext;
+    UploadArena up(ws.upDev, ws.upHost,
+                   UploadArena::bytesFor({4 * nGmax, 4 * nGmax, sizeof(KmTile) * tilesMax, 4 * nGmax, 4 * nGmax,
+                                          4 * nGmax, 4 * (wavesMax + 1), sizeof(KmChunk) * chunksMax,
+                                          sizeof(KmSlot) * slotsMax, 4 * nGmax, 4 * nGmax, sizeof(KmPartRange) * redMax,
                                           opts.seed_mode == 0 ? 4 * nGmax * b : 0}));
+    hipEvent_t outReady = nullptr;  // a level's results are in hOut
+    RBOW_HIP(hipEventCreateWithFlags(&outReady, hipEventDisableTiming));
+    struct EventFree {
+        hipEvent_t e;
+        ~EventFree() { (void)hipEventDestroy(e); }
+    } outGuard{outReady};
+    bool preLaunched = false;  // this level's gather, seeding and reset are already on the stream
 
     std::vector<uint32_t> taskTiles, tileBeg, smallTasks;
     std::vector<KmPartRange> red;
@@ -270,28 +383,28 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         k.nbTasks = nG;
         k.nbTiles = nbTiles;
         k.centroidType = ct;
-        k.X = dX.get();
+        k.X = ws.X.get();
         k.tiles = dTiles;
         k.taskBeg = dTaskBeg;
         k.taskCnt = dTaskCnt;
         if (ct == 0) k.centersU8 = voc->u8rows.get() + nodeBase * b * kDims;
         else k.centersF32 = voc->f32rows.get() + nodeBase * b * kDims;
-        k.assign = dAssign.get();
-        k.counts = dCounts.get();
-        k.sums = dSums.get();
-        k.changed = dChanged.get();
-        k.active = dActive.get();
-        uint32_t* dHist = dOut.get();
+        k.assign = ws.assign.get();
+        k.counts = ws.counts.get();
+        k.sums = ws.sums.get();
+        k.changed = ws.changed.get();
+        k.active = ws.active.get();
+        uint32_t* dHist = ws.out.get();
         k.passes = dHist + uint64_t(nG) * b;
         uint32_t* dMasks = k.passes + nG;
-        k.nbActive = dNbActive.get();
+        k.nbActive = ws.nbActive.get();
         if (f32Mfma) {
-            k.fMu = dFMu.get();
-            k.fHi = dFHi.get();
-            k.fLo = dFLo.get();
-            k.fCn2 = dFCn2.get();
-            k.fE1 = dFE1.get();
-            k.fE2 = dFE2.get();
+            k.fMu = ws.fMu.get();
+            k.fHi = ws.fHi.get();
+            k.fLo = ws.fLo.get();
+            k.fCn2 = ws.fCn2.get();
+            k.fE1 = ws.fE1.get();
+            k.fE2 = ws.fE2.get();
         }
 
         // Tasks of <= persistTiles tiles run their whole Lloyd loop in one launch with
@@ -327,7 +440,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
                                                   part.taskPartBeg[g] + std::min(part.taskPartCnt[g], c + kReduceItemParts)});
             RBOW_REQUIRE(part.chunks.size() <= chunksMax && part.blockSlots.size() <= slotsMax && red.size() <= redMax,
                          RBOW_ERR_HIP, "k-means work partition exceeds its bound");
-            k.XA = dXA.get();
+            k.XA = ws.XA.get();
             k.waveChunkBeg = up.put(part.waveChunkBeg.data(), part.waveChunkBeg.size());
             k.nbWaves = part.nbWaves;
             k.chunks = up.put(part.chunks.data(), part.chunks.size());
@@ -336,9 +449,9 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             k.nbParts = part.nbParts;
             k.taskPartBeg = up.put(part.taskPartBeg.data(), nG);
             k.taskPartCnt = up.put(part.taskPartCnt.data(), nG);
-            k.partSums = dPartSums.get();
-            k.partCnt = dPartCnt.get();
-            k.partChanged = dPartChanged.get();
+            k.partSums = ws.partSums.get();
+            k.partCnt = ws.partCnt.get();
+            k.partChanged = ws.partChanged.get();
             if (!smallTasks.empty()) {  // the update visits the tasks that stream (and the empty ones)
                 const uint32_t* dUpd = nullptr;
                 uint32_t* hUpd = up.take<uint32_t>(nG - smallTasks.size(), &dUpd);
@@ -353,7 +466,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         const KmPartRange* dRed = up.put(red.data(), red.size());
 
         // ---- seeds (kmeans.cpp:170-196 reference picks, or k-means++) ----
-        const uint32_t* dSeedIdx = dSeed.get();
+        const uint32_t* dSeedIdx = ws.seed.get();
         if (opts.seed_mode == 0) {
             uint32_t* seedIdx = up.take<uint32_t>(uint64_t(nG) * b, &dSeedIdx);
             uint32_t pos[32];
@@ -379,13 +492,16 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             }
         }
         up.upload(s);
-        if (level == 0) launchIota(dOrder.get(), nbDesc, s);
-        if (mfmaPath) launchGatherTiles(devDesc, dOrder.get(), dTiles, nbTiles, dXA.get(), s);
-        else launchGatherRows(devDesc, dOrder.get(), nbDesc, dX.get(), s);
-        if (opts.seed_mode == 1)
-            launchKmeansPP(devDesc, dOrder.get(), dTaskBeg, dTaskCnt, nG, b, opts.seed, level, dSeed.get(), dMasks, s);
-        launchInitCenters(k, devDesc, dSeedIdx, s);
-        launchKmLevelReset(k, nbDesc, s);
+        if (!preLaunched) {
+            if (level == 0) launchIota(dOrder.get(), nbDesc, s);
+            if (mfmaPath) launchGatherTiles(devDesc, nbDesc, dOrder.get(), dTiles, nbTiles, ws.XA.get(), s);
+            else launchGatherRows(devDesc, dOrder.get(), nbDesc, ws.X.get(), s, nbDesc);
+            if (opts.seed_mode == 1)
+                launchKmeansPP(devDesc, nbDesc, dOrder.get(), dTaskBeg, dTaskCnt, nG, b, opts.seed, level, ws.seed.get(), dMasks,
+                               s);
+            launchInitCenters(k, devDesc, dSeedIdx, s);
+            launchKmLevelReset(k, nbDesc, s);
+        }
 
         // ---- Lloyd passes (kmeans.cpp:118-133) ----
         if (!smallTasks.empty()) {
@@ -412,16 +528,16 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             if (f32Mfma) {  // the filter's operands of this pass's centers, timed with the assign
                 if (timing) RBOW_HIP(hipEventRecord(ta.start, s));
                 ta.start = nullptr;
-                launchPrepF32Tree(k.centersF32, nG, b, dFMu.get(), dFHi.get(), dFLo.get(), dFCn2.get(), dFE1.get(),
-                                  dFE2.get(), s, k.active);
+                launchPrepF32Tree(k.centersF32, nG, b, ws.fMu.get(), ws.fHi.get(), ws.fLo.get(), ws.fCn2.get(),
+                                  ws.fE1.get(), ws.fE2.get(), s, k.active);
             }
             ranAssign[it] = launchKmAssign(k, kernel, s, ta);
             if (reduced) launchKmReduce(k, dRed, uint32_t(red.size()), s, timer(e + 2));
-            if (poll) launchFillU32(dNbActive.get(), 0u, 1, s);
+            if (poll) launchFillU32(ws.nbActive.get(), 0u, 1, s);
             ranUpdate[it] = launchKmUpdate(k, s, timer(e + 4));
             nbLaunched = it + 1;
             if (poll && (it % 4 == 3)) {  // kmeans.cpp:125-131 polls every 4 passes
-                RBOW_HIP(hipMemcpyAsync(hNbActive, dNbActive.get(), 4, hipMemcpyDeviceToHost, s));
+                RBOW_HIP(hipMemcpyAsync(hNbActive, ws.nbActive.get(), 4, hipMemcpyDeviceToHost, s));
                 RBOW_HIP(hipStreamSynchronize(s));
                 if (*hNbActive == 0) break;
             }
@@ -430,15 +546,43 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         // children's IDF weights from the occurrence counts (Vocabulary.cpp:103-108) ----
         // (the last level needs the occurrence counts only)
         const bool last = level + 1 == L;
-        launchRegroup(k, nbDesc, dOrder.get(), last ? nullptr : dOrderNext.get(), dHist, dSortWs.get(), dSortWs.size(),
+        launchRegroup(k, nbDesc, dOrder.get(), last ? nullptr : dOrderNext.get(), dHist, ws.sortWs.get(), ws.sortWs.size(),
                       s);
         if (!last) std::swap(dOrder, dOrderNext);
         float* dW = reinterpret_cast<float*>(dMasks + nG);
         launchIdfWeights(dHist, uint64_t(nG) * b, nbDoc, dW, s);
         const uint64_t outWords = 2 * uint64_t(nG) * b + 2 * nG;
-        RBOW_HIP(hipMemcpyAsync(hOut, dOut.get(), 4 * outWords, hipMemcpyDeviceToHost, s));
+        RBOW_HIP(hipMemcpyAsync(hOut, ws.out.get(), 4 * outWords, hipMemcpyDeviceToHost, s));
         if (opts.seed_mode == 0 && level + 1 < L) d2h(order.data(), dOrder.get(), nbDesc, s);
-        RBOW_HIP(hipStreamSynchronize(s));
+        RBOW_HIP(hipEventRecord(outReady, s));
+        // While the host reads the results and plans the next level, the device plans the
+        // next level's tasks from the histogram itself and runs what needs no more than that:
+        // the gather, k-means++ seeding, the centers and the reset.  The tree's derived data
+        // for lookups follows the last level.
+        preLaunched = false;
+        if (devicePlan && level + 1 < L) {
+            const uint32_t nG2 = nG * b;
+            const uint32_t tileBound = divUp(nbDesc, 64u) + nG2;
+            launchPlanLevel(dHist, nG2, tileBound, ws.planTaskBeg.get(), ws.planTaskCnt.get(), ws.planTileBeg.get(),
+                            ws.planNbTiles.get(), mfmaPath ? ws.planTiles.get() : nullptr, s);
+            KmLevel k2 = k;
+            k2.nbTasks = nG2;
+            if (ct == 0) k2.centersU8 = voc->u8rows.get() + levelBeg(lg2b, level + 1) * b * kDims;
+            else k2.centersF32 = voc->f32rows.get() + levelBeg(lg2b, level + 1) * b * kDims;
+            k2.passes = dHist + uint64_t(nG2) * b;
+            if (mfmaPath)
+                launchGatherTiles(devDesc, nbDesc, dOrder.get(), ws.planTiles.get(), tileBound, ws.XA.get(), s,
+                                  ws.planNbTiles.get());
+            else
+                launchGatherRows(devDesc, dOrder.get(), nbDesc, ws.X.get(), s, nbDesc);
+            launchKmeansPP(devDesc, nbDesc, dOrder.get(), ws.planTaskBeg.get(), ws.planTaskCnt.get(), nG2, b, opts.seed,
+                           level + 1, ws.seed.get(), k2.passes + nG2, s);
+            launchInitCenters(k2, devDesc, ws.seed.get(), s);
+            launchKmLevelReset(k2, nbDesc, s);
+            preLaunched = true;
+        }
+        if (level + 1 == L) voc->prepare(s);
+        RBOW_HIP(hipEventSynchronize(outReady));
         const uint32_t* childCnt = hOut;
         const uint32_t* hPasses = hOut + uint64_t(nG) * b;
         const uint32_t* hMasks = hPasses + nG;
@@ -482,8 +626,8 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         taskCnt.assign(childCnt, childCnt + uint64_t(nG) * b);
     }
 
-    voc->prepare(s);  // the tree's derived data on the device
     RBOW_HIP(hipStreamSynchronize(s));
+    returnBuildWs(device, std::move(wsOwner));  // the stream is idle: the next build may reuse it
     if (stats) {
         stats->passes = std::move(passesAll);
         stats->assignMs = assignMs;

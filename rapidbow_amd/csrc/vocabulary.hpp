@@ -44,6 +44,8 @@ struct BuildStats {
 // float32 centroids.  Blocks until done.
 Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nbDoc, uint32_t b, uint32_t L,
                             const rbow_build_options& opts, hipStream_t s, BuildStats* stats);
+// Frees the work buffers that completed builds leave with each device for the next build.
+void releaseBuildWorkspaces();
 
 std::vector<uint8_t> serialize(const Vocabulary& v);
 Vocabulary* deserialize(const uint8_t* blob, size_t size);

@@ -138,6 +138,30 @@ def test_build_timing_option():
         assert st[f] == s0[f], f
 
 
+def test_build_workspace_reuse():
+    """Builds one after another reuse the work buffers a completed build leaves
+    with the device (larger than the next build needs, holding the last build's
+    data): a larger build, a smaller one with the other seed mode, a float tree,
+    then the first again, each against the oracle; after the release a fresh
+    build gives the same tree."""
+    big = rb.synth_descriptors(60000, seed=21)
+    small = rb.synth_descriptors(9000, seed=22)
+    res_big = oracle.build(big, 200, 3, max_iters=6, seed_mode=1)
+    res_small = oracle.build(small, 50, 3, max_iters=6, seed_mode=0)
+    first = _gpu_build(big, 200, 3, max_iters=6, seed_mode=1, kernel=0)
+    _check_u8(first, res_big, 3)
+    _check_u8(_gpu_build(small, 50, 3, max_iters=6, seed_mode=0, kernel=0), res_small, 3)
+    res_f = oracle.build(small, 50, 2, b=32, max_iters=5, seed_mode=1, centroid_type=1)
+    vf = _gpu_build(small, 50, 2, branch_factor=32, max_iters=5, seed_mode=1, centroid_type=1, kernel=0)
+    for l in range(2):
+        np.testing.assert_array_equal(vf.centroids(l), res_f["nodes"][level_beg(32, l):level_beg(32, l + 1)])
+    again = _gpu_build(big, 200, 3, max_iters=6, seed_mode=1, kernel=0)
+    assert again.serialize() == first.serialize()
+    rb.release_build_workspaces()
+    fresh = _gpu_build(big, 200, 3, max_iters=6, seed_mode=1, kernel=0)
+    assert fresh.serialize() == first.serialize()
+
+
 @pytest.mark.parametrize("seed_mode", [0, 1])
 def test_build_and_lookup_small_tree_full_size(seed_mode):
     """The small-tree workload at its own size: 200,000 SIFT-like descriptors,
