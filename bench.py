@@ -450,6 +450,9 @@ def main():
     ap.add_argument("--skip-retrieval", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=1.5)
+    ap.add_argument("--profile-builds", type=int, default=0,
+                    help="run this many b16 L4 builds without per-launch timing (after the warmup), "
+                         "print their times and stop: a short command for a profiler")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this library runs on one GPU only; sharding over several GPUs is not implemented",
@@ -470,6 +473,15 @@ def main():
     for _ in range(args.warmup):
         rb.build_vocabulary(d, N, 1000, 4, max_iters=10, seed_mode=1, kernel=0, timing=True)
     hip.synchronize()
+    if args.profile_builds:
+        for i in range(args.profile_builds):
+            t = time.perf_counter()
+            v = rb.build_vocabulary(d, N, 1000, 4, max_iters=10, seed_mode=1, kernel=0)
+            hip.synchronize()
+            log(f"bench: build {i}: {(time.perf_counter() - t) * 1e3:.2f} ms wall, "
+                f"{v.build_stats()['total_ms']:.2f} ms library")
+            del v
+        return
     stats = []
     t0 = time.perf_counter()
     for i in range(args.steps):

@@ -69,16 +69,26 @@ __global__ void k_iota(uint32_t* __restrict__ p, uint64_t n) {
 }
 
 // reformatDesc.cu:38-74: tiles[t][word][slot] = word of desc[idx[t][slot]].
-__global__ void k_reformatTiles(const uint8_t* __restrict__ desc, const uint32_t* __restrict__ idx, uint32_t nbTiles,
-                                uint32_t tileSize, uint32_t* __restrict__ tiles) {
-    const uint64_t total = uint64_t(nbTiles) * 32 * tileSize;
-    for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
-         e += uint64_t(gridDim.x) * blockDim.x) {
-        const uint64_t t = e / (32 * tileSize);
-        const uint32_t w = uint32_t((e / tileSize) % 32), s = uint32_t(e % tileSize);
-        const uint32_t i = idx[t * tileSize + s];
-        tiles[e] = i == kIdxInvalid ? ~0u : reinterpret_cast<const uint32_t*>(desc + uint64_t(i) * kDims)[w];
+// One workgroup per tile: the tile's rows are read whole (16-byte loads, 8 lanes
+// per 128-byte row) into LDS, then written word-major, both coalesced.
+__global__ __launch_bounds__(256) void k_reformatTiles(const uint8_t* __restrict__ desc, const uint32_t* __restrict__ idx,
+                                                      uint32_t tileSize, uint32_t* __restrict__ tiles) {
+    __shared__ uint32_t sx[32][33];  // [slot][word], tileSize <= 32
+    const uint64_t t = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t c = tid; c < tileSize * 8; c += 256) {
+        const uint32_t slot = c >> 3, q = c & 7;
+        const uint32_t i = idx[t * tileSize + slot];
+        const uint4 v = i == kIdxInvalid ? make_uint4(~0u, ~0u, ~0u, ~0u)
+                                         : reinterpret_cast<const uint4*>(desc + uint64_t(i) * kDims)[q];
+        sx[slot][4 * q] = v.x;
+        sx[slot][4 * q + 1] = v.y;
+        sx[slot][4 * q + 2] = v.z;
+        sx[slot][4 * q + 3] = v.w;
     }
+    __syncthreads();
+    uint32_t* out = tiles + t * 32 * tileSize;
+    for (uint32_t e = tid; e < 32 * tileSize; e += 256) out[e] = sx[e % tileSize][e / tileSize];
 }
 
 // ----------------------------------------------------------- seeding ----
@@ -1378,8 +1388,8 @@ void launchIota(uint32_t* p, uint64_t n, hipStream_t s) {
 void launchReformatTiles(const uint8_t* desc, const uint32_t* idx, uint32_t nbTiles, uint32_t tileSize,
                          uint32_t* tiles, hipStream_t s) {
     if (!nbTiles) return;
-    k_reformatTiles<<<gridFor(uint64_t(nbTiles) * 32 * tileSize, 256, 8192), 256, 0, s>>>(desc, idx, nbTiles, tileSize,
-                                                                                       tiles);
+    RBOW_REQUIRE(tileSize >= 1 && tileSize <= 32, RBOW_ERR_INVALID_ARG, "tile size above 32");
+    k_reformatTiles<<<nbTiles, 256, 0, s>>>(desc, idx, tileSize, tiles);
     RBOW_HIP(hipGetLastError());
 }
 
