@@ -127,17 +127,21 @@ __device__ __forceinline__ uint32_t filterCandidates(const v4f (&acc)[CB], float
 // loads of many rows overlap (a tie resolved where it appears stalls the wave
 // on one row's dependent chain).  Entry e: tag[e] (the descriptor row), aux[e]
 // (the caller's handle for the result) and mask[e] (the row's candidates).
-// Capacity kTieCap; a caller pushes one block (<= 16 rows) at a time and
-// flushes once more than kTieCap - 16 wait.
-constexpr uint32_t kTieCap = 64;
-struct TieQueue {
-    uint32_t tag[kTieCap], aux[kTieCap], mask[kTieCap];
+// Capacity Cap; a caller pushes one block (<= 16 rows) at a time and resolves
+// the queue before it could overflow.
+template <uint32_t Cap>
+struct TieQueueT {
+    static constexpr uint32_t kCap = Cap;
+    uint32_t tag[Cap], aux[Cap], mask[Cap];
 };
+constexpr uint32_t kTieCap = 64;
+using TieQueue = TieQueueT<kTieCap>;
 
 // Appends the rows of this block with several candidates: the group-0 lane of
 // a row with tie == true adds (tag, aux, mask).  n: the wave's queue length
 // (uniform).  Returns the number of rows added.
-__device__ inline uint32_t tiePush(TieQueue& tq, uint32_t& n, uint32_t lane, bool tie, uint32_t tag, uint32_t aux,
+template <class Q>
+__device__ inline uint32_t tiePush(Q& tq, uint32_t& n, uint32_t lane, bool tie, uint32_t tag, uint32_t aux,
                                    uint32_t mask) {
     const uint64_t b = __ballot(tie && (lane >> 4) == 0);
     if (tie && (lane >> 4) == 0) {
@@ -156,8 +160,8 @@ __device__ inline uint32_t tiePush(TieQueue& tq, uint32_t& n, uint32_t lane, boo
 // per round; done(aux, best) runs in the row's first lane with the plain
 // argmin (ties to the lowest child).  rowPtr(tag): the row's 128 bytes;
 // childPtr(j): centroid j's float row.  Leaves n = 0.
-template <class RowPtr, class ChildPtr, class Done>
-__device__ __forceinline__ void resolveTies(const TieQueue& tq, uint32_t& n, uint32_t lane, RowPtr rowPtr,
+template <class Q, class RowPtr, class ChildPtr, class Done>
+__device__ __forceinline__ void resolveTies(const Q& tq, uint32_t& n, uint32_t lane, RowPtr rowPtr,
                                             ChildPtr childPtr, Done done) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the queue writes of this wave are visible
     __builtin_amdgcn_wave_barrier();

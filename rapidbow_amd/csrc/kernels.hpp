@@ -12,10 +12,10 @@ namespace rbow_amd {
 //   rows      [node][b][128]  centroid rows, child-major (uint8 or float32)
 //   u8 trees: normU[node*b+j]  = ||c||^2            (plain dot4 form)
 //             normS[node*b+j]  = ||c - 128||^2      (I8 MFMA form, offset operands)
-//   f32 trees: mu[node][128]  = integer centre of the node's children (uint8)
-//             hi/lo           = bf16 split of c - mu (c - mu ~= hi + lo)
-//             cn2[node*b+j]   = ||c - mu||^2 (fp32 of the binary64 value)
-//             e1[node*b+j]    = certified bound coefficient of ||x|| (see lookup.hip)
+//   f32 trees: split[t][node*b+j][128], t = 0..2: int8 terms of c - 128 =
+//               A0 + A1 2^-7 + A2 2^-14 + residual (splitStride bytes per term)
+//             cn2[node*b+j]   = ||c - 128||^2 (fp32 of the binary64 value)
+//             e1[node*b+j]    = certified bound coefficient of ||x - 128|| (see lookup.hip)
 //             e2[node*b+j]    = certified bound constant term
 struct TreeView {
     uint32_t b = 16, lg2b = 4, L = 0, centroidType = 0;
@@ -23,9 +23,8 @@ struct TreeView {
     const int32_t* normU = nullptr;
     const int32_t* normS = nullptr;
     const float* f32rows = nullptr;
-    const uint8_t* mu = nullptr;
-    const uint16_t* hi = nullptr;
-    const uint16_t* lo = nullptr;
+    const int8_t* split = nullptr;
+    uint64_t splitStride = 0;
     const float* cn2 = nullptr;
     const float* e1 = nullptr;
     const float* e2 = nullptr;
@@ -34,11 +33,15 @@ struct TreeView {
                            // fused single-launch descent, larger ones the bucketed one (0: built-in default)
 };
 
-// Derived per-child data (norms, bf16 split, bounds) for nodes [0, nbNodes); the
-// float form skips the nodes whose flag in `active` (optional) is zero.
+// Derived per-child data for nodes [0, nbNodes): uint8 trees' norms; for the
+// float k-means assign, the bf16 split of c - mu and bounds (nodes whose flag in
+// `active`, optional, is zero are skipped); for float-tree lookups, the int8
+// split of c - 128 and bounds (split: 3 terms of splitStride bytes).
 void launchPrepU8Tree(const uint8_t* rows, uint64_t nbChildren, int32_t* normU, int32_t* normS, hipStream_t s);
 void launchPrepF32Tree(const float* rows, uint64_t nbNodes, uint32_t b, uint8_t* mu, uint16_t* hi, uint16_t* lo,
                        float* cn2, float* e1, float* e2, hipStream_t s, const uint32_t* active = nullptr);
+void launchPrepF32Split(const float* rows, uint64_t nbNodes, uint32_t b, int8_t* split, uint64_t splitStride,
+                        float* cn2, float* e1, float* e2, hipStream_t s);
 
 // Plain scalar per-descriptor descent (vocTreeLookUp.cu:48-160 semantics).
 void launchLookupPlain(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s);

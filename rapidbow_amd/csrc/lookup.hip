@@ -127,6 +127,65 @@ __global__ __launch_bounds__(128) void k_prepF32(const float* rows, uint32_t b, 
     }
 }
 
+// One 128-thread block per node (thread k = dimension k), for float-tree lookups.
+// With c'' = c - 128 (exact), binary64 (every step exact):
+//   A0 = rint(c''), r0 = c'' - A0;  A1 = rint(2^7 r0), r1 = r0 - 2^-7 A1;
+//   A2 = rint(2^14 r1), r = r1 - 2^-14 A2;  c'' = Q + r,  Q = A0 + 2^-7 A1 + 2^-14 A2,
+//   A0 in [-128, 127] (c in [0, 255]), A1, A2 in [-64, 64], |r| <= 2^-15.
+// The level kernel's products P_t = x''.A_t (x'' = x - 128) are exact int32, so
+// its score s~ = fl(cn2 - 2 fl(P0 + 2^-7 fl(P1 + 2^-7 P2))) (fma each) against
+// s = ||c''||^2 - 2 x''.c'' = d(x, c) - ||x''||^2 errs by at most e1 ||x''|| + e2 with
+//   e1 = (2||r|| + 4u (||Q|| + ||r0 - r||)) (1 + 2^-10),  e2 = 2u ||c''||^2 (1 + 2^-10),  u = 2^-24:
+//   2 x''.r (Cauchy-Schwarz); the two combining roundings, u|x''.Q| + u 2^-7 |x''.(A1 + 2^-7 A2)|
+//   with A1 + 2^-7 A2 = 2^7 (r0 - r), doubled by the factor -2; the final rounding
+//   u |cn2 - 2 t~| and that of cn2 itself, u ||c''||^2 each (the slack covers the
+//   (1 + u) factors, the binary64 norms and the roundings of e1 and e2).
+// A node with a child value outside [0, 255] (or not finite) has no such split:
+// its terms are zero and e1 = inf, so every child stays a candidate and the plain
+// FP32 chain decides.
+__global__ __launch_bounds__(128) void k_prepF32Split(const float* __restrict__ rows, uint32_t b,
+                                                     int8_t* __restrict__ split, uint64_t stride,
+                                                     float* __restrict__ cn2, float* __restrict__ e1,
+                                                     float* __restrict__ e2) {
+    __shared__ double red[4][2][32];
+    const uint64_t node = blockIdx.x;
+    const uint32_t k = threadIdx.x, lane = k & 63, wave = k >> 6;
+    const float* c = rows + node * b * kDims;
+    bool bad = false;
+    for (uint32_t j = 0; j < b; j++) {
+        const float v = c[j * kDims + k];
+        bad |= !(v >= 0.0f && v <= 255.0f);
+    }
+    bad = __syncthreads_or(bad);
+    for (uint32_t j = 0; j < b; j++) {
+        const uint64_t child = node * b + j;
+        const double cd = bad ? 0.0 : double(c[j * kDims + k]) - 128.0;
+        const double a0 = rint(cd), r0 = cd - a0;
+        const double a1 = rint(r0 * 128.0), r1 = r0 - a1 / 128.0;
+        const double a2 = rint(r1 * 16384.0), r = r1 - a2 / 16384.0;
+        split[child * kDims + k] = int8_t(a0);
+        split[stride + child * kDims + k] = int8_t(a1);
+        split[2 * stride + child * kDims + k] = int8_t(a2);
+        const double qv = cd - r, dr = r0 - r;
+        double v[4] = {cd * cd, r * r, qv * qv, dr * dr};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            for (int d = 32; d >= 1; d >>= 1) v[q] += __shfl_xor(v[q], d, 64);
+            if (lane == 0) red[q][wave][j] = v[q];
+        }
+    }
+    __syncthreads();
+    if (k < b) {
+        const uint64_t child = node * b + k;
+        const double cn = red[0][0][k] + red[0][1][k], rn = red[1][0][k] + red[1][1][k];
+        const double qn = red[2][0][k] + red[2][1][k], dn = red[3][0][k] + red[3][1][k];
+        const double u = 1.0 / 16777216.0, slack = 1.0 + 1.0 / 1024;
+        cn2[child] = bad ? 0.0f : float(cn);
+        e1[child] = bad ? INFINITY : float((2.0 * sqrt(rn) + 4.0 * u * (sqrt(qn) + sqrt(dn))) * slack);
+        e2[child] = bad ? 0.0f : float(2.0 * u * cn * slack);
+    }
+}
+
 // -------------------------------------------------------- plain kernels --
 // 16 lanes per descriptor; 16 descriptors per 256-thread block.
 __global__ __launch_bounds__(256) void k_lookupPlainU8(const uint8_t* __restrict__ rows,
@@ -529,56 +588,94 @@ __global__ __launch_bounds__(256) void k_lookupFusedI8(const uint8_t* __restrict
     }
 }
 
-// ----------------------------------------------- split-BF16 level step --
-// mfma_f32_16x16x32_bf16: lane l feeds 8 dims [8*(l>>4) + 32*kb, +8), kb=0..3,
-// of A row (l&15) and of B column (l&15).  The node's children are A (16 per
-// block cb) and 16 descriptor rows are B, so the result lane l holds children
-// cb*16 + 4*(l>>4) + i of row (l&15): each lane owns one row (filterCandidates);
-// near ties are queued per wave and resolved at the item's end (resolveTies).
+// ------------------------------------------------ int8-split level step --
+// mfma_i32_16x16x64_i8 as in k_levelI8: the node's children are A (16 per block
+// cb, one int8 term t at a time) and 16 descriptor rows are B (x - 128, the
+// bytes with their top bit flipped), so the accumulator of lane l holds the
+// exact products P_t of children cb*16 + 4*(l>>4) + i with row (l&15).  The
+// terms combine into the score's dot part (k_prepF32Split): P1 * 2^7 + P2 is
+// exact in int32, then t = fl(P0 + 2^-14 fl(2^7 P1 + P2)), whose two roundings
+// are within the bound k_prepF32Split derives for the three-step form.  The
+// certified filter keeps every child that can be the argmin; near ties are
+// queued per wave and resolved with the plain FP32 chain at the item's end.
+//
+// Candidate test of one row: s_j = fl(cn2_j - 2 t_j), m = min_j s_j, and child j
+// is kept when s_j <= fl(m + tau) (tau as in filterCandidates).  Rounding is
+// monotone and s_j is a float, so s_j - m <= tau implies s_j <= fl(m + tau): the
+// kept set contains filterCandidates' set and the argmin rule is unchanged.
+template <int CB>
+__device__ __forceinline__ uint32_t filterSplit(const v4f (&t)[CB], float xn2, const float (&cn)[CB][4], float e1max,
+                                                float e2max, uint32_t lane, bool rowValid) {
+    const uint32_t grp = lane >> 4;
+    float sc[CB][4];
+    float m = INFINITY;
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            sc[cb][i] = fmaf(-2.0f, t[cb][i], cn[cb][i]);
+            m = minF(m, sc[cb][i]);
+        }
+    m = minAcrossGroupsF(m);
+    const float xn = __builtin_amdgcn_sqrtf(xn2) * (1.0f + 1.0f / 1048576.0f);  // v_sqrt_f32: <= 1 ulp
+    const float e1v = fmaf(e1max, xn, e2max);
+    const float D = fmaxf(xn2 + m, 0.0f);
+    const float thr = m + kTauScale * fmaf(kGammaPlain, D, e1v * (1.0f + kGammaPlain));
+    uint32_t mask = 0;
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) mask |= sc[cb][i] <= thr ? 1u << (cb * 16 + 4 * grp + i) : 0u;
+    mask = orAcrossGroups(mask);
+    return rowValid ? mask : 1u;
+}
+
+// Sum over lanes l, l ^ 16, l ^ 32, l ^ 48, in every lane (permlane swaps, as minAcrossGroupsF).
+__device__ inline int sumAcrossGroups(int v) {
+    const auto p = __builtin_amdgcn_permlane16_swap(uint32_t(v), uint32_t(v), false, false);
+    v = int(p[0]) + int(p[1]);
+    const auto q = __builtin_amdgcn_permlane32_swap(uint32_t(v), uint32_t(v), false, false);
+    return int(q[0]) + int(q[1]);
+}
+
 template <bool kIdentity, bool kLast, int B>
-__global__ __launch_bounds__(256) void k_levelBf16(
-    const float* __restrict__ rows32, const uint8_t* __restrict__ mu, const uint16_t* __restrict__ hi,
-    const uint16_t* __restrict__ lo, const float* __restrict__ cn2, const float* __restrict__ e1,
-    const float* __restrict__ e2, uint64_t levelBase,
+__global__ __launch_bounds__(256) void k_levelSplit(
+    const float* __restrict__ rows32, const int8_t* __restrict__ split, uint64_t stride,
+    const float* __restrict__ cn2, const float* __restrict__ e1, const float* __restrict__ e2, uint64_t levelBase,
     const uint8_t* __restrict__ desc, uint32_t n, const uint32_t* __restrict__ perm, const LkItem* __restrict__ items,
     const uint32_t* __restrict__ nbItemsPtr, uint32_t nbItemsImplicit, uint32_t* __restrict__ out,
     uint32_t* __restrict__ C, uint32_t* __restrict__ nearTies) {
     constexpr int CB = B / 16;
     constexpr uint32_t LG2B = B == 32 ? 5 : 4;
+    static_assert(kTQ == 256, "an item is four groups of four 16-row blocks");
     __shared__ uint8_t sjAll[4][kTQ];
     __shared__ uint32_t scAll[4][32];
-    __shared__ TieQueue tqAll[4];
+    __shared__ TieQueueT<kTQ> tqAll[4];  // a whole item's rows: resolved once, at the item's end
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 15, grp = lane >> 4;
     uint8_t* sj = sjAll[wave];
-    TieQueue& tq = tqAll[wave];
+    TieQueueT<kTQ>& tq = tqAll[wave];
     const uint32_t nbItems = kIdentity ? nbItemsImplicit : *nbItemsPtr;
     uint32_t ties = 0, nTie = 0;
     for (uint32_t it = blockIdx.x * 4 + wave; it < nbItems; it += gridDim.x * 4) {
         const LkItem t = itemAt<kIdentity>(it, n, items);
         const uint64_t node = levelBase + t.node, childBase = node * B;
-        // near ties of this item: the plain chain over the queued rows' candidates
-        auto flushTies = [&]() {
-            resolveTies(
-                tq, nTie, lane, [&](uint32_t q) { return desc + uint64_t(q) * kDims; },
-                [&](uint32_t c) { return rows32 + (childBase + c) * kDims; },
-                [&](uint32_t aux, uint32_t best) {
-                    if (kLast) out[aux] = t.node * B + best;  // aux: the descriptor
-                    else sj[aux] = uint8_t(best);             // aux: the row in the item
-                });
-        };
-        v8bf bh[CB][4], bl[CB][4];
-        float cn[CB][4];
-        uint2 muw[4];  // the node's centre on this lane's dims
+        // the item's descriptors: lane l holds those of rows g*64 + l, g = 0..3 (group g)
+        uint32_t qv[4];
 #pragma unroll
-        for (int kb = 0; kb < 4; kb++) muw[kb] = *reinterpret_cast<const uint2*>(mu + node * kDims + kb * 32 + grp * 8);
+        for (int g = 0; g < 4; g++) {
+            const uint32_t r = g * 64 + lane;
+            qv[g] = r < t.cnt ? (kIdentity ? t.start + r : perm[t.start + r]) : 0u;
+        }
+        v4i ca[3][CB][2];
+        float cn[CB][4];
 #pragma unroll
         for (int cb = 0; cb < CB; cb++) {
-            const uint64_t child = childBase + cb * 16 + col;
+            const int8_t* crow = split + (childBase + cb * 16 + col) * kDims + grp * 16;
 #pragma unroll
-            for (int kb = 0; kb < 4; kb++) {
-                bh[cb][kb] = *reinterpret_cast<const v8bf*>(hi + child * kDims + kb * 32 + grp * 8);
-                bl[cb][kb] = *reinterpret_cast<const v8bf*>(lo + child * kDims + kb * 32 + grp * 8);
+            for (int tt = 0; tt < 3; tt++) {
+                ca[tt][cb][0] = *reinterpret_cast<const v4i*>(crow + tt * stride);
+                ca[tt][cb][1] = *reinterpret_cast<const v4i*>(crow + tt * stride + 64);
             }
             const float4 c4 = *reinterpret_cast<const float4*>(cn2 + childBase + cb * 16 + 4 * grp);
             cn[cb][0] = c4.x;
@@ -593,88 +690,74 @@ __global__ __launch_bounds__(256) void k_levelBf16(
             e1m = fmaxf(e1m, __shfl_xor(e1m, d, 64));
             e2m = fmaxf(e2m, __shfl_xor(e2m, d, 64));
         }
-        // ||mu||^2 over this lane's dims (the rest of ||x - mu||^2 comes from byte dot products)
-        uint32_t mm = 0;
-#pragma unroll
-        for (int kb = 0; kb < 4; kb++) {
-            mm = __builtin_amdgcn_udot4(muw[kb].x, muw[kb].x, mm, false);
-            mm = __builtin_amdgcn_udot4(muw[kb].y, muw[kb].y, mm, false);
-        }
-        // the descriptor bytes of row blocks rb+1 and rb+2, and the permutation
-        // entries of blocks rb+3 and rb+4, are in flight while rb computes
-        auto permAt = [&](uint32_t rb) {
-            const uint32_t r = rb * 16 + col;
-            return r < t.cnt ? (kIdentity ? t.start + r : perm[t.start + r]) : 0u;
+        // Block rb = 4 g + S of group g: its rows' bytes sit in slot S, loaded four blocks
+        // ahead (the slot's previous block), so a block computes while the next four load.
+        v4i rx0[4], rx1[4];
+        auto rowOf = [&](uint32_t qg, int S) {  // descriptor of this lane's row of block S of a group
+            return uint32_t(__shfl(int(qg), S * 16 + int(col), 64));
         };
-        auto loadRows = [&](uint32_t rb, uint32_t q, uint2 (&w)[4]) {
+        auto loadRows = [&](uint32_t rb, uint32_t q, v4i& x0, v4i& x1) {
+            x0 = v4i{0, 0, 0, 0};  // rows past the item's end contribute nothing (and are not written)
+            x1 = v4i{0, 0, 0, 0};
             if (rb * 16 + col < t.cnt) {
                 const uint8_t* x = desc + uint64_t(q) * kDims;
-#pragma unroll
-                for (int kb = 0; kb < 4; kb++) w[kb] = *reinterpret_cast<const uint2*>(x + kb * 32 + grp * 8);
+                x0 = toSigned(ld16(x + grp * 16));
+                x1 = toSigned(ld16(x + 64 + grp * 16));
             }
         };
-        // rows past the item's end keep the previous (or zero) bytes: their scores are discarded
-        uint2 wn[4] = {}, wn2[4] = {};
-        uint32_t qc = permAt(0), qn = permAt(1), q2 = permAt(2), q3 = permAt(3);
-        loadRows(0, qc, wn);
-        loadRows(1, qn, wn2);
-        for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
-            const uint32_t r = rb * 16 + col;
-            const bool valid = r < t.cnt;
-            uint2 wc[4];
+        unrollSeq([&](auto S) { loadRows(S, rowOf(qv[0], S), rx0[S], rx1[S]); }, std::make_integer_sequence<int, 4>{});
+        for (uint32_t g = 0; g * 64 < t.cnt; g++) {
+            unrollSeq(
+                [&](auto S) {
+                    const uint32_t rb = g * 4 + S;
+                    if (rb * 16 >= t.cnt) return;
+                    const uint32_t r = rb * 16 + col, q = rowOf(qv[0], S);
+                    const bool valid = r < t.cnt;
+                    const v4i x0 = rx0[S], x1 = rx1[S];
+                    if ((rb + 4) * 16 < t.cnt) loadRows(rb + 4, rowOf(qv[1], S), rx0[S], rx1[S]);
+                    // ||x - 128||^2 of row (lane & 15): an exact integer, summed over the row's four lanes
+                    int xs = 0;
 #pragma unroll
-            for (int kb = 0; kb < 4; kb++) {
-                wc[kb] = wn[kb];
-                wn[kb] = wn2[kb];
-            }
-            if ((rb + 2) * 16 < t.cnt) loadRows(rb + 2, q2, wn2);
-            const uint32_t q = qc;  // this block's row (lane & 15)
-            qc = qn;
-            qn = q2;
-            q2 = q3;
-            if ((rb + 4) * 16 < t.cnt) q3 = permAt(rb + 4);
-            v8bf ax[4];
-            uint32_t xx = 0, xm = 0;
+                    for (int e = 0; e < 4; e++) {
+                        xs = __builtin_amdgcn_sdot4(x0[e], x0[e], xs, false);
+                        xs = __builtin_amdgcn_sdot4(x1[e], x1[e], xs, false);
+                    }
+                    xs = sumAcrossGroups(xs);
+                    v4f acc[CB];
 #pragma unroll
-            for (int kb = 0; kb < 4; kb++) {
-                const uint2 w = wc[kb];
-                xx = __builtin_amdgcn_udot4(w.x, w.x, xx, false);
-                xx = __builtin_amdgcn_udot4(w.y, w.y, xx, false);
-                xm = __builtin_amdgcn_udot4(w.x, muw[kb].x, xm, false);
-                xm = __builtin_amdgcn_udot4(w.y, muw[kb].y, xm, false);
+                    for (int cb = 0; cb < CB; cb++) {
+                        v4i p[3];
 #pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    const uint32_t xw = e < 4 ? w.x : w.y, mw = e < 4 ? muw[kb].x : muw[kb].y;
-                    // the difference in [-255, 255] is exact in float and in bf16
-                    const float dv = float((xw >> (8 * (e & 3))) & 0xFFu) - float((mw >> (8 * (e & 3))) & 0xFFu);
-                    ax[kb][e] = __bf16(dv);
-                }
-            }
-            // ||x - mu||^2 of row (lane & 15): exact integers (< 2^24 after the sum over the lanes)
-            int xs = valid ? int(xx) - 2 * int(xm) + int(mm) : 0;
-            xs += __shfl_xor(xs, 16, 64);
-            xs += __shfl_xor(xs, 32, 64);
-            v4f acc[CB];
+                        for (int tt = 0; tt < 3; tt++) {
+                            p[tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ca[tt][cb][0], x0, v4i{0, 0, 0, 0}, 0, 0, 0);
+                            p[tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ca[tt][cb][1], x1, p[tt], 0, 0, 0);
+                        }
 #pragma unroll
-            for (int cb = 0; cb < CB; cb++) {
-                acc[cb] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int kb = 0; kb < 4; kb++) {
-                    acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[cb][kb], ax[kb], acc[cb], 0, 0, 0);
-                    acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[cb][kb], ax[kb], acc[cb], 0, 0, 0);
-                }
-            }
-            const uint32_t mask = filterCandidates<CB>(acc, float(xs), cn, e1m, e2m, lane, valid);
-            const bool tie = (mask & (mask - 1u)) != 0;  // only valid rows can have several candidates
-            if (grp == 0 && valid && !tie) {
-                const uint32_t best = __builtin_ctz(mask);
-                if (kLast) out[q] = t.node * B + best;
-                else sj[r] = uint8_t(best);
-            }
-            ties += tiePush(tq, nTie, lane, tie, q, kLast ? q : r, mask);
-            if (nTie > kTieCap - 16) flushTies();
+                        for (int i = 0; i < 4; i++)
+                            acc[cb][i] = fmaf(float((p[1][i] << 7) + p[2][i]), 1.0f / 16384.0f, float(p[0][i]));
+                    }
+                    const uint32_t mask = filterSplit<CB>(acc, float(xs), cn, e1m, e2m, lane, valid);
+                    const bool tie = (mask & (mask - 1u)) != 0;  // only valid rows can have several candidates
+                    if (grp == 0 && valid && !tie) {
+                        const uint32_t best = __builtin_ctz(mask);
+                        if (kLast) out[q] = t.node * B + best;
+                        else sj[r] = uint8_t(best);
+                    }
+                    ties += tiePush(tq, nTie, lane, tie, q, kLast ? q : r, mask);
+                },
+                std::make_integer_sequence<int, 4>{});
+            qv[0] = qv[1];
+            qv[1] = qv[2];
+            qv[2] = qv[3];
         }
-        if (nTie) flushTies();
+        if (nTie)  // near ties: the plain chain over the queued rows' candidates
+            resolveTies(
+                tq, nTie, lane, [&](uint32_t q) { return desc + uint64_t(q) * kDims; },
+                [&](uint32_t c) { return rows32 + (childBase + c) * kDims; },
+                [&](uint32_t aux, uint32_t best) {
+                    if (kLast) out[aux] = t.node * B + best;  // aux: the descriptor
+                    else sj[aux] = uint8_t(best);             // aux: the row in the item
+                });
         if (!kLast) itemCounts(lane, LG2B, it, t, sj, scAll[wave], out, C);
     }
     if (nearTies && ties && lane == 0) atomicAdd(nearTies, ties);  // ties: rows of this wave (uniform)
@@ -694,10 +777,10 @@ void launchLevel(const TreeView& t, uint64_t levelBase, const uint8_t* desc, uin
         k_levelI8<kIdentity, kLast><<<grid, 256, 0, s>>>(t.u8rows, t.normS, levelBase, desc, n, perm, items, nbItemsPtr,
                                                        itemsMax, out, C);
     } else if (t.b == 32) {
-        k_levelBf16<kIdentity, kLast, 32><<<grid, 256, 0, s>>>(t.f32rows, t.mu, t.hi, t.lo, t.cn2, t.e1, t.e2, levelBase, desc,
+        k_levelSplit<kIdentity, kLast, 32><<<grid, 256, 0, s>>>(t.f32rows, t.split, t.splitStride, t.cn2, t.e1, t.e2, levelBase, desc,
                                                              n, perm, items, nbItemsPtr, itemsMax, out, C, nearTies);
     } else {
-        k_levelBf16<kIdentity, kLast, 16><<<grid, 256, 0, s>>>(t.f32rows, t.mu, t.hi, t.lo, t.cn2, t.e1, t.e2, levelBase, desc,
+        k_levelSplit<kIdentity, kLast, 16><<<grid, 256, 0, s>>>(t.f32rows, t.split, t.splitStride, t.cn2, t.e1, t.e2, levelBase, desc,
                                                              n, perm, items, nbItemsPtr, itemsMax, out, C, nearTies);
     }
     RBOW_HIP(hipGetLastError());
@@ -721,6 +804,13 @@ void launchPrepF32Tree(const float* rows, uint64_t nbNodes, uint32_t b, uint8_t*
                        float* cn2, float* e1, float* e2, hipStream_t s, const uint32_t* active) {
     if (!nbNodes) return;
     k_prepF32<<<uint32_t(nbNodes), 128, 0, s>>>(rows, b, mu, hi, lo, cn2, e1, e2, active);
+    RBOW_HIP(hipGetLastError());
+}
+
+void launchPrepF32Split(const float* rows, uint64_t nbNodes, uint32_t b, int8_t* split, uint64_t splitStride,
+                        float* cn2, float* e1, float* e2, hipStream_t s) {
+    if (!nbNodes) return;
+    k_prepF32Split<<<nbNodes, 128, 0, s>>>(rows, b, split, splitStride, cn2, e1, e2);
     RBOW_HIP(hipGetLastError());
 }
 

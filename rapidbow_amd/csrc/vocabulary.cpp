@@ -34,9 +34,7 @@ void Vocabulary::allocate(uint32_t b_, uint32_t L_, uint32_t ct) {
         normS.resize(nc);
     } else {
         f32rows.resize(nc * kDims);
-        mu.resize(std::max<uint64_t>(nbNodes, 1) * kDims);
-        hi.resize(nc * kDims);
-        lo.resize(nc * kDims);
+        split.resize(3 * nc * kDims);
         cn2.resize(nc);
         e1.resize(nc);
         e2.resize(nc);
@@ -47,7 +45,7 @@ void Vocabulary::allocate(uint32_t b_, uint32_t L_, uint32_t ct) {
 
 void Vocabulary::prepare(hipStream_t s) {
     if (centroidType == 0) launchPrepU8Tree(u8rows.get(), nbChildren(), normU.get(), normS.get(), s);
-    else launchPrepF32Tree(f32rows.get(), nbNodes, b, mu.get(), hi.get(), lo.get(), cn2.get(), e1.get(), e2.get(), s);
+    else launchPrepF32Split(f32rows.get(), nbNodes, b, split.get(), nbChildren() * kDims, cn2.get(), e1.get(), e2.get(), s);
 }
 
 TreeView Vocabulary::view() const {
@@ -60,9 +58,8 @@ TreeView Vocabulary::view() const {
     t.normU = normU.get();
     t.normS = normS.get();
     t.f32rows = f32rows.get();
-    t.mu = mu.get();
-    t.hi = hi.get();
-    t.lo = lo.get();
+    t.split = split.get();
+    t.splitStride = nbChildren() * kDims;
     t.cn2 = cn2.get();
     t.e1 = e1.get();
     t.e2 = e2.get();

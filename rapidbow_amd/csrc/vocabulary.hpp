@@ -12,8 +12,7 @@ struct Vocabulary {
     DevBuf<uint8_t> u8rows;  // [node][b][128]
     DevBuf<int32_t> normU, normS;
     DevBuf<float> f32rows;   // [node][b][128]
-    DevBuf<uint8_t> mu;      // [node][128] centre of the node's children (float trees)
-    DevBuf<uint16_t> hi, lo;
+    DevBuf<int8_t> split;    // [3][node*b+j][128] int8 terms of c - 128 (float trees)
     DevBuf<float> cn2, e1, e2;
     std::vector<uint32_t> masks;   // valid-center masks, low bits (kmeans.cpp:189)
     std::vector<float> weights;    // [node][b] IDF weights (Vocabulary.cpp:103-108), host like the reference
@@ -21,7 +20,7 @@ struct Vocabulary {
     uint32_t lookupFusedMax = 0;   // tuned fused/bucketed crossover of the MFMA form (0: untuned)
 
     void allocate(uint32_t b_, uint32_t L_, uint32_t centroidType_);
-    // derived per-child data used by the kernels (norms, BF16 split, bounds)
+    // derived per-child data used by the kernels (norms, int8 split, bounds)
     void prepare(hipStream_t s);
     TreeView view() const;
     uint64_t nbChildren() const { return nbNodes * b; }

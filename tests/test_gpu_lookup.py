@@ -58,8 +58,9 @@ def test_lookup_empty_is_noop():
 def _f32_tree(rng, b, L, desc, near_ties=True):
     n = level_beg(b, L)
     # centroids near data points so that distances are realistic
+    # (means of uint8 data lie in [0, 255]; out-of-range centroids have a test of their own)
     pick = desc[rng.integers(0, desc.shape[0], n * b)].astype(np.float32).reshape(n, b, 128)
-    rows = pick + rng.normal(0, 3, pick.shape).astype(np.float32)
+    rows = np.clip(pick + rng.normal(0, 3, pick.shape).astype(np.float32), 0, 254)
     if near_ties:
         rows[:, 5] = rows[:, 2]                       # exact duplicate
         rows[:, 9] = rows[:, 8] + np.float32(1e-3)    # near duplicate: forces the FP32 re-score
@@ -134,6 +135,29 @@ def test_lookup_f32_all_rows_tie(b):
     mfma, ties = voc.lookup_host(q, rb.LOOKUP_MFMA, return_near_ties=True)
     np.testing.assert_array_equal(mfma, expect)
     assert ties >= 1000  # at least the root level ties for every row
+
+
+@pytest.mark.parametrize("b", [32, 16])
+def test_lookup_f32_out_of_range_centroids(b):
+    """Float trees deserialized from blobs may hold centroid values outside the
+    [0, 255] of any mean of uint8 data (negative, above 255, huge): such a node
+    has no int8 split, and every row at it is decided by the plain FP32 chain.
+    Nodes with in-range values in the same tree keep the MFMA filter.  Leaf ids
+    equal the plain kernel's and the oracle's."""
+    rng = np.random.default_rng(31)
+    desc = rb.synth_descriptors(6000, seed=32)
+    rows = _f32_tree(rng, b, 2, desc, near_ties=False)
+    n = rows.shape[0]
+    rows[0, 1, 7] = -3.5            # the root: one value below 0
+    rows[2, 4, 100] = 255.75        # a level-1 node: one value above 255
+    rows[5, 0, :] += 1e4            # another: a far child
+    voc = rb.deserialize_vocabulary(rbw2_blob(b, 2, 1, rows, np.full(n, 0xFFFFFFFF), np.ones((n, b))))
+    q = rb.synth_descriptors(8000, seed=33)
+    expect = oracle.lookup_f32(rows, b, 2, q)
+    np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_PLAIN), expect)
+    mfma, ties = voc.lookup_host(q, rb.LOOKUP_MFMA, return_near_ties=True)
+    np.testing.assert_array_equal(mfma, expect)
+    assert ties >= q.shape[0]  # every row is decided by the plain chain at the root
 
 
 def test_blob_roundtrip_u8():
