@@ -147,6 +147,7 @@ struct KmLevel {       // device state of one level's batched k-means
     uint32_t* nearTies = nullptr;  // optional: rows re-scored in FP32
     const uint32_t* updTasks = nullptr;  // tasks the MFMA update visits (null: all); the others run elsewhere
     uint32_t nbUpdTasks = 0;
+    uint32_t reverse = 0;  // MFMA assign: walk each wave's tiles backwards (odd passes)
 };
 
 // centers[task][j] = desc[seedIdx[task*b+j]] (invalid -> 0xFF / 255.0f)

@@ -505,39 +505,61 @@ __global__ __launch_bounds__(256) void k_assignU8Mfma(KmLevel k) {
     // per-task chunks on the host (partitionKmTiles): no wave waits for a
     // second round and the loads balance to within one tile
     const uint32_t w = blockIdx.x * kWaves + wave;
+    const uint32_t chBeg = w < k.nbWaves ? k.waveChunkBeg[w] : 0u;
     const uint32_t chEnd = w < k.nbWaves ? k.waveChunkBeg[w + 1] : 0u;
-    auto nextActive = [&](uint32_t c0) {
-        while (c0 < chEnd && !k.active[k.chunks[c0].task]) c0++;
-        return c0;
+    // Every other pass walks the wave's range backwards (k.reverse): a pass then
+    // starts on the tiles the previous pass read last, which the memory-side cache
+    // (MALL) still holds when a level's operand is about its size.  The sums are
+    // exact integers, so the order changes nothing else.
+    const bool rev = k.reverse != 0;
+    auto advance = [&](uint32_t c0) -> uint32_t {  // next active chunk in this pass's order (kIdxInvalid: none)
+        if (!rev) {
+            uint32_t c = c0 == kIdxInvalid ? chBeg : c0 + 1;
+            while (c < chEnd && !k.active[k.chunks[c].task]) c++;
+            return c < chEnd ? c : kIdxInvalid;
+        }
+        uint32_t c = c0 == kIdxInvalid ? chEnd : c0;
+        while (c > chBeg) {
+            c--;
+            if (k.active[k.chunks[c].task]) return c;
+        }
+        return kIdxInvalid;
     };
-    uint32_t ch = w < k.nbWaves ? nextActive(k.waveChunkBeg[w]) : 0u;
+    auto firstTile = [&](uint32_t c0) {  // a chunk's first tile in this pass's order (chunks are never empty)
+        const KmChunk& c = k.chunks[c0];
+        return rev ? c.tileEnd - 1 : c.tileBeg;
+    };
+    uint32_t ch = advance(kIdxInvalid);
     // the first tile of the wave's next chunk is requested while the last tile of
     // the current chunk computes
     v4i a[4][2];
     KmTile t{};
     uint32_t prev = 0;
-    if (ch < chEnd) {
-        const uint32_t tb = k.chunks[ch].tileBeg;
+    if (ch != kIdxInvalid) {
+        const uint32_t tb = firstTile(ch);
         t = k.tiles[tb];
         loadTileA(k.XA + uint64_t(tb) * 8192, lane, a);
         prev = lane < t.cnt ? k.assign[t.row + lane] : 0u;  // lane r: previous assignment of row r
     }
-    while (ch < chEnd) {
+    while (ch != kIdxInvalid) {
         const KmChunk c = k.chunks[ch];
-        const uint32_t chNext = nextActive(ch + 1);
+        const uint32_t chNext = advance(ch);
         CenterKeys ck;
         centerKeys(k.centersU8 + uint64_t(c.task) * 16 * kDims, lane, ck);
         v4i acc[8];
 #pragma unroll
         for (int db = 0; db < 8; db++) acc[db] = zero;
         uint32_t myCnt = 0, myChanged = 0;
-        for (uint32_t ti = c.tileBeg; ti < c.tileEnd; ti++) {
+        const uint32_t nT = c.tileEnd - c.tileBeg;
+        for (uint32_t step = 0; step < nT; step++) {
+            const uint32_t ti = rev ? c.tileEnd - 1 - step : c.tileBeg + step;
             // next tile's fragments and previous assignments (this chunk's, else the next
             // chunk's first), in flight while this one computes
             v4i an[4][2];
             KmTile tn = t;
             uint32_t prevN = 0;
-            const uint32_t tiN = ti + 1 < c.tileEnd ? ti + 1 : (chNext < chEnd ? k.chunks[chNext].tileBeg : kIdxInvalid);
+            const uint32_t tiN = step + 1 < nT ? (rev ? ti - 1 : ti + 1)
+                                                : (chNext != kIdxInvalid ? firstTile(chNext) : kIdxInvalid);
             if (tiN != kIdxInvalid) {
                 tn = k.tiles[tiN];
                 loadTileA(k.XA + uint64_t(tiN) * 8192, lane, an);

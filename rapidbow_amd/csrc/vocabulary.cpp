@@ -528,6 +528,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
                 launchPrepF32Tree(k.centersF32, nG, b, ws.fMu.get(), ws.fHi.get(), ws.fLo.get(), ws.fCn2.get(),
                                   ws.fE1.get(), ws.fE2.get(), s, k.active);
             }
+            k.reverse = it & 1u;  // alternate directions: a pass starts where the last one ended
             ranAssign[it] = launchKmAssign(k, kernel, s, ta);
             if (reduced) launchKmReduce(k, dRed, uint32_t(red.size()), s, timer(e + 2));
             if (poll) launchFillU32(ws.nbActive.get(), 0u, 1, s);
