@@ -1,31 +1,27 @@
-// bf16filter.hpp -- certified split-BF16 candidate filter with exact FP32
-// re-scoring, shared by the float-tree lookup (lookup.hip) and the float
-// k-means assign (kmeans.hip).
+// certfilter.hpp -- certified candidate filter with exact FP32 re-scoring,
+// shared by the float-tree lookup (lookup.hip) and the float k-means assign
+// (kmeans.hip).
 //
 // The defining distance between a uint8 descriptor x and a float centroid c is
 // the plain FP32 chain  acc = fma(x_k - c_k, x_k - c_k, acc),  k = 0..127
 // ascending (vocTreeLookUp.cu's float analogue, and the builder's float
-// k-means).  Per node (or k-means task) of b centroids, an integer centre mu
-// (0..255) is removed: x' = x - mu is exact in BF16, and c - mu is split as
-// hi + lo (+ residual) in BF16 (k_prepF32 in lookup.hip).  An MFMA pass gives
-//   s_j = ||c_j - mu||^2 - 2 (x'.hi_j + x'.lo_j)    (FP32 accumulation)
-// whose error against d(x, c_j) - ||x'||^2 is at most
-//   E_j = (E1 + 2^-16 (max(||x'||^2 + s_j, 0) + E1)) (1 + 2^-10),
-//   E1  = e1_j ||x'|| + e2_j,
-// e1/e2 from k_prepF32 (operand split, FP32 accumulation, stored norms) and the
-// 2^-16 term (>= 131 * 2^-24) the plain chain's own relative error.  A child
-// that can still be the argmin under these bounds is a candidate
-// (filterCandidates tests a superset with one threshold per row); one
-// candidate is the plain argmin, several ("near ties") are re-scored with the
-// plain chain in child order (resolveTies), so the result equals the plain
-// kernel's exactly (ties to the lowest).
+// k-means).  Each centroid is split once (k_prepF32Split in lookup.hip) as
+// c - 128 = A0 + 2^-7 A1 + 2^-14 A2 + r with int8 terms, so that I8 MFMAs on
+// x - 128 give exact int32 products; their FP32 combination t approximates
+// x''.(c - 128) (x'' = x - 128), and the score s = ||c - 128||^2 - 2t errs
+// against d(x, c) - ||x''||^2 by at most E1 = e1 ||x''|| + e2 (per-child e1, e2
+// from the split).  Adding 2^-16 (>= 131 * 2^-24) of the distance for the plain
+// chain's own relative error, a child that can still be the argmin under these
+// bounds is a candidate (filterSplit tests a superset with one threshold per
+// row); one candidate is the plain argmin, several ("near ties") are re-scored
+// with the plain chain in child order (resolveTies), so the result equals the
+// plain kernel's exactly (ties to the lowest).
 #pragma once
 #include "common.hpp"
 
 namespace rbow_amd {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr float kGammaPlain = 1.0f / 65536.0f;  // 2^-16 >= 131 * 2^-24: relative error of the plain FP32 chain
 
@@ -62,15 +58,19 @@ __device__ inline float plainF32Distance(const uint8_t* x, const float* c) {
 // four lanes l, l^16, l^32, l^48.  Row minima then take an in-lane minimum and
 // two permlane swaps instead of a 16-lane reduction per row.
 //
-// One threshold per row replaces the per-child bounds.  With E1max =
-// max_j e1_j * ||x'|| + max_j e2_j >= E1_j, m = min_j s_j and D = max(||x'||^2
+// Per child, the computed score s_j differs from the plain chain's distance
+// minus ||x''||^2 by at most E_j = (E1_j + 2^-16 (max(||x''||^2 + s_j, 0) + E1_j))
+// (1 + 2^-10): E1_j = e1_j ||x''|| + e2_j against the exact distance, and 2^-16
+// of the distance for the plain chain's own error.  One threshold per row
+// replaces the per-child test s_j - E_j <= min_i (s_i + E_i).  With E1max =
+// max_j e1_j ||x''|| + max_j e2_j >= E1_j, m = min_j s_j and D = max(||x''||^2
 // + m, 0), every child j with s_j - m > tau, tau = 2 sigma (A + g D) / (1 - g
 // sigma), A = E1max (1 + g), g = 2^-16, sigma = 1 + 2^-10, has
-// s_j - E_j > m + E_min (E_j as above grows by at most g sigma per unit of s_j,
-// and E_min <= (A + g D) sigma), so its plain distance exceeds that of the
-// child attaining m: it cannot be the argmin.  The candidates {s_j - m <= tau}
-// are therefore a superset of the per-child filter's, and the argmin rule
-// (a single candidate, or the plain chain over several) is unchanged.
+// s_j - E_j > m + E_min (E_j grows by at most g sigma per unit of s_j, and
+// E_min <= (A + g D) sigma), so its plain distance exceeds that of the child
+// attaining m: it cannot be the argmin.  The candidates {s_j - m <= tau} are
+// therefore a superset of the per-child test's, and the argmin rule (a single
+// candidate, or the plain chain over several) is unchanged.
 constexpr float kTauScale = 2.0f * (1.0f + 1.0f / 1024.0f) * (1.0f + 1.0f / 16384.0f);  // 2 sigma / (1 - g sigma), rounded up
 
 __device__ inline float minF(float a, float b) { return b < a ? b : a; }
@@ -88,39 +88,53 @@ __device__ inline uint32_t orAcrossGroups(uint32_t v) {
     const auto q = __builtin_amdgcn_permlane32_swap(v, v, false, false);
     return q[0] | q[1];
 }
-// One block of 16 rows in the transposed layout.  acc[cb][i]: dot part of
-// child cb*16 + 4*(lane>>4) + i with row (lane & 15); xn2: ||x'||^2 of that row
-// (an exact integer); cn[cb][i]: ||c'||^2 of the same children; e1max, e2max:
-// the node's largest e1 / e2; rowValid: the lane's row exists.  Returns the
-// candidate mask of the lane's row (bit j: child j), the same in the row's four
-// lanes.  A single bit is the plain-chain argmin; a row with several
-// candidates (a near tie) is resolved by resolveTies.
+// Sum over lanes l, l ^ 16, l ^ 32, l ^ 48, in every lane.
+__device__ inline int sumAcrossGroups(int v) {
+    const auto p = __builtin_amdgcn_permlane16_swap(uint32_t(v), uint32_t(v), false, false);
+    v = int(p[0]) + int(p[1]);
+    const auto q = __builtin_amdgcn_permlane32_swap(uint32_t(v), uint32_t(v), false, false);
+    return int(q[0]) + int(q[1]);
+}
+
+// Candidate mask of one row in the transposed layout.  t[cb][i]: the dot part of
+// child cb*16 + 4*(lane>>4) + i with row (lane & 15); xn2: ||x''||^2 of that row
+// (an exact integer); cn[cb][i]: ||c - 128||^2 of the same children; e1max,
+// e2max: the node's largest e1 / e2; rowValid: the lane's row exists.  With s_j
+// = fl(cn_j - 2 t_j) and m = min_j s_j, child j is kept when s_j <= fl(m + tau):
+// rounding is monotone and s_j is a float, so s_j - m <= tau implies it, and the
+// kept set contains the per-row test's above.  Returns the mask (bit j: child
+// j), the same in the row's four lanes; a single bit is the plain-chain argmin.
 template <int CB>
-__device__ __forceinline__ uint32_t filterCandidates(const v4f (&acc)[CB], float xn2, const float (&cn)[CB][4],
-                                                     float e1max, float e2max, uint32_t lane, bool rowValid) {
+__device__ __forceinline__ uint32_t filterSplit(const v4f (&t)[CB], float xn2, const float (&cn)[CB][4], float e1max,
+                                                float e2max, uint32_t lane, bool rowValid) {
     const uint32_t grp = lane >> 4;
-    float s[CB][4];
+    float sc[CB][4];
     float m = INFINITY;
 #pragma unroll
     for (int cb = 0; cb < CB; cb++)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            s[cb][i] = fmaf(-2.0f, acc[cb][i], cn[cb][i]);
-            m = minF(m, s[cb][i]);
+            sc[cb][i] = fmaf(-2.0f, t[cb][i], cn[cb][i]);
+            m = minF(m, sc[cb][i]);
         }
     m = minAcrossGroupsF(m);
     const float xn = __builtin_amdgcn_sqrtf(xn2) * (1.0f + 1.0f / 1048576.0f);  // v_sqrt_f32: <= 1 ulp
     const float e1v = fmaf(e1max, xn, e2max);
     const float D = fmaxf(xn2 + m, 0.0f);
-    const float tau = kTauScale * fmaf(kGammaPlain, D, e1v * (1.0f + kGammaPlain));
+    const float thr = m + kTauScale * fmaf(kGammaPlain, D, e1v * (1.0f + kGammaPlain));
     uint32_t mask = 0;
 #pragma unroll
     for (int cb = 0; cb < CB; cb++)
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-            if (s[cb][i] - m <= tau) mask |= 1u << (cb * 16 + 4 * grp + i);
+        for (int i = 0; i < 4; i++) mask |= sc[cb][i] <= thr ? 1u << (cb * 16 + 4 * grp + i) : 0u;
     mask = orAcrossGroups(mask);
     return rowValid ? mask : 1u;
+}
+
+// The dot part of the score from the three exact products (split terms A0, A1,
+// A2): t = fl(P0 + 2^-14 fl(2^7 P1 + P2)), 2^7 P1 + P2 exact in int32.
+__device__ inline float combineSplit(int p0, int p1, int p2) {
+    return fmaf(float((p1 << 7) + p2), 1.0f / 16384.0f, float(p0));
 }
 
 // Near ties of one wave, queued in LDS and resolved together so that the

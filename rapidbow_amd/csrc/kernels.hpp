@@ -33,15 +33,13 @@ struct TreeView {
                            // fused single-launch descent, larger ones the bucketed one (0: built-in default)
 };
 
-// Derived per-child data for nodes [0, nbNodes): uint8 trees' norms; for the
-// float k-means assign, the bf16 split of c - mu and bounds (nodes whose flag in
-// `active`, optional, is zero are skipped); for float-tree lookups, the int8
-// split of c - 128 and bounds (split: 3 terms of splitStride bytes).
+// Derived per-child data for nodes [0, nbNodes): uint8 trees' norms; for float
+// trees (lookups, and the float k-means assign every pass) the int8 split of
+// c - 128 and its bound coefficients (split: 3 terms of splitStride bytes;
+// nodes whose flag in `active`, optional, is zero are skipped).
 void launchPrepU8Tree(const uint8_t* rows, uint64_t nbChildren, int32_t* normU, int32_t* normS, hipStream_t s);
-void launchPrepF32Tree(const float* rows, uint64_t nbNodes, uint32_t b, uint8_t* mu, uint16_t* hi, uint16_t* lo,
-                       float* cn2, float* e1, float* e2, hipStream_t s, const uint32_t* active = nullptr);
 void launchPrepF32Split(const float* rows, uint64_t nbNodes, uint32_t b, int8_t* split, uint64_t splitStride,
-                        float* cn2, float* e1, float* e2, hipStream_t s);
+                        float* cn2, float* e1, float* e2, hipStream_t s, const uint32_t* active = nullptr);
 
 // Plain scalar per-descriptor descent (vocTreeLookUp.cu:48-160 semantics).
 void launchLookupPlain(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s);
@@ -136,11 +134,10 @@ struct KmLevel {       // device state of one level's batched k-means
     int32_t* partSums = nullptr;     // [part][16][128]  sums of (x - 128)
     uint32_t* partCnt = nullptr;     // [part][16]
     uint32_t* partChanged = nullptr; // [part]
-    // float centroids, BF16 MFMA filter: per task k_prepF32 of its centroids (integer
-    // centre [task][128], bf16 split [task][b][128], norms and bounds [task][b])
-    const uint8_t* fMu = nullptr;
-    const uint16_t* fHi = nullptr;
-    const uint16_t* fLo = nullptr;
+    // float centroids, MFMA filter: this pass's centers split into int8 terms by
+    // launchPrepF32Split, with their norms and bound coefficients [task*b+j]
+    const int8_t* fSplit = nullptr;  // int8 split of this pass's centers [3][task*b+j][128]
+    uint64_t fSplitStride = 0;
     const float* fCn2 = nullptr;
     const float* fE1 = nullptr;
     const float* fE2 = nullptr;

@@ -14,7 +14,7 @@
 #include <hip/hip_ext.h>
 #include <type_traits>
 
-#include "bf16filter.hpp"
+#include "certfilter.hpp"
 #include "kernels.hpp"
 
 namespace rbow_amd {
@@ -294,16 +294,22 @@ __device__ inline uint32_t tileEpilogue(const KmLevel& k, const KmTile& t, uint3
         atomicAdd(&acc.cnt[a], 1u);
     }
     const uint32_t nbChanged = __builtin_popcountll(__ballot(changed));
-    // sums: lane owns dims 2*lane and 2*lane+1
-    for (uint32_t r = 0; r < t.cnt; r++) {
-        const uint32_t ar = __builtin_amdgcn_readlane(a, r);
-        const uint32_t v = reinterpret_cast<const uint16_t*>(k.X + uint64_t(t.row + r) * kDims)[lane];
-        uint2* p = reinterpret_cast<uint2*>(acc.sums + ar * kDims + 2 * lane);
-        uint2 s = *p;
-        s.x += v & 0xFFu;
-        s.y += v >> 8;
-        *p = s;
+    // sums: lane owns dims 2*lane and 2*lane+1; 16 rows' bytes are loaded at once
+    // and added with LDS atomics (no read-modify-write chain between rows)
+    for (uint32_t r0 = 0; r0 < t.cnt; r0 += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (uint32_t j = 0; j < 16; j++)
+            v[j] = r0 + j < t.cnt ? reinterpret_cast<const uint16_t*>(k.X + uint64_t(t.row + r0 + j) * kDims)[lane] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < 16; j++) {
+            if (r0 + j >= t.cnt) break;
+            const uint32_t ar = __builtin_amdgcn_readlane(a, r0 + j);
+            atomicAdd(acc.sums + ar * kDims + 2 * lane, v[j] & 0xFFu);
+            atomicAdd(acc.sums + ar * kDims + 2 * lane + 1, v[j] >> 8);
+        }
     }
+    waveSync();  // the sums are complete before the next tile or a flush reads them
     return nbChanged;
 }
 
@@ -897,15 +903,18 @@ __global__ __launch_bounds__(256) void k_assignF32Plain(KmLevel k) {
     if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
 }
 
-// ---------------------------------------- assign: float, BF16 MFMA filter --
+// ------------------------------------------- assign: float, MFMA filter --
 // Same results as k_assignF32Plain (the plain FP32 chain, ties to the lowest
-// center): per 16-row block, x' = x - mu (exact in BF16) against the task's
-// split centroids (k_prepF32 of this pass's centers) by 2 x 4 x B/16 MFMAs with
-// the centroids as the A operand (each lane then owns one row), the certified
-// one-threshold-per-row candidate filter (filterCandidates), and the plain
-// chain for the tile's near ties, resolved together (resolveTies).  Work split and epilogue as the plain kernel: each wave a
-// contiguous tile range, the task operands reloaded when the task changes,
-// exact sums in LDS; the next block's rows are in flight while one computes.
+// center): per 16-row block, the rows (x - 128, the bytes with the top bit
+// flipped) against the task's centers split into three int8 terms (this pass's
+// launchPrepF32Split) by 3 x 2 x B/16 I8 MFMAs with the centers as the A
+// operand (each lane then owns one row), the terms combined exactly as the
+// float-tree lookup does (combineSplit), the certified one-threshold-per-row
+// candidate filter (filterSplit) and the plain chain for the tile's near ties,
+// resolved together (resolveTies).  Work split and epilogue as the plain kernel:
+// each wave a contiguous tile range, the task operands reloaded when the task
+// changes, exact sums in LDS; the next block's rows are in flight while one
+// computes.
 template <int B>
 __global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
     constexpr int CB = B / 16;
@@ -921,10 +930,7 @@ __global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
     uint32_t beg, end;
     waveRange(k.nbTiles, beg, end);
     uint32_t curTask = kIdxInvalid, changedLocal = 0, ties = 0, nTie = 0;
-    v8bf bh[CB][4], bl[CB][4];
     float cn[CB][4], e1m = 0.0f, e2m = 0.0f;
-    uint2 muw[4];     // the task's centre on this lane's dims
-    uint32_t mm = 0;  // ||mu||^2 over this lane's dims
     for (uint32_t ti = beg; ti < end; ti++) {
         const KmTile t = k.tiles[ti];
         if (!k.active[t.task]) continue;
@@ -933,21 +939,8 @@ __global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
             changedLocal = 0;
             curTask = t.task;
             const uint64_t childBase = uint64_t(curTask) * B;
-            mm = 0;
-#pragma unroll
-            for (int kb = 0; kb < 4; kb++) {
-                muw[kb] = *reinterpret_cast<const uint2*>(k.fMu + uint64_t(curTask) * kDims + kb * 32 + grp * 8);
-                mm = __builtin_amdgcn_udot4(muw[kb].x, muw[kb].x, mm, false);
-                mm = __builtin_amdgcn_udot4(muw[kb].y, muw[kb].y, mm, false);
-            }
 #pragma unroll
             for (int cb = 0; cb < CB; cb++) {
-                const uint64_t child = childBase + cb * 16 + col;
-#pragma unroll
-                for (int kb = 0; kb < 4; kb++) {
-                    bh[cb][kb] = *reinterpret_cast<const v8bf*>(k.fHi + child * kDims + kb * 32 + grp * 8);
-                    bl[cb][kb] = *reinterpret_cast<const v8bf*>(k.fLo + child * kDims + kb * 32 + grp * 8);
-                }
                 const float4 c4 = *reinterpret_cast<const float4*>(k.fCn2 + childBase + cb * 16 + 4 * grp);
                 cn[cb][0] = c4.x;
                 cn[cb][1] = c4.y;
@@ -962,59 +955,56 @@ __global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
                 e2m = fmaxf(e2m, __shfl_xor(e2m, d, 64));
             }
         }
+        // the task's split centers, reloaded per tile (from L2 after the first): live only
+        // while the tile's blocks compute, not during its near ties' resolution
+        v4i ca[3][CB][2];
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++) {
+            const int8_t* crow = k.fSplit + (uint64_t(curTask) * B + cb * 16 + col) * kDims + grp * 16;
+#pragma unroll
+            for (int tt = 0; tt < 3; tt++) {
+                ca[tt][cb][0] = *reinterpret_cast<const v4i*>(crow + tt * k.fSplitStride);
+                ca[tt][cb][1] = *reinterpret_cast<const v4i*>(crow + tt * k.fSplitStride + 64);
+            }
+        }
         const float* centers = k.centersF32 + uint64_t(curTask) * B * kDims;
-        auto loadRows = [&](uint32_t rb, uint2 (&w)[4]) {
+        auto loadRows = [&](uint32_t rb, v4i& x0, v4i& x1) {
+            x0 = v4i{0, 0, 0, 0};  // rows past the tile's end contribute nothing (and are not written)
+            x1 = v4i{0, 0, 0, 0};
             if (rb * 16 + col < t.cnt) {
                 const uint8_t* x = k.X + uint64_t(t.row + rb * 16 + col) * kDims;
-#pragma unroll
-                for (int kb = 0; kb < 4; kb++) w[kb] = *reinterpret_cast<const uint2*>(x + kb * 32 + grp * 8);
+                x0 = toSigned(ld16(x + grp * 16));
+                x1 = toSigned(ld16(x + 64 + grp * 16));
             }
         };
-        uint2 wn[4];
-        loadRows(0, wn);
+        v4i n0, n1;
+        loadRows(0, n0, n1);
         for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
             const uint32_t r = rb * 16 + col;
             const bool valid = r < t.cnt;
-            uint2 wc[4];
+            const v4i x0 = n0, x1 = n1;
+            if ((rb + 1) * 16 < t.cnt) loadRows(rb + 1, n0, n1);
+            // ||x - 128||^2 of row (lane & 15): an exact integer, summed over the row's four lanes
+            int xs = 0;
 #pragma unroll
-            for (int kb = 0; kb < 4; kb++) wc[kb] = wn[kb];
-            if ((rb + 1) * 16 < t.cnt) loadRows(rb + 1, wn);
-            v8bf ax[4];
-            uint32_t xx = 0, xm = 0;
-            if (valid) {
-#pragma unroll
-                for (int kb = 0; kb < 4; kb++) {
-                    const uint2 w = wc[kb];
-                    xx = __builtin_amdgcn_udot4(w.x, w.x, xx, false);
-                    xx = __builtin_amdgcn_udot4(w.y, w.y, xx, false);
-                    xm = __builtin_amdgcn_udot4(w.x, muw[kb].x, xm, false);
-                    xm = __builtin_amdgcn_udot4(w.y, muw[kb].y, xm, false);
-#pragma unroll
-                    for (int e = 0; e < 8; e++) {
-                        const uint32_t xw = e < 4 ? w.x : w.y, mw = e < 4 ? muw[kb].x : muw[kb].y;
-                        const float dv = float((xw >> (8 * (e & 3))) & 0xFFu) - float((mw >> (8 * (e & 3))) & 0xFFu);
-                        ax[kb][e] = __bf16(dv);
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int kb = 0; kb < 4; kb++) ax[kb] = v8bf{};
+            for (int e = 0; e < 4; e++) {
+                xs = __builtin_amdgcn_sdot4(x0[e], x0[e], xs, false);
+                xs = __builtin_amdgcn_sdot4(x1[e], x1[e], xs, false);
             }
-            // ||x - mu||^2 of row (lane & 15): exact integers
-            int xs = valid ? int(xx) - 2 * int(xm) + int(mm) : 0;
-            xs += __shfl_xor(xs, 16, 64);
-            xs += __shfl_xor(xs, 32, 64);
+            xs = sumAcrossGroups(xs);
             v4f d[CB];
 #pragma unroll
             for (int cb = 0; cb < CB; cb++) {
-                d[cb] = v4f{0.f, 0.f, 0.f, 0.f};
+                v4i p[3];
 #pragma unroll
-                for (int kb = 0; kb < 4; kb++) {
-                    d[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[cb][kb], ax[kb], d[cb], 0, 0, 0);
-                    d[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[cb][kb], ax[kb], d[cb], 0, 0, 0);
+                for (int tt = 0; tt < 3; tt++) {
+                    p[tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ca[tt][cb][0], x0, v4i{0, 0, 0, 0}, 0, 0, 0);
+                    p[tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ca[tt][cb][1], x1, p[tt], 0, 0, 0);
                 }
+#pragma unroll
+                for (int i = 0; i < 4; i++) d[cb][i] = combineSplit(p[0][i], p[1][i], p[2][i]);
             }
-            const uint32_t mask = filterCandidates<CB>(d, float(xs), cn, e1m, e2m, lane, valid);
+            const uint32_t mask = filterSplit<CB>(d, float(xs), cn, e1m, e2m, lane, valid);
             const bool tie = (mask & (mask - 1u)) != 0;  // only valid rows can have several candidates
             if (grp == 0 && !tie) sBest[wave][r] = uint8_t(__builtin_ctz(mask));
             ties += tiePush(tq, nTie, lane, tie, t.row + r, r, mask);  // a tile's <= 64 rows fit the queue
@@ -1535,7 +1525,7 @@ bool launchKmAssign(const KmLevel& k, int kernel, hipStream_t s, const KernelTim
         if (kernel == 1) launch(k_assignU8Plain, grid, 256, 0, s, t, k);
         else if (k.nbWaves) launch(k_assignU8Mfma, divUp(k.nbWaves, kWaves), 256, 0, s, t, k);
         else return false;
-    } else if (kernel == 0 && k.fHi) {
+    } else if (kernel == 0 && k.fSplit) {
         if (k.b == 32) launch(k_assignF32Mfma<32>, grid, 256, 0, s, t, k);
         else launch(k_assignF32Mfma<16>, grid, 256, 0, s, t, k);
     } else if (k.b == 32) {

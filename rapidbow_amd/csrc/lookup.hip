@@ -26,7 +26,7 @@
 #include <mutex>
 #include <utility>
 
-#include "bf16filter.hpp"
+#include "certfilter.hpp"
 #include "kernels.hpp"
 
 namespace rbow_amd {
@@ -55,12 +55,6 @@ struct LkItem {
 __device__ inline v4i ld16(const uint8_t* p) { return *reinterpret_cast<const v4i*>(p); }
 __device__ inline v4i toSigned(v4i v) { return v ^ v4i{int(0x80808080), int(0x80808080), int(0x80808080), int(0x80808080)}; }
 
-__device__ inline uint16_t bf16Rne(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
-}
-__device__ inline float bf16ToF(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
 
 // ------------------------------------------------------------ tree prep --
@@ -78,111 +72,65 @@ __global__ void k_prepU8(const uint8_t* rows, uint64_t nbChildren, int32_t* norm
     normS[i] = ss;
 }
 
-// One 128-thread block per node (thread k = dimension k):
-//   mu_k = rint(mean_j c_jk)                    per-node centring (integer, 0..255)
-//   c'   = c - mu (binary64), hi = bf16(c'), lo = bf16(c' - hi), r = c' - hi - lo
-//   cn2  = ||c'||^2
-//   e1   = (2||r|| + 2^-13 (||hi|| + ||lo||)) (1 + 2^-10)
-//            -> |MFMA score - (||c'||^2 - 2 x'.c')| <= e1 ||x'|| + e2 : operand split, worst-case
-//               sequential FP32 accumulation of the 256 products, rounding of the score
-//   e2   = 2^-22 ||c'||^2 + 1e-4
-__global__ __launch_bounds__(128) void k_prepF32(const float* rows, uint32_t b, uint8_t* mu, uint16_t* hi,
-                                                uint16_t* lo, float* cn2, float* e1, float* e2,
-                                                const uint32_t* active) {
-    __shared__ double red[4][2][32];
-    const uint64_t node = blockIdx.x;
-    if (active && !active[node]) return;  // a converged k-means task keeps its operands
-    const uint32_t k = threadIdx.x, lane = k & 63, wave = k >> 6;
-    const float* c = rows + node * b * kDims;
-    double m = 0;
-    for (uint32_t j = 0; j < b; j++) m += c[j * kDims + k];
-    const double muk = fmin(255.0, fmax(0.0, rint(m / b)));
-    mu[node * kDims + k] = uint8_t(muk);
-    for (uint32_t j = 0; j < b; j++) {
-        const uint64_t child = node * b + j;
-        const double cd = double(c[j * kDims + k]) - muk;
-        const uint16_t h = bf16Rne(float(cd));
-        const double hf = bf16ToF(h);
-        const uint16_t l = bf16Rne(float(cd - hf));
-        const double lf = bf16ToF(l);
-        const double r = cd - hf - lf;
-        hi[child * kDims + k] = h;
-        lo[child * kDims + k] = l;
-        double v[4] = {cd * cd, r * r, hf * hf, lf * lf};
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            for (int d = 32; d >= 1; d >>= 1) v[q] += __shfl_xor(v[q], d, 64);
-            if (lane == 0) red[q][wave][j] = v[q];
-        }
-    }
-    __syncthreads();
-    if (k < b) {
-        const uint64_t child = node * b + k;
-        const double cn = red[0][0][k] + red[0][1][k], rn = red[1][0][k] + red[1][1][k];
-        const double hn = red[2][0][k] + red[2][1][k], ln = red[3][0][k] + red[3][1][k];
-        const double slack = 1.0 + 1.0 / 1024;
-        cn2[child] = float(cn);
-        e1[child] = float((2.0 * sqrt(rn) + (sqrt(hn) + sqrt(ln)) / 8192.0) * slack);
-        e2[child] = float(cn / 4194304.0 * slack + 1e-4);
-    }
-}
-
-// One 128-thread block per node (thread k = dimension k), for float-tree lookups.
+// The int8 split of float centroids, one wave per child (lane l: dimensions 2l
+// and 2l + 1), for float-tree lookups and, every pass, the float k-means assign.
 // With c'' = c - 128 (exact), binary64 (every step exact):
 //   A0 = rint(c''), r0 = c'' - A0;  A1 = rint(2^7 r0), r1 = r0 - 2^-7 A1;
 //   A2 = rint(2^14 r1), r = r1 - 2^-14 A2;  c'' = Q + r,  Q = A0 + 2^-7 A1 + 2^-14 A2,
 //   A0 in [-128, 127] (c in [0, 255]), A1, A2 in [-64, 64], |r| <= 2^-15.
-// The level kernel's products P_t = x''.A_t (x'' = x - 128) are exact int32, so
-// its score s~ = fl(cn2 - 2 fl(P0 + 2^-7 fl(P1 + 2^-7 P2))) (fma each) against
-// s = ||c''||^2 - 2 x''.c'' = d(x, c) - ||x''||^2 errs by at most e1 ||x''|| + e2 with
+// The MFMA products P_t = x''.A_t (x'' = x - 128) are exact int32, so the score
+// s~ = fl(cn2 - 2 t~), t~ = fl(P0 + 2^-14 fl(2^7 P1 + P2)) (combineSplit), errs
+// against s = ||c''||^2 - 2 x''.c'' = d(x, c) - ||x''||^2 by at most e1 ||x''|| + e2 with
 //   e1 = (2||r|| + 4u (||Q|| + ||r0 - r||)) (1 + 2^-10),  e2 = 2u ||c''||^2 (1 + 2^-10),  u = 2^-24:
-//   2 x''.r (Cauchy-Schwarz); the two combining roundings, u|x''.Q| + u 2^-7 |x''.(A1 + 2^-7 A2)|
-//   with A1 + 2^-7 A2 = 2^7 (r0 - r), doubled by the factor -2; the final rounding
+//   2 x''.r (Cauchy-Schwarz); the conversion of 2^7 P1 + P2 and the fma,
+//   u 2^-14 |x''.(2^7 A1 + A2)| + u |x''.Q| <= u ||x''|| (||r0 - r|| + ||Q||) (as
+//   2^7 A1 + A2 = 2^14 (r0 - r)), doubled by the factor -2; the final rounding
 //   u |cn2 - 2 t~| and that of cn2 itself, u ||c''||^2 each (the slack covers the
 //   (1 + u) factors, the binary64 norms and the roundings of e1 and e2).
-// A node with a child value outside [0, 255] (or not finite) has no such split:
-// its terms are zero and e1 = inf, so every child stays a candidate and the plain
-// FP32 chain decides.
-__global__ __launch_bounds__(128) void k_prepF32Split(const float* __restrict__ rows, uint32_t b,
-                                                     int8_t* __restrict__ split, uint64_t stride,
+// A child with a value outside [0, 255] (or not finite) has no such split: its
+// terms are zero and e1 = inf, so the node's largest e1 keeps every child a
+// candidate and the plain FP32 chain decides.  `active` (optional, per node):
+// nodes whose flag is zero are skipped (converged k-means tasks keep their
+// operands).
+__global__ __launch_bounds__(256) void k_prepF32Split(const float* __restrict__ rows, uint64_t nbChildren,
+                                                     uint32_t lg2b, int8_t* __restrict__ split, uint64_t stride,
                                                      float* __restrict__ cn2, float* __restrict__ e1,
-                                                     float* __restrict__ e2) {
-    __shared__ double red[4][2][32];
-    const uint64_t node = blockIdx.x;
-    const uint32_t k = threadIdx.x, lane = k & 63, wave = k >> 6;
-    const float* c = rows + node * b * kDims;
-    bool bad = false;
-    for (uint32_t j = 0; j < b; j++) {
-        const float v = c[j * kDims + k];
-        bad |= !(v >= 0.0f && v <= 255.0f);
-    }
-    bad = __syncthreads_or(bad);
-    for (uint32_t j = 0; j < b; j++) {
-        const uint64_t child = node * b + j;
-        const double cd = bad ? 0.0 : double(c[j * kDims + k]) - 128.0;
+                                                     float* __restrict__ e2, const uint32_t* __restrict__ active) {
+    const uint64_t child = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (child >= nbChildren) return;
+    if (active && !active[child >> lg2b]) return;
+    const float2 c = reinterpret_cast<const float2*>(rows + child * kDims)[lane];
+    const bool bad = __ballot(!(c.x >= 0.0f && c.x <= 255.0f && c.y >= 0.0f && c.y <= 255.0f)) != 0;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    uint32_t planes[3] = {0u, 0u, 0u};  // the two dimensions' bytes of each term
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const double cd = bad ? 0.0 : double(e == 0 ? c.x : c.y) - 128.0;
         const double a0 = rint(cd), r0 = cd - a0;
         const double a1 = rint(r0 * 128.0), r1 = r0 - a1 / 128.0;
         const double a2 = rint(r1 * 16384.0), r = r1 - a2 / 16384.0;
-        split[child * kDims + k] = int8_t(a0);
-        split[stride + child * kDims + k] = int8_t(a1);
-        split[2 * stride + child * kDims + k] = int8_t(a2);
+        planes[0] |= (uint32_t(int(a0)) & 0xFFu) << (8 * e);
+        planes[1] |= (uint32_t(int(a1)) & 0xFFu) << (8 * e);
+        planes[2] |= (uint32_t(int(a2)) & 0xFFu) << (8 * e);
         const double qv = cd - r, dr = r0 - r;
-        double v[4] = {cd * cd, r * r, qv * qv, dr * dr};
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            for (int d = 32; d >= 1; d >>= 1) v[q] += __shfl_xor(v[q], d, 64);
-            if (lane == 0) red[q][wave][j] = v[q];
-        }
+        v[0] += cd * cd;
+        v[1] += r * r;
+        v[2] += qv * qv;
+        v[3] += dr * dr;
     }
-    __syncthreads();
-    if (k < b) {
-        const uint64_t child = node * b + k;
-        const double cn = red[0][0][k] + red[0][1][k], rn = red[1][0][k] + red[1][1][k];
-        const double qn = red[2][0][k] + red[2][1][k], dn = red[3][0][k] + red[3][1][k];
+#pragma unroll
+    for (int t = 0; t < 3; t++)
+        reinterpret_cast<uint16_t*>(split + t * stride + child * kDims)[lane] = uint16_t(planes[t]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] += __shfl_xor(v[q], d, 64);
+    if (lane == 0) {
         const double u = 1.0 / 16777216.0, slack = 1.0 + 1.0 / 1024;
-        cn2[child] = bad ? 0.0f : float(cn);
-        e1[child] = bad ? INFINITY : float((2.0 * sqrt(rn) + 4.0 * u * (sqrt(qn) + sqrt(dn))) * slack);
-        e2[child] = bad ? 0.0f : float(2.0 * u * cn * slack);
+        cn2[child] = bad ? 0.0f : float(v[0]);
+        e1[child] = bad ? INFINITY : float((2.0 * sqrt(v[1]) + 4.0 * u * (sqrt(v[2]) + sqrt(v[3]))) * slack);
+        e2[child] = bad ? 0.0f : float(2.0 * u * v[0] * slack);
     }
 }
 
@@ -599,45 +547,6 @@ __global__ __launch_bounds__(256) void k_lookupFusedI8(const uint8_t* __restrict
 // certified filter keeps every child that can be the argmin; near ties are
 // queued per wave and resolved with the plain FP32 chain at the item's end.
 //
-// Candidate test of one row: s_j = fl(cn2_j - 2 t_j), m = min_j s_j, and child j
-// is kept when s_j <= fl(m + tau) (tau as in filterCandidates).  Rounding is
-// monotone and s_j is a float, so s_j - m <= tau implies s_j <= fl(m + tau): the
-// kept set contains filterCandidates' set and the argmin rule is unchanged.
-template <int CB>
-__device__ __forceinline__ uint32_t filterSplit(const v4f (&t)[CB], float xn2, const float (&cn)[CB][4], float e1max,
-                                                float e2max, uint32_t lane, bool rowValid) {
-    const uint32_t grp = lane >> 4;
-    float sc[CB][4];
-    float m = INFINITY;
-#pragma unroll
-    for (int cb = 0; cb < CB; cb++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            sc[cb][i] = fmaf(-2.0f, t[cb][i], cn[cb][i]);
-            m = minF(m, sc[cb][i]);
-        }
-    m = minAcrossGroupsF(m);
-    const float xn = __builtin_amdgcn_sqrtf(xn2) * (1.0f + 1.0f / 1048576.0f);  // v_sqrt_f32: <= 1 ulp
-    const float e1v = fmaf(e1max, xn, e2max);
-    const float D = fmaxf(xn2 + m, 0.0f);
-    const float thr = m + kTauScale * fmaf(kGammaPlain, D, e1v * (1.0f + kGammaPlain));
-    uint32_t mask = 0;
-#pragma unroll
-    for (int cb = 0; cb < CB; cb++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) mask |= sc[cb][i] <= thr ? 1u << (cb * 16 + 4 * grp + i) : 0u;
-    mask = orAcrossGroups(mask);
-    return rowValid ? mask : 1u;
-}
-
-// Sum over lanes l, l ^ 16, l ^ 32, l ^ 48, in every lane (permlane swaps, as minAcrossGroupsF).
-__device__ inline int sumAcrossGroups(int v) {
-    const auto p = __builtin_amdgcn_permlane16_swap(uint32_t(v), uint32_t(v), false, false);
-    v = int(p[0]) + int(p[1]);
-    const auto q = __builtin_amdgcn_permlane32_swap(uint32_t(v), uint32_t(v), false, false);
-    return int(q[0]) + int(q[1]);
-}
-
 template <bool kIdentity, bool kLast, int B>
 __global__ __launch_bounds__(256) void k_levelSplit(
     const float* __restrict__ rows32, const int8_t* __restrict__ split, uint64_t stride,
@@ -733,8 +642,7 @@ __global__ __launch_bounds__(256) void k_levelSplit(
                             p[tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ca[tt][cb][1], x1, p[tt], 0, 0, 0);
                         }
 #pragma unroll
-                        for (int i = 0; i < 4; i++)
-                            acc[cb][i] = fmaf(float((p[1][i] << 7) + p[2][i]), 1.0f / 16384.0f, float(p[0][i]));
+                        for (int i = 0; i < 4; i++) acc[cb][i] = combineSplit(p[0][i], p[1][i], p[2][i]);
                     }
                     const uint32_t mask = filterSplit<CB>(acc, float(xs), cn, e1m, e2m, lane, valid);
                     const bool tie = (mask & (mask - 1u)) != 0;  // only valid rows can have several candidates
@@ -800,17 +708,12 @@ void launchPrepU8Tree(const uint8_t* rows, uint64_t nbChildren, int32_t* normU, 
     RBOW_HIP(hipGetLastError());
 }
 
-void launchPrepF32Tree(const float* rows, uint64_t nbNodes, uint32_t b, uint8_t* mu, uint16_t* hi, uint16_t* lo,
-                       float* cn2, float* e1, float* e2, hipStream_t s, const uint32_t* active) {
-    if (!nbNodes) return;
-    k_prepF32<<<uint32_t(nbNodes), 128, 0, s>>>(rows, b, mu, hi, lo, cn2, e1, e2, active);
-    RBOW_HIP(hipGetLastError());
-}
-
 void launchPrepF32Split(const float* rows, uint64_t nbNodes, uint32_t b, int8_t* split, uint64_t splitStride,
-                        float* cn2, float* e1, float* e2, hipStream_t s) {
+                        float* cn2, float* e1, float* e2, hipStream_t s, const uint32_t* active) {
     if (!nbNodes) return;
-    k_prepF32Split<<<nbNodes, 128, 0, s>>>(rows, b, split, splitStride, cn2, e1, e2);
+    const uint64_t nbChildren = nbNodes * b;
+    k_prepF32Split<<<uint32_t((nbChildren + 3) / 4), 256, 0, s>>>(rows, nbChildren, b == 32 ? 5u : 4u, split,
+                                                                  splitStride, cn2, e1, e2, active);
     RBOW_HIP(hipGetLastError());
 }
 

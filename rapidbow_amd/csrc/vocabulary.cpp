@@ -177,11 +177,11 @@ struct BuildWs {
     DevBuf<uint32_t> order, orderNext, counts, changed, active, nbActive, seed, partCnt, partChanged, out;
     DevBuf<uint32_t> planTaskBeg, planTaskCnt, planTileBeg, planNbTiles;
     DevBuf<KmTile> planTiles;
-    DevBuf<uint8_t> X, assign, sortWs, fMu, upDev;
+    DevBuf<uint8_t> X, assign, sortWs, upDev;
+    DevBuf<int8_t> fSplit;
     DevBuf<long long> sums;
     DevBuf<int8_t> XA;
     DevBuf<int32_t> partSums;
-    DevBuf<uint16_t> fHi, fLo;
     DevBuf<float> fCn2, fE1, fE2;
     PinnedBuf hOut, upHost;
 };
@@ -314,11 +314,9 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     ws.partCnt.reserve(chunksMax * 16);
     ws.partChanged.reserve(chunksMax);
     ws.sortWs.reserve(regroupWorkspaceBytes(tilesMax, b));
-    // float centroids with kernel 0: the BF16 MFMA filter's per-task operands, redone every pass
+    // float centroids with kernel 0: the MFMA filter's int8 split of the centers, redone every pass
     const bool f32Mfma = ct == 1 && kernel == 0;
-    ws.fMu.reserve(f32Mfma ? nGmax * kDims : 0);
-    ws.fHi.reserve(f32Mfma ? nGmax * b * kDims : 0);
-    ws.fLo.reserve(f32Mfma ? nGmax * b * kDims : 0);
+    ws.fSplit.reserve(f32Mfma ? 3 * nGmax * b * kDims : 0);
     ws.fCn2.reserve(f32Mfma ? nGmax * b : 0);
     ws.fE1.reserve(f32Mfma ? nGmax * b : 0);
     ws.fE2.reserve(f32Mfma ? nGmax * b : 0);
@@ -396,9 +394,8 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         uint32_t* dMasks = k.passes + nG;
         k.nbActive = ws.nbActive.get();
         if (f32Mfma) {
-            k.fMu = ws.fMu.get();
-            k.fHi = ws.fHi.get();
-            k.fLo = ws.fLo.get();
+            k.fSplit = ws.fSplit.get();
+            k.fSplitStride = uint64_t(nG) * b * kDims;
             k.fCn2 = ws.fCn2.get();
             k.fE1 = ws.fE1.get();
             k.fE2 = ws.fE2.get();
@@ -525,8 +522,8 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             if (f32Mfma) {  // the filter's operands of this pass's centers, timed with the assign
                 if (timing) RBOW_HIP(hipEventRecord(ta.start, s));
                 ta.start = nullptr;
-                launchPrepF32Tree(k.centersF32, nG, b, ws.fMu.get(), ws.fHi.get(), ws.fLo.get(), ws.fCn2.get(),
-                                  ws.fE1.get(), ws.fE2.get(), s, k.active);
+                launchPrepF32Split(k.centersF32, nG, b, ws.fSplit.get(), uint64_t(nG) * b * kDims, ws.fCn2.get(),
+                                   ws.fE1.get(), ws.fE2.get(), s, k.active);
             }
             k.reverse = it & 1u;  // alternate directions: a pass starts where the last one ended
             ranAssign[it] = launchKmAssign(k, kernel, s, ta);
