@@ -282,11 +282,23 @@ __global__ __launch_bounds__(256) void k_scatter(const LkItem* __restrict__ item
     const uint32_t nbItems = kIdentity ? nbItemsImplicit : *nbItemsPtr;
     for (uint32_t it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nbItems; it += gridDim.x * 4) {
         const LkItem t = itemAt<kIdentity>(it, n, items);
-        for (uint32_t r = lane; r < t.cnt; r += 64) {
-            const uint32_t p = t.start + r, c = code[p];
-            const uint32_t dst = O[uint64_t(t.itemBase) * b + (c & 0xFFu) * t.nItems + (it - t.itemBase)] + (c >> 8);
-            permNext[dst] = kIdentity ? p : perm[p];
+        // the item's <= 256 rows, four per lane, every load of a step in flight together
+        static_assert(kTQ == 256, "an item is four groups of 64 rows");
+        uint32_t c[4], src[4], dst[4];
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const uint32_t r = g * 64 + lane, p = t.start + r;
+            c[g] = r < t.cnt ? code[p] : 0u;
+            src[g] = r < t.cnt ? (kIdentity ? p : perm[p]) : 0u;
         }
+#pragma unroll
+        for (int g = 0; g < 4; g++)
+            dst[g] = g * 64 + lane < t.cnt
+                         ? O[uint64_t(t.itemBase) * b + (c[g] & 0xFFu) * t.nItems + (it - t.itemBase)] + (c[g] >> 8)
+                         : 0u;
+#pragma unroll
+        for (int g = 0; g < 4; g++)
+            if (g * 64 + lane < t.cnt) permNext[dst[g]] = src[g];
     }
 }
 
