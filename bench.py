@@ -564,9 +564,13 @@ def main():
     # autotuned AUTO crossovers (plain below, bucketed MFMA from there on)
     q_tune = syn.descriptors(1 << 20, seed=SEED_QUERY + 20)
     cross_u8 = v1.tune_lookup(q_tune, 1 << 20)
+    fused_u8 = v1.lookup_fused_max
     cross_f32 = v32.tune_lookup(q_tune, 1 << 20)
     del q_tune
-    log(f"bench: tuned AUTO crossovers: b16 L3 uint8 {cross_u8}, b32 L4 float {cross_f32}")
+    log(f"bench: tuned AUTO crossovers: b16 L3 uint8 {cross_u8} (fused up to {fused_u8}), b32 L4 float {cross_f32}")
+    # build autotuner: the persistent on-chip Lloyd loop's size limit for this workload
+    tuned_tiles, tune_ms = rb.tune_build_options(d, N, 1000, 4, max_iters=10, seed_mode=1, kernel=0)
+    log(f"bench: build autotuner: persist_max_tiles {tuned_tiles}, build ms per candidate {tune_ms}")
 
     line = {
         "metric": METRIC, "value": round(tops, 3), "unit": "TOPS", "n_gpus": 1, "steps": args.steps,
@@ -599,7 +603,10 @@ def main():
                                  "hbm_frac_mfma": round(r_m * LOOKUP_BYTES / 1e9 / HBM_PEAK_GBS, 4)},
         "lookup_b16L3_u8_50k": {"mfma_desc_per_s": round(r1_m, 1), "plain_desc_per_s": round(r1_p, 1),
                                 "identical_leaf_ids": id1},
-        "lookup_auto_crossover_tuned": {"b16L3_u8": cross_u8, "b32L4_f32": cross_f32},
+        "lookup_auto_crossover_tuned": {"b16L3_u8": cross_u8, "b32L4_f32": cross_f32,
+                                        "b16L3_u8_fused_max": fused_u8},
+        "build_autotuned": {"persist_max_tiles": tuned_tiles,
+                            "build_ms_per_candidate": {str(c): round(m, 3) for c, m in tune_ms.items()}},
     }
     threads = cpu_threads()
     if not args.skip_retrieval:

@@ -84,6 +84,17 @@ rbow_status rbow_build_vocabulary(rbow_descriptor_attributes attr, uint32_t bran
 rbow_status rbow_build_vocabulary_ex(rbow_descriptor_attributes attr, uint32_t branch_factor, const void* dev_desc,
                                      uint32_t nb_desc, uint32_t nb_doc, uint32_t nb_levels,
                                      const rbow_build_options* opts, rbow_stream stream, rbow_vocabulary** out);
+/* Autotuner (no reference counterpart): builds the tree of these arguments with
+ * each size limit of the persistent on-chip Lloyd loop, persist_max_tiles = 0,
+ * 4, 8, 12 and 16 tiles of 64 points (the other options as in *opts), and
+ * writes the fastest into opts->persist_max_tiles (the better of two builds
+ * each; out_ms, optional, receives the five times in that order).  Every
+ * candidate must build the same tree (RBOW_ERR_HIP otherwise).  Acts on uint8
+ * centroids with the MFMA kernels only; other options are left unchanged.
+ * Blocks. */
+rbow_status rbow_tune_build_options(rbow_descriptor_attributes attr, uint32_t branch_factor, const void* dev_desc,
+                                    uint32_t nb_desc, uint32_t nb_doc, uint32_t nb_levels, rbow_build_options* opts,
+                                    float* out_ms, rbow_stream stream);
 /* No counterpart in the reference.  A completed build leaves its device and
  * pinned work buffers (about 290 MB + 150 B per descriptor for b16 L4 MFMA
  * builds) with the current device for the next build, which then allocates only
@@ -129,11 +140,14 @@ rbow_status rbow_voc_lookup_ex(const rbow_vocabulary* voc, const void* dev_desc,
  * that overlap in time on one GPU race: issue them on one stream. */
 rbow_status rbow_voc_lookup_noalloc(const rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc,
                                     uint32_t* dev_leaf, int32_t kernel, uint32_t* dev_near_ties, rbow_stream stream);
-/* Autotuner (no reference counterpart): times the plain and MFMA lookups on
- * prefixes of dev_desc (powers of two from 4096 up to nb_desc) and sets the
- * vocabulary's AUTO crossover to the smallest size at which MFMA is at least
- * as fast (nb_desc + 1 if never).  Blocks; does not change any leaf id.
- * rbow_voc_set_lookup_crossover sets it directly (0 restores the default). */
+/* Autotuner (no reference counterpart), on prefixes of dev_desc (powers of two
+ * from 4096 up to nb_desc): on uint8 b16 trees it first sets the MFMA form's
+ * fused/bucketed boundary (rbow_voc_lookup_fused_max) to the largest size at
+ * which the fused single-launch descent is at least as fast as the bucketed
+ * one; then it sets the vocabulary's AUTO crossover to the smallest size at
+ * which the MFMA form is at least as fast as the plain kernel (nb_desc + 1 if
+ * never).  Blocks; does not change any leaf id.  rbow_voc_set_lookup_crossover
+ * and rbow_voc_set_lookup_fused_max set them directly (0 restores the defaults). */
 rbow_status rbow_voc_tune_lookup(rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc,
                                  uint32_t* out_crossover, rbow_stream stream);
 rbow_status rbow_voc_set_lookup_crossover(rbow_vocabulary* voc, uint32_t crossover);

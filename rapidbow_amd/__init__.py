@@ -68,6 +68,8 @@ _sig("rbow_build_vocabulary_ex", _i32,
      [DescriptorAttributes, _u32, _vp, _u32, _u32, _u32, ctypes.POINTER(BuildOptions), _vp, ctypes.POINTER(_vp)])
 _sig("rbow_voc_destroy", None, [_vp])
 _sig("rbow_release_build_workspaces", _i32, [])
+_sig("rbow_tune_build_options", _i32,
+     [DescriptorAttributes, _u32, _vp, _u32, _u32, _u32, ctypes.POINTER(BuildOptions), ctypes.POINTER(ctypes.c_float), _vp])
 _sig("rbow_voc_get_attributes", _i32, [_vp, ctypes.POINTER(DescriptorAttributes)])
 _sig("rbow_voc_branch_factor", _u32, [_vp])
 _sig("rbow_voc_nb_levels", _u32, [_vp])
@@ -269,6 +271,16 @@ def build_vocabulary(dev_desc, nb_desc, nb_doc, nb_levels, branch_factor=16, str
     check(lib.rbow_build_vocabulary_ex(attr, branch_factor, _ptr(dev_desc), nb_desc, nb_doc, nb_levels,
                                        ctypes.byref(o), _stream(stream), ctypes.byref(h)))
     return Vocabulary(h.value)
+
+
+def tune_build_options(dev_desc, nb_desc, nb_doc, nb_levels, branch_factor=16, stream=None, attr=SIFT_ATTR, **opts):
+    """Build autotuner: the fastest persist_max_tiles for these data (see rbow_tune_build_options).
+    Returns (tuned persist_max_tiles, {candidate: build ms})."""
+    o = build_options(**opts)
+    ms = (ctypes.c_float * 5)()
+    check(lib.rbow_tune_build_options(attr, branch_factor, _ptr(dev_desc), nb_desc, nb_doc, nb_levels, ctypes.byref(o),
+                                      ms, _stream(stream)))
+    return o.persist_max_tiles, {c: float(m) for c, m in zip((0, 4, 8, 12, 16), ms)}
 
 
 def release_build_workspaces():

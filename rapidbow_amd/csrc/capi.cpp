@@ -211,6 +211,47 @@ rbow_status rbow_build_vocabulary_ex(rbow_descriptor_attributes attr, uint32_t b
     });
 }
 
+rbow_status rbow_tune_build_options(rbow_descriptor_attributes attr, uint32_t branch_factor, const void* dev_desc,
+                                    uint32_t nb_desc, uint32_t nb_doc, uint32_t nb_levels, rbow_build_options* opts,
+                                    float* out_ms, rbow_stream stream) {
+    return guarded([&] {
+        RBOW_REQUIRE(opts != nullptr, RBOW_ERR_INVALID_ARG, "null options");
+        static const uint32_t kCandidates[5] = {0u, 4u, 8u, 12u, 16u};
+        rbow_build_options o = *opts;
+        o.timing = 0;
+        if (out_ms)
+            for (int i = 0; i < 5; i++) out_ms[i] = 0.0f;
+        if (o.centroid_type != 0 || o.kernel != 0) return;  // the option acts on uint8 MFMA builds only
+        std::vector<uint8_t> first;
+        double bestMs = 0;
+        uint32_t best = o.persist_max_tiles;
+        for (int i = 0; i < 5; i++) {
+            o.persist_max_tiles = kCandidates[i];
+            double ms[2];
+            for (int rep = 0; rep < 2; rep++) {
+                rbow_vocabulary* v = nullptr;
+                const rbow_status st = rbow_build_vocabulary_ex(attr, branch_factor, dev_desc, nb_desc, nb_doc,
+                                                                nb_levels, &o, stream, &v);
+                if (st != RBOW_OK) throw Error(st, rbow_last_error());
+                std::unique_ptr<rbow_vocabulary> h(v);
+                ms[rep] = h->stats.totalMs;
+                if (rep == 0) {
+                    std::vector<uint8_t> blob = serialize(*h->v);
+                    if (first.empty()) first = std::move(blob);
+                    else RBOW_REQUIRE(blob == first, RBOW_ERR_HIP, "tuning candidates built different trees");
+                }
+            }
+            const double m = std::min(ms[0], ms[1]);
+            if (out_ms) out_ms[i] = float(m);
+            if (i == 0 || m < bestMs) {
+                bestMs = m;
+                best = kCandidates[i];
+            }
+        }
+        opts->persist_max_tiles = best;
+    });
+}
+
 rbow_status rbow_build_vocabulary(rbow_descriptor_attributes attr, uint32_t branch_factor, const void* dev_desc,
                                   uint32_t nb_desc, uint32_t nb_doc, uint32_t nb_levels, rbow_stream stream,
                                   rbow_vocabulary** out) {
@@ -267,10 +308,11 @@ rbow_status rbow_voc_tune_lookup(rbow_vocabulary* voc, const void* dev_desc, uin
         RBOW_REQUIRE(dev_desc != nullptr || nb_desc == 0, RBOW_ERR_INVALID_ARG, "null descriptors");
         if (nb_desc) checkAligned(dev_desc, "descriptors");
         DevBuf<uint32_t> leaf(std::max(1u, nb_desc));
-        const uint32_t c = tuneLookupCrossover(voc->v->view(), static_cast<const uint8_t*>(dev_desc), nb_desc,
-                                               leaf.get(), hs(stream));
-        voc->v->lookupAutoMin = c;
-        if (out_crossover) *out_crossover = c;
+        const LookupTuning r = tuneLookup(voc->v->view(), static_cast<const uint8_t*>(dev_desc), nb_desc, leaf.get(),
+                                          hs(stream));
+        voc->v->lookupAutoMin = r.crossover;
+        if (r.fusedMax) voc->v->lookupFusedMax = r.fusedMax;
+        if (out_crossover) *out_crossover = r.crossover;
     });
 }
 

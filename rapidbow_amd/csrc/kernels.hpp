@@ -43,7 +43,8 @@ void launchPrepF32Split(const float* rows, uint64_t nbNodes, uint32_t b, int8_t*
 
 // Plain scalar per-descriptor descent (vocTreeLookUp.cu:48-160 semantics).
 void launchLookupPlain(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s);
-// Level-synchronous bucketed MFMA descent (I8 for uint8 trees, split-BF16 with
+// MFMA-form descent: fused single launch (uint8 b16 trees, small batches) or
+// level-synchronous bucketed (I8 for uint8 trees, I8 on the int8 split with
 // certified FP32 re-score for float trees).  Allocates its workspace stream-ordered.
 void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
                       hipStream_t s);
@@ -72,10 +73,15 @@ bool lookupUsesPlain(const TreeView& t, uint32_t n, int kernel);
 size_t lookupWorkspaceBytes(const TreeView& t, uint32_t n, bool plain, hipStream_t s);
 void launchLookupInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, uint32_t* nearTies,
                       bool plain, void* ws, size_t wsBytes, hipStream_t s);
-// Autotuner: times both lookup kernels on prefixes of desc (powers of two from
-// 4096 up to n) and returns the smallest size at which the MFMA form is at
-// least as fast (n + 1 if it never is).  Blocks; uses `leaf` (n words) as output.
-uint32_t tuneLookupCrossover(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s);
+// Autotuner: on prefixes of desc (powers of two from 4096 up to n), first (uint8
+// b16 trees) the largest size at which the fused MFMA descent is at least as fast
+// as the bucketed one, then, with that MFMA form, the smallest size at which it
+// is at least as fast as the plain kernel.  Blocks; uses `leaf` (n words) as output.
+struct LookupTuning {
+    uint32_t crossover;  // AUTO: the MFMA form from this many descriptors on (n + 1: never)
+    uint32_t fusedMax;   // uint8 b16 trees: the fused descent up to this many descriptors (0: not applicable)
+};
+LookupTuning tuneLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s);
 
 // ----------------------------------------------------------------- build --
 // Gather rows: out[p] = desc[idx[p]] (idx == kIdxInvalid -> 0xFF bytes).

@@ -993,18 +993,21 @@ void launchLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* 
     else launchLookupMfma(t, desc, n, leaf, nearTies, s);
 }
 
-uint32_t tuneLookupCrossover(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s) {
+LookupTuning tuneLookup(const TreeView& t0, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s) {
     hipEvent_t e0, e1;
     RBOW_HIP(hipEventCreate(&e0));
     RBOW_HIP(hipEventCreate(&e1));
-    auto timeOne = [&](uint32_t m, int kernel) {  // median of 3 timed runs after one warm run
+    // median of 3 timed runs after one warm run
+    auto timeOne = [&](const TreeView& t, uint32_t m, int kernel) {
         float ms[3];
-        if (kernel == RBOW_LOOKUP_PLAIN) launchLookupPlain(t, desc, m, leaf, s);
-        else launchLookupMfma(t, desc, m, leaf, nullptr, s);
-        for (int r = 0; r < 3; r++) {
-            RBOW_HIP(hipEventRecord(e0, s));
+        auto run = [&]() {
             if (kernel == RBOW_LOOKUP_PLAIN) launchLookupPlain(t, desc, m, leaf, s);
             else launchLookupMfma(t, desc, m, leaf, nullptr, s);
+        };
+        run();
+        for (int r = 0; r < 3; r++) {
+            RBOW_HIP(hipEventRecord(e0, s));
+            run();
             RBOW_HIP(hipEventRecord(e1, s));
             RBOW_HIP(hipEventSynchronize(e1));
             RBOW_HIP(hipEventElapsedTime(&ms[r], e0, e1));
@@ -1012,12 +1015,28 @@ uint32_t tuneLookupCrossover(const TreeView& t, const uint8_t* desc, uint32_t n,
         std::sort(ms, ms + 3);
         return ms[1];
     };
-    uint32_t cross = n + 1;
+    LookupTuning r{n + 1, 0u};
     try {
+        TreeView t = t0;
+        if (t.L >= 1 && lookupFusedDefault(t)) {
+            // the MFMA form's fused/bucketed boundary: the largest size (powers of two from
+            // 4,096) at which the fused descent is still at least as fast as the bucketed one
+            TreeView fused = t, bucketed = t;
+            fused.fusedMax = 0xFFFFFFFFu;
+            bucketed.fusedMax = 1u;
+            r.fusedMax = 4096u;
+            for (uint64_t m = 4096; m <= n; m *= 2) {
+                if (timeOne(fused, uint32_t(m), RBOW_LOOKUP_MFMA) > timeOne(bucketed, uint32_t(m), RBOW_LOOKUP_MFMA))
+                    break;
+                r.fusedMax = uint32_t(m);
+            }
+            t.fusedMax = r.fusedMax;
+        }
+        // the AUTO crossover with that MFMA form: the smallest size at which it is at least as fast as plain
         for (uint64_t m = 4096; m <= n; m *= 2) {
             if (t.L == 0) break;
-            if (timeOne(uint32_t(m), RBOW_LOOKUP_MFMA) <= timeOne(uint32_t(m), RBOW_LOOKUP_PLAIN)) {
-                cross = uint32_t(m);
+            if (timeOne(t, uint32_t(m), RBOW_LOOKUP_MFMA) <= timeOne(t, uint32_t(m), RBOW_LOOKUP_PLAIN)) {
+                r.crossover = uint32_t(m);
                 break;
             }
         }
@@ -1028,7 +1047,7 @@ uint32_t tuneLookupCrossover(const TreeView& t, const uint8_t* desc, uint32_t n,
     }
     RBOW_HIP(hipEventDestroy(e0));
     RBOW_HIP(hipEventDestroy(e1));
-    return cross;
+    return r;
 }
 
 }  // namespace rbow_amd

@@ -138,6 +138,24 @@ def test_build_timing_option():
         assert st[f] == s0[f], f
 
 
+def test_build_autotuner():
+    """rbow_tune_build_options times the persistent-loop size limits on the
+    caller's data and keeps the fastest; every candidate builds the same tree,
+    which equals the oracle's."""
+    desc = rb.synth_descriptors(150000, seed=41, nb_centres=40)
+    d = rb.DeviceArray.from_numpy(desc)
+    best, ms = rb.tune_build_options(d, desc.shape[0], 300, 4, max_iters=6, seed_mode=1)
+    assert best in (0, 4, 8, 12, 16) and all(m > 0 for m in ms.values())
+    assert ms[best] == min(ms.values())
+    tuned = rb.build_vocabulary(d, desc.shape[0], 300, 4, max_iters=6, seed_mode=1, persist_max_tiles=best)
+    res = oracle.build(desc, 300, 4, max_iters=6, seed_mode=1)
+    _check_u8(tuned, res, 4)
+    # float centroids: the option has no effect and is left as given
+    best_f, ms_f = rb.tune_build_options(d, 20000, 50, 2, branch_factor=32, centroid_type=1, max_iters=3,
+                                         persist_max_tiles=7)
+    assert best_f == 7 and all(m == 0 for m in ms_f.values())
+
+
 def test_build_workspace_reuse():
     """Builds one after another reuse the work buffers a completed build leaves
     with the device (larger than the next build needs, holding the last build's

@@ -181,7 +181,10 @@ def test_lookup_autotuner_keeps_leaf_ids():
     d = rb.DeviceArray.from_numpy(q)
     c = voc.tune_lookup(d, q.shape[0])
     assert c == voc.lookup_crossover and (c == 70001 or (c >= 4096 and c & (c - 1) == 0))
+    f = voc.lookup_fused_max  # the tuned fused/bucketed boundary of this b16 uint8 tree
+    assert f >= 4096 and f & (f - 1) == 0 and f <= 65536
     np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_AUTO), expect)
+    np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_MFMA), expect)
     for c in (1, 1 << 30):  # force either kernel through AUTO
         voc.lookup_crossover = c
         np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_AUTO), expect)
