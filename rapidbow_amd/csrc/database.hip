@@ -24,15 +24,14 @@ constexpr uint32_t kTopkRegs = 16;                      // candidate keys per th
 
 __device__ inline float ftz(float x) { return fabsf(x) < 1.17549435e-38f ? copysignf(0.0f, x) : x; }
 __device__ inline float fmulz(float a, float b) { return ftz(ftz(a) * ftz(b)); }
-__device__ inline float faddz(float a, float b) { return ftz(ftz(a) + ftz(b)); }
-__device__ inline float fsubz(float a, float b) { return ftz(ftz(a) - ftz(b)); }
 
 // DataBase.cpp:128-130 for one (node, doc) term, when every input is already
 // normal or zero (IDF weights and 1/L1 norms are flushed when stored, qE when
 // computed): then float(m) * w >= FLT_MIN or 0 and qE + tE >= max(qE, tE), so
 // only the product with rcp and the two differences can leave the normal
-// range, and these are the only flushes kept.  Equal to
-// fsubz(fabsf(fsubz(qE, tE)), faddz(qE, tE)) with tE = fmulz(fmulz(m, w), rcp).
+// range, and these are the only flushes kept.  Equal to the fully flushed
+// |qE - tE| - (qE + tE) with tE = fmulz(fmulz(m, w), rcp), every input and
+// result of each operation flushed to zero.
 __device__ inline float termNormal(uint32_t m, float w, float rcp, float qE) {
     const float tE = ftz(float(m) * w * rcp);
     return ftz(fabsf(ftz(qE - tE)) - (qE + tE));
@@ -441,31 +440,55 @@ __device__ inline uint32_t lowerBound(const uint32_t* p, uint32_t n, uint32_t ke
 // One workgroup per (document range r, query q).  The binary64 accumulators of
 // the range's kRangeDocs slots live in LDS (ds_add_f64), so a posting costs a
 // coalesced 4-byte read and an LDS atomic instead of a scattered global one.
+// An accumulator starts at -0.0 and every term is added as +0.0 or a negative
+// double (never as -0.0), so a slot was scored iff its bits differ from -0.0.
 //   Leaves, in rounds of kSegBatch: the in-range segment of each query leaf
 //   comes from the main CSR's range directory, and (for ranges that hold
 //   delta slots) from a binary search in the leaf's delta slice, in rounds of
-//   their own; a segment's start is kept as a pointer, so the walk does not
-//   care which CSR it reads.  Non-empty segments are compacted by a block
-//   scan, and each wave walks a contiguous share of the concatenated postings
-//   64 at a time (balanced however skewed the leaves are).  A run of one doc's
-//   duplicates (DataBase.cpp:138-149) is scored once, by its first posting,
-//   with m = run length from a ballot of run starts; a document's postings are
-//   all in one CSR, so no run spans two segments.
+//   their own.  Non-empty segments are compacted by a block scan into a table
+//   over the concatenated postings: end position, address of the segment's
+//   position 0 (so the walk does not care which CSR it reads), qE and w.
+//   Each wave walks a contiguous share of the concatenated postings, U steps
+//   of 64 per iteration (balanced however skewed the leaves are).  Per
+//   iteration the last position of every segment ending inside it is marked
+//   in a per-wave bitmap: a position's segment is the current one plus the
+//   marks below it (mbcnt), and a segment starts after a marked position.
+//   A run of one doc's duplicates (DataBase.cpp:138-149) is scored once, by
+//   its first posting, with m = run length from a ballot of run starts; a
+//   document's postings are all in one CSR, so no run spans two segments.
 //   Inner nodes (fewer than 16+4K postings, DataBase.cpp:150-164): one wave
 //   per node, multiplicity by scanning the node's slice in each CSR.
 // The range then keeps its K smallest (score, doc id) keys as candidates for
 // the query's final selection.
+constexpr int kWalkSteps = 6;  // steps of 64 postings per walk iteration: loads in flight per lane
+constexpr uint32_t kInnerCap = 4 * kSegBatch / 16;  // per-wave list of an inner node's in-range postings (u32)
+
+__device__ inline bool scored(double acc) { return __double_as_longlong(acc) != (long long)0x8000000000000000ull; }
+
+__device__ inline uint64_t uniform64(uint64_t x) {
+    return uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(x))))) |
+           (uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(x >> 32))))) << 32);
+}
+
+__device__ inline uint32_t postingAt(uint64_t vbase, uint32_t pos) {
+    return *reinterpret_cast<const uint32_t*>(vbase + 4ull * pos);
+}
+
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_scoreRange(DbView v, const uint64_t* __restrict__ off, uint32_t q0,
                                                     const QNode* __restrict__ nodes,
                                                     const uint32_t* __restrict__ nodeCount, uint32_t K,
                                                     uint64_t* __restrict__ candKeys, uint32_t* __restrict__ candCount) {
     constexpr uint32_t S = Database::kRangeDocs;
+    constexpr int U = kWalkSteps;
     __shared__ double sAcc[S];                 // reused as the keys of the selection
     __shared__ float sRcp[S];
-    __shared__ uint8_t sTouched[S];
-    __shared__ uint64_t sSegBeg[kSegBatch];    // first posting of the (non-empty) segment
+    // per segment: the address of its posting at concatenated position 0, then (qE, w) of its node;
+    // the inner nodes reuse the table as per-wave lists of in-range postings
+    __shared__ uint64_t sSegTab[2 * kSegBatch];
+    uint64_t* const sSegVb = sSegTab;
+    float2* const sSegQW = reinterpret_cast<float2*>(sSegTab + kSegBatch);
     __shared__ uint32_t sSegEnd[kSegBatch];    // inclusive prefix of segment lengths
-    __shared__ float sQE[kSegBatch], sW[kSegBatch];
+    __shared__ uint64_t sMark[16 * U];         // per wave: bit 64u+b marks position base+64u+b as a segment's last
     __shared__ uint32_t sHist[256], sRed[16], sCount;
     const uint32_t r = blockIdx.x, qc = blockIdx.y, q = q0 + qc, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (nodeCount[2 * q] == 0 && nodeCount[2 * q + 1] == 0) {  // nothing to score (an empty query)
@@ -474,10 +497,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     }
     const uint32_t s0 = r * S, ns = min(S, v.nbDocs - s0);
     for (uint32_t i = tid; i < S; i += 1024) {
-        sAcc[i] = 0.0;
-        sTouched[i] = 0;
+        sAcc[i] = -0.0;
         sRcp[i] = i < ns ? v.rcp[s0 + i] : 0.0f;
     }
+    if (tid < 16 * U) sMark[tid] = 0;
     if (tid == 0) sCount = 0;
     __syncthreads();
     const uint64_t beg = off[qc];  // off points at the launch's first query
@@ -487,6 +510,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     const bool hasMain = r < v.Rm;                                      // block-uniform
     const bool hasDelta = v.nbPostD != 0 && s0 + S > v.slotD0;
     const uint32_t nStage = (hasMain ? nLeaf : 0u) + (hasDelta ? nLeaf : 0u);
+    uint64_t* mark = sMark + wave * U;
     for (uint32_t k0 = 0; k0 < nStage; k0 += kSegBatch) {
         // ---- stage this round's segments (one query leaf and CSR per thread) ----
         uint32_t len = 0;
@@ -514,132 +538,138 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
         const uint32_t ex = blockScan(len, sRed, &total);
         const uint32_t cx = blockScan(len ? 1u : 0u, sRed, &nc);
         if (len) {
-            sSegBeg[cx] = sb;
+            sSegVb[cx] = sb - 4ull * ex;  // wraps below zero as an integer; only vbase + 4 * position is read
             sSegEnd[cx] = ex + len;
-            sQE[cx] = qE;
-            sW[cx] = w;
+            sSegQW[cx] = make_float2(qE, w);
         }
         __syncthreads();
-        // ---- walk the postings: wave-contiguous shares, 64 per step, U steps per iteration ----
+        // ---- walk the postings: wave-contiguous shares, U steps of 64 per iteration ----
         const uint32_t per = ((total + 16 * 64 - 1) / (16 * 64)) * 64;
-        const uint32_t wEnd = min(total, wave * per + per);
-        uint32_t seg = 0;
-        if (wave * per < wEnd) {  // first segment whose end exceeds the share's start (wave-uniform)
+        const uint32_t wBeg = wave * per, wEnd = min(total, wBeg + per);
+        uint32_t seg = 0, carry = 0;  // carry: the slot at position base - 1 when base is inside a segment
+        if (wBeg < wEnd) {  // first segment whose end exceeds the share's start
             uint32_t lo = 0, hi = nc - 1;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) / 2;
-                if (sSegEnd[mid] > wave * per) hi = mid;
+                if (sSegEnd[mid] > wBeg) hi = mid;
                 else lo = mid + 1;
             }
-            seg = lo;
+            seg = uint32_t(__builtin_amdgcn_readfirstlane(int(lo)));
+            if (wBeg > (seg ? sSegEnd[seg - 1] : 0u)) carry = postingAt(sSegVb[seg], wBeg - 1);
         }
-        constexpr int U = 6;  // steps of 64 postings per iteration: U loads in flight per lane
-        for (uint32_t base = wave * per; base < wEnd; base += 64 * U) {
-            uint32_t sgs[U], slots[U], is[U], prev0 = 0;
-            bool valids[U];
+        for (uint32_t base = wBeg; base < wEnd; base += 64 * U) {
             const uint32_t itEnd = min(base + 64 * U, wEnd);
-            while (sSegEnd[seg] <= base) seg++;  // the segment holding `base` (wave-uniform)
-            // The segments that end inside this iteration are seg, seg + 1, ...: one LDS read
-            // per lane finds them (lane j: segment seg + j), and each position's segment is
-            // then a wave-uniform walk over those boundaries, with no dependent LDS reads.
-            const uint32_t jSeg = seg + lane;
-            const uint32_t endJ = jSeg < nc ? sSegEnd[jSeg] : 0xFFFFFFFFu;
-            const uint64_t ends = __ballot(endJ < itEnd);  // a prefix of the lanes
-            const uint32_t nb = uint32_t(__builtin_popcountll(ends));
-            if (nb < 64) {
-                // lane j also holds the start and the posting base of the segment after
-                // boundary j - 1 (lane 0: of segment seg itself)
-                const uint32_t prevEnd = uint32_t(__shfl(int(endJ), int(lane ? lane - 1 : 0), 64));  // all lanes
-                const uint32_t startJ = lane == 0 ? (seg ? sSegEnd[seg - 1] : 0u) : prevEnd;
-                const uint64_t begJ = lane <= nb ? sSegBeg[jSeg] : 0ull;
-                uint32_t idx[U], k[U];
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    idx[u] = base + 64 * u + lane;
-                    k[u] = 0;
+            while (sSegEnd[seg] <= base) seg++;
+            seg = uint32_t(__builtin_amdgcn_readfirstlane(int(seg)));
+            const bool baseStarts = base == (seg ? sSegEnd[seg - 1] : 0u);
+            // mark the last position of each segment that ends inside this iteration
+            // (ends ascend, so the first chunk of lanes with an end past it is the last)
+            for (uint32_t j0 = seg;; j0 += 64) {
+                const uint32_t j = j0 + lane;
+                const uint32_t e = j < nc ? sSegEnd[j] : 0xFFFFFFFFu;
+                if (e < itEnd) {
+                    const uint32_t pm = e - 1 - base;  // e > base: segment seg ends after base
+                    atomicOr(reinterpret_cast<uint32_t*>(mark) + (pm >> 5), 1u << (pm & 31));
                 }
-                for (uint32_t b = 0; b < nb; b++) {  // k[u]: boundaries at or below the position
-                    const uint32_t e = uint32_t(__builtin_amdgcn_readlane(int(endJ), int(b)));
-#pragma unroll
-                    for (int u = 0; u < U; u++) k[u] += idx[u] >= e;
-                }
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    sgs[u] = seg + k[u];
-                    const uint32_t st = uint32_t(__shfl(int(startJ), int(k[u]), 64));
-                    const uint64_t sb = uint64_t(uint32_t(__shfl(int(uint32_t(begJ)), int(k[u]), 64))) |
-                                        (uint64_t(uint32_t(__shfl(int(uint32_t(begJ >> 32)), int(k[u]), 64))) << 32);
-                    valids[u] = idx[u] < wEnd;
-                    is[u] = idx[u] - st;
-                    const uint32_t* p = reinterpret_cast<const uint32_t*>(sb);
-                    slots[u] = valids[u] ? p[is[u]] : 0xFFFFFFFFu;
-                    if (u == 0 && lane == 0 && valids[0] && is[0] > 0) prev0 = p[is[0] - 1];
-                }
-            } else {  // more than 63 segments end in this iteration: search per position
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const uint32_t idx = base + 64 * u + lane;
-                    valids[u] = idx < wEnd;
-                    uint32_t lo = seg, hi = nc - 1;
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi) / 2;
-                        if (sSegEnd[mid] > idx) hi = mid;
-                        else lo = mid + 1;
-                    }
-                    const uint32_t sg = valids[u] ? lo : seg;
-                    sgs[u] = sg;
-                    is[u] = idx - (sg ? sSegEnd[sg - 1] : 0u);
-                    const uint32_t* p = reinterpret_cast<const uint32_t*>(sSegBeg[sg]);
-                    slots[u] = valids[u] ? p[is[u]] : 0xFFFFFFFFu;
-                    if (u == 0 && lane == 0 && valids[0] && is[0] > 0) prev0 = p[is[0] - 1];
-                }
+                if (__ballot(e >= itEnd)) break;
             }
+            uint64_t M[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) M[u] = uniform64(mark[u]);
+            if (lane < U) mark[lane] = 0;  // after the reads: a wave's LDS accesses complete in order
+            // step u, lane l: position base + 64u + l; sgs: its segment, bit 31 set where a segment starts
+            uint32_t slots[U], sgs[U];
+            uint32_t cum = 0;  // marks in the earlier steps
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t idx = base + 64 * u + lane;
+                const uint32_t sg =
+                    seg + cum + __builtin_amdgcn_mbcnt_hi(uint32_t(M[u] >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(M[u]), 0u));
+                cum += uint32_t(__builtin_popcountll(M[u]));
+                const uint64_t starts = (M[u] << 1) | (u == 0 ? uint64_t(baseStarts) : (M[u ? u - 1 : 0] >> 63));
+                sgs[u] = sg | (uint32_t((starts >> lane) & 1ull) << 31);  // sg < nc for every lane: marks lie below itEnd
+                slots[u] = idx < itEnd ? postingAt(sSegVb[sg], idx) : 0xFFFFFFFFu;
+            }
+            // run starts (a new segment or a new slot); slots[u] keeps the slot of a run start only
             uint64_t bnd[U];
-            bool heads[U];
+            uint32_t prevLast = carry;  // the slot before the step
 #pragma unroll
-            for (int u = 0; u < U; u++) {  // run starts: a new segment or a new slot
-                const uint32_t up = uint32_t(__shfl_up(int(slots[u]), 1, 64));
-                const uint32_t last = u ? uint32_t(__shfl(int(slots[u ? u - 1 : 0]), 63, 64)) : prev0;
-                const uint32_t before = lane ? up : last;  // same segment whenever is[u] > 0
-                heads[u] = valids[u] && (is[u] == 0 || before != slots[u]);
-                bnd[u] = __ballot(heads[u] || !valids[u]);
+            for (int u = 0; u < U; u++) {
+                const uint32_t slot = slots[u];
+                const bool valid = base + 64 * u + lane < itEnd;
+                // wave_shr:1 -- lane l gets lane l-1's slot, lane 0 the slot before the step
+                const uint32_t before = uint32_t(__builtin_amdgcn_update_dpp(int(prevLast), int(slot), 0x138, 0xF, 0xF, false));
+                prevLast = uint32_t(__builtin_amdgcn_readlane(int(slot), 63));
+                const bool head = valid && ((sgs[u] >> 31) != 0 || before != slot);
+                bnd[u] = __ballot(head || !valid);
+                slots[u] = head ? slot : 0xFFFFFFFFu;
             }
-            // first boundary after step u, as a position of the iteration (64 * U: none);
-            // wave-uniform, so the run lengths below need no per-lane loop over steps
-            uint32_t after[U];
+            carry = prevLast;  // position base + 64U - 1
+            // steps in reverse: nxt = first boundary after step u, as a position of the
+            // iteration (64 * U: none), wave-uniform, so a run length needs no per-lane loop
             uint32_t nxt = 64 * U;
 #pragma unroll
             for (int u = U - 1; u >= 0; u--) {
-                after[u] = nxt;
+                const uint32_t after = nxt;
                 if (bnd[u]) nxt = 64 * u + uint32_t(__builtin_ctzll(bnd[u]));
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                if (!heads[u]) continue;
+                const uint32_t slot = slots[u];
+                if (slot == 0xFFFFFFFFu) continue;  // not a run start
                 // run length: distance to the next boundary over this iteration's steps
                 const uint64_t above = lane == 63 ? 0ull : (bnd[u] >> (lane + 1));
-                const bool found = above != 0 || after[u] < 64 * U;
-                uint32_t m = above ? uint32_t(__builtin_ctzll(above)) + 1 : after[u] - (64 * u + lane);
-                const uint32_t sg = sgs[u], slot = slots[u], i = is[u];
-                if (!found || base + 64 * u + lane + m >= wEnd) {
+                const bool found = above != 0 || after < 64 * U;
+                uint32_t m = above ? uint32_t(__builtin_ctzll(above)) + 1 : after - (64 * u + lane);
+                const uint32_t idx = base + 64 * u + lane, sg = sgs[u] & 0x7FFFFFFFu;
+                if (!found || idx + m >= wEnd) {
                     // no run start follows in this iteration: the run may go on past it or past the share
-                    const uint32_t segLen = sSegEnd[sg] - (sg ? sSegEnd[sg - 1] : 0u);
-                    const uint32_t* p = reinterpret_cast<const uint32_t*>(sSegBeg[sg]);
-                    while (i + m < segLen && p[i + m] == slot) m++;
+                    const uint32_t segEnd = sSegEnd[sg];
+                    const uint64_t vb = sSegVb[sg];
+                    while (idx + m < segEnd && postingAt(vb, idx + m) == slot) m++;
                 }
+                const float2 qw = sSegQW[sg];
                 const uint32_t j = slot - s0;
-                atomicAdd(&sAcc[j], double(termNormal(m, sW[sg], sRcp[j], sQE[sg])));
-                sTouched[j] = 1;
+                atomicAdd(&sAcc[j], double(termNormal(m, qw.y, sRcp[j], qw.x)) + 0.0);  // + 0.0: -0.0 -> +0.0
             }
         }
         __syncthreads();  // segment arrays are reused by the next round
     }
+    // Inner nodes: a node's slice is its leaves' lists one after the other, so a
+    // doc's postings in it need not be adjacent.  The wave compacts the slice's
+    // in-range postings (in slice order) into its LDS list, and each of them counts
+    // its equals there: the first of equal slots scores the doc with m = count.
+    uint32_t* inList = reinterpret_cast<uint32_t*>(sSegTab) + wave * kInnerCap;
     for (uint32_t k = wave; k < nInner; k += 16) {
         const QNode node = nd[n + k];
         for (int c = 0; c < 2; c++) {  // the node's slice in the main CSR, then in the delta
             const uint32_t* p = c ? v.postD + node.aD : v.post + node.a;
             const uint32_t len = uint32_t(c ? node.eD - node.aD : node.e - node.a);
             if (c ? !hasDelta : !hasMain) continue;
+            uint32_t nIn = 0;
+            for (uint32_t i0 = 0; i0 < len; i0 += 64) {
+                const uint32_t slot = i0 + lane < len ? p[i0 + lane] : 0xFFFFFFFFu;
+                const bool in = slot - s0 < S;  // slots are below 2^32 - 1 - S
+                const uint64_t bal = __ballot(in);
+                const uint32_t pos = nIn + __builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u));
+                if (in && pos < kInnerCap) inList[pos] = slot;
+                nIn += uint32_t(__builtin_popcountll(bal));
+            }
+            if (nIn <= kInnerCap) {  // the wave's own LDS writes are complete before its later reads
+                for (uint32_t i = lane; i < nIn; i += 64) {
+                    const uint32_t slot = inList[i];
+                    bool first = true;
+                    uint32_t m = 0;
+                    for (uint32_t x = 0; x < nIn; x++) {
+                        if (inList[x] == slot) {
+                            m++;
+                            if (x < i) first = false;
+                        }
+                    }
+                    if (!first) continue;
+                    const uint32_t j = slot - s0;
+                    atomicAdd(&sAcc[j], double(termNormal(m, node.w, sRcp[j], node.qE)) + 0.0);  // + 0.0: -0.0 -> +0.0
+                }
+                continue;
+            }
+            // more in-range postings than the list holds (only with K above 60): count in the slice itself
             for (uint32_t i = lane; i < len; i += 64) {
                 const uint32_t slot = p[i], j = slot - s0;
                 if (j >= S) continue;
@@ -652,10 +682,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
                     }
                 }
                 if (!first) continue;
-                const float tE = fmulz(fmulz(float(m), node.w), sRcp[j]);
-                const float e = fsubz(fabsf(fsubz(node.qE, tE)), faddz(node.qE, tE));
-                atomicAdd(&sAcc[j], double(e));
-                sTouched[j] = 1;
+                atomicAdd(&sAcc[j], double(termNormal(m, node.w, sRcp[j], node.qE)) + 0.0);
             }
         }
     }
@@ -664,7 +691,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     uint64_t* sKey = reinterpret_cast<uint64_t*>(sAcc);
     uint32_t mine = 0;
     for (uint32_t i = tid; i < S; i += 1024) {
-        const bool t = i < ns && sTouched[i];
+        const bool t = i < ns && scored(sAcc[i]);
         sKey[i] = t ? rankKey(sAcc[i], v.docId[s0 + i]) : ~0ull;
         mine += t;
     }
