@@ -42,6 +42,8 @@ def build(verbose=False):
         cmd = [HIPCC] + DEVICE + COMMON + ["-c", src, "-o", obj]  # host sources too: gfx950 is the only target
         if s == "synth.cpp":
             cmd += ["-fopenmp"]
+        if s == "database.hip":  # binary32 denormals flushed in hardware: the reference's DAZ/FTZ scoring
+            cmd += ["-fgpu-flush-denormals-to-zero"]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
     errors = []
     for cmd, p in procs:

@@ -22,19 +22,20 @@ constexpr uint32_t kSortCap = Database::kMaxQueryDesc;  // leaves per query sort
 constexpr uint32_t kSegBatch = 1024;                    // query leaves staged per scoring round (one per thread)
 constexpr uint32_t kTopkRegs = 16;                      // candidate keys per thread kept in registers by k_topk
 
+// This file is compiled with binary32 denormals flushed by the hardware
+// (-fgpu-flush-denormals-to-zero: the inputs and the result of every binary32
+// operation, sign kept), which is the reference's DAZ/FTZ host arithmetic
+// (-ffast-math, CMakeLists.txt:16).  ftz() states the flush where it matters.
 __device__ inline float ftz(float x) { return fabsf(x) < 1.17549435e-38f ? copysignf(0.0f, x) : x; }
 __device__ inline float fmulz(float a, float b) { return ftz(ftz(a) * ftz(b)); }
 
-// DataBase.cpp:128-130 for one (node, doc) term, when every input is already
-// normal or zero (IDF weights and 1/L1 norms are flushed when stored, qE when
-// computed): then float(m) * w >= FLT_MIN or 0 and qE + tE >= max(qE, tE), so
-// only the product with rcp and the two differences can leave the normal
-// range, and these are the only flushes kept.  Equal to the fully flushed
-// |qE - tE| - (qE + tE) with tE = fmulz(fmulz(m, w), rcp), every input and
-// result of each operation flushed to zero.
+// DataBase.cpp:128-130 for one (node, doc) term: |qE - tE| - (qE + tE) with
+// tE = m * w * rcp, each operation rounded (no contraction into fma) and
+// flushed by the hardware mode above, as the reference computes it.
 __device__ inline float termNormal(uint32_t m, float w, float rcp, float qE) {
-    const float tE = ftz(float(m) * w * rcp);
-    return ftz(fabsf(ftz(qE - tE)) - (qE + tE));
+#pragma clang fp contract(off)
+    const float tE = float(m) * w * rcp;
+    return fabsf(qE - tE) - (qE + tE);
 }
 
 struct QNode {
@@ -470,8 +471,11 @@ __device__ inline uint64_t uniform64(uint64_t x) {
            (uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(x >> 32))))) << 32);
 }
 
+// A posting read through a global (not flat) pointer: a flat load also counts as
+// an LDS access, so every LDS wait behind it would wait for the load as well.
+typedef const uint32_t __attribute__((address_space(1)))* GlobalU32;
 __device__ inline uint32_t postingAt(uint64_t vbase, uint32_t pos) {
-    return *reinterpret_cast<const uint32_t*>(vbase + 4ull * pos);
+    return *reinterpret_cast<GlobalU32>(vbase + 4ull * pos);
 }
 
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_scoreRange(DbView v, const uint64_t* __restrict__ off, uint32_t q0,
