@@ -205,12 +205,13 @@ __device__ void bitonicSortBlock(uint32_t* s, uint32_t n2) {
 __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __restrict__ leaf,
                                                    const uint64_t* __restrict__ off, uint32_t K,
                                                    QNode* __restrict__ nodes, uint32_t* __restrict__ nodeCount,
-                                                   uint32_t* __restrict__ bad) {
+                                                   uint32_t* __restrict__ work, uint32_t* __restrict__ bad) {
     __shared__ uint32_t s[kSortCap];
     __shared__ uint16_t sStart[kSortCap + 1];  // run starts of one level (positions in s), then n
     __shared__ double red[16];
     __shared__ uint32_t redU[16];
     __shared__ uint32_t nNodes, nInner;
+    __shared__ unsigned long long sWork;
     const uint32_t q = blockIdx.x, tid = threadIdx.x;
     const uint64_t beg = off[q];
     const uint32_t n = uint32_t(off[q + 1] - beg);
@@ -218,6 +219,7 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
         if (tid == 0) {
             nodeCount[2 * q] = 0;
             nodeCount[2 * q + 1] = 0;
+            work[q] = 0;
         }
         return;
     }
@@ -227,6 +229,7 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
     if (tid == 0) {
         nNodes = 0;
         nInner = 0;
+        sWork = 0;
     }
     __syncthreads();
     // bitonic sort, ascending, of n2 >= 1024 keys (padded with 0xFFFFFFFF)
@@ -242,6 +245,7 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
         if (tid == 0) {
             nodeCount[2 * q] = 0;
             nodeCount[2 * q + 1] = 0;
+            work[q] = 0;
             if (bad) atomicOr(bad, 1u);
         }
         return;
@@ -259,6 +263,7 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
     // per level: the runs of equal node ids (leaf >> shift) in the sorted leaves;
     // thread tid scans the contiguous chunk [c0, c1) for run starts
     const uint32_t E = (n + 1023) / 1024, c0 = min(n, tid * E), c1 = min(n, c0 + E);
+    unsigned long long myWork = 0;  // postings of the query's leaves: the scoring's share of work
     for (uint32_t t = 0; t < v.L; t++) {
         const uint32_t shift = t * v.lg2b;
         const float* W = v.W + levelBeg(v.lg2b, v.L - 1 - t) * v.b;
@@ -300,6 +305,7 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 if (k0 + u * 1024 >= nRuns || (t != 0 && (e[u] - a[u]) + (eD[u] - aD[u]) >= thr)) continue;
+                if (t == 0) myWork += (e[u] - a[u]) + (eD[u] - aD[u]);
                 QNode nd;
                 nd.v = x[u];
                 nd.t = t;
@@ -315,11 +321,13 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
                 else out[n + atomicAdd(&nInner, 1u)] = nd;
             }
         }
+        if (t == 0 && myWork) atomicAdd(&sWork, myWork);
         __syncthreads();  // sStart is rewritten by the next level
     }
     if (tid == 0) {
         nodeCount[2 * q] = nNodes;
         nodeCount[2 * q + 1] = nInner;
+        work[q] = uint32_t(min(sWork, 0xFFFFFFFFull));
     }
 }
 
@@ -438,6 +446,26 @@ __device__ inline uint32_t lowerBound(const uint32_t* p, uint32_t n, uint32_t ke
     return lo;
 }
 
+// The launch's queries by descending work (ties by index): order[rank] = query.
+// The scoring grid then starts the longest queries first, so the last
+// workgroups to start are the shortest ones.
+constexpr uint32_t kOrderCap = 4096;  // larger batches keep their own order
+__global__ __launch_bounds__(1024) void k_orderQueries(const uint32_t* __restrict__ work, uint32_t nq,
+                                                       uint32_t* __restrict__ order) {
+    __shared__ uint32_t sW[kOrderCap];
+    for (uint32_t i = threadIdx.x; i < nq; i += 1024) sW[i] = work[i];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nq; i += 1024) {
+        const uint32_t wi = sW[i];
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < nq; j++) {
+            const uint32_t wj = sW[j];
+            rank += wj > wi || (wj == wi && j < i);
+        }
+        order[rank] = i;
+    }
+}
+
 // One workgroup per (document range r, query q).  The binary64 accumulators of
 // the range's kRangeDocs slots live in LDS (ds_add_f64), so a posting costs a
 // coalesced 4-byte read and an LDS atomic instead of a scattered global one.
@@ -460,7 +488,8 @@ __device__ inline uint32_t lowerBound(const uint32_t* p, uint32_t n, uint32_t ke
 //   Inner nodes (fewer than 16+4K postings, DataBase.cpp:150-164): one wave
 //   per node, multiplicity by scanning the node's slice in each CSR.
 // The range then keeps its K smallest (score, doc id) keys as candidates for
-// the query's final selection.
+// the query's final selection.  Query blockIdx.y of the launch is order[y]
+// when an order is given.
 constexpr int kWalkSteps = 6;  // steps of 64 postings per walk iteration: loads in flight per lane
 constexpr uint32_t kInnerCap = 4 * kSegBatch / 16;  // per-wave list of an inner node's in-range postings (u32)
 
@@ -480,7 +509,8 @@ __device__ inline uint32_t postingAt(uint64_t vbase, uint32_t pos) {
 
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_scoreRange(DbView v, const uint64_t* __restrict__ off, uint32_t q0,
                                                     const QNode* __restrict__ nodes,
-                                                    const uint32_t* __restrict__ nodeCount, uint32_t K,
+                                                    const uint32_t* __restrict__ nodeCount,
+                                                    const uint32_t* __restrict__ order, uint32_t K,
                                                     uint64_t* __restrict__ candKeys, uint32_t* __restrict__ candCount) {
     constexpr uint32_t S = Database::kRangeDocs;
     constexpr int U = kWalkSteps;
@@ -494,7 +524,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     __shared__ uint32_t sSegEnd[kSegBatch];    // inclusive prefix of segment lengths
     __shared__ uint64_t sMark[16 * U];         // per wave: bit 64u+b marks position base+64u+b as a segment's last
     __shared__ uint32_t sHist[256], sRed[16], sCount;
-    const uint32_t r = blockIdx.x, qc = blockIdx.y, q = q0 + qc, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t r = blockIdx.x, qc = order ? order[blockIdx.y] : blockIdx.y, q = q0 + qc, tid = threadIdx.x,
+                   lane = tid & 63, wave = tid >> 6;
     if (nodeCount[2 * q] == 0 && nodeCount[2 * q + 1] == 0) {  // nothing to score (an empty query)
         if (tid == 0) candCount[uint64_t(q) * v.R + r] = 0;
         return;
@@ -888,7 +919,7 @@ void launchMergeCsr(uint64_t nbLeaves, const DevCsr& a, const DevCsr& b, DevCsr&
 size_t queryWorkspaceBytes(const DbView& v, uint64_t totalDesc, uint32_t nbQueries, uint32_t K) {
     const uint64_t nbCand = uint64_t(nbQueries) * v.R * K;
     const uint64_t nbNodes = std::max<uint64_t>(totalDesc * v.L, 1);
-    return sizeof(QNode) * nbNodes + 8 * nbCand + 4ull * nbQueries * v.R + 8ull * nbQueries + 1024;
+    return sizeof(QNode) * nbNodes + 8 * nbCand + 4ull * nbQueries * v.R + 16ull * nbQueries + 1024;
 }
 
 void launchQueryBatchInWs(const DbView& v, const uint32_t* dLeaf, const uint64_t* dOffsets, uint64_t totalDesc,
@@ -904,11 +935,15 @@ void launchQueryBatchInWs(const DbView& v, const uint32_t* dLeaf, const uint64_t
     uint64_t* candKeys = reinterpret_cast<uint64_t*>(nodes + nbNodes);
     uint32_t* candCount = reinterpret_cast<uint32_t*>(candKeys + nbCand);
     uint32_t* nodeCount = candCount + uint64_t(nbQueries) * v.R;
-    k_queryPrep<<<nbQueries, 1024, 0, s>>>(v, dLeaf, dOffsets, K, nodes, nodeCount, badFlag);
+    uint32_t* work = nodeCount + 2ull * nbQueries;
+    uint32_t* order = work + nbQueries;
+    k_queryPrep<<<nbQueries, 1024, 0, s>>>(v, dLeaf, dOffsets, K, nodes, nodeCount, work, badFlag);
+    const bool ordered = nbQueries > 1 && nbQueries <= kOrderCap;
+    if (ordered) k_orderQueries<<<1, 1024, 0, s>>>(work, nbQueries, order);
     // ranges fastest, so the tiles of one query run together and share its node list in L2
     for (uint32_t q0 = 0; v.R && q0 < nbQueries; q0 += 65535)
-        k_scoreRange<<<dim3(v.R, std::min(nbQueries - q0, 65535u)), 1024, 0, s>>>(v, dOffsets + q0, q0, nodes,
-                                                                                  nodeCount, K, candKeys, candCount);
+        k_scoreRange<<<dim3(v.R, std::min(nbQueries - q0, 65535u)), 1024, 0, s>>>(
+            v, dOffsets + q0, q0, nodes, nodeCount, ordered ? order : nullptr, K, candKeys, candCount);
     k_topk<<<nbQueries, 1024, 0, s>>>(v, K, candKeys, candCount, dDocs, dScores, dCounts);
     RBOW_HIP(hipGetLastError());
 }

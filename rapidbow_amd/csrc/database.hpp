@@ -91,7 +91,7 @@ public:
     bool hasPendingInserts() const;  // inserts not yet folded into the device state
     // HBM scratch of one query of n descriptors, top-K, during scoring
     uint64_t scratchBytes(uint32_t n, uint32_t K) const {
-        return uint64_t(divUp(nbDocs(), kRangeDocs)) * (8ull * K + 4) + uint64_t(n) * L_ * kQNodeBytes + 64;
+        return uint64_t(divUp(nbDocs(), kRangeDocs)) * (8ull * K + 4) + uint64_t(n) * L_ * kQNodeBytes + 80;
     }
 
     uint32_t nbDocs() const;
