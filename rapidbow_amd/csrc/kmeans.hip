@@ -968,22 +968,23 @@ __global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
             }
         }
         const float* centers = k.centersF32 + uint64_t(curTask) * B * kDims;
+        // Raw bytes: the sign flip happens at use, so the next block's loads stay in flight
+        // while one computes.  A row past the tile's end reads the tile's first row; its
+        // column of the products is discarded (filterSplit's rowValid) and its sums come
+        // from the epilogue, which skips it.
         auto loadRows = [&](uint32_t rb, v4i& x0, v4i& x1) {
-            x0 = v4i{0, 0, 0, 0};  // rows past the tile's end contribute nothing (and are not written)
-            x1 = v4i{0, 0, 0, 0};
-            if (rb * 16 + col < t.cnt) {
-                const uint8_t* x = k.X + uint64_t(t.row + rb * 16 + col) * kDims;
-                x0 = toSigned(ld16(x + grp * 16));
-                x1 = toSigned(ld16(x + 64 + grp * 16));
-            }
+            const uint32_t r = rb * 16 + col;
+            const uint8_t* x = k.X + uint64_t(t.row + (r < t.cnt ? r : 0u)) * kDims;
+            x0 = ld16(x + grp * 16);
+            x1 = ld16(x + 64 + grp * 16);
         };
         v4i n0, n1;
         loadRows(0, n0, n1);
         for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
             const uint32_t r = rb * 16 + col;
             const bool valid = r < t.cnt;
-            const v4i x0 = n0, x1 = n1;
-            if ((rb + 1) * 16 < t.cnt) loadRows(rb + 1, n0, n1);
+            const v4i x0 = toSigned(n0), x1 = toSigned(n1);
+            loadRows(rb + 1, n0, n1);  // unconditional (past the end: the tile's first row), so no copy at a join waits for it
             // ||x - 128||^2 of row (lane & 15): an exact integer, summed over the row's four lanes
             int xs = 0;
 #pragma unroll

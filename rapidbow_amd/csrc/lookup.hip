@@ -373,28 +373,30 @@ __global__ __launch_bounds__(256) void k_levelI8(const uint8_t* __restrict__ row
         const v4i ns = *reinterpret_cast<const v4i*>(normS + childBase + 4 * grp);  // ||c - 128||^2 of children 4*grp + i
         // the lane's row of a block: its descriptor (input index) and bytes; the next
         // block's bytes and the permutation entry of the one after are in flight
+        // A row past the item's end stands for its last row (a valid descriptor): its column of
+        // the products is discarded and nothing is written for it.  With every index valid, the
+        // loads below are unconditional: a load under a branch ends in a copy at the join that
+        // waits for it, which would undo the prefetch.
         auto qAt = [&](uint32_t rb) {
-            const uint32_t r = rb * 16 + col;
-            return r < t.cnt ? (kIdentity ? t.start + r : perm[t.start + r]) : 0u;
+            const uint32_t r = min(rb * 16 + col, t.cnt - 1u);
+            return kIdentity ? t.start + r : perm[t.start + r];
         };
-        auto loadRows = [&](uint32_t rb, uint32_t q, v4i& x0, v4i& x1) {
-            x0 = v4i{0, 0, 0, 0};  // rows past the item's end contribute nothing (and are not written)
-            x1 = v4i{0, 0, 0, 0};
-            if (rb * 16 + col < t.cnt) {
-                const uint8_t* x = desc + uint64_t(q) * kDims;
-                x0 = toSigned(ld16(x + grp * 16));
-                x1 = toSigned(ld16(x + 64 + grp * 16));
-            }
+        // Raw bytes: the sign flip happens at use, so the loads stay in flight while the
+        // current block computes.
+        auto loadRows = [&](uint32_t q, v4i& x0, v4i& x1) {
+            const uint8_t* x = desc + uint64_t(q) * kDims;
+            x0 = ld16(x + grp * 16);
+            x1 = ld16(x + 64 + grp * 16);
         };
         uint32_t qc = qAt(0), qn = qAt(1);
         v4i n0, n1;
-        loadRows(0, qc, n0, n1);
+        loadRows(qc, n0, n1);
         for (uint32_t rb = 0; rb * 16 < t.cnt; rb++) {
             const uint32_t r = rb * 16 + col, q = qc;
-            const v4i a0 = n0, a1 = n1;
+            const v4i a0 = toSigned(n0), a1 = toSigned(n1);
             qc = qn;
-            if ((rb + 1) * 16 < t.cnt) loadRows(rb + 1, qc, n0, n1);
-            if ((rb + 2) * 16 < t.cnt) qn = qAt(rb + 2);
+            loadRows(qc, n0, n1);  // block rb + 1 (past the end: the item's last row again)
+            qn = qAt(rb + 2);
             v4i acc = {0, 0, 0, 0};
             acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(b0, a0, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(b1, a1, acc, 0, 0, 0);
@@ -476,8 +478,8 @@ __device__ inline uint32_t fusedLevel(const uint8_t* sRows, const int32_t* sNorm
         } else {
             const uint64_t child = (lb + v) * 16;
             const uint8_t* c = rows + (child + col) * kDims;
-            x0 = toSigned(ld16(c + grp * 16));
-            x1 = toSigned(ld16(c + 64 + grp * 16));
+            x0 = ld16(c + grp * 16);  // raw bytes, flipped at use: the ring's loads stay in flight
+            x1 = ld16(c + 64 + grp * 16);
             n4 = *reinterpret_cast<const v4i*>(normS + child + 4 * grp);
         }
     };
@@ -489,8 +491,8 @@ __device__ inline uint32_t fusedLevel(const uint8_t* sRows, const int32_t* sNorm
         const uint32_t v = uint32_t(__builtin_amdgcn_readlane(int(node), r));  // row r's node
         const v4i b0 = rowBcast<r>(x0), b1 = rowBcast<r>(x1);                   // row r in every column
         v4i d = {0, 0, 0, 0};
-        d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[r % D], b0, d, 0, 0, 0);
-        d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[r % D], b1, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_i32_16x16x64_i8(kLds ? a0[r % D] : toSigned(a0[r % D]), b0, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_i32_16x16x64_i8(kLds ? a1[r % D] : toSigned(a1[r % D]), b1, d, 0, 0, 0);
         const uint32_t key = rowKey(d, ns[r % D], grp);
         if (r + D < 16) loadA(uint32_t(__builtin_amdgcn_readlane(int(node), r + D)), a0[r % D], a1[r % D], ns[r % D]);
         if (col == uint32_t(r)) next = v * 16 + (key & 15u);
@@ -560,7 +562,7 @@ __global__ __launch_bounds__(256) void k_lookupFusedI8(const uint8_t* __restrict
 // queued per wave and resolved with the plain FP32 chain at the item's end.
 //
 template <bool kIdentity, bool kLast, int B>
-__global__ __launch_bounds__(256) void k_levelSplit(
+__global__ __launch_bounds__(256, 2) void k_levelSplit(  // two waves per SIMD: at most 256 registers
     const float* __restrict__ rows32, const int8_t* __restrict__ split, uint64_t stride,
     const float* __restrict__ cn2, const float* __restrict__ e1, const float* __restrict__ e2, uint64_t levelBase,
     const uint8_t* __restrict__ desc, uint32_t n, const uint32_t* __restrict__ perm, const LkItem* __restrict__ items,
@@ -582,11 +584,13 @@ __global__ __launch_bounds__(256) void k_levelSplit(
         const LkItem t = itemAt<kIdentity>(it, n, items);
         const uint64_t node = levelBase + t.node, childBase = node * B;
         // the item's descriptors: lane l holds those of rows g*64 + l, g = 0..3 (group g)
+        // (a row past the item's end stands for its last row, a valid descriptor: its column
+        // of the products is discarded and nothing is written for it)
         uint32_t qv[4];
 #pragma unroll
         for (int g = 0; g < 4; g++) {
-            const uint32_t r = g * 64 + lane;
-            qv[g] = r < t.cnt ? (kIdentity ? t.start + r : perm[t.start + r]) : 0u;
+            const uint32_t r = min(g * 64 + lane, t.cnt - 1u);
+            qv[g] = kIdentity ? t.start + r : perm[t.start + r];
         }
         v4i ca[3][CB][2];
         float cn[CB][4];
@@ -617,25 +621,24 @@ __global__ __launch_bounds__(256) void k_levelSplit(
         auto rowOf = [&](uint32_t qg, int S) {  // descriptor of this lane's row of block S of a group
             return uint32_t(__shfl(int(qg), S * 16 + int(col), 64));
         };
-        auto loadRows = [&](uint32_t rb, uint32_t q, v4i& x0, v4i& x1) {
-            x0 = v4i{0, 0, 0, 0};  // rows past the item's end contribute nothing (and are not written)
-            x1 = v4i{0, 0, 0, 0};
-            if (rb * 16 + col < t.cnt) {
-                const uint8_t* x = desc + uint64_t(q) * kDims;
-                x0 = toSigned(ld16(x + grp * 16));
-                x1 = toSigned(ld16(x + 64 + grp * 16));
-            }
+        // Raw bytes: the sign flip happens at use, so a slot's loads stay in flight for four
+        // blocks.  The loads are unconditional (every qv is a valid descriptor): a load under a
+        // branch ends in a copy at the join that waits for it.
+        auto loadRows = [&](uint32_t q, v4i& x0, v4i& x1) {
+            const uint8_t* x = desc + uint64_t(q) * kDims;
+            x0 = ld16(x + grp * 16);
+            x1 = ld16(x + 64 + grp * 16);
         };
-        unrollSeq([&](auto S) { loadRows(S, rowOf(qv[0], S), rx0[S], rx1[S]); }, std::make_integer_sequence<int, 4>{});
+        unrollSeq([&](auto S) { loadRows(rowOf(qv[0], S), rx0[S], rx1[S]); }, std::make_integer_sequence<int, 4>{});
         for (uint32_t g = 0; g * 64 < t.cnt; g++) {
             unrollSeq(
                 [&](auto S) {
                     const uint32_t rb = g * 4 + S;
+                    const v4i x0 = toSigned(rx0[S]), x1 = toSigned(rx1[S]);
+                    loadRows(rowOf(qv[1], S), rx0[S], rx1[S]);  // block rb + 4 (or a valid row past the end)
                     if (rb * 16 >= t.cnt) return;
                     const uint32_t r = rb * 16 + col, q = rowOf(qv[0], S);
                     const bool valid = r < t.cnt;
-                    const v4i x0 = rx0[S], x1 = rx1[S];
-                    if ((rb + 4) * 16 < t.cnt) loadRows(rb + 4, rowOf(qv[1], S), rx0[S], rx1[S]);
                     // ||x - 128||^2 of row (lane & 15): an exact integer, summed over the row's four lanes
                     int xs = 0;
 #pragma unroll
