@@ -73,7 +73,10 @@ __device__ inline float plainF32Distance(const uint8_t* x, const float* c) {
 // candidate, or the plain chain over several) is unchanged.
 constexpr float kTauScale = 2.0f * (1.0f + 1.0f / 1024.0f) * (1.0f + 1.0f / 16384.0f);  // 2 sigma / (1 - g sigma), rounded up
 
-__device__ inline float minF(float a, float b) { return b < a ? b : a; }
+// Minimum of two finite floats: one v_min_f32 (a compare and select would take two).
+// The scores are never NaN; for -0 against +0 either may come back, which changes no
+// threshold (m + tau) and no test s <= thr.
+__device__ inline float minF(float a, float b) { return __builtin_fminf(a, b); }
 
 // Minimum over lanes l, l ^ 16, l ^ 32, l ^ 48, in every lane.
 __device__ inline float minAcrossGroupsF(float v) {

@@ -36,6 +36,8 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int kScoreBias = 1 << 25;   // makes integer scores positive before key packing
 constexpr uint32_t kTQ = 256;         // descriptors per MFMA work item (16 row blocks of 16)
+constexpr uint32_t kTQLeafF32 = 512;  // items of a float tree's leaf level: most leaf nodes are then one item, so
+                                      // a node's split centroids (12 KB at b = 32) are read once, not once per item
 
 // A work item: up to kTQ descriptors that sit at the same node.  Items of a
 // node are consecutive; itemBase/nItems locate the node's items so that the
@@ -315,7 +317,7 @@ __device__ inline void childRange(const uint32_t* O, const uint32_t* pBase, cons
 
 __global__ void k_nodeItemCounts(const uint32_t* __restrict__ O, const uint32_t* __restrict__ pBase,
                                  const uint32_t* __restrict__ pN, uint32_t nItems0, uint32_t nb, uint32_t lg2b,
-                                 uint32_t* __restrict__ cN) {
+                                 uint32_t itemRows, uint32_t* __restrict__ cN) {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= nb; v += gridDim.x * blockDim.x) {
         if (v == nb) {
             cN[v] = 0;  // scan tail: cBase[nb] = total items
@@ -323,13 +325,13 @@ __global__ void k_nodeItemCounts(const uint32_t* __restrict__ O, const uint32_t*
         }
         uint32_t s0, c0;
         childRange(O, pBase, pN, nItems0, lg2b, v, s0, c0);
-        cN[v] = (c0 + kTQ - 1) / kTQ;
+        cN[v] = (c0 + itemRows - 1) / itemRows;
     }
 }
 
 __global__ void k_writeItems(const uint32_t* __restrict__ O, const uint32_t* __restrict__ pBase,
                              const uint32_t* __restrict__ pN, uint32_t nItems0, uint32_t nb, uint32_t lg2b,
-                             const uint32_t* __restrict__ cBase, LkItem* __restrict__ items,
+                             uint32_t itemRows, const uint32_t* __restrict__ cBase, LkItem* __restrict__ items,
                              uint32_t* __restrict__ nbItems) {
     // one node per block: its items are written by the block's threads (a node
     // near the root holds thousands of items)
@@ -340,9 +342,9 @@ __global__ void k_writeItems(const uint32_t* __restrict__ O, const uint32_t* __r
         }
         uint32_t s0, c0;
         childRange(O, pBase, pN, nItems0, lg2b, v, s0, c0);
-        const uint32_t base = cBase[v], ni = (c0 + kTQ - 1) / kTQ;
+        const uint32_t base = cBase[v], ni = (c0 + itemRows - 1) / itemRows;
         for (uint32_t t = threadIdx.x; t < ni; t += blockDim.x)
-            items[base + t] = LkItem{v, s0 + t * kTQ, min(kTQ, c0 - t * kTQ), base, ni, 0u};
+            items[base + t] = LkItem{v, s0 + t * itemRows, min(itemRows, c0 - t * itemRows), base, ni, 0u};
     }
 }
 
@@ -561,7 +563,7 @@ __global__ __launch_bounds__(256) void k_lookupFusedI8(const uint8_t* __restrict
 // certified filter keeps every child that can be the argmin; near ties are
 // queued per wave and resolved with the plain FP32 chain at the item's end.
 //
-template <bool kIdentity, bool kLast, int B>
+template <bool kIdentity, bool kLast, int B, uint32_t NQ = kTQ>
 __global__ __launch_bounds__(256, 2) void k_levelSplit(  // two waves per SIMD: at most 256 registers
     const float* __restrict__ rows32, const int8_t* __restrict__ split, uint64_t stride,
     const float* __restrict__ cn2, const float* __restrict__ e1, const float* __restrict__ e2, uint64_t levelBase,
@@ -570,25 +572,26 @@ __global__ __launch_bounds__(256, 2) void k_levelSplit(  // two waves per SIMD: 
     uint32_t* __restrict__ C, uint32_t* __restrict__ nearTies) {
     constexpr int CB = B / 16;
     constexpr uint32_t LG2B = B == 32 ? 5 : 4;
-    static_assert(kTQ == 256, "an item is four groups of four 16-row blocks");
-    __shared__ uint8_t sjAll[4][kTQ];
+    constexpr int NG = int(NQ / 64);  // an item: NG groups of four 16-row blocks
+    static_assert(NQ % 64 == 0 && (kLast || NQ == kTQ) && (!kIdentity || NQ == kTQ), "item size");
+    __shared__ uint8_t sjAll[4][NQ];
     __shared__ uint32_t scAll[4][32];
-    __shared__ TieQueueT<kTQ> tqAll[4];  // a whole item's rows: resolved once, at the item's end
+    __shared__ TieQueueT<NQ> tqAll[4];  // a whole item's rows: resolved once, at the item's end
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 15, grp = lane >> 4;
     uint8_t* sj = sjAll[wave];
-    TieQueueT<kTQ>& tq = tqAll[wave];
+    TieQueueT<NQ>& tq = tqAll[wave];
     const uint32_t nbItems = kIdentity ? nbItemsImplicit : *nbItemsPtr;
     uint32_t ties = 0, nTie = 0;
     for (uint32_t it = blockIdx.x * 4 + wave; it < nbItems; it += gridDim.x * 4) {
         const LkItem t = itemAt<kIdentity>(it, n, items);
         const uint64_t node = levelBase + t.node, childBase = node * B;
-        // the item's descriptors: lane l holds those of rows g*64 + l, g = 0..3 (group g)
+        // the item's descriptors: lane l holds those of rows g*64 + l, g < NG (group g)
         // (a row past the item's end stands for its last row, a valid descriptor: its column
         // of the products is discarded and nothing is written for it)
-        uint32_t qv[4];
+        uint32_t qv[NG];
 #pragma unroll
-        for (int g = 0; g < 4; g++) {
+        for (int g = 0; g < NG; g++) {
             const uint32_t r = min(g * 64 + lane, t.cnt - 1u);
             qv[g] = kIdentity ? t.start + r : perm[t.start + r];
         }
@@ -639,14 +642,13 @@ __global__ __launch_bounds__(256, 2) void k_levelSplit(  // two waves per SIMD: 
                     if (rb * 16 >= t.cnt) return;
                     const uint32_t r = rb * 16 + col, q = rowOf(qv[0], S);
                     const bool valid = r < t.cnt;
-                    // ||x - 128||^2 of row (lane & 15): an exact integer, summed over the row's four lanes
-                    int xs = 0;
-#pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        xs = __builtin_amdgcn_sdot4(x0[e], x0[e], xs, false);
-                        xs = __builtin_amdgcn_sdot4(x1[e], x1[e], xs, false);
-                    }
-                    xs = sumAcrossGroups(xs);
+                    // ||x - 128||^2 of row col: the diagonal of the rows' Gram matrix (exact int32) on the
+                    // matrix cores, which have slack here.  The rows are A and B alike, so lane l holds
+                    // G[4 (l >> 4) + i][l & 15]: G[r][r] is component r & 3 of lane 16 (r >> 2) + r.
+                    v4i gm = __builtin_amdgcn_mfma_i32_16x16x64_i8(x0, x0, v4i{0, 0, 0, 0}, 0, 0, 0);
+                    gm = __builtin_amdgcn_mfma_i32_16x16x64_i8(x1, x1, gm, 0, 0, 0);
+                    const int gOwn = (col & 3) == 0 ? gm[0] : (col & 3) == 1 ? gm[1] : (col & 3) == 2 ? gm[2] : gm[3];
+                    const int xs = __shfl(gOwn, int(16 * (col >> 2) + col), 64);
                     v4f acc[CB];
 #pragma unroll
                     for (int cb = 0; cb < CB; cb++) {
@@ -666,12 +668,12 @@ __global__ __launch_bounds__(256, 2) void k_levelSplit(  // two waves per SIMD: 
                         if (kLast) out[q] = t.node * B + best;
                         else sj[r] = uint8_t(best);
                     }
-                    ties += tiePush(tq, nTie, lane, tie, q, kLast ? q : r, mask);
+                    if (__ballot(tie))  // rare: most blocks skip the queue's bookkeeping
+                        ties += tiePush(tq, nTie, lane, tie, q, kLast ? q : r, mask);
                 },
                 std::make_integer_sequence<int, 4>{});
-            qv[0] = qv[1];
-            qv[1] = qv[2];
-            qv[2] = qv[3];
+#pragma unroll
+            for (int g2 = 0; g2 + 1 < NG; g2++) qv[g2] = qv[g2 + 1];
         }
         if (nTie)  // near ties: the plain chain over the queued rows' candidates
             resolveTies(
@@ -709,6 +711,16 @@ uint32_t residentBlocks() {
     return blocks;
 }
 
+template <bool kIdentity, bool kLast, int B, uint32_t NQ>
+void launchSplit(const TreeView& t, uint32_t grid, uint64_t levelBase, const uint8_t* desc, uint32_t n,
+                 const uint32_t* perm, const LkItem* items, const uint32_t* nbItemsPtr, uint32_t itemsMax, uint32_t* out,
+                 uint32_t* C, uint32_t* nearTies, hipStream_t s) {
+    const uint32_t g = std::min(grid, residentBlocks<k_levelSplit<kIdentity, kLast, B, NQ>>());
+    k_levelSplit<kIdentity, kLast, B, NQ><<<g, 256, 0, s>>>(t.f32rows, t.split, t.splitStride, t.cn2, t.e1, t.e2,
+                                                            levelBase, desc, n, perm, items, nbItemsPtr, itemsMax, out,
+                                                            C, nearTies);
+}
+
 template <bool kIdentity, bool kLast>
 void launchLevel(const TreeView& t, uint64_t levelBase, const uint8_t* desc, uint32_t n, const uint32_t* perm,
                  const LkItem* items, const uint32_t* nbItemsPtr, uint32_t itemsMax, uint32_t* out, uint32_t* C,
@@ -717,18 +729,17 @@ void launchLevel(const TreeView& t, uint64_t levelBase, const uint8_t* desc, uin
     if (t.centroidType == 0) {
         k_levelI8<kIdentity, kLast><<<grid, 256, 0, s>>>(t.u8rows, t.normS, levelBase, desc, n, perm, items, nbItemsPtr,
                                                        itemsMax, out, C);
-    } else if (t.b == 32) {
-        // only resident workgroups, each looping over its items: 3.6 % faster on the 10M-query
-        // b32 L4 lookup than one round of workgroups per 1.2 items
-        const uint32_t g = std::min(grid, residentBlocks<k_levelSplit<kIdentity, kLast, 32>>());
-        k_levelSplit<kIdentity, kLast, 32><<<g, 256, 0, s>>>(t.f32rows, t.split, t.splitStride, t.cn2, t.e1, t.e2,
-                                                          levelBase, desc, n, perm, items, nbItemsPtr, itemsMax, out,
-                                                          C, nearTies);
     } else {
-        const uint32_t g = std::min(grid, residentBlocks<k_levelSplit<kIdentity, kLast, 16>>());
-        k_levelSplit<kIdentity, kLast, 16><<<g, 256, 0, s>>>(t.f32rows, t.split, t.splitStride, t.cn2, t.e1, t.e2,
-                                                          levelBase, desc, n, perm, items, nbItemsPtr, itemsMax, out,
-                                                          C, nearTies);
+        // only resident workgroups, each looping over its items: 3.6 % faster on the 10M-query
+        // b32 L4 lookup than one round of workgroups per 1.2 items.  The leaf level's items
+        // hold up to kTQLeafF32 rows (itemRowsForLevel).
+        constexpr uint32_t NQ = kLast && !kIdentity ? kTQLeafF32 : kTQ;
+        if (t.b == 32)
+            launchSplit<kIdentity, kLast, 32, NQ>(t, grid, levelBase, desc, n, perm, items, nbItemsPtr, itemsMax, out, C,
+                                                  nearTies, s);
+        else
+            launchSplit<kIdentity, kLast, 16, NQ>(t, grid, levelBase, desc, n, perm, items, nbItemsPtr, itemsMax, out, C,
+                                                  nearTies, s);
     }
     RBOW_HIP(hipGetLastError());
 }
@@ -826,15 +837,18 @@ void lookupMfmaInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t
     RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, C, O, int(cLen), s));
     k_scatter<true><<<gridFor(items0, 4, 8192), 256, 0, s>>>(nullptr, nullptr, items0, n, lg2b, nullptr, code, O,
                                                             perm[0]);
-    auto buildItems = [&](const uint32_t* pBase, const uint32_t* pN, uint32_t nItems0, uint32_t nbChildren, int dst) {
+    // items of level l (the leaf level of a float tree: larger ones, launchLevel)
+    auto itemRowsForLevel = [&](uint32_t l) { return l + 1 == t.L && t.centroidType != 0 ? kTQLeafF32 : kTQ; };
+    auto buildItems = [&](const uint32_t* pBase, const uint32_t* pN, uint32_t nItems0, uint32_t nbChildren, int dst,
+                          uint32_t itemRows) {
         k_nodeItemCounts<<<gridFor(nbChildren + 1, 256, 4096), 256, 0, s>>>(O, pBase, pN, nItems0, nbChildren, lg2b,
-                                                                          nCnt[dst]);
+                                                                          itemRows, nCnt[dst]);
         RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, nCnt[dst], nBase[dst], int(nbChildren + 1), s));
-        k_writeItems<<<std::min(nbChildren + 1, 65536u), 64, 0, s>>>(O, pBase, pN, nItems0, nbChildren, lg2b,
+        k_writeItems<<<std::min(nbChildren + 1, 65536u), 64, 0, s>>>(O, pBase, pN, nItems0, nbChildren, lg2b, itemRows,
                                                                     nBase[dst], items[dst], nbItems);
         RBOW_HIP(hipGetLastError());
     };
-    buildItems(nullptr, nullptr, items0, b, 0);
+    buildItems(nullptr, nullptr, items0, b, 0, itemRowsForLevel(1));
     int cur = 0;
     for (uint32_t l = 1; l < t.L; l++) {
         const uint64_t nodesL = levelSize(lg2b, l);
@@ -851,7 +865,7 @@ void lookupMfmaInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t
         RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, C, O, int(cLen), s));
         k_scatter<false><<<gridFor(itemsL, 4, 8192), 256, 0, s>>>(items[cur], nbItems, 0, n, lg2b, perm[cur], code, O,
                                                                  perm[cur ^ 1]);
-        buildItems(nBase[cur], nCnt[cur], 0, uint32_t(nodesL * b), cur ^ 1);
+        buildItems(nBase[cur], nCnt[cur], 0, uint32_t(nodesL * b), cur ^ 1, itemRowsForLevel(l + 1));
         cur ^= 1;
     }
 }
