@@ -186,13 +186,16 @@ struct KernelTimer {
 bool launchKmReduce(const KmLevel& k, const KmPartRange* red, uint32_t nbRed, hipStream_t s,
                     const KernelTimer& t = {});
 // Whole Lloyd loops (assign + update, up to maxIters passes, per-task stop) of
-// the listed uint8 tasks, one workgroup each with the task's points in LDS;
-// leaves them inactive with their pass counts.  Every listed task has 1 to
-// maxTiles tiles (maxTiles <= lloydTasksTileCap(blocksPerCu), blocksPerCu = 1
-// or 2 workgroups sharing a CU); taskTileBeg[g] = first tile of task g in k.XA.
+// the listed uint8 tasks, one workgroup each; leaves them inactive with their
+// pass counts.  The first nbStream listed tasks (1 to kStreamTaskTiles tiles;
+// blocksPerCu = 1 only) read their tiles from k.XA every pass (L2-resident after
+// the first); the others have their points in LDS and 1 to maxTiles tiles
+// (maxTiles <= lloydTasksTileCap(blocksPerCu), blocksPerCu = 1 or 2 workgroups
+// sharing a CU).  taskTileBeg[g] = first tile of task g in k.XA.
+constexpr uint32_t kStreamTaskTiles = 128;
 void launchLloydTasks(const KmLevel& k, const uint32_t* taskList, uint32_t nbList, uint32_t maxTiles,
                       uint32_t blocksPerCu, const uint32_t* taskTileBeg, uint32_t maxIters, hipStream_t s,
-                      const KernelTimer& t = {});
+                      const KernelTimer& t = {}, uint32_t nbStream = 0);
 // Largest task (in 64-row tiles) whose points fit one workgroup's LDS when
 // blocksPerCu workgroups of launchLloydTasks share a CU.
 uint32_t lloydTasksTileCap(uint32_t blocksPerCu);

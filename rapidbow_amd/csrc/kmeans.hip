@@ -637,23 +637,27 @@ __device__ inline uint32_t roundedMean(uint32_t s, uint32_t c) {
     return q;
 }
 
-// Persistent Lloyd loop with the points on chip, one workgroup of W waves per
-// listed task (tasks of at most maxTiles 64-row tiles).  The task's tiles are
-// read from HBM once into LDS, row-major (dynamic: maxTiles x 9 KB, then one
-// assignment byte per row); every pass then takes both MFMA operands from
-// LDS, so the loop is not bound by HBM.  Per pass the waves take the tiles
-// round robin (the next tile's fragments in flight while one computes), their
-// one-hot sums and counts meet in LDS (int32 atomics), and the same rule as
-// k_updateU8Part applies: stop when no assignment changed, else move the
-// centers to round(float(sum)/float(count)).  Two barriers per pass: each
-// thread clears the sums it has just read, and the counters alternate between
-// two buffers.  Results equal those of the one-launch-per-pass path for any
-// pass cap.  At the end the assignments and centers go to HBM and the task is
-// marked inactive with its pass count.
+// Persistent Lloyd loops, one workgroup of W waves per listed task.  The first
+// nbStream listed tasks (up to kStreamTaskTiles 64-row tiles) stream: every pass
+// reads their tiles again from the gathered operand (XA), which the first pass
+// leaves in L2 and the memory-side cache, and each wave puts its current tile in
+// LDS for the transposing reads of the sums, as k_assignU8Mfma does.  The others
+// (at most maxTiles tiles) have their tiles read from HBM once into LDS,
+// row-major (dynamic: maxTiles x 9 KB, then one assignment byte per row), and
+// every pass takes both MFMA operands from LDS.  One launch holds both kinds,
+// largest tasks first, so short tasks fill the CUs while long ones run.  Per
+// pass the waves take the tiles round robin (the next tiles in flight while one
+// computes), their one-hot sums and counts meet in LDS (int32 atomics), and the
+// same rule as k_updateU8Part applies: stop when no assignment changed, else
+// move the centers to round(float(sum)/float(count)).  Two barriers per pass:
+// each thread clears the sums it has just read, and the counters alternate
+// between two buffers.  Results equal those of the one-launch-per-pass path for
+// any pass cap.  At the end the assignments and centers go to HBM and the task
+// is marked inactive with its pass count.
 template <int W>
 __global__ __launch_bounds__(64 * W) void k_lloydTasks(KmLevel k, const uint32_t* __restrict__ taskList,
                                                       const uint32_t* __restrict__ taskTileBeg, uint32_t maxTiles,
-                                                      uint32_t maxIters) {
+                                                      uint32_t maxIters, uint32_t nbStream) {
     constexpr uint32_t kThreads = 64 * W;
     extern __shared__ __align__(16) uint8_t dynLds[];
     __shared__ __align__(16) uint8_t sNew[W][64];
@@ -666,24 +670,29 @@ __global__ __launch_bounds__(64 * W) void k_lloydTasks(KmLevel k, const uint32_t
     const uint32_t g = taskList[blockIdx.x], tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 15, grp = lane >> 4;
     const uint32_t n = k.taskCnt[g], row0 = k.taskBeg[g], nt = (n + 63) / 64;
-    if (nt == 0 || nt > maxTiles) return;  // the host lists tasks of 1..maxTiles tiles only
-    uint8_t* sA = dynLds;                                   // [nt] row-major tiles of kTileLds bytes
-    uint8_t* sAsg = dynLds + uint64_t(maxTiles) * kTileLds;  // [nt * 64]
-    const uint4* src = reinterpret_cast<const uint4*>(k.XA + uint64_t(taskTileBeg[g]) * 8192);
-    // XA fragment (tile, rb, kb, lane) -> its rows in LDS; 8 loads in flight per thread
-    constexpr uint32_t kLoadBatch = 8;
-    for (uint32_t i0 = tid; i0 < nt * 512; i0 += kLoadBatch * kThreads) {
-        uint4 v[kLoadBatch];
+    const bool streamed = blockIdx.x < nbStream;  // block-uniform
+    if (nt == 0 || nt > (streamed ? kStreamTaskTiles : maxTiles)) return;  // the host lists only tasks that fit
+    // streamed: [W] wave tiles, then the assignments; resident: [nt] tiles, then the assignments
+    uint8_t* sA = dynLds;
+    uint8_t* sAsg = dynLds + uint64_t(streamed ? W : maxTiles) * kTileLds;
+    const int8_t* xa = k.XA + uint64_t(taskTileBeg[g]) * 8192;
+    if (!streamed) {
+        const uint4* src = reinterpret_cast<const uint4*>(xa);
+        // XA fragment (tile, rb, kb, lane) -> its rows in LDS; 8 loads in flight per thread
+        constexpr uint32_t kLoadBatch = 8;
+        for (uint32_t i0 = tid; i0 < nt * 512; i0 += kLoadBatch * kThreads) {
+            uint4 v[kLoadBatch];
 #pragma unroll
-        for (uint32_t j = 0; j < kLoadBatch; j++) {
-            const uint32_t i = i0 + j * kThreads;
-            v[j] = i < nt * 512 ? src[i] : uint4{0, 0, 0, 0};
-        }
+            for (uint32_t j = 0; j < kLoadBatch; j++) {
+                const uint32_t i = i0 + j * kThreads;
+                v[j] = i < nt * 512 ? src[i] : uint4{0, 0, 0, 0};
+            }
 #pragma unroll
-        for (uint32_t j = 0; j < kLoadBatch; j++) {
-            const uint32_t i = i0 + j * kThreads, c = i & 511;
-            if (i < nt * 512)
-                *reinterpret_cast<uint4*>(sA + (i >> 9) * kTileLds + fragOffset(c >> 7, (c >> 6) & 1, c & 63)) = v[j];
+            for (uint32_t j = 0; j < kLoadBatch; j++) {
+                const uint32_t i = i0 + j * kThreads, c = i & 511;
+                if (i < nt * 512)
+                    *reinterpret_cast<uint4*>(sA + (i >> 9) * kTileLds + fragOffset(c >> 7, (c >> 6) & 1, c & 63)) = v[j];
+            }
         }
     }
     for (uint32_t i = tid; i < nt * 64; i += kThreads) sAsg[i] = 0xFF;  // no previous assignment
@@ -703,22 +712,49 @@ __global__ __launch_bounds__(64 * W) void k_lloydTasks(KmLevel k, const uint32_t
 #pragma unroll
         for (int db = 0; db < 8; db++) acc[db] = zero;
         uint32_t myCnt = 0, myChanged = 0;
-        v4i a[4][2];
-        if (wave < nt) loadTileLds(sA + wave * kTileLds, lane, a);
-        for (uint32_t i = wave; i < nt; i += W) {
-            v4i an[4][2];
-            const bool more = i + W < nt;
-            if (more) loadTileLds(sA + (i + W) * kTileLds, lane, an);
+        auto tileDone = [&](uint32_t i, const v4i (&a)[4][2], const uint8_t* tileLds) {
             const uint32_t cnt = min(64u, n - 64 * i);
             const uint32_t prev = sAsg[i * 64 + lane];
-            const uint32_t mine =
-                assignTileMfma(a, sA + i * kTileLds, cnt, prev, ck, sNew[wave], lane, acc, myCnt, myChanged);
+            const uint32_t mine = assignTileMfma(a, tileLds, cnt, prev, ck, sNew[wave], lane, acc, myCnt, myChanged);
             if (lane < cnt) sAsg[i * 64 + lane] = uint8_t(mine);
-            if (more) {
+        };
+        if (streamed && wave < nt) {
+            // the wave's tiles wave, wave + W, ... through a ring of three (two in flight while
+            // one computes), unrolled by three so that every slot index is a constant; a request
+            // past the task's end repeats its last tile, so no load sits under a branch
+            v4i ring[3][4][2];
+            uint8_t* myTile = sA + wave * kTileLds;
+            auto request = [&](auto S, uint32_t i) { loadTileA(xa + uint64_t(min(i, nt - 1)) * 8192, lane, ring[S]); };
+            auto step = [&](auto S, uint32_t i) -> bool {
+                if (i >= nt) return false;
+                request(std::integral_constant<int, (S + 2) % 3>{}, i + 2 * W);
+                waveSync();  // the previous tile's transposing reads are done
+                storeTileLds(myTile, lane, ring[S]);
+                waveSync();
+                tileDone(i, ring[S], myTile);
+                return true;
+            };
+            request(std::integral_constant<int, 0>{}, wave);
+            request(std::integral_constant<int, 1>{}, wave + W);
+            for (uint32_t i = wave;; i += 3 * W) {
+                if (!step(std::integral_constant<int, 0>{}, i)) break;
+                if (!step(std::integral_constant<int, 1>{}, i + W)) break;
+                if (!step(std::integral_constant<int, 2>{}, i + 2 * W)) break;
+            }
+        } else if (wave < nt) {
+            v4i a[4][2];
+            loadTileLds(sA + wave * kTileLds, lane, a);
+            for (uint32_t i = wave; i < nt; i += W) {
+                v4i an[4][2];
+                const bool more = i + W < nt;
+                if (more) loadTileLds(sA + (i + W) * kTileLds, lane, an);
+                tileDone(i, a, sA + i * kTileLds);
+                if (more) {
 #pragma unroll
-                for (int rb = 0; rb < 4; rb++) {
-                    a[rb][0] = an[rb][0];
-                    a[rb][1] = an[rb][1];
+                    for (int rb = 0; rb < 4; rb++) {
+                        a[rb][0] = an[rb][0];
+                        a[rb][1] = an[rb][1];
+                    }
                 }
             }
         }
@@ -1502,20 +1538,26 @@ uint32_t lloydTasksTileCap(uint32_t blocksPerCu) {
 
 void launchLloydTasks(const KmLevel& k, const uint32_t* taskList, uint32_t nbList, uint32_t maxTiles,
                       uint32_t blocksPerCu, const uint32_t* taskTileBeg, uint32_t maxIters, hipStream_t s,
-                      const KernelTimer& t) {
+                      const KernelTimer& t, uint32_t nbStream) {
     if (!nbList) return;
-    RBOW_REQUIRE(maxTiles >= 1 && maxTiles <= lloydTasksTileCap(blocksPerCu), RBOW_ERR_INVALID_ARG,
-                 "persistent k-means tasks exceed the LDS capacity");
-    const int dyn = int(maxTiles * kLloydTileLds);
+    RBOW_REQUIRE(nbStream <= nbList && (nbStream == 0 || blocksPerCu <= 1), RBOW_ERR_INVALID_ARG,
+                 "streamed persistent tasks need a workgroup per CU");
+    RBOW_REQUIRE(nbStream == nbList || (maxTiles >= 1 && maxTiles <= lloydTasksTileCap(blocksPerCu)),
+                 RBOW_ERR_INVALID_ARG, "persistent k-means tasks exceed the LDS capacity");
     // one workgroup per CU: 8 waves (2 per SIMD); two per CU: 4 waves each
     if (blocksPerCu <= 1) {
+        const uint64_t resident = uint64_t(maxTiles) * kLloydTileLds;
+        const uint64_t streamed = nbStream ? 8ull * kTileLds + kStreamTaskTiles * 64ull : 0ull;
+        const int dyn = int(std::max(resident, streamed));
         RBOW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_lloydTasks<8>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
-        launch(k_lloydTasks<8>, nbList, 512, uint32_t(dyn), s, t, k, taskList, taskTileBeg, maxTiles, maxIters);
+        launch(k_lloydTasks<8>, nbList, 512, uint32_t(dyn), s, t, k, taskList, taskTileBeg, maxTiles, maxIters,
+               nbStream);
     } else {
+        const int dyn = int(maxTiles * kLloydTileLds);
         RBOW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_lloydTasks<4>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
-        launch(k_lloydTasks<4>, nbList, 256, uint32_t(dyn), s, t, k, taskList, taskTileBeg, maxTiles, maxIters);
+        launch(k_lloydTasks<4>, nbList, 256, uint32_t(dyn), s, t, k, taskList, taskTileBeg, maxTiles, maxIters, 0u);
     }
 }
 

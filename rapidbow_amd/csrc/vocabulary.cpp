@@ -348,7 +348,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
     } outGuard{outReady};
     bool preLaunched = false;  // this level's gather, seeding and reset are already on the stream
 
-    std::vector<uint32_t> taskTiles, tileBeg, smallTasks;
+    std::vector<uint32_t> taskTiles, tileBeg, smallTasks, midTasks;
     std::vector<KmPartRange> red;
     std::mt19937_64 rng(opts.seed);
     const uint32_t cus = deviceCUs();
@@ -409,6 +409,7 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         uint32_t smallTilesMax = 0, nbLarger = 0;  // smallTasks[0, nbLarger): more than persistTiles2 tiles
         const uint32_t* dSmall = nullptr;
         const uint32_t* dTileBeg = nullptr;
+        midTasks.clear();
         if (mfmaPath) {
             if (persistTiles) {
                 for (uint32_t g = 0; g < nG; g++)
@@ -422,8 +423,24 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
                     nbLarger += taskTiles[g] > persistTiles2;
                     taskTiles[g] = 0;  // not streamed
                 }
-                dSmall = up.put(smallTasks.data(), smallTasks.size());
                 dTileBeg = up.put(tileBeg.data(), nG);
+                // When every task left to stream has at most kStreamTaskTiles tiles, those tasks
+                // run their whole Lloyd loop in the persistent launch as well (tiles read through
+                // L2 each pass) and the level needs no launch per pass.
+                midTasks.clear();
+                uint32_t leftMax = 0;
+                for (uint32_t g = 0; g < nG; g++) leftMax = std::max(leftMax, taskTiles[g]);
+                if (leftMax && leftMax <= kStreamTaskTiles) {
+                    for (uint32_t g = 0; g < nG; g++)
+                        if (taskTiles[g]) midTasks.push_back(g);
+                    std::stable_sort(midTasks.begin(), midTasks.end(),  // largest first: they bound the launch
+                                     [&](uint32_t x, uint32_t y) { return taskTiles[x] > taskTiles[y]; });
+                    for (uint32_t g : midTasks) taskTiles[g] = 0;  // not streamed per pass
+                }
+                // one launch list: the streamed persistent tasks, then the LDS ones
+                std::vector<uint32_t> list(midTasks);
+                list.insert(list.end(), smallTasks.begin(), smallTasks.end());
+                dSmall = up.put(list.data(), list.size());
             }
             const KmPartition part = partitionKmTiles(taskTiles, tileBeg);
             red.clear();
@@ -446,13 +463,14 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
             k.partSums = ws.partSums.get();
             k.partCnt = ws.partCnt.get();
             k.partChanged = ws.partChanged.get();
-            if (!smallTasks.empty()) {  // the update visits the tasks that stream (and the empty ones)
+            if (!smallTasks.empty() || !midTasks.empty()) {  // the update visits the tasks that stream (and the empty ones)
+                const uint64_t nbPersist = smallTasks.size() + midTasks.size();
                 const uint32_t* dUpd = nullptr;
-                uint32_t* hUpd = up.take<uint32_t>(nG - smallTasks.size(), &dUpd);
+                uint32_t* hUpd = up.take<uint32_t>(nG - nbPersist, &dUpd);
                 uint32_t nbUpd = 0;
                 for (uint32_t g = 0; g < nG; g++)
                     if (taskTiles[g] || !taskCnt[g]) hUpd[nbUpd++] = g;
-                RBOW_REQUIRE(nbUpd == nG - smallTasks.size(), RBOW_ERR_HIP, "update task list size");
+                RBOW_REQUIRE(nbUpd == nG - nbPersist, RBOW_ERR_HIP, "update task list size");
                 k.updTasks = dUpd;
                 k.nbUpdTasks = nbUpd;
             }
@@ -498,15 +516,16 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         }
 
         // ---- Lloyd passes (kmeans.cpp:118-133) ----
-        if (!smallTasks.empty()) {
-            // two LDS classes: tasks that allow two workgroups per CU run in a launch of their own;
-            // timed from the start of the first launch to the end of the second
-            const uint32_t nbSmaller = uint32_t(smallTasks.size()) - nbLarger;
+        if (!smallTasks.empty() || !midTasks.empty()) {
+            // one launch holds the streamed tasks and the LDS tasks of one workgroup per CU (largest
+            // first); tasks that allow two workgroups per CU run in a launch of their own; timed
+            // from the start of the first launch to the end of the second
+            const uint32_t nbSmaller = uint32_t(smallTasks.size()) - nbLarger, nbMid = uint32_t(midTasks.size());
             const KernelTimer t = timer(0);
-            launchLloydTasks(k, dSmall, nbLarger, smallTilesMax, 1, dTileBeg, opts.max_iters, s,
-                             KernelTimer{t.start, nbSmaller ? nullptr : t.stop});
-            launchLloydTasks(k, dSmall + nbLarger, nbSmaller, std::min(smallTilesMax, persistTiles2), 2, dTileBeg,
-                             opts.max_iters, s, KernelTimer{nbLarger ? nullptr : t.start, t.stop});
+            launchLloydTasks(k, dSmall, nbMid + nbLarger, nbLarger ? smallTilesMax : 0u, 1, dTileBeg, opts.max_iters, s,
+                             KernelTimer{t.start, nbSmaller ? nullptr : t.stop}, nbMid);
+            launchLloydTasks(k, dSmall + nbMid + nbLarger, nbSmaller, std::min(smallTilesMax, persistTiles2), 2,
+                             dTileBeg, opts.max_iters, s, KernelTimer{nbLarger || nbMid ? nullptr : t.start, t.stop});
         }
         // with no streamed task left, one update pass retires the empty tasks
         const bool streamed = !mfmaPath || k.nbWaves > 0;
