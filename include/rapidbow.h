@@ -146,7 +146,9 @@ rbow_status rbow_voc_lookup_noalloc(const rbow_vocabulary* voc, const void* dev_
  * which the fused single-launch descent is at least as fast as the bucketed
  * one; then it sets the vocabulary's AUTO crossover to the smallest size at
  * which the MFMA form is at least as fast as the plain kernel (nb_desc + 1 if
- * never).  Blocks; does not change any leaf id.  rbow_voc_set_lookup_crossover
+ * never); with at least 65,536 descriptors it first picks the bucketed
+ * descent's leaf item size (rbow_voc_lookup_leaf_item_rows).  Blocks; does not
+ * change any leaf id.  rbow_voc_set_lookup_crossover
  * and rbow_voc_set_lookup_fused_max set them directly (0 restores the defaults). */
 rbow_status rbow_voc_tune_lookup(rbow_vocabulary* voc, const void* dev_desc, uint32_t nb_desc,
                                  uint32_t* out_crossover, rbow_stream stream);
@@ -157,6 +159,13 @@ uint32_t rbow_voc_lookup_crossover(const rbow_vocabulary* voc); /* current AUTO 
  * above (same leaf ids); 0 restores the default (196,608); 0 on other trees. */
 rbow_status rbow_voc_set_lookup_fused_max(rbow_vocabulary* voc, uint32_t fused_max);
 uint32_t rbow_voc_lookup_fused_max(const rbow_vocabulary* voc);
+/* (No reference counterpart.)  Tile shape of the bucketed MFMA descent: rows per
+ * work item at the leaf level, 256 or 512 (the levels above use 256).  The
+ * default is 512 on float trees, 256 on uint8 trees; rbow_voc_tune_lookup
+ * measures both on the caller's descriptors (batches of at least 65,536) and
+ * keeps the faster.  Leaf ids do not depend on it; 0 restores the default. */
+rbow_status rbow_voc_set_lookup_leaf_item_rows(rbow_vocabulary* voc, uint32_t rows);
+uint32_t rbow_voc_lookup_leaf_item_rows(const rbow_vocabulary* voc);
 
 /* RapidBoW.h:69 IVocabulary::serialize.  Two-call form: blob may be NULL to
  * query *size.  uint8/b16 trees use the reference blob byte layout
@@ -201,6 +210,17 @@ rbow_status rbow_db_add_docs_device(rbow_database* db, const uint32_t* host_doc_
 rbow_status rbow_db_query_batch(const rbow_database* db, const uint32_t* dev_leaf, const uint64_t* host_offsets,
                                 uint32_t nb_queries, uint32_t max_results, uint32_t* dev_docs, float* dev_scores,
                                 uint32_t* dev_counts, rbow_stream stream);
+/* (No reference counterpart.)  Scoring tile shape: steps of 64 postings each
+ * wave walks per iteration (its loads in flight per lane), 4 or 6 (0: the
+ * default, 6).  rbow_db_tune_scoring times one query batch (arguments as
+ * rbow_db_query_batch) with each, checks that they rank and score alike
+ * (scores within 2e-6), keeps the faster for the database and returns it with
+ * the two median times (out_ms[2], optional).  Neither changes a result. */
+rbow_status rbow_db_set_walk_steps(rbow_database* db, uint32_t steps);
+uint32_t rbow_db_walk_steps(const rbow_database* db);
+rbow_status rbow_db_tune_scoring(rbow_database* db, const uint32_t* dev_leaf, const uint64_t* host_offsets,
+                                 uint32_t nb_queries, uint32_t max_results, uint32_t* out_steps, float* out_ms,
+                                 rbow_stream stream);
 uint32_t rbow_db_nb_docs(const rbow_database* db);
 uint64_t rbow_db_nb_postings(const rbow_database* db);
 /* Incremental inserts (no counterpart in the reference, whose addDoc inserts

@@ -91,6 +91,8 @@ _sig("rbow_voc_set_lookup_crossover", _i32, [_vp, _u32])
 _sig("rbow_voc_lookup_crossover", _u32, [_vp])
 _sig("rbow_voc_set_lookup_fused_max", _i32, [_vp, _u32])
 _sig("rbow_voc_lookup_fused_max", _u32, [_vp])
+_sig("rbow_voc_set_lookup_leaf_item_rows", _i32, [_vp, _u32])
+_sig("rbow_voc_lookup_leaf_item_rows", _u32, [_vp])
 _sig("rbow_db_create", _i32, [_vp, ctypes.POINTER(_vp)])
 _sig("rbow_db_destroy", None, [_vp])
 _sig("rbow_db_add_doc", _i32, [_vp, _u32, _vp, _u32])
@@ -98,6 +100,9 @@ _sig("rbow_db_query", _i32, [_vp, _vp, _u32, _u32, _vp, _vp, ctypes.POINTER(_u32
 _sig("rbow_db_query_and_add_doc", _i32, [_vp, _u32, _vp, _u32, _u32, _vp, ctypes.POINTER(_u32)])
 _sig("rbow_db_add_docs_device", _i32, [_vp, _vp, _u32, _vp, _vp, _vp])
 _sig("rbow_db_query_batch", _i32, [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp])
+_sig("rbow_db_set_walk_steps", _i32, [_vp, _u32])
+_sig("rbow_db_walk_steps", _u32, [_vp])
+_sig("rbow_db_tune_scoring", _i32, [_vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp])
 _sig("rbow_db_nb_docs", _u32, [_vp])
 _sig("rbow_db_nb_postings", _u64, [_vp])
 _sig("rbow_db_nb_delta_postings", _u64, [_vp])
@@ -219,6 +224,15 @@ class Vocabulary:
     @lookup_fused_max.setter
     def lookup_fused_max(self, n):
         check(lib.rbow_voc_set_lookup_fused_max(self.handle, n))
+
+    @property
+    def lookup_leaf_item_rows(self):
+        """Rows per work item at the leaf level of the bucketed MFMA descent (256 or 512)."""
+        return lib.rbow_voc_lookup_leaf_item_rows(self.handle)
+
+    @lookup_leaf_item_rows.setter
+    def lookup_leaf_item_rows(self, n):
+        check(lib.rbow_voc_set_lookup_leaf_item_rows(self.handle, n))
 
     def serialize(self):
         size = _sz(0)
@@ -418,6 +432,25 @@ class DataBase:
         off = np.ascontiguousarray(offsets, np.uint64)
         check(lib.rbow_db_query_batch(self.handle, _ptr(dev_leaf), off.ctypes.data, off.size - 1, K, _ptr(dev_docs),
                                       _ptr(dev_scores), _ptr(dev_counts), _stream(stream)))
+
+    @property
+    def walk_steps(self):
+        """Scoring tile shape: steps of 64 postings per wave and walk iteration (4 or 6)."""
+        return lib.rbow_db_walk_steps(self.handle)
+
+    @walk_steps.setter
+    def walk_steps(self, n):
+        check(lib.rbow_db_set_walk_steps(self.handle, n))
+
+    def tune_scoring(self, dev_leaf, offsets, K, stream=None):
+        """Times one query batch with each scoring tile shape and keeps the faster:
+        returns (walk steps, {steps: median ms})."""
+        off = np.ascontiguousarray(offsets, np.uint64)
+        steps = ctypes.c_uint32()
+        ms = (ctypes.c_float * 2)()
+        check(lib.rbow_db_tune_scoring(self.handle, _ptr(dev_leaf), off.ctypes.data, off.size - 1, K,
+                                       ctypes.byref(steps), ms, _stream(stream)))
+        return steps.value, {4: ms[0], 6: ms[1]}
 
     @property
     def nb_docs(self):

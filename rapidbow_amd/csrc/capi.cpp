@@ -312,6 +312,7 @@ rbow_status rbow_voc_tune_lookup(rbow_vocabulary* voc, const void* dev_desc, uin
                                           hs(stream));
         voc->v->lookupAutoMin = r.crossover;
         if (r.fusedMax) voc->v->lookupFusedMax = r.fusedMax;
+        if (r.leafItemRows) voc->v->lookupLeafItemRows = r.leafItemRows;
         if (out_crossover) *out_crossover = r.crossover;
     });
 }
@@ -334,6 +335,20 @@ uint32_t rbow_voc_lookup_fused_max(const rbow_vocabulary* voc) {
     if (!voc) return 0;
     const TreeView t = voc->v->view();
     return t.fusedMax ? t.fusedMax : lookupFusedDefault(t);
+}
+
+rbow_status rbow_voc_set_lookup_leaf_item_rows(rbow_vocabulary* voc, uint32_t rows) {
+    return guarded([&] {
+        RBOW_REQUIRE(voc != nullptr, RBOW_ERR_INVALID_ARG, "null vocabulary");
+        RBOW_REQUIRE(rows == 0 || rows == 256 || rows == 512, RBOW_ERR_INVALID_ARG, "leaf item rows must be 256 or 512");
+        voc->v->lookupLeafItemRows = rows;
+    });
+}
+
+uint32_t rbow_voc_lookup_leaf_item_rows(const rbow_vocabulary* voc) {
+    if (!voc) return 0;
+    const TreeView t = voc->v->view();
+    return t.leafItemRows ? t.leafItemRows : lookupLeafItemRowsDefault(t);
 }
 
 uint32_t rbow_voc_lookup_crossover(const rbow_vocabulary* voc) {
@@ -545,6 +560,84 @@ rbow_status rbow_db_query_batch(const rbow_database* db, const uint32_t* dev_lea
         RBOW_REQUIRE(dev_docs && dev_counts && max_results > 0, RBOW_ERR_INVALID_ARG, "null output");
         db->db->queryBatch(dev_leaf, host_offsets, nb_queries, max_results, dev_docs, dev_scores, dev_counts,
                            hs(stream));
+    });
+}
+
+rbow_status rbow_db_set_walk_steps(rbow_database* db, uint32_t steps) {
+    return guarded([&] {
+        RBOW_REQUIRE(db != nullptr, RBOW_ERR_INVALID_ARG, "null database");
+        db->db->setWalkSteps(steps);
+    });
+}
+
+uint32_t rbow_db_walk_steps(const rbow_database* db) { return db ? db->db->walkSteps() : 0u; }
+
+rbow_status rbow_db_tune_scoring(rbow_database* db, const uint32_t* dev_leaf, const uint64_t* host_offsets,
+                                 uint32_t nb_queries, uint32_t max_results, uint32_t* out_steps, float* out_ms,
+                                 rbow_stream stream) {
+    return guarded([&] {
+        RBOW_REQUIRE(db != nullptr, RBOW_ERR_INVALID_ARG, "null database");
+        RBOW_REQUIRE(nb_queries > 0 && max_results > 0 && host_offsets && dev_leaf, RBOW_ERR_INVALID_ARG,
+                     "the tuner needs a query batch");
+        static const uint32_t kCandidates[2] = {4u, 6u};
+        const uint64_t slots = uint64_t(nb_queries) * max_results;
+        DevBuf<uint32_t> docs(slots), counts(nb_queries), docs0(slots), counts0(nb_queries);
+        DevBuf<float> scores(slots), scores0(slots);
+        hipStream_t s = hs(stream);
+        hipEvent_t e0, e1;
+        RBOW_HIP(hipEventCreate(&e0));
+        RBOW_HIP(hipEventCreate(&e1));
+        const uint32_t before = db->db->walkSteps();
+        float best = 0.0f;
+        uint32_t bestSteps = before;
+        try {
+            for (int c = 0; c < 2; c++) {
+                db->db->setWalkSteps(kCandidates[c]);
+                // one warm run (its results kept for the comparison), then the median of three
+                db->db->queryBatch(dev_leaf, host_offsets, nb_queries, max_results, c ? docs.get() : docs0.get(),
+                                   c ? scores.get() : scores0.get(), c ? counts.get() : counts0.get(), s);
+                float ms[3];
+                for (int r = 0; r < 3; r++) {
+                    RBOW_HIP(hipEventRecord(e0, s));
+                    db->db->queryBatch(dev_leaf, host_offsets, nb_queries, max_results, docs.get(), scores.get(),
+                                       counts.get(), s);
+                    RBOW_HIP(hipEventRecord(e1, s));
+                    RBOW_HIP(hipEventSynchronize(e1));
+                    RBOW_HIP(hipEventElapsedTime(&ms[r], e0, e1));
+                }
+                std::sort(ms, ms + 3);
+                if (out_ms) out_ms[c] = ms[1];
+                if (c == 0 || ms[1] < best) {
+                    best = ms[1];
+                    bestSteps = kCandidates[c];
+                }
+            }
+            // the tile shape changes no result: both candidates' rankings and scores are compared
+            std::vector<uint32_t> a(slots), b(slots), ca(nb_queries), cb(nb_queries);
+            std::vector<float> sa(slots), sb(slots);
+            RBOW_HIP(hipMemcpy(a.data(), docs0.get(), 4 * slots, hipMemcpyDeviceToHost));
+            RBOW_HIP(hipMemcpy(b.data(), docs.get(), 4 * slots, hipMemcpyDeviceToHost));
+            RBOW_HIP(hipMemcpy(sa.data(), scores0.get(), 4 * slots, hipMemcpyDeviceToHost));
+            RBOW_HIP(hipMemcpy(sb.data(), scores.get(), 4 * slots, hipMemcpyDeviceToHost));
+            RBOW_HIP(hipMemcpy(ca.data(), counts0.get(), 4ull * nb_queries, hipMemcpyDeviceToHost));
+            RBOW_HIP(hipMemcpy(cb.data(), counts.get(), 4ull * nb_queries, hipMemcpyDeviceToHost));
+            bool same = ca == cb;
+            for (uint32_t q = 0; same && q < nb_queries; q++)
+                for (uint32_t i = 0; i < ca[q]; i++) {
+                    const uint64_t x = uint64_t(q) * max_results + i;
+                    same = same && a[x] == b[x] && std::fabs(sa[x] - sb[x]) <= 2e-6f;
+                }
+            RBOW_REQUIRE(same, RBOW_ERR_HIP, "scoring tile shapes gave different results");
+        } catch (...) {
+            db->db->setWalkSteps(before == Database::kWalkStepsDefault ? 0u : before);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        RBOW_HIP(hipEventDestroy(e0));
+        RBOW_HIP(hipEventDestroy(e1));
+        db->db->setWalkSteps(bestSteps);
+        if (out_steps) *out_steps = bestSteps;
     });
 }
 

@@ -57,7 +57,15 @@ DbView Database::view() const {
     v.dir = dirStale_ ? nullptr : dir_.get();  // a stale directory is never handed to a kernel
     v.Rm = dirR_;
     v.R = divUp(docsOnDevice_, kRangeDocs);
+    v.walkSteps = walkSteps();
     return v;
+}
+
+void Database::setWalkSteps(uint32_t steps) {
+    RBOW_REQUIRE(steps == 0 || steps == 4 || steps == 6, RBOW_ERR_INVALID_ARG, "walk steps must be 4 or 6");
+    std::unique_lock<std::shared_mutex> lk(mtx_);  // exclusive, as an insert
+    if (steps != walkSteps_) version_++;
+    walkSteps_ = steps;
 }
 
 void Database::growDocs(uint32_t nb, hipStream_t s) const {

@@ -490,7 +490,6 @@ __global__ __launch_bounds__(1024) void k_orderQueries(const uint32_t* __restric
 // The range then keeps its K smallest (score, doc id) keys as candidates for
 // the query's final selection.  Query blockIdx.y of the launch is order[y]
 // when an order is given.
-constexpr int kWalkSteps = 6;  // steps of 64 postings per walk iteration: loads in flight per lane
 constexpr uint32_t kInnerCap = 4 * kSegBatch / 16;  // per-wave list of an inner node's in-range postings (u32)
 
 __device__ inline bool scored(double acc) { return __double_as_longlong(acc) != (long long)0x8000000000000000ull; }
@@ -507,13 +506,16 @@ __device__ inline uint32_t postingAt(uint64_t vbase, uint32_t pos) {
     return *reinterpret_cast<GlobalU32>(vbase + 4ull * pos);
 }
 
+// U: steps of 64 postings per walk iteration (the loads in flight per lane), the
+// scoring tile shape the database's tuner picks (4 or 6: at 8 the walk's
+// registers no longer fit 64 VGPRs, the budget of 8 waves per SIMD).
+template <int U>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_scoreRange(DbView v, const uint64_t* __restrict__ off, uint32_t q0,
                                                     const QNode* __restrict__ nodes,
                                                     const uint32_t* __restrict__ nodeCount,
                                                     const uint32_t* __restrict__ order, uint32_t K,
                                                     uint64_t* __restrict__ candKeys, uint32_t* __restrict__ candCount) {
     constexpr uint32_t S = Database::kRangeDocs;
-    constexpr int U = kWalkSteps;
     __shared__ double sAcc[S];                 // reused as the keys of the selection
     __shared__ float sRcp[S];
     // per segment: the address of its posting at concatenated position 0, then (qE, w) of its node;
@@ -941,9 +943,15 @@ void launchQueryBatchInWs(const DbView& v, const uint32_t* dLeaf, const uint64_t
     const bool ordered = nbQueries > 1 && nbQueries <= kOrderCap;
     if (ordered) k_orderQueries<<<1, 1024, 0, s>>>(work, nbQueries, order);
     // ranges fastest, so the tiles of one query run together and share its node list in L2
-    for (uint32_t q0 = 0; v.R && q0 < nbQueries; q0 += 65535)
-        k_scoreRange<<<dim3(v.R, std::min(nbQueries - q0, 65535u)), 1024, 0, s>>>(
-            v, dOffsets + q0, q0, nodes, nodeCount, ordered ? order : nullptr, K, candKeys, candCount);
+    for (uint32_t q0 = 0; v.R && q0 < nbQueries; q0 += 65535) {
+        const dim3 grid(v.R, std::min(nbQueries - q0, 65535u));
+        if (v.walkSteps == 4)
+            k_scoreRange<4><<<grid, 1024, 0, s>>>(v, dOffsets + q0, q0, nodes, nodeCount, ordered ? order : nullptr, K,
+                                                 candKeys, candCount);
+        else
+            k_scoreRange<6><<<grid, 1024, 0, s>>>(v, dOffsets + q0, q0, nodes, nodeCount, ordered ? order : nullptr, K,
+                                                 candKeys, candCount);
+    }
     k_topk<<<nbQueries, 1024, 0, s>>>(v, K, candKeys, candCount, dDocs, dScores, dCounts);
     RBOW_HIP(hipGetLastError());
 }

@@ -52,6 +52,7 @@ struct DbView {  // device view used by the scoring kernels
     const uint32_t* dir;       // [leaf][Rm + 1] range directory of the main CSR
     uint32_t Rm;               // document ranges of the main CSR
     uint32_t R;                // document ranges of all device slots
+    uint32_t walkSteps;        // scoring tile shape: steps of 64 postings per wave iteration (4 or 6)
 };
 
 // A leaf-major CSR of document slots on the device.
@@ -89,6 +90,11 @@ public:
     DbView scoringView() const;
     uint64_t version() const;
     bool hasPendingInserts() const;  // inserts not yet folded into the device state
+    // Scoring tile shape: steps of 64 postings per wave and walk iteration, 4 or 6
+    // (0: the default, 6).  A change bumps version() (captured graphs bake it in).
+    void setWalkSteps(uint32_t steps);
+    uint32_t walkSteps() const { return walkSteps_ ? walkSteps_ : kWalkStepsDefault; }
+    static constexpr uint32_t kWalkStepsDefault = 6;
     // HBM scratch of one query of n descriptors, top-K, during scoring
     uint64_t scratchBytes(uint32_t n, uint32_t K) const {
         return uint64_t(divUp(nbDocs(), kRangeDocs)) * (8ull * K + 4) + uint64_t(n) * L_ * kQNodeBytes + 80;
@@ -136,6 +142,7 @@ private:
     mutable bool dirStale_ = true;
     mutable DevBuf<uint32_t> flag_;  // device error flag of the leaf-id checks
     mutable uint64_t version_ = 0;   // changes with every change of the scoring view
+    uint32_t walkSteps_ = 0;         // scoring tile shape (0: default)
     mutable std::vector<uint32_t> pendLeaf_, pendSlot_;
     mutable std::shared_mutex mtx_;
     mutable std::mutex mergeMtx_;

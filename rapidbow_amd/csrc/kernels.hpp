@@ -31,7 +31,13 @@ struct TreeView {
     uint32_t autoMin = 0;  // AUTO lookup: batches below this use the plain kernel (0: built-in default)
     uint32_t fusedMax = 0; // uint8 b16 trees: MFMA-form batches of at most this many descriptors run the
                            // fused single-launch descent, larger ones the bucketed one (0: built-in default)
+    uint32_t leafItemRows = 0;  // bucketed descent: rows per work item at the leaf level, 256 or 512
+                                // (0: built-in default, lookupLeafItemRowsDefault)
 };
+// Default leaf-level item size of the bucketed descent: 512 rows on float trees
+// (a leaf node's split centroids, 12 KB at b = 32, are then read once rather
+// than once per item), 256 on uint8 trees.
+uint32_t lookupLeafItemRowsDefault(const TreeView& t);
 
 // Derived per-child data for nodes [0, nbNodes): uint8 trees' norms; for float
 // trees (lookups, and the float k-means assign every pass) the int8 split of
@@ -78,8 +84,9 @@ void launchLookupInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32
 // as the bucketed one, then, with that MFMA form, the smallest size at which it
 // is at least as fast as the plain kernel.  Blocks; uses `leaf` (n words) as output.
 struct LookupTuning {
-    uint32_t crossover;  // AUTO: the MFMA form from this many descriptors on (n + 1: never)
-    uint32_t fusedMax;   // uint8 b16 trees: the fused descent up to this many descriptors (0: not applicable)
+    uint32_t crossover;     // AUTO: the MFMA form from this many descriptors on (n + 1: never)
+    uint32_t fusedMax;      // uint8 b16 trees: the fused descent up to this many descriptors (0: not applicable)
+    uint32_t leafItemRows;  // bucketed descent: leaf-level item size, 256 or 512 (0: not measured)
 };
 LookupTuning tuneLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s);
 

@@ -36,8 +36,7 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int kScoreBias = 1 << 25;   // makes integer scores positive before key packing
 constexpr uint32_t kTQ = 256;         // descriptors per MFMA work item (16 row blocks of 16)
-constexpr uint32_t kTQLeafF32 = 512;  // items of a float tree's leaf level: most leaf nodes are then one item, so
-                                      // a node's split centroids (12 KB at b = 32) are read once, not once per item
+constexpr uint32_t kTQLeaf = 512;  // the larger leaf-level item size (lookupLeafItemRowsDefault)
 
 // A work item: up to kTQ descriptors that sit at the same node.  Items of a
 // node are consecutive; itemBase/nItems locate the node's items so that the
@@ -722,7 +721,8 @@ void launchSplit(const TreeView& t, uint32_t grid, uint64_t levelBase, const uin
 }
 
 template <bool kIdentity, bool kLast>
-void launchLevel(const TreeView& t, uint64_t levelBase, const uint8_t* desc, uint32_t n, const uint32_t* perm,
+void launchLevel(const TreeView& t, uint32_t itemRows, uint64_t levelBase, const uint8_t* desc, uint32_t n,
+                 const uint32_t* perm,
                  const LkItem* items, const uint32_t* nbItemsPtr, uint32_t itemsMax, uint32_t* out, uint32_t* C,
                  uint32_t* nearTies, hipStream_t s) {
     const uint32_t grid = gridFor(itemsMax, 4, 8192);
@@ -732,14 +732,22 @@ void launchLevel(const TreeView& t, uint64_t levelBase, const uint8_t* desc, uin
     } else {
         // only resident workgroups, each looping over its items: 3.6 % faster on the 10M-query
         // b32 L4 lookup than one round of workgroups per 1.2 items.  The leaf level's items
-        // hold up to kTQLeafF32 rows (itemRowsForLevel).
-        constexpr uint32_t NQ = kLast && !kIdentity ? kTQLeafF32 : kTQ;
-        if (t.b == 32)
-            launchSplit<kIdentity, kLast, 32, NQ>(t, grid, levelBase, desc, n, perm, items, nbItemsPtr, itemsMax, out, C,
-                                                  nearTies, s);
-        else
-            launchSplit<kIdentity, kLast, 16, NQ>(t, grid, levelBase, desc, n, perm, items, nbItemsPtr, itemsMax, out, C,
-                                                  nearTies, s);
+        // hold up to itemRows rows (256 or 512).
+        constexpr bool kLeaf = kLast && !kIdentity;
+        if (kLeaf && itemRows == kTQLeaf) {
+            if (t.b == 32)
+                launchSplit<kIdentity, kLast, 32, kLeaf ? kTQLeaf : kTQ>(t, grid, levelBase, desc, n, perm, items,
+                                                                           nbItemsPtr, itemsMax, out, C, nearTies, s);
+            else
+                launchSplit<kIdentity, kLast, 16, kLeaf ? kTQLeaf : kTQ>(t, grid, levelBase, desc, n, perm, items,
+                                                                           nbItemsPtr, itemsMax, out, C, nearTies, s);
+        } else if (t.b == 32) {
+            launchSplit<kIdentity, kLast, 32, kTQ>(t, grid, levelBase, desc, n, perm, items, nbItemsPtr, itemsMax, out,
+                                                   C, nearTies, s);
+        } else {
+            launchSplit<kIdentity, kLast, 16, kTQ>(t, grid, levelBase, desc, n, perm, items, nbItemsPtr, itemsMax, out,
+                                                   C, nearTies, s);
+        }
     }
     RBOW_HIP(hipGetLastError());
 }
@@ -833,12 +841,13 @@ void lookupMfmaInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t
     // level 0: all descriptors against the root
     uint64_t cLen = uint64_t(items0) * b + 1;
     launchFillU32(C, 0u, cLen, s);
-    launchLevel<true, false>(t, 0, desc, n, nullptr, nullptr, nullptr, items0, code, C, nearTies, s);
+    launchLevel<true, false>(t, kTQ, 0, desc, n, nullptr, nullptr, nullptr, items0, code, C, nearTies, s);
     RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, C, O, int(cLen), s));
     k_scatter<true><<<gridFor(items0, 4, 8192), 256, 0, s>>>(nullptr, nullptr, items0, n, lg2b, nullptr, code, O,
                                                             perm[0]);
-    // items of level l (the leaf level of a float tree: larger ones, launchLevel)
-    auto itemRowsForLevel = [&](uint32_t l) { return l + 1 == t.L && t.centroidType != 0 ? kTQLeafF32 : kTQ; };
+    // items of level l: kTQ rows, at the leaf level the vocabulary's leaf item size (256 or 512)
+    const uint32_t leafRows = t.leafItemRows ? t.leafItemRows : lookupLeafItemRowsDefault(t);
+    auto itemRowsForLevel = [&](uint32_t l) { return l + 1 == t.L ? leafRows : kTQ; };
     auto buildItems = [&](const uint32_t* pBase, const uint32_t* pN, uint32_t nItems0, uint32_t nbChildren, int dst,
                           uint32_t itemRows) {
         k_nodeItemCounts<<<gridFor(nbChildren + 1, 256, 4096), 256, 0, s>>>(O, pBase, pN, nItems0, nbChildren, lg2b,
@@ -854,13 +863,13 @@ void lookupMfmaInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t
         const uint64_t nodesL = levelSize(lg2b, l);
         const uint32_t itemsL = uint32_t(items0 + nodesL);
         if (l + 1 == t.L) {
-            launchLevel<false, true>(t, levelBeg(lg2b, l), desc, n, perm[cur], items[cur], nbItems, itemsL, leaf,
+            launchLevel<false, true>(t, leafRows, levelBeg(lg2b, l), desc, n, perm[cur], items[cur], nbItems, itemsL, leaf,
                                      nullptr, nearTies, s);
             break;
         }
         cLen = uint64_t(itemsL) * b + 1;
         launchFillU32(C, 0u, cLen, s);
-        launchLevel<false, false>(t, levelBeg(lg2b, l), desc, n, perm[cur], items[cur], nbItems, itemsL, code, C,
+        launchLevel<false, false>(t, kTQ, levelBeg(lg2b, l), desc, n, perm[cur], items[cur], nbItems, itemsL, code, C,
                                   nearTies, s);
         RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, C, O, int(cLen), s));
         k_scatter<false><<<gridFor(itemsL, 4, 8192), 256, 0, s>>>(items[cur], nbItems, 0, n, lg2b, perm[cur], code, O,
@@ -943,7 +952,7 @@ void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32
         return;
     }
     if (t.L == 1) {
-        launchLevel<true, true>(t, 0, desc, n, nullptr, nullptr, nullptr, (n + kTQ - 1) / kTQ, leaf, nullptr,
+        launchLevel<true, true>(t, kTQ, 0, desc, n, nullptr, nullptr, nullptr, (n + kTQ - 1) / kTQ, leaf, nullptr,
                                 nearTies, s);
         return;
     }
@@ -1010,6 +1019,8 @@ void launchLookup(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* 
     else launchLookupMfma(t, desc, n, leaf, nearTies, s);
 }
 
+uint32_t lookupLeafItemRowsDefault(const TreeView& t) { return t.centroidType != 0 ? kTQLeaf : kTQ; }
+
 LookupTuning tuneLookup(const TreeView& t0, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s) {
     hipEvent_t e0, e1;
     RBOW_HIP(hipEventCreate(&e0));
@@ -1032,9 +1043,22 @@ LookupTuning tuneLookup(const TreeView& t0, const uint8_t* desc, uint32_t n, uin
         std::sort(ms, ms + 3);
         return ms[1];
     };
-    LookupTuning r{n + 1, 0u};
+    LookupTuning r{n + 1, 0u, 0u};
     try {
         TreeView t = t0;
+        if (t.L >= 2 && n >= 65536) {
+            // the bucketed descent's leaf-level item size, on the largest power-of-two prefix (at
+            // most 2^22 descriptors): 512-row items read a leaf node's centroids once where
+            // 256-row ones read them once per item, but leave more rows of a block unused
+            TreeView a = t, b = t;
+            a.fusedMax = b.fusedMax = 1u;  // the bucketed form
+            a.leafItemRows = kTQ;
+            b.leafItemRows = kTQLeaf;
+            uint32_t m = 65536;
+            while (uint64_t(m) * 2 <= n && m < (1u << 22)) m *= 2;
+            r.leafItemRows = timeOne(b, m, RBOW_LOOKUP_MFMA) <= timeOne(a, m, RBOW_LOOKUP_MFMA) ? kTQLeaf : kTQ;
+            t.leafItemRows = r.leafItemRows;
+        }
         if (t.L >= 1 && lookupFusedDefault(t)) {
             // the MFMA form's fused/bucketed boundary: the largest size (powers of two from
             // 4,096) at which the fused descent is still at least as fast as the bucketed one

@@ -65,6 +65,7 @@ TreeView Vocabulary::view() const {
     t.e2 = e2.get();
     t.autoMin = lookupAutoMin;
     t.fusedMax = lookupFusedMax;
+    t.leafItemRows = lookupLeafItemRows;
     return t;
 }
 

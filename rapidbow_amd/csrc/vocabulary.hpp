@@ -18,6 +18,7 @@ struct Vocabulary {
     std::vector<float> weights;    // [node][b] IDF weights (Vocabulary.cpp:103-108), host like the reference
     uint32_t lookupAutoMin = 0;    // tuned AUTO crossover (0: untuned)
     uint32_t lookupFusedMax = 0;   // tuned fused/bucketed crossover of the MFMA form (0: untuned)
+    uint32_t lookupLeafItemRows = 0;  // tuned leaf-level item size of the bucketed descent (0: untuned)
 
     void allocate(uint32_t b_, uint32_t L_, uint32_t centroidType_);
     // derived per-child data used by the kernels (norms, int8 split, bounds)

@@ -269,10 +269,26 @@ def test_db_many_document_ranges():
             gd, gs = g.query(q, K)
             od, os_ = o.query(q, K)
             assert_ranked_equal(gd, gs, od, os_)
-    res = g.query_batch(np.concatenate(qs), np.concatenate([[0], np.cumsum([x.size for x in qs])]).astype(np.uint64), 60)
-    for q, (gd, gs) in zip(qs, res):
-        od, os_ = o.query(q, 60)
-        assert_ranked_equal(gd, gs, od, os_)
+    qcat = np.concatenate(qs)
+    qoff = np.concatenate([[0], np.cumsum([x.size for x in qs])]).astype(np.uint64)
+    assert rb.lib.rbow_db_walk_steps(g.h) == 6  # the default scoring tile shape
+    for steps in (6, 4):  # both scoring tile shapes: 6 and 4 steps of 64 postings per wave iteration
+        rb.check(rb.lib.rbow_db_set_walk_steps(g.h, steps))
+        assert rb.lib.rbow_db_walk_steps(g.h) == steps
+        res = g.query_batch(qcat, qoff, 60)
+        for q, (gd, gs) in zip(qs, res):
+            od, os_ = o.query(q, 60)
+            assert_ranked_equal(gd, gs, od, os_)
+    # the tuner times both on this batch, checks they agree, and keeps the faster
+    d = rb.DeviceArray.from_numpy(qcat)
+    best, ms = ctypes.c_uint32(), (ctypes.c_float * 2)()
+    rb.check(rb.lib.rbow_db_tune_scoring(g.h, ctypes.c_void_p(d.ptr), qoff.ctypes.data_as(ctypes.c_void_p),
+                                         ctypes.c_uint32(len(qs)), ctypes.c_uint32(60), ctypes.byref(best), ms, None))
+    assert best.value in (4, 6) and rb.lib.rbow_db_walk_steps(g.h) == best.value and ms[0] > 0 and ms[1] > 0
+    assert best.value == (4 if ms[0] < ms[1] else 6)
+    assert rb.lib.rbow_db_set_walk_steps(g.h, 8) == rb.ERR_INVALID_ARG
+    rb.check(rb.lib.rbow_db_set_walk_steps(g.h, 0))
+    assert rb.lib.rbow_db_walk_steps(g.h) == 6
 
 
 def test_db_subnormal_weights_read_as_zero():

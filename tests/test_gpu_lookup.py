@@ -192,6 +192,42 @@ def test_lookup_autotuner_keeps_leaf_ids():
     assert voc.lookup_crossover == 12288
 
 
+@pytest.mark.parametrize("b,L,ct", [(32, 3, 1), (16, 3, 1), (16, 3, 0)])
+def test_lookup_leaf_item_rows(b, L, ct):
+    """The bucketed descent's leaf-level tile shape (256- or 512-row work items)
+    gives the oracle's leaf ids either way, on float and uint8 trees, with leaf
+    nodes holding from a few rows to more than one item; the autotuner keeps
+    one of the two; other values are rejected and 0 restores the default."""
+    rng = np.random.default_rng(41 + b + ct)
+    desc = rb.synth_descriptors(20000, seed=9)
+    n = level_beg(b, L)
+    if ct:
+        rows = _f32_tree(rng, b, L, desc)
+        voc = rb.deserialize_vocabulary(rbw2_blob(b, L, 1, rows, np.full(n, 0xFFFFFFFF), np.ones((n, b))))
+    else:
+        rows = desc[rng.integers(0, desc.shape[0], n * 16)].reshape(n, 16, 128)
+        nodes = oracle.rows_to_ref_nodes(rows)
+        voc = rb.deserialize_vocabulary(ref_blob(L, nodes, np.full(n, 0xFFFF), np.ones((n, 16), np.float32)))
+    assert voc.lookup_leaf_item_rows == (512 if ct else 256)
+    q = rb.synth_descriptors(150_000, seed=10)  # ~37 rows per leaf at b32 L3, ~37 per node below level 2 at b16
+    q[:2000] = q[0]                            # one leaf node holding more than 512 rows
+    expect = oracle.lookup_f32(rows, b, L, q) if ct else oracle.lookup_u8(nodes, L, q)
+    voc.lookup_fused_max = 1  # the bucketed descent on uint8 b16 trees too
+    for rows_per_item in (256, 512):
+        voc.lookup_leaf_item_rows = rows_per_item
+        assert voc.lookup_leaf_item_rows == rows_per_item
+        np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_MFMA), expect)
+    d = rb.DeviceArray.from_numpy(q)
+    voc.tune_lookup(d, q.shape[0])
+    assert voc.lookup_leaf_item_rows in (256, 512)
+    np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_MFMA), expect)
+    with pytest.raises(rb.RbowError) as e:
+        voc.lookup_leaf_item_rows = 384
+    assert e.value.status == rb.ERR_INVALID_ARG
+    voc.lookup_leaf_item_rows = 0
+    assert voc.lookup_leaf_item_rows == (512 if ct else 256)
+
+
 def test_lookup_u8_fused_bucketed_plain_agree():
     """The MFMA form's two descents on uint8 b16 trees -- the fused single launch
     (small batches) and the level-synchronous bucketed one -- and the plain
