@@ -260,6 +260,12 @@ def bench_retrieval(rb, syn, v32, with_cpu, threads):
     cn = rb.DeviceArray((B,), np.uint32)
     maxb = max(int(qoff[(bi + 1) * B] - qoff[bi * B]) for bi in range(NB))
     pipe = rb.QueryPipeline(v32, db, B, maxb, K)
+    # scoring tile shape (walk depth) tuned on the first batch, kept for the database
+    o = qoff[0:B + 1]
+    v32.lookup(qdesc.view(int(o[0]), int(o[-1] - o[0])), int(o[-1] - o[0]), qleaf.view(int(o[0]), int(o[-1] - o[0])),
+               rb.LOOKUP_MFMA)
+    walk_steps, walk_ms = db.tune_scoring(qleaf, o, K)
+    log(f"bench: scoring tile shape: walk steps {walk_steps}, batch ms per candidate {walk_ms}")
     hip.synchronize()
     lat, res_docs, res_sc = [], [], []
     lat_p = {False: [], True: []}
@@ -300,7 +306,9 @@ def bench_retrieval(rb, syn, v32, with_cpu, threads):
                         "graph_batch_ms_p99": round(float(np.percentile(lat_p[True], 99)), 3),
                         "graph_queries_per_s": round(B * NB / (np.sum(lat_p[True]) * 1e-3), 1),
                         "graphs_captured": int(pipe.graphs), "same_results": same_pipe},
-           "self_first": float(np.mean([d.size > 0 and d[0] == g for d, g in zip(res_docs, qimg)]))}
+           "self_first": float(np.mean([d.size > 0 and d[0] == g for d, g in zip(res_docs, qimg)])),
+           "scoring_tile_tuned": {"walk_steps": walk_steps,
+                                  "batch_ms_per_candidate": {str(c): round(m, 3) for c, m in walk_ms.items()}}}
     if with_cpu:
         import oracle
         log("bench: retrieval oracle DB (host)")
@@ -604,7 +612,9 @@ def main():
         "lookup_b16L3_u8_50k": {"mfma_desc_per_s": round(r1_m, 1), "plain_desc_per_s": round(r1_p, 1),
                                 "identical_leaf_ids": id1},
         "lookup_auto_crossover_tuned": {"b16L3_u8": cross_u8, "b32L4_f32": cross_f32,
-                                        "b16L3_u8_fused_max": fused_u8},
+                                        "b16L3_u8_fused_max": fused_u8,
+                                        "b32L4_f32_leaf_item_rows": v32.lookup_leaf_item_rows,
+                                        "b16L3_u8_leaf_item_rows": v1.lookup_leaf_item_rows},
         "build_autotuned": {"persist_max_tiles": tuned_tiles,
                             "build_ms_per_candidate": {str(c): round(m, 3) for c, m in tune_ms.items()}},
     }
