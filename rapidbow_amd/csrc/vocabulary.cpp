@@ -604,17 +604,18 @@ Vocabulary* buildVocabulary(const uint8_t* devDesc, uint32_t nbDesc, uint32_t nb
         std::memcpy(voc->weights.data() + nodeBase * b, hMasks + nG, 4ull * nG * b);
         if (opts.seed_mode == 1)
             for (uint32_t g = 0; g < nG; g++) voc->masks[nodeBase + g] = hMasks[g];
-        std::vector<uint8_t> isSmall(nG, 0);
-        for (uint32_t g : smallTasks) isSmall[g] = 1;
+        std::vector<uint8_t> persistent(nG, 0);  // tasks of the persistent launches, not of the streamed assign
+        for (uint32_t g : smallTasks) persistent[g] = 1;
+        for (uint32_t g : midTasks) persistent[g] = 1;
         for (uint32_t g = 0; g < nG; g++) {
             passesAll[nodeBase + g] = hPasses[g];
             pointPasses += uint64_t(taskCnt[g]) * hPasses[g];
-            if (!isSmall[g]) streamPointPasses += uint64_t(taskCnt[g]) * hPasses[g];
+            if (!persistent[g]) streamPointPasses += uint64_t(taskCnt[g]) * hPasses[g];
         }
         if (streamed) streamLaunches += nbLaunched;
         if (timing) {
             float pm = 0;
-            if (!smallTasks.empty()) RBOW_HIP(hipEventElapsedTime(&pm, ev[0], ev[1]));
+            if (!smallTasks.empty() || !midTasks.empty()) RBOW_HIP(hipEventElapsedTime(&pm, ev[0], ev[1]));
             for (uint32_t it = 0; it < nbLaunched; it++) {
                 const size_t e = 2 + 6 * size_t(it);
                 float a = 0, r = 0, u = 0;

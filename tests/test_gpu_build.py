@@ -120,9 +120,17 @@ def test_build_u8_persistent_and_streamed_tasks(iters):
     _check_u8(voc, res, L)
     st = voc.build_stats()
     assert st["persist_ms"] > 0 and st["assign_launches"] > 0  # both paths ran
+    # the streamed assign processed exactly levels 0-2 (each holds a task above 128 tiles and
+    # fewer small tasks than the device has CUs); level 3 ran in the persistent launches
+    sizes = [np.array([desc.shape[0]])] + [res["occ"][level_beg(16, l - 1):level_beg(16, l)].ravel().astype(np.int64)
+                                           for l in (1, 2)]
+    per_level = [int((sizes[l] * res["passes"][level_beg(16, l):level_beg(16, l + 1)]).sum()) for l in range(3)]
+    assert all(((z + 63) // 64).max() > 128 for z in sizes)
+    assert st["stream_point_passes"] == sum(per_level) < st["point_passes"]
     # the same tree with every task streamed, one launch per pass
     voc0 = _gpu_build(desc, nb_doc, L, max_iters=iters, seed_mode=1, kernel=0, persist_max_tiles=0, timing=True)
-    assert voc0.build_stats()["persist_ms"] == 0
+    st0 = voc0.build_stats()
+    assert st0["persist_ms"] == 0 and st0["stream_point_passes"] == st0["point_passes"] == st["point_passes"]
     assert voc0.serialize() == voc.serialize()
 
 
