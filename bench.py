@@ -16,10 +16,10 @@ below the I8 MFMA ridge of ~625 ops/byte).
 Beside it on the same JSON line:
   * the same build with the plain scalar kernel (assign TOPS, "MFMA vs plain");
   * the build of a 1,048,576-word tree (branch 32, depth 4, float32
-    centroids) on the 2M descriptors, BF16-MFMA-filtered k-means vs plain
-    FP32 chains, with the identity check of the trees;
+    centroids) on the 2M descriptors, k-means with the int8-split MFMA
+    candidate filter vs plain FP32 chains, with the identity check of the trees;
   * leaf lookups/s on that tree for 10,000,000
-    queries: split-BF16 MFMA candidate filter + FP32 re-score vs the plain FP32
+    queries: int8-split MFMA candidate filter + FP32 re-score vs the plain FP32
     kernel, with the identity check of the leaf ids and the near-tie count;
   * leaf lookups/s on a 4,096-word uint8 tree (branch 16, depth 3) for 50,000
     queries (I8 MFMA vs plain);
@@ -542,7 +542,7 @@ def main():
     st32p = v32p.build_stats()
     same32 = v32p.serialize() == v32.serialize()
     del v32p
-    log(f"bench: b32 L4 float build: BF16-MFMA filter {st32['total_ms']:.1f} ms, plain {st32p['total_ms']:.1f} ms, "
+    log(f"bench: b32 L4 float build: int8-split MFMA filter {st32['total_ms']:.1f} ms, plain {st32p['total_ms']:.1f} ms, "
         f"identical={same32}")
     NQ = 10_000_000
     q = syn.descriptors(NQ, seed=SEED_QUERY)
