@@ -74,16 +74,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def timed_lookup(rb, voc, q, n, out, kernel, reps, ties=None):
-    voc.lookup(q, n, out, kernel, ties)  # warm
+def timed_lookup(rb, voc, q, n, out, kernel, reps, ties=None, rounds=3):
+    """Steady-state rate: three warm lookups (the first also counts near ties), then
+    `rounds` rounds of `reps` lookups; the median round's rate (the clocks take a few
+    tens of ms of load to settle, so a lone round after one warm call reads low)."""
+    voc.lookup(q, n, out, kernel, ties)
+    for _ in range(2):
+        voc.lookup(q, n, out, kernel)
     rb.hip.synchronize()
     e0, e1 = rb.Event(), rb.Event()
-    e0.record()
-    for _ in range(reps):
-        voc.lookup(q, n, out, kernel)
-    e1.record()
-    e1.synchronize()
-    return n * reps / (e0.elapsed_ms(e1) * 1e-3)
+    rates = []
+    for _ in range(rounds):
+        e0.record()
+        for _ in range(reps):
+            voc.lookup(q, n, out, kernel)
+        e1.record()
+        e1.synchronize()
+        rates.append(n * reps / (e0.elapsed_ms(e1) * 1e-3))
+    return float(np.median(rates))
 
 
 def copy_rate(rb, reps=5):

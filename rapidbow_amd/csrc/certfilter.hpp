@@ -78,12 +78,14 @@ constexpr float kTauScale = 2.0f * (1.0f + 1.0f / 1024.0f) * (1.0f + 1.0f / 1638
 // threshold (m + tau) and no test s <= thr.
 __device__ inline float minF(float a, float b) { return __builtin_fminf(a, b); }
 
-// Minimum over lanes l, l ^ 16, l ^ 32, l ^ 48, in every lane.
+// Minimum over lanes l, l ^ 16, l ^ 32, l ^ 48, in every lane.  IEEE minimum
+// (v_minimum3_f32): fminf would quiet each swapped-in value first (two more
+// instructions per step), which finite scores do not need.
 __device__ inline float minAcrossGroupsF(float v) {
     const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
-    v = minF(__uint_as_float(p[0]), __uint_as_float(p[1]));
+    v = __builtin_elementwise_minimum(__uint_as_float(p[0]), __uint_as_float(p[1]));
     const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
-    return minF(__uint_as_float(q[0]), __uint_as_float(q[1]));
+    return __builtin_elementwise_minimum(__uint_as_float(q[0]), __uint_as_float(q[1]));
 }
 __device__ inline uint32_t orAcrossGroups(uint32_t v) {
     const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);

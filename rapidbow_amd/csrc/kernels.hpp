@@ -46,6 +46,9 @@ uint32_t lookupLeafItemRowsDefault(const TreeView& t);
 void launchPrepU8Tree(const uint8_t* rows, uint64_t nbChildren, int32_t* normU, int32_t* normS, hipStream_t s);
 void launchPrepF32Split(const float* rows, uint64_t nbNodes, uint32_t b, int8_t* split, uint64_t splitStride,
                         float* cn2, float* e1, float* e2, hipStream_t s, const uint32_t* active = nullptr);
+// Children equal (bit for bit) to a lower sibling get cn2 = +inf: never a
+// candidate of the float lookup's filter (the lowest of equal children wins).
+void launchMarkDuplicateChildren(const float* rows, uint64_t nbNodes, uint32_t b, float* cn2, hipStream_t s);
 
 // Plain scalar per-descriptor descent (vocTreeLookUp.cu:48-160 semantics).
 void launchLookupPlain(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t* leaf, hipStream_t s);

@@ -53,6 +53,14 @@ struct LkItem {
     uint32_t pad;
 };
 
+// The leaf level's near-tie list: row[i] (the descriptor) and mask[i] (its
+// candidate children) for i < *count.
+struct LeafTies {
+    uint32_t* row;
+    uint32_t* mask;
+    uint32_t* count;
+};
+
 __device__ inline v4i ld16(const uint8_t* p) { return *reinterpret_cast<const v4i*>(p); }
 __device__ inline v4i toSigned(v4i v) { return v ^ v4i{int(0x80808080), int(0x80808080), int(0x80808080), int(0x80808080)}; }
 
@@ -132,6 +140,33 @@ __global__ __launch_bounds__(256) void k_prepF32Split(const float* __restrict__ 
         cn2[child] = bad ? 0.0f : float(v[0]);
         e1[child] = bad ? INFINITY : float((2.0 * sqrt(v[1]) + 4.0 * u * (sqrt(v[2]) + sqrt(v[3]))) * slack);
         e2[child] = bad ? 0.0f : float(2.0 * u * v[0] * slack);
+    }
+}
+
+// A child whose float row equals that of a lower sibling, bit for bit, has the
+// same plain FP32 distance to every descriptor, so with ties going to the lowest
+// child it is never the argmin.  Its norm is set to +inf: its filter score is
+// +inf, it is never a candidate, and rows that reach a node with repeated
+// children (k-means on a node with fewer points than children seeds repeated
+// centres, which stay equal) are not near ties.  The lowest of equal children
+// keeps its score, so the minimum and the threshold are unchanged.
+__global__ __launch_bounds__(256) void k_markDuplicateChildren(const float* __restrict__ rows, uint64_t nbChildren,
+                                                              uint32_t bMask, float* __restrict__ cn2) {
+    for (uint64_t child = uint64_t(blockIdx.x) * 256 + threadIdx.x; child < nbChildren;
+         child += uint64_t(gridDim.x) * 256) {
+        const uint32_t j = uint32_t(child) & bMask;
+        const uint4* rj = reinterpret_cast<const uint4*>(rows + child * kDims);
+        bool dup = false;
+        for (uint32_t i = 0; i < j && !dup; i++) {
+            const uint4* ri = reinterpret_cast<const uint4*>(rows + (child - j + i) * kDims);
+            uint32_t k = 0;
+            for (; k < kDims / 4; k++) {
+                const uint4 x = ri[k], y = rj[k];
+                if (x.x != y.x || x.y != y.y || x.z != y.z || x.w != y.w) break;
+            }
+            dup = k == kDims / 4;
+        }
+        if (dup) cn2[child] = INFINITY;
     }
 }
 
@@ -272,6 +307,15 @@ __device__ inline void itemCounts(uint32_t lane, uint32_t lg2b, uint32_t it, con
     waveSync();
 }
 
+// The scan of a level's counts runs over itemsMax items (the bound known on the
+// host): the entries past the last item, and the tail entry, are zeroed here by
+// the level kernel's threads once its items are done.
+__device__ inline void zeroCountTail(uint32_t* __restrict__ C, uint32_t nbItems, uint32_t itemsMax, uint32_t b) {
+    const uint64_t end = uint64_t(itemsMax) * b + 1, step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(nbItems) * b + uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < end; i += step)
+        C[i] = 0;
+}
+
 // Moves every descriptor of an item to its place in the next level's order.
 template <bool kIdentity>
 __global__ __launch_bounds__(256) void k_scatter(const LkItem* __restrict__ items,
@@ -331,12 +375,15 @@ __global__ void k_nodeItemCounts(const uint32_t* __restrict__ O, const uint32_t*
 __global__ void k_writeItems(const uint32_t* __restrict__ O, const uint32_t* __restrict__ pBase,
                              const uint32_t* __restrict__ pN, uint32_t nItems0, uint32_t nb, uint32_t lg2b,
                              uint32_t itemRows, const uint32_t* __restrict__ cBase, LkItem* __restrict__ items,
-                             uint32_t* __restrict__ nbItems) {
+                             uint32_t* __restrict__ nbItems, uint32_t* __restrict__ tieCount) {
     // one node per block: its items are written by the block's threads (a node
     // near the root holds thousands of items)
     for (uint32_t v = blockIdx.x; v <= nb; v += gridDim.x) {
         if (v == nb) {
-            if (threadIdx.x == 0) *nbItems = cBase[nb];
+            if (threadIdx.x == 0) {
+                *nbItems = cBase[nb];
+                *tieCount = 0;  // the leaf level's near-tie list starts empty
+            }
             continue;
         }
         uint32_t s0, c0;
@@ -415,6 +462,7 @@ __global__ __launch_bounds__(256) void k_levelI8(const uint8_t* __restrict__ row
         }
         if (!kLast) itemCounts(lane, 4, it, t, sj, scAll[wave], out, C);
     }
+    if (!kLast) zeroCountTail(C, nbItems, nbItemsImplicit, 16);
 }
 
 // ------------------------------------------- fused I8 descent (b = 16) --
@@ -568,10 +616,11 @@ __global__ __launch_bounds__(256, 2) void k_levelSplit(  // two waves per SIMD: 
     const float* __restrict__ cn2, const float* __restrict__ e1, const float* __restrict__ e2, uint64_t levelBase,
     const uint8_t* __restrict__ desc, uint32_t n, const uint32_t* __restrict__ perm, const LkItem* __restrict__ items,
     const uint32_t* __restrict__ nbItemsPtr, uint32_t nbItemsImplicit, uint32_t* __restrict__ out,
-    uint32_t* __restrict__ C, uint32_t* __restrict__ nearTies) {
+    uint32_t* __restrict__ C, uint32_t* __restrict__ nearTies, LeafTies leafTies) {
     constexpr int CB = B / 16;
     constexpr uint32_t LG2B = B == 32 ? 5 : 4;
     constexpr int NG = int(NQ / 64);  // an item: NG groups of four 16-row blocks
+    constexpr bool kTieList = kLast && !kIdentity;  // the leaf level of a tree with L >= 2 lists its near ties
     static_assert(NQ % 64 == 0 && (kLast || NQ == kTQ) && (!kIdentity || NQ == kTQ), "item size");
     __shared__ uint8_t sjAll[4][NQ];
     __shared__ uint32_t scAll[4][32];
@@ -662,10 +711,10 @@ __global__ __launch_bounds__(256, 2) void k_levelSplit(  // two waves per SIMD: 
                     }
                     const uint32_t mask = filterSplit<CB>(acc, float(xs), cn, e1m, e2m, lane, valid);
                     const bool tie = (mask & (mask - 1u)) != 0;  // only valid rows can have several candidates
-                    if (grp == 0 && valid && !tie) {
-                        const uint32_t best = __builtin_ctz(mask);
-                        if (kLast) out[q] = t.node * B + best;
-                        else sj[r] = uint8_t(best);
+                    if (kLast) {  // a listed near tie's lowest candidate stands until k_leafTies replaces it
+                        if (grp == 0 && valid && (kTieList || !tie)) out[q] = t.node * B + __builtin_ctz(mask);
+                    } else if (grp == 0 && valid && !tie) {
+                        sj[r] = uint8_t(__builtin_ctz(mask));
                     }
                     if (__ballot(tie))  // rare: most blocks skip the queue's bookkeeping
                         ties += tiePush(tq, nTie, lane, tie, q, kLast ? q : r, mask);
@@ -674,7 +723,22 @@ __global__ __launch_bounds__(256, 2) void k_levelSplit(  // two waves per SIMD: 
 #pragma unroll
             for (int g2 = 0; g2 + 1 < NG; g2++) qv[g2] = qv[g2 + 1];
         }
-        if (nTie)  // near ties: the plain chain over the queued rows' candidates
+        if (kTieList && nTie) {
+            // leaf level: the item's near ties go to the level's list (one atomic per item with
+            // ties), resolved by k_leafTies after this launch, so the wave moves on at once
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(leafTies.count, nTie);
+            base = uint32_t(__builtin_amdgcn_readfirstlane(int(__shfl(int(base), 0, 64))));
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the queue writes of this wave are visible
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t i = lane; i < nTie; i += 64) {
+                leafTies.row[base + i] = tq.tag[i];
+                leafTies.mask[base + i] = tq.mask[i];
+            }
+            nTie = 0;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // reads done before the queue is reused
+            __builtin_amdgcn_wave_barrier();
+        } else if (nTie) {  // near ties: the plain chain over the queued rows' candidates
             resolveTies(
                 tq, nTie, lane, [&](uint32_t q) { return desc + uint64_t(q) * kDims; },
                 [&](uint32_t c) { return rows32 + (childBase + c) * kDims; },
@@ -682,9 +746,76 @@ __global__ __launch_bounds__(256, 2) void k_levelSplit(  // two waves per SIMD: 
                     if (kLast) out[aux] = t.node * B + best;  // aux: the descriptor
                     else sj[aux] = uint8_t(best);             // aux: the row in the item
                 });
+        }
         if (!kLast) itemCounts(lane, LG2B, it, t, sj, scAll[wave], out, C);
     }
+    if (!kLast) zeroCountTail(C, nbItems, nbItemsImplicit, B);
     if (nearTies && ties && lane == 0) atomicAdd(nearTies, ties);  // ties: rows of this wave (uniform)
+}
+
+// The leaf level's near ties (row, candidate mask), listed by k_levelSplit<.., kLast>.
+// One lane per (row, candidate) pair: a wave takes 16 rows, spreads their
+// candidates over its lanes (64 per round) and computes each pair's plain FP32
+// chain; the row's lane then takes the lexicographic (distance, child) minimum
+// of its candidates.  The leaf kernel wrote node * B + (lowest candidate) for
+// the row, which gives the node.
+constexpr uint32_t kTieRows = 16;
+template <int B>
+__global__ __launch_bounds__(256) void k_leafTies(const float* __restrict__ rows32, uint64_t levelBase,
+                                                 const uint8_t* __restrict__ desc, LeafTies leafTies,
+                                                 uint32_t* __restrict__ out) {
+    __shared__ float sD[4][kTieRows * B];  // a wave's pair distances, by pair index
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t n = *leafTies.count, waves = gridDim.x * 4;
+    float* dist = sD[wave];
+    for (uint32_t e0 = (blockIdx.x * 4 + wave) * kTieRows; e0 < n; e0 += waves * kTieRows) {
+        // lane j < 16: row e0 + j, its candidates and their offset among the chunk's pairs
+        const bool live = lane < kTieRows && e0 + lane < n;
+        const uint32_t q = live ? leafTies.row[e0 + lane] : 0u, mask = live ? leafTies.mask[e0 + lane] : 0u;
+        const uint32_t node = live ? out[q] / B : 0u, nc = __builtin_popcount(mask);
+        uint32_t off = nc;  // inclusive prefix over lanes 0..15, then exclusive
+#pragma unroll
+        for (int d = 1; d < int(kTieRows); d <<= 1) {
+            const uint32_t o = uint32_t(__shfl_up(int(off), d, 64));
+            if (lane >= uint32_t(d)) off += o;
+        }
+        const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(off), int(kTieRows) - 1));
+        off -= nc;
+        for (uint32_t p0 = 0; p0 < total; p0 += 64) {
+            const uint32_t p = p0 + lane;
+            // the pair's row: the last of the 16 whose offset is <= p
+            uint32_t j = 0;
+#pragma unroll
+            for (int k = 1; k < int(kTieRows); k++)
+                if (uint32_t(__builtin_amdgcn_readlane(int(off), k)) <= p) j = uint32_t(k);
+            // every lane takes part in the shuffles: one from a lane that is off in this round
+            // (p >= total) would read a stale value
+            const uint32_t mj = uint32_t(__shfl(int(mask), int(j), 64)), qj = uint32_t(__shfl(int(q), int(j), 64));
+            const uint32_t nj = uint32_t(__shfl(int(node), int(j), 64)), oj = uint32_t(__shfl(int(off), int(j), 64));
+            if (p < total) {
+                uint32_t mm = mj;  // the (p - oj)-th candidate of row j (p - oj < its candidate count <= B)
+                for (uint32_t r = min(p - oj, uint32_t(B) - 1u); r; r--) mm &= mm - 1u;
+                const uint32_t c = __builtin_ctz(mm | (1u << (B - 1)));  // masks hold bits below B only
+                dist[p] = plainF32Distance(desc + uint64_t(qj) * kDims, rows32 + ((levelBase + nj) * B + c) * kDims);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the distances of this wave are visible
+        __builtin_amdgcn_wave_barrier();
+        if (live) {  // candidates in ascending child order: the first minimum is the lowest child
+            float bd = INFINITY;
+            uint32_t bj = __builtin_ctz(mask), mm = mask;
+            for (uint32_t k = 0; k < nc; k++, mm &= mm - 1u) {
+                const float a = dist[off + k];
+                if (a < bd) {
+                    bd = a;
+                    bj = __builtin_ctz(mm);
+                }
+            }
+            out[q] = node * B + bj;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // reads done before the next chunk's writes
+        __builtin_amdgcn_wave_barrier();
+    }
 }
 
 uint32_t gridFor(uint64_t work, uint32_t perBlock, uint32_t cap) {
@@ -713,18 +844,23 @@ uint32_t residentBlocks() {
 template <bool kIdentity, bool kLast, int B, uint32_t NQ>
 void launchSplit(const TreeView& t, uint32_t grid, uint64_t levelBase, const uint8_t* desc, uint32_t n,
                  const uint32_t* perm, const LkItem* items, const uint32_t* nbItemsPtr, uint32_t itemsMax, uint32_t* out,
-                 uint32_t* C, uint32_t* nearTies, hipStream_t s) {
+                 uint32_t* C, uint32_t* nearTies, const LeafTies& lt, hipStream_t s) {
     const uint32_t g = std::min(grid, residentBlocks<k_levelSplit<kIdentity, kLast, B, NQ>>());
     k_levelSplit<kIdentity, kLast, B, NQ><<<g, 256, 0, s>>>(t.f32rows, t.split, t.splitStride, t.cn2, t.e1, t.e2,
                                                             levelBase, desc, n, perm, items, nbItemsPtr, itemsMax, out,
-                                                            C, nearTies);
+                                                            C, nearTies, lt);
+    if (kLast && !kIdentity) {  // the leaf level's near ties
+        RBOW_HIP(hipGetLastError());
+        k_leafTies<B><<<std::min(grid, residentBlocks<k_leafTies<B>>()), 256, 0, s>>>(t.f32rows, levelBase, desc, lt,
+                                                                                     out);
+    }
 }
 
 template <bool kIdentity, bool kLast>
 void launchLevel(const TreeView& t, uint32_t itemRows, uint64_t levelBase, const uint8_t* desc, uint32_t n,
                  const uint32_t* perm,
                  const LkItem* items, const uint32_t* nbItemsPtr, uint32_t itemsMax, uint32_t* out, uint32_t* C,
-                 uint32_t* nearTies, hipStream_t s) {
+                 uint32_t* nearTies, const LeafTies& lt, hipStream_t s) {
     const uint32_t grid = gridFor(itemsMax, 4, 8192);
     if (t.centroidType == 0) {
         k_levelI8<kIdentity, kLast><<<grid, 256, 0, s>>>(t.u8rows, t.normS, levelBase, desc, n, perm, items, nbItemsPtr,
@@ -737,16 +873,18 @@ void launchLevel(const TreeView& t, uint32_t itemRows, uint64_t levelBase, const
         if (kLeaf && itemRows == kTQLeaf) {
             if (t.b == 32)
                 launchSplit<kIdentity, kLast, 32, kLeaf ? kTQLeaf : kTQ>(t, grid, levelBase, desc, n, perm, items,
-                                                                           nbItemsPtr, itemsMax, out, C, nearTies, s);
+                                                                           nbItemsPtr, itemsMax, out, C, nearTies, lt,
+                                                                           s);
             else
                 launchSplit<kIdentity, kLast, 16, kLeaf ? kTQLeaf : kTQ>(t, grid, levelBase, desc, n, perm, items,
-                                                                           nbItemsPtr, itemsMax, out, C, nearTies, s);
+                                                                           nbItemsPtr, itemsMax, out, C, nearTies, lt,
+                                                                           s);
         } else if (t.b == 32) {
             launchSplit<kIdentity, kLast, 32, kTQ>(t, grid, levelBase, desc, n, perm, items, nbItemsPtr, itemsMax, out,
-                                                   C, nearTies, s);
+                                                   C, nearTies, lt, s);
         } else {
             launchSplit<kIdentity, kLast, 16, kTQ>(t, grid, levelBase, desc, n, perm, items, nbItemsPtr, itemsMax, out,
-                                                   C, nearTies, s);
+                                                   C, nearTies, lt, s);
         }
     }
     RBOW_HIP(hipGetLastError());
@@ -772,6 +910,13 @@ void launchPrepF32Split(const float* rows, uint64_t nbNodes, uint32_t b, int8_t*
     const uint64_t nbChildren = nbNodes * b;
     k_prepF32Split<<<uint32_t((nbChildren + 3) / 4), 256, 0, s>>>(rows, nbChildren, b == 32 ? 5u : 4u, split,
                                                                   splitStride, cn2, e1, e2, active);
+    RBOW_HIP(hipGetLastError());
+}
+
+void launchMarkDuplicateChildren(const float* rows, uint64_t nbNodes, uint32_t b, float* cn2, hipStream_t s) {
+    if (!nbNodes) return;
+    const uint64_t nbChildren = nbNodes * b;
+    k_markDuplicateChildren<<<gridFor(nbChildren, 256, 1u << 20), 256, 0, s>>>(rows, nbChildren, b - 1, cn2);
     RBOW_HIP(hipGetLastError());
 }
 
@@ -812,7 +957,7 @@ LookupWs lookupWsLayout(const TreeView& t, uint32_t n, hipStream_t s) {
     RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes2, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                               int(w.maxNodes + 1), s));
     w.scanBytes = std::max(w.scanBytes, scanBytes2);
-    const size_t words = 3ull * n + 2 * w.cMax + 4 * (w.maxNodes + 1) + 1;
+    const size_t words = 3ull * n + 2 * w.cMax + 4 * (w.maxNodes + 1) + 2;
     w.bytes = 4 * words + 256 + 2 * sizeof(LkItem) * w.itemsMax + 256 + w.scanBytes + 256;
     return w;
 }
@@ -832,16 +977,16 @@ void lookupMfmaInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t
     uint32_t* nBase[2] = {O + cMax, O + cMax + (maxNodes + 1)};
     uint32_t* nCnt[2] = {nBase[1] + (maxNodes + 1), nBase[1] + 2 * (maxNodes + 1)};
     uint32_t* nbItems = nCnt[1] + (maxNodes + 1);
+    uint32_t* tieCount = nbItems + 1;
     auto align = [](void* p) { return reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(p) + 255) & ~uintptr_t(255)); };
     LkItem* items[2];
-    items[0] = static_cast<LkItem*>(align(nbItems + 1));
+    items[0] = static_cast<LkItem*>(align(tieCount + 1));
     items[1] = items[0] + itemsMax;
     void* temp = align(items[1] + itemsMax);
 
-    // level 0: all descriptors against the root
+    // level 0: all descriptors against the root (the level kernels write every count entry the scan reads)
     uint64_t cLen = uint64_t(items0) * b + 1;
-    launchFillU32(C, 0u, cLen, s);
-    launchLevel<true, false>(t, kTQ, 0, desc, n, nullptr, nullptr, nullptr, items0, code, C, nearTies, s);
+    launchLevel<true, false>(t, kTQ, 0, desc, n, nullptr, nullptr, nullptr, items0, code, C, nearTies, {}, s);
     RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, C, O, int(cLen), s));
     k_scatter<true><<<gridFor(items0, 4, 8192), 256, 0, s>>>(nullptr, nullptr, items0, n, lg2b, nullptr, code, O,
                                                             perm[0]);
@@ -854,7 +999,7 @@ void lookupMfmaInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t
                                                                           itemRows, nCnt[dst]);
         RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, nCnt[dst], nBase[dst], int(nbChildren + 1), s));
         k_writeItems<<<std::min(nbChildren + 1, 65536u), 64, 0, s>>>(O, pBase, pN, nItems0, nbChildren, lg2b, itemRows,
-                                                                    nBase[dst], items[dst], nbItems);
+                                                                    nBase[dst], items[dst], nbItems, tieCount);
         RBOW_HIP(hipGetLastError());
     };
     buildItems(nullptr, nullptr, items0, b, 0, itemRowsForLevel(1));
@@ -863,14 +1008,15 @@ void lookupMfmaInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t
         const uint64_t nodesL = levelSize(lg2b, l);
         const uint32_t itemsL = uint32_t(items0 + nodesL);
         if (l + 1 == t.L) {
+            // the leaf level's near ties are listed in the free permutation and code arrays (n entries each)
+            const LeafTies lt{code, perm[cur ^ 1], tieCount};
             launchLevel<false, true>(t, leafRows, levelBeg(lg2b, l), desc, n, perm[cur], items[cur], nbItems, itemsL, leaf,
-                                     nullptr, nearTies, s);
+                                     nullptr, nearTies, lt, s);
             break;
         }
         cLen = uint64_t(itemsL) * b + 1;
-        launchFillU32(C, 0u, cLen, s);
         launchLevel<false, false>(t, kTQ, levelBeg(lg2b, l), desc, n, perm[cur], items[cur], nbItems, itemsL, code, C,
-                                  nearTies, s);
+                                  nearTies, {}, s);
         RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, C, O, int(cLen), s));
         k_scatter<false><<<gridFor(itemsL, 4, 8192), 256, 0, s>>>(items[cur], nbItems, 0, n, lg2b, perm[cur], code, O,
                                                                  perm[cur ^ 1]);
@@ -953,7 +1099,7 @@ void launchLookupMfma(const TreeView& t, const uint8_t* desc, uint32_t n, uint32
     }
     if (t.L == 1) {
         launchLevel<true, true>(t, kTQ, 0, desc, n, nullptr, nullptr, nullptr, (n + kTQ - 1) / kTQ, leaf, nullptr,
-                                nearTies, s);
+                                nearTies, {}, s);
         return;
     }
     const LookupWs w = lookupWsLayout(t, n, s);

@@ -45,7 +45,10 @@ void Vocabulary::allocate(uint32_t b_, uint32_t L_, uint32_t ct) {
 
 void Vocabulary::prepare(hipStream_t s) {
     if (centroidType == 0) launchPrepU8Tree(u8rows.get(), nbChildren(), normU.get(), normS.get(), s);
-    else launchPrepF32Split(f32rows.get(), nbNodes, b, split.get(), nbChildren() * kDims, cn2.get(), e1.get(), e2.get(), s);
+    else {
+        launchPrepF32Split(f32rows.get(), nbNodes, b, split.get(), nbChildren() * kDims, cn2.get(), e1.get(), e2.get(), s);
+        launchMarkDuplicateChildren(f32rows.get(), nbNodes, b, cn2.get(), s);
+    }
 }
 
 TreeView Vocabulary::view() const {

@@ -117,16 +117,23 @@ def test_lookup_f32_candidate_band(b):
 
 
 @pytest.mark.parametrize("b", [32, 16])
-def test_lookup_f32_all_rows_tie(b):
-    """Every query sits on a duplicated child, so every row of every work item is a
-    near tie: the per-wave tie queue fills and is resolved several times within an
-    item.  Leaf ids equal the plain kernel's (the lower of the equal children)."""
+@pytest.mark.parametrize("exact", [False, True])
+def test_lookup_f32_all_rows_tie(b, exact):
+    """Every query sits on a child with a twin: child b-1 equals child 3 in every
+    node but for one value one ulp apart (exact=False), or bit for bit (exact=True).
+    Near twins make every row of every work item a near tie at the root: the
+    per-wave tie queue fills and is resolved several times within an item.  An
+    exact twin has the same plain distance as child 3, so it can never win (ties go
+    to the lowest child): it is no candidate at all, and the rows are not near ties.
+    Leaf ids equal the plain kernel's and the oracle's."""
     rng = np.random.default_rng(21)
     desc = rb.synth_descriptors(2000, seed=22)
     rows = _f32_tree(rng, b, 2, desc, near_ties=False)
     n = rows.shape[0]
     for v in range(n):
-        rows[v, b - 1] = rows[v, 3]  # child 3 and child b-1 are equal in every node
+        rows[v, b - 1] = rows[v, 3]
+        if not exact:
+            rows[v, b - 1, 17] = np.nextafter(rows[v, 3, 17], np.float32(255))
     voc = rb.deserialize_vocabulary(rbw2_blob(b, 2, 1, rows, np.full(n, 0xFFFFFFFF), np.ones((n, b))))
     target = np.clip(np.rint(rows[0, 3]), 0, 255).astype(np.uint8)
     q = np.repeat(target[None], 1000, axis=0)
@@ -134,7 +141,10 @@ def test_lookup_f32_all_rows_tie(b):
     np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_PLAIN), expect)
     mfma, ties = voc.lookup_host(q, rb.LOOKUP_MFMA, return_near_ties=True)
     np.testing.assert_array_equal(mfma, expect)
-    assert ties >= 1000  # at least the root level ties for every row
+    if exact:
+        assert ties < 1000  # the twins are no candidates
+    else:
+        assert ties >= 1000  # at least the root level ties for every row
 
 
 @pytest.mark.parametrize("b", [32, 16])
