@@ -316,34 +316,71 @@ __device__ inline void zeroCountTail(uint32_t* __restrict__ C, uint32_t nbItems,
         C[i] = 0;
 }
 
-// Moves every descriptor of an item to its place in the next level's order.
+// Moves every descriptor of an item to its place in the next level's order.  The
+// wave first orders the item's rows by child in LDS (child c's rows start at the
+// item's count of children below c), so each store instruction writes runs of
+// consecutive positions instead of one or two entries in each of b runs.  The
+// destination of (item, child) is O at its flat index, its count the next entry
+// of O minus that one (O is the exclusive scan of the counts in the same order).
 template <bool kIdentity>
 __global__ __launch_bounds__(256) void k_scatter(const LkItem* __restrict__ items,
                                                 const uint32_t* __restrict__ nbItemsPtr, uint32_t nbItemsImplicit,
                                                 uint32_t n, uint32_t lg2b, const uint32_t* __restrict__ perm,
                                                 const uint32_t* __restrict__ code, const uint32_t* __restrict__ O,
                                                 uint32_t* __restrict__ permNext) {
-    const uint32_t lane = threadIdx.x & 63, b = 1u << lg2b;
+    __shared__ uint32_t sSrcAll[4][kTQ];
+    __shared__ uint8_t sChildAll[4][kTQ];
+    __shared__ uint32_t sOffAll[4][32], sDstAll[4][32];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, b = 1u << lg2b;
+    uint32_t* sSrc = sSrcAll[wave];
+    uint8_t* sChild = sChildAll[wave];
+    uint32_t *sOff = sOffAll[wave], *sDst = sDstAll[wave];
     const uint32_t nbItems = kIdentity ? nbItemsImplicit : *nbItemsPtr;
-    for (uint32_t it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nbItems; it += gridDim.x * 4) {
+    for (uint32_t it = blockIdx.x * 4 + wave; it < nbItems; it += gridDim.x * 4) {
         const LkItem t = itemAt<kIdentity>(it, n, items);
-        // the item's <= 256 rows, four per lane, every load of a step in flight together
         static_assert(kTQ == 256, "an item is four groups of 64 rows");
-        uint32_t c[4], src[4], dst[4];
+        // the item's rows, four per lane, every load in flight together
+        uint32_t c[4], src[4];
 #pragma unroll
         for (int g = 0; g < 4; g++) {
             const uint32_t r = g * 64 + lane, p = t.start + r;
             c[g] = r < t.cnt ? code[p] : 0u;
             src[g] = r < t.cnt ? (kIdentity ? p : perm[p]) : 0u;
         }
+        // lane j < b: child j's count, its offset among the item's rows and its destination
+        const uint64_t ci = uint64_t(t.itemBase) * b + uint64_t(lane < b ? lane : 0u) * t.nItems + (it - t.itemBase);
+        const uint32_t d0 = O[ci], cnt = lane < b ? O[ci + 1] - d0 : 0u;
+        uint32_t off = cnt;  // inclusive prefix over lanes 0..b-1, then exclusive
+#pragma unroll
+        for (int d = 1; d < 32; d <<= 1) {
+            const uint32_t o = uint32_t(__shfl_up(int(off), d, 64));
+            if (lane >= uint32_t(d)) off += o;
+        }
+        if (lane < b) {
+            sOff[lane] = off - cnt;
+            sDst[lane] = d0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int g = 0; g < 4; g++)
-            dst[g] = g * 64 + lane < t.cnt
-                         ? O[uint64_t(t.itemBase) * b + (c[g] & 0xFFu) * t.nItems + (it - t.itemBase)] + (c[g] >> 8)
-                         : 0u;
+            if (g * 64 + lane < t.cnt) {
+                const uint32_t ch = c[g] & 0xFFu, lp = sOff[ch] + (c[g] >> 8);
+                sSrc[lp] = src[g];
+                sChild[lp] = uint8_t(ch);
+            }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int g = 0; g < 4; g++)
-            if (g * 64 + lane < t.cnt) permNext[dst[g]] = src[g];
+        for (int g = 0; g < 4; g++) {
+            const uint32_t p = g * 64 + lane;
+            if (p < t.cnt) {
+                const uint32_t ch = sChild[p];
+                permNext[sDst[ch] + p - sOff[ch]] = sSrc[p];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // reads done before the next item's writes
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
