@@ -316,6 +316,29 @@ __device__ inline void zeroCountTail(uint32_t* __restrict__ C, uint32_t nbItems,
         C[i] = 0;
 }
 
+// Items of the next level.  Child v = u*b + j of parent u occupies
+// O[base + j*nI] .. O[base + (j+1)*nI) of the scanned counts (base, nI: the
+// parent's items; level 0 has the single parent 0 with nItems0 items).
+__device__ inline void childRange(const uint32_t* O, const uint32_t* pBase, const uint32_t* pN, uint32_t nItems0,
+                                  uint32_t lg2b, uint32_t v, uint32_t& s0, uint32_t& c0) {
+    const uint32_t b = 1u << lg2b, u = v >> lg2b, j = v & (b - 1);
+    const uint32_t base = pBase ? pBase[u] : 0u, ni = pBase ? pN[u] : nItems0;
+    s0 = O[uint64_t(base) * b + j * ni];
+    c0 = O[uint64_t(base) * b + (j + 1) * ni] - s0;
+}
+
+// The next level's children and their item size, for the item counts (the
+// first step of building the next level's items, done by k_scatter's threads):
+// cN[v] = items of child v for v < nb, cN[nb] = 0 (the scan's tail).
+struct NextItems {
+    const uint32_t* pBase;  // the parents' first items and item counts (null at level 0)
+    const uint32_t* pN;
+    uint32_t nItems0;  // level 0: the root's items
+    uint32_t nb;       // children
+    uint32_t itemRows;
+    uint32_t* cN;
+};
+
 // Moves every descriptor of an item to its place in the next level's order.  The
 // wave first orders the item's rows by child in LDS (child c's rows start at the
 // item's count of children below c), so each store instruction writes runs of
@@ -327,7 +350,7 @@ __global__ __launch_bounds__(256) void k_scatter(const LkItem* __restrict__ item
                                                 const uint32_t* __restrict__ nbItemsPtr, uint32_t nbItemsImplicit,
                                                 uint32_t n, uint32_t lg2b, const uint32_t* __restrict__ perm,
                                                 const uint32_t* __restrict__ code, const uint32_t* __restrict__ O,
-                                                uint32_t* __restrict__ permNext) {
+                                                uint32_t* __restrict__ permNext, NextItems nx) {
     __shared__ uint32_t sSrcAll[4][kTQ];
     __shared__ uint8_t sChildAll[4][kTQ];
     __shared__ uint32_t sOffAll[4][32], sDstAll[4][32];
@@ -382,30 +405,14 @@ __global__ __launch_bounds__(256) void k_scatter(const LkItem* __restrict__ item
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // reads done before the next item's writes
         __builtin_amdgcn_wave_barrier();
     }
-}
-
-// Items of the next level.  Child v = u*b + j of parent u occupies
-// O[base + j*nI] .. O[base + (j+1)*nI) of the scanned counts (base, nI: the
-// parent's items; level 0 has the single parent 0 with nItems0 items).
-__device__ inline void childRange(const uint32_t* O, const uint32_t* pBase, const uint32_t* pN, uint32_t nItems0,
-                                  uint32_t lg2b, uint32_t v, uint32_t& s0, uint32_t& c0) {
-    const uint32_t b = 1u << lg2b, u = v >> lg2b, j = v & (b - 1);
-    const uint32_t base = pBase ? pBase[u] : 0u, ni = pBase ? pN[u] : nItems0;
-    s0 = O[uint64_t(base) * b + j * ni];
-    c0 = O[uint64_t(base) * b + (j + 1) * ni] - s0;
-}
-
-__global__ void k_nodeItemCounts(const uint32_t* __restrict__ O, const uint32_t* __restrict__ pBase,
-                                 const uint32_t* __restrict__ pN, uint32_t nItems0, uint32_t nb, uint32_t lg2b,
-                                 uint32_t itemRows, uint32_t* __restrict__ cN) {
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= nb; v += gridDim.x * blockDim.x) {
-        if (v == nb) {
-            cN[v] = 0;  // scan tail: cBase[nb] = total items
-            continue;
+    for (uint32_t v = blockIdx.x * 256 + threadIdx.x; v <= nx.nb; v += gridDim.x * 256) {
+        uint32_t cnt = 0;  // v == nb: the scan's tail
+        if (v < nx.nb) {
+            uint32_t s0, c0;
+            childRange(O, nx.pBase, nx.pN, nx.nItems0, lg2b, v, s0, c0);
+            cnt = (c0 + nx.itemRows - 1) / nx.itemRows;
         }
-        uint32_t s0, c0;
-        childRange(O, pBase, pN, nItems0, lg2b, v, s0, c0);
-        cN[v] = (c0 + itemRows - 1) / itemRows;
+        nx.cN[v] = cnt;
     }
 }
 
@@ -1025,15 +1032,16 @@ void lookupMfmaInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t
     uint64_t cLen = uint64_t(items0) * b + 1;
     launchLevel<true, false>(t, kTQ, 0, desc, n, nullptr, nullptr, nullptr, items0, code, C, nearTies, {}, s);
     RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, C, O, int(cLen), s));
-    k_scatter<true><<<gridFor(items0, 4, 8192), 256, 0, s>>>(nullptr, nullptr, items0, n, lg2b, nullptr, code, O,
-                                                            perm[0]);
-    // items of level l: kTQ rows, at the leaf level the vocabulary's leaf item size (256 or 512)
+    // items of level l: kTQ rows, at the leaf level the vocabulary's leaf item size (256 or 512);
+    // the scatter also writes the next level's item count per child
     const uint32_t leafRows = t.leafItemRows ? t.leafItemRows : lookupLeafItemRowsDefault(t);
     auto itemRowsForLevel = [&](uint32_t l) { return l + 1 == t.L ? leafRows : kTQ; };
+    k_scatter<true><<<gridFor(items0, 4, 8192), 256, 0, s>>>(nullptr, nullptr, items0, n, lg2b, nullptr, code, O,
+                                                            perm[0],
+                                                            NextItems{nullptr, nullptr, items0, b, itemRowsForLevel(1),
+                                                                      nCnt[0]});
     auto buildItems = [&](const uint32_t* pBase, const uint32_t* pN, uint32_t nItems0, uint32_t nbChildren, int dst,
                           uint32_t itemRows) {
-        k_nodeItemCounts<<<gridFor(nbChildren + 1, 256, 4096), 256, 0, s>>>(O, pBase, pN, nItems0, nbChildren, lg2b,
-                                                                          itemRows, nCnt[dst]);
         RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, nCnt[dst], nBase[dst], int(nbChildren + 1), s));
         k_writeItems<<<std::min(nbChildren + 1, 65536u), 64, 0, s>>>(O, pBase, pN, nItems0, nbChildren, lg2b, itemRows,
                                                                     nBase[dst], items[dst], nbItems, tieCount);
@@ -1055,8 +1063,9 @@ void lookupMfmaInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t
         launchLevel<false, false>(t, kTQ, levelBeg(lg2b, l), desc, n, perm[cur], items[cur], nbItems, itemsL, code, C,
                                   nearTies, {}, s);
         RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, C, O, int(cLen), s));
-        k_scatter<false><<<gridFor(itemsL, 4, 8192), 256, 0, s>>>(items[cur], nbItems, 0, n, lg2b, perm[cur], code, O,
-                                                                 perm[cur ^ 1]);
+        k_scatter<false><<<gridFor(itemsL, 4, 8192), 256, 0, s>>>(
+            items[cur], nbItems, 0, n, lg2b, perm[cur], code, O, perm[cur ^ 1],
+            NextItems{nBase[cur], nCnt[cur], 0, uint32_t(nodesL * b), itemRowsForLevel(l + 1), nCnt[cur ^ 1]});
         buildItems(nBase[cur], nCnt[cur], 0, uint32_t(nodesL * b), cur ^ 1, itemRowsForLevel(l + 1));
         cur ^= 1;
     }
