@@ -45,6 +45,7 @@ Beside it on the same JSON line:
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                        [--skip-retrieval] [--skip-serving]
+       python bench.py --profile-builds N | --profile-lookups N   (short runs for a profiler)
 """
 import argparse
 import json
@@ -469,6 +470,9 @@ def main():
     ap.add_argument("--profile-builds", type=int, default=0,
                     help="run this many b16 L4 builds without per-launch timing (after the warmup), "
                          "print their times and stop: a short command for a profiler")
+    ap.add_argument("--profile-lookups", type=int, default=0,
+                    help="build the b32 L4 float tree, run three warm and then this many 10M-query MFMA "
+                         "lookups, print their times and stop: a short command for a profiler")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this library runs on one GPU only; sharding over several GPUs is not implemented",
@@ -483,6 +487,23 @@ def main():
     log(f"bench: generating {N} SIFT-like descriptors in HBM")
     d = syn.descriptors(N, seed=SEED_TRAIN)
     hip.synchronize()
+    if args.profile_lookups:
+        v32 = rb.build_vocabulary(d, N, 1000, 4, branch_factor=32, max_iters=10, seed_mode=1, centroid_type=1,
+                                  kernel=0)
+        nq = 10_000_000
+        q = syn.descriptors(nq, seed=SEED_QUERY)
+        out = rb.DeviceArray((nq,), np.uint32)
+        for _ in range(3):
+            v32.lookup(q, nq, out, rb.LOOKUP_MFMA)
+        e0, e1 = rb.Event(), rb.Event()
+        for i in range(args.profile_lookups):
+            e0.record()
+            v32.lookup(q, nq, out, rb.LOOKUP_MFMA)
+            e1.record()
+            e1.synchronize()
+            ms = e0.elapsed_ms(e1)
+            log(f"bench: lookup {i}: {ms:.3f} ms, {nq / (ms * 1e-3) / 1e9:.3f} G desc/s")
+        return
 
     # ---------------- batched vocabulary build (the timed steps) ----------------
     log("bench: warmup builds")
