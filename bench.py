@@ -75,14 +75,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def timed_lookup(rb, voc, q, n, out, kernel, reps, ties=None, rounds=3):
-    """Steady-state rate: three warm lookups (the first also counts near ties), then
-    `rounds` rounds of `reps` lookups; the median round's rate (the clocks take a few
-    tens of ms of load to settle, so a lone round after one warm call reads low)."""
+def timed_lookup(rb, voc, q, n, out, kernel, reps, ties=None, rounds=3, warm_s=0.06):
+    """Steady-state rate: warm lookups for at least `warm_s` seconds (three at least; the
+    first also counts near ties), then `rounds` rounds of `reps` lookups; the median
+    round's rate.  The GPU's clocks take a few tens of ms of load to settle: rounds right
+    after one warm call read up to 10 % low."""
     voc.lookup(q, n, out, kernel, ties)
-    for _ in range(2):
+    t0, calls = time.perf_counter(), 1
+    while calls < 3 or time.perf_counter() - t0 < warm_s:
         voc.lookup(q, n, out, kernel)
-    rb.hip.synchronize()
+        rb.hip.synchronize()
+        calls += 1
     e0, e1 = rb.Event(), rb.Event()
     rates = []
     for _ in range(rounds):

@@ -732,14 +732,15 @@ __global__ __launch_bounds__(256, 2) void k_levelSplit(  // two waves per SIMD: 
                     const v4i x0 = toSigned(rx0[S]), x1 = toSigned(rx1[S]);
                     loadRows(rowOf(qv[1], S), rx0[S], rx1[S]);  // block rb + 4 (or a valid row past the end)
                     if (rb * 16 >= t.cnt) return;
-                    const uint32_t r = rb * 16 + col, q = rowOf(qv[0], S);
+                    const uint32_t r = rb * 16 + col;
                     const bool valid = r < t.cnt;
                     // ||x - 128||^2 of row col: the diagonal of the rows' Gram matrix (exact int32) on the
                     // matrix cores, which have slack here.  The rows are A and B alike, so lane l holds
                     // G[4 (l >> 4) + i][l & 15]: G[r][r] is component r & 3 of lane 16 (r >> 2) + r.
                     v4i gm = __builtin_amdgcn_mfma_i32_16x16x64_i8(x0, x0, v4i{0, 0, 0, 0}, 0, 0, 0);
                     gm = __builtin_amdgcn_mfma_i32_16x16x64_i8(x1, x1, gm, 0, 0, 0);
-                    const int gOwn = (col & 3) == 0 ? gm[0] : (col & 3) == 1 ? gm[1] : (col & 3) == 2 ? gm[2] : gm[3];
+                    const int g01 = (col & 1) ? gm[1] : gm[0], g23 = (col & 1) ? gm[3] : gm[2];
+                    const int gOwn = (col & 2) ? g23 : g01;
                     const int xs = __shfl(gOwn, int(16 * (col >> 2) + col), 64);
                     v4f acc[CB];
 #pragma unroll
@@ -756,12 +757,14 @@ __global__ __launch_bounds__(256, 2) void k_levelSplit(  // two waves per SIMD: 
                     const uint32_t mask = filterSplit<CB>(acc, float(xs), cn, e1m, e2m, lane, valid);
                     const bool tie = (mask & (mask - 1u)) != 0;  // only valid rows can have several candidates
                     if (kLast) {  // a listed near tie's lowest candidate stands until k_leafTies replaces it
+                        const uint32_t q = rowOf(qv[0], S);
                         if (grp == 0 && valid && (kTieList || !tie)) out[q] = t.node * B + __builtin_ctz(mask);
-                    } else if (grp == 0 && valid && !tie) {
-                        sj[r] = uint8_t(__builtin_ctz(mask));
+                        if (__ballot(tie)) ties += tiePush(tq, nTie, lane, tie, q, q, mask);
+                    } else {
+                        if (grp == 0 && valid && !tie) sj[r] = uint8_t(__builtin_ctz(mask));
+                        if (__ballot(tie))  // rare: most blocks skip the queue's bookkeeping (and the row's index)
+                            ties += tiePush(tq, nTie, lane, tie, rowOf(qv[0], S), r, mask);
                     }
-                    if (__ballot(tie))  // rare: most blocks skip the queue's bookkeeping
-                        ties += tiePush(tq, nTie, lane, tie, q, kLast ? q : r, mask);
                 },
                 std::make_integer_sequence<int, 4>{});
 #pragma unroll
