@@ -149,24 +149,53 @@ __global__ void k_fill64(uint64_t* p, uint64_t v, uint64_t n) {
 // with E consecutive keys per thread in registers: steps whose partners i ^ j
 // are in the same thread (j < E) swap registers, steps within one wave
 // (j < 64 E) exchange with the partner lane, and only steps across waves pass
-// through LDS (two barriers each).  s is complete and visible on return.
+// through LDS (two barriers each).  s (16-byte aligned) is complete and visible
+// on return.  A thread's E keys move to and from LDS as 16-byte vectors: scalar
+// accesses E words apart from lane to lane hit the same banks (16-way at E = 8).
+template <int E>
+__device__ inline void ldsLoadKeys(const uint32_t* p, uint32_t (&r)[E]) {
+    if constexpr (E % 4 == 0) {
+#pragma unroll
+        for (int c = 0; c < E / 4; c++) {
+            const uint4 v = reinterpret_cast<const uint4*>(p)[c];
+            r[4 * c] = v.x;
+            r[4 * c + 1] = v.y;
+            r[4 * c + 2] = v.z;
+            r[4 * c + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; e++) r[e] = p[e];
+    }
+}
+template <int E>
+__device__ inline void ldsStoreKeys(uint32_t* p, const uint32_t (&r)[E]) {
+    if constexpr (E % 4 == 0) {
+#pragma unroll
+        for (int c = 0; c < E / 4; c++)
+            reinterpret_cast<uint4*>(p)[c] = make_uint4(r[4 * c], r[4 * c + 1], r[4 * c + 2], r[4 * c + 3]);
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; e++) p[e] = r[e];
+    }
+}
 template <int E>
 __device__ void bitonicSortBlock(uint32_t* s, uint32_t n2) {
     const uint32_t tid = threadIdx.x, i0 = tid * E;
     uint32_t r[E];
-#pragma unroll
-    for (int e = 0; e < E; e++) r[e] = s[i0 + e];
+    ldsLoadKeys<E>(s + i0, r);
     for (uint32_t k = 2; k <= n2; k <<= 1) {
         uint32_t j = k >> 1;
         for (; j >= uint32_t(64 * E); j >>= 1) {  // across waves
             __syncthreads();
-#pragma unroll
-            for (int e = 0; e < E; e++) s[i0 + e] = r[e];
+            ldsStoreKeys<E>(s + i0, r);
             __syncthreads();
+            uint32_t o[E];  // the partners: (i0 + e) ^ j = (i0 ^ j) + e, as j >= E
+            ldsLoadKeys<E>(s + (i0 ^ j), o);
 #pragma unroll
             for (int e = 0; e < E; e++) {
-                const uint32_t i = i0 + e, o = s[i ^ j];
-                r[e] = ((i & j) == 0) == ((i & k) == 0) ? min(r[e], o) : max(r[e], o);
+                const uint32_t i = i0 + e;
+                r[e] = ((i & j) == 0) == ((i & k) == 0) ? min(r[e], o[e]) : max(r[e], o[e]);
             }
         }
         for (; j >= uint32_t(E); j >>= 1) {  // across lanes of one wave
@@ -191,8 +220,7 @@ __device__ void bitonicSortBlock(uint32_t* s, uint32_t n2) {
         }
     }
     __syncthreads();
-#pragma unroll
-    for (int e = 0; e < E; e++) s[i0 + e] = r[e];
+    ldsStoreKeys<E>(s + i0, r);
     __syncthreads();
 }
 
@@ -206,7 +234,7 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
                                                    const uint64_t* __restrict__ off, uint32_t K,
                                                    QNode* __restrict__ nodes, uint32_t* __restrict__ nodeCount,
                                                    uint32_t* __restrict__ work, uint32_t* __restrict__ bad) {
-    __shared__ uint32_t s[kSortCap];
+    __shared__ __align__(16) uint32_t s[kSortCap];
     __shared__ uint16_t sStart[kSortCap + 1];  // run starts of one level (positions in s), then n
     __shared__ double red[16];
     __shared__ uint32_t redU[16];
@@ -225,7 +253,16 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
     }
     uint32_t n2 = 1024;
     while (n2 < n) n2 <<= 1;
-    for (uint32_t i = tid; i < n2; i += 1024) s[i] = i < n ? leaf[beg + i] : 0xFFFFFFFFu;
+    {  // this thread's leaf ids, every load in flight together (a row past n reads the last id, unused)
+        uint32_t lv[kSortCap / 1024];
+#pragma unroll
+        for (uint32_t e = 0; e < kSortCap / 1024; e++) lv[e] = leaf[beg + min(tid + e * 1024, n - 1)];
+#pragma unroll
+        for (uint32_t e = 0; e < kSortCap / 1024; e++) {
+            const uint32_t i = tid + e * 1024;
+            if (i < n2) s[i] = i < n ? lv[e] : 0xFFFFFFFFu;
+        }
+    }
     if (tid == 0) {
         nNodes = 0;
         nInner = 0;
@@ -250,10 +287,18 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
         }
         return;
     }
+    // the binary64 norm, per level a sum over i = tid, tid + 1024, ...: the level's weight loads
+    // of this thread are issued together (a row past n reads the last leaf's weight, unused),
+    // not one round trip per leaf
     double part = 0;
     for (uint32_t t = 0; t < v.L; t++) {
         const float* W = v.W + levelBeg(v.lg2b, v.L - 1 - t) * v.b;
-        for (uint32_t i = tid; i < n; i += 1024) part += double(W[s[i] >> (t * v.lg2b)]);
+        float wv[kSortCap / 1024];
+#pragma unroll
+        for (uint32_t e = 0; e < kSortCap / 1024; e++) wv[e] = W[s[min(tid + e * 1024, n - 1)] >> (t * v.lg2b)];
+#pragma unroll
+        for (uint32_t e = 0; e < kSortCap / 1024; e++)
+            if (tid + e * 1024 < n) part += double(wv[e]);
     }
     const double l1 = blockSumF64(part, red);
     const float qrcp = ftz(float(1.0 / l1));  // DataBase.cpp:77-94
@@ -263,17 +308,26 @@ __global__ __launch_bounds__(1024) void k_queryPrep(DbView v, const uint32_t* __
     // per level: the runs of equal node ids (leaf >> shift) in the sorted leaves;
     // thread tid scans the contiguous chunk [c0, c1) for run starts
     const uint32_t E = (n + 1023) / 1024, c0 = min(n, tid * E), c1 = min(n, c0 + E);
+    // the thread's chunk and the key before it, read from LDS once for all levels
+    constexpr uint32_t kChunk = kSortCap / 1024;
+    uint32_t ck[kChunk];
+#pragma unroll
+    for (uint32_t e = 0; e < kChunk; e++) ck[e] = c0 + e < c1 ? s[c0 + e] : 0u;
+    const uint32_t before = c0 > 0 && c0 < c1 ? s[c0 - 1] : 0u;
     unsigned long long myWork = 0;  // postings of the query's leaves: the scoring's share of work
     for (uint32_t t = 0; t < v.L; t++) {
         const uint32_t shift = t * v.lg2b;
         const float* W = v.W + levelBeg(v.lg2b, v.L - 1 - t) * v.b;
-        auto isStart = [&](uint32_t i) { return i == 0 || (s[i - 1] >> shift) != (s[i] >> shift); };
-        uint32_t mine = 0;
-        for (uint32_t i = c0; i < c1; i++) mine += isStart(i);
+        uint32_t startMask = 0;  // bit e: position c0 + e starts a run of node ids (leaf >> shift)
+#pragma unroll
+        for (uint32_t e = 0; e < kChunk; e++) {
+            const uint32_t prev = e ? ck[e - 1] : before;
+            const bool st = c0 + e < c1 && (c0 + e == 0 || (prev >> shift) != (ck[e] >> shift));
+            startMask |= uint32_t(st) << e;
+        }
         uint32_t nRuns;
-        uint32_t pos = blockScan(mine, redU, &nRuns);
-        for (uint32_t i = c0; i < c1; i++)
-            if (isStart(i)) sStart[pos++] = uint16_t(i);
+        uint32_t pos = blockScan(uint32_t(__builtin_popcount(startMask)), redU, &nRuns);
+        for (uint32_t m = startMask; m; m &= m - 1u) sStart[pos++] = uint16_t(c0 + __builtin_ctz(m));
         if (tid == 0) sStart[nRuns] = uint16_t(n);
         __syncthreads();
         // runs tid, tid + 1024, ... four at a time: their posting-offset loads are in flight together
