@@ -230,7 +230,7 @@ def image_descriptors(rb, syn, first_image, counts, desc_seed, noise_seed, noise
     return off
 
 
-def bench_retrieval(rb, syn, v32, with_cpu, threads):
+def bench_retrieval(rb, syn, v32, with_cpu, threads, graphs=True):
     """100k images on the 1M-word tree; batches of 64 re-noised query images, top-50."""
     from rapidbow_amd import hip
     NI, K, B, NB = 100_000, 50, 64, 16
@@ -298,7 +298,7 @@ def bench_retrieval(rb, syn, v32, with_cpu, threads):
                 res_docs += [D[q, :c[q]].copy() for q in range(B)]
                 res_sc += [S[q, :c[q]].copy() for q in range(B)]
             # the same batch through the pipeline: direct launches, then one graph launch
-            for g in (False, True):
+            for g in ((False, True) if graphs else (False,)):
                 t0 = time.perf_counter()
                 pipe.run(qdesc, o, docs, sc, cn, lookup_kernel=rb.LOOKUP_MFMA, use_graph=g)  # returns when done
                 dt = time.perf_counter() - t0
@@ -314,9 +314,9 @@ def bench_retrieval(rb, syn, v32, with_cpu, threads):
            "queries_per_s": round(B * NB / (lat.sum() * 1e-3), 1),
            "pipeline": {"direct_batch_ms_p50": round(float(np.percentile(lat_p[False], 50)), 3),
                         "direct_batch_ms_p99": round(float(np.percentile(lat_p[False], 99)), 3),
-                        "graph_batch_ms_p50": round(float(np.percentile(lat_p[True], 50)), 3),
-                        "graph_batch_ms_p99": round(float(np.percentile(lat_p[True], 99)), 3),
-                        "graph_queries_per_s": round(B * NB / (np.sum(lat_p[True]) * 1e-3), 1),
+                        "graph_batch_ms_p50": round(float(np.percentile(lat_p[True], 50)), 3) if graphs else None,
+                        "graph_batch_ms_p99": round(float(np.percentile(lat_p[True], 99)), 3) if graphs else None,
+                        "graph_queries_per_s": round(B * NB / (np.sum(lat_p[True]) * 1e-3), 1) if graphs else None,
                         "graphs_captured": int(pipe.graphs), "same_results": same_pipe},
            "self_first": float(np.mean([d.size > 0 and d[0] == g for d, g in zip(res_docs, qimg)])),
            "scoring_tile_tuned": {"walk_steps": walk_steps,
@@ -473,6 +473,8 @@ def main():
     ap.add_argument("--profile-builds", type=int, default=0,
                     help="run this many b16 L4 builds without per-launch timing (after the warmup), "
                          "print their times and stop: a short command for a profiler")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="retrieval through the query pipeline with direct launches only (for counter profiling)")
     ap.add_argument("--profile-lookups", type=int, default=0,
                     help="build the b32 L4 float tree, run three warm and then this many 10M-query MFMA "
                          "lookups, print their times and stop: a short command for a profiler")
@@ -652,7 +654,7 @@ def main():
     }
     threads = cpu_threads()
     if not args.skip_retrieval:
-        line["retrieval_100k_b32L4"] = bench_retrieval(rb, syn, v32, not args.no_cpu, threads)
+        line["retrieval_100k_b32L4"] = bench_retrieval(rb, syn, v32, not args.no_cpu, threads, not args.no_graphs)
         log(f"bench: retrieval {line['retrieval_100k_b32L4']}")
     if not args.skip_serving:
         line["serving_1M_b32L4"] = bench_serving(rb, syn, v32, (200.0, 1000.0, 5000.0), args.serving_seconds)
