@@ -416,6 +416,38 @@ __global__ __launch_bounds__(256) void k_scatter(const LkItem* __restrict__ item
     }
 }
 
+// Exclusive scan of n <= kSmallScanMax values by one 1024-thread block: thread t
+// sums its kSmallScanPer consecutive values (all loads in flight together), the
+// block scans the sums, and each thread writes its values' prefixes.  One launch
+// where a device-wide scan takes two (the per-child item counts of a level).
+constexpr uint32_t kSmallScanPer = 33, kSmallScanMax = kSmallScanPer * 1024;
+__global__ __launch_bounds__(1024) void k_scanSmall(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                    uint32_t n) {
+    __shared__ uint32_t sWave[16];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, i0 = tid * kSmallScanPer;
+    uint32_t x[kSmallScanPer], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kSmallScanPer; k++) x[k] = i0 + k < n ? in[i0 + k] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < kSmallScanPer; k++) sum += x[k];
+    uint32_t inc = sum;  // inclusive prefix of the sums over the wave, then the block
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = uint32_t(__shfl_up(int(inc), d, 64));
+        if (lane >= uint32_t(d)) inc += o;
+    }
+    if (lane == 63) sWave[wave] = inc;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t w = 0; w < wave; w++) base += sWave[w];
+    uint32_t run = base + inc - sum;
+#pragma unroll
+    for (uint32_t k = 0; k < kSmallScanPer; k++) {
+        if (i0 + k < n) out[i0 + k] = run;
+        run += x[k];
+    }
+}
+
 __global__ void k_writeItems(const uint32_t* __restrict__ O, const uint32_t* __restrict__ pBase,
                              const uint32_t* __restrict__ pN, uint32_t nItems0, uint32_t nb, uint32_t lg2b,
                              uint32_t itemRows, const uint32_t* __restrict__ cBase, LkItem* __restrict__ items,
@@ -1045,7 +1077,10 @@ void lookupMfmaInWs(const TreeView& t, const uint8_t* desc, uint32_t n, uint32_t
                                                                       nCnt[0]});
     auto buildItems = [&](const uint32_t* pBase, const uint32_t* pN, uint32_t nItems0, uint32_t nbChildren, int dst,
                           uint32_t itemRows) {
-        RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, nCnt[dst], nBase[dst], int(nbChildren + 1), s));
+        if (nbChildren + 1 <= kSmallScanMax)
+            k_scanSmall<<<1, 1024, 0, s>>>(nCnt[dst], nBase[dst], nbChildren + 1);
+        else
+            RBOW_HIP(hipcub::DeviceScan::ExclusiveSum(temp, scanBytes, nCnt[dst], nBase[dst], int(nbChildren + 1), s));
         k_writeItems<<<std::min(nbChildren + 1, 65536u), 64, 0, s>>>(O, pBase, pN, nItems0, nbChildren, lg2b, itemRows,
                                                                     nBase[dst], items[dst], nbItems, tieCount);
         RBOW_HIP(hipGetLastError());
