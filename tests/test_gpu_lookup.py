@@ -264,3 +264,23 @@ def test_lookup_u8_fused_bucketed_plain_agree():
             np.testing.assert_array_equal(fused, exp, err_msg=f"fused L={L} n={nq}")
             np.testing.assert_array_equal(bucketed, exp, err_msg=f"bucketed L={L} n={nq}")
             np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_PLAIN), exp)
+
+
+def test_lookup_u8_deep_tree_bucketed():
+    """A b16 L5 tree (1,048,576 words): its leaf level has 65,536 parents, more than
+    the one-block scan of the per-child item counts takes, so that level's counts
+    go through the device-wide scan; the levels above use the one-block scan.
+    Leaf ids of the bucketed MFMA descent equal the plain kernel's and the oracle's."""
+    rng = np.random.default_rng(41)
+    L = 5
+    n = level_beg(16, L)
+    rows = rng.integers(0, 256, (n, 16, 128), dtype=np.uint8)
+    rows[:, 13] = rows[:, 2]  # duplicate children: ties go to the lower index
+    nodes = oracle.rows_to_ref_nodes(rows)
+    voc = rb.deserialize_vocabulary(ref_blob(L, nodes, np.full(n, 0xFFFF), np.ones((n, 16), np.float32)))
+    voc.lookup_fused_max = 1  # every batch above one descriptor bucketed
+    q = rng.integers(0, 256, (70_001, 128), dtype=np.uint8)
+    q[:64] = rows[rng.integers(0, n, 64), rng.integers(0, 16, 64)]  # on a child
+    exp = oracle.lookup_u8(nodes, L, q)
+    np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_MFMA), exp)
+    np.testing.assert_array_equal(voc.lookup_host(q, rb.LOOKUP_PLAIN), exp)
