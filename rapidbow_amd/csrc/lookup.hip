@@ -105,23 +105,31 @@ __global__ __launch_bounds__(256) void k_prepF32Split(const float* __restrict__ 
                                                      uint32_t lg2b, int8_t* __restrict__ split, uint64_t stride,
                                                      float* __restrict__ cn2, float* __restrict__ e1,
                                                      float* __restrict__ e2, const uint32_t* __restrict__ active) {
-    const uint64_t child = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-    const uint32_t lane = threadIdx.x & 63;
-    if (child >= nbChildren) return;
+    // 16 lanes per child, 8 dimensions per lane: the norms' binary64 sums take four lane
+    // swaps within the child's lanes (a wave per child with 2 dimensions per lane took six,
+    // for a quarter of the children per wave)
+    const uint64_t child = (uint64_t(blockIdx.x) * 256 + threadIdx.x) >> 4;
+    const uint32_t lane = threadIdx.x & 63, sub = lane & 15;
+    if (child >= nbChildren) return;                  // the child's 16 lanes together
     if (active && !active[child >> lg2b]) return;
-    const float2 c = reinterpret_cast<const float2*>(rows + child * kDims)[lane];
-    const bool bad = __ballot(!(c.x >= 0.0f && c.x <= 255.0f && c.y >= 0.0f && c.y <= 255.0f)) != 0;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    uint32_t planes[3] = {0u, 0u, 0u};  // the two dimensions' bytes of each term
+    const float4* cr = reinterpret_cast<const float4*>(rows + child * kDims) + 2 * sub;
+    const float4 ca = cr[0], cb = cr[1];
+    const float c[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+    bool out = false;
 #pragma unroll
-    for (int e = 0; e < 2; e++) {
-        const double cd = bad ? 0.0 : double(e == 0 ? c.x : c.y) - 128.0;
+    for (int e = 0; e < 8; e++) out |= !(c[e] >= 0.0f && c[e] <= 255.0f);
+    const bool bad = ((__ballot(out) >> (lane & 48)) & 0xFFFFull) != 0;  // any value of the child
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    uint64_t planes[3] = {0u, 0u, 0u};  // the eight dimensions' bytes of each term
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const double cd = bad ? 0.0 : double(c[e]) - 128.0;
         const double a0 = rint(cd), r0 = cd - a0;
         const double a1 = rint(r0 * 128.0), r1 = r0 - a1 / 128.0;
         const double a2 = rint(r1 * 16384.0), r = r1 - a2 / 16384.0;
-        planes[0] |= (uint32_t(int(a0)) & 0xFFu) << (8 * e);
-        planes[1] |= (uint32_t(int(a1)) & 0xFFu) << (8 * e);
-        planes[2] |= (uint32_t(int(a2)) & 0xFFu) << (8 * e);
+        planes[0] |= uint64_t(uint32_t(int(a0)) & 0xFFu) << (8 * e);
+        planes[1] |= uint64_t(uint32_t(int(a1)) & 0xFFu) << (8 * e);
+        planes[2] |= uint64_t(uint32_t(int(a2)) & 0xFFu) << (8 * e);
         const double qv = cd - r, dr = r0 - r;
         v[0] += cd * cd;
         v[1] += r * r;
@@ -129,13 +137,12 @@ __global__ __launch_bounds__(256) void k_prepF32Split(const float* __restrict__ 
         v[3] += dr * dr;
     }
 #pragma unroll
-    for (int t = 0; t < 3; t++)
-        reinterpret_cast<uint16_t*>(split + t * stride + child * kDims)[lane] = uint16_t(planes[t]);
+    for (int t = 0; t < 3; t++) reinterpret_cast<uint64_t*>(split + t * stride + child * kDims)[sub] = planes[t];
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1)
+    for (int d = 8; d >= 1; d >>= 1)
 #pragma unroll
-        for (int q = 0; q < 4; q++) v[q] += __shfl_xor(v[q], d, 64);
-    if (lane == 0) {
+        for (int q = 0; q < 4; q++) v[q] += __shfl_xor(v[q], d, 16);
+    if (sub == 0) {
         const double u = 1.0 / 16777216.0, slack = 1.0 + 1.0 / 1024;
         cn2[child] = bad ? 0.0f : float(v[0]);
         e1[child] = bad ? INFINITY : float((2.0 * sqrt(v[1]) + 4.0 * u * (sqrt(v[2]) + sqrt(v[3]))) * slack);
@@ -987,7 +994,7 @@ void launchPrepF32Split(const float* rows, uint64_t nbNodes, uint32_t b, int8_t*
                         float* cn2, float* e1, float* e2, hipStream_t s, const uint32_t* active) {
     if (!nbNodes) return;
     const uint64_t nbChildren = nbNodes * b;
-    k_prepF32Split<<<uint32_t((nbChildren + 3) / 4), 256, 0, s>>>(rows, nbChildren, b == 32 ? 5u : 4u, split,
+    k_prepF32Split<<<uint32_t((nbChildren + 15) / 16), 256, 0, s>>>(rows, nbChildren, b == 32 ? 5u : 4u, split,
                                                                   splitStride, cn2, e1, e2, active);
     RBOW_HIP(hipGetLastError());
 }
