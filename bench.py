@@ -75,17 +75,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def timed_lookup(rb, voc, q, n, out, kernel, reps, ties=None, rounds=3, warm_s=0.06):
-    """Steady-state rate: warm lookups for at least `warm_s` seconds (three at least; the
-    first also counts near ties), then `rounds` rounds of `reps` lookups; the median
-    round's rate.  The GPU's clocks take a few tens of ms of load to settle: rounds right
-    after one warm call read up to 10 % low."""
+def timed_lookup(rb, voc, q, n, out, kernel, reps, ties=None, rounds=3, warm_s=0.1):
+    """Steady-state rate: one lookup (it also counts near ties), then back-to-back warm
+    lookups for about `warm_s` seconds, then `rounds` rounds of `reps` lookups; the median
+    round's rate.  The GPU's clocks take a few tens of ms of continuous load to settle:
+    rounds right after one warm call, or after warm calls separated by host waits, read up
+    to 10 % low."""
     voc.lookup(q, n, out, kernel, ties)
-    t0, calls = time.perf_counter(), 1
-    while calls < 3 or time.perf_counter() - t0 < warm_s:
+    rb.hip.synchronize()
+    t0 = time.perf_counter()
+    voc.lookup(q, n, out, kernel)
+    rb.hip.synchronize()
+    one = max(time.perf_counter() - t0, 1e-6)
+    for _ in range(max(2, min(1000, int(warm_s / one)))):
         voc.lookup(q, n, out, kernel)
-        rb.hip.synchronize()
-        calls += 1
+    rb.hip.synchronize()
     e0, e1 = rb.Event(), rb.Event()
     rates = []
     for _ in range(rounds):
