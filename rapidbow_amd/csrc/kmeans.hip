@@ -279,6 +279,28 @@ struct WaveAcc {
         if (lane == 0 && changedLocal) atomicAdd(&k.changed[task], changedLocal);
         waveSync();
     }
+    // As flush, for a task whose every tile this wave processed (sole): no other wave adds
+    // to its totals, which hold zero since the last update, so plain stores of the centers
+    // that received points replace the atomics.
+    __device__ void flushTask(uint32_t lane, uint32_t task, uint32_t changedLocal, const KmLevel& k, bool sole) {
+        if (!sole) {
+            flush(lane, task, changedLocal, k);
+            return;
+        }
+        unsigned long long* gs = reinterpret_cast<unsigned long long*>(k.sums) + uint64_t(task) * B * kDims;
+        waveSync();
+        for (uint32_t e = lane; e < B * kDims; e += 64) {
+            if (cnt[e / kDims]) gs[e] = sums[e];
+            sums[e] = 0;
+        }
+        waveSync();  // the counts are read above before they are cleared
+        if (lane < B) {
+            k.counts[task * B + lane] = cnt[lane];
+            cnt[lane] = 0;
+        }
+        if (lane == 0) k.changed[task] = changedLocal;
+        waveSync();
+    }
 };
 
 // Epilogue shared by all assign kernels: lane r of the wave holds the new
@@ -966,12 +988,15 @@ __global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
     uint32_t beg, end;
     waveRange(k.nbTiles, beg, end);
     uint32_t curTask = kIdxInvalid, changedLocal = 0, ties = 0, nTie = 0;
+    bool curStartsHere = false;  // the current task's first tile is in this wave's range
     float cn[CB][4], e1m = 0.0f, e2m = 0.0f;
     for (uint32_t ti = beg; ti < end; ti++) {
         const KmTile t = k.tiles[ti];
         if (!k.active[t.task]) continue;
         if (t.task != curTask) {
-            if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
+            // a task's tiles are contiguous: the one ending here ended inside the range
+            if (curTask != kIdxInvalid) acc.flushTask(lane, curTask, changedLocal, k, curStartsHere);
+            curStartsHere = ti > beg || ti == 0 || k.tiles[ti - 1].task != t.task;
             changedLocal = 0;
             curTask = t.task;
             const uint64_t childBase = uint64_t(curTask) * B;
@@ -1056,7 +1081,9 @@ __global__ __launch_bounds__(256) void k_assignF32Mfma(KmLevel k) {
         waveSync();
         changedLocal += tileEpilogue<B>(k, t, lane, myA, acc);
     }
-    if (curTask != kIdxInvalid) acc.flush(lane, curTask, changedLocal, k);
+    if (curTask != kIdxInvalid)
+        acc.flushTask(lane, curTask, changedLocal, k,
+                      curStartsHere && (end == k.nbTiles || k.tiles[end].task != curTask));
     if (k.nearTies && ties && lane == 0) atomicAdd(k.nearTies, ties);  // ties: rows of this wave (uniform)
 }
 
