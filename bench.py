@@ -574,6 +574,8 @@ def main():
 
     # ---------------- 1M-word float32 tree, 10M lookups ----------------
     log("bench: building the b32 L4 float32 tree (1,048,576 words) on the 2M descriptors")
+    # one warm float build first: the timed one then carries no first-use costs (workspaces, code objects)
+    rb.build_vocabulary(d, N, 1000, 4, branch_factor=32, max_iters=10, seed_mode=1, centroid_type=1, kernel=0)
     v32 = rb.build_vocabulary(d, N, 1000, 4, branch_factor=32, max_iters=10, seed_mode=1, centroid_type=1, kernel=0)
     st32 = v32.build_stats()
     v32p = rb.build_vocabulary(d, N, 1000, 4, branch_factor=32, max_iters=10, seed_mode=1, centroid_type=1, kernel=1)
